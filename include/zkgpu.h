@@ -1,0 +1,148 @@
+/*
+ * zkgpu.h -- C ABI of the MI355X (gfx950) MSM and NTT library.
+ *
+ * Part 1 re-exports zikkurat-algebra's generated C entry points for the two hot paths with
+ * identical names, signatures, limb layout (flat little-endian u64 limbs, Montgomery form
+ * R = 2^(64 L)), infinity encodings and synchronous host-in/host-out semantics, so that a C
+ * caller (or the Haskell FFI of lib/src/ZK/Algebra/Curves/...) links this library as a
+ * drop-in.  Each declaration cites the reference declaration it replaces.  Differences that a
+ * caller can observe are listed beside the entry point.
+ *
+ * Part 2 is the extension API (zkg_*): device-resident buffers, schedules, batches, the
+ * autotuner/cost model and the serving loop.  These have no reference counterpart.
+ *
+ * Errors: the reference entry points return void and abort() with a message on failure, like
+ * the reference's assert() (lib/cbits/curves/g1/proj/bn128_G1_proj.c:517).  zkg_* functions
+ * return 0 on success and a nonzero status otherwise; zkg_last_error() describes it.
+ * There is no CPU fallback: without a usable GPU every entry point fails.
+ */
+#ifndef ZKGPU_H
+#define ZKGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* =========================================================================================
+ * Part 1: reference entry points (curve prefix bn128_ = BN254, bls12_381_ = BLS12-381)
+ *
+ * MSM: tgt = sum_i expos[i] * grps[i].  expos: npoints scalars of expo_nlimbs u64 limbs
+ * (std = standard form, processed as 64*expo_nlimbs-bit integers exactly like the reference;
+ * mont = Montgomery-form Fr, converted to standard form first, which needs expo_nlimbs = 4).
+ * grps: npoints affine points (x, y Montgomery Fp), affine infinity encoded as (0,0).
+ * affine_out: tgt = 2 Fp elements, infinity = all 0xFF bytes.  proj_out / jac_out: tgt =
+ * 3 Fp elements; the point equals the reference's but is returned normalised (Z = 1, or the
+ * infinity (0:1:0) / [1:1:0]), while the reference returns an unnormalised representative.
+ * npoints <= 0 yields infinity (the reference evaluates log2(0) there).
+ * ========================================================================================= */
+
+/* lib/cbits/curves/g1/proj/bn128_G1_proj.h:43 (impl bn128_G1_proj.c:597-605) */
+void bn128_G1_proj_MSM_std_coeff_proj_out(int npoints, const uint64_t *expos, const uint64_t *grps, uint64_t *tgt, int expo_nlimbs);
+/* bn128_G1_proj.h:44 (impl bn128_G1_proj.c:631-645) */
+void bn128_G1_proj_MSM_mont_coeff_proj_out(int npoints, const uint64_t *expos, const uint64_t *grps, uint64_t *tgt, int expo_nlimbs);
+/* bn128_G1_proj.h:46 (impl bn128_G1_proj.c:656-660) */
+void bn128_G1_proj_MSM_std_coeff_affine_out(int npoints, const uint64_t *expos, const uint64_t *grps, uint64_t *tgt, int expo_nlimbs);
+/* bn128_G1_proj.h:47 (impl bn128_G1_proj.c:669-673) */
+void bn128_G1_proj_MSM_mont_coeff_affine_out(int npoints, const uint64_t *expos, const uint64_t *grps, uint64_t *tgt, int expo_nlimbs);
+/* lib/cbits/curves/g1/jac/bn128_G1_jac.h:43-47 (impl bn128_G1_jac.c:555-722) */
+void bn128_G1_jac_MSM_std_coeff_jac_out(int npoints, const uint64_t *expos, const uint64_t *grps, uint64_t *tgt, int expo_nlimbs);
+void bn128_G1_jac_MSM_mont_coeff_jac_out(int npoints, const uint64_t *expos, const uint64_t *grps, uint64_t *tgt, int expo_nlimbs);
+void bn128_G1_jac_MSM_std_coeff_affine_out(int npoints, const uint64_t *expos, const uint64_t *grps, uint64_t *tgt, int expo_nlimbs);
+void bn128_G1_jac_MSM_mont_coeff_affine_out(int npoints, const uint64_t *expos, const uint64_t *grps, uint64_t *tgt, int expo_nlimbs);
+
+/* lib/cbits/curves/g1/proj/bls12_381_G1_proj.h:43-47 (impl bls12_381_G1_proj.c:598-674) */
+void bls12_381_G1_proj_MSM_std_coeff_proj_out(int npoints, const uint64_t *expos, const uint64_t *grps, uint64_t *tgt, int expo_nlimbs);
+void bls12_381_G1_proj_MSM_mont_coeff_proj_out(int npoints, const uint64_t *expos, const uint64_t *grps, uint64_t *tgt, int expo_nlimbs);
+void bls12_381_G1_proj_MSM_std_coeff_affine_out(int npoints, const uint64_t *expos, const uint64_t *grps, uint64_t *tgt, int expo_nlimbs);
+void bls12_381_G1_proj_MSM_mont_coeff_affine_out(int npoints, const uint64_t *expos, const uint64_t *grps, uint64_t *tgt, int expo_nlimbs);
+/* lib/cbits/curves/g1/jac/bls12_381_G1_jac.h:43-47 */
+void bls12_381_G1_jac_MSM_std_coeff_jac_out(int npoints, const uint64_t *expos, const uint64_t *grps, uint64_t *tgt, int expo_nlimbs);
+void bls12_381_G1_jac_MSM_mont_coeff_jac_out(int npoints, const uint64_t *expos, const uint64_t *grps, uint64_t *tgt, int expo_nlimbs);
+void bls12_381_G1_jac_MSM_std_coeff_affine_out(int npoints, const uint64_t *expos, const uint64_t *grps, uint64_t *tgt, int expo_nlimbs);
+void bls12_381_G1_jac_MSM_mont_coeff_affine_out(int npoints, const uint64_t *expos, const uint64_t *grps, uint64_t *tgt, int expo_nlimbs);
+
+/* NTT over Fr: src and tgt are 2^m Montgomery Fr elements (4 u64 each); gen is the
+ * Montgomery-form generator of the size-2^m subgroup (the *forward* one for the inverse too);
+ * eta is the Montgomery-form coset shift.  src may equal tgt. */
+/* lib/cbits/curves/poly/mont/bn128_poly_mont.h:27 (impl bn128_poly_mont.c:452-473) */
+void bn128_poly_mont_ntt_forward(int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt);
+/* bn128_poly_mont.h:28 (impl bn128_poly_mont.c:601-622) */
+void bn128_poly_mont_ntt_inverse(int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt);
+/* bn128_poly_mont.h:30 (impl bn128_poly_mont.c:477-489) */
+void bn128_poly_mont_ntt_forward_shifted(const uint64_t *eta, int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt);
+/* bn128_poly_mont.h:31 (impl bn128_poly_mont.c:626-638) */
+void bn128_poly_mont_ntt_inverse_shifted(const uint64_t *eta, int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt);
+/* bn128_poly_mont.h:33 (impl bn128_poly_mont.c:492-557) */
+void bn128_poly_mont_ntt_forward_asymmetric(int m_src, int m_tgt, const uint64_t *gen_src, const uint64_t *gen_tgt, const uint64_t *src, uint64_t *tgt);
+
+/* lib/cbits/curves/poly/mont/bls12_381_poly_mont.h:27-33 */
+void bls12_381_poly_mont_ntt_forward(int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt);
+void bls12_381_poly_mont_ntt_inverse(int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt);
+void bls12_381_poly_mont_ntt_forward_shifted(const uint64_t *eta, int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt);
+void bls12_381_poly_mont_ntt_inverse_shifted(const uint64_t *eta, int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt);
+void bls12_381_poly_mont_ntt_forward_asymmetric(int m_src, int m_tgt, const uint64_t *gen_src, const uint64_t *gen_tgt, const uint64_t *src, uint64_t *tgt);
+
+/* =========================================================================================
+ * Part 2: extension API (no reference counterpart)
+ * ========================================================================================= */
+
+#define ZKG_BN254 0
+#define ZKG_BLS12_381 1
+
+/* status and device memory */
+const char *zkg_last_error(void);
+int zkg_device_info(int *num_cus, size_t *hbm_bytes);
+int zkg_malloc(void **dptr, size_t bytes);
+int zkg_free(void *dptr);
+int zkg_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int zkg_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int zkg_synchronize(void);
+
+/* MSM schedule: window width c (1..20), and the split of the bucket-accumulation work.
+ * A zero field means "choose" (cost model, then heuristic). */
+typedef struct {
+  int window;        /* c: signed-digit window width */
+  int seg_log;       /* log2 of sorted entries per accumulation thread */
+  int reserved[6];
+} zkg_msm_schedule;
+
+/* MSM on device buffers.  flags: bit0 = scalars in Montgomery form, bit1 = output
+ * projective (X:Y:Z, normalised) instead of affine, bit2 = output Jacobian.  tgt_host
+ * receives the result (host memory).  sched may be NULL. */
+#define ZKG_MSM_MONT 1
+#define ZKG_MSM_PROJ_OUT 2
+#define ZKG_MSM_JAC_OUT 4
+int zkg_msm(int curve, int npoints, const uint64_t *d_expos, int expo_nlimbs, const uint64_t *d_grps,
+            uint64_t *tgt_host, int flags, const zkg_msm_schedule *sched);
+
+/* NTT on device buffers (flags: 0 forward, 1 inverse); eta NULL = no coset shift.
+ * gen/eta are host pointers. */
+int zkg_ntt(int curve, int m, const uint64_t *gen, const uint64_t *eta, int flags, const uint64_t *d_src,
+            uint64_t *d_tgt);
+/* generator of the size-2^m FFT subgroup, Montgomery form (lib/src/ZK/Algebra/Class/FFT.hs:60-68) */
+int zkg_fft_generator(int curve, int m, uint64_t *gen_out);
+/* NTT fast-path schedule (passes of at most 2^smax points, LDS tiles of 2^tile_log elements) */
+int zkg_ntt_set_tuning(int smax, int tile_log);
+
+/* device elementwise field ops (field: 0 = Fp, 1 = Fr; op: see zikkurat-algebra_amd/csrc/peak.hpp) */
+int zkg_field_op(int curve, int field, int op, size_t n, const uint64_t *d_a, const uint64_t *d_b, uint64_t *d_out);
+
+/* batch fixed-base scalar multiplication k_i * G of the standard generator, affine Montgomery
+ * output (input generation for tests and benchmarks) */
+int zkg_g1_scalar_mul_gen(int curve, int n, const uint64_t *d_scalars, uint64_t *d_out_affine);
+
+/* microbenchmarks: 0 = v_mad_u64_u32 per second, 1 = HBM copy bytes per second,
+ * 2..5 = Montgomery multiplications per second (BN254 Fp, BN254 Fr, BLS Fp, BLS Fr) */
+int zkg_microbench(int which, double *result);
+
+/* timing of the last zkg_msm / zkg_ntt call on the library stream (HIP events), per phase */
+int zkg_last_timing(double *ms_total, double *ms_phases, int max_phases, int *nphases);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ZKGPU_H */

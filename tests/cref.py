@@ -1,0 +1,170 @@
+"""ctypes loaders for the CPU checkers: the oracle restatement (oracle/libzkoracle.so) and,
+where it has been built, the reference's own C (oracle/_ref/libzkref.so).
+
+Test infrastructure only.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+from zkmath import to_limbs
+
+REPO = Path(__file__).resolve().parent.parent
+ORACLE_SO = REPO / "oracle" / "libzkoracle.so"
+REF_SO = REPO / "oracle" / "_ref" / "libzkref.so"
+
+u64p = ctypes.POINTER(ctypes.c_uint64)
+
+
+def ptr(a):
+    assert a.dtype == np.uint64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(u64p)
+
+
+_oracle = None
+_ref = None
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        if not ORACLE_SO.exists():
+            import subprocess
+            subprocess.run(["make", "-C", str(REPO / "oracle")], check=True, capture_output=True)
+        _oracle = ctypes.CDLL(str(ORACLE_SO))
+    return _oracle
+
+
+def ref():
+    """The reference C built from its sources, or None when it has not been built."""
+    global _ref
+    if _ref is None and REF_SO.exists():
+        _ref = ctypes.CDLL(str(REF_SO))
+    return _ref
+
+
+class Oracle:
+    """Thin numpy-facing wrapper over oracle/zko.h."""
+
+    def __init__(self, curve):
+        self.c = curve
+        self.lib = oracle()
+        self.lp = curve.lp
+
+    def f_mul(self, field, a, b):
+        n = 4 if field == 1 else self.lp
+        out = np.zeros(n, np.uint64)
+        self.lib.zko_f_mul(self.c.id, field, ptr(a), ptr(b), ptr(out))
+        return out
+
+    def msm_affine(self, scalars, points, enl=4, mont=False):
+        n = points.shape[0]
+        out = np.zeros(2 * self.lp, np.uint64)
+        fn = self.lib.zko_msm_mont_affine if mont else self.lib.zko_msm_std_affine
+        fn(self.c.id, n, ptr(np.ascontiguousarray(scalars)), ptr(np.ascontiguousarray(points)),
+           ptr(out), enl)
+        return out
+
+    def msm_proj_variable(self, scalars, points, window, enl=4):
+        out = np.zeros(3 * self.lp, np.uint64)
+        self.lib.zko_msm_std_proj_variable(self.c.id, points.shape[0], ptr(scalars), ptr(points),
+                                           ptr(out), enl, window)
+        return out
+
+    def fft_generator(self, m):
+        out = np.zeros(4, np.uint64)
+        self.lib.zko_fft_generator(self.c.id, m, ptr(out))
+        return out
+
+    def ntt(self, m, gen, src, inverse=False):
+        out = np.zeros_like(src)
+        fn = self.lib.zko_ntt_inverse if inverse else self.lib.zko_ntt_forward
+        fn(self.c.id, m, ptr(gen), ptr(src), ptr(out))
+        return out
+
+    def ntt_shifted(self, eta, m, gen, src, inverse=False):
+        out = np.zeros_like(src)
+        fn = self.lib.zko_ntt_inverse_shifted if inverse else self.lib.zko_ntt_forward_shifted
+        fn(self.c.id, ptr(eta), m, ptr(gen), ptr(src), ptr(out))
+        return out
+
+    def ntt_asym(self, ms, mt, gs, gt, src):
+        out = np.zeros(((1 << mt), 4), np.uint64)
+        self.lib.zko_ntt_forward_asymmetric(self.c.id, ms, mt, ptr(gs), ptr(gt), ptr(src), ptr(out))
+        return out
+
+    def points_from_scalars(self, ks):
+        """Affine Montgomery points k_i * G (0 -> the affine infinity (0,0))."""
+        L = self.lp
+        out = np.zeros((len(ks), 2 * L), np.uint64)
+        G = np.array(self.c.affine_to_words(self.c.g) + [0] * L, np.uint64)
+        G[2 * L:3 * L] = np.array(to_limbs(self.c.fp_to_mont(1), L), np.uint64)
+        P = np.zeros(3 * L, np.uint64)
+        for i, k in enumerate(ks):
+            if k == 0:
+                continue
+            K = np.array(to_limbs(k, 4), np.uint64)
+            self.lib.zko_g1_proj_scl(self.c.id, ptr(K), 4, ptr(G), ptr(P))
+            self.lib.zko_g1_proj_to_affine(self.c.id, ptr(P), ptr(out[i]))
+        return out
+
+    def proj_to_affine(self, P):
+        out = np.zeros(2 * self.lp, np.uint64)
+        self.lib.zko_g1_proj_to_affine(self.c.id, ptr(P), ptr(out))
+        return out
+
+
+class Ref:
+    """The reference's own entry points (names from lib/cbits/.../*.h)."""
+
+    def __init__(self, curve):
+        self.c = curve
+        self.lib = ref()
+        self.lp = curve.lp
+        self.pre = curve.name
+
+    def fn(self, name):
+        return getattr(self.lib, f"{self.pre}_{name}")
+
+    def msm(self, scalars, points, coord="proj", mont=False, out="affine", enl=4):
+        n = points.shape[0]
+        size = 2 * self.lp if out == "affine" else 3 * self.lp
+        res = np.zeros(size, np.uint64)
+        kind = "mont" if mont else "std"
+        outname = "affine" if out == "affine" else coord
+        self.fn(f"G1_{coord}_MSM_{kind}_coeff_{outname}_out")(
+            n, ptr(np.ascontiguousarray(scalars)), ptr(np.ascontiguousarray(points)), ptr(res), enl)
+        return res
+
+    def msm_variable(self, scalars, points, window, enl=4):
+        res = np.zeros(3 * self.lp, np.uint64)
+        self.fn("G1_proj_MSM_std_coeff_proj_out_variable")(
+            points.shape[0], ptr(scalars), ptr(points), ptr(res), enl, window)
+        return res
+
+    def ntt(self, m, gen, src, inverse=False):
+        out = np.zeros_like(src)
+        self.fn("poly_mont_ntt_inverse" if inverse else "poly_mont_ntt_forward")(
+            m, ptr(gen), ptr(src), ptr(out))
+        return out
+
+    def ntt_shifted(self, eta, m, gen, src, inverse=False):
+        out = np.zeros_like(src)
+        self.fn("poly_mont_ntt_inverse_shifted" if inverse else "poly_mont_ntt_forward_shifted")(
+            ptr(eta), m, ptr(gen), ptr(src), ptr(out))
+        return out
+
+    def ntt_asym(self, ms, mt, gs, gt, src):
+        out = np.zeros(((1 << mt), 4), np.uint64)
+        self.fn("poly_mont_ntt_forward_asymmetric")(ms, mt, ptr(gs), ptr(gt), ptr(src), ptr(out))
+        return out
+
+    def f_mul(self, field, a, b):
+        n = 4 if field == 1 else self.lp
+        out = np.zeros(n, np.uint64)
+        self.fn("Fr_mont_mul" if field == 1 else "Fp_mont_mul")(ptr(a), ptr(b), ptr(out))
+        return out

@@ -1,0 +1,152 @@
+"""GPU MSM parity: bit-exact against the oracle (the reference's bucket method restated) on
+small inputs with every exceptional case, and against the closed form
+sum_i s_i (k_i G) = (sum_i s_i k_i mod r) G at the larger sizes.
+
+Reference behaviour: lib/cbits/curves/g1/proj/bn128_G1_proj.c:506-673 (proj) and
+lib/cbits/curves/g1/jac/bn128_G1_jac.c:555-745 (jac); affine infinity input (0,0)
+(bn128_G1_affine.c:57-60), infinity output all-0xFF (bn128_G1_proj.c:132-137).
+"""
+import numpy as np
+import pytest
+
+from cref import Oracle
+from zkmath import BN, BLS, ints_to_array, random_scalars, to_limbs, splitmix64
+
+pytestmark = pytest.mark.gpu
+CURVES = [BN, BLS]
+IDS = ["bn254", "bls12-381"]
+
+
+def make_points(C, ks):
+    """k_i * G via the oracle's C scalar multiplication (affine infinity for k = 0)."""
+    return Oracle(C).points_from_scalars(ks)
+
+
+def edge_case_inputs(C, n, seed):
+    """Scalars/points exercising zero digits, r-1, all-ones limbs, duplicates, P and -P,
+    affine infinity and a top window straddling a limb."""
+    ks = random_scalars(seed, n, C.r)
+    ss = random_scalars(seed + 1, n, C.r)
+    if n >= 8:
+        ss[0] = 0
+        ss[1] = C.r - 1
+        ss[2] = (1 << 64) - 1
+        ss[3] = (1 << 253) + 12345
+        ks[4] = ks[5]                 # duplicate point in the same buckets: doubling
+        ss[4] = ss[5]
+        ks[6] = (C.r - ks[7]) % C.r   # P and -P
+        ss[6] = ss[7]
+        ks[3] = 0                     # affine infinity (0,0) input
+    return ks, ss
+
+
+@pytest.mark.parametrize("C", CURVES, ids=IDS)
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 17, 64, 255, 256, 1000])
+def test_msm_vs_oracle(zk, C, n):
+    o = Oracle(C)
+    ks, ss = edge_case_inputs(C, n, 1000 + n)
+    P = make_points(C, ks)
+    S = ints_to_array(ss, 4)
+    exp = o.msm_affine(S, P)
+    assert (zk.msm_std(C.id, S, P) == exp).all()
+    # Montgomery scalars (the Haskell msm path, G1/Proj.hs:254)
+    Sm = ints_to_array([C.fr_to_mont(s % C.r) for s in ss], 4)
+    assert (zk.msm(C.id, Sm, P) == o.msm_affine(Sm, P, mont=True)).all()
+    # closed form
+    assert C.words_to_affine(exp) == C.mul(sum(s * k for s, k in zip(ss, ks)) % C.r, C.g)
+
+
+@pytest.mark.parametrize("C", CURVES, ids=IDS)
+def test_all_reference_msm_symbols(zk, C):
+    """Every exported MSM entry point (proj/jac x std/mont x affine/proj/jac out)."""
+    o = Oracle(C)
+    n = 300
+    ks, ss = edge_case_inputs(C, n, 77)
+    P = make_points(C, ks)
+    S = ints_to_array(ss, 4)
+    Sm = ints_to_array([C.fr_to_mont(s % C.r) for s in ss], 4)
+    exp = C.words_to_affine(o.msm_affine(S, P))
+    for coord in ("proj", "jac"):
+        for kind, scal in (("std", S), ("mont", Sm)):
+            a = zk.msm_ref_symbol(C.id, coord, kind, "affine", scal, P)
+            assert C.words_to_affine(a) == exp
+            full = zk.msm_ref_symbol(C.id, coord, kind, coord, scal, P)
+            dec = C.proj_words_to_affine(full) if coord == "proj" else C.jac_words_to_affine(full)
+            assert dec == exp
+
+
+@pytest.mark.parametrize("C", CURVES, ids=IDS)
+def test_msm_infinity_and_empty(zk, C):
+    o = Oracle(C)
+    P = make_points(C, [5, 9])
+    S = ints_to_array([0, 0], 4)
+    res = zk.msm_std(C.id, S, P)
+    assert (res == np.uint64(0xFFFFFFFFFFFFFFFF)).all()
+    assert (res == o.msm_affine(S, P)).all()
+    # P + (-P) = infinity
+    P2 = make_points(C, [5, C.r - 5])
+    assert (zk.msm_std(C.id, ints_to_array([3, 3], 4), P2) == np.uint64(0xFFFFFFFFFFFFFFFF)).all()
+    # projective / Jacobian encodings of infinity
+    pj = zk.msm_ref_symbol(C.id, "proj", "std", "proj", S, P)
+    assert C.proj_words_to_affine(pj) is None
+    jc = zk.msm_ref_symbol(C.id, "jac", "std", "jac", S, P)
+    assert C.jac_words_to_affine(jc) is None
+
+
+@pytest.mark.parametrize("C", CURVES, ids=IDS)
+@pytest.mark.parametrize("enl", [1, 2, 3, 5, 8])
+def test_msm_scalar_widths(zk, C, enl):
+    """expo_nlimbs != 4: scalars are plain 64*enl-bit integers (bn128_G1_proj.c:510-556)."""
+    o = Oracle(C)
+    n = 200
+    ks = random_scalars(5 + enl, n, C.r)
+    rng = splitmix64(9 + enl)
+    S = np.array([[next(rng) for _ in range(enl)] for _ in range(n)], dtype=np.uint64)
+    S[0] = np.uint64(0xFFFFFFFFFFFFFFFF)
+    P = make_points(C, ks)
+    assert (zk.msm_std(C.id, S, P, expo_nlimbs=enl) == o.msm_affine(S, P, enl=enl)).all()
+
+
+def device_points(zk, C, ks):
+    dK = zk.DeviceArray.from_host(ints_to_array(ks, 4))
+    return zk.scalar_mul_gen(C.id, dK, len(ks))
+
+
+@pytest.mark.parametrize("C", CURVES, ids=IDS)
+def test_scalar_mul_gen_samples(zk, C):
+    n = 4096
+    ks = random_scalars(31, n, C.r)
+    ks[10] = 0
+    dP = device_points(zk, C, ks)
+    P = dP.to_host((n, 2 * C.lp))
+    for i in (0, 1, 10, 777, n - 1):
+        exp = C.mul(ks[i], C.g) if ks[i] else None
+        got = C.words_to_affine(P[i]) if ks[i] else (None if not P[i].any() else "nonzero")
+        assert got == exp, i
+
+
+@pytest.mark.parametrize("C", CURVES, ids=IDS)
+@pytest.mark.parametrize("logn,dist", [(12, "uniform"), (16, "uniform"), (16, "u64"), (16, "sparse")])
+def test_msm_closed_form(zk, C, logn, dist):
+    n = 1 << logn
+    ks = random_scalars(100 + logn, n, C.r)
+    ss = random_scalars(200 + logn, n, C.r, dist)
+    dP = device_points(zk, C, ks)
+    dS = zk.DeviceArray.from_host(ints_to_array(ss, 4))
+    res = zk.msm_device(C.id, n, dS, dP)
+    assert C.words_to_affine(res) == C.mul(sum(s * k for s, k in zip(ss, ks)) % C.r, C.g)
+
+
+@pytest.mark.parametrize("C", CURVES, ids=IDS)
+def test_msm_schedules_agree(zk, C):
+    """Window width and segment length change the schedule, never the result."""
+    n = 5000
+    ks = random_scalars(55, n, C.r)
+    ss = random_scalars(56, n, C.r, "sparse")
+    dP = device_points(zk, C, ks)
+    dS = zk.DeviceArray.from_host(ints_to_array(ss, 4))
+    exp = C.mul(sum(s * k for s, k in zip(ss, ks)) % C.r, C.g)
+    for c in (1, 2, 5, 8, 11, 13, 16):
+        for seg in (1, 3, 5, 8):
+            res = zk.msm_device(C.id, n, dS, dP, window=c, seg_log=seg)
+            assert C.words_to_affine(res) == exp, (c, seg)

@@ -1,0 +1,295 @@
+// C ABI: the reference's generated entry points (synchronous, host buffers) and the zkg_*
+// extension API (device buffers).  Every reference entry point stages its host buffers
+// through the pinned ring, runs the device kernels on the library stream and copies the
+// result back before returning, like the single-threaded C it replaces.
+#include <string.h>
+
+#include <mutex>
+
+#include "../../include/zkgpu.h"
+#include "hostfield.hpp"
+#include "msm.hpp"
+#include "ntt.hpp"
+#include "peak.hpp"
+#include "runtime.hpp"
+
+using namespace zk;
+
+namespace {
+
+int with_ctx(Context **c) { return context(c); }
+
+void check(const char *fn, int status) {
+  if (status != ZK_OK) fatal(fn, last_error());
+}
+
+// host-in / host-out NTT used by every reference NTT symbol
+int ntt_host(int curve, int m, const uint64_t *gen, const uint64_t *eta, int flags, const uint64_t *src,
+             uint64_t *tgt) {
+  if (m < 0 || m > 30) {
+    set_error("ntt: log size out of range");
+    return ZK_EINVAL;
+  }
+  Context *ctx;
+  ZK_TRY(with_ctx(&ctx));
+  std::lock_guard<std::recursive_mutex> lk(ctx->lock);
+  const size_t bytes = ((size_t)1 << m) * 32;
+  DevBuf a, b;
+  ZK_TRY(a.alloc(bytes));
+  ZK_TRY(b.alloc(bytes));
+  ZK_TRY(copy_h2d(a.p, src, bytes, ctx->stream));
+  ZK_TRY(ntt_device(curve, m, gen, eta, flags, a.p, b.p, ctx->stream, nullptr));
+  ZK_TRY(copy_d2h(tgt, b.p, bytes, ctx->stream));
+  return ZK_OK;
+}
+
+const zkh::FieldDesc &fr(int curve) { return curve == 0 ? zkh::BN254_FR : zkh::BLS381_FR; }
+
+// bn128_poly_mont.c:492-557: K = 2^(m_tgt - m_src) shifted NTTs with eta = gen_tgt^k,
+// interleaved as tgt[i*K + k].  When gen_src = gen_tgt^K (subgroups of one domain) this is
+// one size-2^m_tgt NTT of the zero-padded input, which is what the device runs.
+int ntt_asym_host(int curve, int ms, int mt, const uint64_t *gs, const uint64_t *gt, const uint64_t *src,
+                  uint64_t *tgt) {
+  if (ms < 0 || mt < ms || mt > 30) {
+    set_error("ntt_forward_asymmetric: need 0 <= m_src <= m_tgt <= 30");
+    return ZK_EINVAL;
+  }
+  const zkh::FieldDesc &f = fr(curve);
+  Context *ctx;
+  ZK_TRY(with_ctx(&ctx));
+  std::lock_guard<std::recursive_mutex> lk(ctx->lock);
+  const size_t Ns = (size_t)1 << ms, Nt = (size_t)1 << mt, K = Nt / Ns;
+  zkh::Fe g = zkh::load(f, gt);
+  zkh::Fe gK = g;
+  for (int i = 0; i < mt - ms; i++) gK = zkh::sqr(f, gK);
+  const bool same_domain = zkh::is_eq(f, gK, zkh::load(f, gs)) && ntt_generator_ok(curve, mt, gt);
+  DevBuf a, b;
+  if (same_domain) {
+    ZK_TRY(a.alloc(Nt * 32));
+    ZK_TRY(b.alloc(Nt * 32));
+    ZK_HIP(hipMemsetAsync(a.p, 0, Nt * 32, ctx->stream));
+    ZK_TRY(copy_h2d(a.p, src, Ns * 32, ctx->stream));
+    ZK_TRY(ntt_device(curve, mt, gt, nullptr, NTT_FORWARD, a.p, b.p, ctx->stream, nullptr));
+    ZK_TRY(copy_d2h(tgt, b.p, Nt * 32, ctx->stream));
+    return ZK_OK;
+  }
+  // general case: one shifted NTT per coset, interleaved on the host side of the copy
+  ZK_TRY(a.alloc(Ns * 32));
+  ZK_TRY(b.alloc(Ns * 32));
+  ZK_TRY(copy_h2d(a.p, src, Ns * 32, ctx->stream));
+  uint64_t *small = new uint64_t[Ns * 4];
+  zkh::Fe eta = zkh::one(f);
+  int st = ZK_OK;
+  for (size_t k = 0; k < K && st == ZK_OK; k++) {
+    if (k) eta = zkh::mul(f, eta, g);
+    uint64_t e[4];
+    zkh::store(f, e, eta);
+    st = ntt_device(curve, ms, gs, k ? e : nullptr, NTT_FORWARD, a.p, b.p, ctx->stream, nullptr);
+    if (st == ZK_OK) st = copy_d2h(small, b.p, Ns * 32, ctx->stream);
+    if (st == ZK_OK)
+      for (size_t i = 0; i < Ns; i++) memcpy(tgt + (i * K + k) * 4, small + i * 4, 32);
+  }
+  delete[] small;
+  return st;
+}
+
+int msm_host(int curve, int npoints, const uint64_t *expos, const uint64_t *grps, uint64_t *tgt, int enl,
+             int flags) {
+  const int lp = curve == 0 ? 4 : 6;
+  if (npoints <= 0) {
+    MsmResult r;
+    msm_infinity(curve, &r);
+    return msm_write_result(curve, r, tgt, flags);
+  }
+  if (enl < 1 || enl > 16 || ((flags & ZKG_MSM_MONT) && enl != 4)) {
+    set_error("msm: expo_nlimbs must be 1..16 (4 for Montgomery scalars)");
+    return ZK_EINVAL;
+  }
+  Context *ctx;
+  ZK_TRY(with_ctx(&ctx));
+  std::lock_guard<std::recursive_mutex> lk(ctx->lock);
+  DevBuf ds, dp;
+  const size_t sb = (size_t)npoints * enl * 8, pb = (size_t)npoints * 2 * lp * 8;
+  ZK_TRY(ds.alloc(sb));
+  ZK_TRY(dp.alloc(pb));
+  ZK_TRY(copy_h2d(ds.p, expos, sb, ctx->stream));
+  ZK_TRY(copy_h2d(dp.p, grps, pb, ctx->stream));
+  MsmResult r;
+  ZK_TRY(msm_device(curve, npoints, ds.as<uint64_t>(), enl, flags & ZKG_MSM_MONT, dp.as<uint64_t>(), nullptr,
+                    ctx->stream, &r));
+  return msm_write_result(curve, r, tgt, flags);
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---------------- reference NTT symbols ----------------
+#define NTT_SYMBOLS(PFX, CID)                                                                          \
+  void PFX##_poly_mont_ntt_forward(int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt) {   \
+    check(#PFX "_poly_mont_ntt_forward", ntt_host(CID, m, gen, nullptr, NTT_FORWARD, src, tgt));      \
+  }                                                                                                    \
+  void PFX##_poly_mont_ntt_inverse(int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt) {   \
+    check(#PFX "_poly_mont_ntt_inverse", ntt_host(CID, m, gen, nullptr, NTT_INVERSE, src, tgt));      \
+  }                                                                                                    \
+  void PFX##_poly_mont_ntt_forward_shifted(const uint64_t *eta, int m, const uint64_t *gen,           \
+                                           const uint64_t *src, uint64_t *tgt) {                       \
+    check(#PFX "_poly_mont_ntt_forward_shifted", ntt_host(CID, m, gen, eta, NTT_FORWARD, src, tgt));  \
+  }                                                                                                    \
+  void PFX##_poly_mont_ntt_inverse_shifted(const uint64_t *eta, int m, const uint64_t *gen,           \
+                                           const uint64_t *src, uint64_t *tgt) {                       \
+    check(#PFX "_poly_mont_ntt_inverse_shifted", ntt_host(CID, m, gen, eta, NTT_INVERSE, src, tgt));  \
+  }                                                                                                    \
+  void PFX##_poly_mont_ntt_forward_asymmetric(int m_src, int m_tgt, const uint64_t *gen_src,          \
+                                              const uint64_t *gen_tgt, const uint64_t *src,           \
+                                              uint64_t *tgt) {                                         \
+    check(#PFX "_poly_mont_ntt_forward_asymmetric",                                                    \
+          ntt_asym_host(CID, m_src, m_tgt, gen_src, gen_tgt, src, tgt));                               \
+  }
+
+NTT_SYMBOLS(bn128, 0)
+NTT_SYMBOLS(bls12_381, 1)
+
+// ---------------- reference MSM symbols ----------------
+#define MSM_SYMBOL(NAME, CID, FLAGS)                                                                   \
+  void NAME(int npoints, const uint64_t *expos, const uint64_t *grps, uint64_t *tgt, int expo_nlimbs) { \
+    check(#NAME, msm_host(CID, npoints, expos, grps, tgt, expo_nlimbs, FLAGS));                        \
+  }
+
+MSM_SYMBOL(bn128_G1_proj_MSM_std_coeff_proj_out, 0, ZKG_MSM_PROJ_OUT)
+MSM_SYMBOL(bn128_G1_proj_MSM_mont_coeff_proj_out, 0, ZKG_MSM_PROJ_OUT | ZKG_MSM_MONT)
+MSM_SYMBOL(bn128_G1_proj_MSM_std_coeff_affine_out, 0, 0)
+MSM_SYMBOL(bn128_G1_proj_MSM_mont_coeff_affine_out, 0, ZKG_MSM_MONT)
+MSM_SYMBOL(bn128_G1_jac_MSM_std_coeff_jac_out, 0, ZKG_MSM_JAC_OUT)
+MSM_SYMBOL(bn128_G1_jac_MSM_mont_coeff_jac_out, 0, ZKG_MSM_JAC_OUT | ZKG_MSM_MONT)
+MSM_SYMBOL(bn128_G1_jac_MSM_std_coeff_affine_out, 0, 0)
+MSM_SYMBOL(bn128_G1_jac_MSM_mont_coeff_affine_out, 0, ZKG_MSM_MONT)
+MSM_SYMBOL(bls12_381_G1_proj_MSM_std_coeff_proj_out, 1, ZKG_MSM_PROJ_OUT)
+MSM_SYMBOL(bls12_381_G1_proj_MSM_mont_coeff_proj_out, 1, ZKG_MSM_PROJ_OUT | ZKG_MSM_MONT)
+MSM_SYMBOL(bls12_381_G1_proj_MSM_std_coeff_affine_out, 1, 0)
+MSM_SYMBOL(bls12_381_G1_proj_MSM_mont_coeff_affine_out, 1, ZKG_MSM_MONT)
+MSM_SYMBOL(bls12_381_G1_jac_MSM_std_coeff_jac_out, 1, ZKG_MSM_JAC_OUT)
+MSM_SYMBOL(bls12_381_G1_jac_MSM_mont_coeff_jac_out, 1, ZKG_MSM_JAC_OUT | ZKG_MSM_MONT)
+MSM_SYMBOL(bls12_381_G1_jac_MSM_std_coeff_affine_out, 1, 0)
+MSM_SYMBOL(bls12_381_G1_jac_MSM_mont_coeff_affine_out, 1, ZKG_MSM_MONT)
+
+// ---------------- extension API ----------------
+
+const char *zkg_last_error(void) { return last_error(); }
+
+int zkg_device_info(int *num_cus, size_t *hbm_bytes) {
+  Context *ctx;
+  ZK_TRY(with_ctx(&ctx));
+  if (num_cus) *num_cus = ctx->num_cus;
+  if (hbm_bytes) {
+    size_t fr = 0, tot = 0;
+    ZK_HIP(hipMemGetInfo(&fr, &tot));
+    *hbm_bytes = tot;
+  }
+  return ZK_OK;
+}
+
+int zkg_malloc(void **dptr, size_t bytes) {
+  Context *ctx;
+  ZK_TRY(with_ctx(&ctx));
+  return dev_alloc(dptr, bytes);
+}
+int zkg_free(void *dptr) {
+  dev_free(dptr);
+  return ZK_OK;
+}
+int zkg_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+  Context *ctx;
+  ZK_TRY(with_ctx(&ctx));
+  std::lock_guard<std::recursive_mutex> lk(ctx->lock);
+  return copy_h2d(dst, src, bytes, ctx->stream);
+}
+int zkg_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+  Context *ctx;
+  ZK_TRY(with_ctx(&ctx));
+  std::lock_guard<std::recursive_mutex> lk(ctx->lock);
+  return copy_d2h(dst, src, bytes, ctx->stream);
+}
+int zkg_synchronize(void) {
+  Context *ctx;
+  ZK_TRY(with_ctx(&ctx));
+  ZK_HIP(hipStreamSynchronize(ctx->stream));
+  return ZK_OK;
+}
+
+int zkg_msm(int curve, int npoints, const uint64_t *d_expos, int expo_nlimbs, const uint64_t *d_grps,
+            uint64_t *tgt_host, int flags, const zkg_msm_schedule *sched) {
+  if (curve != 0 && curve != 1) {
+    set_error("msm: bad curve id");
+    return ZK_EINVAL;
+  }
+  if (npoints <= 0) {
+    MsmResult r;
+    msm_infinity(curve, &r);
+    return msm_write_result(curve, r, tgt_host, flags);
+  }
+  if (expo_nlimbs < 1 || expo_nlimbs > 16 || ((flags & ZKG_MSM_MONT) && expo_nlimbs != 4)) {
+    set_error("msm: expo_nlimbs must be 1..16 (4 for Montgomery scalars)");
+    return ZK_EINVAL;
+  }
+  Context *ctx;
+  ZK_TRY(with_ctx(&ctx));
+  std::lock_guard<std::recursive_mutex> lk(ctx->lock);
+  MsmResult r;
+  ZK_TRY(msm_device(curve, npoints, d_expos, expo_nlimbs, flags & ZKG_MSM_MONT, d_grps, sched, ctx->stream, &r));
+  return msm_write_result(curve, r, tgt_host, flags);
+}
+
+int zkg_ntt(int curve, int m, const uint64_t *gen, const uint64_t *eta, int flags, const uint64_t *d_src,
+            uint64_t *d_tgt) {
+  Context *ctx;
+  ZK_TRY(with_ctx(&ctx));
+  std::lock_guard<std::recursive_mutex> lk(ctx->lock);
+  return ntt_device(curve, m, gen, eta, flags & 1, d_src, d_tgt, ctx->stream, nullptr);
+}
+
+int zkg_fft_generator(int curve, int m, uint64_t *gen_out) {
+  // 2-adic generators of Fr (lib/src/ZK/Algebra/Curves/BN128/Fr/Mont.hs:148-151, BLS12_381/Fr/Mont.hs:148-151)
+  static const uint64_t bn_gen[4] = {0x636e735580d13d9cULL, 0xa22bf3742445ffd6ULL, 0x56452ac01eb203d8ULL,
+                                     0x1860ef942963f9e7ULL};
+  static const uint64_t bls_gen[4] = {0xb9b58d8c5f0e466aULL, 0x5b1b4c801819d7ecULL, 0x0af53ae352a31e64ULL,
+                                      0x5bf3adda19e9b27bULL};
+  const int M = curve == 0 ? 28 : 32;
+  if ((curve != 0 && curve != 1) || m < 0 || m > M) {
+    set_error("fft_generator: bad curve or size");
+    return ZK_EINVAL;
+  }
+  const zkh::FieldDesc &f = fr(curve);
+  zkh::Fe g = zkh::load(f, curve == 0 ? bn_gen : bls_gen);
+  for (int i = 0; i < M - m; i++) g = zkh::sqr(f, g);
+  zkh::store(f, gen_out, g);
+  return ZK_OK;
+}
+
+int zkg_ntt_set_tuning(int smax, int tile_log) {
+  if (smax < 1 || smax > 11 || tile_log < 4 || tile_log > 12) {
+    set_error("ntt tuning out of range");
+    return ZK_EINVAL;
+  }
+  ntt_set_tuning(NttTuning{smax, tile_log});
+  return ZK_OK;
+}
+
+int zkg_field_op(int curve, int field, int op, size_t n, const uint64_t *d_a, const uint64_t *d_b, uint64_t *d_out) {
+  return field_op_device(curve, field, op, n, d_a, d_b, d_out);
+}
+
+int zkg_g1_scalar_mul_gen(int curve, int n, const uint64_t *d_scalars, uint64_t *d_out_affine) {
+  Context *ctx;
+  ZK_TRY(with_ctx(&ctx));
+  std::lock_guard<std::recursive_mutex> lk(ctx->lock);
+  return g1_scalar_mul_gen(curve, n, d_scalars, d_out_affine, ctx->stream);
+}
+
+int zkg_microbench(int which, double *result) { return microbench(which, result); }
+
+int zkg_last_timing(double *ms_total, double *ms_phases, int max_phases, int *nphases) {
+  return msm_last_timing(ms_total, ms_phases, max_phases, nphases);
+}
+
+}  // extern "C"

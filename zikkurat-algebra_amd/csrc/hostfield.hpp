@@ -1,0 +1,233 @@
+// Host-side prime-field and G1 arithmetic of the library (64-bit limbs, unsigned __int128).
+//
+// The device does all data-parallel work.  The host side only handles O(1)-sized steps that
+// are inherently serial: checking that an NTT generator has the right order, N^-1 and the
+// seeds of twiddle tables, and the final combination of MSM window sums sum_k 2^(kc) S_k
+// (about 255 doublings on one point, i.e. ~0.1 ms on one core against several ms on a single
+// GPU lane), followed by the single inversion of to_affine.  Same Montgomery convention as
+// the device layer and the reference (R = 2^(64 L), canonical outputs).
+#pragma once
+#include <stdint.h>
+#include <string.h>
+
+namespace zkh {
+
+typedef unsigned __int128 u128;
+
+struct FieldDesc {
+  int nl;                 // 64-bit limbs
+  uint64_t p[6];
+  uint64_t np;            // -p^-1 mod 2^64
+  uint64_t one[6];        // R mod p
+  uint64_t r2[6];         // R^2 mod p
+};
+
+extern const FieldDesc BN254_FP, BN254_FR, BLS381_FP, BLS381_FR;
+
+struct Fe {
+  uint64_t w[6];
+};
+
+inline bool is_zero(const FieldDesc &f, const Fe &a) {
+  uint64_t acc = 0;
+  for (int i = 0; i < f.nl; i++) acc |= a.w[i];
+  return acc == 0;
+}
+inline bool is_eq(const FieldDesc &f, const Fe &a, const Fe &b) {
+  for (int i = 0; i < f.nl; i++)
+    if (a.w[i] != b.w[i]) return false;
+  return true;
+}
+inline Fe zero() {
+  Fe r;
+  memset(&r, 0, sizeof r);
+  return r;
+}
+inline Fe one(const FieldDesc &f) {
+  Fe r = zero();
+  memcpy(r.w, f.one, 8 * f.nl);
+  return r;
+}
+inline Fe load(const FieldDesc &f, const uint64_t *p) {
+  Fe r = zero();
+  memcpy(r.w, p, 8 * f.nl);
+  return r;
+}
+inline void store(const FieldDesc &f, uint64_t *p, const Fe &a) { memcpy(p, a.w, 8 * f.nl); }
+
+inline bool geq_p(const FieldDesc &f, const uint64_t *t) {
+  for (int i = f.nl - 1; i >= 0; i--) {
+    if (t[i] > f.p[i]) return true;
+    if (t[i] < f.p[i]) return false;
+  }
+  return true;
+}
+inline void sub_p(const FieldDesc &f, uint64_t *t) {
+  uint64_t br = 0;
+  for (int i = 0; i < f.nl; i++) {
+    u128 d = (u128)t[i] - f.p[i] - br;
+    t[i] = (uint64_t)d;
+    br = (uint64_t)(d >> 64) & 1;
+  }
+}
+inline Fe add(const FieldDesc &f, const Fe &a, const Fe &b) {
+  Fe r = zero();
+  u128 c = 0;
+  for (int i = 0; i < f.nl; i++) {
+    c += (u128)a.w[i] + b.w[i];
+    r.w[i] = (uint64_t)c;
+    c >>= 64;
+  }
+  if (c || geq_p(f, r.w)) sub_p(f, r.w);
+  return r;
+}
+inline Fe sub(const FieldDesc &f, const Fe &a, const Fe &b) {
+  Fe r = zero();
+  uint64_t br = 0;
+  for (int i = 0; i < f.nl; i++) {
+    u128 d = (u128)a.w[i] - b.w[i] - br;
+    r.w[i] = (uint64_t)d;
+    br = (uint64_t)(d >> 64) & 1;
+  }
+  if (br) {
+    u128 c = 0;
+    for (int i = 0; i < f.nl; i++) {
+      c += (u128)r.w[i] + f.p[i];
+      r.w[i] = (uint64_t)c;
+      c >>= 64;
+    }
+  }
+  return r;
+}
+inline Fe neg(const FieldDesc &f, const Fe &a) { return is_zero(f, a) ? a : sub(f, zero(), a); }
+inline Fe mul(const FieldDesc &f, const Fe &a, const Fe &b) {
+  const int n = f.nl;
+  uint64_t t[14] = {0};
+  for (int i = 0; i < n; i++) {
+    u128 c = 0;
+    for (int j = 0; j < n; j++) {
+      c += (u128)a.w[i] * b.w[j] + t[j];
+      t[j] = (uint64_t)c;
+      c >>= 64;
+    }
+    u128 s = (u128)t[n] + c;
+    t[n] = (uint64_t)s;
+    t[n + 1] = (uint64_t)(s >> 64);
+    uint64_t m = t[0] * f.np;
+    c = ((u128)m * f.p[0] + t[0]) >> 64;
+    for (int j = 1; j < n; j++) {
+      c += (u128)m * f.p[j] + t[j];
+      t[j - 1] = (uint64_t)c;
+      c >>= 64;
+    }
+    s = (u128)t[n] + c;
+    t[n - 1] = (uint64_t)s;
+    t[n] = t[n + 1] + (uint64_t)(s >> 64);
+  }
+  Fe r = zero();
+  memcpy(r.w, t, 8 * n);
+  if (t[n] || geq_p(f, r.w)) sub_p(f, r.w);
+  return r;
+}
+inline Fe sqr(const FieldDesc &f, const Fe &a) { return mul(f, a, a); }
+inline Fe from_std(const FieldDesc &f, const Fe &a) {
+  Fe r2 = zero();
+  memcpy(r2.w, f.r2, 8 * f.nl);
+  return mul(f, a, r2);
+}
+inline Fe to_std(const FieldDesc &f, const Fe &a) {
+  Fe u = zero();
+  u.w[0] = 1;
+  return mul(f, a, u);
+}
+inline Fe from_u64(const FieldDesc &f, uint64_t x) {
+  Fe a = zero();
+  a.w[0] = x;
+  return from_std(f, a);
+}
+inline Fe pow_u64(const FieldDesc &f, const Fe &a, uint64_t e) {
+  Fe acc = one(f), base = a;
+  while (e) {
+    if (e & 1) acc = mul(f, acc, base);
+    base = sqr(f, base);
+    e >>= 1;
+  }
+  return acc;
+}
+// Fermat inverse a^(p-2); 0 -> 0
+inline Fe inv(const FieldDesc &f, const Fe &a) {
+  uint64_t e[6];
+  memcpy(e, f.p, sizeof e);
+  e[0] -= 2;  // p is odd and > 2, no borrow
+  Fe acc = one(f);
+  for (int i = 64 * f.nl - 1; i >= 0; i--) {
+    acc = sqr(f, acc);
+    if ((e[i / 64] >> (i % 64)) & 1) acc = mul(f, acc, a);
+  }
+  return acc;
+}
+
+// ---------------------------------------------------------------------------------------
+// G1 in XYZZ coordinates (x = X/ZZ, y = Y/ZZZ, ZZ^3 = ZZZ^2); infinity has ZZ = ZZZ = 0.
+// Formulas: EFD g1p/auto-shortw-xyzz (add-2008-s, dbl-2008-s-1) for a = 0.
+
+struct Xyzz {
+  Fe x, y, zz, zzz;
+};
+
+inline Xyzz xyzz_inf() {
+  Xyzz r;
+  r.x = r.y = r.zz = r.zzz = zero();
+  return r;
+}
+inline bool xyzz_is_inf(const FieldDesc &f, const Xyzz &p) { return is_zero(f, p.zz); }
+
+inline Xyzz xyzz_dbl(const FieldDesc &f, const Xyzz &p) {
+  if (xyzz_is_inf(f, p) || is_zero(f, p.y)) return xyzz_inf();
+  Fe u = add(f, p.y, p.y);
+  Fe v = sqr(f, u);
+  Fe w = mul(f, u, v);
+  Fe s = mul(f, p.x, v);
+  Fe xx = sqr(f, p.x);
+  Fe m = add(f, add(f, xx, xx), xx);
+  Xyzz r;
+  r.x = sub(f, sub(f, sqr(f, m), s), s);
+  r.y = sub(f, mul(f, m, sub(f, s, r.x)), mul(f, w, p.y));
+  r.zz = mul(f, v, p.zz);
+  r.zzz = mul(f, w, p.zzz);
+  return r;
+}
+
+inline Xyzz xyzz_add(const FieldDesc &f, const Xyzz &p, const Xyzz &q) {
+  if (xyzz_is_inf(f, p)) return q;
+  if (xyzz_is_inf(f, q)) return p;
+  Fe u1 = mul(f, p.x, q.zz), u2 = mul(f, q.x, p.zz);
+  Fe s1 = mul(f, p.y, q.zzz), s2 = mul(f, q.y, p.zzz);
+  Fe P = sub(f, u2, u1), R = sub(f, s2, s1);
+  if (is_zero(f, P)) {
+    if (is_zero(f, R)) return xyzz_dbl(f, p);
+    return xyzz_inf();
+  }
+  Fe pp = sqr(f, P), ppp = mul(f, P, pp), q1 = mul(f, u1, pp);
+  Xyzz r;
+  r.x = sub(f, sub(f, sub(f, sqr(f, R), ppp), q1), q1);
+  r.y = sub(f, mul(f, R, sub(f, q1, r.x)), mul(f, s1, ppp));
+  r.zz = mul(f, mul(f, p.zz, q.zz), pp);
+  r.zzz = mul(f, mul(f, p.zzz, q.zzz), ppp);
+  return r;
+}
+
+// affine (Montgomery) output; infinity -> all 0xFF bytes (lib/cbits/curves/g1/proj/bn128_G1_proj.c:132-137)
+inline void xyzz_to_affine(const FieldDesc &f, const Xyzz &p, uint64_t *out) {
+  if (xyzz_is_inf(f, p)) {
+    memset(out, 0xff, 16 * f.nl);
+    return;
+  }
+  Fe zzzi = inv(f, p.zzz);           // 1/ZZZ = 1/Z^3
+  Fe zi = mul(f, zzzi, p.zz);        // Z^2 / Z^3 = 1/Z
+  Fe zzi = sqr(f, zi);               // 1/Z^2
+  store(f, out, mul(f, p.x, zzi));
+  store(f, out + f.nl, mul(f, p.y, zzzi));
+}
+
+}  // namespace zkh

@@ -1,0 +1,649 @@
+// Pippenger MSM on BN254 / BLS12-381 G1 for gfx950.
+//
+// Result contract: sum_i s_i P_i exactly as lib/cbits/curves/g1/proj/bn128_G1_proj.c:506-605
+// computes it (scalars are 64*expo_nlimbs-bit integers, affine (0,0) inputs contribute
+// nothing, Montgomery scalars are converted with Fr to_std first, :631-645).  The group sum
+// is unique, so after normalisation the output matches the reference bit for bit.
+//
+// Device pipeline (one MSM, window c, B = 2^(c-1) buckets per window, W windows):
+//   1 digits   signed c-bit digits d in [-2^(c-1), 2^(c-1)] per (window, point); the carry
+//              makes W = ceil((64 enl + 1) / c) windows exact for any scalar
+//   2 count    per-(window, bucket) histogram (global atomics)
+//   3 scan     exclusive prefix sum -> bucket offsets into one sorted array of all windows
+//   4 scatter  counting-sort placement of (bucket key, point index | sign) pairs
+//   5 accum    load-balanced bucket accumulation: thread t owns sorted entries
+//              [t 2^s, (t+1) 2^s) and folds runs of equal keys with mixed XYZZ + affine adds;
+//              a run that began in an earlier segment is written as a continuation partial
+//   6 fix      each bucket adds its continuation partials (heavy buckets: pairwise tree)
+//   7 pyramid  bucket reduction sum_b (b+1) B_b without serial running sums: level k pairs
+//              neighbours (P'_i = P_2i + P_2i+1) and keeps the odd children Q^(k) = P_2i+1,
+//              which are pairwise reduced alongside; then sum_b (b+1) B_b =
+//              Sum(B) + sum_k 2^(k-1) U_k with U_k the reduced Q^(k) (depth c-1 additions)
+//   8 host     the W*c window terms are combined by one Horner pass on the host
+//              (sum_w 2^(wc) (Sum_w + sum_k 2^(k-1) U_wk), ~255 doublings) and normalised.
+#include <string.h>
+
+#include <chrono>
+#include <vector>
+
+#include "curve.hpp"
+#include "hostfield.hpp"
+#include "msm.hpp"
+#include "runtime.hpp"
+
+namespace zk {
+
+static constexpr int kThreads = 256;
+static constexpr uint32_t kZeroDigit = 0xffffffffu;
+
+// ---------------------------------------------------------------------------------------
+// 1. signed digits
+
+__device__ __forceinline__ uint32_t window_bits(const uint64_t *__restrict__ s, int enl, int bit, int c) {
+  const int nbits = 64 * enl;
+  if (bit >= nbits) return 0;
+  const int limb = bit >> 6, off = bit & 63;
+  uint64_t v = s[limb] >> off;
+  if (off + c > 64 && limb + 1 < enl) v |= s[limb + 1] << (64 - off);
+  return (uint32_t)(v & ((1ull << c) - 1));
+}
+
+__device__ __forceinline__ uint32_t encode_digit(uint32_t raw, int c, uint32_t &carry) {
+  const uint32_t half = 1u << (c - 1);
+  if (raw > half) {
+    const uint32_t mag = (1u << c) - raw;  // d = raw - 2^c < 0
+    carry = 1;
+    return mag ? (((mag - 1) << 1) | 1u) : kZeroDigit;
+  }
+  carry = 0;
+  return raw ? ((raw - 1) << 1) : kZeroDigit;
+}
+
+template <class C>
+__global__ __launch_bounds__(kThreads) void msm_digits_kernel(const uint64_t *__restrict__ sc, int enl, int mont,
+                                                             uint32_t n, int c, int W, uint32_t *__restrict__ dig) {
+  using Fr = typename C::Fr;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t carry = 0;
+  if (enl == 4) {
+    uint64_t s[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) s[j] = sc[(size_t)i * 4 + j];
+    if (mont) {
+      Fe<Fr> x;
+#pragma unroll
+      for (int j = 0; j < 8; j++) x.w[j] = (uint32_t)(s[j >> 1] >> (32 * (j & 1)));
+      fto_std(x, x);
+#pragma unroll
+      for (int j = 0; j < 4; j++) s[j] = (uint64_t)x.w[2 * j] | ((uint64_t)x.w[2 * j + 1] << 32);
+    }
+    for (int w = 0; w < W; w++) {
+      const uint32_t raw = window_bits(s, 4, w * c, c) + carry;
+      dig[(size_t)w * n + i] = encode_digit(raw, c, carry);
+    }
+  } else {
+    const uint64_t *s = sc + (size_t)i * enl;
+    for (int w = 0; w < W; w++) {
+      const uint32_t raw = window_bits(s, enl, w * c, c) + carry;
+      dig[(size_t)w * n + i] = encode_digit(raw, c, carry);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// 2-4. counting sort by (window, bucket)
+
+__global__ __launch_bounds__(kThreads) void msm_count_kernel(const uint32_t *__restrict__ dig, uint32_t n, int logB,
+                                                            uint32_t *__restrict__ count) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t w = blockIdx.y;
+  if (i >= n) return;
+  const uint32_t code = dig[(size_t)w * n + i];
+  if (code == kZeroDigit) return;
+  atomicAdd(&count[(w << logB) + (code >> 1)], 1u);
+}
+
+static constexpr int kScanPer = 16;  // elements per thread in the block scan
+
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *sh, uint32_t *total) {
+  // Hillis-Steele over 256 threads in LDS
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int o = 1; o < kThreads; o <<= 1) {
+    uint32_t x = t >= o ? sh[t - o] : 0u;
+    __syncthreads();
+    sh[t] += x;
+    __syncthreads();
+  }
+  const uint32_t incl = sh[t];
+  *total = sh[kThreads - 1];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ __launch_bounds__(kThreads) void scan_block_sums_kernel(const uint32_t *__restrict__ in, size_t len,
+                                                                  uint32_t *__restrict__ bsums) {
+  __shared__ uint32_t sh[kThreads];
+  const size_t base = (size_t)blockIdx.x * kThreads * kScanPer + (size_t)threadIdx.x * kScanPer;
+  uint32_t s = 0;
+  for (int k = 0; k < kScanPer; k++)
+    if (base + k < len) s += in[base + k];
+  uint32_t tot;
+  block_exclusive_scan(s, sh, &tot);
+  if (threadIdx.x == 0) bsums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kThreads) void scan_top_kernel(uint32_t *__restrict__ bsums, uint32_t nb) {
+  __shared__ uint32_t sh[kThreads];
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < nb; base += kThreads) {
+    const uint32_t i = base + threadIdx.x;
+    const uint32_t v = i < nb ? bsums[i] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_exclusive_scan(v, sh, &tot);
+    if (i < nb) bsums[i] = ex + carry;
+    carry += tot;
+  }
+}
+
+// out[i] = exclusive prefix of in; out[len] = total
+__global__ __launch_bounds__(kThreads) void scan_final_kernel(const uint32_t *__restrict__ in, size_t len,
+                                                             const uint32_t *__restrict__ bsums,
+                                                             uint32_t *__restrict__ out, uint32_t *__restrict__ cursor) {
+  __shared__ uint32_t sh[kThreads];
+  const size_t base = (size_t)blockIdx.x * kThreads * kScanPer + (size_t)threadIdx.x * kScanPer;
+  uint32_t v[kScanPer];
+  uint32_t s = 0;
+  for (int k = 0; k < kScanPer; k++) {
+    v[k] = base + k < len ? in[base + k] : 0u;
+    s += v[k];
+  }
+  uint32_t tot;
+  uint32_t run = block_exclusive_scan(s, sh, &tot) + bsums[blockIdx.x];
+  for (int k = 0; k < kScanPer; k++) {
+    if (base + k < len) {
+      out[base + k] = run;
+      cursor[base + k] = run;
+    } else if (base + k == len) {
+      out[len] = run;
+    }
+    run += v[k];
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void msm_scatter_kernel(const uint32_t *__restrict__ dig, uint32_t n, int logB,
+                                                              uint32_t *__restrict__ cursor, uint32_t *__restrict__ keys,
+                                                              uint32_t *__restrict__ vals) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t w = blockIdx.y;
+  if (i >= n) return;
+  const uint32_t code = dig[(size_t)w * n + i];
+  if (code == kZeroDigit) return;
+  const uint32_t key = (w << logB) + (code >> 1);
+  const uint32_t pos = atomicAdd(&cursor[key], 1u);
+  keys[pos] = key;
+  vals[pos] = i | (code << 31);
+}
+
+// ---------------------------------------------------------------------------------------
+// 5. bucket accumulation
+
+template <class C>
+__device__ __forceinline__ void flush_run(const Xyzz<C> &acc, bool cont_run, uint64_t seg, uint32_t key,
+                                          uint32_t *bsum, uint32_t *cont) {
+  constexpr int PW = 4 * C::Fp::N;
+  uint32_t *dst = cont_run ? cont + seg * PW : bsum + (size_t)key * PW;
+  xyzz_store(dst, acc);
+}
+
+template <class C>
+__global__ __launch_bounds__(kThreads) void msm_accum_kernel(const uint32_t *__restrict__ keys,
+                                                            const uint32_t *__restrict__ vals,
+                                                            const uint32_t *__restrict__ off, uint32_t nkeys,
+                                                            int seg_log, const uint32_t *__restrict__ points,
+                                                            uint32_t *__restrict__ bsum, uint32_t *__restrict__ cont) {
+  constexpr int AW = 2 * C::Fp::N;
+  const uint32_t E = off[nkeys];
+  const uint64_t seg = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t e0 = seg << seg_log;
+  if (e0 >= E) return;
+  uint64_t e1 = e0 + (1ull << seg_log);
+  if (e1 > E) e1 = E;
+  uint32_t cur = keys[e0];
+  bool cont_run = off[cur] < e0;
+  Xyzz<C> acc;
+  xyzz_set_inf(acc);
+  for (uint64_t e = e0; e < e1; e++) {
+    const uint32_t k = keys[e];
+    if (k != cur) {
+      flush_run<C>(acc, cont_run, seg, cur, bsum, cont);
+      cur = k;
+      cont_run = false;
+      xyzz_set_inf(acc);
+    }
+    const uint32_t v = vals[e];
+    Aff<C> a;
+    aff_load(a, points + (size_t)(v & 0x7fffffffu) * AW);
+    if (aff_is_inf(a)) continue;
+    if (v >> 31) aff_neg(a);
+    xyzz_madd(acc, a);
+  }
+  flush_run<C>(acc, cont_run, seg, cur, bsum, cont);
+}
+
+// ---------------------------------------------------------------------------------------
+// 6. continuation partials
+
+template <class C>
+__global__ __launch_bounds__(kThreads) void msm_fix_kernel(const uint32_t *__restrict__ off, uint32_t nkeys,
+                                                          int seg_log, uint32_t *__restrict__ bsum,
+                                                          const uint32_t *__restrict__ cont,
+                                                          uint32_t *__restrict__ heavy, uint32_t heavy_cap,
+                                                          uint32_t limit) {
+  constexpr int PW = 4 * C::Fp::N;
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nkeys) return;
+  const uint32_t s = off[b], t = off[b + 1];
+  if (s == t) return;
+  const uint64_t fs = s >> seg_log, ls = (uint64_t)(t - 1) >> seg_log;
+  if (ls == fs) return;
+  const uint64_t k = ls - fs;
+  if (k > limit) {
+    const uint32_t idx = atomicAdd(&heavy[0], 1u);
+    atomicMax(&heavy[1], (uint32_t)k);
+    if (idx < heavy_cap) heavy[2 + idx] = b;
+    return;
+  }
+  Xyzz<C> acc, q;
+  xyzz_load(acc, bsum + (size_t)b * PW);
+  for (uint64_t j = fs + 1; j <= ls; j++) {
+    xyzz_load(q, cont + j * PW);
+    xyzz_add(acc, acc, q);
+  }
+  xyzz_store(bsum + (size_t)b * PW, acc);
+}
+
+// one pairwise round over the continuation partials of heavy buckets
+template <class C>
+__global__ __launch_bounds__(kThreads) void msm_heavy_round_kernel(const uint32_t *__restrict__ off,
+                                                                  const uint32_t *__restrict__ heavy, int seg_log,
+                                                                  uint32_t *__restrict__ cont, int r,
+                                                                  uint32_t pairs_per_bucket) {
+  constexpr int PW = 4 * C::Fp::N;
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t h = (uint32_t)(t / pairs_per_bucket), j = (uint32_t)(t % pairs_per_bucket);
+  if (h >= heavy[0]) return;
+  const uint32_t b = heavy[2 + h];
+  const uint64_t fs = off[b] >> seg_log, ls = (uint64_t)(off[b + 1] - 1) >> seg_log;
+  const uint64_t K = ls - fs, base = fs + 1;
+  const uint64_t i0 = (uint64_t)j << (r + 1), i1 = i0 + (1ull << r);
+  if (i1 >= K) return;
+  Xyzz<C> a, q;
+  xyzz_load(a, cont + (base + i0) * PW);
+  xyzz_load(q, cont + (base + i1) * PW);
+  xyzz_add(a, a, q);
+  xyzz_store(cont + (base + i0) * PW, a);
+}
+
+template <class C>
+__global__ __launch_bounds__(kThreads) void msm_heavy_final_kernel(const uint32_t *__restrict__ off,
+                                                                  const uint32_t *__restrict__ heavy, int seg_log,
+                                                                  uint32_t *__restrict__ bsum,
+                                                                  const uint32_t *__restrict__ cont) {
+  constexpr int PW = 4 * C::Fp::N;
+  const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= heavy[0]) return;
+  const uint32_t b = heavy[2 + h];
+  const uint64_t fs = off[b] >> seg_log;
+  Xyzz<C> a, q;
+  xyzz_load(a, bsum + (size_t)b * PW);
+  xyzz_load(q, cont + (fs + 1) * PW);
+  xyzz_add(a, a, q);
+  xyzz_store(bsum + (size_t)b * PW, a);
+}
+
+// ---------------------------------------------------------------------------------------
+// 7. bucket pyramid: input [A][W][M] -> output [A+1][W][M/2]
+
+template <class C>
+__global__ __launch_bounds__(kThreads) void msm_pyramid_kernel(const uint32_t *__restrict__ in,
+                                                              uint32_t *__restrict__ out, uint32_t W, uint32_t M,
+                                                              uint32_t A) {
+  constexpr int PW = 4 * C::Fp::N;
+  const uint32_t half = M >> 1;
+  const uint64_t per = (uint64_t)W * half;
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= per * A) return;
+  const uint32_t a = (uint32_t)(t / per);
+  const uint64_t rem = t % per;  // = w * half + i
+  const uint64_t src = (uint64_t)a * W * M + 2 * rem;
+  Xyzz<C> p0, p1, s;
+  xyzz_load(p0, in + src * PW);
+  xyzz_load(p1, in + (src + 1) * PW);
+  xyzz_add(s, p0, p1);
+  xyzz_store(out + ((uint64_t)a * per + rem) * PW, s);
+  if (a == 0) xyzz_store(out + ((uint64_t)A * per + rem) * PW, p1);
+}
+
+// ---------------------------------------------------------------------------------------
+// fixed-base scalar multiplication k * G (input generation), affine output, 0 -> (0,0)
+
+template <class C>
+__global__ __launch_bounds__(kThreads) void scalar_mul_gen_kernel(const uint64_t *__restrict__ sc, uint32_t n,
+                                                                 uint32_t *__restrict__ out) {
+  using F = typename C::Fp;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t s[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) s[j] = sc[(size_t)i * 4 + j];
+  Aff<C> g;
+#pragma unroll
+  for (int j = 0; j < F::N; j++) {
+    g.x.w[j] = C::GX[j];
+    g.y.w[j] = C::GY[j];
+  }
+  Xyzz<C> acc;
+  xyzz_set_inf(acc);
+  for (int bit = 255; bit >= 0; bit--) {
+    xyzz_dbl(acc, acc);
+    if ((s[bit >> 6] >> (bit & 63)) & 1) xyzz_madd(acc, g);
+  }
+  Aff<C> r;
+  if (xyzz_is_inf(acc)) {
+    fset_zero(r.x);
+    fset_zero(r.y);
+  } else {
+    Fe<F> zzzi, zi;
+    finv(zzzi, acc.zzz);
+    fmul(zi, zzzi, acc.zz);
+    fsqr(zi, zi);
+    fmul(r.x, acc.x, zi);
+    fmul(r.y, acc.y, zzzi);
+  }
+  fstore(out + (size_t)i * 2 * F::N, r.x);
+  fstore(out + (size_t)i * 2 * F::N + F::N, r.y);
+}
+
+// ---------------------------------------------------------------------------------------
+// host side
+
+static const zkh::FieldDesc &fp_desc(int curve) { return curve == 0 ? zkh::BN254_FP : zkh::BLS381_FP; }
+
+int msm_heuristic_window(int curve, long long n, int scalar_bits) {
+  (void)curve;
+  if (n <= 1) return 2;
+  double best = 1e300;
+  int bc = 2;
+  for (int c = 2; c <= 20; c++) {
+    const double W = (double)((scalar_bits + 1 + c - 1) / c);
+    const double cost = W * (10.0 * (double)n + 28.0 * (double)(1ll << (c - 1)));
+    if (cost < best) {
+      best = cost;
+      bc = c;
+    }
+  }
+  return bc;
+}
+
+void msm_infinity(int curve, MsmResult *r) {
+  memset(r, 0, sizeof *r);
+  r->curve = curve;
+}
+
+int msm_write_result(int curve, const MsmResult &r, uint64_t *tgt, int flags) {
+  const zkh::FieldDesc &f = fp_desc(curve);
+  const int L = f.nl;
+  zkh::Xyzz p;
+  p.x = zkh::load(f, r.x);
+  p.y = zkh::load(f, r.y);
+  p.zz = zkh::load(f, r.zz);
+  p.zzz = zkh::load(f, r.zzz);
+  if (!(flags & (ZKG_MSM_PROJ_OUT | ZKG_MSM_JAC_OUT))) {
+    zkh::xyzz_to_affine(f, p, tgt);
+    return ZK_OK;
+  }
+  const zkh::Fe one = zkh::one(f), zero = zkh::zero();
+  if (zkh::xyzz_is_inf(f, p)) {
+    // projective infinity (0:1:0) (bn128_G1_proj.c:178-182); Jacobian [1:1:0] (bn128_G1_jac.c:183-188)
+    const bool jac = flags & ZKG_MSM_JAC_OUT;
+    zkh::store(f, tgt, jac ? one : zero);
+    zkh::store(f, tgt + L, one);
+    zkh::store(f, tgt + 2 * L, zero);
+    return ZK_OK;
+  }
+  uint64_t aff[12];
+  zkh::xyzz_to_affine(f, p, aff);
+  memcpy(tgt, aff, 16 * L);
+  zkh::store(f, tgt + 2 * L, one);
+  return ZK_OK;
+}
+
+// per-phase timing of the last MSM (HIP events on the library stream + host wall clock)
+static double g_ms_total = 0, g_ms_phase[8] = {0};
+static int g_nphase = 0;
+
+int msm_last_timing(double *ms_total, double *ms_phases, int max_phases, int *nphases) {
+  if (ms_total) *ms_total = g_ms_total;
+  const int k = g_nphase < max_phases ? g_nphase : max_phases;
+  for (int i = 0; i < k; i++) ms_phases[i] = g_ms_phase[i];
+  if (nphases) *nphases = g_nphase;
+  return ZK_OK;
+}
+
+struct Events {
+  hipEvent_t e[8];
+  int n = 0;
+  bool ok = true;
+  Events() {
+    for (int i = 0; i < 8; i++)
+      if (hipEventCreate(&e[i]) != hipSuccess) ok = false;
+  }
+  ~Events() {
+    for (int i = 0; i < 8; i++) (void)hipEventDestroy(e[i]);
+  }
+  void mark(hipStream_t st) {
+    if (n < 8) (void)hipEventRecord(e[n++], st);
+  }
+};
+
+static MsmSchedule resolve_schedule(int curve, int n, int enl, const zkg_msm_schedule *user) {
+  MsmSchedule s;
+  const int fr_bits = curve == 0 ? 254 : 255;
+  const int bits_for_cost = enl * 64 < fr_bits ? enl * 64 : fr_bits;
+  s.c = (user && user->window > 0) ? user->window : msm_heuristic_window(curve, n, bits_for_cost);
+  if (s.c < 1) s.c = 1;
+  if (s.c > 20) s.c = 20;
+  s.windows = (64 * enl + 1 + s.c - 1) / s.c;
+  s.seg_log = (user && user->seg_log > 0) ? user->seg_log : 5;
+  if (s.seg_log > 12) s.seg_log = 12;
+  return s;
+}
+
+template <class C>
+static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const uint64_t *d_points,
+                   const zkg_msm_schedule *user, hipStream_t st, MsmResult *out) {
+  using F = typename C::Fp;
+  constexpr int PW = 4 * F::N;  // u32 words per XYZZ point
+  const MsmSchedule sch = resolve_schedule(C::id, n, enl, user);
+  const int c = sch.c, W = sch.windows, logB = c - 1;
+  const uint32_t B = 1u << logB;
+  const uint64_t nkeys = (uint64_t)W * B;
+  if (nkeys >= (1ull << 32) || (uint64_t)W * n >= (1ull << 32)) {
+    set_error("msm: problem too large for 32-bit sort keys");
+    return ZK_EINVAL;
+  }
+  const uint64_t total = (uint64_t)W * n;
+  const int seg_log = sch.seg_log;
+  const uint64_t nseg = (total >> seg_log) + 1;
+
+  DevBuf dig, keys, vals, count, off, cursor, bsum, cont, heavy, pyrA, pyrB, scan_b;
+  ZK_TRY(dig.alloc(total * 4));
+  ZK_TRY(keys.alloc(total * 4));
+  ZK_TRY(vals.alloc(total * 4));
+  ZK_TRY(count.alloc((nkeys + 1) * 4));
+  ZK_TRY(off.alloc((nkeys + 1) * 4));
+  ZK_TRY(cursor.alloc((nkeys + 1) * 4));
+  ZK_TRY(bsum.alloc(nkeys * PW * 4));
+  ZK_TRY(cont.alloc(nseg * PW * 4));
+  const uint32_t heavy_cap = 1u << 16;
+  ZK_TRY(heavy.alloc((heavy_cap + 2) * 4));
+  ZK_TRY(pyrA.alloc(nkeys * PW * 4));
+  ZK_TRY(pyrB.alloc(nkeys * PW * 4));
+  const uint64_t per_block = (uint64_t)kThreads * kScanPer;
+  const uint64_t nscan = nkeys + 1;
+  const uint32_t nsb = (uint32_t)((nscan + per_block - 1) / per_block);
+  ZK_TRY(scan_b.alloc((uint64_t)nsb * 4));
+
+  Events ev;
+  auto t_host0 = std::chrono::steady_clock::now();
+  ev.mark(st);
+  ZK_HIP(hipMemsetAsync(count.p, 0, (nkeys + 1) * 4, st));
+  ZK_HIP(hipMemsetAsync(bsum.p, 0, nkeys * PW * 4, st));
+  ZK_HIP(hipMemsetAsync(heavy.p, 0, 8, st));
+  const dim3 blk(kThreads);
+  const dim3 gpts((n + kThreads - 1) / kThreads);
+  hipLaunchKernelGGL(msm_digits_kernel<C>, gpts, blk, 0, st, d_scalars, enl, mont, (uint32_t)n, c, W,
+                     dig.as<uint32_t>());
+  const dim3 gpw((n + kThreads - 1) / kThreads, W);
+  hipLaunchKernelGGL(msm_count_kernel, gpw, blk, 0, st, dig.as<uint32_t>(), (uint32_t)n, logB,
+                     count.as<uint32_t>());
+  hipLaunchKernelGGL(scan_block_sums_kernel, dim3(nsb), blk, 0, st, count.as<uint32_t>(), (size_t)nscan,
+                     scan_b.as<uint32_t>());
+  hipLaunchKernelGGL(scan_top_kernel, dim3(1), blk, 0, st, scan_b.as<uint32_t>(), nsb);
+  hipLaunchKernelGGL(scan_final_kernel, dim3(nsb), blk, 0, st, count.as<uint32_t>(), (size_t)nkeys,
+                     scan_b.as<uint32_t>(), off.as<uint32_t>(), cursor.as<uint32_t>());
+  hipLaunchKernelGGL(msm_scatter_kernel, gpw, blk, 0, st, dig.as<uint32_t>(), (uint32_t)n, logB,
+                     cursor.as<uint32_t>(), keys.as<uint32_t>(), vals.as<uint32_t>());
+  ZK_HIP(hipGetLastError());
+  ev.mark(st);
+  hipLaunchKernelGGL(msm_accum_kernel<C>, dim3((unsigned)((nseg + kThreads - 1) / kThreads)), blk, 0, st,
+                     keys.as<uint32_t>(), vals.as<uint32_t>(), off.as<uint32_t>(), (uint32_t)nkeys, seg_log,
+                     (const uint32_t *)d_points, bsum.as<uint32_t>(), cont.as<uint32_t>());
+  ZK_HIP(hipGetLastError());
+  ev.mark(st);
+  const uint32_t limit = 16;
+  hipLaunchKernelGGL(msm_fix_kernel<C>, dim3((unsigned)((nkeys + kThreads - 1) / kThreads)), blk, 0, st,
+                     off.as<uint32_t>(), (uint32_t)nkeys, seg_log, bsum.as<uint32_t>(), cont.as<uint32_t>(),
+                     heavy.as<uint32_t>(), heavy_cap, limit);
+  ZK_HIP(hipGetLastError());
+  uint32_t hh[2];
+  ZK_HIP(hipMemcpyAsync(hh, heavy.p, 8, hipMemcpyDeviceToHost, st));
+  ZK_HIP(hipStreamSynchronize(st));
+  if (hh[0] > heavy_cap) {
+    set_error("msm: too many heavy buckets");
+    return ZK_EINVAL;
+  }
+  if (hh[0]) {
+    const uint32_t nh = hh[0], kmax = hh[1];
+    for (int r = 0; (1ull << r) < kmax; r++) {
+      const uint32_t pairs = (uint32_t)((kmax + (2ull << r) - 1) >> (r + 1));
+      const uint64_t thr = (uint64_t)nh * pairs;
+      hipLaunchKernelGGL(msm_heavy_round_kernel<C>, dim3((unsigned)((thr + kThreads - 1) / kThreads)), blk, 0, st,
+                         off.as<uint32_t>(), heavy.as<uint32_t>(), seg_log, cont.as<uint32_t>(), r, pairs);
+    }
+    hipLaunchKernelGGL(msm_heavy_final_kernel<C>, dim3((nh + kThreads - 1) / kThreads), blk, 0, st,
+                       off.as<uint32_t>(), heavy.as<uint32_t>(), seg_log, bsum.as<uint32_t>(), cont.as<uint32_t>());
+    ZK_HIP(hipGetLastError());
+  }
+  ev.mark(st);
+  // pyramid
+  const uint32_t *in = bsum.as<uint32_t>();
+  uint32_t *bufs[2] = {pyrA.as<uint32_t>(), pyrB.as<uint32_t>()};
+  uint32_t A = 1, M = B;
+  int k = 0;
+  while (M > 1) {
+    const uint64_t thr = (uint64_t)A * W * (M / 2);
+    hipLaunchKernelGGL(msm_pyramid_kernel<C>, dim3((unsigned)((thr + kThreads - 1) / kThreads)), blk, 0, st, in,
+                       bufs[k], (uint32_t)W, M, A);
+    in = bufs[k];
+    k ^= 1;
+    A += 1;
+    M >>= 1;
+  }
+  ZK_HIP(hipGetLastError());
+  ev.mark(st);
+  // A arrays of W points: [0] = Sum_w, [j] = U_{w, j-1}
+  std::vector<uint32_t> res((size_t)A * W * PW);
+  ZK_HIP(hipMemcpyAsync(res.data(), in, res.size() * 4, hipMemcpyDeviceToHost, st));
+  ZK_HIP(hipStreamSynchronize(st));
+  ev.mark(st);
+  auto t_host1 = std::chrono::steady_clock::now();
+
+  // host Horner: sum over (w, j) of 2^(w c + max(j-1, 0)) * term
+  const zkh::FieldDesc &f = fp_desc(C::id);
+  auto get = [&](uint32_t a, uint32_t w) {
+    const uint32_t *p = res.data() + ((size_t)a * W + w) * PW;
+    zkh::Xyzz q;
+    zkh::Fe *dst[4] = {&q.x, &q.y, &q.zz, &q.zzz};
+    for (int e = 0; e < 4; e++) {
+      *dst[e] = zkh::zero();
+      for (int j = 0; j < F::N; j++) dst[e]->w[j / 2] |= (uint64_t)p[e * F::N + j] << (32 * (j & 1));
+    }
+    return q;
+  };
+  const int emax = (W - 1) * c + (int)(A > 1 ? A - 2 : 0);
+  std::vector<std::vector<uint32_t>> at(emax + 1);  // encoded (a << 16) | w
+  for (uint32_t w = 0; w < (uint32_t)W; w++) {
+    at[w * c].push_back((0u << 16) | w);
+    for (uint32_t a = 1; a < A; a++) at[w * c + (a - 1)].push_back((a << 16) | w);
+  }
+  zkh::Xyzz acc = zkh::xyzz_inf();
+  for (int e = emax; e >= 0; e--) {
+    acc = zkh::xyzz_dbl(f, acc);
+    for (uint32_t code : at[e]) acc = zkh::xyzz_add(f, acc, get(code >> 16, code & 0xffff));
+  }
+  auto t_host2 = std::chrono::steady_clock::now();
+
+  memset(out, 0, sizeof *out);
+  out->curve = C::id;
+  zkh::store(f, out->x, acc.x);
+  zkh::store(f, out->y, acc.y);
+  zkh::store(f, out->zz, acc.zz);
+  zkh::store(f, out->zzz, acc.zzz);
+
+  // timing: sort, accumulate, fix, pyramid, d2h, host combine
+  g_nphase = 0;
+  if (ev.ok) {
+    for (int i = 0; i + 1 < ev.n; i++) {
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, ev.e[i], ev.e[i + 1]);
+      g_ms_phase[g_nphase++] = ms;
+    }
+  }
+  g_ms_phase[g_nphase++] = std::chrono::duration<double, std::milli>(t_host2 - t_host1).count();
+  g_ms_total = std::chrono::duration<double, std::milli>(t_host2 - t_host0).count();
+  return ZK_OK;
+}
+
+int msm_device(int curve, int n, const uint64_t *d_scalars, int enl, int mont, const uint64_t *d_points,
+               const zkg_msm_schedule *sched, hipStream_t st, MsmResult *out) {
+  if (n <= 0) {
+    msm_infinity(curve, out);
+    return ZK_OK;
+  }
+  if (curve == 0) return msm_run<Bn254G1>(n, d_scalars, enl, mont, d_points, sched, st, out);
+  if (curve == 1) return msm_run<Bls381G1>(n, d_scalars, enl, mont, d_points, sched, st, out);
+  set_error("msm: bad curve id");
+  return ZK_EINVAL;
+}
+
+int g1_scalar_mul_gen(int curve, int n, const uint64_t *d_scalars, uint64_t *d_out, hipStream_t st) {
+  if (n <= 0) return ZK_OK;
+  const dim3 g((n + kThreads - 1) / kThreads), b(kThreads);
+  if (curve == 0)
+    hipLaunchKernelGGL(scalar_mul_gen_kernel<Bn254G1>, g, b, 0, st, d_scalars, (uint32_t)n, (uint32_t *)d_out);
+  else if (curve == 1)
+    hipLaunchKernelGGL(scalar_mul_gen_kernel<Bls381G1>, g, b, 0, st, d_scalars, (uint32_t)n, (uint32_t *)d_out);
+  else {
+    set_error("scalar_mul_gen: bad curve id");
+    return ZK_EINVAL;
+  }
+  ZK_HIP(hipGetLastError());
+  ZK_HIP(hipStreamSynchronize(st));
+  return ZK_OK;
+}
+
+}  // namespace zk

@@ -1,0 +1,38 @@
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/zkgpu.h"
+
+namespace zk {
+
+// Final MSM value in XYZZ Montgomery coordinates on the host.
+struct MsmResult {
+  int curve = 0;
+  uint64_t x[6] = {0}, y[6] = {0}, zz[6] = {0}, zzz[6] = {0};
+};
+
+// Resolved schedule of one MSM.
+struct MsmSchedule {
+  int c = 0;          // window width
+  int windows = 0;    // number of signed-digit windows
+  int seg_log = 5;    // log2 sorted entries per accumulation thread
+};
+
+// analytic window choice: minimises W * (10 n + 28 * 2^(c-1)) group-op cost (madd ~ 10
+// mulmods, bucket pyramid ~ 2 adds of 14 mulmods per bucket) over c in [1, 20]
+int msm_heuristic_window(int curve, long long n, int scalar_bits);
+
+void msm_infinity(int curve, MsmResult *r);
+int msm_write_result(int curve, const MsmResult &r, uint64_t *tgt, int flags);
+
+// Device MSM; d_scalars: n * enl u64 (standard form, or Montgomery Fr when mont != 0),
+// d_points: n affine Montgomery points.  Synchronous; the result is left in *out.
+int msm_device(int curve, int n, const uint64_t *d_scalars, int enl, int mont, const uint64_t *d_points,
+               const zkg_msm_schedule *sched, hipStream_t st, MsmResult *out);
+
+int g1_scalar_mul_gen(int curve, int n, const uint64_t *d_scalars, uint64_t *d_out, hipStream_t st);
+
+int msm_last_timing(double *ms_total, double *ms_phases, int max_phases, int *nphases);
+
+}  // namespace zk

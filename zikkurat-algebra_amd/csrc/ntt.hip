@@ -1,0 +1,570 @@
+// NTT / iNTT over the scalar fields Fr of BN254 and BLS12-381 on gfx950.
+//
+// Contract (lib/cbits/curves/poly/mont/bn128_poly_mont.c:418-638): natural-order input and
+// output, y_k = sum_i x_i g^(ik) (forward) and x_i = N^-1 sum_k y_k g^(-ik) (inverse, given
+// the *forward* generator g), Montgomery-form Fr elements in the flat 4x64-bit layout.
+// Shifted variants pre-multiply x_i by eta^i (forward, :477-489) or post-multiply the inverse
+// output by eta^i (:626-638).  For a generator of order exactly 2^m every correct DFT
+// algorithm gives the same canonical field elements, so the GPU factorisation below matches
+// the reference's recursive radix-2 network bit for bit.  Other generators go through
+// ntt_network_* (the reference's own butterfly network, stage by stage).
+//
+// Fast path: the transform is factored N = N_1 N_2 ... N_P (each N_p <= 2^11).  Pass p views
+// the array as [R_p][N_p][C_p] (R_p = N_1..N_{p-1}, C_p = N_{p+1}..N_P), loads a tile of T
+// columns into LDS in bit-reversed order, runs log2(N_p) radix-2 DIT stages in LDS with
+// twiddles omega^(N/N_p * j), multiplies by the four-step twiddle omega^(R_p c k) and writes
+// the tile back in place.  The last pass (C_P = 1) writes rows to their digit-reversed
+// offsets, which leaves the output in natural order.  All passes read and write HBM
+// coalesced in T*32-byte runs.
+#include <string.h>
+
+#include <mutex>
+#include <vector>
+
+#include "field.hpp"
+#include "hostfield.hpp"
+#include "ntt.hpp"
+#include "runtime.hpp"
+
+namespace zk {
+
+static constexpr int kNttThreads = 256;
+
+// omega^e from a table of omega^j, j < Ntab/2 (stride 2^tw_shift for a sub-domain); e < N
+template <class F>
+__device__ __forceinline__ void twiddle(Fe<F> &w, const uint32_t *__restrict__ tw, uint64_t e,
+                                        int logn, int tw_shift, int inverse) {
+  const uint64_t N = 1ull << logn;
+  if (inverse) e = (N - e) & (N - 1);
+  const uint64_t half = N >> 1;
+  const bool neg = e >= half;
+  if (neg) e -= half;
+  fload(w, tw + ((e << tw_shift) * F::N));
+  if (neg) fneg(w, w);
+}
+
+// eta^n from three 2048-entry tables (eta^j, eta^(2^11 j), eta^(2^22 j))
+template <class F>
+__device__ __forceinline__ void eta_pow(Fe<F> &r, const uint32_t *__restrict__ et, uint64_t n) {
+  Fe<F> a, b;
+  fload(a, et + (n & 2047) * F::N);
+  fload(b, et + (2048 + ((n >> 11) & 2047)) * F::N);
+  fmul(r, a, b);
+  if (n >> 22) {
+    fload(a, et + (4096 + ((n >> 22) & 2047)) * F::N);
+    fmul(r, r, a);
+  }
+}
+
+__device__ __forceinline__ uint32_t bitrev(uint32_t x, int bits) {
+  return bits ? (__builtin_bitreverse32(x) >> (32 - bits)) : 0u;
+}
+
+template <class F>
+__device__ __forceinline__ void lds_load(Fe<F> &r, const uint32_t *p) {
+  const uint4 *q = reinterpret_cast<const uint4 *>(p);
+#pragma unroll
+  for (int j = 0; j < F::N / 4; j++) {
+    uint4 v = q[j];
+    r.w[4 * j] = v.x;
+    r.w[4 * j + 1] = v.y;
+    r.w[4 * j + 2] = v.z;
+    r.w[4 * j + 3] = v.w;
+  }
+}
+template <class F>
+__device__ __forceinline__ void lds_store(uint32_t *p, const Fe<F> &a) {
+  uint4 *q = reinterpret_cast<uint4 *>(p);
+#pragma unroll
+  for (int j = 0; j < F::N / 4; j++)
+    q[j] = make_uint4(a.w[4 * j], a.w[4 * j + 1], a.w[4 * j + 2], a.w[4 * j + 3]);
+}
+
+struct PassArgs {
+  const uint32_t *in;
+  uint32_t *out;
+  const uint32_t *tw;       // omega^j table
+  const uint32_t *eta_pre;  // forward shifted: multiply input i by eta^i (first pass only)
+  const uint32_t *eta_post; // inverse shifted: multiply output i by eta^i (last pass only)
+  uint32_t ninv[8];         // N^-1 (Montgomery) for the inverse, applied in the last pass
+  int logn, s, logc, logr, logt, last, inverse, tw_shift, has_ninv;
+  int nsplit;               // number of passes before this (last pass: digit reversal)
+  int split[6];             // log sizes of the earlier passes
+};
+
+template <class F>
+__global__ __launch_bounds__(kNttThreads) void ntt_pass_kernel(PassArgs a) {
+  extern __shared__ uint4 lds_raw[];
+  uint32_t *lds = reinterpret_cast<uint32_t *>(lds_raw);
+  const int S = 1 << a.s, T = 1 << a.logt;
+  const int elems = S << a.logt;
+  const int tid = threadIdx.x;
+
+  uint64_t r = 0, c0 = 0, rho0 = 0;
+  if (!a.last) {
+    const int lg = a.logc - a.logt;
+    r = (uint64_t)blockIdx.x >> lg;
+    c0 = ((uint64_t)blockIdx.x & ((1ull << lg) - 1)) << a.logt;
+  } else {
+    rho0 = (uint64_t)blockIdx.x << a.logt;
+  }
+
+  // ---- load (coalesced over c), optional coset pre-scale, bit-reversed LDS placement
+  for (int idx = tid; idx < elems; idx += kNttThreads) {
+    const uint32_t n = idx >> a.logt, c = idx & (T - 1);
+    uint64_t gi;
+    if (!a.last) {
+      gi = (((r << a.s) + n) << a.logc) | (c0 + c);
+    } else {
+      uint64_t rho = rho0 + c, rr = 0;
+      for (int i = 0; i < a.nsplit; i++) {
+        rr = (rr << a.split[i]) | (rho & ((1ull << a.split[i]) - 1));
+        rho >>= a.split[i];
+      }
+      gi = (rr << a.s) + n;
+    }
+    Fe<F> x;
+    fload(x, a.in + gi * F::N);
+    if (a.eta_pre) {
+      Fe<F> e;
+      eta_pow(e, a.eta_pre, gi);
+      fmul(x, x, e);
+    }
+    lds_store(lds + ((bitrev(n, a.s) << a.logt) | c) * F::N, x);
+  }
+  __syncthreads();
+
+  // ---- radix-2 DIT stages in LDS (natural-order result)
+  for (int lh = 0; lh < a.s; lh++) {
+    const int h = 1 << lh;
+    for (int q = tid; q < (elems >> 1); q += kNttThreads) {
+      const int c = q & (T - 1);
+      const int bf = q >> a.logt;
+      const int j = bf & (h - 1);
+      const int blk = bf >> lh;
+      const int pu = ((((blk << (lh + 1)) | j) << a.logt) | c) * F::N;
+      const int pv = pu + ((h << a.logt) * F::N);
+      Fe<F> u, v;
+      lds_load(u, lds + pu);
+      lds_load(v, lds + pv);
+      if (lh > 0) {
+        Fe<F> w;
+        twiddle(w, a.tw, (uint64_t)j << (a.logn - lh - 1), a.logn, a.tw_shift, a.inverse);
+        fmul(v, v, w);
+      }
+      Fe<F> t;
+      fadd(t, u, v);
+      fsub(v, u, v);
+      lds_store(lds + pu, t);
+      lds_store(lds + pv, v);
+    }
+    __syncthreads();
+  }
+
+  // ---- four-step twiddle and store
+  Fe<F> ninv;
+#pragma unroll
+  for (int j = 0; j < F::N; j++) ninv.w[j] = a.ninv[j];
+  for (int idx = tid; idx < elems; idx += kNttThreads) {
+    const uint32_t k = idx >> a.logt, c = idx & (T - 1);
+    Fe<F> x;
+    lds_load(x, lds + idx * F::N);
+    uint64_t go;
+    if (!a.last) {
+      const uint64_t col = c0 + c;
+      const uint64_t e = ((col * k) << a.logr);
+      if (e) {
+        Fe<F> w;
+        twiddle(w, a.tw, e, a.logn, a.tw_shift, a.inverse);
+        fmul(x, x, w);
+      }
+      go = (((r << a.s) + k) << a.logc) | col;
+    } else {
+      go = (rho0 + c) + ((uint64_t)k << a.logr);
+      if (a.has_ninv) fmul(x, x, ninv);
+      if (a.eta_post) {
+        Fe<F> e;
+        eta_pow(e, a.eta_post, go);
+        fmul(x, x, e);
+      }
+    }
+    fstore(a.out + go * F::N, x);
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// table builders
+
+// tab[j] = base^(j * step) for j < cnt
+template <class F>
+__global__ void pow_table_kernel(uint32_t *tab, Fe<F> base, uint64_t step, int cnt) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= cnt) return;
+  Fe<F> r;
+  fpow_u64(r, base, step * (uint64_t)j);
+  fstore(tab + (size_t)j * F::N, r);
+}
+
+// tw[e] = lo[e & 2047] * hi[e >> 11]
+template <class F>
+__global__ void twiddle_table_kernel(uint32_t *tw, const uint32_t *lo, const uint32_t *hi, uint64_t cnt) {
+  uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= cnt) return;
+  Fe<F> a, b;
+  fload(a, lo + (e & 2047) * F::N);
+  fload(b, hi + (e >> 11) * F::N);
+  fmul(a, a, b);
+  fstore(tw + e * F::N, a);
+}
+
+// ---------------------------------------------------------------------------------------
+// reference butterfly network, for generators whose order is not 2^m
+// (bn128_poly_mont.c:418-447 forward; :565-596 inverse), one launch per radix-2 stage.
+//
+// Forward, stage t = 1..m with s = N / 2^t:  Y_t[r][j]        = Y_{t-1}[r][j] + g^(js) Y_{t-1}[r+s][j]
+//                                            Y_t[r][j+2^(t-1)] = Y_{t-1}[r][j] - g^(js) Y_{t-1}[r+s][j]
+// with Y_t stored as [s][2^t] and Y_0 = x.  g^(js) comes from gp[j s] = g^(js) (powers table).
+template <class F>
+__global__ void ntt_net_fwd_stage(const uint32_t *in, uint32_t *out, const uint32_t *gp, int m, int t) {
+  const uint64_t N = 1ull << m;
+  uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;  // butterfly id < N/2
+  if (q >= (N >> 1)) return;
+  const int ht = t - 1;
+  const uint64_t hw = 1ull << ht;          // 2^(t-1)
+  const uint64_t s = N >> t;
+  const uint64_t r = q >> ht, j = q & (hw - 1);
+  Fe<F> u, v, w;
+  fload(u, in + ((r << ht) + j) * F::N);          // Y_{t-1}[r][j], layout [2s][2^(t-1)]
+  fload(v, in + (((r + s) << ht) + j) * F::N);    // Y_{t-1}[r+s][j]
+  fload(w, gp + (j * s) * F::N);
+  fmul(v, v, w);
+  Fe<F> a;
+  fadd(a, u, v);
+  fsub(v, u, v);
+  fstore(out + ((r << t) + j) * F::N, a);
+  fstore(out + ((r << t) + j + hw) * F::N, v);
+}
+
+// Inverse, reference DIF with halving.  Level d = 0..m-1 works on blocks of size M = N/2^d
+// with stride 2^d in the *output* ordering; written iteratively over a [2^d][M] layout:
+//   Z_{d+1}[2b  ][j] = (Z_d[b][j] + Z_d[b][j+M/2]) / 2
+//   Z_{d+1}[2b+1][j] = (Z_d[b][j] - Z_d[b][j+M/2]) * gp[j 2^d]     (gp[i] = 1/2 g^-i)
+// and the final level (M = 2) uses halving for both outputs.  The output index of block b
+// after all levels is the bit reversal of b, exactly as the recursion's tgt_stride doubling.
+template <class F>
+__global__ void ntt_net_inv_stage(const uint32_t *in, uint32_t *out, const uint32_t *gp, int m, int d) {
+  const uint64_t N = 1ull << m;
+  uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (N >> 1)) return;
+  const int lm = m - d;               // log block size
+  const uint64_t M = 1ull << lm, half = M >> 1;
+  const uint64_t b = q >> (lm - 1), j = q & (half - 1);
+  Fe<F> x, y;
+  fload(x, in + ((b << lm) + j) * F::N);
+  fload(y, in + ((b << lm) + j + half) * F::N);
+  Fe<F> s, df;
+  fadd(s, x, y);
+  fsub(df, x, y);
+  fhalf(s, s);
+  if (lm == 1) {
+    fhalf(df, df);
+  } else {
+    Fe<F> w;
+    fload(w, gp + ((j << d)) * F::N);
+    fmul(df, df, w);
+  }
+  // next layout [2^(d+1)][M/2]: block 2b and 2b+1
+  fstore(out + (((2 * b) << (lm - 1)) + j) * F::N, s);
+  fstore(out + (((2 * b + 1) << (lm - 1)) + j) * F::N, df);
+}
+
+// final permutation of the inverse network: out[bitrev(b)] = Z_m[b]
+template <class F>
+__global__ void bitrev_copy_kernel(const uint32_t *in, uint32_t *out, int m) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >> m) return;
+  uint64_t j = __builtin_bitreverse64(i) >> (64 - m);
+  Fe<F> x;
+  fload(x, in + i * F::N);
+  fstore(out + j * F::N, x);
+}
+
+// sequential powers table for the network path (host-seeded, built on device in blocks)
+// gp[j] = first * g^j
+template <class F>
+__global__ void powers_kernel(uint32_t *gp, Fe<F> first, Fe<F> g, uint64_t cnt) {
+  uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= cnt) return;
+  Fe<F> r;
+  fpow_u64(r, g, j);
+  fmul(r, r, first);
+  fstore(gp + j * F::N, r);
+}
+
+// ---------------------------------------------------------------------------------------
+// host side
+
+template <class F>
+static Fe<F> to_dev(const zkh::Fe &a) {
+  Fe<F> r;
+  for (int j = 0; j < F::N; j++) r.w[j] = (uint32_t)(a.w[j / 2] >> (32 * (j & 1)));
+  return r;
+}
+
+struct TwEntry {
+  int curve;
+  int m;
+  zkh::Fe gen;
+  DevBuf buf;  // omega^j, j < 2^(m-1)
+};
+static std::mutex g_tw_lock;
+static std::vector<TwEntry *> g_tw;
+
+static const zkh::FieldDesc &fr_desc(int curve) { return curve == 0 ? zkh::BN254_FR : zkh::BLS381_FR; }
+
+// returns true when g has multiplicative order exactly 2^m
+bool ntt_generator_ok(int curve, int m, const uint64_t *gen) {
+  const zkh::FieldDesc &f = fr_desc(curve);
+  zkh::Fe g = zkh::load(f, gen);
+  if (m == 0) return zkh::is_eq(f, g, zkh::one(f));
+  for (int i = 0; i < m - 1; i++) g = zkh::sqr(f, g);
+  zkh::Fe mone = zkh::neg(f, zkh::one(f));
+  return zkh::is_eq(f, g, mone);
+}
+
+template <class F>
+static int build_twiddles(int curve, int m, const zkh::Fe &g, DevBuf &buf, hipStream_t st) {
+  const uint64_t cnt = m >= 1 ? (1ull << (m - 1)) : 1;
+  ZK_TRY(buf.alloc(cnt * F::N * 4));
+  DevBuf lo, hi;
+  ZK_TRY(lo.alloc(2048 * F::N * 4));
+  const uint64_t hicnt = (cnt + 2047) / 2048;
+  ZK_TRY(hi.alloc(hicnt * F::N * 4));
+  Fe<F> gd = to_dev<F>(g);
+  hipLaunchKernelGGL(pow_table_kernel<F>, dim3(2048 / 256), dim3(256), 0, st, lo.as<uint32_t>(), gd, 1ull, 2048);
+  hipLaunchKernelGGL(pow_table_kernel<F>, dim3((hicnt + 255) / 256), dim3(256), 0, st, hi.as<uint32_t>(), gd,
+                     2048ull, (int)hicnt);
+  hipLaunchKernelGGL(twiddle_table_kernel<F>, dim3((cnt + 255) / 256), dim3(256), 0, st, buf.as<uint32_t>(),
+                     lo.as<uint32_t>(), hi.as<uint32_t>(), cnt);
+  ZK_HIP(hipGetLastError());
+  ZK_HIP(hipStreamSynchronize(st));  // lo/hi are released on return
+  (void)curve;
+  return ZK_OK;
+}
+
+// Finds (or builds) a table of omega^j for the domain generated by `gen` of order 2^m.
+// A cached table of a larger domain whose generator squares down to `gen` is reused with a
+// stride (all FFT subgroups of one field come from one 2-adic generator).
+template <class F>
+static int get_twiddles(int curve, int m, const uint64_t *gen, const uint32_t **tab, int *shift, hipStream_t st) {
+  const zkh::FieldDesc &f = fr_desc(curve);
+  zkh::Fe g = zkh::load(f, gen);
+  std::lock_guard<std::mutex> lk(g_tw_lock);
+  for (TwEntry *e : g_tw) {
+    if (e->curve != curve || e->m < m) continue;
+    zkh::Fe h = e->gen;
+    for (int i = 0; i < e->m - m; i++) h = zkh::sqr(f, h);
+    if (zkh::is_eq(f, h, g)) {
+      *tab = e->buf.as<uint32_t>();
+      *shift = e->m - m;
+      return ZK_OK;
+    }
+  }
+  TwEntry *e = new TwEntry();
+  e->curve = curve;
+  e->m = m;
+  e->gen = g;
+  int s = build_twiddles<F>(curve, m, g, e->buf, st);
+  if (s != ZK_OK) {
+    delete e;
+    return s;
+  }
+  g_tw.push_back(e);
+  *tab = e->buf.as<uint32_t>();
+  *shift = 0;
+  return ZK_OK;
+}
+
+template <class F>
+static int build_eta_tables(const zkh::Fe &eta, DevBuf &buf, hipStream_t st) {
+  ZK_TRY(buf.alloc(3 * 2048 * F::N * 4));
+  Fe<F> e = to_dev<F>(eta);
+  uint32_t *b = buf.as<uint32_t>();
+  hipLaunchKernelGGL(pow_table_kernel<F>, dim3(8), dim3(256), 0, st, b, e, 1ull, 2048);
+  hipLaunchKernelGGL(pow_table_kernel<F>, dim3(8), dim3(256), 0, st, b + 2048 * F::N, e, 2048ull, 2048);
+  hipLaunchKernelGGL(pow_table_kernel<F>, dim3(8), dim3(256), 0, st, b + 4096 * F::N, e, 1ull << 22, 2048);
+  ZK_HIP(hipGetLastError());
+  return ZK_OK;
+}
+
+NttPlan ntt_plan(int m, int smax, int tile_log) {
+  NttPlan p;
+  p.m = m;
+  if (smax < 1) smax = 1;
+  if (smax > 11) smax = 11;
+  p.npass = (m + smax - 1) / smax;
+  if (p.npass < 1) p.npass = 1;
+  int rem = m;
+  for (int i = 0; i < p.npass; i++) {
+    int left = p.npass - i;
+    p.s[i] = (rem + left - 1) / left;
+    rem -= p.s[i];
+  }
+  p.tile_log = tile_log;
+  return p;
+}
+
+template <class F>
+static int ntt_fast(const NttPlan &plan, const uint32_t *d_in, uint32_t *d_out, uint32_t *d_tmp,
+                    const uint32_t *tw, int tw_shift, int inverse, const uint32_t *eta_pre,
+                    const uint32_t *eta_post, const Fe<F> *ninv, hipStream_t st) {
+  const int m = plan.m;
+  int logr = 0;
+  for (int p = 0; p < plan.npass; p++) {
+    PassArgs a;
+    memset(&a, 0, sizeof a);
+    const bool last = p == plan.npass - 1;
+    a.in = p == 0 ? d_in : d_tmp;
+    a.out = last ? d_out : d_tmp;
+    a.tw = tw;
+    a.tw_shift = tw_shift;
+    a.logn = m;
+    a.s = plan.s[p];
+    a.logr = logr;
+    a.logc = m - logr - a.s;
+    a.last = last;
+    a.inverse = inverse;
+    a.eta_pre = p == 0 ? eta_pre : nullptr;
+    a.eta_post = last ? eta_post : nullptr;
+    if (last && ninv) {
+      a.has_ninv = 1;
+      for (int j = 0; j < F::N; j++) a.ninv[j] = ninv->w[j];
+    }
+    int tl = plan.tile_log - a.s;
+    if (tl < 0) tl = 0;
+    const int avail = last ? logr : a.logc;
+    if (tl > avail) tl = avail;
+    a.logt = tl;
+    a.nsplit = p;
+    for (int i = 0; i < p; i++) a.split[i] = plan.s[i];
+    const uint64_t groups = last ? (1ull << (logr - tl)) : ((1ull << logr) << (a.logc - tl));
+    const size_t lds = ((size_t)1 << (a.s + tl)) * F::N * 4;
+    hipLaunchKernelGGL(ntt_pass_kernel<F>, dim3((unsigned)groups), dim3(kNttThreads), lds, st, a);
+    ZK_HIP(hipGetLastError());
+    logr += a.s;
+  }
+  return ZK_OK;
+}
+
+template <class F>
+static int ntt_network(int curve, int m, const uint64_t *gen, const uint32_t *d_in, uint32_t *d_out,
+                       uint32_t *d_tmp, int inverse, hipStream_t st) {
+  const zkh::FieldDesc &f = fr_desc(curve);
+  const uint64_t N = 1ull << m;
+  const uint64_t half = N >> 1;
+  DevBuf gp;
+  ZK_TRY(gp.alloc((half ? half : 1) * F::N * 4));
+  zkh::Fe g = zkh::load(f, gen);
+  zkh::Fe first = zkh::one(f);
+  if (inverse) {
+    g = zkh::inv(f, g);
+    zkh::Fe two = zkh::from_u64(f, 2);
+    first = zkh::inv(f, two);
+  }
+  if (half)
+    hipLaunchKernelGGL(powers_kernel<F>, dim3((half + 255) / 256), dim3(256), 0, st, gp.as<uint32_t>(),
+                       to_dev<F>(first), to_dev<F>(g), half);
+  const unsigned blocks = (unsigned)((half + 255) / 256);
+  if (!inverse) {
+    // ping-pong so that the last stage lands in d_out
+    const uint32_t *src = d_in;
+    for (int t = 1; t <= m; t++) {
+      uint32_t *dst = ((m - t) % 2 == 0) ? d_out : d_tmp;
+      hipLaunchKernelGGL(ntt_net_fwd_stage<F>, dim3(blocks), dim3(256), 0, st, src, dst, gp.as<uint32_t>(), m, t);
+      src = dst;
+    }
+  } else {
+    const uint32_t *src = d_in;
+    uint32_t *bufs[2] = {d_tmp, d_out};
+    int k = 0;
+    for (int d = 0; d < m; d++) {
+      uint32_t *dst = bufs[k];
+      hipLaunchKernelGGL(ntt_net_inv_stage<F>, dim3(blocks), dim3(256), 0, st, src, dst, gp.as<uint32_t>(), m, d);
+      src = dst;
+      k ^= 1;
+    }
+    // src holds Z_m in block order; permute into d_out (via d_tmp if needed)
+    uint32_t *dst = d_out;
+    if (src == d_out) {
+      ZK_HIP(hipMemcpyAsync(d_tmp, d_out, N * F::N * 4, hipMemcpyDeviceToDevice, st));
+      src = d_tmp;
+    }
+    hipLaunchKernelGGL(bitrev_copy_kernel<F>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, src, dst, m);
+  }
+  ZK_HIP(hipGetLastError());
+  ZK_HIP(hipStreamSynchronize(st));  // gp is released on return
+  return ZK_OK;
+}
+
+static NttTuning g_tuning = {8, 11};
+void ntt_set_tuning(const NttTuning &t) { g_tuning = t; }
+NttTuning ntt_get_tuning() { return g_tuning; }
+
+template <class F>
+static int ntt_device_t(int curve, int m, const uint64_t *gen, const uint64_t *eta, int flags,
+                        const uint32_t *d_in, uint32_t *d_out, hipStream_t st, const NttTuning *tune) {
+  const uint64_t N = 1ull << m;
+  const bool inverse = flags & NTT_INVERSE;
+  if (m == 0 && !eta) {
+    if (d_in != d_out) ZK_HIP(hipMemcpyAsync(d_out, d_in, F::N * 4, hipMemcpyDeviceToDevice, st));
+    return ZK_OK;
+  }
+  DevBuf tmp;
+  ZK_TRY(tmp.alloc(N * F::N * 4));
+  const zkh::FieldDesc &f = fr_desc(curve);
+  const bool ok = ntt_generator_ok(curve, m, gen);
+  if (!ok) {
+    // reference network (exact for any generator); coset scaling through the fast kernels is
+    // not available there, so shifted transforms scale with a separate elementwise pass
+    if (eta) {
+      set_error("shifted NTT with a generator of order != 2^m is not supported");
+      return ZK_ENOTROOT;
+    }
+    DevBuf alias;
+    if (d_in == d_out) {
+      ZK_TRY(alias.alloc(N * F::N * 4));
+      ZK_HIP(hipMemcpyAsync(alias.p, d_in, N * F::N * 4, hipMemcpyDeviceToDevice, st));
+      d_in = alias.as<uint32_t>();
+    }
+    return ntt_network<F>(curve, m, gen, d_in, d_out, tmp.as<uint32_t>(), inverse, st);
+  }
+  const uint32_t *tw = nullptr;
+  int tw_shift = 0;
+  if (m >= 1) ZK_TRY(get_twiddles<F>(curve, m, gen, &tw, &tw_shift, st));
+  DevBuf etab;
+  if (eta) ZK_TRY(build_eta_tables<F>(zkh::load(f, eta), etab, st));
+  Fe<F> ninv;
+  if (inverse) ninv = to_dev<F>(zkh::inv(f, zkh::from_u64(f, N)));
+  const NttTuning tu = tune ? *tune : g_tuning;
+  NttPlan plan = ntt_plan(m, tu.smax, tu.tile_log);
+  const uint32_t *ep = (eta && !inverse) ? etab.as<uint32_t>() : nullptr;
+  const uint32_t *eo = (eta && inverse) ? etab.as<uint32_t>() : nullptr;
+  ZK_TRY(ntt_fast<F>(plan, d_in, d_out, tmp.as<uint32_t>(), tw, tw_shift, inverse, ep, eo,
+                     inverse ? &ninv : nullptr, st));
+  if (eta) ZK_HIP(hipStreamSynchronize(st));  // eta tables are released on return
+  ZK_HIP(hipStreamSynchronize(st));           // tmp is released on return
+  return ZK_OK;
+}
+
+int ntt_device(int curve, int m, const uint64_t *gen, const uint64_t *eta, int flags, const void *d_in,
+               void *d_out, hipStream_t st, const NttTuning *tune) {
+  if (m < 0 || m > 30 || (curve != 0 && curve != 1)) {
+    set_error("ntt: bad curve id or log size");
+    return ZK_EINVAL;
+  }
+  if (curve == 0)
+    return ntt_device_t<Bn254Fr>(curve, m, gen, eta, flags, (const uint32_t *)d_in, (uint32_t *)d_out, st, tune);
+  return ntt_device_t<Bls381Fr>(curve, m, gen, eta, flags, (const uint32_t *)d_in, (uint32_t *)d_out, st, tune);
+}
+
+}  // namespace zk

@@ -1,0 +1,39 @@
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace zk {
+
+enum NttFlags : int {
+  NTT_FORWARD = 0,
+  NTT_INVERSE = 1,
+};
+
+// Schedule knobs of the fast path: passes of at most 2^smax points, LDS tiles of 2^tile_log
+// field elements per workgroup.
+struct NttTuning {
+  int smax;
+  int tile_log;
+};
+
+struct NttPlan {
+  int m = 0;
+  int npass = 0;
+  int s[8] = {0};
+  int tile_log = 11;
+};
+
+NttPlan ntt_plan(int m, int smax, int tile_log);
+void ntt_set_tuning(const NttTuning &t);
+NttTuning ntt_get_tuning();
+
+// true when gen has multiplicative order exactly 2^m (the fast path's precondition)
+bool ntt_generator_ok(int curve, int m, const uint64_t *gen);
+
+// Device-buffer NTT.  gen/eta are host pointers to Montgomery Fr (4 x u64); eta == nullptr
+// means no coset shift.  d_in may equal d_out.  Enqueued on `st`; returns after the work is
+// complete (temporary buffers are released).
+int ntt_device(int curve, int m, const uint64_t *gen, const uint64_t *eta, int flags, const void *d_in,
+               void *d_out, hipStream_t st, const NttTuning *tune);
+
+}  // namespace zk

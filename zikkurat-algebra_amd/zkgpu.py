@@ -1,0 +1,280 @@
+"""Python host interface of the MI355X MSM/NTT library (ctypes over include/zkgpu.h).
+
+The names mirror the reference's polymorphic API so that parity tests read like the
+reference's own properties:
+
+* ``msm`` / ``msm_std``       -- Class/Curve.hs:92 ``msm`` (Montgomery scalars) and the
+                                 standard-form ``msmStd`` (G1/Proj.hs:241-255)
+* ``ntt`` / ``intt``          -- Class/Poly.hs:57-58 (BN128/Poly.hs:406-429)
+* ``shifted_ntt`` / ``shifted_intt`` -- Class/Poly.hs:60-63 (BN128/Poly.hs:433-464)
+* ``asymm_ntt``               -- Class/Poly.hs:65 (BN128/Poly.hs:468-485)
+
+Arrays are numpy ``uint64`` with the reference's flat limb layout.  Size mismatches raise
+``ValueError`` like the Haskell layer's ``error`` (Poly.hs:409, Proj.hs:248); device failures
+raise ``ZkgError``.  There is no CPU fallback: without the built library (or a GPU) every call
+fails.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libzkgpu.so"
+
+BN254, BLS12_381 = 0, 1
+PREFIX = {BN254: "bn128", BLS12_381: "bls12_381"}
+FP_LIMBS = {BN254: 4, BLS12_381: 6}
+FR_LIMBS = 4
+
+MSM_MONT, MSM_PROJ_OUT, MSM_JAC_OUT = 1, 2, 4
+NTT_FORWARD, NTT_INVERSE = 0, 1
+
+u64p = ctypes.POINTER(ctypes.c_uint64)
+vp = ctypes.c_void_p
+
+
+class ZkgError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Loads libzkgpu.so (built by zikkurat-algebra_amd/build.py); fails loudly if absent."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ZkgError(f"{LIB_PATH} is not built; run python zikkurat-algebra_amd/build.py")
+        L = ctypes.CDLL(str(LIB_PATH))
+        L.zkg_last_error.restype = ctypes.c_char_p
+        for name in ("zkg_malloc",):
+            getattr(L, name).argtypes = [ctypes.POINTER(vp), ctypes.c_size_t]
+        L.zkg_free.argtypes = [vp]
+        L.zkg_memcpy_h2d.argtypes = [vp, vp, ctypes.c_size_t]
+        L.zkg_memcpy_d2h.argtypes = [vp, vp, ctypes.c_size_t]
+        L.zkg_msm.argtypes = [ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, vp, u64p, ctypes.c_int, vp]
+        L.zkg_ntt.argtypes = [ctypes.c_int, ctypes.c_int, u64p, u64p, ctypes.c_int, vp, vp]
+        L.zkg_fft_generator.argtypes = [ctypes.c_int, ctypes.c_int, u64p]
+        L.zkg_field_op.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_size_t, vp, vp, vp]
+        L.zkg_g1_scalar_mul_gen.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp]
+        L.zkg_microbench.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+        L.zkg_last_timing.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                      ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.zkg_device_info.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_size_t)]
+        _lib = L
+    return _lib
+
+
+def _check(status):
+    if status != 0:
+        raise ZkgError(lib().zkg_last_error().decode(errors="replace"))
+
+
+def _p(a):
+    return a.ctypes.data_as(u64p)
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.uint64)
+
+
+# ---------------------------------------------------------------------------------------
+# device buffers
+
+class DeviceArray:
+    """A device buffer holding uint64 words (freed into the library's caching allocator)."""
+
+    def __init__(self, nwords: int):
+        self.nwords = int(nwords)
+        self.ptr = vp()
+        _check(lib().zkg_malloc(ctypes.byref(self.ptr), max(8, 8 * self.nwords)))
+
+    @classmethod
+    def from_host(cls, a):
+        a = _c(a)
+        d = cls(a.size)
+        _check(lib().zkg_memcpy_h2d(d.ptr, a.ctypes.data_as(vp), 8 * a.size))
+        return d
+
+    def to_host(self, shape=None):
+        out = np.empty(self.nwords, dtype=np.uint64)
+        _check(lib().zkg_memcpy_d2h(out.ctypes.data_as(vp), self.ptr, 8 * self.nwords))
+        return out.reshape(shape) if shape is not None else out
+
+    def free(self):
+        if self.ptr:
+            lib().zkg_free(self.ptr)
+            self.ptr = vp()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def synchronize():
+    _check(lib().zkg_synchronize())
+
+
+# ---------------------------------------------------------------------------------------
+# reference entry points (host buffers)
+
+def _ref_fn(curve, name):
+    f = getattr(lib(), f"{PREFIX[curve]}_{name}")
+    return f
+
+
+def msm_ref_symbol(curve, coord, kind, out, scalars, points, expo_nlimbs=4):
+    """Calls <c>_G1_<coord>_MSM_<kind>_coeff_<out>_out exactly as a C caller would."""
+    scalars, points = _c(scalars), _c(points)
+    n = points.shape[0] if points.ndim == 2 else points.size // (2 * FP_LIMBS[curve])
+    size = 2 * FP_LIMBS[curve] if out == "affine" else 3 * FP_LIMBS[curve]
+    res = np.zeros(size, np.uint64)
+    f = _ref_fn(curve, f"G1_{coord}_MSM_{kind}_coeff_{out}_out")
+    f.argtypes = [ctypes.c_int, u64p, u64p, u64p, ctypes.c_int]
+    f(n, _p(scalars), _p(points), _p(res), expo_nlimbs)
+    return res
+
+
+def msm_std(curve, scalars, points, expo_nlimbs=4):
+    """Sum s_i P_i (standard-form scalars), affine output; infinity = all 0xFF words."""
+    scalars, points = _c(scalars), _c(points)
+    if scalars.size != points.size // (2 * FP_LIMBS[curve]) * expo_nlimbs:
+        raise ValueError("msm: scalar and point counts differ")
+    return msm_ref_symbol(curve, "proj", "std", "affine", scalars, points, expo_nlimbs)
+
+
+def msm(curve, scalars, points):
+    """Montgomery-form scalars (Class/Curve.hs:92 via G1/Proj.hs:254)."""
+    scalars, points = _c(scalars), _c(points)
+    if scalars.size // 4 != points.size // (2 * FP_LIMBS[curve]):
+        raise ValueError("msm: scalar and point counts differ")
+    return msm_ref_symbol(curve, "proj", "mont", "affine", scalars, points, 4)
+
+
+def _ntt_call(name, curve, *args):
+    f = _ref_fn(curve, name)
+    f(*args)
+
+
+def ntt(curve, m, gen, src):
+    src = _c(src)
+    if src.shape[0] != (1 << m):
+        raise ValueError("forwardNTT: subgroup size differs from the array size")
+    out = np.zeros_like(src)
+    f = _ref_fn(curve, "poly_mont_ntt_forward")
+    f.argtypes = [ctypes.c_int, u64p, u64p, u64p]
+    f(m, _p(_c(gen)), _p(src), _p(out))
+    return out
+
+
+def intt(curve, m, gen, src):
+    src = _c(src)
+    if src.shape[0] != (1 << m):
+        raise ValueError("inverseNTT: subgroup size differs from the array size")
+    out = np.zeros_like(src)
+    f = _ref_fn(curve, "poly_mont_ntt_inverse")
+    f.argtypes = [ctypes.c_int, u64p, u64p, u64p]
+    f(m, _p(_c(gen)), _p(src), _p(out))
+    return out
+
+
+def shifted_ntt(curve, m, gen, eta, src):
+    src = _c(src)
+    if src.shape[0] != (1 << m):
+        raise ValueError("shiftedForwardNTT: subgroup size differs from the array size")
+    out = np.zeros_like(src)
+    f = _ref_fn(curve, "poly_mont_ntt_forward_shifted")
+    f.argtypes = [u64p, ctypes.c_int, u64p, u64p, u64p]
+    f(_p(_c(eta)), m, _p(_c(gen)), _p(src), _p(out))
+    return out
+
+
+def shifted_intt(curve, m, gen, eta, src):
+    src = _c(src)
+    if src.shape[0] != (1 << m):
+        raise ValueError("shiftedInverseNTT: subgroup size differs from the array size")
+    out = np.zeros_like(src)
+    f = _ref_fn(curve, "poly_mont_ntt_inverse_shifted")
+    f.argtypes = [u64p, ctypes.c_int, u64p, u64p, u64p]
+    f(_p(_c(eta)), m, _p(_c(gen)), _p(src), _p(out))
+    return out
+
+
+def asymm_ntt(curve, m_src, gen_src, src, m_tgt, gen_tgt):
+    src = _c(src)
+    if src.shape[0] != (1 << m_src):
+        raise ValueError("asymmForwardNTT: subgroup size differs from the array size")
+    if m_tgt < m_src:
+        raise ValueError("asymmForwardNTT: target subgroup size should be at least the source subgroup src")
+    out = np.zeros(((1 << m_tgt), 4), np.uint64)
+    f = _ref_fn(curve, "poly_mont_ntt_forward_asymmetric")
+    f.argtypes = [ctypes.c_int, ctypes.c_int, u64p, u64p, u64p, u64p]
+    f(m_src, m_tgt, _p(_c(gen_src)), _p(_c(gen_tgt)), _p(src), _p(out))
+    return out
+
+
+def fft_generator(curve, m):
+    out = np.zeros(4, np.uint64)
+    _check(lib().zkg_fft_generator(curve, m, _p(out)))
+    return out
+
+
+# ---------------------------------------------------------------------------------------
+# extension API (device buffers)
+
+def msm_device(curve, n, d_scalars, d_points, expo_nlimbs=4, flags=0, window=0, seg_log=0):
+    size = 3 * FP_LIMBS[curve] if flags & (MSM_PROJ_OUT | MSM_JAC_OUT) else 2 * FP_LIMBS[curve]
+    res = np.zeros(size, np.uint64)
+    sched = None
+    if window or seg_log:
+        sched = (ctypes.c_int * 8)(window, seg_log, 0, 0, 0, 0, 0, 0)
+    _check(lib().zkg_msm(curve, n, d_scalars.ptr, expo_nlimbs, d_points.ptr, _p(res), flags,
+                         ctypes.cast(sched, vp) if sched is not None else None))
+    return res
+
+
+def ntt_device(curve, m, gen, d_src, d_tgt, inverse=False, eta=None):
+    _check(lib().zkg_ntt(curve, m, _p(_c(gen)), _p(_c(eta)) if eta is not None else None,
+                         NTT_INVERSE if inverse else NTT_FORWARD, d_src.ptr, d_tgt.ptr))
+
+
+def field_op(curve, field, op, d_a, d_b, d_out, n):
+    _check(lib().zkg_field_op(curve, field, op, n, d_a.ptr, d_b.ptr if d_b is not None else None, d_out.ptr))
+
+
+def scalar_mul_gen(curve, d_scalars, n):
+    out = DeviceArray(n * 2 * FP_LIMBS[curve])
+    _check(lib().zkg_g1_scalar_mul_gen(curve, n, d_scalars.ptr, out.ptr))
+    return out
+
+
+def microbench(which):
+    r = ctypes.c_double()
+    _check(lib().zkg_microbench(which, ctypes.byref(r)))
+    return r.value
+
+
+def last_timing():
+    tot = ctypes.c_double()
+    ph = (ctypes.c_double * 8)()
+    n = ctypes.c_int()
+    _check(lib().zkg_last_timing(ctypes.byref(tot), ph, 8, ctypes.byref(n)))
+    return tot.value, list(ph)[:n.value]
+
+
+def device_info():
+    cus = ctypes.c_int()
+    hbm = ctypes.c_size_t()
+    _check(lib().zkg_device_info(ctypes.byref(cus), ctypes.byref(hbm)))
+    return cus.value, hbm.value
+
+
+FOP = dict(add=0, sub=1, mul=2, sqr=3, neg=4, inv=5, half=6, to_std=7, from_std=8)
+MB_IMUL, MB_COPY, MB_FMUL = 0, 1, 2
