@@ -140,6 +140,8 @@ int zkg_microbench(int which, double *result);
 
 /* timing of the last zkg_msm / zkg_ntt call on the library stream (HIP events), per phase */
 int zkg_last_timing(double *ms_total, double *ms_phases, int max_phases, int *nphases);
+/* schedule facts of the last MSM: {window c, windows W, sorted (nonzero-digit) entries, heavy buckets} */
+int zkg_msm_last_stats(long long *out4);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,105 @@
+"""Writes the golden fixtures under tests/golden/ from the reference's own C.
+
+The reference path (lib/cbits/...) is compiled from its sources by `make -C oracle ref` into
+oracle/_ref/libzkref.so; this script calls its entry points on seeded inputs and stores inputs
+and outputs (small cases) or output SHA-256 digests (large cases, inputs regenerated from the
+seed by tests/zkmath.py:seeded_fr_vector).  Run from the repository root:
+
+    make -C oracle ref && python tests/golden/gen_golden.py
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+sys.path.insert(0, str(HERE.parent))
+
+from cref import Oracle, Ref, ref  # noqa: E402
+from zkmath import BN, BLS, ints_to_array, random_scalars, seeded_fr_vector, to_limbs  # noqa: E402
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype=np.uint64).tobytes()).hexdigest()
+
+
+def msm_inputs(C, n, seed):
+    ks = random_scalars(seed, n, C.r)
+    ss = random_scalars(seed + 1, n, C.r)
+    if n >= 8:
+        ss[0], ss[1], ss[2], ss[3] = 0, C.r - 1, (1 << 64) - 1, (1 << 253) + 12345
+        ks[4] = ks[5]
+        ss[4] = ss[5]
+        ks[6] = (C.r - ks[7]) % C.r
+        ss[6] = ss[7]
+        ks[3] = 0
+    return ks, ss
+
+
+def main():
+    assert ref() is not None, "build the reference first: make -C oracle ref"
+    out = {}
+    manifest = {"generator": "tests/golden/gen_golden.py", "reference": "oracle/_ref/libzkref.so (lib/cbits sources, clang -O2)",
+                "msm": [], "ntt": [], "large": []}
+    for C in (BN, BLS):
+        R, O = Ref(C), Oracle(C)
+        for n in (1, 3, 17, 256, 1000):
+            ks, ss = msm_inputs(C, n, 4000 + n + 100 * C.id)
+            P = O.points_from_scalars(ks)
+            S = ints_to_array(ss, 4)
+            Sm = ints_to_array([C.fr_to_mont(s % C.r) for s in ss], 4)
+            key = f"{C.name}_msm_{n}"
+            out[key + "_points"] = P
+            out[key + "_scalars"] = S
+            out[key + "_scalars_mont"] = Sm
+            for coord in ("proj", "jac"):
+                for kind, sc in (("std", S), ("mont", Sm)):
+                    out[f"{key}_{coord}_{kind}_affine"] = R.msm(sc, P, coord=coord, mont=kind == "mont")
+            manifest["msm"].append(key)
+        for m in range(2, 13):
+            X = seeded_fr_vector(7000 + m + 100 * C.id, 1 << m, C.r)
+            X = ints_to_array([C.fr_to_mont(int(sum(int(w) << (64 * i) for i, w in enumerate(row)))) for row in X], 4)
+            g = np.array(to_limbs(C.fr_to_mont(C.fft_root(m)), 4), np.uint64)
+            key = f"{C.name}_ntt_{m}"
+            out[key + "_src"] = X
+            out[key + "_gen"] = g
+            out[key + "_fwd"] = R.ntt(m, g, X)
+            out[key + "_inv"] = R.ntt(m, g, X, inverse=True)
+            if m in (2, 5, 9):
+                eta = np.array(to_limbs(C.fr_to_mont(random_scalars(8000 + m, 1, C.r)[0]), 4), np.uint64)
+                out[key + "_eta"] = eta
+                out[key + "_fwd_shifted"] = R.ntt_shifted(eta, m, g, X)
+                out[key + "_inv_shifted"] = R.ntt_shifted(eta, m, g, X, inverse=True)
+            manifest["ntt"].append(key)
+        for ms, mt in ((2, 4), (3, 7), (5, 6)):
+            X = out[f"{C.name}_ntt_{ms}_src"]
+            gs, gt = out[f"{C.name}_ntt_{ms}_gen"], np.array(to_limbs(C.fr_to_mont(C.fft_root(mt)), 4), np.uint64)
+            out[f"{C.name}_asym_{ms}_{mt}"] = R.ntt_asym(ms, mt, gs, gt, X)
+        # large NTTs: output digests only (inputs from seeded_fr_vector, taken as Montgomery words)
+        for m in (16, 20):
+            X = seeded_fr_vector(9000 + m + 100 * C.id, 1 << m, C.r)
+            g = np.array(to_limbs(C.fr_to_mont(C.fft_root(m)), 4), np.uint64)
+            manifest["large"].append({"curve": C.name, "kind": "ntt_fwd", "m": m, "seed": 9000 + m + 100 * C.id,
+                                      "sha256": sha(R.ntt(m, g, X)), "inv_sha256": sha(R.ntt(m, g, X, inverse=True))})
+        print(C.name, "done", flush=True)
+    # large MSM: reference output for points k_i G and scalars s_i from seeded vectors
+    for C, logn in ((BN, 14), (BN, 16), (BLS, 14)):
+        n = 1 << logn
+        kseed, sseed = 11000 + logn + 100 * C.id, 12000 + logn + 100 * C.id
+        K = seeded_fr_vector(kseed, n, C.r)
+        S = seeded_fr_vector(sseed, n, C.r)
+        P = Oracle(C).points_from_scalars([int(sum(int(w) << (64 * i) for i, w in enumerate(row))) for row in K])
+        res = Ref(C).msm(S, P)
+        manifest["large"].append({"curve": C.name, "kind": "msm_std_affine", "logn": logn, "kseed": kseed,
+                                  "sseed": sseed, "result": [int(x) for x in res], "points_sha256": sha(P)})
+        print(C.name, "msm", logn, flush=True)
+    np.savez_compressed(HERE / "golden_small.npz", **out)
+    (HERE / "manifest.json").write_text(json.dumps(manifest, indent=1))
+
+
+if __name__ == "__main__":
+    main()

@@ -206,3 +206,25 @@ def horner(coeffs, x, r):
     for c in reversed(coeffs):
         acc = (acc * x + c) % r
     return acc
+
+
+def seeded_fr_vector(seed, n, r):
+    """n uniform elements of [0, r) as an (n, 4) uint64 array (numpy PCG64 stream, rejection
+    on the top word); deterministic for a seed, so large inputs need not be stored."""
+    rng = np.random.default_rng(seed)
+    bits = r.bit_length()
+    top_mask = np.uint64((1 << (bits - 192)) - 1)
+    r_top = np.uint64(r >> 192)
+    out = np.empty((n, 4), np.uint64)
+    filled = 0
+    while filled < n:
+        k = int((n - filled) * 1.4) + 16
+        cand = rng.integers(0, 2**64, size=(k, 4), dtype=np.uint64)
+        cand[:, 3] &= top_mask
+        ok = cand[:, 3] < r_top
+        for i in np.nonzero(cand[:, 3] == r_top)[0]:
+            ok[i] = from_limbs(cand[i]) < r
+        acc = cand[ok][: n - filled]
+        out[filled:filled + len(acc)] = acc
+        filled += len(acc)
+    return out

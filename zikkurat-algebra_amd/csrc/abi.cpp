@@ -292,4 +292,6 @@ int zkg_last_timing(double *ms_total, double *ms_phases, int max_phases, int *np
   return msm_last_timing(ms_total, ms_phases, max_phases, nphases);
 }
 
+int zkg_msm_last_stats(long long *out4) { return msm_last_stats(out4); }
+
 }  // extern "C"

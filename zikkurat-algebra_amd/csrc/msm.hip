@@ -424,6 +424,12 @@ int msm_write_result(int curve, const MsmResult &r, uint64_t *tgt, int flags) {
 // per-phase timing of the last MSM (HIP events on the library stream + host wall clock)
 static double g_ms_total = 0, g_ms_phase[8] = {0};
 static int g_nphase = 0;
+static long long g_stats[4] = {0, 0, 0, 0};  // window, windows, sorted entries, heavy buckets
+
+int msm_last_stats(long long *out4) {
+  for (int i = 0; i < 4; i++) out4[i] = g_stats[i];
+  return ZK_OK;
+}
 
 int msm_last_timing(double *ms_total, double *ms_phases, int max_phases, int *nphases) {
   if (ms_total) *ms_total = g_ms_total;
@@ -529,9 +535,14 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
                      off.as<uint32_t>(), (uint32_t)nkeys, seg_log, bsum.as<uint32_t>(), cont.as<uint32_t>(),
                      heavy.as<uint32_t>(), heavy_cap, limit);
   ZK_HIP(hipGetLastError());
-  uint32_t hh[2];
+  uint32_t hh[2], nent = 0;
   ZK_HIP(hipMemcpyAsync(hh, heavy.p, 8, hipMemcpyDeviceToHost, st));
+  ZK_HIP(hipMemcpyAsync(&nent, off.as<uint32_t>() + nkeys, 4, hipMemcpyDeviceToHost, st));
   ZK_HIP(hipStreamSynchronize(st));
+  g_stats[0] = c;
+  g_stats[1] = W;
+  g_stats[2] = nent;
+  g_stats[3] = hh[0];
   if (hh[0] > heavy_cap) {
     set_error("msm: too many heavy buckets");
     return ZK_EINVAL;

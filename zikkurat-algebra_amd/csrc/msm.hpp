@@ -34,5 +34,6 @@ int msm_device(int curve, int n, const uint64_t *d_scalars, int enl, int mont, c
 int g1_scalar_mul_gen(int curve, int n, const uint64_t *d_scalars, uint64_t *d_out, hipStream_t st);
 
 int msm_last_timing(double *ms_total, double *ms_phases, int max_phases, int *nphases);
+int msm_last_stats(long long *out4);
 
 }  // namespace zk

@@ -269,6 +269,12 @@ def last_timing():
     return tot.value, list(ph)[:n.value]
 
 
+def last_msm_stats():
+    out = (ctypes.c_longlong * 4)()
+    _check(lib().zkg_msm_last_stats(out))
+    return dict(window=out[0], windows=out[1], entries=out[2], heavy=out[3])
+
+
 def device_info():
     cus = ctypes.c_int()
     hbm = ctypes.c_size_t()
