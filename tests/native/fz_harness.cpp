@@ -1,0 +1,96 @@
+// Host build of the device field/curve code in zikkurat-algebra_amd/csrc/fz.hpp and
+// curvez.hpp (the same __host__ __device__ functions the GPU kernels inline), exposed to the
+// CPU tests through a C ABI.  Test infrastructure only.
+#include <string.h>
+
+#include "../../zikkurat-algebra_amd/csrc/curvez.hpp"
+
+using namespace zk;
+
+template <class P>
+static void run_op(int op, int n, const int32_t *a, const int32_t *b, int32_t *out) {
+  for (int k = 0; k < n; k++) {
+    Fz<P> x, y, r;
+    memcpy(x.l, a + k * P::N, sizeof x.l);
+    if (b) memcpy(y.l, b + k * P::N, sizeof y.l);
+    switch (op) {
+      case 0: zmul(r, x, y); break;
+      case 1: zadd(r, x, y); break;
+      case 2: zsub(r, x, y); break;
+      case 3: r = x; znorm(r); break;
+      case 4: zcanon(r, x); break;
+      case 5: zset_zero(r); r.l[0] = zis_zero_mod(x) ? 1 : 0; break;
+      default: r = x; break;
+    }
+    memcpy(out + k * P::N, r.l, sizeof r.l);
+  }
+}
+
+template <class P>
+static void run_conv(int dir, int n, const uint32_t *win, int32_t *lout, const int32_t *lin, uint32_t *wout) {
+  for (int k = 0; k < n; k++) {
+    Fz<P> r;
+    if (dir == 0) {  // external words -> internal limbs
+      zfrom_ext(r, win + k * P::EXT_WORDS);
+      memcpy(lout + k * P::N, r.l, sizeof r.l);
+    } else if (dir == 1) {  // internal limbs -> external words
+      memcpy(r.l, lin + k * P::N, sizeof r.l);
+      zto_ext(wout + k * P::EXT_WORDS, r);
+    } else if (dir == 2) {  // unpack only
+      zunpack(r, win + k * P::EXT_WORDS);
+      memcpy(lout + k * P::N, r.l, sizeof r.l);
+    } else {  // pack only
+      memcpy(r.l, lin + k * P::N, sizeof r.l);
+      zpack(wout + k * P::EXT_WORDS, r);
+    }
+  }
+}
+
+template <class C>
+static void run_curve(int op, int n, const int32_t *a, const int32_t *b, int32_t *out) {
+  constexpr int N = ZP<C>::T::N;
+  for (int k = 0; k < n; k++) {
+    XyzzZ<C> p, q, r;
+    memcpy(&p, a + k * 4 * N, sizeof p);
+    if (op == 0) {  // madd: b holds affine points
+      AffZ<C> aff;
+      memcpy(&aff, b + k * 2 * N, sizeof aff);
+      r = p;
+      zxyzz_madd(r, aff);
+    } else if (op == 1) {
+      memcpy(&q, b + k * 4 * N, sizeof q);
+      zxyzz_add(r, p, q);
+    } else {
+      zxyzz_dbl(r, p);
+    }
+    memcpy(out + k * 4 * N, &r, sizeof r);
+  }
+}
+
+extern "C" {
+// field: 0 BN254 Fp, 1 BN254 Fr, 2 BLS Fp, 3 BLS Fr
+int fzh_limbs(int field) {
+  const int n[4] = {ZBn254Fp::N, ZBn254Fr::N, ZBls381Fp::N, ZBls381Fr::N};
+  return n[field];
+}
+void fzh_op(int field, int op, int n, const int32_t *a, const int32_t *b, int32_t *out) {
+  switch (field) {
+    case 0: run_op<ZBn254Fp>(op, n, a, b, out); break;
+    case 1: run_op<ZBn254Fr>(op, n, a, b, out); break;
+    case 2: run_op<ZBls381Fp>(op, n, a, b, out); break;
+    default: run_op<ZBls381Fr>(op, n, a, b, out); break;
+  }
+}
+void fzh_conv(int field, int dir, int n, const uint32_t *win, int32_t *lout, const int32_t *lin, uint32_t *wout) {
+  switch (field) {
+    case 0: run_conv<ZBn254Fp>(dir, n, win, lout, lin, wout); break;
+    case 1: run_conv<ZBn254Fr>(dir, n, win, lout, lin, wout); break;
+    case 2: run_conv<ZBls381Fp>(dir, n, win, lout, lin, wout); break;
+    default: run_conv<ZBls381Fr>(dir, n, win, lout, lin, wout); break;
+  }
+}
+void fzh_curve(int curve, int op, int n, const int32_t *a, const int32_t *b, int32_t *out) {
+  if (curve == 0) run_curve<Bn254G1>(op, n, a, b, out);
+  else run_curve<Bls381G1>(op, n, a, b, out);
+}
+}

@@ -1,0 +1,197 @@
+"""CPU test of the reduced-radix field and XYZZ curve code that the GPU kernels inline
+(zikkurat-algebra_amd/csrc/fz.hpp, curvez.hpp), compiled for the host through
+tests/native/fz_harness.cpp and checked against Python integers.
+
+The inputs deliberately sit at the documented bounds (|value| < 8p, unnormalised limb-wise
+sums and differences, signed top limbs, multiples of p for the zero test) so that an
+accumulator overflow or a wrong normalisation shows up here rather than as a rare wrong MSM.
+"""
+import ctypes
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from zkmath import BN, BLS, splitmix64, random_below
+
+REPO = Path(__file__).resolve().parent.parent
+SRC = REPO / "tests" / "native" / "fz_harness.cpp"
+OUT = REPO / "tests" / "native" / "_build" / "libfzh.so"
+
+# field id -> (prime, BITS, N, external 32-bit words)
+FIELDS = {0: (BN.p, 29, 9, 8), 1: (BN.r, 29, 9, 8), 2: (BLS.p, 28, 14, 12), 3: (BLS.r, 29, 9, 8)}
+i32p = ctypes.POINTER(ctypes.c_int32)
+u32p = ctypes.POINTER(ctypes.c_uint32)
+
+
+@pytest.fixture(scope="module")
+def h():
+    OUT.parent.mkdir(exist_ok=True)
+    if not OUT.exists() or OUT.stat().st_mtime < max(SRC.stat().st_mtime,
+                                                     *(p.stat().st_mtime for p in (REPO / "zikkurat-algebra_amd" / "csrc").glob("*.hpp"))):
+        subprocess.run(["/opt/rocm/bin/hipcc", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", str(OUT), str(SRC)],
+                       check=True, capture_output=True)
+    return ctypes.CDLL(str(OUT))
+
+
+def to_limbs(v, bits, n):
+    """Normalised signed-radix limbs of an integer v (limbs 0..n-2 in [0, 2^bits), signed top)."""
+    out = []
+    for _ in range(n - 1):
+        out.append(v & ((1 << bits) - 1))
+        v >>= bits
+    out.append(v)
+    assert -(1 << 31) <= out[-1] < (1 << 31)
+    return out
+
+
+def value(limbs, bits):
+    return sum(int(l) << (bits * i) for i, l in enumerate(limbs))
+
+
+def arr(rows):
+    return np.ascontiguousarray(np.array(rows, dtype=np.int64).astype(np.int32))
+
+
+def op(h, field, code, a, b=None):
+    n = a.shape[0]
+    N = FIELDS[field][2]
+    out = np.zeros((n, N), np.int32)
+    h.fzh_op(field, code, n, a.ctypes.data_as(i32p), b.ctypes.data_as(i32p) if b is not None else None,
+             out.ctypes.data_as(i32p))
+    return out
+
+
+def is_normalised(row, bits):
+    return all(0 <= int(x) < (1 << bits) for x in row[:-1])
+
+
+def sample_values(p, rng, k, bound):
+    """Random integers in (-bound*p, bound*p), plus the extremes."""
+    vals = [0, 1, -1, p - 1, -(p - 1), bound * p - 1, -(bound * p - 1), p, -p]
+    while len(vals) < k:
+        mag = random_below(rng, bound * p, 8)
+        vals.append(mag if next(rng) & 1 else -mag)
+    return vals
+
+
+@pytest.mark.parametrize("field", [0, 1, 2, 3], ids=["bn-fp", "bn-fr", "bls-fp", "bls-fr"])
+def test_zmul_matches_integers_at_bounds(h, field):
+    p, bits, N, _ = FIELDS[field]
+    Rp = 1 << (bits * N)
+    Rinv = pow(Rp, -1, p)
+    rng = splitmix64(100 + field)
+    bound = 8 if field != 3 else 4  # BLS12-381 Fr: R'/r ~ 70, products of |a|,|b| < 4r
+    xs = sample_values(p, rng, 400, bound)
+    ys = sample_values(p, rng, 400, bound)[::-1]
+    A = arr([to_limbs(x, bits, N) for x in xs])
+    B = arr([to_limbs(y, bits, N) for y in ys])
+    R = op(h, field, 0, A, B)
+    for i, (x, y) in enumerate(zip(xs, ys)):
+        got = value(R[i], bits)
+        assert got % p == x * y * Rinv % p, i
+        assert is_normalised(R[i], bits)
+        assert -p < got < 2 * p, (i, got / p)
+    # one operand a limb-wise difference of two normalised values (as Q - X3 in the madd)
+    D = arr(A.astype(np.int64) - arr([to_limbs(z, bits, N) for z in sample_values(p, rng, 400, 2)]).astype(np.int64))
+    R = op(h, field, 0, D, B)
+    for i in range(len(xs)):
+        d = value(D[i], bits)
+        assert value(R[i], bits) % p == d * ys[i] * Rinv % p, i
+
+
+@pytest.mark.parametrize("field", [0, 1, 2, 3], ids=["bn-fp", "bn-fr", "bls-fp", "bls-fr"])
+def test_zero_test_norm_and_canon(h, field):
+    p, bits, N, _ = FIELDS[field]
+    rng = splitmix64(7 + field)
+    mults = [j * p for j in range(-7, 8)]
+    near = [j * p + d for j in range(-7, 8) for d in (1, -1, 1 << bits, -(1 << bits))]
+    rand = sample_values(p, rng, 200, 7)
+    vals = mults + near + rand
+    A = arr([to_limbs(v, bits, N) for v in vals])
+    Z = op(h, field, 5, A)
+    for i, v in enumerate(vals):
+        assert bool(Z[i, 0]) == (v % p == 0), v
+    # normalisation of limb-wise sums keeps the value
+    S = arr(A[:100].astype(np.int64) + A[100:200].astype(np.int64))
+    Nn = op(h, field, 3, S)
+    for i in range(100):
+        assert value(Nn[i], bits) == vals[i] + vals[100 + i]
+        assert is_normalised(Nn[i], bits)
+    # canonical representative of values in (-p, 2p)
+    cvals = [0, 1, p - 1, p, p + 1, 2 * p - 1, -1, -(p - 1)] + [random_below(rng, 3 * p, 8) - p for _ in range(100)]
+    Cc = op(h, field, 4, arr([to_limbs(v, bits, N) for v in cvals]))
+    for i, v in enumerate(cvals):
+        assert value(Cc[i], bits) == v % p
+
+
+@pytest.mark.parametrize("field", [0, 1, 2, 3], ids=["bn-fp", "bn-fr", "bls-fp", "bls-fr"])
+def test_external_conversion_roundtrip(h, field):
+    p, bits, N, W = FIELDS[field]
+    Rp, R = 1 << (bits * N), 1 << (32 * W)
+    rng = splitmix64(55 + field)
+    xs = [0, 1, p - 1] + [random_below(rng, p, 8) for _ in range(300)]
+    words = np.array([[(x >> (32 * k)) & 0xFFFFFFFF for k in range(W)] for x in xs], dtype=np.uint32)
+    lim = np.zeros((len(xs), N), np.int32)
+    h.fzh_conv(field, 0, len(xs), words.ctypes.data_as(u32p), lim.ctypes.data_as(i32p), None, None)
+    for i, x in enumerate(xs):   # x is an external Montgomery word string x = a R: internal a R'
+        assert value(lim[i], bits) % p == x * pow(R, -1, p) * Rp % p
+    back = np.zeros_like(words)
+    h.fzh_conv(field, 1, len(xs), None, None, lim.ctypes.data_as(i32p), back.ctypes.data_as(u32p))
+    assert (back == words).all()
+
+
+def xyzz_rows(C, pts, rng, lazy):
+    """Internal XYZZ limbs (x z^2, y z^3, z^2, z^3) * R' of affine points, random z; lazy adds
+    multiples of p inside the stored-coordinate bound (|v| < 6p)."""
+    p, bits, N = C.p, (29 if C.id == 0 else 28), (9 if C.id == 0 else 14)
+    Rp = 1 << (bits * N)
+    rows = []
+    for (x, y) in pts:
+        z = random_below(rng, p, 8) or 1
+        coords = [x * z * z, y * z * z * z, z * z, z * z * z]
+        limbs = []
+        for c in coords:
+            v = c * Rp % p
+            if lazy:
+                v += ((next(rng) % 9) - 4) * p
+            limbs += to_limbs(v, bits, N)
+        rows.append(limbs)
+    return arr(rows)
+
+
+def decode_xyzz(C, row):
+    p, bits, N = C.p, (29 if C.id == 0 else 28), (9 if C.id == 0 else 14)
+    Rinv = pow(1 << (bits * N), -1, p)
+    X, Y, ZZ, ZZZ = (value(row[k * N:(k + 1) * N], bits) * Rinv % p for k in range(4))
+    if all(int(v) == 0 for v in row[2 * N:3 * N]):
+        return None
+    return (X * pow(ZZ, -1, p) % p, Y * pow(ZZZ, -1, p) % p)
+
+
+@pytest.mark.parametrize("C", [BN, BLS], ids=["bn254", "bls12-381"])
+def test_xyzz_formulas(h, C):
+    rng = splitmix64(900 + C.id)
+    bits, N = (29, 9) if C.id == 0 else (28, 14)
+    Rp = 1 << (bits * N)
+    ks = [random_below(rng, C.r, 4) for _ in range(24)]
+    P = [C.mul(k, C.g) for k in ks]
+    Q = [C.mul(k + 7, C.g) for k in ks]
+    # exceptional pairs: equal points (doubling) and opposite points (infinity)
+    Q[0], Q[1] = P[0], C.neg(P[1])
+    for lazy in (False, True):
+        A = xyzz_rows(C, P, rng, lazy)
+        B = xyzz_rows(C, Q, rng, lazy)
+        aff = arr([to_limbs(q[0] * Rp % C.p, bits, N) + to_limbs(q[1] * Rp % C.p, bits, N) for q in Q])
+        n = len(P)
+        out = np.zeros_like(A)
+        h.fzh_curve(C.id, 0, n, A.ctypes.data_as(i32p), aff.ctypes.data_as(i32p), out.ctypes.data_as(i32p))
+        for i in range(n):
+            assert decode_xyzz(C, out[i]) == C.add(P[i], Q[i]), ("madd", lazy, i)
+        h.fzh_curve(C.id, 1, n, A.ctypes.data_as(i32p), B.ctypes.data_as(i32p), out.ctypes.data_as(i32p))
+        for i in range(n):
+            assert decode_xyzz(C, out[i]) == C.add(P[i], Q[i]), ("add", lazy, i)
+        h.fzh_curve(C.id, 2, n, A.ctypes.data_as(i32p), None, out.ctypes.data_as(i32p))
+        for i in range(n):
+            assert decode_xyzz(C, out[i]) == C.add(P[i], P[i]), ("dbl", lazy, i)

@@ -1,0 +1,252 @@
+// G1 bucket arithmetic on the reduced-radix field (fz.hpp): XYZZ accumulators, mixed
+// XYZZ + affine additions (EFD madd-2008-s, 8M + 2S), full XYZZ additions (add-2008-s) and
+// doublings (dbl/mdbl-2008-s-1), with every exceptional case exact (infinite accumulator,
+// P + P, P + (-P)).
+//
+// Bounds (p = field prime, all values kept normalised, see fz.hpp): products of operands
+// below 8p in magnitude are below 1.5p; the formulas below combine at most four products
+// before normalising, so every stored coordinate stays below 6p and every multiplication
+// input below 8p.  Infinity is the explicit all-zero ZZ (never produced by a product).
+#pragma once
+#include "curve.hpp"
+#include "fz.hpp"
+
+namespace zk {
+
+template <class C>
+struct ZP;  // reduced-radix pack of the curve's base field
+template <>
+struct ZP<Bn254G1> {
+  using T = ZBn254Fp;
+};
+template <>
+struct ZP<Bls381G1> {
+  using T = ZBls381Fp;
+};
+
+template <class C>
+using FzP = Fz<typename ZP<C>::T>;
+
+template <class C>
+struct AffZ {
+  FzP<C> x, y;
+};
+template <class C>
+struct XyzzZ {
+  FzP<C> x, y, zz, zzz;
+};
+
+// words per stored element in device memory
+template <class C>
+struct ZLayout {
+  static constexpr int N = ZP<C>::T::N;
+  static constexpr int AFF = (2 * N + 3) / 4 * 4;  // affine point, padded to 16 bytes
+  static constexpr int XYZZ = 4 * N;               // 36 or 56 words: multiples of 4
+};
+
+template <class C>
+ZK_HD void zxyzz_set_inf(XyzzZ<C> &p) {
+  zset_zero(p.x);
+  zset_zero(p.y);
+  zset_zero(p.zz);
+  zset_zero(p.zzz);
+}
+template <class C>
+ZK_HD bool zxyzz_is_inf(const XyzzZ<C> &p) {
+  return zis_raw_zero(p.zz);
+}
+template <class C>
+ZK_HD bool zaff_is_inf(const AffZ<C> &a) {
+  return zis_raw_zero(a.x) && zis_raw_zero(a.y);
+}
+
+// r = 2 (x, y) for an affine point
+template <class C>
+ZK_HD void zxyzz_mdbl(XyzzZ<C> &r, const AffZ<C> &a) {
+  using F = FzP<C>;
+  if (zis_zero_mod(a.y)) {
+    zxyzz_set_inf(r);
+    return;
+  }
+  F u, v, w, s, m, t;
+  zadd(u, a.y, a.y);
+  znorm(u);
+  zsqr(v, u);
+  zmul(w, u, v);
+  zmul(s, a.x, v);
+  zsqr(t, a.x);
+  zadd(m, t, t);
+  zadd(m, m, t);
+  znorm(m);
+  zsqr(t, m);
+  zsub(t, t, s);
+  zsub(t, t, s);
+  znorm(t);  // X3 = M^2 - 2S
+  r.x = t;
+  zsub(t, s, r.x);
+  zmul(t, m, t);
+  zmul(s, w, a.y);
+  zsub(r.y, t, s);
+  znorm(r.y);
+  r.zz = v;
+  r.zzz = w;
+}
+
+// r = 2 p (XYZZ)
+template <class C>
+ZK_HD void zxyzz_dbl(XyzzZ<C> &r, const XyzzZ<C> &p) {
+  using F = FzP<C>;
+  if (zxyzz_is_inf(p) || zis_zero_mod(p.y)) {
+    zxyzz_set_inf(r);
+    return;
+  }
+  F u, v, w, s, m, t, y1 = p.y, zz1 = p.zz, zzz1 = p.zzz;
+  zadd(u, p.y, p.y);
+  znorm(u);
+  zsqr(v, u);
+  zmul(w, u, v);
+  zmul(s, p.x, v);
+  zsqr(t, p.x);
+  zadd(m, t, t);
+  zadd(m, m, t);
+  znorm(m);
+  zsqr(t, m);
+  zsub(t, t, s);
+  zsub(t, t, s);
+  znorm(t);
+  r.x = t;
+  zsub(t, s, r.x);
+  zmul(t, m, t);
+  zmul(s, w, y1);
+  zsub(r.y, t, s);
+  znorm(r.y);
+  zmul(r.zz, v, zz1);
+  zmul(r.zzz, w, zzz1);
+}
+
+// acc += a (a affine, not infinity)
+template <class C>
+ZK_HD void zxyzz_madd(XyzzZ<C> &acc, const AffZ<C> &a) {
+  using F = FzP<C>;
+  if (zxyzz_is_inf(acc)) {
+    acc.x = a.x;
+    acc.y = a.y;
+    zset_one(acc.zz);
+    zset_one(acc.zzz);
+    return;
+  }
+  F P, R, pp, ppp, q, t;
+  zmul(P, a.x, acc.zz);
+  zsub(P, P, acc.x);
+  znorm(P);  // P = U2 - X1
+  zmul(R, a.y, acc.zzz);
+  zsub(R, R, acc.y);
+  znorm(R);  // R = S2 - Y1
+  if (zis_zero_mod(P)) {
+    if (zis_zero_mod(R)) zxyzz_mdbl(acc, a);
+    else zxyzz_set_inf(acc);
+    return;
+  }
+  zsqr(pp, P);
+  zmul(ppp, P, pp);
+  zmul(q, acc.x, pp);
+  zmul(acc.zz, acc.zz, pp);
+  zmul(acc.zzz, acc.zzz, ppp);
+  zsqr(t, R);
+  zsub(t, t, ppp);
+  zsub(t, t, q);
+  zsub(t, t, q);
+  znorm(t);
+  acc.x = t;              // X3 = R^2 - PPP - 2Q
+  zsub(t, q, acc.x);      // Q - X3 (one unnormalised operand)
+  zmul(t, R, t);
+  zmul(q, acc.y, ppp);
+  zsub(acc.y, t, q);
+  znorm(acc.y);           // Y3 = R (Q - X3) - Y1 PPP
+}
+
+// r = p + q (XYZZ, complete)
+template <class C>
+ZK_HD void zxyzz_add(XyzzZ<C> &r, const XyzzZ<C> &p, const XyzzZ<C> &q) {
+  using F = FzP<C>;
+  if (zxyzz_is_inf(p)) {
+    r = q;
+    return;
+  }
+  if (zxyzz_is_inf(q)) {
+    r = p;
+    return;
+  }
+  F u1, s1, P, R, pp, ppp, t, zz, zzz;
+  zmul(u1, p.x, q.zz);
+  zmul(P, q.x, p.zz);
+  zsub(P, P, u1);
+  znorm(P);
+  zmul(s1, p.y, q.zzz);
+  zmul(R, q.y, p.zzz);
+  zsub(R, R, s1);
+  znorm(R);
+  if (zis_zero_mod(P)) {
+    if (zis_zero_mod(R)) {
+      XyzzZ<C> pc = p;
+      zxyzz_dbl(r, pc);
+    } else {
+      zxyzz_set_inf(r);
+    }
+    return;
+  }
+  zsqr(pp, P);
+  zmul(ppp, P, pp);
+  zmul(zz, p.zz, q.zz);
+  zmul(zz, zz, pp);
+  zmul(zzz, p.zzz, q.zzz);
+  zmul(zzz, zzz, ppp);
+  zmul(u1, u1, pp);  // Q
+  zsqr(t, R);
+  zsub(t, t, ppp);
+  zsub(t, t, u1);
+  zsub(t, t, u1);
+  znorm(t);
+  F x3 = t;
+  zsub(t, u1, x3);
+  zmul(t, R, t);
+  zmul(s1, s1, ppp);
+  zsub(r.y, t, s1);
+  znorm(r.y);
+  r.x = x3;
+  r.zz = zz;
+  r.zzz = zzz;
+}
+
+// ---- device memory
+
+template <class C>
+__device__ __forceinline__ void zaff_load(AffZ<C> &a, const int32_t *__restrict__ p) {
+  constexpr int N = ZP<C>::T::N;
+  zload(a.x, p);
+  zload(a.y, p + N);
+}
+template <class C>
+__device__ __forceinline__ void zaff_store(int32_t *__restrict__ p, const AffZ<C> &a) {
+  constexpr int N = ZP<C>::T::N;
+  zstore(p, a.x);
+  zstore(p + N, a.y);
+}
+template <class C>
+__device__ __forceinline__ void zxyzz_load(XyzzZ<C> &a, const int32_t *__restrict__ p) {
+  constexpr int N = ZP<C>::T::N;
+  zload(a.x, p);
+  zload(a.y, p + N);
+  zload(a.zz, p + 2 * N);
+  zload(a.zzz, p + 3 * N);
+}
+template <class C>
+__device__ __forceinline__ void zxyzz_store(int32_t *__restrict__ p, const XyzzZ<C> &a) {
+  constexpr int N = ZP<C>::T::N;
+  zstore(p, a.x);
+  zstore(p + N, a.y);
+  zstore(p + 2 * N, a.zz);
+  zstore(p + 3 * N, a.zzz);
+}
+
+}  // namespace zk

@@ -1,0 +1,288 @@
+// Reduced-radix Montgomery arithmetic for the hot kernels (MSM buckets, NTT butterflies).
+//
+// Why: on gfx950 v_mad_u64_u32 / v_mad_i64_i32, v_mul_lo_u32 and 64-bit adds all issue at
+// full VALU rate, so a field multiplication costs its instruction count, not its multiply
+// count.  With 32-bit limbs every partial product needs carry-flag adds and zero-extension
+// moves (~420 instructions per 256-bit multiplication).  Here an element is N signed 32-bit
+// limbs in radix 2^BITS (BITS = 29 for the 254/255-bit fields, N = 9; BITS = 28 for the
+// 381-bit BLS12-381 base field, N = 14).  Partial products then leave room in a 64-bit
+// accumulator, so a Montgomery product is a product-scanning loop of plain multiply-adds
+// (v_mad_i64_i32) with one shift per column and no carry chain (~250 instructions), and
+// additions/subtractions are limb-wise.
+//
+// Representation.  value(a) = sum_i a.l[i] 2^(BITS i).  "Normalised" means a.l[i] in
+// [0, 2^BITS) for i < N-1 and a small signed top limb; every zmul output is normalised and
+// values stay in (-8p, 8p).  The Montgomery radix is R' = 2^(BITS N) (2^261, 2^392), chosen so
+// that R' > 64 p: for |a|, |b| < 8p the product satisfies |zmul(a,b)| < 64 p^2 / R' + p < 2p.
+// Column sums stay below 2^63 when both inputs are normalised, or one is normalised and the
+// other is the limb-wise sum/difference of two normalised values.
+//
+// Relation to the reference's Montgomery form (R = 2^256 or 2^384): an external element
+// x R is converted with one product by K_IN = R'^2/R mod p (giving x R') and back with
+// K_OUT = R mod p.  Products of external data with internal constants (NTT twiddles w R')
+// stay external: zmul(x R, w R') = x w R.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define ZK_HD __host__ __device__ __forceinline__
+
+namespace zk {
+
+// parameter packs: limbs of p, -p^-1 mod 2^BITS, conversion constants, R' mod p, and the
+// low limbs (j p) mod 2^BITS for j = -8..8 used by the zero test (values generated from the
+// primes of lib/cbits/curves/fields/mont/*_mont.c)
+struct ZBn254Fp {
+  static constexpr int BITS = 29;
+  static constexpr int N = 9;
+  static constexpr int EXT_WORDS = 8;
+  static constexpr uint32_t NP = 75916169u;
+  static constexpr int32_t P[N] = {410844487, 17064118, 477274959, 47522512, 361093496, 47923392, 10936641, 240920116, 3171406};
+  static constexpr int32_t K_IN[N] = {322215073, 442336424, 171859116, 268585440, 314135016, 244503300, 348886451, 68918589, 360451};
+  static constexpr int32_t K_OUT[N] = {93261213, 451550318, 297979764, 299258347, 342016167, 297253948, 482187706, 406012155, 920183};
+  static constexpr int32_t ONE[N] = {360500257, 337389400, 408039635, 21759001, 178483129, 490881230, 299191303, 86689704, 903222};
+  static constexpr uint32_t ZLOW[17] = {471340488, 345314063, 219287638, 93261213, 504105700, 378079275, 252052850, 126026425, 0, 410844487, 284818062, 158791637, 32765212, 443609699, 317583274, 191556849, 65530424};
+};
+struct ZBn254Fr {
+  static constexpr int BITS = 29;
+  static constexpr int N = 9;
+  static constexpr int EXT_WORDS = 8;
+  static constexpr uint32_t NP = 268435455u;
+  static constexpr int32_t P[N] = {268435457, 521120927, 240919632, 131109107, 361091715, 47923392, 10936641, 240920116, 3171406};
+  static constexpr int32_t K_IN[N] = {268430039, 492061940, 71535269, 62181526, 323781850, 244503300, 348886451, 68918589, 360451};
+  static constexpr int32_t K_OUT[N] = {268435451, 78749922, 406014571, 418196286, 342025071, 297253948, 482187706, 406012155, 920183};
+  static constexpr int32_t ONE[N] = {268435287, 514263732, 86771339, 391139145, 178784091, 490881230, 299191303, 86689704, 903222};
+  static constexpr uint32_t ZLOW[17] = {536870904, 268435449, 536870906, 268435451, 536870908, 268435453, 536870910, 268435455, 0, 268435457, 2, 268435459, 4, 268435461, 6, 268435463, 8};
+};
+struct ZBls381Fr {
+  static constexpr int BITS = 29;
+  static constexpr int N = 9;
+  static constexpr int EXT_WORDS = 8;
+  static constexpr uint32_t NP = 536870911u;
+  static constexpr int32_t P[N] = {1, 536870904, 529989567, 457704959, 494949691, 201589968, 354471143, 174404527, 7597479};
+  static constexpr int32_t K_IN[N] = {536868651, 18087, 526333248, 216203804, 294598241, 8484555, 89016444, 271483280, 1968430};
+  static constexpr int32_t K_OUT[N] = {536870910, 15, 13762688, 158331904, 83842440, 133690974, 364799537, 188061856, 1582257};
+  static constexpr int32_t ONE[N] = {536870842, 559, 481694080, 172907520, 250130850, 384216799, 419952827, 139714039, 5047359};
+  static constexpr uint32_t ZLOW[17] = {536870904, 536870905, 536870906, 536870907, 536870908, 536870909, 536870910, 536870911, 0, 1, 2, 3, 4, 5, 6, 7, 8};
+};
+struct ZBls381Fp {
+  static constexpr int BITS = 28;
+  static constexpr int N = 14;
+  static constexpr int EXT_WORDS = 12;
+  static constexpr uint32_t NP = 268238845u;
+  static constexpr int32_t P[N] = {268413611, 267386879, 67108793, 268430101, 103030443, 168782607, 258420946, 255349035, 74926980, 79351158, 195540547, 241804465, 32127359, 106513};
+  static constexpr int32_t K_IN[N] = {135160473, 55574580, 246491222, 233514649, 211322480, 82798231, 118064104, 172912617, 65577043, 248456990, 203477795, 116089733, 75436678, 87770};
+  static constexpr int32_t K_OUT[N] = {196605, 9437184, 201327222, 48192, 146467828, 91569269, 90130584, 117777780, 130963539, 91145943, 119183266, 239678912, 247724672, 89957};
+  static constexpr int32_t ONE[N] = {55049400, 226492416, 176409, 13493970, 208802528, 138027152, 3630751, 228652862, 162569026, 19495856, 85318031, 1231484, 106560762, 22394};
+  static constexpr uint32_t ZLOW[17] = {174760, 152915, 131070, 109225, 87380, 65535, 43690, 21845, 0, 268413611, 268391766, 268369921, 268348076, 268326231, 268304386, 268282541, 268260696};
+};
+
+template <class P>
+struct Fz {
+  int32_t l[P::N];
+};
+
+template <class P>
+ZK_HD void zset_zero(Fz<P> &r) {
+#pragma unroll
+  for (int i = 0; i < P::N; i++) r.l[i] = 0;
+}
+template <class P>
+ZK_HD void zset_one(Fz<P> &r) {
+#pragma unroll
+  for (int i = 0; i < P::N; i++) r.l[i] = P::ONE[i];
+}
+template <class P>
+ZK_HD void zset_const(Fz<P> &r, const int32_t (&c)[P::N]) {
+#pragma unroll
+  for (int i = 0; i < P::N; i++) r.l[i] = c[i];
+}
+// all limbs zero (the explicit infinity marker of XYZZ points; not a test modulo p)
+template <class P>
+ZK_HD bool zis_raw_zero(const Fz<P> &a) {
+  int32_t acc = 0;
+#pragma unroll
+  for (int i = 0; i < P::N; i++) acc |= a.l[i];
+  return acc == 0;
+}
+
+template <class P>
+ZK_HD void zadd(Fz<P> &r, const Fz<P> &a, const Fz<P> &b) {
+#pragma unroll
+  for (int i = 0; i < P::N; i++) r.l[i] = a.l[i] + b.l[i];
+}
+template <class P>
+ZK_HD void zsub(Fz<P> &r, const Fz<P> &a, const Fz<P> &b) {
+#pragma unroll
+  for (int i = 0; i < P::N; i++) r.l[i] = a.l[i] - b.l[i];
+}
+template <class P>
+ZK_HD void zneg(Fz<P> &r, const Fz<P> &a) {
+#pragma unroll
+  for (int i = 0; i < P::N; i++) r.l[i] = -a.l[i];
+}
+
+// carry propagation: limbs 0..N-2 into [0, 2^BITS), signed top limb (value unchanged)
+template <class P>
+ZK_HD void znorm(Fz<P> &r) {
+  constexpr int32_t MASK = (1 << P::BITS) - 1;
+#pragma unroll
+  for (int i = 0; i < P::N - 1; i++) {
+    const int32_t c = r.l[i] >> P::BITS;  // arithmetic shift: floor division
+    r.l[i] &= MASK;
+    r.l[i + 1] += c;
+  }
+}
+
+// Montgomery product a b / R' (product scanning, two interleaved 64-bit accumulators per
+// column so that consecutive multiply-adds are independent).  Output normalised.
+template <class P>
+ZK_HD void zmul(Fz<P> &r, const Fz<P> &a, const Fz<P> &b) {
+  constexpr int N = P::N, B = P::BITS;
+  constexpr uint32_t MASK = (1u << B) - 1;
+  uint32_t m[N];
+  int64_t carry = 0;
+#pragma unroll
+  for (int c = 0; c < 2 * N - 1; c++) {
+    const int lo = c < N ? 0 : c - N + 1;
+    const int hi = c < N ? c : N - 1;
+    int64_t s0 = carry, s1 = 0;
+#pragma unroll
+    for (int i = lo; i <= hi; i++) {
+      const int64_t pr = (int64_t)a.l[i] * (int64_t)b.l[c - i];
+      if ((i - lo) & 1) s1 += pr;
+      else s0 += pr;
+    }
+#pragma unroll
+    for (int i = lo; i <= hi; i++) {
+      if (i >= c) continue;  // m_c is computed below
+      const int64_t pr = (int64_t)((uint64_t)m[i] * (uint64_t)(uint32_t)P::P[c - i]);
+      if ((i - lo) & 1) s0 += pr;
+      else s1 += pr;
+    }
+    int64_t s = s0 + s1;
+    if (c < N) {
+      const uint32_t mc = ((uint32_t)s * P::NP) & MASK;
+      m[c] = mc;
+      s += (int64_t)((uint64_t)mc * (uint64_t)(uint32_t)P::P[0]);
+    } else {
+      r.l[c - N] = (int32_t)((uint32_t)s & MASK);
+    }
+    carry = s >> B;
+  }
+  r.l[N - 1] = (int32_t)carry;
+}
+
+template <class P>
+ZK_HD void zsqr(Fz<P> &r, const Fz<P> &a) {
+  zmul(r, a, a);
+}
+
+// a == 0 (mod p) for a normalised a with |a| < 8p.  The low limb of j p (j = -8..8) filters;
+// the exact check runs only on a filter hit.
+template <class P>
+ZK_HD bool zis_zero_mod(const Fz<P> &a) {
+  constexpr int N = P::N, B = P::BITS;
+  constexpr int64_t MASK = (1ll << B) - 1;
+  const uint32_t low = (uint32_t)a.l[0];
+  bool cand = false;
+#pragma unroll
+  for (int j = 0; j < 17; j++) cand |= (low == P::ZLOW[j]);
+  if (!cand) return false;
+  for (int j = -8; j <= 8; j++) {
+    if (low != P::ZLOW[j + 8]) continue;
+    int64_t c = 0;
+    bool z = true;
+    for (int i = 0; i < N; i++) {
+      const int64_t d = (int64_t)a.l[i] - (int64_t)j * P::P[i] + c;
+      if (i < N - 1) {
+        if ((d & MASK) != 0) z = false;
+        c = d >> B;
+      } else if (d != 0) {
+        z = false;
+      }
+    }
+    if (z) return true;
+  }
+  return false;
+}
+
+// canonical representative in [0, p) of a normalised a with -p < a < 2p
+template <class P>
+ZK_HD void zcanon(Fz<P> &r, const Fz<P> &a) {
+  constexpr int N = P::N;
+  Fz<P> t = a;
+  if (t.l[N - 1] < 0) {
+#pragma unroll
+    for (int i = 0; i < N; i++) t.l[i] += P::P[i];
+    znorm(t);
+  }
+  Fz<P> d;
+#pragma unroll
+  for (int i = 0; i < N; i++) d.l[i] = t.l[i] - P::P[i];
+  znorm(d);
+  r = d.l[N - 1] >= 0 ? d : t;
+}
+
+// ---- external layout: EXT_WORDS little-endian u32 words (the reference's u64 limbs)
+
+// value-preserving unpack of a canonical external word string into normalised limbs
+template <class P>
+ZK_HD void zunpack(Fz<P> &r, const uint32_t *w) {
+  constexpr int B = P::BITS;
+  constexpr uint32_t MASK = (1u << B) - 1;
+#pragma unroll
+  for (int i = 0; i < P::N; i++) {
+    const int bit = B * i, k = bit >> 5, o = bit & 31;
+    uint64_t v = w[k];
+    if (k + 1 < P::EXT_WORDS) v |= (uint64_t)w[k + 1] << 32;
+    r.l[i] = (int32_t)((uint32_t)(v >> o) & MASK);
+  }
+}
+
+// pack a canonical value (limbs in [0, 2^BITS), value < 2^(32 EXT_WORDS)) into words
+template <class P>
+ZK_HD void zpack(uint32_t *w, const Fz<P> &a) {
+  constexpr int B = P::BITS;
+#pragma unroll
+  for (int k = 0; k < P::EXT_WORDS; k++) {
+    const int bit = 32 * k;
+    const int i = bit / B, o = bit % B;
+    uint64_t v = (uint32_t)a.l[i];
+    if (i + 1 < P::N) v |= (uint64_t)(uint32_t)a.l[i + 1] << B;
+    if (i + 2 < P::N) v |= (uint64_t)(uint32_t)a.l[i + 2] << (2 * B);
+    w[k] = (uint32_t)(v >> o);
+  }
+}
+
+// external Montgomery (x R) -> internal (x R'), and back to a canonical external element
+template <class P>
+ZK_HD void zfrom_ext(Fz<P> &r, const uint32_t *w) {
+  Fz<P> a, k;
+  zunpack(a, w);
+  zset_const(k, P::K_IN);
+  zmul(r, a, k);
+}
+template <class P>
+ZK_HD void zto_ext(uint32_t *w, const Fz<P> &a) {
+  Fz<P> k, t;
+  zset_const(k, P::K_OUT);
+  zmul(t, a, k);
+  zcanon(t, t);
+  zpack(w, t);
+}
+
+// ---- device memory: N words per element
+
+template <class P>
+__device__ __forceinline__ void zload(Fz<P> &r, const int32_t *__restrict__ p) {
+#pragma unroll
+  for (int i = 0; i < P::N; i++) r.l[i] = p[i];
+}
+template <class P>
+__device__ __forceinline__ void zstore(int32_t *__restrict__ p, const Fz<P> &a) {
+#pragma unroll
+  for (int i = 0; i < P::N; i++) p[i] = a.l[i];
+}
+
+}  // namespace zk

@@ -55,43 +55,47 @@ inline Fe load(const FieldDesc &f, const uint64_t *p) {
 }
 inline void store(const FieldDesc &f, uint64_t *p, const Fe &a) { memcpy(p, a.w, 8 * f.nl); }
 
-inline bool geq_p(const FieldDesc &f, const uint64_t *t) {
-  for (int i = f.nl - 1; i >= 0; i--) {
+template <int N>
+inline bool geq_p_n(const FieldDesc &f, const uint64_t *t) {
+  for (int i = N - 1; i >= 0; i--) {
     if (t[i] > f.p[i]) return true;
     if (t[i] < f.p[i]) return false;
   }
   return true;
 }
-inline void sub_p(const FieldDesc &f, uint64_t *t) {
+template <int N>
+inline void sub_p_n(const FieldDesc &f, uint64_t *t) {
   uint64_t br = 0;
-  for (int i = 0; i < f.nl; i++) {
+  for (int i = 0; i < N; i++) {
     u128 d = (u128)t[i] - f.p[i] - br;
     t[i] = (uint64_t)d;
     br = (uint64_t)(d >> 64) & 1;
   }
 }
-inline Fe add(const FieldDesc &f, const Fe &a, const Fe &b) {
+template <int N>
+inline Fe add_n(const FieldDesc &f, const Fe &a, const Fe &b) {
   Fe r = zero();
   u128 c = 0;
-  for (int i = 0; i < f.nl; i++) {
+  for (int i = 0; i < N; i++) {
     c += (u128)a.w[i] + b.w[i];
     r.w[i] = (uint64_t)c;
     c >>= 64;
   }
-  if (c || geq_p(f, r.w)) sub_p(f, r.w);
+  if (c || geq_p_n<N>(f, r.w)) sub_p_n<N>(f, r.w);
   return r;
 }
-inline Fe sub(const FieldDesc &f, const Fe &a, const Fe &b) {
+template <int N>
+inline Fe sub_n(const FieldDesc &f, const Fe &a, const Fe &b) {
   Fe r = zero();
   uint64_t br = 0;
-  for (int i = 0; i < f.nl; i++) {
+  for (int i = 0; i < N; i++) {
     u128 d = (u128)a.w[i] - b.w[i] - br;
     r.w[i] = (uint64_t)d;
     br = (uint64_t)(d >> 64) & 1;
   }
   if (br) {
     u128 c = 0;
-    for (int i = 0; i < f.nl; i++) {
+    for (int i = 0; i < N; i++) {
       c += (u128)r.w[i] + f.p[i];
       r.w[i] = (uint64_t)c;
       c >>= 64;
@@ -99,36 +103,42 @@ inline Fe sub(const FieldDesc &f, const Fe &a, const Fe &b) {
   }
   return r;
 }
-inline Fe neg(const FieldDesc &f, const Fe &a) { return is_zero(f, a) ? a : sub(f, zero(), a); }
-inline Fe mul(const FieldDesc &f, const Fe &a, const Fe &b) {
-  const int n = f.nl;
-  uint64_t t[14] = {0};
-  for (int i = 0; i < n; i++) {
+// CIOS with 64-bit limbs, fully unrolled for N limbs
+template <int N>
+inline Fe mul_n(const FieldDesc &f, const Fe &a, const Fe &b) {
+  uint64_t t[N + 2] = {0};
+  for (int i = 0; i < N; i++) {
     u128 c = 0;
-    for (int j = 0; j < n; j++) {
+    for (int j = 0; j < N; j++) {
       c += (u128)a.w[i] * b.w[j] + t[j];
       t[j] = (uint64_t)c;
       c >>= 64;
     }
-    u128 s = (u128)t[n] + c;
-    t[n] = (uint64_t)s;
-    t[n + 1] = (uint64_t)(s >> 64);
-    uint64_t m = t[0] * f.np;
+    u128 s = (u128)t[N] + c;
+    t[N] = (uint64_t)s;
+    t[N + 1] = (uint64_t)(s >> 64);
+    const uint64_t m = t[0] * f.np;
     c = ((u128)m * f.p[0] + t[0]) >> 64;
-    for (int j = 1; j < n; j++) {
+    for (int j = 1; j < N; j++) {
       c += (u128)m * f.p[j] + t[j];
       t[j - 1] = (uint64_t)c;
       c >>= 64;
     }
-    s = (u128)t[n] + c;
-    t[n - 1] = (uint64_t)s;
-    t[n] = t[n + 1] + (uint64_t)(s >> 64);
+    s = (u128)t[N] + c;
+    t[N - 1] = (uint64_t)s;
+    t[N] = t[N + 1] + (uint64_t)(s >> 64);
   }
   Fe r = zero();
-  memcpy(r.w, t, 8 * n);
-  if (t[n] || geq_p(f, r.w)) sub_p(f, r.w);
+  for (int i = 0; i < N; i++) r.w[i] = t[i];
+  if (t[N] || geq_p_n<N>(f, r.w)) sub_p_n<N>(f, r.w);
   return r;
 }
+
+inline bool geq_p(const FieldDesc &f, const uint64_t *t) { return f.nl == 4 ? geq_p_n<4>(f, t) : geq_p_n<6>(f, t); }
+inline Fe add(const FieldDesc &f, const Fe &a, const Fe &b) { return f.nl == 4 ? add_n<4>(f, a, b) : add_n<6>(f, a, b); }
+inline Fe sub(const FieldDesc &f, const Fe &a, const Fe &b) { return f.nl == 4 ? sub_n<4>(f, a, b) : sub_n<6>(f, a, b); }
+inline Fe neg(const FieldDesc &f, const Fe &a) { return is_zero(f, a) ? a : sub(f, zero(), a); }
+inline Fe mul(const FieldDesc &f, const Fe &a, const Fe &b) { return f.nl == 4 ? mul_n<4>(f, a, b) : mul_n<6>(f, a, b); }
 inline Fe sqr(const FieldDesc &f, const Fe &a) { return mul(f, a, a); }
 inline Fe from_std(const FieldDesc &f, const Fe &a) {
   Fe r2 = zero();

@@ -8,11 +8,12 @@
 // Device pipeline (one MSM, window c, B = 2^(c-1) buckets per window, W windows):
 //   1 digits   signed c-bit digits d in [-2^(c-1), 2^(c-1)] per (window, point); the carry
 //              makes W = ceil((64 enl + 1) / c) windows exact for any scalar
-//   2 count    per-(window, bucket) histogram (global atomics)
+//   2 count    per-(chunk, window) bucket histograms in LDS (packed 16-bit counters)
 //   3 scan     exclusive prefix sum -> bucket offsets into one sorted array of all windows
-//   4 scatter  counting-sort placement of (bucket key, point index | sign) pairs
+//   4 scatter  counting-sort placement of (point index | sign) words, ranks from LDS atomics
 //   5 accum    load-balanced bucket accumulation: thread t owns sorted entries
-//              [t 2^s, (t+1) 2^s) and folds runs of equal keys with mixed XYZZ + affine adds;
+//              [t 2^s, (t+1) 2^s), finds their buckets from the offsets (binary search, then
+//              a walk) and folds each run with mixed XYZZ + affine adds;
 //              a run that began in an earlier segment is written as a continuation partial
 //   6 fix      each bucket adds its continuation partials (heavy buckets: pairwise tree)
 //   7 pyramid  bucket reduction sum_b (b+1) B_b without serial running sums: level k pairs
@@ -27,6 +28,7 @@
 #include <vector>
 
 #include "curve.hpp"
+#include "curvez.hpp"
 #include "hostfield.hpp"
 #include "msm.hpp"
 #include "runtime.hpp"
@@ -174,8 +176,7 @@ __global__ __launch_bounds__(kThreads) void scan_final_kernel(const uint32_t *__
 }
 
 __global__ __launch_bounds__(kThreads) void msm_scatter_kernel(const uint32_t *__restrict__ dig, uint32_t n, int logB,
-                                                              uint32_t *__restrict__ cursor, uint32_t *__restrict__ keys,
-                                                              uint32_t *__restrict__ vals) {
+                                                              uint32_t *__restrict__ cursor, uint32_t *__restrict__ vals) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t w = blockIdx.y;
   if (i >= n) return;
@@ -183,52 +184,216 @@ __global__ __launch_bounds__(kThreads) void msm_scatter_kernel(const uint32_t *_
   if (code == kZeroDigit) return;
   const uint32_t key = (w << logB) + (code >> 1);
   const uint32_t pos = atomicAdd(&cursor[key], 1u);
-  keys[pos] = key;
   vals[pos] = i | (code << 31);
+}
+
+// LDS-staged counting sort for windows of up to 17 bits.  Workgroup (chunk, window) counts
+// the digits of at most 65535 consecutive points in LDS with packed 16-bit counters (two
+// buckets per u32 word, 2^(c-1) buckets <= 64 Ki -> <= 128 KiB of LDS), so one bucket's
+// count in a chunk never exceeds 16 bits.
+static constexpr int kSortThreads = 1024;
+static constexpr uint32_t kChunk = 32768;  // < 2^16: a chunk's bucket count fits 16 bits
+
+__global__ __launch_bounds__(kSortThreads) void msm_chunk_hist_kernel(const uint32_t *__restrict__ dig, uint32_t n,
+                                                                      int logB, uint16_t *__restrict__ hist) {
+  extern __shared__ uint32_t cnt[];  // 2^(logB-1) words (2^logB packed u16 counters)
+  const uint32_t B = 1u << logB, words = (B + 1) >> 1;
+  const uint32_t chunk = blockIdx.x, w = blockIdx.y, nchunks = gridDim.x;
+  for (uint32_t j = threadIdx.x; j < words; j += kSortThreads) cnt[j] = 0;
+  __syncthreads();
+  const uint32_t i0 = chunk * kChunk;
+  uint32_t i1 = i0 + kChunk;
+  if (i1 > n) i1 = n;
+  const uint32_t *d = dig + (size_t)w * n;
+  for (uint32_t i = i0 + threadIdx.x; i < i1; i += kSortThreads) {
+    const uint32_t code = d[i];
+    if (code == kZeroDigit) continue;
+    const uint32_t b = code >> 1;
+    atomicAdd(&cnt[b >> 1], 1u << (16 * (b & 1)));
+  }
+  __syncthreads();
+  uint16_t *h = hist + ((size_t)w * nchunks + chunk) * B;
+  for (uint32_t b = threadIdx.x; b < B; b += kSortThreads) h[b] = (uint16_t)(cnt[b >> 1] >> (16 * (b & 1)));
+}
+
+// count[w B + b] = sum over chunks
+__global__ __launch_bounds__(kThreads) void msm_bucket_total_kernel(const uint16_t *__restrict__ hist, uint32_t W,
+                                                                   uint32_t logB, uint32_t nchunks,
+                                                                   uint32_t *__restrict__ count) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t B = 1u << logB;
+  if (t >= (uint64_t)W * B) return;
+  const uint32_t w = (uint32_t)(t >> logB), b = (uint32_t)(t & (B - 1));
+  uint32_t s = 0;
+  for (uint32_t c = 0; c < nchunks; c++) s += hist[((size_t)w * nchunks + c) * B + b];
+  count[t] = s;
+}
+
+// pre[w][chunk][b] = off[w B + b] + sum of the bucket's counts in earlier chunks
+__global__ __launch_bounds__(kThreads) void msm_chunk_prefix_kernel(const uint16_t *__restrict__ hist,
+                                                                   const uint32_t *__restrict__ off, uint32_t W,
+                                                                   uint32_t logB, uint32_t nchunks,
+                                                                   uint32_t *__restrict__ pre) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t B = 1u << logB;
+  if (t >= (uint64_t)W * B) return;
+  const uint32_t w = (uint32_t)(t >> logB), b = (uint32_t)(t & (B - 1));
+  uint32_t run = off[t];
+  for (uint32_t c = 0; c < nchunks; c++) {
+    const size_t k = ((size_t)w * nchunks + c) * B + b;
+    pre[k] = run;
+    run += hist[k];
+  }
+}
+
+__global__ __launch_bounds__(kSortThreads) void msm_chunk_scatter_kernel(const uint32_t *__restrict__ dig, uint32_t n,
+                                                                         int logB, const uint32_t *__restrict__ pre,
+                                                                         uint32_t *__restrict__ vals) {
+  extern __shared__ uint32_t rank[];
+  const uint32_t B = 1u << logB, words = (B + 1) >> 1;
+  const uint32_t chunk = blockIdx.x, w = blockIdx.y, nchunks = gridDim.x;
+  for (uint32_t j = threadIdx.x; j < words; j += kSortThreads) rank[j] = 0;
+  __syncthreads();
+  const uint32_t i0 = chunk * kChunk;
+  uint32_t i1 = i0 + kChunk;
+  if (i1 > n) i1 = n;
+  const uint32_t *d = dig + (size_t)w * n;
+  const uint32_t *pb = pre + ((size_t)w * nchunks + chunk) * B;
+  for (uint32_t i = i0 + threadIdx.x; i < i1; i += kSortThreads) {
+    const uint32_t code = d[i];
+    if (code == kZeroDigit) continue;
+    const uint32_t b = code >> 1;
+    const uint32_t sh = 16 * (b & 1);
+    const uint32_t r = (atomicAdd(&rank[b >> 1], 1u << sh) >> sh) & 0xffffu;
+    vals[pb[b] + r] = i | (code << 31);
+  }
+}
+
+// windows of up to 16 bits: the chunk's absolute bucket bases fit in LDS (2^15 u32 = 128 KiB)
+// and the LDS atomic returns the final position directly
+__global__ __launch_bounds__(kSortThreads) void msm_chunk_scatter_lds_kernel(const uint32_t *__restrict__ dig,
+                                                                             uint32_t n, int logB,
+                                                                             const uint32_t *__restrict__ pre,
+                                                                             uint32_t *__restrict__ vals) {
+  extern __shared__ uint32_t base[];
+  const uint32_t B = 1u << logB;
+  const uint32_t chunk = blockIdx.x, w = blockIdx.y, nchunks = gridDim.x;
+  const uint32_t *pb = pre + ((size_t)w * nchunks + chunk) * B;
+  for (uint32_t j = threadIdx.x; j < B; j += kSortThreads) base[j] = pb[j];
+  __syncthreads();
+  const uint32_t i0 = chunk * kChunk;
+  uint32_t i1 = i0 + kChunk;
+  if (i1 > n) i1 = n;
+  const uint32_t *d = dig + (size_t)w * n;
+  for (uint32_t i = i0 + threadIdx.x; i < i1; i += kSortThreads) {
+    const uint32_t code = d[i];
+    if (code == kZeroDigit) continue;
+    const uint32_t pos = atomicAdd(&base[code >> 1], 1u);
+    vals[pos] = i | (code << 31);
+  }
 }
 
 // ---------------------------------------------------------------------------------------
 // 5. bucket accumulation
 
 template <class C>
-__device__ __forceinline__ void flush_run(const Xyzz<C> &acc, bool cont_run, uint64_t seg, uint32_t key,
-                                          uint32_t *bsum, uint32_t *cont) {
-  constexpr int PW = 4 * C::Fp::N;
-  uint32_t *dst = cont_run ? cont + seg * PW : bsum + (size_t)key * PW;
-  xyzz_store(dst, acc);
+__device__ __forceinline__ void flush_run(const XyzzZ<C> &acc, bool cont_run, uint64_t seg, uint32_t key,
+                                          int32_t *bsum, int32_t *cont) {
+  constexpr int PW = ZLayout<C>::XYZZ;
+  int32_t *dst = cont_run ? cont + seg * PW : bsum + (size_t)key * PW;
+  zxyzz_store(dst, acc);
+}
+
+// affine points, external Montgomery words -> internal reduced-radix form (once per MSM)
+template <class C>
+__global__ __launch_bounds__(kThreads) void msm_points_in_kernel(const uint32_t *__restrict__ pts, uint32_t n,
+                                                                int32_t *__restrict__ out) {
+  using F = typename C::Fp;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t w[2 * F::N];
+  const uint4 *q = reinterpret_cast<const uint4 *>(pts + (size_t)i * 2 * F::N);
+#pragma unroll
+  for (int j = 0; j < F::N / 2; j++) {
+    uint4 v = q[j];
+    w[4 * j] = v.x;
+    w[4 * j + 1] = v.y;
+    w[4 * j + 2] = v.z;
+    w[4 * j + 3] = v.w;
+  }
+  AffZ<C> a;
+  zfrom_ext(a.x, w);
+  zfrom_ext(a.y, w + F::N);
+  int32_t *o = out + (size_t)i * ZLayout<C>::AFF;
+  zaff_store(o, a);
+#pragma unroll
+  for (int j = 2 * ZP<C>::T::N; j < ZLayout<C>::AFF; j++) o[j] = 0;
+}
+
+// final window terms: internal XYZZ -> external canonical words (4 field elements each)
+template <class C>
+__global__ __launch_bounds__(kThreads) void msm_points_out_kernel(const int32_t *__restrict__ in, uint32_t cnt,
+                                                                 uint32_t *__restrict__ out) {
+  using F = typename C::Fp;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= cnt) return;
+  XyzzZ<C> p;
+  zxyzz_load(p, in + (size_t)i * ZLayout<C>::XYZZ);
+  uint32_t *o = out + (size_t)i * 4 * F::N;
+  if (zxyzz_is_inf(p)) {
+    for (int j = 0; j < 4 * F::N; j++) o[j] = 0;
+    return;
+  }
+  zto_ext(o, p.x);
+  zto_ext(o + F::N, p.y);
+  zto_ext(o + 2 * F::N, p.zz);
+  zto_ext(o + 3 * F::N, p.zzz);
+}
+
+// bucket holding sorted entry e: the last b with off[b] <= e (off is non-decreasing)
+__device__ __forceinline__ uint32_t bucket_of(const uint32_t *__restrict__ off, uint32_t nkeys, uint32_t e) {
+  uint32_t lo = 0, hi = nkeys;  // answer in [lo, hi)
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (off[mid] <= e) lo = mid;
+    else hi = mid;
+  }
+  return lo;
 }
 
 template <class C>
-__global__ __launch_bounds__(kThreads) void msm_accum_kernel(const uint32_t *__restrict__ keys,
-                                                            const uint32_t *__restrict__ vals,
+__global__ __launch_bounds__(kThreads) void msm_accum_kernel(const uint32_t *__restrict__ vals,
                                                             const uint32_t *__restrict__ off, uint32_t nkeys,
-                                                            int seg_log, const uint32_t *__restrict__ points,
-                                                            uint32_t *__restrict__ bsum, uint32_t *__restrict__ cont) {
-  constexpr int AW = 2 * C::Fp::N;
+                                                            int seg_log, const int32_t *__restrict__ points,
+                                                            int32_t *__restrict__ bsum, int32_t *__restrict__ cont) {
+  constexpr int AW = ZLayout<C>::AFF;
   const uint32_t E = off[nkeys];
   const uint64_t seg = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t e0 = seg << seg_log;
   if (e0 >= E) return;
   uint64_t e1 = e0 + (1ull << seg_log);
   if (e1 > E) e1 = E;
-  uint32_t cur = keys[e0];
+  uint32_t cur = bucket_of(off, nkeys, (uint32_t)e0);
+  uint32_t next = off[cur + 1];
   bool cont_run = off[cur] < e0;
-  Xyzz<C> acc;
-  xyzz_set_inf(acc);
+  XyzzZ<C> acc;
+  zxyzz_set_inf(acc);
   for (uint64_t e = e0; e < e1; e++) {
-    const uint32_t k = keys[e];
-    if (k != cur) {
+    if (e == next) {
       flush_run<C>(acc, cont_run, seg, cur, bsum, cont);
-      cur = k;
       cont_run = false;
-      xyzz_set_inf(acc);
+      zxyzz_set_inf(acc);
+      do {
+        cur++;
+        next = off[cur + 1];
+      } while (next == e);
     }
     const uint32_t v = vals[e];
-    Aff<C> a;
-    aff_load(a, points + (size_t)(v & 0x7fffffffu) * AW);
-    if (aff_is_inf(a)) continue;
-    if (v >> 31) aff_neg(a);
-    xyzz_madd(acc, a);
+    AffZ<C> a;
+    zaff_load(a, points + (size_t)(v & 0x7fffffffu) * AW);
+    if (zaff_is_inf(a)) continue;
+    if (v >> 31) zneg(a.y, a.y);
+    zxyzz_madd(acc, a);
   }
   flush_run<C>(acc, cont_run, seg, cur, bsum, cont);
 }
@@ -238,11 +403,11 @@ __global__ __launch_bounds__(kThreads) void msm_accum_kernel(const uint32_t *__r
 
 template <class C>
 __global__ __launch_bounds__(kThreads) void msm_fix_kernel(const uint32_t *__restrict__ off, uint32_t nkeys,
-                                                          int seg_log, uint32_t *__restrict__ bsum,
-                                                          const uint32_t *__restrict__ cont,
+                                                          int seg_log, int32_t *__restrict__ bsum,
+                                                          const int32_t *__restrict__ cont,
                                                           uint32_t *__restrict__ heavy, uint32_t heavy_cap,
                                                           uint32_t limit) {
-  constexpr int PW = 4 * C::Fp::N;
+  constexpr int PW = ZLayout<C>::XYZZ;
   const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nkeys) return;
   const uint32_t s = off[b], t = off[b + 1];
@@ -256,22 +421,22 @@ __global__ __launch_bounds__(kThreads) void msm_fix_kernel(const uint32_t *__res
     if (idx < heavy_cap) heavy[2 + idx] = b;
     return;
   }
-  Xyzz<C> acc, q;
-  xyzz_load(acc, bsum + (size_t)b * PW);
+  XyzzZ<C> acc, q;
+  zxyzz_load(acc, bsum + (size_t)b * PW);
   for (uint64_t j = fs + 1; j <= ls; j++) {
-    xyzz_load(q, cont + j * PW);
-    xyzz_add(acc, acc, q);
+    zxyzz_load(q, cont + j * PW);
+    zxyzz_add(acc, acc, q);
   }
-  xyzz_store(bsum + (size_t)b * PW, acc);
+  zxyzz_store(bsum + (size_t)b * PW, acc);
 }
 
 // one pairwise round over the continuation partials of heavy buckets
 template <class C>
 __global__ __launch_bounds__(kThreads) void msm_heavy_round_kernel(const uint32_t *__restrict__ off,
                                                                   const uint32_t *__restrict__ heavy, int seg_log,
-                                                                  uint32_t *__restrict__ cont, int r,
+                                                                  int32_t *__restrict__ cont, int r,
                                                                   uint32_t pairs_per_bucket) {
-  constexpr int PW = 4 * C::Fp::N;
+  constexpr int PW = ZLayout<C>::XYZZ;
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t h = (uint32_t)(t / pairs_per_bucket), j = (uint32_t)(t % pairs_per_bucket);
   if (h >= heavy[0]) return;
@@ -280,38 +445,38 @@ __global__ __launch_bounds__(kThreads) void msm_heavy_round_kernel(const uint32_
   const uint64_t K = ls - fs, base = fs + 1;
   const uint64_t i0 = (uint64_t)j << (r + 1), i1 = i0 + (1ull << r);
   if (i1 >= K) return;
-  Xyzz<C> a, q;
-  xyzz_load(a, cont + (base + i0) * PW);
-  xyzz_load(q, cont + (base + i1) * PW);
-  xyzz_add(a, a, q);
-  xyzz_store(cont + (base + i0) * PW, a);
+  XyzzZ<C> a, q;
+  zxyzz_load(a, cont + (base + i0) * PW);
+  zxyzz_load(q, cont + (base + i1) * PW);
+  zxyzz_add(a, a, q);
+  zxyzz_store(cont + (base + i0) * PW, a);
 }
 
 template <class C>
 __global__ __launch_bounds__(kThreads) void msm_heavy_final_kernel(const uint32_t *__restrict__ off,
                                                                   const uint32_t *__restrict__ heavy, int seg_log,
-                                                                  uint32_t *__restrict__ bsum,
-                                                                  const uint32_t *__restrict__ cont) {
-  constexpr int PW = 4 * C::Fp::N;
+                                                                  int32_t *__restrict__ bsum,
+                                                                  const int32_t *__restrict__ cont) {
+  constexpr int PW = ZLayout<C>::XYZZ;
   const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
   if (h >= heavy[0]) return;
   const uint32_t b = heavy[2 + h];
   const uint64_t fs = off[b] >> seg_log;
-  Xyzz<C> a, q;
-  xyzz_load(a, bsum + (size_t)b * PW);
-  xyzz_load(q, cont + (fs + 1) * PW);
-  xyzz_add(a, a, q);
-  xyzz_store(bsum + (size_t)b * PW, a);
+  XyzzZ<C> a, q;
+  zxyzz_load(a, bsum + (size_t)b * PW);
+  zxyzz_load(q, cont + (fs + 1) * PW);
+  zxyzz_add(a, a, q);
+  zxyzz_store(bsum + (size_t)b * PW, a);
 }
 
 // ---------------------------------------------------------------------------------------
 // 7. bucket pyramid: input [A][W][M] -> output [A+1][W][M/2]
 
 template <class C>
-__global__ __launch_bounds__(kThreads) void msm_pyramid_kernel(const uint32_t *__restrict__ in,
-                                                              uint32_t *__restrict__ out, uint32_t W, uint32_t M,
+__global__ __launch_bounds__(kThreads) void msm_pyramid_kernel(const int32_t *__restrict__ in,
+                                                              int32_t *__restrict__ out, uint32_t W, uint32_t M,
                                                               uint32_t A) {
-  constexpr int PW = 4 * C::Fp::N;
+  constexpr int PW = ZLayout<C>::XYZZ;
   const uint32_t half = M >> 1;
   const uint64_t per = (uint64_t)W * half;
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -319,12 +484,12 @@ __global__ __launch_bounds__(kThreads) void msm_pyramid_kernel(const uint32_t *_
   const uint32_t a = (uint32_t)(t / per);
   const uint64_t rem = t % per;  // = w * half + i
   const uint64_t src = (uint64_t)a * W * M + 2 * rem;
-  Xyzz<C> p0, p1, s;
-  xyzz_load(p0, in + src * PW);
-  xyzz_load(p1, in + (src + 1) * PW);
-  xyzz_add(s, p0, p1);
-  xyzz_store(out + ((uint64_t)a * per + rem) * PW, s);
-  if (a == 0) xyzz_store(out + ((uint64_t)A * per + rem) * PW, p1);
+  XyzzZ<C> p0, p1, s;
+  zxyzz_load(p0, in + src * PW);
+  zxyzz_load(p1, in + (src + 1) * PW);
+  zxyzz_add(s, p0, p1);
+  zxyzz_store(out + ((uint64_t)a * per + rem) * PW, s);
+  if (a == 0) zxyzz_store(out + ((uint64_t)A * per + rem) * PW, p1);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -472,7 +637,9 @@ template <class C>
 static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const uint64_t *d_points,
                    const zkg_msm_schedule *user, hipStream_t st, MsmResult *out) {
   using F = typename C::Fp;
-  constexpr int PW = 4 * F::N;  // u32 words per XYZZ point
+  constexpr int PW = 4 * F::N;               // u32 words per external XYZZ point (host side)
+  constexpr int ZW = ZLayout<C>::XYZZ;       // words per internal XYZZ point (device buffers)
+  constexpr int AW = ZLayout<C>::AFF;        // words per internal affine point
   const MsmSchedule sch = resolve_schedule(C::id, n, enl, user);
   const int c = sch.c, W = sch.windows, logB = c - 1;
   const uint32_t B = 1u << logB;
@@ -485,54 +652,99 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   const int seg_log = sch.seg_log;
   const uint64_t nseg = (total >> seg_log) + 1;
 
-  DevBuf dig, keys, vals, count, off, cursor, bsum, cont, heavy, pyrA, pyrB, scan_b;
+  DevBuf dig, vals, count, off, cursor, bsum, cont, heavy, pyrA, pyrB, scan_b, hist, pre_big, zpts, outx;
   ZK_TRY(dig.alloc(total * 4));
-  ZK_TRY(keys.alloc(total * 4));
   ZK_TRY(vals.alloc(total * 4));
   ZK_TRY(count.alloc((nkeys + 1) * 4));
   ZK_TRY(off.alloc((nkeys + 1) * 4));
-  ZK_TRY(cursor.alloc((nkeys + 1) * 4));
-  ZK_TRY(bsum.alloc(nkeys * PW * 4));
-  ZK_TRY(cont.alloc(nseg * PW * 4));
+  ZK_TRY(bsum.alloc(nkeys * ZW * 4));
+  ZK_TRY(cont.alloc(nseg * ZW * 4));
+  ZK_TRY(zpts.alloc((uint64_t)n * AW * 4));
   const uint32_t heavy_cap = 1u << 16;
   ZK_TRY(heavy.alloc((heavy_cap + 2) * 4));
-  ZK_TRY(pyrA.alloc(nkeys * PW * 4));
-  ZK_TRY(pyrB.alloc(nkeys * PW * 4));
+  ZK_TRY(pyrA.alloc(nkeys * ZW * 4));
+  ZK_TRY(pyrB.alloc(nkeys * ZW * 4));
   const uint64_t per_block = (uint64_t)kThreads * kScanPer;
   const uint64_t nscan = nkeys + 1;
   const uint32_t nsb = (uint32_t)((nscan + per_block - 1) / per_block);
   ZK_TRY(scan_b.alloc((uint64_t)nsb * 4));
+  const bool lds_sort = logB <= 16;
+  const uint32_t nchunks = (uint32_t)((n + kChunk - 1) / kChunk);
+  if (lds_sort) {
+    ZK_TRY(hist.alloc((uint64_t)W * nchunks * B * 2));
+    // the chunk prefixes (u32) live in the pyramid buffer, idle until the reduction, when it is
+    // large enough
+    if ((uint64_t)W * nchunks * B * 4 > nkeys * ZW * 4) ZK_TRY(pre_big.alloc((uint64_t)W * nchunks * B * 4));
+    static bool attr_done = false;
+    if (!attr_done) {
+      ZK_HIP(hipFuncSetAttribute((const void *)msm_chunk_hist_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 128 * 1024));
+      ZK_HIP(hipFuncSetAttribute((const void *)msm_chunk_scatter_kernel,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024));
+      ZK_HIP(hipFuncSetAttribute((const void *)msm_chunk_scatter_lds_kernel,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024));
+      attr_done = true;
+    }
+  } else {
+    ZK_TRY(cursor.alloc((nkeys + 1) * 4));
+  }
 
   Events ev;
   auto t_host0 = std::chrono::steady_clock::now();
   ev.mark(st);
-  ZK_HIP(hipMemsetAsync(count.p, 0, (nkeys + 1) * 4, st));
-  ZK_HIP(hipMemsetAsync(bsum.p, 0, nkeys * PW * 4, st));
+  if (!lds_sort) ZK_HIP(hipMemsetAsync(count.p, 0, (nkeys + 1) * 4, st));
+  ZK_HIP(hipMemsetAsync(bsum.p, 0, nkeys * ZW * 4, st));
   ZK_HIP(hipMemsetAsync(heavy.p, 0, 8, st));
   const dim3 blk(kThreads);
   const dim3 gpts((n + kThreads - 1) / kThreads);
+  hipLaunchKernelGGL(msm_points_in_kernel<C>, gpts, blk, 0, st, (const uint32_t *)d_points, (uint32_t)n,
+                     zpts.as<int32_t>());
   hipLaunchKernelGGL(msm_digits_kernel<C>, gpts, blk, 0, st, d_scalars, enl, mont, (uint32_t)n, c, W,
                      dig.as<uint32_t>());
   const dim3 gpw((n + kThreads - 1) / kThreads, W);
-  hipLaunchKernelGGL(msm_count_kernel, gpw, blk, 0, st, dig.as<uint32_t>(), (uint32_t)n, logB,
-                     count.as<uint32_t>());
+  const dim3 gkeys((unsigned)((nkeys + kThreads - 1) / kThreads));
+  const size_t lds_words = ((size_t)B + 1) / 2 * 4;
+  if (lds_sort) {
+    const dim3 gch(nchunks, W);
+    hipLaunchKernelGGL(msm_chunk_hist_kernel, gch, dim3(kSortThreads), lds_words, st, dig.as<uint32_t>(),
+                       (uint32_t)n, logB, hist.as<uint16_t>());
+    hipLaunchKernelGGL(msm_bucket_total_kernel, gkeys, blk, 0, st, hist.as<uint16_t>(), (uint32_t)W,
+                       (uint32_t)logB, nchunks, count.as<uint32_t>());
+    ZK_HIP(hipMemsetAsync(count.as<uint32_t>() + nkeys, 0, 4, st));
+  } else {
+    hipLaunchKernelGGL(msm_count_kernel, gpw, blk, 0, st, dig.as<uint32_t>(), (uint32_t)n, logB,
+                       count.as<uint32_t>());
+  }
   hipLaunchKernelGGL(scan_block_sums_kernel, dim3(nsb), blk, 0, st, count.as<uint32_t>(), (size_t)nscan,
                      scan_b.as<uint32_t>());
   hipLaunchKernelGGL(scan_top_kernel, dim3(1), blk, 0, st, scan_b.as<uint32_t>(), nsb);
+  uint32_t *cur_or_unused = lds_sort ? pyrB.as<uint32_t>() : cursor.as<uint32_t>();
   hipLaunchKernelGGL(scan_final_kernel, dim3(nsb), blk, 0, st, count.as<uint32_t>(), (size_t)nkeys,
-                     scan_b.as<uint32_t>(), off.as<uint32_t>(), cursor.as<uint32_t>());
-  hipLaunchKernelGGL(msm_scatter_kernel, gpw, blk, 0, st, dig.as<uint32_t>(), (uint32_t)n, logB,
-                     cursor.as<uint32_t>(), keys.as<uint32_t>(), vals.as<uint32_t>());
+                     scan_b.as<uint32_t>(), off.as<uint32_t>(), cur_or_unused);
+  if (lds_sort) {
+    uint32_t *pre = pre_big.p ? pre_big.as<uint32_t>() : pyrA.as<uint32_t>();
+    hipLaunchKernelGGL(msm_chunk_prefix_kernel, gkeys, blk, 0, st, hist.as<uint16_t>(), off.as<uint32_t>(),
+                       (uint32_t)W, (uint32_t)logB, nchunks, pre);
+    if (logB <= 15)
+      hipLaunchKernelGGL(msm_chunk_scatter_lds_kernel, dim3(nchunks, W), dim3(kSortThreads), (size_t)B * 4, st,
+                         dig.as<uint32_t>(), (uint32_t)n, logB, pre, vals.as<uint32_t>());
+    else
+      hipLaunchKernelGGL(msm_chunk_scatter_kernel, dim3(nchunks, W), dim3(kSortThreads), lds_words, st,
+                         dig.as<uint32_t>(), (uint32_t)n, logB, pre, vals.as<uint32_t>());
+  } else {
+    hipLaunchKernelGGL(msm_scatter_kernel, gpw, blk, 0, st, dig.as<uint32_t>(), (uint32_t)n, logB,
+                       cursor.as<uint32_t>(), vals.as<uint32_t>());
+  }
   ZK_HIP(hipGetLastError());
   ev.mark(st);
   hipLaunchKernelGGL(msm_accum_kernel<C>, dim3((unsigned)((nseg + kThreads - 1) / kThreads)), blk, 0, st,
-                     keys.as<uint32_t>(), vals.as<uint32_t>(), off.as<uint32_t>(), (uint32_t)nkeys, seg_log,
-                     (const uint32_t *)d_points, bsum.as<uint32_t>(), cont.as<uint32_t>());
+                     vals.as<uint32_t>(), off.as<uint32_t>(), (uint32_t)nkeys, seg_log,
+                     zpts.as<int32_t>(), bsum.as<int32_t>(), cont.as<int32_t>());
   ZK_HIP(hipGetLastError());
   ev.mark(st);
   const uint32_t limit = 16;
   hipLaunchKernelGGL(msm_fix_kernel<C>, dim3((unsigned)((nkeys + kThreads - 1) / kThreads)), blk, 0, st,
-                     off.as<uint32_t>(), (uint32_t)nkeys, seg_log, bsum.as<uint32_t>(), cont.as<uint32_t>(),
+                     off.as<uint32_t>(), (uint32_t)nkeys, seg_log, bsum.as<int32_t>(), cont.as<int32_t>(),
                      heavy.as<uint32_t>(), heavy_cap, limit);
   ZK_HIP(hipGetLastError());
   uint32_t hh[2], nent = 0;
@@ -553,16 +765,16 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
       const uint32_t pairs = (uint32_t)((kmax + (2ull << r) - 1) >> (r + 1));
       const uint64_t thr = (uint64_t)nh * pairs;
       hipLaunchKernelGGL(msm_heavy_round_kernel<C>, dim3((unsigned)((thr + kThreads - 1) / kThreads)), blk, 0, st,
-                         off.as<uint32_t>(), heavy.as<uint32_t>(), seg_log, cont.as<uint32_t>(), r, pairs);
+                         off.as<uint32_t>(), heavy.as<uint32_t>(), seg_log, cont.as<int32_t>(), r, pairs);
     }
     hipLaunchKernelGGL(msm_heavy_final_kernel<C>, dim3((nh + kThreads - 1) / kThreads), blk, 0, st,
-                       off.as<uint32_t>(), heavy.as<uint32_t>(), seg_log, bsum.as<uint32_t>(), cont.as<uint32_t>());
+                       off.as<uint32_t>(), heavy.as<uint32_t>(), seg_log, bsum.as<int32_t>(), cont.as<int32_t>());
     ZK_HIP(hipGetLastError());
   }
   ev.mark(st);
   // pyramid
-  const uint32_t *in = bsum.as<uint32_t>();
-  uint32_t *bufs[2] = {pyrA.as<uint32_t>(), pyrB.as<uint32_t>()};
+  const int32_t *in = bsum.as<int32_t>();
+  int32_t *bufs[2] = {pyrA.as<int32_t>(), pyrB.as<int32_t>()};
   uint32_t A = 1, M = B;
   int k = 0;
   while (M > 1) {
@@ -576,9 +788,14 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   }
   ZK_HIP(hipGetLastError());
   ev.mark(st);
-  // A arrays of W points: [0] = Sum_w, [j] = U_{w, j-1}
-  std::vector<uint32_t> res((size_t)A * W * PW);
-  ZK_HIP(hipMemcpyAsync(res.data(), in, res.size() * 4, hipMemcpyDeviceToHost, st));
+  // A arrays of W points: [0] = Sum_w, [j] = U_{w, j-1}; converted to external canonical words
+  const uint32_t nres = A * (uint32_t)W;
+  ZK_TRY(outx.alloc((uint64_t)nres * PW * 4));
+  hipLaunchKernelGGL(msm_points_out_kernel<C>, dim3((nres + kThreads - 1) / kThreads), blk, 0, st, in, nres,
+                     outx.as<uint32_t>());
+  ZK_HIP(hipGetLastError());
+  std::vector<uint32_t> res((size_t)nres * PW);
+  ZK_HIP(hipMemcpyAsync(res.data(), outx.p, res.size() * 4, hipMemcpyDeviceToHost, st));
   ZK_HIP(hipStreamSynchronize(st));
   ev.mark(st);
   auto t_host1 = std::chrono::steady_clock::now();
