@@ -189,6 +189,51 @@ def cpu_baseline(case, sample_log):
             "matches_gpu": bool((gpu == out).all())}
 
 
+def bench_ntt_batch(njobs=256, seed=4):
+    """256 BN254 Fr NTTs, log sizes uniform in 10..22, fair-coin forward/inverse, half on a coset
+    (seeded eta): one hipGraph replay against one plain call per job; outputs must agree."""
+    rng = np.random.default_rng(seed)
+    ms = rng.integers(10, 23, size=njobs)
+    inv = rng.integers(0, 2, size=njobs)
+    coset = rng.integers(0, 2, size=njobs)
+    etas = uniform_scalars(rng, njobs, BN_R)
+    jobs, plain_out = [], []
+    total = 0
+    for j in range(njobs):
+        N = 1 << int(ms[j])
+        total += N
+        d_src = zkgpu.fill_random(0, 1, N, 1000 + j)
+        jobs.append((int(ms[j]), int(inv[j]), etas[j] if coset[j] else None, d_src, zkgpu.DeviceArray(4 * N)))
+        plain_out.append(zkgpu.DeviceArray(4 * N))
+    gens = {m: zkgpu.fft_generator(0, m) for m in range(10, 23)}
+
+    def plain():
+        for (m, iv, eta, s, _), t in zip(jobs, plain_out):
+            zkgpu.ntt_device(0, m, gens[m], s, t, inverse=bool(iv), eta=eta)
+    plain()  # warm-up (builds the shared twiddle table)
+    zkgpu.synchronize()
+    t0 = time.perf_counter()
+    plain()
+    zkgpu.synchronize()
+    t_plain = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    plan = zkgpu.NttBatch(0, jobs)
+    t_create = time.perf_counter() - t0
+    plan.run()
+    reps = 3
+    t0 = time.perf_counter()
+    dev_ms = [plan.run() for _ in range(reps)]
+    t_batch = (time.perf_counter() - t0) / reps
+    same = all(bool((j[4].to_host() == p.to_host()).all()) for j, p in zip(jobs, plain_out))
+    plan.close()
+    bytes_moved = 64 * total
+    return {"ntt_batch_jobs": njobs, "ntt_batch_elements": int(total),
+            "ntt_batch_ms": round(t_batch * 1e3, 3), "ntt_batch_device_ms": round(min(dev_ms), 3),
+            "ntt_plain_ms": round(t_plain * 1e3, 3), "ntt_batch_create_ms": round(t_create * 1e3, 1),
+            "ntt_batch_GBps": round(bytes_moved / t_batch / 1e9, 1), "ntt_plain_GBps": round(bytes_moved / t_plain / 1e9, 1),
+            "ntt_batched_vs_plain": round(t_plain / t_batch, 3), "ntt_batch_bit_identical_to_plain": same}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -270,6 +315,9 @@ def main():
                       "bn254_ntt_2p22_imul_roofline_frac": round(ntt_mul / tf / peak_imul, 4),
                       "bn254_ntt_2p22_roundtrip_ok": rt_ok})
         log(f"ntt 2^22: fwd {tf * 1e3:.3f} ms inv {ti * 1e3:.3f} ms roundtrip {rt_ok}")
+        del dX, dY, dZ
+        extra.update(bench_ntt_batch())
+        log(f"ntt batch: {extra['ntt_batch_ms']} ms batched vs {extra['ntt_plain_ms']} ms plain, identical {extra['ntt_batch_bit_identical_to_plain']}")
 
     line = {"metric": "G1 MSM Mpoints/s @2^20 BN254+BLS12-381, NTT GB/s; batched vs plain, bit-exact",
             "value": round(value, 2), "unit": "Mpoints/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,

@@ -122,6 +122,22 @@ int zkg_msm(int curve, int npoints, const uint64_t *d_expos, int expo_nlimbs, co
  * gen/eta are host pointers. */
 int zkg_ntt(int curve, int m, const uint64_t *gen, const uint64_t *eta, int flags, const uint64_t *d_src,
             uint64_t *d_tgt);
+/* Batched NTTs of uneven sizes over the standard FFT domains (generator of order 2^m from
+ * lib/src/ZK/Algebra/Class/FFT.hs:60-68).  create() builds twiddle/coset tables and descriptor
+ * tables and captures one launch per pass round into a hipGraph; run() replays it
+ * synchronously (optionally returning the device time in ms).  Buffers must stay valid for
+ * the plan's lifetime; src != tgt. */
+typedef struct {
+  int m;                  /* log2 size, 1..30 */
+  int inverse;            /* 0 forward, 1 inverse */
+  const uint64_t *eta;    /* host Montgomery Fr coset shift (shifted variants), or NULL */
+  const uint64_t *d_src;
+  uint64_t *d_tgt;
+} zkg_ntt_job;
+int zkg_ntt_batch_create(int curve, int njobs, const zkg_ntt_job *jobs, void **plan);
+int zkg_ntt_batch_run(void *plan, double *ms);
+int zkg_ntt_batch_destroy(void *plan);
+
 /* generator of the size-2^m FFT subgroup, Montgomery form (lib/src/ZK/Algebra/Class/FFT.hs:60-68) */
 int zkg_fft_generator(int curve, int m, uint64_t *gen_out);
 /* NTT fast-path schedule (passes of at most 2^smax points, LDS tiles of 2^tile_log elements) */
@@ -129,6 +145,10 @@ int zkg_ntt_set_tuning(int smax, int tile_log);
 
 /* device elementwise field ops (field: 0 = Fp, 1 = Fr; op: see zikkurat-algebra_amd/csrc/peak.hpp) */
 int zkg_field_op(int curve, int field, int op, size_t n, const uint64_t *d_a, const uint64_t *d_b, uint64_t *d_out);
+
+/* n canonical pseudo-random field elements (field 0 = Fp, 1 = Fr) from a seed, on the device
+ * (synthetic benchmark inputs) */
+int zkg_fill_random(int curve, int field, size_t n, uint64_t seed, uint64_t *d_out);
 
 /* batch fixed-base scalar multiplication k_i * G of the standard generator, affine Montgomery
  * output (input generation for tests and benchmarks) */

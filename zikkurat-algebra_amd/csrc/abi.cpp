@@ -5,6 +5,7 @@
 #include <string.h>
 
 #include <mutex>
+#include <vector>
 
 #include "../../include/zkgpu.h"
 #include "hostfield.hpp"
@@ -248,6 +249,22 @@ int zkg_ntt(int curve, int m, const uint64_t *gen, const uint64_t *eta, int flag
   return ntt_device(curve, m, gen, eta, flags & 1, d_src, d_tgt, ctx->stream, nullptr);
 }
 
+int zkg_ntt_batch_create(int curve, int njobs, const zkg_ntt_job *jobs, void **plan) {
+  Context *ctx;
+  ZK_TRY(with_ctx(&ctx));
+  std::vector<NttBatchJob> js(njobs > 0 ? njobs : 0);
+  for (int j = 0; j < njobs; j++) js[j] = NttBatchJob{jobs[j].m, jobs[j].inverse, jobs[j].eta, jobs[j].d_src, jobs[j].d_tgt};
+  NttBatchPlan *pl = nullptr;
+  ZK_TRY(ntt_batch_create(curve, njobs, js.data(), &pl));
+  *plan = pl;
+  return ZK_OK;
+}
+int zkg_ntt_batch_run(void *plan, double *ms) { return ntt_batch_run((NttBatchPlan *)plan, ms); }
+int zkg_ntt_batch_destroy(void *plan) {
+  ntt_batch_destroy((NttBatchPlan *)plan);
+  return ZK_OK;
+}
+
 int zkg_fft_generator(int curve, int m, uint64_t *gen_out) {
   // 2-adic generators of Fr (lib/src/ZK/Algebra/Curves/BN128/Fr/Mont.hs:148-151, BLS12_381/Fr/Mont.hs:148-151)
   static const uint64_t bn_gen[4] = {0x636e735580d13d9cULL, 0xa22bf3742445ffd6ULL, 0x56452ac01eb203d8ULL,
@@ -277,6 +294,10 @@ int zkg_ntt_set_tuning(int smax, int tile_log) {
 
 int zkg_field_op(int curve, int field, int op, size_t n, const uint64_t *d_a, const uint64_t *d_b, uint64_t *d_out) {
   return field_op_device(curve, field, op, n, d_a, d_b, d_out);
+}
+
+int zkg_fill_random(int curve, int field, size_t n, uint64_t seed, uint64_t *d_out) {
+  return fill_random_device(curve, field, n, seed, d_out);
 }
 
 int zkg_g1_scalar_mul_gen(int curve, int n, const uint64_t *d_scalars, uint64_t *d_out_affine) {

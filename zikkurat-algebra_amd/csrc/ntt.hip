@@ -92,10 +92,9 @@ struct PassArgs {
   int split[6];             // log sizes of the earlier passes
 };
 
+// one workgroup's share of a pass: `blk` indexes the pass's tiles
 template <class F>
-__global__ __launch_bounds__(kNttThreads) void ntt_pass_kernel(PassArgs a) {
-  extern __shared__ uint4 lds_raw[];
-  uint32_t *lds = reinterpret_cast<uint32_t *>(lds_raw);
+__device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t blk, uint32_t *lds) {
   const int S = 1 << a.s, T = 1 << a.logt;
   const int elems = S << a.logt;
   const int tid = threadIdx.x;
@@ -103,10 +102,10 @@ __global__ __launch_bounds__(kNttThreads) void ntt_pass_kernel(PassArgs a) {
   uint64_t r = 0, c0 = 0, rho0 = 0;
   if (!a.last) {
     const int lg = a.logc - a.logt;
-    r = (uint64_t)blockIdx.x >> lg;
-    c0 = ((uint64_t)blockIdx.x & ((1ull << lg) - 1)) << a.logt;
+    r = blk >> lg;
+    c0 = (blk & ((1ull << lg) - 1)) << a.logt;
   } else {
-    rho0 = (uint64_t)blockIdx.x << a.logt;
+    rho0 = blk << a.logt;
   }
 
   // ---- load (coalesced over c), optional coset pre-scale, bit-reversed LDS placement
@@ -190,6 +189,29 @@ __global__ __launch_bounds__(kNttThreads) void ntt_pass_kernel(PassArgs a) {
     }
     fstore(a.out + go * F::N, x);
   }
+}
+
+template <class F>
+__global__ __launch_bounds__(kNttThreads) void ntt_pass_kernel(PassArgs a) {
+  extern __shared__ uint4 lds_raw[];
+  ntt_pass_body<F>(a, blockIdx.x, reinterpret_cast<uint32_t *>(lds_raw));
+}
+
+// batched pass: item i owns blocks [prefix[i], prefix[i+1]) of the launch
+template <class F>
+__global__ __launch_bounds__(kNttThreads) void ntt_batch_pass_kernel(const PassArgs *__restrict__ items,
+                                                                     const uint32_t *__restrict__ prefix,
+                                                                     int nitems) {
+  extern __shared__ uint4 lds_raw[];
+  const uint32_t b = blockIdx.x;
+  int lo = 0, hi = nitems;  // last i with prefix[i] <= b
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (prefix[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  const PassArgs a = items[lo];
+  ntt_pass_body<F>(a, b - prefix[lo], reinterpret_cast<uint32_t *>(lds_raw));
 }
 
 // ---------------------------------------------------------------------------------------
@@ -555,6 +577,219 @@ static int ntt_device_t(int curve, int m, const uint64_t *gen, const uint64_t *e
   ZK_HIP(hipStreamSynchronize(st));           // tmp is released on return
   return ZK_OK;
 }
+
+// ---------------------------------------------------------------------------------------
+// batched NTTs of uneven sizes: one launch per pass round across all jobs, captured once
+// into a hipGraph and replayed
+
+zkh::Fe fft_generator_host(int curve, int m) {
+  // 2-adic generators of Fr (lib/src/ZK/Algebra/Curves/BN128/Fr/Mont.hs:148-151,
+  // BLS12_381/Fr/Mont.hs:148-151), Montgomery form
+  static const uint64_t bn_gen[4] = {0x636e735580d13d9cULL, 0xa22bf3742445ffd6ULL, 0x56452ac01eb203d8ULL,
+                                     0x1860ef942963f9e7ULL};
+  static const uint64_t bls_gen[4] = {0xb9b58d8c5f0e466aULL, 0x5b1b4c801819d7ecULL, 0x0af53ae352a31e64ULL,
+                                      0x5bf3adda19e9b27bULL};
+  const zkh::FieldDesc &f = fr_desc(curve);
+  const int M = curve == 0 ? 28 : 32;
+  zkh::Fe g = zkh::load(f, curve == 0 ? bn_gen : bls_gen);
+  for (int i = 0; i < M - m; i++) g = zkh::sqr(f, g);
+  return g;
+}
+
+struct NttBatchPlan {
+  int curve = 0;
+  int njobs = 0;
+  hipStream_t st = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  DevBuf tmp, eta, items, prefix;
+  uint64_t bytes_moved = 0;  // 2 * 32 * sum(N) (one read + one write of every vector)
+  ~NttBatchPlan() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+
+template <class F>
+static int ntt_batch_create_t(int curve, int njobs, const NttBatchJob *jobs, NttBatchPlan *pl) {
+  const zkh::FieldDesc &f = fr_desc(curve);
+  int mmax = 0;
+  for (int j = 0; j < njobs; j++) {
+    if (jobs[j].m < 1 || jobs[j].m > 30) {
+      set_error("ntt batch: job log sizes must be in [1, 30]");
+      return ZK_EINVAL;
+    }
+    if (jobs[j].d_src == jobs[j].d_tgt) {
+      set_error("ntt batch: in-place jobs are not supported");
+      return ZK_EINVAL;
+    }
+    if (jobs[j].m > mmax) mmax = jobs[j].m;
+  }
+  ZK_HIP(hipStreamCreateWithFlags(&pl->st, hipStreamNonBlocking));
+  hipStream_t st = pl->st;
+  // one twiddle table for the largest domain serves every job through a stride
+  zkh::Fe gmax = fft_generator_host(curve, mmax);
+  uint64_t gw[4];
+  zkh::store(f, gw, gmax);
+  const uint32_t *tw = nullptr;
+  int shift0 = 0;
+  ZK_TRY(get_twiddles<F>(curve, mmax, gw, &tw, &shift0, st));
+  // scratch for multi-pass jobs and coset tables
+  const NttTuning tu = g_tuning;
+  uint64_t tmp_words = 0, ncoset = 0;
+  std::vector<NttPlan> plans(njobs);
+  for (int j = 0; j < njobs; j++) {
+    plans[j] = ntt_plan(jobs[j].m, tu.smax, tu.tile_log);
+    if (plans[j].npass > 1) tmp_words += ((uint64_t)1 << jobs[j].m) * F::N;
+    if (jobs[j].eta) ncoset++;
+    pl->bytes_moved += 64ull << jobs[j].m;
+  }
+  if (tmp_words) ZK_TRY(pl->tmp.alloc(tmp_words * 4));
+  if (ncoset) ZK_TRY(pl->eta.alloc(ncoset * 3 * 2048 * F::N * 4));
+  std::vector<std::vector<PassArgs>> rounds;
+  std::vector<std::vector<uint32_t>> blocks;
+  uint64_t toff = 0, eoff = 0;
+  for (int j = 0; j < njobs; j++) {
+    const NttPlan &pj = plans[j];
+    const int m = jobs[j].m;
+    const bool inverse = jobs[j].inverse;
+    uint32_t *tmpj = nullptr;
+    if (pj.npass > 1) {
+      tmpj = pl->tmp.as<uint32_t>() + toff;
+      toff += ((uint64_t)1 << m) * F::N;
+    }
+    const uint32_t *etab = nullptr;
+    if (jobs[j].eta) {
+      uint32_t *e = pl->eta.as<uint32_t>() + eoff;
+      eoff += 3 * 2048 * F::N;
+      Fe<F> ed = to_dev<F>(zkh::load(f, jobs[j].eta));
+      hipLaunchKernelGGL(pow_table_kernel<F>, dim3(8), dim3(256), 0, st, e, ed, 1ull, 2048);
+      hipLaunchKernelGGL(pow_table_kernel<F>, dim3(8), dim3(256), 0, st, e + 2048 * F::N, ed, 2048ull, 2048);
+      hipLaunchKernelGGL(pow_table_kernel<F>, dim3(8), dim3(256), 0, st, e + 4096 * F::N, ed, 1ull << 22, 2048);
+      etab = e;
+    }
+    Fe<F> ninv = to_dev<F>(zkh::inv(f, zkh::from_u64(f, (uint64_t)1 << m)));
+    int logr = 0;
+    for (int p = 0; p < pj.npass; p++) {
+      PassArgs a;
+      memset(&a, 0, sizeof a);
+      const bool last = p == pj.npass - 1;
+      a.in = p == 0 ? (const uint32_t *)jobs[j].d_src : tmpj;
+      a.out = last ? (uint32_t *)jobs[j].d_tgt : tmpj;
+      a.tw = tw;
+      a.tw_shift = shift0 + (mmax - m);
+      a.logn = m;
+      a.s = pj.s[p];
+      a.logr = logr;
+      a.logc = m - logr - a.s;
+      a.last = last;
+      a.inverse = inverse;
+      a.eta_pre = (p == 0 && !inverse) ? etab : nullptr;
+      a.eta_post = (last && inverse) ? etab : nullptr;
+      if (last && inverse) {
+        a.has_ninv = 1;
+        for (int k = 0; k < F::N; k++) a.ninv[k] = ninv.w[k];
+      }
+      int tl = pj.tile_log - a.s;
+      if (tl < 0) tl = 0;
+      const int avail = last ? logr : a.logc;
+      if (tl > avail) tl = avail;
+      a.logt = tl;
+      a.nsplit = p;
+      for (int i = 0; i < p; i++) a.split[i] = pj.s[i];
+      const uint64_t groups = last ? (1ull << (logr - tl)) : ((1ull << logr) << (a.logc - tl));
+      if ((int)rounds.size() <= p) {
+        rounds.resize(p + 1);
+        blocks.resize(p + 1);
+      }
+      rounds[p].push_back(a);
+      blocks[p].push_back((uint32_t)groups);
+      logr += a.s;
+    }
+  }
+  ZK_HIP(hipGetLastError());
+  // device tables: all rounds' items and block prefixes, back to back
+  uint64_t nitems_total = 0;
+  for (auto &r : rounds) nitems_total += r.size();
+  ZK_TRY(pl->items.alloc(nitems_total * sizeof(PassArgs)));
+  ZK_TRY(pl->prefix.alloc((nitems_total + rounds.size()) * 4));
+  std::vector<PassArgs> h_items;
+  std::vector<uint32_t> h_prefix;
+  std::vector<uint32_t> round_blocks, round_item0, round_prefix0;
+  std::vector<size_t> round_lds;
+  for (size_t r = 0; r < rounds.size(); r++) {
+    round_item0.push_back((uint32_t)h_items.size());
+    round_prefix0.push_back((uint32_t)h_prefix.size());
+    uint32_t acc = 0;
+    size_t lds = 0;
+    for (size_t i = 0; i < rounds[r].size(); i++) {
+      h_items.push_back(rounds[r][i]);
+      h_prefix.push_back(acc);
+      acc += blocks[r][i];
+      const size_t l = ((size_t)1 << (rounds[r][i].s + rounds[r][i].logt)) * F::N * 4;
+      if (l > lds) lds = l;
+    }
+    h_prefix.push_back(acc);
+    round_blocks.push_back(acc);
+    round_lds.push_back(lds);
+  }
+  ZK_HIP(hipMemcpyAsync(pl->items.p, h_items.data(), h_items.size() * sizeof(PassArgs), hipMemcpyHostToDevice, st));
+  ZK_HIP(hipMemcpyAsync(pl->prefix.p, h_prefix.data(), h_prefix.size() * 4, hipMemcpyHostToDevice, st));
+  ZK_HIP(hipStreamSynchronize(st));
+  // capture the pass rounds
+  ZK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  for (size_t r = 0; r < rounds.size(); r++) {
+    hipLaunchKernelGGL(ntt_batch_pass_kernel<F>, dim3(round_blocks[r]), dim3(kNttThreads), round_lds[r], st,
+                       pl->items.as<PassArgs>() + round_item0[r], pl->prefix.as<uint32_t>() + round_prefix0[r],
+                       (int)rounds[r].size());
+  }
+  hipError_t ce = hipStreamEndCapture(st, &pl->graph);
+  if (ce != hipSuccess) {
+    set_error(std::string("ntt batch capture: ") + hipGetErrorString(ce));
+    return ZK_EHIP;
+  }
+  ZK_HIP(hipGraphInstantiate(&pl->exec, pl->graph, nullptr, nullptr, 0));
+  pl->curve = curve;
+  pl->njobs = njobs;
+  return ZK_OK;
+}
+
+int ntt_batch_create(int curve, int njobs, const NttBatchJob *jobs, NttBatchPlan **out) {
+  if ((curve != 0 && curve != 1) || njobs <= 0) {
+    set_error("ntt batch: bad curve or job count");
+    return ZK_EINVAL;
+  }
+  NttBatchPlan *pl = new NttBatchPlan();
+  int s = curve == 0 ? ntt_batch_create_t<Bn254Fr>(curve, njobs, jobs, pl)
+                     : ntt_batch_create_t<Bls381Fr>(curve, njobs, jobs, pl);
+  if (s != ZK_OK) {
+    delete pl;
+    return s;
+  }
+  *out = pl;
+  return ZK_OK;
+}
+
+int ntt_batch_run(NttBatchPlan *pl, double *ms) {
+  hipEvent_t a, b;
+  ZK_HIP(hipEventCreate(&a));
+  ZK_HIP(hipEventCreate(&b));
+  ZK_HIP(hipEventRecord(a, pl->st));
+  ZK_HIP(hipGraphLaunch(pl->exec, pl->st));
+  ZK_HIP(hipEventRecord(b, pl->st));
+  ZK_HIP(hipEventSynchronize(b));
+  float t = 0;
+  ZK_HIP(hipEventElapsedTime(&t, a, b));
+  if (ms) *ms = t;
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return ZK_OK;
+}
+
+void ntt_batch_destroy(NttBatchPlan *pl) { delete pl; }
+
+uint64_t ntt_batch_bytes(const NttBatchPlan *pl) { return pl->bytes_moved; }
 
 int ntt_device(int curve, int m, const uint64_t *gen, const uint64_t *eta, int flags, const void *d_in,
                void *d_out, hipStream_t st, const NttTuning *tune) {

@@ -36,4 +36,17 @@ bool ntt_generator_ok(int curve, int m, const uint64_t *gen);
 int ntt_device(int curve, int m, const uint64_t *gen, const uint64_t *eta, int flags, const void *d_in,
                void *d_out, hipStream_t st, const NttTuning *tune);
 
+struct NttBatchJob {
+  int m;
+  int inverse;
+  const uint64_t *eta;  // host Montgomery Fr coset shift, or nullptr
+  const void *d_src;
+  void *d_tgt;
+};
+struct NttBatchPlan;
+int ntt_batch_create(int curve, int njobs, const NttBatchJob *jobs, NttBatchPlan **out);
+int ntt_batch_run(NttBatchPlan *pl, double *ms);
+void ntt_batch_destroy(NttBatchPlan *pl);
+uint64_t ntt_batch_bytes(const NttBatchPlan *pl);
+
 }  // namespace zk

@@ -177,6 +177,37 @@ __global__ void field_op_kernel(int op, size_t n, const uint32_t *a, const uint3
   fstore(out + i * F::N, r);
 }
 
+// synthetic inputs: element i = splitmix64-style hash of (seed, i, word), top word masked so
+// that the value is below 2^(bits-2) < p (canonical; any canonical word string is also a valid
+// Montgomery-form element)
+__global__ void fill_random_kernel(uint64_t *out, size_t n, int words, uint64_t seed, uint64_t top_mask) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int k = 0; k < words; k++) {
+    uint64_t z = seed + 0x9E3779B97F4A7C15ull * (i * words + k + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    if (k == words - 1) z &= top_mask;
+    out[i * words + k] = z;
+  }
+}
+
+int fill_random_device(int curve, int field, size_t n, uint64_t seed, void *out) {
+  Context *ctx;
+  ZK_TRY(context(&ctx));
+  const int words = (field == 0 && curve == 1) ? 6 : 4;
+  // top word below the prime's top word: BN254 p, r < 0x30644e72..., BLS r < 0x73eda753...,
+  // BLS p < 0x1a0111ea...
+  const uint64_t mask = (curve == 0) ? 0x0fffffffffffffffull : (field == 0 ? 0x0fffffffffffffffull : 0x3fffffffffffffffull);
+  if (!n) return ZK_OK;
+  hipLaunchKernelGGL(fill_random_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                     (uint64_t *)out, n, words, seed, mask);
+  ZK_HIP(hipGetLastError());
+  ZK_HIP(hipStreamSynchronize(ctx->stream));
+  return ZK_OK;
+}
+
 int field_op_device(int curve, int field, int op, size_t n, const void *a, const void *b, void *out) {
   Context *ctx;
   ZK_TRY(context(&ctx));

@@ -25,6 +25,7 @@ enum FieldOp : int {
 };
 
 int microbench(int which, double *result);
+int fill_random_device(int curve, int field, size_t n, uint64_t seed, void *out);
 int field_op_device(int curve, int field, int op, size_t n, const void *a, const void *b, void *out);
 
 }  // namespace zk

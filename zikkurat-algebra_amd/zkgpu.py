@@ -65,6 +65,10 @@ def lib():
         L.zkg_microbench.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
         L.zkg_last_timing.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                       ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.zkg_ntt_batch_create.argtypes = [ctypes.c_int, ctypes.c_int, vp, ctypes.POINTER(vp)]
+        L.zkg_ntt_batch_run.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
+        L.zkg_ntt_batch_destroy.argtypes = [vp]
+        L.zkg_fill_random.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_size_t, ctypes.c_uint64, vp]
         L.zkg_device_info.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_size_t)]
         _lib = L
     return _lib
@@ -245,8 +249,50 @@ def ntt_device(curve, m, gen, d_src, d_tgt, inverse=False, eta=None):
                          NTT_INVERSE if inverse else NTT_FORWARD, d_src.ptr, d_tgt.ptr))
 
 
+class _NttJob(ctypes.Structure):
+    _fields_ = [("m", ctypes.c_int), ("inverse", ctypes.c_int), ("eta", u64p), ("d_src", vp), ("d_tgt", vp)]
+
+
+class NttBatch:
+    """Batched uneven NTTs: one hipGraph replay per run() (zkg_ntt_batch_*)."""
+
+    def __init__(self, curve, jobs):
+        """jobs: list of (m, inverse, eta_or_None, d_src DeviceArray, d_tgt DeviceArray)."""
+        self._etas = [(_c(e) if e is not None else None) for (_, _, e, _, _) in jobs]
+        arr = (_NttJob * len(jobs))()
+        for i, (m, inv, _, s, t) in enumerate(jobs):
+            arr[i].m, arr[i].inverse = m, int(inv)
+            arr[i].eta = _p(self._etas[i]) if self._etas[i] is not None else None
+            arr[i].d_src, arr[i].d_tgt = s.ptr, t.ptr
+        self.plan = vp()
+        _check(lib().zkg_ntt_batch_create(curve, len(jobs), arr, ctypes.byref(self.plan)))
+
+    def run(self):
+        ms = ctypes.c_double()
+        _check(lib().zkg_ntt_batch_run(self.plan, ctypes.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if self.plan:
+            lib().zkg_ntt_batch_destroy(self.plan)
+            self.plan = vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def field_op(curve, field, op, d_a, d_b, d_out, n):
     _check(lib().zkg_field_op(curve, field, op, n, d_a.ptr, d_b.ptr if d_b is not None else None, d_out.ptr))
+
+
+def fill_random(curve, field, n, seed):
+    words = 6 if (field == 0 and curve == BLS12_381) else 4
+    d = DeviceArray(n * words)
+    _check(lib().zkg_fill_random(curve, field, n, seed, d.ptr))
+    return d
 
 
 def scalar_mul_gen(curve, d_scalars, n):
