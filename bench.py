@@ -234,6 +234,37 @@ def bench_ntt_batch(njobs=256, seed=4):
             "ntt_batched_vs_plain": round(t_plain / t_batch, 3), "ntt_batch_bit_identical_to_plain": same}
 
 
+def bench_serve(njobs=64, qps_list=(100, 400), seed=5):
+    """KZG commits on BLS12-381: iNTT of n evaluations (n = 2^14..2^21, log-uniform) then an MSM
+    against a resident SRS of 2^21 points tau^i G; Poisson arrivals at each stated QPS; served
+    (4 workers, continuous batching) against the plain path (one job at a time)."""
+    rng = np.random.default_rng(seed)
+    srs_n = 1 << 21
+    tau = int(uniform_scalars(rng, 1, BLS_R)[0][0])
+    d_srs = zkgpu.srs_generate(1, srs_n, tau)
+    srs = zkgpu.PreparedPoints(1, d_srs, srs_n)
+    logns = [int(x) for x in rng.integers(14, 22, size=njobs)]
+    values = [uniform_scalars(rng, 1 << m, BLS_R) for m in logns]  # canonical words = valid Montgomery elements
+    out = {"serve_jobs": njobs, "serve_srs_log": 21, "serve_sizes": "2^14..2^21 log-uniform"}
+    for qps in qps_list:
+        arrivals = list(np.cumsum(rng.exponential(1.0 / qps, size=njobs)) - 0.0)
+        res = {}
+        for name, workers in (("plain", 1), ("served", 4)):
+            srv = zkgpu.Server(1, srs, 48 << 30, workers)
+            srv.run_trace(logns[:4], [0.0] * 4, values[:4])  # warm-up (twiddles, allocator)
+            outs, lat, wall = srv.run_trace(logns, arrivals, values)
+            peak, conc = srv.stats()
+            srv.close()
+            lat = np.array(lat)
+            res[name] = (outs, {"p50_ms": round(float(np.percentile(lat, 50)), 2),
+                                "p99_ms": round(float(np.percentile(lat, 99)), 2),
+                                "commits_per_s": round(njobs / wall, 1), "max_concurrent": conc,
+                                "peak_admitted_GB": round(peak / 2**30, 2)})
+        same = all((a == b_).all() for a, b_ in zip(res["plain"][0], res["served"][0]))
+        out[f"serve_qps{qps}"] = {"plain": res["plain"][1], "served": res["served"][1], "bit_identical": same}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -318,6 +349,8 @@ def main():
         del dX, dY, dZ
         extra.update(bench_ntt_batch())
         log(f"ntt batch: {extra['ntt_batch_ms']} ms batched vs {extra['ntt_plain_ms']} ms plain, identical {extra['ntt_batch_bit_identical_to_plain']}")
+        extra.update(bench_serve())
+        log(f"serve: {[(k, v) for k, v in extra.items() if k.startswith('serve_qps')]}")
 
     line = {"metric": "G1 MSM Mpoints/s @2^20 BN254+BLS12-381, NTT GB/s; batched vs plain, bit-exact",
             "value": round(value, 2), "unit": "Mpoints/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,

@@ -115,8 +115,13 @@ typedef struct {
 #define ZKG_MSM_MONT 1
 #define ZKG_MSM_PROJ_OUT 2
 #define ZKG_MSM_JAC_OUT 4
+#define ZKG_MSM_POINTS_INTERNAL 8   /* d_grps was written by zkg_msm_prepare_points */
 int zkg_msm(int curve, int npoints, const uint64_t *d_expos, int expo_nlimbs, const uint64_t *d_grps,
             uint64_t *tgt_host, int flags, const zkg_msm_schedule *sched);
+
+/* Converts npoints affine points once into the device-internal reduced-radix form (for
+ * bases reused across many MSMs, e.g. a KZG SRS); free with zkg_free. */
+int zkg_msm_prepare_points(int curve, int npoints, const uint64_t *d_grps, void **d_prepared);
 
 /* NTT on device buffers (flags: 0 forward, 1 inverse); eta NULL = no coset shift.
  * gen/eta are host pointers. */
@@ -153,6 +158,23 @@ int zkg_fill_random(int curve, int field, size_t n, uint64_t seed, uint64_t *d_o
 /* batch fixed-base scalar multiplication k_i * G of the standard generator, affine Montgomery
  * output (input generation for tests and benchmarks) */
 int zkg_g1_scalar_mul_gen(int curve, int n, const uint64_t *d_scalars, uint64_t *d_out_affine);
+
+/* [tau^i G]_{i<n} for tau in standard form (examples/KZG.hs:59-62), affine Montgomery output */
+int zkg_srs_generate(int curve, int n, const uint64_t *tau_std, uint64_t *d_out_affine);
+
+/* Serving loop for KZG commitments (examples/KZG.hs:91-97: iNTT of the evaluations on the
+ * size-2^logn domain, then an MSM of the Montgomery coefficients against the SRS prefix).
+ * nworkers host threads, each with its own stream, admit jobs FIFO by device footprint
+ * against hbm_budget bytes.  values/out stay owned by the caller until the job completes. */
+int zkg_serve_create(int curve, const void *d_srs_prepared, int srs_n, size_t hbm_budget, int nworkers, void **server);
+int zkg_serve_submit(void *server, int logn, const uint64_t *values, uint64_t *out_affine, long long *job_id);
+int zkg_serve_drain(void *server);
+int zkg_serve_job_times(void *server, long long id, double *queue_ms, double *run_ms, double *latency_ms);
+int zkg_serve_stats(void *server, size_t *peak_bytes, int *max_concurrent);
+/* submits job k at arrival_s[k] seconds after the call starts, waits for all of them */
+int zkg_serve_run_trace(void *server, int njobs, const int *logn, const double *arrival_s,
+                        const uint64_t *const *values, uint64_t *const *outs, double *latency_ms, double *wall_s);
+int zkg_serve_destroy(void *server);
 
 /* microbenchmarks: 0 = v_mad_u64_u32 per second, 1 = HBM copy bytes per second,
  * 2..5 = Montgomery multiplications per second (BN254 Fp, BN254 Fr, BLS Fp, BLS Fr) */

@@ -116,8 +116,8 @@ int msm_host(int curve, int npoints, const uint64_t *expos, const uint64_t *grps
   ZK_TRY(copy_h2d(ds.p, expos, sb, ctx->stream));
   ZK_TRY(copy_h2d(dp.p, grps, pb, ctx->stream));
   MsmResult r;
-  ZK_TRY(msm_device(curve, npoints, ds.as<uint64_t>(), enl, flags & ZKG_MSM_MONT, dp.as<uint64_t>(), nullptr,
-                    ctx->stream, &r));
+  ZK_TRY(msm_device(curve, npoints, ds.as<uint64_t>(), enl, flags & ZKG_MSM_MONT, dp.as<uint64_t>(), false,
+                    nullptr, ctx->stream, &r));
   return msm_write_result(curve, r, tgt, flags);
 }
 
@@ -237,8 +237,17 @@ int zkg_msm(int curve, int npoints, const uint64_t *d_expos, int expo_nlimbs, co
   ZK_TRY(with_ctx(&ctx));
   std::lock_guard<std::recursive_mutex> lk(ctx->lock);
   MsmResult r;
-  ZK_TRY(msm_device(curve, npoints, d_expos, expo_nlimbs, flags & ZKG_MSM_MONT, d_grps, sched, ctx->stream, &r));
+  ZK_TRY(msm_device(curve, npoints, d_expos, expo_nlimbs, flags & ZKG_MSM_MONT, d_grps,
+                    (flags & ZKG_MSM_POINTS_INTERNAL) != 0, sched, ctx->stream, &r));
   return msm_write_result(curve, r, tgt_host, flags);
+}
+
+int zkg_msm_prepare_points(int curve, int npoints, const uint64_t *d_grps, void **d_prepared) {
+  Context *ctx;
+  ZK_TRY(with_ctx(&ctx));
+  std::lock_guard<std::recursive_mutex> lk(ctx->lock);
+  ZK_TRY(dev_alloc(d_prepared, (size_t)(npoints > 0 ? npoints : 1) * msm_internal_point_bytes(curve)));
+  return msm_prepare_points(curve, npoints, d_grps, *d_prepared, ctx->stream);
 }
 
 int zkg_ntt(int curve, int m, const uint64_t *gen, const uint64_t *eta, int flags, const uint64_t *d_src,
@@ -308,6 +317,38 @@ int zkg_g1_scalar_mul_gen(int curve, int n, const uint64_t *d_scalars, uint64_t 
 }
 
 int zkg_microbench(int which, double *result) { return microbench(which, result); }
+
+int zkg_srs_generate(int curve, int n, const uint64_t *tau_std, uint64_t *d_out_affine) {
+  Context *ctx;
+  ZK_TRY(with_ctx(&ctx));
+  std::lock_guard<std::recursive_mutex> lk(ctx->lock);
+  return srs_generate(curve, n, tau_std, d_out_affine, ctx->stream);
+}
+
+int zkg_serve_create(int curve, const void *d_srs_prepared, int srs_n, size_t hbm_budget, int nworkers, void **server) {
+  Server *s = nullptr;
+  ZK_TRY(serve_create(curve, d_srs_prepared, srs_n, hbm_budget, nworkers, &s));
+  *server = s;
+  return ZK_OK;
+}
+int zkg_serve_submit(void *server, int logn, const uint64_t *values, uint64_t *out_affine, long long *job_id) {
+  return serve_submit((Server *)server, logn, values, out_affine, job_id);
+}
+int zkg_serve_drain(void *server) { return serve_drain((Server *)server); }
+int zkg_serve_job_times(void *server, long long id, double *queue_ms, double *run_ms, double *latency_ms) {
+  return serve_job_times((Server *)server, id, queue_ms, run_ms, latency_ms);
+}
+int zkg_serve_stats(void *server, size_t *peak_bytes, int *max_concurrent) {
+  return serve_stats((Server *)server, peak_bytes, max_concurrent);
+}
+int zkg_serve_run_trace(void *server, int njobs, const int *logn, const double *arrival_s,
+                        const uint64_t *const *values, uint64_t *const *outs, double *latency_ms, double *wall_s) {
+  return serve_run_trace((Server *)server, njobs, logn, arrival_s, values, outs, latency_ms, wall_s);
+}
+int zkg_serve_destroy(void *server) {
+  serve_destroy((Server *)server);
+  return ZK_OK;
+}
 
 int zkg_last_timing(double *ms_total, double *ms_phases, int max_phases, int *nphases) {
   return msm_last_timing(ms_total, ms_phases, max_phases, nphases);

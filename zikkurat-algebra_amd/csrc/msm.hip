@@ -25,6 +25,7 @@
 #include <string.h>
 
 #include <chrono>
+#include <mutex>
 #include <vector>
 
 #include "curve.hpp"
@@ -587,9 +588,10 @@ int msm_write_result(int curve, const MsmResult &r, uint64_t *tgt, int flags) {
 }
 
 // per-phase timing of the last MSM (HIP events on the library stream + host wall clock)
-static double g_ms_total = 0, g_ms_phase[8] = {0};
-static int g_nphase = 0;
-static long long g_stats[4] = {0, 0, 0, 0};  // window, windows, sorted entries, heavy buckets
+// per calling thread (several host threads may each drive their own stream)
+static thread_local double g_ms_total = 0, g_ms_phase[8] = {0};
+static thread_local int g_nphase = 0;
+static thread_local long long g_stats[4] = {0, 0, 0, 0};  // window, windows, sorted entries, heavy buckets
 
 int msm_last_stats(long long *out4) {
   for (int i = 0; i < 4; i++) out4[i] = g_stats[i];
@@ -620,6 +622,16 @@ struct Events {
   }
 };
 
+static int msm_set_lds_attributes() {
+  ZK_HIP(hipFuncSetAttribute((const void *)msm_chunk_hist_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             128 * 1024));
+  ZK_HIP(hipFuncSetAttribute((const void *)msm_chunk_scatter_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             128 * 1024));
+  ZK_HIP(hipFuncSetAttribute((const void *)msm_chunk_scatter_lds_kernel,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024));
+  return ZK_OK;
+}
+
 static MsmSchedule resolve_schedule(int curve, int n, int enl, const zkg_msm_schedule *user) {
   MsmSchedule s;
   const int fr_bits = curve == 0 ? 254 : 255;
@@ -635,7 +647,7 @@ static MsmSchedule resolve_schedule(int curve, int n, int enl, const zkg_msm_sch
 
 template <class C>
 static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const uint64_t *d_points,
-                   const zkg_msm_schedule *user, hipStream_t st, MsmResult *out) {
+                   bool points_internal, const zkg_msm_schedule *user, hipStream_t st, MsmResult *out) {
   using F = typename C::Fp;
   constexpr int PW = 4 * F::N;               // u32 words per external XYZZ point (host side)
   constexpr int ZW = ZLayout<C>::XYZZ;       // words per internal XYZZ point (device buffers)
@@ -659,7 +671,7 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   ZK_TRY(off.alloc((nkeys + 1) * 4));
   ZK_TRY(bsum.alloc(nkeys * ZW * 4));
   ZK_TRY(cont.alloc(nseg * ZW * 4));
-  ZK_TRY(zpts.alloc((uint64_t)n * AW * 4));
+  if (!points_internal) ZK_TRY(zpts.alloc((uint64_t)n * AW * 4));
   const uint32_t heavy_cap = 1u << 16;
   ZK_TRY(heavy.alloc((heavy_cap + 2) * 4));
   ZK_TRY(pyrA.alloc(nkeys * ZW * 4));
@@ -675,16 +687,12 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
     // the chunk prefixes (u32) live in the pyramid buffer, idle until the reduction, when it is
     // large enough
     if ((uint64_t)W * nchunks * B * 4 > nkeys * ZW * 4) ZK_TRY(pre_big.alloc((uint64_t)W * nchunks * B * 4));
-    static bool attr_done = false;
-    if (!attr_done) {
-      ZK_HIP(hipFuncSetAttribute((const void *)msm_chunk_hist_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 128 * 1024));
-      ZK_HIP(hipFuncSetAttribute((const void *)msm_chunk_scatter_kernel,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024));
-      ZK_HIP(hipFuncSetAttribute((const void *)msm_chunk_scatter_lds_kernel,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024));
-      attr_done = true;
-    }
+    static std::once_flag attr_once;
+    int attr_status = ZK_OK;
+    std::call_once(attr_once, [&] {
+      attr_status = msm_set_lds_attributes();
+    });
+    ZK_TRY(attr_status);
   } else {
     ZK_TRY(cursor.alloc((nkeys + 1) * 4));
   }
@@ -697,8 +705,10 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   ZK_HIP(hipMemsetAsync(heavy.p, 0, 8, st));
   const dim3 blk(kThreads);
   const dim3 gpts((n + kThreads - 1) / kThreads);
-  hipLaunchKernelGGL(msm_points_in_kernel<C>, gpts, blk, 0, st, (const uint32_t *)d_points, (uint32_t)n,
-                     zpts.as<int32_t>());
+  const int32_t *zp = points_internal ? (const int32_t *)d_points : zpts.as<int32_t>();
+  if (!points_internal)
+    hipLaunchKernelGGL(msm_points_in_kernel<C>, gpts, blk, 0, st, (const uint32_t *)d_points, (uint32_t)n,
+                       zpts.as<int32_t>());
   hipLaunchKernelGGL(msm_digits_kernel<C>, gpts, blk, 0, st, d_scalars, enl, mont, (uint32_t)n, c, W,
                      dig.as<uint32_t>());
   const dim3 gpw((n + kThreads - 1) / kThreads, W);
@@ -739,7 +749,7 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   ev.mark(st);
   hipLaunchKernelGGL(msm_accum_kernel<C>, dim3((unsigned)((nseg + kThreads - 1) / kThreads)), blk, 0, st,
                      vals.as<uint32_t>(), off.as<uint32_t>(), (uint32_t)nkeys, seg_log,
-                     zpts.as<int32_t>(), bsum.as<int32_t>(), cont.as<int32_t>());
+                     zp, bsum.as<int32_t>(), cont.as<int32_t>());
   ZK_HIP(hipGetLastError());
   ev.mark(st);
   const uint32_t limit = 16;
@@ -847,15 +857,85 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
 }
 
 int msm_device(int curve, int n, const uint64_t *d_scalars, int enl, int mont, const uint64_t *d_points,
+               bool points_internal,
                const zkg_msm_schedule *sched, hipStream_t st, MsmResult *out) {
   if (n <= 0) {
     msm_infinity(curve, out);
     return ZK_OK;
   }
-  if (curve == 0) return msm_run<Bn254G1>(n, d_scalars, enl, mont, d_points, sched, st, out);
-  if (curve == 1) return msm_run<Bls381G1>(n, d_scalars, enl, mont, d_points, sched, st, out);
+  if (curve == 0) return msm_run<Bn254G1>(n, d_scalars, enl, mont, d_points, points_internal, sched, st, out);
+  if (curve == 1) return msm_run<Bls381G1>(n, d_scalars, enl, mont, d_points, points_internal, sched, st, out);
   set_error("msm: bad curve id");
   return ZK_EINVAL;
+}
+
+template <class C>
+static int msm_prepare_points_t(int n, const uint64_t *d_ext, void *d_int, hipStream_t st) {
+  hipLaunchKernelGGL(msm_points_in_kernel<C>, dim3((n + kThreads - 1) / kThreads), dim3(kThreads), 0, st,
+                     (const uint32_t *)d_ext, (uint32_t)n, (int32_t *)d_int);
+  ZK_HIP(hipGetLastError());
+  ZK_HIP(hipStreamSynchronize(st));
+  return ZK_OK;
+}
+
+size_t msm_internal_point_bytes(int curve) {
+  return curve == 0 ? ZLayout<Bn254G1>::AFF * 4 : ZLayout<Bls381G1>::AFF * 4;
+}
+
+int msm_prepare_points(int curve, int n, const uint64_t *d_ext, void *d_int, hipStream_t st) {
+  if (n <= 0) return ZK_OK;
+  if (curve == 0) return msm_prepare_points_t<Bn254G1>(n, d_ext, d_int, st);
+  if (curve == 1) return msm_prepare_points_t<Bls381G1>(n, d_ext, d_int, st);
+  set_error("msm_prepare_points: bad curve id");
+  return ZK_EINVAL;
+}
+
+// device bytes one MSM of n points allocates (admission control of the serving loop)
+size_t msm_footprint(int curve, int n, int enl, bool points_internal, const zkg_msm_schedule *sched) {
+  const MsmSchedule s = resolve_schedule(curve, n, enl, sched);
+  const uint64_t W = s.windows, B = 1ull << (s.c - 1), nk = W * B, total = W * (uint64_t)n;
+  const uint64_t zw = curve == 0 ? ZLayout<Bn254G1>::XYZZ : ZLayout<Bls381G1>::XYZZ;
+  const uint64_t aw = curve == 0 ? ZLayout<Bn254G1>::AFF : ZLayout<Bls381G1>::AFF;
+  const uint64_t nchunks = (n + kChunk - 1) / kChunk;
+  uint64_t b = total * 8 + nk * 12 + 3 * nk * zw * 4 + ((total >> s.seg_log) + 1) * zw * 4 + W * nchunks * B * 6;
+  if (!points_internal) b += (uint64_t)n * aw * 4;
+  return (size_t)b;
+}
+
+// SRS scalars tau^i in standard form (KZG.hs:59-62 builds [tau^i G]_{i<n})
+template <class C>
+__global__ __launch_bounds__(kThreads) void srs_scalars_kernel(Fe<typename C::Fr> tau, uint32_t n,
+                                                              uint32_t *__restrict__ out) {
+  using FR = typename C::Fr;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Fe<FR> r;
+  fpow_u64(r, tau, i);
+  fto_std(r, r);
+  fstore(out + (size_t)i * FR::N, r);
+}
+
+int srs_generate(int curve, int n, const uint64_t *tau_std, uint64_t *d_out, hipStream_t st) {
+  if (n <= 0) return ZK_OK;
+  const zkh::FieldDesc &fr = curve == 0 ? zkh::BN254_FR : zkh::BLS381_FR;
+  const zkh::Fe t = zkh::from_std(fr, zkh::load(fr, tau_std));
+  DevBuf sc;
+  ZK_TRY(sc.alloc((size_t)n * 32));
+  const dim3 g((n + kThreads - 1) / kThreads), b(kThreads);
+  if (curve == 0) {
+    Fe<Bn254Fr> td;
+    for (int j = 0; j < 8; j++) td.w[j] = (uint32_t)(t.w[j / 2] >> (32 * (j & 1)));
+    hipLaunchKernelGGL(srs_scalars_kernel<Bn254G1>, g, b, 0, st, td, (uint32_t)n, sc.as<uint32_t>());
+  } else if (curve == 1) {
+    Fe<Bls381Fr> td;
+    for (int j = 0; j < 8; j++) td.w[j] = (uint32_t)(t.w[j / 2] >> (32 * (j & 1)));
+    hipLaunchKernelGGL(srs_scalars_kernel<Bls381G1>, g, b, 0, st, td, (uint32_t)n, sc.as<uint32_t>());
+  } else {
+    set_error("srs_generate: bad curve id");
+    return ZK_EINVAL;
+  }
+  ZK_HIP(hipGetLastError());
+  return g1_scalar_mul_gen(curve, n, sc.as<uint64_t>(), d_out, st);
 }
 
 int g1_scalar_mul_gen(int curve, int n, const uint64_t *d_scalars, uint64_t *d_out, hipStream_t st) {

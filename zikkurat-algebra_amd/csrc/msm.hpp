@@ -28,10 +28,25 @@ int msm_write_result(int curve, const MsmResult &r, uint64_t *tgt, int flags);
 
 // Device MSM; d_scalars: n * enl u64 (standard form, or Montgomery Fr when mont != 0),
 // d_points: n affine Montgomery points.  Synchronous; the result is left in *out.
+// points_internal: d_points already holds the reduced-radix form written by msm_prepare_points
 int msm_device(int curve, int n, const uint64_t *d_scalars, int enl, int mont, const uint64_t *d_points,
-               const zkg_msm_schedule *sched, hipStream_t st, MsmResult *out);
+               bool points_internal, const zkg_msm_schedule *sched, hipStream_t st, MsmResult *out);
+int msm_prepare_points(int curve, int n, const uint64_t *d_ext, void *d_int, hipStream_t st);
+size_t msm_internal_point_bytes(int curve);
+size_t msm_footprint(int curve, int n, int enl, bool points_internal, const zkg_msm_schedule *sched);
 
 int g1_scalar_mul_gen(int curve, int n, const uint64_t *d_scalars, uint64_t *d_out, hipStream_t st);
+int srs_generate(int curve, int n, const uint64_t *tau_std, uint64_t *d_out, hipStream_t st);
+
+struct Server;
+int serve_create(int curve, const void *d_srs_internal, int srs_n, size_t budget, int nworkers, Server **out);
+int serve_submit(Server *s, int logn, const uint64_t *values, uint64_t *out, long long *id);
+int serve_drain(Server *s);
+int serve_job_times(Server *s, long long id, double *queue_ms, double *run_ms, double *latency_ms);
+int serve_stats(Server *s, size_t *peak_bytes, int *max_concurrent);
+int serve_run_trace(Server *s, int njobs, const int *logn, const double *arrival_s, const uint64_t *const *values,
+                    uint64_t *const *outs, double *latency_ms, double *wall_s);
+void serve_destroy(Server *s);
 
 int msm_last_timing(double *ms_total, double *ms_phases, int max_phases, int *nphases);
 int msm_last_stats(long long *out4);

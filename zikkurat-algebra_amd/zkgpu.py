@@ -30,7 +30,7 @@ PREFIX = {BN254: "bn128", BLS12_381: "bls12_381"}
 FP_LIMBS = {BN254: 4, BLS12_381: 6}
 FR_LIMBS = 4
 
-MSM_MONT, MSM_PROJ_OUT, MSM_JAC_OUT = 1, 2, 4
+MSM_MONT, MSM_PROJ_OUT, MSM_JAC_OUT, MSM_POINTS_INTERNAL = 1, 2, 4, 8
 NTT_FORWARD, NTT_INVERSE = 0, 1
 
 u64p = ctypes.POINTER(ctypes.c_uint64)
@@ -69,6 +69,17 @@ def lib():
         L.zkg_ntt_batch_run.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
         L.zkg_ntt_batch_destroy.argtypes = [vp]
         L.zkg_fill_random.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_size_t, ctypes.c_uint64, vp]
+        L.zkg_srs_generate.argtypes = [ctypes.c_int, ctypes.c_int, u64p, vp]
+        L.zkg_msm_prepare_points.argtypes = [ctypes.c_int, ctypes.c_int, vp, ctypes.POINTER(vp)]
+        L.zkg_serve_create.argtypes = [ctypes.c_int, vp, ctypes.c_int, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(vp)]
+        L.zkg_serve_submit.argtypes = [vp, ctypes.c_int, u64p, u64p, ctypes.POINTER(ctypes.c_longlong)]
+        L.zkg_serve_drain.argtypes = [vp]
+        L.zkg_serve_job_times.argtypes = [vp, ctypes.c_longlong] + [ctypes.POINTER(ctypes.c_double)] * 3
+        L.zkg_serve_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_int)]
+        L.zkg_serve_run_trace.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(u64p), ctypes.POINTER(u64p), ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(ctypes.c_double)]
+        L.zkg_serve_destroy.argtypes = [vp]
         L.zkg_device_info.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_size_t)]
         _lib = L
     return _lib
@@ -234,6 +245,9 @@ def fft_generator(curve, m):
 # extension API (device buffers)
 
 def msm_device(curve, n, d_scalars, d_points, expo_nlimbs=4, flags=0, window=0, seg_log=0):
+    """d_points: DeviceArray of affine points, or PreparedPoints (flag added automatically)."""
+    if isinstance(d_points, PreparedPoints):
+        flags |= MSM_POINTS_INTERNAL
     size = 3 * FP_LIMBS[curve] if flags & (MSM_PROJ_OUT | MSM_JAC_OUT) else 2 * FP_LIMBS[curve]
     res = np.zeros(size, np.uint64)
     sched = None
@@ -276,6 +290,89 @@ class NttBatch:
         if self.plan:
             lib().zkg_ntt_batch_destroy(self.plan)
             self.plan = vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def srs_generate(curve, n, tau):
+    """[tau^i G]_{i<n}, affine Montgomery, on the device (tau: standard-form integer)."""
+    t = np.array([(tau >> (64 * k)) & 0xFFFFFFFFFFFFFFFF for k in range(4)], dtype=np.uint64)
+    out = DeviceArray(n * 2 * FP_LIMBS[curve])
+    _check(lib().zkg_srs_generate(curve, n, _p(t), out.ptr))
+    return out
+
+
+class PreparedPoints:
+    """Points converted once into the device-internal form (zkg_msm_prepare_points)."""
+
+    def __init__(self, curve, d_points, n):
+        self.n = n
+        self.ptr = vp()
+        _check(lib().zkg_msm_prepare_points(curve, n, d_points.ptr, ctypes.byref(self.ptr)))
+
+    def free(self):
+        if self.ptr:
+            lib().zkg_free(self.ptr)
+            self.ptr = vp()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Server:
+    """KZG-commit serving loop (zkg_serve_*): iNTT then MSM against a resident SRS."""
+
+    def __init__(self, curve, srs: PreparedPoints, hbm_budget, nworkers):
+        self.curve = curve
+        self.h = vp()
+        self._keep = []
+        _check(lib().zkg_serve_create(curve, srs.ptr, srs.n, int(hbm_budget), nworkers, ctypes.byref(self.h)))
+
+    def submit(self, logn, values):
+        values = _c(values)
+        out = np.zeros(2 * FP_LIMBS[self.curve], np.uint64)
+        jid = ctypes.c_longlong()
+        self._keep.append((values, out))
+        _check(lib().zkg_serve_submit(self.h, logn, _p(values), _p(out), ctypes.byref(jid)))
+        return jid.value, out
+
+    def drain(self):
+        _check(lib().zkg_serve_drain(self.h))
+
+    def times(self, jid):
+        q, r, l = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _check(lib().zkg_serve_job_times(self.h, jid, ctypes.byref(q), ctypes.byref(r), ctypes.byref(l)))
+        return q.value, r.value, l.value
+
+    def stats(self):
+        peak, conc = ctypes.c_size_t(), ctypes.c_int()
+        _check(lib().zkg_serve_stats(self.h, ctypes.byref(peak), ctypes.byref(conc)))
+        return peak.value, conc.value
+
+    def run_trace(self, logns, arrivals, values):
+        n = len(logns)
+        vals = [_c(v) for v in values]
+        outs = [np.zeros(2 * FP_LIMBS[self.curve], np.uint64) for _ in range(n)]
+        L = (ctypes.c_int * n)(*logns)
+        A = (ctypes.c_double * n)(*arrivals)
+        V = (u64p * n)(*[_p(v) for v in vals])
+        O = (u64p * n)(*[_p(o) for o in outs])
+        lat = (ctypes.c_double * n)()
+        wall = ctypes.c_double()
+        _check(lib().zkg_serve_run_trace(self.h, n, L, A, V, O, lat, ctypes.byref(wall)))
+        return outs, list(lat), wall.value
+
+    def close(self):
+        if self.h:
+            lib().zkg_serve_destroy(self.h)
+            self.h = vp()
 
     def __del__(self):
         try:
