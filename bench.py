@@ -234,6 +234,63 @@ def bench_ntt_batch(njobs=256, seed=4):
             "ntt_batched_vs_plain": round(t_plain / t_batch, 3), "ntt_batch_bit_identical_to_plain": same}
 
 
+def bench_learned_msm(njobs=24, seed=3, nstreams=4):
+    """The uneven MSM mix: BN254 and BLS12-381 jobs, log2 n uniform in [14, 22] (seeded), scalars
+    uniform in [0, r), uniform 64-bit, or sparse (80% in {0, 1}).  Plain = one MSM at a time with
+    the analytic window on the declared width; learned = scalar statistics + the committed cost
+    model's schedule, the jobs spread over concurrent streams.  Outputs must agree bit for bit."""
+    import costmodel
+    stem = costmodel.DEFAULT_MODEL
+    if not Path(str(stem) + ".json").exists():
+        return {"learned_msm": "no trained cost model under zikkurat-algebra_amd/tuned"}
+    net, man = costmodel.load(stem)
+    jobs = costmodel.mix_jobs([0, 1], njobs, seed)
+    pts = costmodel.PointSets(max(n for _, n, _, _ in jobs))
+    dS = []
+    for c, n, d, sd in jobs:
+        dS.append(zkgpu.DeviceArray.from_host(costmodel.scalars(c, n, d, np.random.default_rng(sd))))
+    curves = [c for c, _, _, _ in jobs]
+    ns = [n for _, n, _, _ in jobs]
+    dP = [pts.d[c] for c in curves]
+
+    def plain():
+        return [zkgpu.msm_device(c, n, s, p) for c, n, s, p in zip(curves, ns, dS, dP)]
+
+    def timed(fn, reps=3):
+        fn()
+        best, out = None, None
+        for _ in range(reps):
+            zkgpu.synchronize()
+            t0 = time.perf_counter()
+            out = fn()
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        return best, out
+
+    t_plain, ref = timed(plain)
+    net.install()
+    t_learned_seq, out_seq = timed(lambda: [zkgpu.msm_device(c, n, s, p, policy=zkgpu.SCHED_LEARNED)
+                                            for c, n, s, p in zip(curves, ns, dS, dP)])
+    t_plain_batch, out_pb = timed(lambda: zkgpu.msm_batch(curves, ns, dS, dP, policy=zkgpu.SCHED_PLAIN,
+                                                          nstreams=nstreams)[0])
+    t_learned_batch, out_lb = timed(lambda: zkgpu.msm_batch(curves, ns, dS, dP, policy=zkgpu.SCHED_LEARNED,
+                                                            nstreams=nstreams)[0])
+    zkgpu.Mlp.uninstall()
+    same = all((a == b).all() for o in (out_seq, out_pb, out_lb) for a, b in zip(o, ref))
+    pts_total = sum(ns)
+    return {"learned_msm_jobs": njobs, "learned_msm_points": pts_total,
+            "learned_msm_mix": "bn254+bls12-381, 2^14..2^22 log-uniform, uniform/u64/sparse scalars",
+            "learned_msm_plain_ms": round(t_plain * 1e3, 3),
+            "learned_msm_learned_seq_ms": round(t_learned_seq * 1e3, 3),
+            "learned_msm_plain_batched_ms": round(t_plain_batch * 1e3, 3),
+            "learned_msm_learned_batched_ms": round(t_learned_batch * 1e3, 3),
+            "learned_msm_batched_learned_vs_plain": round(t_plain / t_learned_batch, 3),
+            "learned_msm_plain_mpts": round(pts_total / t_plain / 1e6, 1),
+            "learned_msm_learned_batched_mpts": round(pts_total / t_learned_batch / 1e6, 1),
+            "learned_msm_streams": nstreams, "learned_msm_model": Path(str(stem)).name,
+            "learned_msm_bit_identical": bool(same)}
+
+
 def bench_serve(njobs=64, qps_list=(100, 400), seed=5):
     """KZG commits on BLS12-381: iNTT of n evaluations (n = 2^14..2^21, log-uniform) then an MSM
     against a resident SRS of 2^21 points tau^i G; Poisson arrivals at each stated QPS; served
@@ -349,6 +406,8 @@ def main():
         del dX, dY, dZ
         extra.update(bench_ntt_batch())
         log(f"ntt batch: {extra['ntt_batch_ms']} ms batched vs {extra['ntt_plain_ms']} ms plain, identical {extra['ntt_batch_bit_identical_to_plain']}")
+        extra.update(bench_learned_msm())
+        log(f"learned msm mix: {[(k, v) for k, v in extra.items() if k.startswith('learned_msm')]}")
         extra.update(bench_serve())
         log(f"serve: {[(k, v) for k, v in extra.items() if k.startswith('serve_qps')]}")
 

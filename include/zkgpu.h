@@ -102,12 +102,29 @@ int zkg_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int zkg_synchronize(void);
 
 /* MSM schedule: window width c (1..20), and the split of the bucket-accumulation work.
- * A zero field means "choose" (cost model, then heuristic). */
+ * A zero window / seg_log means "choose" by the policy.  Under ZKG_SCHED_STATS and
+ * ZKG_SCHED_LEARNED a device pass first measures the scalars (largest bit length, count of
+ * zeros and ones); the number of windows then covers only the largest scalar's bits, also
+ * when window and seg_log are given.  None of this changes a result bit. */
+#define ZKG_SCHED_PLAIN 0    /* analytic Pippenger heuristic from n and the declared width */
+#define ZKG_SCHED_STATS 1    /* statistics pass, analytic heuristic on the measured scalars */
+#define ZKG_SCHED_LEARNED 2  /* statistics pass, loaded cost model (else as ZKG_SCHED_STATS) */
 typedef struct {
   int window;        /* c: signed-digit window width */
   int seg_log;       /* log2 of sorted entries per accumulation thread */
-  int reserved[6];
+  int policy;        /* ZKG_SCHED_* */
+  int fix_limit;     /* buckets spanning more segments are summed by a pairwise tree (0 = 16) */
+  int reserved[4];
 } zkg_msm_schedule;
+
+/* Statistics of a scalar vector (standard form after conversion). */
+typedef struct {
+  long long n;       /* number of scalars */
+  long long n_zero;  /* scalars equal to 0 */
+  long long n_one;   /* scalars equal to 1 */
+  int max_bits;      /* bit length of the largest scalar */
+  int reserved;
+} zkg_scalar_stats;
 
 /* MSM on device buffers.  flags: bit0 = scalars in Montgomery form, bit1 = output
  * projective (X:Y:Z, normalised) instead of affine, bit2 = output Jacobian.  tgt_host
@@ -175,6 +192,46 @@ int zkg_serve_stats(void *server, size_t *peak_bytes, int *max_concurrent);
 int zkg_serve_run_trace(void *server, int njobs, const int *logn, const double *arrival_s,
                         const uint64_t *const *values, uint64_t *const *outs, double *latency_ms, double *wall_s);
 int zkg_serve_destroy(void *server);
+
+/* ---- schedule selection: statistics, autotuner, cost model, batched MSMs ---- */
+
+/* scalar statistics on the device (flags: ZKG_MSM_MONT when the scalars are Montgomery Fr) */
+int zkg_msm_scalar_stats(int curve, int npoints, const uint64_t *d_expos, int expo_nlimbs, int flags,
+                         zkg_scalar_stats *out);
+/* cost-model input row for one (job, candidate schedule); ZKG_MSM_NFEAT floats */
+#define ZKG_MSM_NFEAT 11
+int zkg_msm_features(int curve, const zkg_scalar_stats *st, int window, int seg_log, float *out);
+/* candidate schedules searched by the autotuner and the cost model; returns the count */
+int zkg_msm_candidates(int curve, const zkg_scalar_stats *st, int max, int *windows, int *seg_logs);
+/* the schedule a policy picks for a job (window, seg_log filled in) */
+int zkg_msm_pick(int curve, const zkg_scalar_stats *st, int policy, zkg_msm_schedule *out);
+/* Exhaustive autotuning: runs the MSM under each candidate (statistics-trimmed windows),
+ * one warm-up then reps timed runs, ms_out[k] = median wall-clock ms of candidate k.
+ * *identical = 1 when every candidate produced the same point. */
+int zkg_msm_tune(int curve, int npoints, const uint64_t *d_expos, int expo_nlimbs, const uint64_t *d_grps, int flags,
+                 int ncand, const int *windows, const int *seg_logs, int reps, double *ms_out, int *identical);
+/* Many MSMs of uneven sizes (either curve, curves[k]) on nstreams concurrent streams, largest
+ * first; each job's schedule follows sched->policy (window/seg_log ignored).  outs[k] host,
+ * flags as zkg_msm.  Results are those of one zkg_msm call per job. */
+int zkg_msm_batch(int njobs, const int *curves, const int *npoints, const uint64_t *const *d_expos, int expo_nlimbs,
+                  const uint64_t *const *d_grps, uint64_t *const *outs, int flags, const zkg_msm_schedule *sched,
+                  int nstreams, double *wall_ms);
+
+/* Dense fp32 MLP (ReLU hidden layers, linear output, MSE loss, Adam) run by the library's
+ * own HIP kernels: the schedule cost model.  dims has nlayers+1 entries; parameters are
+ * laid out W_0 (dims[1] x dims[0], row-major), b_0, W_1, b_1, ... */
+int zkg_mlp_create(int nlayers, const int *dims, unsigned seed, void **mlp);
+int zkg_mlp_destroy(void *mlp);
+long long zkg_mlp_nparams(void *mlp);
+int zkg_mlp_get_params(void *mlp, float *host);
+int zkg_mlp_set_params(void *mlp, const float *host);
+int zkg_mlp_forward(void *mlp, int rows, const float *x_host, float *y_host);
+int zkg_mlp_gradients(void *mlp, int rows, const float *x_host, const float *y_host, float *grad_host, float *loss);
+int zkg_mlp_train(void *mlp, int rows, const float *x_host, const float *y_host, int epochs, float lr,
+                  float *final_loss);
+/* installs an MLP (dims[0] == ZKG_MSM_NFEAT, one output: predicted ln ms) as the cost model
+ * of ZKG_SCHED_LEARNED; NULL removes it.  The caller keeps ownership. */
+int zkg_msm_set_cost_model(void *mlp);
 
 /* microbenchmarks: 0 = v_mad_u64_u32 per second, 1 = HBM copy bytes per second,
  * 2..5 = Montgomery multiplications per second (BN254 Fp, BN254 Fr, BLS Fp, BLS Fr) */

@@ -24,8 +24,8 @@ ARCH = "gfx950"
 COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wno-unused-command-line-argument", f"-I{PKG.parent / 'include'}"]
-SOURCES = ["runtime.cpp", "hostfield.cpp", "abi.cpp", "ntt.hip", "msm.hip", "peak.hip"]
-EXTRA = ["batch.hip", "mlp.hip", "tune.cpp", "serve.cpp"]
+SOURCES = ["runtime.cpp", "hostfield.cpp", "abi.cpp", "ntt.hip", "msm.hip", "peak.hip", "mlp.hip", "tune.cpp",
+           "serve.cpp"]
 
 
 def compile_one(src: Path, verbose: bool) -> Path:
@@ -43,7 +43,7 @@ def compile_one(src: Path, verbose: bool) -> Path:
 
 def build(jobs: int = 8, verbose: bool = False) -> Path:
     OBJ.mkdir(exist_ok=True)
-    srcs = [CSRC / s for s in SOURCES + EXTRA if (CSRC / s).exists()]
+    srcs = [CSRC / s for s in SOURCES]
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(lambda s: compile_one(s, verbose), srcs))
     tmp = LIB.with_suffix(".so.tmp")

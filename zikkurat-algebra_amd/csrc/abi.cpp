@@ -9,6 +9,7 @@
 
 #include "../../include/zkgpu.h"
 #include "hostfield.hpp"
+#include "mlp.hpp"
 #include "msm.hpp"
 #include "ntt.hpp"
 #include "peak.hpp"
@@ -355,5 +356,115 @@ int zkg_last_timing(double *ms_total, double *ms_phases, int max_phases, int *np
 }
 
 int zkg_msm_last_stats(long long *out4) { return msm_last_stats(out4); }
+
+// ---------------- schedule selection ----------------
+static int check_msm_args(int curve, int npoints, int enl, int flags) {
+  if (curve != 0 && curve != 1) {
+    set_error("msm: bad curve id");
+    return ZK_EINVAL;
+  }
+  if (npoints < 0 || enl < 1 || enl > 16 || ((flags & ZKG_MSM_MONT) && enl != 4)) {
+    set_error("msm: bad size, or expo_nlimbs not 1..16 (4 for Montgomery scalars)");
+    return ZK_EINVAL;
+  }
+  return ZK_OK;
+}
+
+int zkg_msm_scalar_stats(int curve, int npoints, const uint64_t *d_expos, int expo_nlimbs, int flags,
+                         zkg_scalar_stats *out) {
+  ZK_TRY(check_msm_args(curve, npoints, expo_nlimbs, flags));
+  Context *ctx;
+  ZK_TRY(with_ctx(&ctx));
+  std::lock_guard<std::recursive_mutex> lk(ctx->lock);
+  return msm_scalar_stats(curve, npoints, d_expos, expo_nlimbs, flags & ZKG_MSM_MONT, ctx->stream, out);
+}
+
+int zkg_msm_features(int curve, const zkg_scalar_stats *st, int window, int seg_log, float *out) {
+  if ((curve != 0 && curve != 1) || !st || window < 1 || window > 20 || seg_log < 1 || seg_log > 12) {
+    set_error("msm_features: bad curve, window or seg_log");
+    return ZK_EINVAL;
+  }
+  msm_features(curve, *st, window, seg_log, out);
+  return ZK_OK;
+}
+
+int zkg_msm_candidates(int curve, const zkg_scalar_stats *st, int max, int *windows, int *seg_logs) {
+  if ((curve != 0 && curve != 1) || !st || max < 0) {
+    set_error("msm_candidates: bad arguments");
+    return -1;
+  }
+  return msm_candidates(curve, *st, max, windows, seg_logs);
+}
+
+int zkg_msm_pick(int curve, const zkg_scalar_stats *st, int policy, zkg_msm_schedule *out) {
+  if ((curve != 0 && curve != 1) || !st || !out) {
+    set_error("msm_pick: bad arguments");
+    return ZK_EINVAL;
+  }
+  zkg_msm_schedule u;
+  memset(&u, 0, sizeof u);
+  u.policy = policy;
+  MsmSchedule s;
+  ZK_TRY(msm_resolve(curve, (int)st->n, 4, &u, policy == ZKG_SCHED_PLAIN ? nullptr : st, &s));
+  memset(out, 0, sizeof *out);
+  out->window = s.c;
+  out->seg_log = s.seg_log;
+  out->policy = policy;
+  return ZK_OK;
+}
+
+int zkg_msm_tune(int curve, int npoints, const uint64_t *d_expos, int expo_nlimbs, const uint64_t *d_grps, int flags,
+                 int ncand, const int *windows, const int *seg_logs, int reps, double *ms_out, int *identical) {
+  ZK_TRY(check_msm_args(curve, npoints, expo_nlimbs, flags));
+  for (int k = 0; k < ncand; k++)
+    if (windows[k] < 1 || windows[k] > 20 || seg_logs[k] < 1 || seg_logs[k] > 12) {
+      set_error("msm_tune: candidate out of range");
+      return ZK_EINVAL;
+    }
+  Context *ctx;
+  ZK_TRY(with_ctx(&ctx));
+  std::lock_guard<std::recursive_mutex> lk(ctx->lock);
+  return msm_tune(curve, npoints, d_expos, expo_nlimbs, flags & ZKG_MSM_MONT, d_grps,
+                  (flags & ZKG_MSM_POINTS_INTERNAL) != 0, ncand, windows, seg_logs, reps, ms_out, identical,
+                  ctx->stream);
+}
+
+int zkg_msm_batch(int njobs, const int *curves, const int *npoints, const uint64_t *const *d_expos, int expo_nlimbs,
+                  const uint64_t *const *d_grps, uint64_t *const *outs, int flags, const zkg_msm_schedule *sched,
+                  int nstreams, double *wall_ms) {
+  for (int k = 0; k < njobs; k++) ZK_TRY(check_msm_args(curves[k], npoints[k], expo_nlimbs, flags));
+  Context *ctx;
+  ZK_TRY(with_ctx(&ctx));
+  return msm_batch(njobs, curves, npoints, d_expos, expo_nlimbs, flags & ZKG_MSM_MONT, d_grps,
+                   (flags & ZKG_MSM_POINTS_INTERNAL) != 0, outs, flags & (ZKG_MSM_PROJ_OUT | ZKG_MSM_JAC_OUT), sched,
+                   nstreams, wall_ms);
+}
+
+// ---------------- cost-model network ----------------
+int zkg_mlp_create(int nlayers, const int *dims, unsigned seed, void **mlp) {
+  Mlp *m = nullptr;
+  ZK_TRY(mlp_create(nlayers, dims, seed, &m));
+  *mlp = m;
+  return ZK_OK;
+}
+int zkg_mlp_destroy(void *mlp) {
+  cost_model_forget((Mlp *)mlp);
+  mlp_destroy((Mlp *)mlp);
+  return ZK_OK;
+}
+long long zkg_mlp_nparams(void *mlp) { return (long long)mlp_nparams((Mlp *)mlp); }
+int zkg_mlp_get_params(void *mlp, float *host) { return mlp_get_params((Mlp *)mlp, host); }
+int zkg_mlp_set_params(void *mlp, const float *host) { return mlp_set_params((Mlp *)mlp, host); }
+int zkg_mlp_forward(void *mlp, int rows, const float *x_host, float *y_host) {
+  return mlp_forward((Mlp *)mlp, rows, x_host, y_host);
+}
+int zkg_mlp_gradients(void *mlp, int rows, const float *x_host, const float *y_host, float *grad_host, float *loss) {
+  return mlp_gradients((Mlp *)mlp, rows, x_host, y_host, grad_host, loss);
+}
+int zkg_mlp_train(void *mlp, int rows, const float *x_host, const float *y_host, int epochs, float lr,
+                  float *final_loss) {
+  return mlp_train((Mlp *)mlp, rows, x_host, y_host, epochs, lr, final_loss);
+}
+int zkg_msm_set_cost_model(void *mlp) { return cost_model_set((Mlp *)mlp); }
 
 }  // extern "C"

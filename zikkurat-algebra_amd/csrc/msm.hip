@@ -7,7 +7,8 @@
 //
 // Device pipeline (one MSM, window c, B = 2^(c-1) buckets per window, W windows):
 //   1 digits   signed c-bit digits d in [-2^(c-1), 2^(c-1)] per (window, point); the carry
-//              makes W = ceil((64 enl + 1) / c) windows exact for any scalar
+//              makes W = ceil((bits + 1) / c) windows exact for scalars below 2^bits, with
+//              bits = 64 enl, or the measured largest bit length under the statistics policies
 //   2 count    per-(chunk, window) bucket histograms in LDS (packed 16-bit counters)
 //   3 scan     exclusive prefix sum -> bucket offsets into one sorted array of all windows
 //   4 scatter  counting-sort placement of (point index | sign) words, ranks from LDS atomics
@@ -20,8 +21,10 @@
 //              neighbours (P'_i = P_2i + P_2i+1) and keeps the odd children Q^(k) = P_2i+1,
 //              which are pairwise reduced alongside; then sum_b (b+1) B_b =
 //              Sum(B) + sum_k 2^(k-1) U_k with U_k the reduced Q^(k) (depth c-1 additions)
-//   8 host     the W*c window terms are combined by one Horner pass on the host
-//              (sum_w 2^(wc) (Sum_w + sum_k 2^(k-1) U_wk), ~255 doublings) and normalised.
+//   8 host     the window terms are combined by one Horner pass on the host
+//              (sum_w 2^(start_w) (Sum_w + sum_k 2^(k-1) U_wk), ~255 doublings) and normalised.
+// Window widths are balanced (c or c-1 bits each, see MsmSchedule), so every window spreads
+// its digits over most of its buckets.
 #include <string.h>
 
 #include <chrono>
@@ -64,7 +67,8 @@ __device__ __forceinline__ uint32_t encode_digit(uint32_t raw, int c, uint32_t &
 
 template <class C>
 __global__ __launch_bounds__(kThreads) void msm_digits_kernel(const uint64_t *__restrict__ sc, int enl, int mont,
-                                                             uint32_t n, int c, int W, uint32_t *__restrict__ dig) {
+                                                             uint32_t n, int c, int W, int wide,
+                                                             uint32_t *__restrict__ dig) {
   using Fr = typename C::Fr;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -81,16 +85,71 @@ __global__ __launch_bounds__(kThreads) void msm_digits_kernel(const uint64_t *__
 #pragma unroll
       for (int j = 0; j < 4; j++) s[j] = (uint64_t)x.w[2 * j] | ((uint64_t)x.w[2 * j + 1] << 32);
     }
+    int bit = 0;
     for (int w = 0; w < W; w++) {
-      const uint32_t raw = window_bits(s, 4, w * c, c) + carry;
-      dig[(size_t)w * n + i] = encode_digit(raw, c, carry);
+      const int cw = w < wide ? c : c - 1;
+      const uint32_t raw = window_bits(s, 4, bit, cw) + carry;
+      dig[(size_t)w * n + i] = encode_digit(raw, cw, carry);
+      bit += cw;
     }
   } else {
     const uint64_t *s = sc + (size_t)i * enl;
+    int bit = 0;
     for (int w = 0; w < W; w++) {
-      const uint32_t raw = window_bits(s, enl, w * c, c) + carry;
-      dig[(size_t)w * n + i] = encode_digit(raw, c, carry);
+      const int cw = w < wide ? c : c - 1;
+      const uint32_t raw = window_bits(s, enl, bit, cw) + carry;
+      dig[(size_t)w * n + i] = encode_digit(raw, cw, carry);
+      bit += cw;
     }
+  }
+}
+
+// scalar statistics: stats[0] = largest bit length, stats[1] = zeros, stats[2] = ones
+template <class C>
+__global__ __launch_bounds__(kThreads) void msm_stats_kernel(const uint64_t *__restrict__ sc, int enl, int mont,
+                                                            uint32_t n, uint32_t *__restrict__ stats) {
+  using Fr = typename C::Fr;
+  __shared__ uint32_t s_max, s_zero, s_one;
+  if (threadIdx.x == 0) {
+    s_max = 0;
+    s_zero = 0;
+    s_one = 0;
+  }
+  __syncthreads();
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  int bits = -1;
+  if (i < n) {
+    bits = 0;
+    if (mont && enl == 4) {
+      Fe<Fr> x;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const uint64_t v = sc[(size_t)i * 4 + j];
+        x.w[2 * j] = (uint32_t)v;
+        x.w[2 * j + 1] = (uint32_t)(v >> 32);
+      }
+      fto_std(x, x);
+#pragma unroll
+      for (int j = 0; j < 8; j++)
+        if (x.w[j]) bits = 32 * j + 32 - __clz(x.w[j]);
+    } else {
+      for (int j = 0; j < enl; j++) {
+        const uint64_t v = sc[(size_t)i * enl + j];
+        if (v) bits = 64 * j + 64 - __clzll(v);
+      }
+    }
+  }
+  const uint64_t zmask = __ballot(bits == 0), omask = __ballot(bits == 1);
+  if ((threadIdx.x & 63) == 0) {
+    if (zmask) atomicAdd(&s_zero, (uint32_t)__popcll(zmask));
+    if (omask) atomicAdd(&s_one, (uint32_t)__popcll(omask));
+  }
+  if (bits > 0) atomicMax(&s_max, (uint32_t)bits);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicMax(&stats[0], s_max);
+    if (s_zero) atomicAdd(&stats[1], s_zero);
+    if (s_one) atomicAdd(&stats[2], s_one);
   }
 }
 
@@ -362,6 +421,28 @@ __device__ __forceinline__ uint32_t bucket_of(const uint32_t *__restrict__ off, 
   return lo;
 }
 
+// Bucket holding sorted entry e when bucket `from` starts at or before e: the last b >= from
+// with off[b] <= e.  Usually from itself; runs of empty buckets (windows narrower than the
+// bucket space) are crossed by a galloping then binary search, not one bucket at a time.
+__device__ __forceinline__ uint32_t bucket_from(const uint32_t *__restrict__ off, uint32_t nkeys, uint32_t from,
+                                                uint32_t e) {
+  if (off[from + 1] > e) return from;
+  uint32_t lo = from + 1, step = 1, hi;
+  for (;;) {
+    hi = lo + step;
+    if (hi >= nkeys || off[hi] > e) break;
+    lo = hi;
+    step <<= 1;
+  }
+  if (hi > nkeys) hi = nkeys;  // off[nkeys] = total entries > e
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (off[mid] <= e) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
 template <class C>
 __global__ __launch_bounds__(kThreads) void msm_accum_kernel(const uint32_t *__restrict__ vals,
                                                             const uint32_t *__restrict__ off, uint32_t nkeys,
@@ -384,10 +465,8 @@ __global__ __launch_bounds__(kThreads) void msm_accum_kernel(const uint32_t *__r
       flush_run<C>(acc, cont_run, seg, cur, bsum, cont);
       cont_run = false;
       zxyzz_set_inf(acc);
-      do {
-        cur++;
-        next = off[cur + 1];
-      } while (next == e);
+      cur = bucket_from(off, nkeys, cur + 1, (uint32_t)e);
+      next = off[cur + 1];
     }
     const uint32_t v = vals[e];
     AffZ<C> a;
@@ -451,6 +530,40 @@ __global__ __launch_bounds__(kThreads) void msm_heavy_round_kernel(const uint32_
   zxyzz_load(q, cont + (base + i1) * PW);
   zxyzz_add(a, a, q);
   zxyzz_store(cont + (base + i0) * PW, a);
+}
+
+// segment -> bucket holding its first entry (the owner of a continuation partial)
+__global__ __launch_bounds__(kThreads) void msm_seg_owner_kernel(const uint32_t *__restrict__ off, uint32_t nkeys,
+                                                                int seg_log, uint32_t nseg,
+                                                                uint32_t *__restrict__ own) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nseg) return;
+  own[j] = bucket_of(off, nkeys, (uint32_t)((uint64_t)j << seg_log));
+}
+
+// The same pairwise round with one thread per segment: the work per round is bounded by the
+// segment count whatever the spread of partial counts over heavy buckets.
+template <class C>
+__global__ __launch_bounds__(kThreads) void msm_heavy_round_seg_kernel(const uint32_t *__restrict__ off,
+                                                                      const uint32_t *__restrict__ own,
+                                                                      uint32_t nseg, int seg_log, uint32_t limit,
+                                                                      int32_t *__restrict__ cont, int r) {
+  constexpr int PW = ZLayout<C>::XYZZ;
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nseg) return;
+  const uint32_t b = own[j];
+  const uint64_t fs = off[b] >> seg_log;
+  if (j <= fs) return;  // not a continuation partial
+  const uint64_t ls = (uint64_t)(off[b + 1] - 1) >> seg_log, K = ls - fs;
+  if (K <= limit) return;  // summed serially by the fix kernel
+  const uint64_t i = j - fs - 1;
+  if (i & ((2ull << r) - 1)) return;
+  if (i + (1ull << r) >= K) return;
+  XyzzZ<C> a, q;
+  zxyzz_load(a, cont + (uint64_t)j * PW);
+  zxyzz_load(q, cont + ((uint64_t)j + (1ull << r)) * PW);
+  zxyzz_add(a, a, q);
+  zxyzz_store(cont + (uint64_t)j * PW, a);
 }
 
 template <class C>
@@ -538,12 +651,16 @@ __global__ __launch_bounds__(kThreads) void scalar_mul_gen_kernel(const uint64_t
 
 static const zkh::FieldDesc &fp_desc(int curve) { return curve == 0 ? zkh::BN254_FP : zkh::BLS381_FP; }
 
+// windows up to this width are sorted in LDS; wider ones fall back to global atomics, which
+// are slower than any saving in buckets
+static constexpr int kMaxAnalyticWindow = 17;
+
 int msm_heuristic_window(int curve, long long n, int scalar_bits) {
   (void)curve;
   if (n <= 1) return 2;
   double best = 1e300;
   int bc = 2;
-  for (int c = 2; c <= 20; c++) {
+  for (int c = 2; c <= kMaxAnalyticWindow; c++) {
     const double W = (double)((scalar_bits + 1 + c - 1) / c);
     const double cost = W * (10.0 * (double)n + 28.0 * (double)(1ll << (c - 1)));
     if (cost < best) {
@@ -632,16 +749,91 @@ static int msm_set_lds_attributes() {
   return ZK_OK;
 }
 
-static MsmSchedule resolve_schedule(int curve, int n, int enl, const zkg_msm_schedule *user) {
+int msm_stats_window(int curve, const zkg_scalar_stats &st) {
+  (void)curve;
+  const double n = (double)st.n, nfull = (double)(st.n - st.n_zero - st.n_one), ones = (double)st.n_one;
+  double best = 1e300;
+  int bc = 2;
+  for (int c = 2; c <= kMaxAnalyticWindow; c++) {
+    int w, cm, wide;
+    msm_window_layout(st.max_bits, c, true, &w, &cm, &wide);
+    const double W = (double)w;
+    const double cost = 10.0 * (W * nfull + ones) + 3.0 * W * n + 28.0 * W * (double)(1ll << (cm - 1));
+    if (cost < best) {
+      best = cost;
+      bc = c;
+    }
+  }
+  return bc;
+}
+
+int msm_scalar_stats(int curve, int n, const uint64_t *d_scalars, int enl, int mont, hipStream_t st,
+                     zkg_scalar_stats *out) {
+  memset(out, 0, sizeof *out);
+  out->n = n;
+  if (n <= 0) return ZK_OK;
+  DevBuf d;
+  ZK_TRY(d.alloc(16));
+  ZK_HIP(hipMemsetAsync(d.p, 0, 16, st));
+  const dim3 g((n + kThreads - 1) / kThreads), b(kThreads);
+  if (curve == 0)
+    hipLaunchKernelGGL(msm_stats_kernel<Bn254G1>, g, b, 0, st, d_scalars, enl, mont, (uint32_t)n, d.as<uint32_t>());
+  else
+    hipLaunchKernelGGL(msm_stats_kernel<Bls381G1>, g, b, 0, st, d_scalars, enl, mont, (uint32_t)n, d.as<uint32_t>());
+  ZK_HIP(hipGetLastError());
+  uint32_t h[4];
+  ZK_HIP(hipMemcpyAsync(h, d.p, 16, hipMemcpyDeviceToHost, st));
+  ZK_HIP(hipStreamSynchronize(st));
+  out->max_bits = (int)h[0];
+  out->n_zero = h[1];
+  out->n_one = h[2];
+  return ZK_OK;
+}
+
+// Schedule of one MSM.  Without statistics the windows cover the declared scalar width
+// (64 * enl bits, plus one for the signed-digit carry); with them they cover max_bits + 1.
+int msm_resolve(int curve, int n, int enl, const zkg_msm_schedule *user, const zkg_scalar_stats *st,
+                MsmSchedule *out) {
   MsmSchedule s;
   const int fr_bits = curve == 0 ? 254 : 255;
   const int bits_for_cost = enl * 64 < fr_bits ? enl * 64 : fr_bits;
-  s.c = (user && user->window > 0) ? user->window : msm_heuristic_window(curve, n, bits_for_cost);
-  if (s.c < 1) s.c = 1;
-  if (s.c > 20) s.c = 20;
-  s.windows = (64 * enl + 1 + s.c - 1) / s.c;
-  s.seg_log = (user && user->seg_log > 0) ? user->seg_log : 5;
-  if (s.seg_log > 12) s.seg_log = 12;
+  const int policy = user ? user->policy : ZKG_SCHED_PLAIN;
+  int c = (user && user->window > 0) ? user->window : 0;
+  int seg = (user && user->seg_log > 0) ? user->seg_log : 0;
+  if (st && (c == 0 || seg == 0) && policy == ZKG_SCHED_LEARNED) {
+    int mc = 0, ms = 0;
+    if (cost_model_pick(curve, *st, &mc, &ms)) {
+      if (c == 0) c = mc;
+      if (seg == 0) seg = ms;
+    }
+  }
+  if (c == 0) c = st ? msm_stats_window(curve, *st) : msm_heuristic_window(curve, n, bits_for_cost);
+  if (seg == 0) seg = 5;
+  c = c < 1 ? 1 : (c > 20 ? 20 : c);
+  s.seg_log = seg < 1 ? 1 : (seg > 12 ? 12 : seg);
+  s.bits = st ? st->max_bits : 64 * enl;
+  msm_window_layout(s.bits, c, st != nullptr, &s.windows, &s.c, &s.wide);
+  *out = s;
+  return ZK_OK;
+}
+
+void msm_window_layout(int bits, int c_req, bool balanced, int *W, int *c, int *wide) {
+  const int total = bits + 1;
+  const int w = (total + c_req - 1) / c_req;
+  *W = w;
+  if (!balanced) {
+    *c = c_req;
+    *wide = w;
+    return;
+  }
+  const int cm = (total + w - 1) / w;
+  *c = cm;
+  *wide = total - w * (cm - 1);
+}
+
+static MsmSchedule resolve_schedule(int curve, int n, int enl, const zkg_msm_schedule *user) {
+  MsmSchedule s;
+  msm_resolve(curve, n, enl, user, nullptr, &s);
   return s;
 }
 
@@ -652,7 +844,11 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   constexpr int PW = 4 * F::N;               // u32 words per external XYZZ point (host side)
   constexpr int ZW = ZLayout<C>::XYZZ;       // words per internal XYZZ point (device buffers)
   constexpr int AW = ZLayout<C>::AFF;        // words per internal affine point
-  const MsmSchedule sch = resolve_schedule(C::id, n, enl, user);
+  zkg_scalar_stats stats;
+  const bool use_stats = user && user->policy != ZKG_SCHED_PLAIN;
+  if (use_stats) ZK_TRY(msm_scalar_stats(C::id, n, d_scalars, enl, mont, st, &stats));
+  MsmSchedule sch;
+  ZK_TRY(msm_resolve(C::id, n, enl, user, use_stats ? &stats : nullptr, &sch));
   const int c = sch.c, W = sch.windows, logB = c - 1;
   const uint32_t B = 1u << logB;
   const uint64_t nkeys = (uint64_t)W * B;
@@ -672,8 +868,9 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   ZK_TRY(bsum.alloc(nkeys * ZW * 4));
   ZK_TRY(cont.alloc(nseg * ZW * 4));
   if (!points_internal) ZK_TRY(zpts.alloc((uint64_t)n * AW * 4));
-  const uint32_t heavy_cap = 1u << 16;
-  ZK_TRY(heavy.alloc((heavy_cap + 2) * 4));
+  // every bucket may be heavy (small seg_log, few entries per segment)
+  const uint32_t heavy_cap = (uint32_t)nkeys;
+  ZK_TRY(heavy.alloc(((uint64_t)heavy_cap + 2) * 4));
   ZK_TRY(pyrA.alloc(nkeys * ZW * 4));
   ZK_TRY(pyrB.alloc(nkeys * ZW * 4));
   const uint64_t per_block = (uint64_t)kThreads * kScanPer;
@@ -709,7 +906,7 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   if (!points_internal)
     hipLaunchKernelGGL(msm_points_in_kernel<C>, gpts, blk, 0, st, (const uint32_t *)d_points, (uint32_t)n,
                        zpts.as<int32_t>());
-  hipLaunchKernelGGL(msm_digits_kernel<C>, gpts, blk, 0, st, d_scalars, enl, mont, (uint32_t)n, c, W,
+  hipLaunchKernelGGL(msm_digits_kernel<C>, gpts, blk, 0, st, d_scalars, enl, mont, (uint32_t)n, c, W, sch.wide,
                      dig.as<uint32_t>());
   const dim3 gpw((n + kThreads - 1) / kThreads, W);
   const dim3 gkeys((unsigned)((nkeys + kThreads - 1) / kThreads));
@@ -752,7 +949,7 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
                      zp, bsum.as<int32_t>(), cont.as<int32_t>());
   ZK_HIP(hipGetLastError());
   ev.mark(st);
-  const uint32_t limit = 16;
+  const uint32_t limit = (user && user->fix_limit > 0) ? (uint32_t)user->fix_limit : 16u;
   hipLaunchKernelGGL(msm_fix_kernel<C>, dim3((unsigned)((nkeys + kThreads - 1) / kThreads)), blk, 0, st,
                      off.as<uint32_t>(), (uint32_t)nkeys, seg_log, bsum.as<int32_t>(), cont.as<int32_t>(),
                      heavy.as<uint32_t>(), heavy_cap, limit);
@@ -771,11 +968,25 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   }
   if (hh[0]) {
     const uint32_t nh = hh[0], kmax = hh[1];
+    // segments holding sorted entries; every continuation partial lies below this bound
+    const uint32_t nseg_used = (uint32_t)(((uint64_t)nent + (1ull << seg_log) - 1) >> seg_log);
+    DevBuf own;
     for (int r = 0; (1ull << r) < kmax; r++) {
       const uint32_t pairs = (uint32_t)((kmax + (2ull << r) - 1) >> (r + 1));
       const uint64_t thr = (uint64_t)nh * pairs;
-      hipLaunchKernelGGL(msm_heavy_round_kernel<C>, dim3((unsigned)((thr + kThreads - 1) / kThreads)), blk, 0, st,
-                         off.as<uint32_t>(), heavy.as<uint32_t>(), seg_log, cont.as<int32_t>(), r, pairs);
+      if (thr <= nseg_used) {
+        hipLaunchKernelGGL(msm_heavy_round_kernel<C>, dim3((unsigned)((thr + kThreads - 1) / kThreads)), blk, 0,
+                           st, off.as<uint32_t>(), heavy.as<uint32_t>(), seg_log, cont.as<int32_t>(), r, pairs);
+      } else {
+        // a few buckets with very many partials: a bucket-by-pair grid would be nh * pairs
+        if (!own.p) {
+          ZK_TRY(own.alloc((uint64_t)nseg_used * 4));
+          hipLaunchKernelGGL(msm_seg_owner_kernel, dim3((nseg_used + kThreads - 1) / kThreads), blk, 0, st,
+                             off.as<uint32_t>(), (uint32_t)nkeys, seg_log, nseg_used, own.as<uint32_t>());
+        }
+        hipLaunchKernelGGL(msm_heavy_round_seg_kernel<C>, dim3((nseg_used + kThreads - 1) / kThreads), blk, 0, st,
+                           off.as<uint32_t>(), own.as<uint32_t>(), nseg_used, seg_log, limit, cont.as<int32_t>(), r);
+      }
     }
     hipLaunchKernelGGL(msm_heavy_final_kernel<C>, dim3((nh + kThreads - 1) / kThreads), blk, 0, st,
                        off.as<uint32_t>(), heavy.as<uint32_t>(), seg_log, bsum.as<int32_t>(), cont.as<int32_t>());
@@ -810,7 +1021,7 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   ev.mark(st);
   auto t_host1 = std::chrono::steady_clock::now();
 
-  // host Horner: sum over (w, j) of 2^(w c + max(j-1, 0)) * term
+  // host Horner: sum over (w, j) of 2^(start_w + max(j-1, 0)) * term
   const zkh::FieldDesc &f = fp_desc(C::id);
   auto get = [&](uint32_t a, uint32_t w) {
     const uint32_t *p = res.data() + ((size_t)a * W + w) * PW;
@@ -822,11 +1033,12 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
     }
     return q;
   };
-  const int emax = (W - 1) * c + (int)(A > 1 ? A - 2 : 0);
+  const int emax = sch.start(W - 1) + (int)(A > 1 ? A - 2 : 0);
   std::vector<std::vector<uint32_t>> at(emax + 1);  // encoded (a << 16) | w
   for (uint32_t w = 0; w < (uint32_t)W; w++) {
-    at[w * c].push_back((0u << 16) | w);
-    for (uint32_t a = 1; a < A; a++) at[w * c + (a - 1)].push_back((a << 16) | w);
+    const int s0 = sch.start((int)w);
+    at[s0].push_back((0u << 16) | w);
+    for (uint32_t a = 1; a < A; a++) at[s0 + (a - 1)].push_back((a << 16) | w);
   }
   zkh::Xyzz acc = zkh::xyzz_inf();
   for (int e = emax; e >= 0; e--) {
@@ -897,7 +1109,7 @@ size_t msm_footprint(int curve, int n, int enl, bool points_internal, const zkg_
   const uint64_t zw = curve == 0 ? ZLayout<Bn254G1>::XYZZ : ZLayout<Bls381G1>::XYZZ;
   const uint64_t aw = curve == 0 ? ZLayout<Bn254G1>::AFF : ZLayout<Bls381G1>::AFF;
   const uint64_t nchunks = (n + kChunk - 1) / kChunk;
-  uint64_t b = total * 8 + nk * 12 + 3 * nk * zw * 4 + ((total >> s.seg_log) + 1) * zw * 4 + W * nchunks * B * 6;
+  uint64_t b = total * 8 + nk * 16 + 3 * nk * zw * 4 + ((total >> s.seg_log) + 1) * zw * 4 + W * nchunks * B * 6;
   if (!points_internal) b += (uint64_t)n * aw * 4;
   return (size_t)b;
 }

@@ -13,15 +13,37 @@ struct MsmResult {
 };
 
 // Resolved schedule of one MSM.
+// Windows 0..wide-1 are c bits wide, the others c-1 bits.  With measured scalars (largest bit
+// length known) the widths are balanced: W = ceil((bits + 1) / c_requested), c = ceil((bits +
+// 1) / W), together exactly bits + 1, so no window is left with only a few bits (a few
+// buckets holding most of the entries).  With only the declared width known, every window is
+// c_requested wide and the top one holds the few high bits the scalars may have (usually only
+// the signed-digit carry).
 struct MsmSchedule {
-  int c = 0;          // window width
+  int c = 0;          // widest window; 2^(c-1) buckets per window
   int windows = 0;    // number of signed-digit windows
+  int wide = 0;       // windows of width c (the rest have c - 1)
   int seg_log = 5;    // log2 sorted entries per accumulation thread
+  int bits = 0;       // scalar bits the windows cover
+  int start(int w) const { return w < wide ? w * c : wide * c + (w - wide) * (c - 1); }
 };
+// the window layout for a requested width (balanced or uniform, see above)
+void msm_window_layout(int bits, int c_req, bool balanced, int *W, int *c, int *wide);
 
 // analytic window choice: minimises W * (10 n + 28 * 2^(c-1)) group-op cost (madd ~ 10
 // mulmods, bucket pyramid ~ 2 adds of 14 mulmods per bucket) over c in [1, 20]
 int msm_heuristic_window(int curve, long long n, int scalar_bits);
+// the same cost model on measured scalars: windows cover max_bits, zeros cost nothing, ones
+// land in window 0 only, and each digit costs a little sort work
+int msm_stats_window(int curve, const zkg_scalar_stats &st);
+int msm_scalar_stats(int curve, int n, const uint64_t *d_scalars, int enl, int mont, hipStream_t st,
+                     zkg_scalar_stats *out);
+// cost-model choice for a job (false when no model is installed)
+bool cost_model_pick(int curve, const zkg_scalar_stats &st, int *c, int *seg_log);
+void msm_features(int curve, const zkg_scalar_stats &st, int c, int seg_log, float *x);
+int msm_candidates(int curve, const zkg_scalar_stats &st, int max, int *cs, int *segs);
+int msm_resolve(int curve, int n, int enl, const zkg_msm_schedule *user, const zkg_scalar_stats *st,
+                MsmSchedule *out);
 
 void msm_infinity(int curve, MsmResult *r);
 int msm_write_result(int curve, const MsmResult &r, uint64_t *tgt, int flags);
@@ -47,6 +69,13 @@ int serve_stats(Server *s, size_t *peak_bytes, int *max_concurrent);
 int serve_run_trace(Server *s, int njobs, const int *logn, const double *arrival_s, const uint64_t *const *values,
                     uint64_t *const *outs, double *latency_ms, double *wall_s);
 void serve_destroy(Server *s);
+
+int msm_tune(int curve, int n, const uint64_t *d_scalars, int enl, int mont, const uint64_t *d_points,
+             bool points_internal, int ncand, const int *cs, const int *segs, int reps, double *ms_out,
+             int *identical, hipStream_t st);
+int msm_batch(int njobs, const int *curves, const int *ns, const uint64_t *const *d_scalars, int enl, int mont,
+              const uint64_t *const *d_points, bool points_internal, uint64_t *const *outs, int out_flags,
+              const zkg_msm_schedule *sched, int nstreams, double *wall_ms);
 
 int msm_last_timing(double *ms_total, double *ms_phases, int max_phases, int *nphases);
 int msm_last_stats(long long *out4);

@@ -41,6 +41,25 @@ class ZkgError(RuntimeError):
     pass
 
 
+SCHED_PLAIN, SCHED_STATS, SCHED_LEARNED = 0, 1, 2
+MSM_NFEAT = 11
+
+
+class ScalarStats(ctypes.Structure):
+    """zkg_scalar_stats: counts and the largest bit length of a scalar vector."""
+    _fields_ = [("n", ctypes.c_longlong), ("n_zero", ctypes.c_longlong), ("n_one", ctypes.c_longlong),
+                ("max_bits", ctypes.c_int), ("reserved", ctypes.c_int)]
+
+    def as_dict(self):
+        return dict(n=self.n, n_zero=self.n_zero, n_one=self.n_one, max_bits=self.max_bits)
+
+
+class MsmSchedule(ctypes.Structure):
+    """zkg_msm_schedule."""
+    _fields_ = [("window", ctypes.c_int), ("seg_log", ctypes.c_int), ("policy", ctypes.c_int),
+                ("fix_limit", ctypes.c_int), ("reserved", ctypes.c_int * 4)]
+
+
 _lib = None
 
 
@@ -81,6 +100,27 @@ def lib():
                                           ctypes.POINTER(ctypes.c_double)]
         L.zkg_serve_destroy.argtypes = [vp]
         L.zkg_device_info.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_size_t)]
+        ip, fp_, dp = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_double)
+        L.zkg_msm_scalar_stats.argtypes = [ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int,
+                                           ctypes.POINTER(ScalarStats)]
+        L.zkg_msm_features.argtypes = [ctypes.c_int, ctypes.POINTER(ScalarStats), ctypes.c_int, ctypes.c_int, fp_]
+        L.zkg_msm_candidates.argtypes = [ctypes.c_int, ctypes.POINTER(ScalarStats), ctypes.c_int, ip, ip]
+        L.zkg_msm_pick.argtypes = [ctypes.c_int, ctypes.POINTER(ScalarStats), ctypes.c_int, ctypes.POINTER(MsmSchedule)]
+        L.zkg_msm_tune.argtypes = [ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int,
+                                   ip, ip, ctypes.c_int, dp, ip]
+        L.zkg_msm_batch.argtypes = [ctypes.c_int, ip, ip, ctypes.POINTER(vp), ctypes.c_int,
+                                    ctypes.POINTER(vp), ctypes.POINTER(u64p), ctypes.c_int,
+                                    ctypes.POINTER(MsmSchedule), ctypes.c_int, dp]
+        L.zkg_mlp_create.argtypes = [ctypes.c_int, ip, ctypes.c_uint, ctypes.POINTER(vp)]
+        L.zkg_mlp_destroy.argtypes = [vp]
+        L.zkg_mlp_nparams.argtypes = [vp]
+        L.zkg_mlp_nparams.restype = ctypes.c_longlong
+        L.zkg_mlp_get_params.argtypes = [vp, fp_]
+        L.zkg_mlp_set_params.argtypes = [vp, fp_]
+        L.zkg_mlp_forward.argtypes = [vp, ctypes.c_int, fp_, fp_]
+        L.zkg_mlp_gradients.argtypes = [vp, ctypes.c_int, fp_, fp_, fp_, fp_]
+        L.zkg_mlp_train.argtypes = [vp, ctypes.c_int, fp_, fp_, ctypes.c_int, ctypes.c_float, fp_]
+        L.zkg_msm_set_cost_model.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -244,18 +284,157 @@ def fft_generator(curve, m):
 # ---------------------------------------------------------------------------------------
 # extension API (device buffers)
 
-def msm_device(curve, n, d_scalars, d_points, expo_nlimbs=4, flags=0, window=0, seg_log=0):
+def msm_device(curve, n, d_scalars, d_points, expo_nlimbs=4, flags=0, window=0, seg_log=0, policy=SCHED_PLAIN,
+               fix_limit=0):
     """d_points: DeviceArray of affine points, or PreparedPoints (flag added automatically)."""
     if isinstance(d_points, PreparedPoints):
         flags |= MSM_POINTS_INTERNAL
     size = 3 * FP_LIMBS[curve] if flags & (MSM_PROJ_OUT | MSM_JAC_OUT) else 2 * FP_LIMBS[curve]
     res = np.zeros(size, np.uint64)
     sched = None
-    if window or seg_log:
-        sched = (ctypes.c_int * 8)(window, seg_log, 0, 0, 0, 0, 0, 0)
+    if window or seg_log or policy or fix_limit:
+        sched = MsmSchedule(window, seg_log, policy, fix_limit)
     _check(lib().zkg_msm(curve, n, d_scalars.ptr, expo_nlimbs, d_points.ptr, _p(res), flags,
-                         ctypes.cast(sched, vp) if sched is not None else None))
+                         ctypes.cast(ctypes.byref(sched), vp) if sched is not None else None))
     return res
+
+
+# ---------------------------------------------------------------------------------------
+# schedule selection: statistics, candidates, autotuner, cost model, batched MSMs
+
+def scalar_stats(curve, n, d_scalars, expo_nlimbs=4, flags=0):
+    st = ScalarStats()
+    _check(lib().zkg_msm_scalar_stats(curve, n, d_scalars.ptr, expo_nlimbs, flags, ctypes.byref(st)))
+    return st
+
+
+def stats_from(n, n_zero, n_one, max_bits):
+    st = ScalarStats()
+    st.n, st.n_zero, st.n_one, st.max_bits = n, n_zero, n_one, max_bits
+    return st
+
+
+def msm_features(curve, st, window, seg_log):
+    out = np.zeros(MSM_NFEAT, np.float32)
+    _check(lib().zkg_msm_features(curve, ctypes.byref(st), window, seg_log,
+                                  out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+    return out
+
+
+def msm_candidates(curve, st, max_count=256):
+    cs, ss = (ctypes.c_int * max_count)(), (ctypes.c_int * max_count)()
+    k = lib().zkg_msm_candidates(curve, ctypes.byref(st), max_count, cs, ss)
+    if k < 0:
+        _check(1)
+    return [(cs[i], ss[i]) for i in range(k)]
+
+
+def msm_pick(curve, st, policy):
+    out = MsmSchedule()
+    _check(lib().zkg_msm_pick(curve, ctypes.byref(st), policy, ctypes.byref(out)))
+    return out.window, out.seg_log
+
+
+def msm_tune(curve, n, d_scalars, d_points, cands, reps=3, expo_nlimbs=4, flags=0):
+    """Times every (window, seg_log) candidate; returns (ms list, all results identical)."""
+    if isinstance(d_points, PreparedPoints):
+        flags |= MSM_POINTS_INTERNAL
+    k = len(cands)
+    cs = (ctypes.c_int * k)(*[c for c, _ in cands])
+    ss = (ctypes.c_int * k)(*[s for _, s in cands])
+    ms = (ctypes.c_double * k)()
+    same = ctypes.c_int()
+    _check(lib().zkg_msm_tune(curve, n, d_scalars.ptr, expo_nlimbs, d_points.ptr, flags, k, cs, ss, reps, ms,
+                              ctypes.byref(same)))
+    return list(ms), bool(same.value)
+
+
+def msm_batch(curves, ns, d_scalars, d_points, policy=SCHED_PLAIN, nstreams=4, expo_nlimbs=4, flags=0):
+    """Many MSMs of uneven sizes (curves: one id for all jobs, or one per job) on concurrent
+    streams; returns (affine results, wall ms)."""
+    if any(isinstance(p, PreparedPoints) for p in d_points):
+        if not all(isinstance(p, PreparedPoints) for p in d_points):
+            raise ValueError("msm_batch: mix of prepared and plain point sets")
+        flags |= MSM_POINTS_INTERNAL
+    k = len(ns)
+    if isinstance(curves, int):
+        curves = [curves] * k
+    wide = flags & (MSM_PROJ_OUT | MSM_JAC_OUT)
+    outs = [np.zeros((3 if wide else 2) * FP_LIMBS[c], np.uint64) for c in curves]
+    C = (ctypes.c_int * k)(*curves)
+    N = (ctypes.c_int * k)(*ns)
+    S = (vp * k)(*[d.ptr for d in d_scalars])
+    P = (vp * k)(*[d.ptr for d in d_points])
+    O = (u64p * k)(*[_p(o) for o in outs])
+    sched = MsmSchedule(0, 0, policy)
+    wall = ctypes.c_double()
+    _check(lib().zkg_msm_batch(k, C, N, S, expo_nlimbs, P, O, flags, ctypes.byref(sched), nstreams,
+                               ctypes.byref(wall)))
+    return outs, wall.value
+
+
+def _fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class Mlp:
+    """Dense fp32 network trained and run by the library's HIP kernels (zkg_mlp_*)."""
+
+    def __init__(self, dims, seed=1):
+        self.dims = [int(d) for d in dims]
+        self.h = vp()
+        D = (ctypes.c_int * len(self.dims))(*self.dims)
+        _check(lib().zkg_mlp_create(len(self.dims) - 1, D, seed, ctypes.byref(self.h)))
+        self.nparams = int(lib().zkg_mlp_nparams(self.h))
+
+    def get_params(self):
+        out = np.zeros(self.nparams, np.float32)
+        _check(lib().zkg_mlp_get_params(self.h, _fp(out)))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        if p.size != self.nparams:
+            raise ValueError("mlp: parameter count differs")
+        _check(lib().zkg_mlp_set_params(self.h, _fp(p)))
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.zeros((X.shape[0], self.dims[-1]), np.float32)
+        _check(lib().zkg_mlp_forward(self.h, X.shape[0], _fp(X), _fp(y)))
+        return y
+
+    def gradients(self, X, y):
+        X, y = np.ascontiguousarray(X, np.float32), np.ascontiguousarray(y, np.float32)
+        g = np.zeros(self.nparams, np.float32)
+        loss = ctypes.c_float()
+        _check(lib().zkg_mlp_gradients(self.h, X.shape[0], _fp(X), _fp(y), _fp(g), ctypes.byref(loss)))
+        return g, loss.value
+
+    def train(self, X, y, epochs, lr):
+        X, y = np.ascontiguousarray(X, np.float32), np.ascontiguousarray(y, np.float32)
+        loss = ctypes.c_float()
+        _check(lib().zkg_mlp_train(self.h, X.shape[0], _fp(X), _fp(y), epochs, lr, ctypes.byref(loss)))
+        return loss.value
+
+    def install(self):
+        """Makes this network the cost model of SCHED_LEARNED."""
+        _check(lib().zkg_msm_set_cost_model(self.h))
+
+    @staticmethod
+    def uninstall():
+        _check(lib().zkg_msm_set_cost_model(None))
+
+    def close(self):
+        if self.h:
+            lib().zkg_mlp_destroy(self.h)
+            self.h = vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def ntt_device(curve, m, gen, d_src, d_tgt, inverse=False, eta=None):
