@@ -189,6 +189,29 @@ def cpu_baseline(case, sample_log):
             "matches_gpu": bool((gpu == out).all())}
 
 
+def bench_ntt_pair(steps, warmup, peak_imul=None, m=22):
+    """BN254 Fr NTT then iNTT of 2^m uniform Montgomery elements on the 2^m domain; round trip
+    checked bit for bit."""
+    N = 1 << m
+    rng = np.random.default_rng(22)
+    X = uniform_scalars(rng, N, BN_R)  # uniform field elements (any canonical value is a valid Montgomery form)
+    dX, dY, dZ = zkgpu.DeviceArray.from_host(X), zkgpu.DeviceArray(4 * N), zkgpu.DeviceArray(4 * N)
+    g = zkgpu.fft_generator(0, m)
+    fwd = lambda: zkgpu.ntt_device(0, m, g, dX, dY)
+    inv = lambda: zkgpu.ntt_device(0, m, g, dY, dZ, inverse=True)
+    tf = time_steps(fwd, steps, warmup)
+    ti = time_steps(inv, steps, warmup)
+    rt_ok = bool((dZ.to_host((N, 4)) == X).all())
+    out = {"bn254_ntt_2p22_fwd_ms": round(tf * 1e3, 3), "bn254_ntt_2p22_inv_ms": round(ti * 1e3, 3),
+           "bn254_ntt_2p22_fwd_GBps": round(64 * N / tf / 1e9, 1),
+           "bn254_ntt_2p22_inv_GBps": round(64 * N / ti / 1e9, 1),
+           "bn254_ntt_2p22_hbm_frac": round(64 * N / tf / 8e12, 4),
+           "bn254_ntt_2p22_roundtrip_ok": rt_ok}
+    if peak_imul:
+        out["bn254_ntt_2p22_imul_roofline_frac"] = round((N // 2) * m * m32(8) / tf / peak_imul, 4)
+    return out
+
+
 def bench_ntt_batch(njobs=256, seed=4):
     """256 BN254 Fr NTTs, log sizes uniform in 10..22, fair-coin forward/inverse, half on a coset
     (seeded eta): one hipGraph replay against one plain call per job; outputs must agree."""
@@ -331,7 +354,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--cpu-sample-log", type=int, default=18)
+    ap.add_argument("--leg", choices=["all", "msm", "ntt", "copy"], default="all",
+                    help="profiling runs: only the BN254 MSM, only the 2^22 NTT pair, or the HBM copy kernel")
     a = ap.parse_args()
+    if a.leg == "msm":
+        a.no_extra = a.no_cpu = True
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
               "(multi-GPU sharding is a design note only)", file=sys.stderr)
@@ -342,6 +369,17 @@ def main():
 
     cus, hbm = zkgpu.device_info()
     log(f"device: {cus} CUs, {hbm / 2**30:.0f} GiB")
+    import costmodel
+    ntt_entries = costmodel.install_ntt_tuning()  # autotuned NTT schedules, when saved
+    log(f"ntt tuning entries installed: {ntt_entries}")
+    if a.leg == "copy":
+        bw = zkgpu.microbench(zkgpu.MB_COPY)
+        log(f"copy kernel: 1 GiB read + 1 GiB written per launch, {bw / 1e12:.2f} TB/s")
+        return
+    if a.leg == "ntt":
+        for k, v in bench_ntt_pair(a.steps, a.warmup).items():
+            log(f"{k}: {v}")
+        return
     peak_imul = zkgpu.microbench(zkgpu.MB_IMUL)
     log(f"measured v_mad_u64_u32 peak: {peak_imul / 1e12:.2f} T/s")
 
@@ -370,7 +408,7 @@ def main():
         roof["traffic"] = json.loads(traffic.read_text()).get("bytes_per_launch")
     whole_frac = a_msm(bn.n, 0) / t / peak_imul
 
-    extra = {"bn254_msm_2p20_imul_roofline_frac": round(whole_frac, 4),
+    extra = {"ntt_tuning_entries": ntt_entries, "bn254_msm_2p20_imul_roofline_frac": round(whole_frac, 4),
              "bn254_msm_closed_form_ok": ok_bn,
              "bn254_msm_phases_ms": [round(x, 4) for x in phases]}
     if not a.no_extra:
@@ -383,27 +421,9 @@ def main():
         extra["bls12_381_msm_2p20_imul_roofline_frac"] = round(a_msm(bls.n, 1) / tb / peak_imul, 4)
         log(f"bls msm: {tb * 1e3:.3f} ms, closed form {extra['bls12_381_msm_closed_form_ok']}")
         del bls
-        # NTT 2^22 forward + inverse, BN254 Fr
-        m = 22
-        N = 1 << m
-        rng = np.random.default_rng(22)
-        X = uniform_scalars(rng, N, BN_R)  # uniform field elements (any canonical value is a valid Montgomery form)
-        dX, dY, dZ = zkgpu.DeviceArray.from_host(X), zkgpu.DeviceArray(4 * N), zkgpu.DeviceArray(4 * N)
-        g = zkgpu.fft_generator(0, m)
-        fwd = lambda: zkgpu.ntt_device(0, m, g, dX, dY)
-        inv = lambda: zkgpu.ntt_device(0, m, g, dY, dZ, inverse=True)
-        tf = time_steps(fwd, a.steps, a.warmup)
-        ti = time_steps(inv, a.steps, a.warmup)
-        rt_ok = bool((dZ.to_host((N, 4)) == X).all())
-        ntt_mul = (N // 2) * m * m32(8)
-        extra.update({"bn254_ntt_2p22_fwd_ms": round(tf * 1e3, 3), "bn254_ntt_2p22_inv_ms": round(ti * 1e3, 3),
-                      "bn254_ntt_2p22_fwd_GBps": round(64 * N / tf / 1e9, 1),
-                      "bn254_ntt_2p22_inv_GBps": round(64 * N / ti / 1e9, 1),
-                      "bn254_ntt_2p22_hbm_frac": round(64 * N / tf / 8e12, 4),
-                      "bn254_ntt_2p22_imul_roofline_frac": round(ntt_mul / tf / peak_imul, 4),
-                      "bn254_ntt_2p22_roundtrip_ok": rt_ok})
-        log(f"ntt 2^22: fwd {tf * 1e3:.3f} ms inv {ti * 1e3:.3f} ms roundtrip {rt_ok}")
-        del dX, dY, dZ
+        extra.update(bench_ntt_pair(a.steps, a.warmup, peak_imul))
+        log(f"ntt 2^22: fwd {extra['bn254_ntt_2p22_fwd_ms']} ms inv {extra['bn254_ntt_2p22_inv_ms']} ms "
+            f"roundtrip {extra['bn254_ntt_2p22_roundtrip_ok']}")
         extra.update(bench_ntt_batch())
         log(f"ntt batch: {extra['ntt_batch_ms']} ms batched vs {extra['ntt_plain_ms']} ms plain, identical {extra['ntt_batch_bit_identical_to_plain']}")
         extra.update(bench_learned_msm())

@@ -162,8 +162,15 @@ int zkg_ntt_batch_destroy(void *plan);
 
 /* generator of the size-2^m FFT subgroup, Montgomery form (lib/src/ZK/Algebra/Class/FFT.hs:60-68) */
 int zkg_fft_generator(int curve, int m, uint64_t *gen_out);
-/* NTT fast-path schedule (passes of at most 2^smax points, LDS tiles of 2^tile_log elements) */
+/* NTT fast-path schedule (passes of at most 2^smax points, LDS tiles of 2^tile_log elements):
+ * the default for every size (clears per-size entries), or one (curve, m) entry */
 int zkg_ntt_set_tuning(int smax, int tile_log);
+int zkg_ntt_set_tuning_for(int curve, int m, int smax, int tile_log);
+/* autotuner: times every (smax 5..11, tile_log 7..12) on a random 2^m vector (median of reps
+ * HIP-event timings), checks that all give the same output, and installs the fastest for
+ * (curve, m) when install != 0 */
+int zkg_ntt_tune(int curve, int m, int inverse, int reps, int install, int *smax, int *tile_log, double *ms,
+                 int *identical);
 
 /* device elementwise field ops (field: 0 = Fp, 1 = Fr; op: see zikkurat-algebra_amd/csrc/peak.hpp) */
 int zkg_field_op(int curve, int field, int op, size_t n, const uint64_t *d_a, const uint64_t *d_b, uint64_t *d_out);

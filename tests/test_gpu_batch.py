@@ -57,3 +57,26 @@ def test_ntt_batch_large_vs_plain(zk):
     for (m, _, _), (_, _, _, _, dY), dP in zip(specs, jobs, plain):
         assert (dY.to_host() == dP.to_host()).all(), m
     plan.close()
+
+
+def test_ntt_tuning_table_bit_exact(zk):
+    """Per-size NTT schedules from the autotuner (and deliberately small ones) give the oracle's
+    output; the batch plan picks them up per job."""
+    o = Oracle(BN)
+    try:
+        for m in (10, 13):
+            smax, tl, ms, same = zk.ntt_tune(BN.id, m, reps=1, install=True)
+            assert same and ms > 0 and 5 <= smax <= 11 and 7 <= tl <= 12
+        zk.ntt_set_tuning(5, 7, curve=BN.id, m=12)
+        for m in (10, 12, 13):
+            X = ints_to_array([BN.fr_to_mont(x) for x in random_scalars(700 + m, 1 << m, BN.r)], 4)
+            g = o.fft_generator(m)
+            dX, dY = zk.DeviceArray.from_host(X), zk.DeviceArray(X.size)
+            zk.ntt_device(BN.id, m, g, dX, dY)
+            assert (dY.to_host(X.shape) == o.ntt(m, g, X)).all(), m
+            plan = zk.NttBatch(BN.id, [(m, 1, None, dX, dY)])
+            plan.run()
+            assert (dY.to_host(X.shape) == o.ntt(m, g, X, inverse=True)).all(), m
+            plan.close()
+    finally:
+        zk.ntt_set_tuning(8, 9)

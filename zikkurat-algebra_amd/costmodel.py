@@ -20,6 +20,10 @@ installed into the library with ``Mlp.install()`` and used by the SCHED_LEARNED 
     python costmodel.py collect --curves 0 1 --mix 24 --seed 8 --out eval.json
     python costmodel.py train --data train.json --model tuned/msm_cost_gfx950
     python costmodel.py evaluate --data eval.json --model tuned/msm_cost_gfx950
+    python costmodel.py ntt --out tuned/ntt_tuning_gfx950.json
+
+The NTT has its own, smaller autotuner: per (curve, log size) the pass split bound smax and
+the LDS tile size are timed exhaustively and the table is saved and reinstalled the same way.
 """
 from __future__ import annotations
 
@@ -42,6 +46,7 @@ R = {0: 218882428718392752222464057452572750885483644004160343436982041865758084
 DISTS = ("uniform", "u64", "sparse")
 HIDDEN = (32, 32)
 DEFAULT_MODEL = PKG / "tuned" / "msm_cost_gfx950"
+NTT_TUNING = PKG / "tuned" / "ntt_tuning_gfx950.json"
 
 
 def uniform_below(rng, n, r):
@@ -210,6 +215,30 @@ def evaluate(records, net):
                 exhaustive=agg("best_ms"), rows=rows)
 
 
+def tune_ntt(curves=(0, 1), lo=10, hi=22, reps=3, log=None):
+    """NTT autotuning per (curve, log size): forward transform, every (smax, tile_log); the
+    winner is installed in the library and returned."""
+    table = []
+    for c in curves:
+        for m in range(lo, hi + 1):
+            smax, tl, ms, same = zkgpu.ntt_tune(c, m, reps=reps, install=True)
+            table.append(dict(curve=c, m=m, smax=smax, tile_log=tl, ms=round(ms, 5), identical=same))
+            if log:
+                log(f"ntt curve={c} m={m}: smax={smax} tile_log={tl} {ms:.4f} ms identical={same}")
+    return table
+
+
+def install_ntt_tuning(path=NTT_TUNING):
+    """Installs a saved NTT tuning table; returns the number of entries (0 if none saved)."""
+    path = Path(path)
+    if not path.exists():
+        return 0
+    rows = json.loads(path.read_text())["table"]
+    for r in rows:
+        zkgpu.ntt_set_tuning(r["smax"], r["tile_log"], curve=r["curve"], m=r["m"])
+    return len(rows)
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -225,6 +254,11 @@ def main():
     t.add_argument("--model", default=str(DEFAULT_MODEL))
     t.add_argument("--epochs", type=int, default=4000)
     t.add_argument("--lr", type=float, default=3e-3)
+    nt = sub.add_parser("ntt")
+    nt.add_argument("--curves", type=int, nargs="+", default=[0, 1])
+    nt.add_argument("--logn", type=int, nargs=2, default=[10, 22])
+    nt.add_argument("--reps", type=int, default=5)
+    nt.add_argument("--out", default=str(NTT_TUNING))
     e = sub.add_parser("evaluate")
     e.add_argument("--data", nargs="+", required=True)
     e.add_argument("--model", default=str(DEFAULT_MODEL))
@@ -234,6 +268,11 @@ def main():
     def log(*x):
         print(*x, flush=True)
 
+    if a.cmd == "ntt":
+        table = tune_ntt(a.curves, a.logn[0], a.logn[1], a.reps, log)
+        Path(a.out).write_text(json.dumps(dict(device=zkgpu.device_info(), table=table), indent=1))
+        log(f"all schedules identical: {all(r['identical'] for r in table)}")
+        return
     if a.cmd == "collect":
         jobs = (mix_jobs(a.curves, a.mix, a.seed, a.logn[0], a.logn[1]) if a.mix
                 else grid_jobs(a.curves, a.logn[0], a.logn[1], a.seed))

@@ -302,6 +302,28 @@ int zkg_ntt_set_tuning(int smax, int tile_log) {
   return ZK_OK;
 }
 
+int zkg_ntt_set_tuning_for(int curve, int m, int smax, int tile_log) {
+  if ((curve != 0 && curve != 1) || m < 0 || m > 30 || smax < 1 || smax > 11 || tile_log < 4 || tile_log > 12) {
+    set_error("ntt tuning out of range");
+    return ZK_EINVAL;
+  }
+  ntt_set_tuning_for(curve, m, NttTuning{smax, tile_log});
+  return ZK_OK;
+}
+
+int zkg_ntt_tune(int curve, int m, int inverse, int reps, int install, int *smax, int *tile_log, double *ms,
+                 int *identical) {
+  Context *ctx;
+  ZK_TRY(with_ctx(&ctx));
+  std::lock_guard<std::recursive_mutex> lk(ctx->lock);
+  NttTuning best{0, 0};
+  ZK_TRY(ntt_tune(curve, m, inverse, reps, &best, ms, identical, ctx->stream));
+  if (install) ntt_set_tuning_for(curve, m, best);
+  if (smax) *smax = best.smax;
+  if (tile_log) *tile_log = best.tile_log;
+  return ZK_OK;
+}
+
 int zkg_field_op(int curve, int field, int op, size_t n, const uint64_t *d_a, const uint64_t *d_b, uint64_t *d_out) {
   return field_op_device(curve, field, op, n, d_a, d_b, d_out);
 }

@@ -40,7 +40,10 @@ struct XyzzZ {
 template <class C>
 struct ZLayout {
   static constexpr int N = ZP<C>::T::N;
-  static constexpr int AFF = (2 * N + 3) / 4 * 4;  // affine point, padded to 16 bytes
+  // affine point: one 128-byte line (the MSM reads points at random indices, so a record
+  // that straddled two lines would fetch both)
+  static constexpr int AFF = 32;
+  static_assert(2 * N <= AFF, "affine point must fit one 128-byte line");
   static constexpr int XYZZ = 4 * N;               // 36 or 56 words: multiples of 4
 };
 

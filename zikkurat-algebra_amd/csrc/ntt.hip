@@ -18,12 +18,14 @@
 // coalesced in T*32-byte runs.
 #include <string.h>
 
+#include <algorithm>
 #include <mutex>
 #include <vector>
 
 #include "field.hpp"
 #include "hostfield.hpp"
 #include "ntt.hpp"
+#include "peak.hpp"
 #include "runtime.hpp"
 
 namespace zk {
@@ -528,9 +530,31 @@ static int ntt_network(int curve, int m, const uint64_t *gen, const uint32_t *d_
   return ZK_OK;
 }
 
-static NttTuning g_tuning = {8, 11};
-void ntt_set_tuning(const NttTuning &t) { g_tuning = t; }
-NttTuning ntt_get_tuning() { return g_tuning; }
+// Default schedule (passes of at most 2^8 points, LDS tiles of 2^9 elements: 16 KiB per
+// workgroup keeps several workgroups per CU and enough of them for small transforms), and a
+// per-(curve, size) table filled by the autotuner.  A set entry overrides the default.
+static std::mutex g_tuning_mu;
+static NttTuning g_tuning = {8, 9};
+static NttTuning g_tuning_for[2][31] = {};
+
+void ntt_set_tuning(const NttTuning &t) {
+  std::lock_guard<std::mutex> lk(g_tuning_mu);
+  g_tuning = t;
+  memset(g_tuning_for, 0, sizeof g_tuning_for);
+}
+NttTuning ntt_get_tuning() {
+  std::lock_guard<std::mutex> lk(g_tuning_mu);
+  return g_tuning;
+}
+void ntt_set_tuning_for(int curve, int m, const NttTuning &t) {
+  std::lock_guard<std::mutex> lk(g_tuning_mu);
+  if (curve >= 0 && curve < 2 && m >= 0 && m <= 30) g_tuning_for[curve][m] = t;
+}
+NttTuning ntt_tuning_for(int curve, int m) {
+  std::lock_guard<std::mutex> lk(g_tuning_mu);
+  if (curve >= 0 && curve < 2 && m >= 0 && m <= 30 && g_tuning_for[curve][m].smax > 0) return g_tuning_for[curve][m];
+  return g_tuning;
+}
 
 template <class F>
 static int ntt_device_t(int curve, int m, const uint64_t *gen, const uint64_t *eta, int flags,
@@ -567,7 +591,7 @@ static int ntt_device_t(int curve, int m, const uint64_t *gen, const uint64_t *e
   if (eta) ZK_TRY(build_eta_tables<F>(zkh::load(f, eta), etab, st));
   Fe<F> ninv;
   if (inverse) ninv = to_dev<F>(zkh::inv(f, zkh::from_u64(f, N)));
-  const NttTuning tu = tune ? *tune : g_tuning;
+  const NttTuning tu = tune ? *tune : ntt_tuning_for(curve, m);
   NttPlan plan = ntt_plan(m, tu.smax, tu.tile_log);
   const uint32_t *ep = (eta && !inverse) ? etab.as<uint32_t>() : nullptr;
   const uint32_t *eo = (eta && inverse) ? etab.as<uint32_t>() : nullptr;
@@ -636,10 +660,10 @@ static int ntt_batch_create_t(int curve, int njobs, const NttBatchJob *jobs, Ntt
   int shift0 = 0;
   ZK_TRY(get_twiddles<F>(curve, mmax, gw, &tw, &shift0, st));
   // scratch for multi-pass jobs and coset tables
-  const NttTuning tu = g_tuning;
   uint64_t tmp_words = 0, ncoset = 0;
   std::vector<NttPlan> plans(njobs);
   for (int j = 0; j < njobs; j++) {
+    const NttTuning tu = ntt_tuning_for(curve, jobs[j].m);
     plans[j] = ntt_plan(jobs[j].m, tu.smax, tu.tile_log);
     if (plans[j].npass > 1) tmp_words += ((uint64_t)1 << jobs[j].m) * F::N;
     if (jobs[j].eta) ncoset++;
@@ -790,6 +814,81 @@ int ntt_batch_run(NttBatchPlan *pl, double *ms) {
 void ntt_batch_destroy(NttBatchPlan *pl) { delete pl; }
 
 uint64_t ntt_batch_bytes(const NttBatchPlan *pl) { return pl->bytes_moved; }
+
+// ---------------------------------------------------------------------------------------
+// autotuner: every (smax, tile_log) on one random vector, timed with HIP events; all outputs
+// must agree word for word
+
+__global__ void count_mismatch_kernel(const uint4 *__restrict__ a, const uint4 *__restrict__ b, uint64_t n,
+                                      uint32_t *__restrict__ bad) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint4 x = a[i], y = b[i];
+  if (x.x != y.x || x.y != y.y || x.z != y.z || x.w != y.w) atomicAdd(bad, 1u);
+}
+
+int ntt_tune(int curve, int m, int inverse, int reps, NttTuning *best, double *best_ms, int *identical,
+             hipStream_t st) {
+  if (m < 1 || m > 26 || (curve != 0 && curve != 1)) {
+    set_error("ntt_tune: bad curve id or log size");
+    return ZK_EINVAL;
+  }
+  const uint64_t N = 1ull << m, words = N * 8;
+  DevBuf in, out, ref, bad;
+  ZK_TRY(in.alloc(words * 4));
+  ZK_TRY(out.alloc(words * 4));
+  ZK_TRY(ref.alloc(words * 4));
+  ZK_TRY(bad.alloc(16));
+  ZK_TRY(fill_random_device(curve, 1, N, 0x5eed0000u + (uint64_t)m, in.p));
+  const zkh::Fe g = fft_generator_host(curve, m);
+  uint64_t gw[4];
+  zkh::store(fr_desc(curve), gw, g);
+  hipEvent_t e0, e1;
+  ZK_HIP(hipEventCreate(&e0));
+  ZK_HIP(hipEventCreate(&e1));
+  bool first = true, same = true;
+  double bms = 1e300;
+  NttTuning bt = ntt_tuning_for(curve, m);
+  for (int smax = 5; smax <= 11; smax++) {
+    for (int tl = 7; tl <= 12; tl++) {
+      const NttTuning t{smax, tl};
+      std::vector<float> ts;
+      for (int r = -1; r < std::max(1, reps); r++) {
+        ZK_HIP(hipEventRecord(e0, st));
+        ZK_TRY(ntt_device(curve, m, gw, nullptr, inverse ? NTT_INVERSE : NTT_FORWARD, in.p, out.p, st, &t));
+        ZK_HIP(hipEventRecord(e1, st));
+        ZK_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        ZK_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (r >= 0) ts.push_back(ms);
+      }
+      std::sort(ts.begin(), ts.end());
+      const double med = ts[ts.size() / 2];
+      if (first) {
+        ZK_HIP(hipMemcpyAsync(ref.p, out.p, words * 4, hipMemcpyDeviceToDevice, st));
+        first = false;
+      } else {
+        uint32_t nbad = 0;
+        ZK_HIP(hipMemsetAsync(bad.p, 0, 4, st));
+        hipLaunchKernelGGL(count_mismatch_kernel, dim3((unsigned)((words / 4 + 255) / 256)), dim3(256), 0, st,
+                           out.as<uint4>(), ref.as<uint4>(), words / 4, bad.as<uint32_t>());
+        ZK_HIP(hipMemcpyAsync(&nbad, bad.p, 4, hipMemcpyDeviceToHost, st));
+        ZK_HIP(hipStreamSynchronize(st));
+        if (nbad) same = false;
+      }
+      if (med < bms) {
+        bms = med;
+        bt = t;
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (best) *best = bt;
+  if (best_ms) *best_ms = bms;
+  if (identical) *identical = same ? 1 : 0;
+  return ZK_OK;
+}
 
 int ntt_device(int curve, int m, const uint64_t *gen, const uint64_t *eta, int flags, const void *d_in,
                void *d_out, hipStream_t st, const NttTuning *tune) {

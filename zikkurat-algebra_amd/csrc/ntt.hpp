@@ -24,8 +24,13 @@ struct NttPlan {
 };
 
 NttPlan ntt_plan(int m, int smax, int tile_log);
-void ntt_set_tuning(const NttTuning &t);
+void ntt_set_tuning(const NttTuning &t);  // default for every size; clears the per-size table
 NttTuning ntt_get_tuning();
+void ntt_set_tuning_for(int curve, int m, const NttTuning &t);
+NttTuning ntt_tuning_for(int curve, int m);
+// times every (smax, tile_log) for one size; best is not installed
+int ntt_tune(int curve, int m, int inverse, int reps, NttTuning *best, double *best_ms, int *identical,
+             hipStream_t st);
 
 // true when gen has multiplicative order exactly 2^m (the fast path's precondition)
 bool ntt_generator_ok(int curve, int m, const uint64_t *gen);

@@ -121,6 +121,9 @@ def lib():
         L.zkg_mlp_gradients.argtypes = [vp, ctypes.c_int, fp_, fp_, fp_, fp_]
         L.zkg_mlp_train.argtypes = [vp, ctypes.c_int, fp_, fp_, ctypes.c_int, ctypes.c_float, fp_]
         L.zkg_msm_set_cost_model.argtypes = [vp]
+        L.zkg_ntt_set_tuning.argtypes = [ctypes.c_int, ctypes.c_int]
+        L.zkg_ntt_set_tuning_for.argtypes = [ctypes.c_int] * 4
+        L.zkg_ntt_tune.argtypes = [ctypes.c_int] * 5 + [ip, ip, dp, ip]
         _lib = L
     return _lib
 
@@ -435,6 +438,23 @@ class Mlp:
             self.close()
         except Exception:
             pass
+
+
+def ntt_set_tuning(smax, tile_log, curve=None, m=None):
+    """Default NTT schedule, or one (curve, m) entry when both are given."""
+    if curve is None:
+        _check(lib().zkg_ntt_set_tuning(smax, tile_log))
+    else:
+        _check(lib().zkg_ntt_set_tuning_for(curve, m, smax, tile_log))
+
+
+def ntt_tune(curve, m, inverse=False, reps=3, install=True):
+    """Autotunes the NTT schedule for one size; returns (smax, tile_log, ms, identical)."""
+    s, t, ok = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    ms = ctypes.c_double()
+    _check(lib().zkg_ntt_tune(curve, m, int(inverse), reps, int(install), ctypes.byref(s), ctypes.byref(t),
+                              ctypes.byref(ms), ctypes.byref(ok)))
+    return s.value, t.value, ms.value, bool(ok.value)
 
 
 def ntt_device(curve, m, gen, d_src, d_tgt, inverse=False, eta=None):
