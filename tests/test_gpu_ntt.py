@@ -102,3 +102,48 @@ def test_large_roundtrip_and_evaluation(zk, C, m):
     w = C.fft_root(m)
     for k in (1, N // 2 + 3):
         assert array_to_ints(Y[k:k + 1])[0] == horner(coeffs, pow(w, k, C.r), C.r), k
+
+
+@pytest.mark.parametrize("C", CURVES, ids=["bn254", "bls12-381"])
+@pytest.mark.parametrize("m", [1, 3, 8, 11, 12, 17, 22])
+def test_extreme_inputs_closed_form(zk, C, m):
+    """Largest canonical inputs drive the butterflies' intermediate bounds hardest.  Closed forms:
+    a constant vector c maps to (N c, 0, ..., 0); c on even indices only maps to N/2 c at k = 0
+    and k = N/2; the inverse of (N c, 0, ..., 0) is the constant c."""
+    N = 1 << m
+    c = C.r - 1
+    g = gen_arr(C, m)
+    mont = lambda v: C.fr_to_mont(v % C.r)
+    # constant
+    X = np.tile(np.array(to_limbs(mont(c), 4), np.uint64), (N, 1))
+    dX, dY, dZ = zk.DeviceArray.from_host(X), zk.DeviceArray(X.size), zk.DeviceArray(X.size)
+    zk.ntt_device(C.id, m, g, dX, dY)
+    Y = dY.to_host(X.shape)
+    assert array_to_ints(Y[:1]) == [mont(N * c)] and not Y[1:].any()
+    zk.ntt_device(C.id, m, g, dY, dZ, inverse=True)
+    assert (dZ.to_host(X.shape) == X).all()
+    # even indices only
+    if m >= 1:
+        X2 = X.copy()
+        X2[1::2] = 0
+        dX2 = zk.DeviceArray.from_host(X2)
+        zk.ntt_device(C.id, m, g, dX2, dY)
+        Y = dY.to_host(X.shape)
+        half = mont((N // 2) * c)
+        exp = np.zeros_like(X)
+        exp[0] = to_limbs(half, 4)
+        exp[N // 2] = to_limbs(half, 4)
+        assert (Y == exp).all()
+        zk.ntt_device(C.id, m, g, dY, dZ, inverse=True)
+        assert (dZ.to_host(X.shape) == X2).all()
+    # coset forward of the constant: sum_i c eta^i g^(ik) = c (eta^N - 1) / (eta g^k - 1)
+    if m <= 12:
+        eta = 5
+        etam = np.array(to_limbs(mont(eta), 4), np.uint64)
+        zk.ntt_device(C.id, m, g, dX, dY, eta=etam)
+        got = array_to_ints(dY.to_host(X.shape))
+        gr = C.fft_root(m)
+        for k in (0, 1, N - 1):
+            num = c * (pow(eta, N, C.r) - 1) % C.r
+            den = (eta * pow(gr, k, C.r) - 1) % C.r
+            assert got[k] == mont(num * pow(den, -1, C.r))

@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "field.hpp"
+#include "fz.hpp"
 #include "hostfield.hpp"
 #include "ntt.hpp"
 #include "peak.hpp"
@@ -32,54 +33,8 @@ namespace zk {
 
 static constexpr int kNttThreads = 256;
 
-// omega^e from a table of omega^j, j < Ntab/2 (stride 2^tw_shift for a sub-domain); e < N
-template <class F>
-__device__ __forceinline__ void twiddle(Fe<F> &w, const uint32_t *__restrict__ tw, uint64_t e,
-                                        int logn, int tw_shift, int inverse) {
-  const uint64_t N = 1ull << logn;
-  if (inverse) e = (N - e) & (N - 1);
-  const uint64_t half = N >> 1;
-  const bool neg = e >= half;
-  if (neg) e -= half;
-  fload(w, tw + ((e << tw_shift) * F::N));
-  if (neg) fneg(w, w);
-}
-
-// eta^n from three 2048-entry tables (eta^j, eta^(2^11 j), eta^(2^22 j))
-template <class F>
-__device__ __forceinline__ void eta_pow(Fe<F> &r, const uint32_t *__restrict__ et, uint64_t n) {
-  Fe<F> a, b;
-  fload(a, et + (n & 2047) * F::N);
-  fload(b, et + (2048 + ((n >> 11) & 2047)) * F::N);
-  fmul(r, a, b);
-  if (n >> 22) {
-    fload(a, et + (4096 + ((n >> 22) & 2047)) * F::N);
-    fmul(r, r, a);
-  }
-}
-
 __device__ __forceinline__ uint32_t bitrev(uint32_t x, int bits) {
   return bits ? (__builtin_bitreverse32(x) >> (32 - bits)) : 0u;
-}
-
-template <class F>
-__device__ __forceinline__ void lds_load(Fe<F> &r, const uint32_t *p) {
-  const uint4 *q = reinterpret_cast<const uint4 *>(p);
-#pragma unroll
-  for (int j = 0; j < F::N / 4; j++) {
-    uint4 v = q[j];
-    r.w[4 * j] = v.x;
-    r.w[4 * j + 1] = v.y;
-    r.w[4 * j + 2] = v.z;
-    r.w[4 * j + 3] = v.w;
-  }
-}
-template <class F>
-__device__ __forceinline__ void lds_store(uint32_t *p, const Fe<F> &a) {
-  uint4 *q = reinterpret_cast<uint4 *>(p);
-#pragma unroll
-  for (int j = 0; j < F::N / 4; j++)
-    q[j] = make_uint4(a.w[4 * j], a.w[4 * j + 1], a.w[4 * j + 2], a.w[4 * j + 3]);
 }
 
 struct PassArgs {
@@ -88,15 +43,79 @@ struct PassArgs {
   const uint32_t *tw;       // omega^j table
   const uint32_t *eta_pre;  // forward shifted: multiply input i by eta^i (first pass only)
   const uint32_t *eta_post; // inverse shifted: multiply output i by eta^i (last pass only)
-  uint32_t ninv[8];         // N^-1 (Montgomery) for the inverse, applied in the last pass
+  uint32_t ninv[8];         // last pass factor, internal form: N^-1 R' (inverse) or R' (forward)
   int logn, s, logc, logr, logt, last, inverse, tw_shift, has_ninv;
   int nsplit;               // number of passes before this (last pass: digit reversal)
   int split[6];             // log sizes of the earlier passes
 };
 
+// ---- reduced-radix butterflies (fz.hpp).  Data stays in the reference's Montgomery form
+// x R in HBM (canonical 8-word strings); twiddles, coset powers and N^-1 are kept in the
+// internal form w R' (R' = 2^261), so zmul(x R, w R') = x w R stays external.  Bounds: loads
+// are canonical (< p); a twiddle product of a normalised |v| < 32 p is in (-p/2, 3p/2) for
+// both fields (p / R' <= 2^-6); each radix-2 stage adds at most 2p, so after s <= 11 stages
+// every value is below 22 p, inside zmul's input range; every stored value goes through a
+// final product and zcanon.
+template <class F> struct ZOf;
+template <> struct ZOf<Bn254Fr> { using T = ZBn254Fr; };
+template <> struct ZOf<Bls381Fr> { using T = ZBls381Fr; };
+
+static constexpr int kZW = 9;  // LDS words per element (9 limbs of 29 bits)
+
+// omega^e as an internal-form table entry; returns true when the table value must be negated
+template <class Z>
+__device__ __forceinline__ bool twiddle_z(Fz<Z> &w, const uint32_t *__restrict__ tw, uint64_t e, int logn,
+                                          int tw_shift, int inverse) {
+  const uint64_t N = 1ull << logn;
+  if (inverse) e = (N - e) & (N - 1);
+  const uint64_t half = N >> 1;
+  const bool neg = e >= half;
+  if (neg) e -= half;
+  zunpack(w, tw + ((e << tw_shift) * 8));
+  return neg;
+}
+
+// eta^n (internal form, normalised, < 2p) from three internal-form tables
+template <class Z>
+__device__ __forceinline__ void eta_pow_z(Fz<Z> &r, const uint32_t *__restrict__ et, uint64_t n) {
+  Fz<Z> a, b;
+  zunpack(a, et + (n & 2047) * 8);
+  zunpack(b, et + (2048 + ((n >> 11) & 2047)) * 8);
+  zmul(r, a, b);
+  if (n >> 22) {
+    zunpack(a, et + (4096 + ((n >> 22) & 2047)) * 8);
+    zmul(r, r, a);
+  }
+}
+
+// canonical (p - t) mod p for canonical t
+template <class Z>
+__device__ __forceinline__ void zneg_canon(Fz<Z> &r, const Fz<Z> &t) {
+  if (zis_raw_zero(t)) {
+    r = t;
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < Z::N; i++) r.l[i] = Z::P[i] - t.l[i];
+  znorm(r);
+}
+
+template <class Z>
+__device__ __forceinline__ void zlds_load(Fz<Z> &r, const int32_t *p) {
+#pragma unroll
+  for (int i = 0; i < Z::N; i++) r.l[i] = p[i];
+}
+template <class Z>
+__device__ __forceinline__ void zlds_store(int32_t *p, const Fz<Z> &a) {
+#pragma unroll
+  for (int i = 0; i < Z::N; i++) p[i] = a.l[i];
+}
+
 // one workgroup's share of a pass: `blk` indexes the pass's tiles
 template <class F>
-__device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t blk, uint32_t *lds) {
+__device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t blk, int32_t *lds) {
+  using Z = typename ZOf<F>::T;
+  using E = Fz<Z>;
   const int S = 1 << a.s, T = 1 << a.logt;
   const int elems = S << a.logt;
   const int tid = threadIdx.x;
@@ -124,14 +143,14 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t blk, u
       }
       gi = (rr << a.s) + n;
     }
-    Fe<F> x;
-    fload(x, a.in + gi * F::N);
+    E x;
+    zunpack(x, a.in + gi * 8);
     if (a.eta_pre) {
-      Fe<F> e;
-      eta_pow(e, a.eta_pre, gi);
-      fmul(x, x, e);
+      E e;
+      eta_pow_z(e, a.eta_pre, gi);
+      zmul(x, x, e);
     }
-    lds_store(lds + ((bitrev(n, a.s) << a.logt) | c) * F::N, x);
+    zlds_store(lds + ((bitrev(n, a.s) << a.logt) | c) * kZW, x);
   }
   __syncthreads();
 
@@ -143,60 +162,67 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t blk, u
       const int bf = q >> a.logt;
       const int j = bf & (h - 1);
       const int blk = bf >> lh;
-      const int pu = ((((blk << (lh + 1)) | j) << a.logt) | c) * F::N;
-      const int pv = pu + ((h << a.logt) * F::N);
-      Fe<F> u, v;
-      lds_load(u, lds + pu);
-      lds_load(v, lds + pv);
+      const int pu = ((((blk << (lh + 1)) | j) << a.logt) | c) * kZW;
+      const int pv = pu + ((h << a.logt) * kZW);
+      E u, v, t;
+      zlds_load(u, lds + pu);
+      zlds_load(v, lds + pv);
+      bool neg = false;
       if (lh > 0) {
-        Fe<F> w;
-        twiddle(w, a.tw, (uint64_t)j << (a.logn - lh - 1), a.logn, a.tw_shift, a.inverse);
-        fmul(v, v, w);
+        E w;
+        neg = twiddle_z(w, a.tw, (uint64_t)j << (a.logn - lh - 1), a.logn, a.tw_shift, a.inverse);
+        zmul(t, v, w);
+      } else {
+        t = v;
       }
-      Fe<F> t;
-      fadd(t, u, v);
-      fsub(v, u, v);
-      lds_store(lds + pu, t);
-      lds_store(lds + pv, v);
+      E s0, s1;
+      zadd(s0, u, t);
+      zsub(s1, u, t);
+      znorm(s0);
+      znorm(s1);
+      zlds_store(lds + pu, neg ? s1 : s0);
+      zlds_store(lds + pv, neg ? s0 : s1);
     }
     __syncthreads();
   }
 
-  // ---- four-step twiddle and store
-  Fe<F> ninv;
+  // ---- four-step twiddle (or N^-1 and coset post-scale), canonical store
+  E ninv;
 #pragma unroll
-  for (int j = 0; j < F::N; j++) ninv.w[j] = a.ninv[j];
+  for (int j = 0; j < Z::N; j++) ninv.l[j] = 0;
+  if (a.last) zunpack(ninv, a.ninv);  // internal-form factor: N^-1 R' (inverse) or R' (forward)
   for (int idx = tid; idx < elems; idx += kNttThreads) {
     const uint32_t k = idx >> a.logt, c = idx & (T - 1);
-    Fe<F> x;
-    lds_load(x, lds + idx * F::N);
+    E x;
+    zlds_load(x, lds + idx * kZW);
     uint64_t go;
+    bool neg = false;
     if (!a.last) {
       const uint64_t col = c0 + c;
-      const uint64_t e = ((col * k) << a.logr);
-      if (e) {
-        Fe<F> w;
-        twiddle(w, a.tw, e, a.logn, a.tw_shift, a.inverse);
-        fmul(x, x, w);
-      }
+      E w;
+      neg = twiddle_z(w, a.tw, (col * k) << a.logr, a.logn, a.tw_shift, a.inverse);
+      zmul(x, x, w);
       go = (((r << a.s) + k) << a.logc) | col;
     } else {
       go = (rho0 + c) + ((uint64_t)k << a.logr);
-      if (a.has_ninv) fmul(x, x, ninv);
+      E f = ninv;
       if (a.eta_post) {
-        Fe<F> e;
-        eta_pow(e, a.eta_post, go);
-        fmul(x, x, e);
+        E e;
+        eta_pow_z(e, a.eta_post, go);
+        zmul(f, f, e);
       }
+      zmul(x, x, f);
     }
-    fstore(a.out + go * F::N, x);
+    zcanon(x, x);
+    if (neg) zneg_canon(x, x);
+    zpack(a.out + go * 8, x);
   }
 }
 
 template <class F>
 __global__ __launch_bounds__(kNttThreads) void ntt_pass_kernel(PassArgs a) {
   extern __shared__ uint4 lds_raw[];
-  ntt_pass_body<F>(a, blockIdx.x, reinterpret_cast<uint32_t *>(lds_raw));
+  ntt_pass_body<F>(a, blockIdx.x, reinterpret_cast<int32_t *>(lds_raw));
 }
 
 // batched pass: item i owns blocks [prefix[i], prefix[i+1]) of the launch
@@ -213,7 +239,7 @@ __global__ __launch_bounds__(kNttThreads) void ntt_batch_pass_kernel(const PassA
     else hi = mid;
   }
   const PassArgs a = items[lo];
-  ntt_pass_body<F>(a, b - prefix[lo], reinterpret_cast<uint32_t *>(lds_raw));
+  ntt_pass_body<F>(a, b - prefix[lo], reinterpret_cast<int32_t *>(lds_raw));
 }
 
 // ---------------------------------------------------------------------------------------
@@ -356,6 +382,48 @@ bool ntt_generator_ok(int curve, int m, const uint64_t *gen) {
   return zkh::is_eq(f, g, mone);
 }
 
+// table entries x R (canonical, 8 words) -> x R' (canonical, 8 words), in place
+template <class F>
+__global__ void tab_to_internal_kernel(uint32_t *tab, uint64_t cnt) {
+  using Z = typename ZOf<F>::T;
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= cnt) return;
+  Fz<Z> a, k;
+  zunpack(a, tab + e * 8);
+  zset_const(k, Z::K_IN);
+  zmul(a, a, k);
+  zcanon(a, a);
+  zpack(tab + e * 8, a);
+}
+
+template <class F>
+static int tab_to_internal(uint32_t *tab, uint64_t cnt, hipStream_t st) {
+  hipLaunchKernelGGL(tab_to_internal_kernel<F>, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, st, tab, cnt);
+  ZK_HIP(hipGetLastError());
+  return ZK_OK;
+}
+
+// internal form (x R', canonical words) of a host Montgomery element x R
+template <class F>
+static void internal_words(uint32_t out[8], const zkh::Fe &x) {
+  using Z = typename ZOf<F>::T;
+  const Fe<F> d = to_dev<F>(x);
+  Fz<Z> a, k;
+  zunpack(a, d.w);
+  zset_const(k, Z::K_IN);
+  zmul(a, a, k);
+  zcanon(a, a);
+  zpack(out, a);
+}
+
+// last-pass factor: N^-1 for the inverse, 1 for the forward transform (internal form)
+template <class F>
+static void last_factor(uint32_t out[8], int curve, int m, bool inverse) {
+  const zkh::FieldDesc &f = fr_desc(curve);
+  const zkh::Fe v = inverse ? zkh::inv(f, zkh::from_u64(f, (uint64_t)1 << m)) : zkh::one(f);
+  internal_words<F>(out, v);
+}
+
 template <class F>
 static int build_twiddles(int curve, int m, const zkh::Fe &g, DevBuf &buf, hipStream_t st) {
   const uint64_t cnt = m >= 1 ? (1ull << (m - 1)) : 1;
@@ -371,6 +439,7 @@ static int build_twiddles(int curve, int m, const zkh::Fe &g, DevBuf &buf, hipSt
   hipLaunchKernelGGL(twiddle_table_kernel<F>, dim3((cnt + 255) / 256), dim3(256), 0, st, buf.as<uint32_t>(),
                      lo.as<uint32_t>(), hi.as<uint32_t>(), cnt);
   ZK_HIP(hipGetLastError());
+  ZK_TRY(tab_to_internal<F>(buf.as<uint32_t>(), cnt, st));  // the pass kernels read w R'
   ZK_HIP(hipStreamSynchronize(st));  // lo/hi are released on return
   (void)curve;
   return ZK_OK;
@@ -418,7 +487,7 @@ static int build_eta_tables(const zkh::Fe &eta, DevBuf &buf, hipStream_t st) {
   hipLaunchKernelGGL(pow_table_kernel<F>, dim3(8), dim3(256), 0, st, b + 2048 * F::N, e, 2048ull, 2048);
   hipLaunchKernelGGL(pow_table_kernel<F>, dim3(8), dim3(256), 0, st, b + 4096 * F::N, e, 1ull << 22, 2048);
   ZK_HIP(hipGetLastError());
-  return ZK_OK;
+  return tab_to_internal<F>(b, 3 * 2048, st);
 }
 
 NttPlan ntt_plan(int m, int smax, int tile_log) {
@@ -441,7 +510,7 @@ NttPlan ntt_plan(int m, int smax, int tile_log) {
 template <class F>
 static int ntt_fast(const NttPlan &plan, const uint32_t *d_in, uint32_t *d_out, uint32_t *d_tmp,
                     const uint32_t *tw, int tw_shift, int inverse, const uint32_t *eta_pre,
-                    const uint32_t *eta_post, const Fe<F> *ninv, hipStream_t st) {
+                    const uint32_t *eta_post, const uint32_t *last_fac, hipStream_t st) {
   const int m = plan.m;
   int logr = 0;
   for (int p = 0; p < plan.npass; p++) {
@@ -460,9 +529,9 @@ static int ntt_fast(const NttPlan &plan, const uint32_t *d_in, uint32_t *d_out, 
     a.inverse = inverse;
     a.eta_pre = p == 0 ? eta_pre : nullptr;
     a.eta_post = last ? eta_post : nullptr;
-    if (last && ninv) {
+    if (last) {
       a.has_ninv = 1;
-      for (int j = 0; j < F::N; j++) a.ninv[j] = ninv->w[j];
+      for (int j = 0; j < 8; j++) a.ninv[j] = last_fac[j];
     }
     int tl = plan.tile_log - a.s;
     if (tl < 0) tl = 0;
@@ -472,7 +541,7 @@ static int ntt_fast(const NttPlan &plan, const uint32_t *d_in, uint32_t *d_out, 
     a.nsplit = p;
     for (int i = 0; i < p; i++) a.split[i] = plan.s[i];
     const uint64_t groups = last ? (1ull << (logr - tl)) : ((1ull << logr) << (a.logc - tl));
-    const size_t lds = ((size_t)1 << (a.s + tl)) * F::N * 4;
+    const size_t lds = ((size_t)1 << (a.s + tl)) * kZW * 4;
     hipLaunchKernelGGL(ntt_pass_kernel<F>, dim3((unsigned)groups), dim3(kNttThreads), lds, st, a);
     ZK_HIP(hipGetLastError());
     logr += a.s;
@@ -589,14 +658,14 @@ static int ntt_device_t(int curve, int m, const uint64_t *gen, const uint64_t *e
   if (m >= 1) ZK_TRY(get_twiddles<F>(curve, m, gen, &tw, &tw_shift, st));
   DevBuf etab;
   if (eta) ZK_TRY(build_eta_tables<F>(zkh::load(f, eta), etab, st));
-  Fe<F> ninv;
-  if (inverse) ninv = to_dev<F>(zkh::inv(f, zkh::from_u64(f, N)));
+  uint32_t lastf[8];
+  last_factor<F>(lastf, curve, m, inverse);
   const NttTuning tu = tune ? *tune : ntt_tuning_for(curve, m);
   NttPlan plan = ntt_plan(m, tu.smax, tu.tile_log);
   const uint32_t *ep = (eta && !inverse) ? etab.as<uint32_t>() : nullptr;
   const uint32_t *eo = (eta && inverse) ? etab.as<uint32_t>() : nullptr;
   ZK_TRY(ntt_fast<F>(plan, d_in, d_out, tmp.as<uint32_t>(), tw, tw_shift, inverse, ep, eo,
-                     inverse ? &ninv : nullptr, st));
+                     lastf, st));
   if (eta) ZK_HIP(hipStreamSynchronize(st));  // eta tables are released on return
   ZK_HIP(hipStreamSynchronize(st));           // tmp is released on return
   return ZK_OK;
@@ -691,9 +760,11 @@ static int ntt_batch_create_t(int curve, int njobs, const NttBatchJob *jobs, Ntt
       hipLaunchKernelGGL(pow_table_kernel<F>, dim3(8), dim3(256), 0, st, e, ed, 1ull, 2048);
       hipLaunchKernelGGL(pow_table_kernel<F>, dim3(8), dim3(256), 0, st, e + 2048 * F::N, ed, 2048ull, 2048);
       hipLaunchKernelGGL(pow_table_kernel<F>, dim3(8), dim3(256), 0, st, e + 4096 * F::N, ed, 1ull << 22, 2048);
+      ZK_TRY(tab_to_internal<F>(e, 3 * 2048, st));
       etab = e;
     }
-    Fe<F> ninv = to_dev<F>(zkh::inv(f, zkh::from_u64(f, (uint64_t)1 << m)));
+    uint32_t lastf[8];
+    last_factor<F>(lastf, curve, m, inverse);
     int logr = 0;
     for (int p = 0; p < pj.npass; p++) {
       PassArgs a;
@@ -711,9 +782,9 @@ static int ntt_batch_create_t(int curve, int njobs, const NttBatchJob *jobs, Ntt
       a.inverse = inverse;
       a.eta_pre = (p == 0 && !inverse) ? etab : nullptr;
       a.eta_post = (last && inverse) ? etab : nullptr;
-      if (last && inverse) {
+      if (last) {
         a.has_ninv = 1;
-        for (int k = 0; k < F::N; k++) a.ninv[k] = ninv.w[k];
+        for (int k = 0; k < 8; k++) a.ninv[k] = lastf[k];
       }
       int tl = pj.tile_log - a.s;
       if (tl < 0) tl = 0;
@@ -751,7 +822,7 @@ static int ntt_batch_create_t(int curve, int njobs, const NttBatchJob *jobs, Ntt
       h_items.push_back(rounds[r][i]);
       h_prefix.push_back(acc);
       acc += blocks[r][i];
-      const size_t l = ((size_t)1 << (rounds[r][i].s + rounds[r][i].logt)) * F::N * 4;
+      const size_t l = ((size_t)1 << (rounds[r][i].s + rounds[r][i].logt)) * kZW * 4;
       if (l > lds) lds = l;
     }
     h_prefix.push_back(acc);
