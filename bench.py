@@ -314,6 +314,43 @@ def bench_learned_msm(njobs=24, seed=3, nstreams=4):
             "learned_msm_bit_identical": bool(same)}
 
 
+def bench_g1_fft(m=12, cpu_m=7, seed=6):
+    """BN254 G1 FFT (bn128_G1_proj_fft_forward / _inverse) of 2^m points k_i G given as
+    projective (x, y, 1), on the 2^m subgroup generator; the inverse of the forward transform
+    must return the input bit for bit.  CPU: the reference's own C on 2^cpu_m of the points."""
+    N = 1 << m
+    rng = np.random.default_rng(seed)
+    K = uniform_scalars(rng, N, BN_R)
+    dK = zkgpu.DeviceArray.from_host(K)
+    aff = zkgpu.scalar_mul_gen(0, dK, N).to_host((N, 8))
+    one = np.array([((1 << 256) % BN_P >> (64 * i)) & 0xFFFFFFFFFFFFFFFF for i in range(4)], np.uint64)
+    P = np.concatenate([aff, np.tile(one, (N, 1))], axis=1)
+    g = zkgpu.fft_generator(0, m)
+    dP, dY, dB = zkgpu.DeviceArray.from_host(P), zkgpu.DeviceArray(P.size), zkgpu.DeviceArray(P.size)
+    fwd = lambda: zkgpu.g1_fft_device(0, m, g, dP, dY)
+    inv = lambda: zkgpu.g1_fft_device(0, m, g, dY, dB, inverse=True)
+    tf = time_steps(fwd, 2, 1)
+    ti = time_steps(inv, 2, 1)
+    ok = bool((dB.to_host(P.shape) == P).all())
+    out = {"g1_fft_bn254_log": m, "g1_fft_fwd_ms": round(tf * 1e3, 2), "g1_fft_inv_ms": round(ti * 1e3, 2),
+           "g1_fft_fwd_mpts": round(N / tf / 1e6, 4), "g1_fft_roundtrip_ok": ok}
+    ref_so = REPO / "oracle" / "_ref" / "libzkref.so"
+    if ref_so.exists():
+        lib = ctypes.CDLL(str(ref_so))
+        n = 1 << cpu_m
+        u64p = ctypes.POINTER(ctypes.c_uint64)
+        src = np.ascontiguousarray(P[:n])
+        tgt = np.zeros_like(src)
+        gc = zkgpu.fft_generator(0, cpu_m)
+        t0 = time.perf_counter()
+        lib.bn128_G1_proj_fft_forward(cpu_m, gc.ctypes.data_as(u64p), src.ctypes.data_as(u64p), tgt.ctypes.data_as(u64p))
+        dt = time.perf_counter() - t0
+        gpu = zkgpu.g1_fft(0, cpu_m, gc, src)
+        out["g1_fft_cpu_ref"] = {"log": cpu_m, "ms": round(dt * 1e3, 1), "cores": 1,
+                                 "mpts": round(n / dt / 1e6, 6), "matches_gpu": bool((gpu == tgt).all())}
+    return out
+
+
 def bench_serve(njobs=64, qps_list=(100, 400), seed=5):
     """KZG commits on BLS12-381: iNTT of n evaluations (n = 2^14..2^21, log-uniform) then an MSM
     against a resident SRS of 2^21 points tau^i G; Poisson arrivals at each stated QPS; served
@@ -354,8 +391,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--cpu-sample-log", type=int, default=18)
-    ap.add_argument("--leg", choices=["all", "msm", "ntt", "copy"], default="all",
-                    help="profiling runs: only the BN254 MSM, only the 2^22 NTT pair, or the HBM copy kernel")
+    ap.add_argument("--leg", choices=["all", "msm", "ntt", "copy", "nttbatch", "learned", "g1fft"], default="all",
+                    help="profiling runs: only the BN254 MSM, the 2^22 NTT pair, the HBM copy kernel, the "
+                         "batched NTTs or the learned MSM mix")
     a = ap.parse_args()
     if a.leg == "msm":
         a.no_extra = a.no_cpu = True
@@ -376,8 +414,10 @@ def main():
         bw = zkgpu.microbench(zkgpu.MB_COPY)
         log(f"copy kernel: 1 GiB read + 1 GiB written per launch, {bw / 1e12:.2f} TB/s")
         return
-    if a.leg == "ntt":
-        for k, v in bench_ntt_pair(a.steps, a.warmup).items():
+    if a.leg in ("ntt", "nttbatch", "learned", "g1fft"):
+        fn = {"ntt": lambda: bench_ntt_pair(a.steps, a.warmup), "nttbatch": bench_ntt_batch,
+              "learned": bench_learned_msm, "g1fft": bench_g1_fft}[a.leg]
+        for k, v in fn().items():
             log(f"{k}: {v}")
         return
     peak_imul = zkgpu.microbench(zkgpu.MB_IMUL)
@@ -426,6 +466,8 @@ def main():
             f"roundtrip {extra['bn254_ntt_2p22_roundtrip_ok']}")
         extra.update(bench_ntt_batch())
         log(f"ntt batch: {extra['ntt_batch_ms']} ms batched vs {extra['ntt_plain_ms']} ms plain, identical {extra['ntt_batch_bit_identical_to_plain']}")
+        extra.update(bench_g1_fft())
+        log(f"g1 fft: {[(k, v) for k, v in extra.items() if k.startswith('g1_fft')]}")
         extra.update(bench_learned_msm())
         log(f"learned msm mix: {[(k, v) for k, v in extra.items() if k.startswith('learned_msm')]}")
         extra.update(bench_serve())

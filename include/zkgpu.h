@@ -85,6 +85,17 @@ void bls12_381_poly_mont_ntt_forward_shifted(const uint64_t *eta, int m, const u
 void bls12_381_poly_mont_ntt_inverse_shifted(const uint64_t *eta, int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt);
 void bls12_381_poly_mont_ntt_forward_asymmetric(int m_src, int m_tgt, const uint64_t *gen_src, const uint64_t *gen_tgt, const uint64_t *src, uint64_t *tgt);
 
+/* FFT of 2^m G1 points in projective Montgomery form (X:Y:Z), twiddles gen^j (Montgomery Fr);
+ * the inverse is the reference's DIF network with the halvings, i.e. N^-1 sum gen^-ij.
+ * Outputs are normalised (X/Z, Y/Z, 1) or (0, 1, 0), as the reference normalises them. */
+/* lib/cbits/curves/g1/proj/bn128_G1_proj.h:54 (impl bn128_G1_proj.c:791-866) */
+void bn128_G1_proj_fft_forward(int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt);
+/* bn128_G1_proj.h:55 (impl bn128_G1_proj.c:871-902) */
+void bn128_G1_proj_fft_inverse(int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt);
+/* lib/cbits/curves/g1/proj/bls12_381_G1_proj.h:54-55 */
+void bls12_381_G1_proj_fft_forward(int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt);
+void bls12_381_G1_proj_fft_inverse(int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt);
+
 /* =========================================================================================
  * Part 2: extension API (no reference counterpart)
  * ========================================================================================= */
@@ -114,7 +125,8 @@ typedef struct {
   int seg_log;       /* log2 of sorted entries per accumulation thread */
   int policy;        /* ZKG_SCHED_* */
   int fix_limit;     /* buckets spanning more segments are summed by a pairwise tree (0 = 16) */
-  int reserved[4];
+  int prefetch;      /* accumulation loads the next point early: 0 = default (off), 1 = off, 2 = on */
+  int reserved[3];
 } zkg_msm_schedule;
 
 /* Statistics of a scalar vector (standard form after conversion). */
@@ -182,6 +194,9 @@ int zkg_fill_random(int curve, int field, size_t n, uint64_t seed, uint64_t *d_o
 /* batch fixed-base scalar multiplication k_i * G of the standard generator, affine Montgomery
  * output (input generation for tests and benchmarks) */
 int zkg_g1_scalar_mul_gen(int curve, int n, const uint64_t *d_scalars, uint64_t *d_out_affine);
+
+/* G1 FFT on device buffers (2^m projective points in, normalised projective points out) */
+int zkg_g1_fft(int curve, int m, const uint64_t *gen, int inverse, const uint64_t *d_src, uint64_t *d_tgt);
 
 /* [tau^i G]_{i<n} for tau in standard form (examples/KZG.hs:59-62), affine Montgomery output */
 int zkg_srs_generate(int curve, int n, const uint64_t *tau_std, uint64_t *d_out_affine);

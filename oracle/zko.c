@@ -16,6 +16,7 @@
  *   NTT / iNTT          lib/cbits/curves/poly/mont/bn128_poly_mont.c:418-473 (recursive DIT),
  *                       :477-489 (shifted), :492-557 (asymmetric), :565-638 (inverse)
  *   FFT domain          lib/src/ZK/Algebra/Class/FFT.hs:60-68 (g_m = gen_max^(2^(M-m)))
+ *   G1 FFT              bn128_G1_proj.c:791-902 (recursive DIT / DIF over points)
  * The BLS12-381 files are the same templates with 6-limb Fp.
  *
  * Departures, each of which leaves every in-contract result unchanged:
@@ -587,4 +588,98 @@ void zko_fft_generator(int cv, int m, uint64_t *gen) {
   if (m < 0 || m > M) { memset(gen, 0, 8 * f->nl); return; }
   for (int i = 0; i < M - m; i++) f_sqr(f, g, g);
   cpy(f, gen, g);
+}
+
+/* ---------------- G1 FFT (bn128_G1_proj.c:791-902) ----------------
+ * Forward: recursive radix-2 DIT over projective points with Fr twiddles gen^j (Montgomery),
+ * each butterfly (u + g^j v, u - g^j v) computed through a scalar multiplication.  Inverse:
+ * the mirrored DIF recursion with (x + y)/2 and (x - y) g^-j / 2 at every level.  Both
+ * normalise every output (X/Z, Y/Z, 1), infinity (0, 1, 0). */
+static void proj_neg(const curve_t *C, const uint64_t *P, uint64_t *R) {
+  const field_t *f = C->fp; int n = f->nl;
+  cpy(f, PX(R), PX(P)); f_neg(f, PY(P), PY(R)); cpy(f, PZ(R), PZ(P));
+}
+static void proj_sub(const curve_t *C, const uint64_t *P, const uint64_t *Q, uint64_t *R) {
+  uint64_t t[3 * MAXL];
+  proj_neg(C, Q, t);
+  proj_add(C, P, t, R);
+}
+/* scalar in Montgomery Fr form (bn128_G1_proj_scl_Fr_mont) */
+static void proj_scl_mont(int cv, const uint64_t *k, const uint64_t *P, uint64_t *R) {
+  uint64_t ks[4];
+  f_to_std(F(cv, ZKO_FR), k, ks);
+  zko_g1_proj_scl(cv, ks, 4, P, R);
+}
+
+static void g1_fft_fwd_rec(int cv, int m, size_t stride, const uint64_t *gen, const uint64_t *src, uint64_t *buf,
+                           uint64_t *tgt) {
+  const curve_t *C = &CURVES[cv]; const field_t *fr = F(cv, ZKO_FR); int n = C->fp->nl;
+  const size_t GN = 3 * (size_t)n;
+  if (m == 0) { memcpy(tgt, src, 8 * GN); return; }
+  if (m == 1) {
+    proj_add(C, src, src + stride * GN, tgt);
+    proj_sub(C, src, src + stride * GN, tgt + GN);
+    return;
+  }
+  size_t N = (size_t)1 << m, half = N >> 1;
+  uint64_t gpow[4];
+  f_mul(fr, gen, gen, gpow);
+  g1_fft_fwd_rec(cv, m - 1, stride << 1, gpow, src, buf + N * GN, buf);
+  g1_fft_fwd_rec(cv, m - 1, stride << 1, gpow, src + stride * GN, buf + N * GN, buf + half * GN);
+  cpy(fr, gpow, fr->one);
+  for (size_t j = 0; j < half; j++) {
+    proj_scl_mont(cv, gpow, buf + (j + half) * GN, tgt + j * GN);
+    proj_neg(C, tgt + j * GN, tgt + (j + half) * GN);
+    proj_add(C, tgt + j * GN, buf + j * GN, tgt + j * GN);
+    proj_add(C, tgt + (j + half) * GN, buf + j * GN, tgt + (j + half) * GN);
+    f_mul(fr, gpow, gen, gpow);
+  }
+}
+
+static void g1_fft_inv_rec(int cv, int m, size_t tstride, const uint64_t *gen, const uint64_t *src, uint64_t *buf,
+                           uint64_t *tgt) {
+  const curve_t *C = &CURVES[cv]; const field_t *fr = F(cv, ZKO_FR); int n = C->fp->nl;
+  const size_t GN = 3 * (size_t)n;
+  uint64_t half2[4];  /* 1/2 in Montgomery form */
+  f_div2(fr, fr->one, half2);
+  if (m == 0) { memcpy(tgt, src, 8 * GN); return; }
+  if (m == 1) {
+    proj_add(C, src, src + GN, tgt);
+    proj_sub(C, src, src + GN, tgt + tstride * GN);
+    proj_scl_mont(cv, half2, tgt, tgt);
+    proj_scl_mont(cv, half2, tgt + tstride * GN, tgt + tstride * GN);
+    return;
+  }
+  size_t N = (size_t)1 << m, half = N >> 1;
+  uint64_t ginv[4], gpow[4];
+  f_inv(fr, gen, ginv);
+  cpy(fr, gpow, half2);
+  for (size_t j = 0; j < half; j++) {
+    proj_add(C, src + j * GN, src + (j + half) * GN, buf + j * GN);
+    proj_sub(C, src + j * GN, src + (j + half) * GN, buf + (j + half) * GN);
+    proj_scl_mont(cv, half2, buf + j * GN, buf + j * GN);
+    proj_scl_mont(cv, gpow, buf + (j + half) * GN, buf + (j + half) * GN);
+    f_mul(fr, gpow, ginv, gpow);
+  }
+  f_mul(fr, gen, gen, gpow);
+  g1_fft_inv_rec(cv, m - 1, tstride << 1, gpow, buf, buf + N * GN, tgt);
+  g1_fft_inv_rec(cv, m - 1, tstride << 1, gpow, buf + half * GN, buf + N * GN, tgt + tstride * GN);
+}
+
+static void g1_fft(int cv, int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt, int inverse) {
+  const curve_t *C = &CURVES[cv]; int n = C->fp->nl;
+  const size_t GN = 3 * (size_t)n, N = (size_t)1 << m;
+  uint64_t *buf = (uint64_t *)malloc(8 * GN * 2 * N);
+  if (!buf) abort();
+  if (inverse) g1_fft_inv_rec(cv, m, 1, gen, src, buf, tgt);
+  else g1_fft_fwd_rec(cv, m, 1, gen, src, buf, tgt);
+  free(buf);
+  for (size_t i = 0; i < N; i++) zko_g1_proj_normalize(cv, tgt + i * GN, tgt + i * GN);
+}
+
+void zko_g1_fft_forward(int cv, int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt) {
+  g1_fft(cv, m, gen, src, tgt, 0);
+}
+void zko_g1_fft_inverse(int cv, int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt) {
+  g1_fft(cv, m, gen, src, tgt, 1);
 }

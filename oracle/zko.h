@@ -66,6 +66,10 @@ void zko_ntt_inverse_shifted(int curve, const uint64_t *eta, int m, const uint64
                              const uint64_t *src, uint64_t *tgt);
 void zko_ntt_forward_asymmetric(int curve, int m_src, int m_tgt, const uint64_t *gen_src,
                                 const uint64_t *gen_tgt, const uint64_t *src, uint64_t *tgt);
+/* FFT of G1 points (projective, 2^m of them) with Fr twiddles gen^j; outputs normalised
+ * (bn128_G1_proj.c:791-902) */
+void zko_g1_fft_forward(int curve, int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt);
+void zko_g1_fft_inverse(int curve, int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt);
 /* generator of the size-2^m FFT subgroup, Montgomery form (Class/FFT.hs:60-68) */
 void zko_fft_generator(int curve, int m, uint64_t *gen);
 

@@ -117,6 +117,27 @@ class Oracle:
         self.lib.zko_g1_proj_to_affine(self.c.id, ptr(P), ptr(out))
         return out
 
+    def proj_points(self, ks, zs=None):
+        """Projective Montgomery points k_i G with a non-trivial Z (X z, Y z, z), infinity (0:1:0)."""
+        L = self.lp
+        aff = self.points_from_scalars(ks)
+        out = np.zeros((len(ks), 3 * L), np.uint64)
+        one = np.array(to_limbs(self.c.fp_to_mont(1), L), np.uint64)
+        for i, k in enumerate(ks):
+            if k == 0:
+                out[i, L:2 * L] = one
+                continue
+            z = np.array(to_limbs(self.c.fp_to_mont((zs[i] if zs else 7 + i) % self.c.p), L), np.uint64)
+            for j, w in enumerate((aff[i, :L], aff[i, L:], one)):
+                self.lib.zko_f_mul(self.c.id, 0, ptr(np.ascontiguousarray(w)), ptr(z), ptr(out[i, j * L:(j + 1) * L]))
+        return out
+
+    def g1_fft(self, m, gen, src, inverse=False):
+        out = np.zeros_like(src)
+        fn = self.lib.zko_g1_fft_inverse if inverse else self.lib.zko_g1_fft_forward
+        fn(self.c.id, m, ptr(np.ascontiguousarray(gen)), ptr(np.ascontiguousarray(src)), ptr(out))
+        return out
+
 
 class Ref:
     """The reference's own entry points (names from lib/cbits/.../*.h)."""
@@ -167,4 +188,10 @@ class Ref:
         n = 4 if field == 1 else self.lp
         out = np.zeros(n, np.uint64)
         self.fn("Fr_mont_mul" if field == 1 else "Fp_mont_mul")(ptr(a), ptr(b), ptr(out))
+        return out
+
+    def g1_fft(self, m, gen, src, inverse=False):
+        out = np.zeros_like(src)
+        self.fn("G1_proj_fft_inverse" if inverse else "G1_proj_fft_forward")(
+            m, ptr(np.ascontiguousarray(gen)), ptr(np.ascontiguousarray(src)), ptr(out))
         return out

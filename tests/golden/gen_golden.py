@@ -44,7 +44,7 @@ def main():
     assert ref() is not None, "build the reference first: make -C oracle ref"
     out = {}
     manifest = {"generator": "tests/golden/gen_golden.py", "reference": "oracle/_ref/libzkref.so (lib/cbits sources, clang -O2)",
-                "msm": [], "ntt": [], "large": []}
+                "msm": [], "ntt": [], "large": [], "g1fft": []}
     for C in (BN, BLS):
         R, O = Ref(C), Oracle(C)
         for n in (1, 3, 17, 256, 1000):
@@ -79,6 +79,22 @@ def main():
             X = out[f"{C.name}_ntt_{ms}_src"]
             gs, gt = out[f"{C.name}_ntt_{ms}_gen"], np.array(to_limbs(C.fr_to_mont(C.fft_root(mt)), 4), np.uint64)
             out[f"{C.name}_asym_{ms}_{mt}"] = R.ntt_asym(ms, mt, gs, gt, X)
+        # G1 FFT (bn128_G1_proj.c:791-902): projective points k_i G with Z != 1 and an infinity
+        for m in (0, 1, 2, 3, 5):
+            ks = random_scalars(8000 + m + 100 * C.id, 1 << m, C.r)
+            if m >= 2:
+                ks[2] = 0
+            P = O.proj_points(ks)
+            key = f"{C.name}_g1fft_{m}"
+            out[key + "_src"] = P
+            gens = {"root": np.array(to_limbs(C.fr_to_mont(C.fft_root(m)), 4), np.uint64)}
+            if m == 3:
+                gens["other"] = np.array(to_limbs(C.fr_to_mont(987654321), 4), np.uint64)
+            for gname, g in gens.items():
+                out[f"{key}_{gname}_gen"] = g
+                out[f"{key}_{gname}_fwd"] = R.g1_fft(m, g, P)
+                out[f"{key}_{gname}_inv"] = R.g1_fft(m, g, P, inverse=True)
+                manifest["g1fft"].append(f"{key}_{gname}")
         # large NTTs: output digests only (inputs from seeded_fr_vector, taken as Montgomery words)
         for m in (16, 20):
             X = seeded_fr_vector(9000 + m + 100 * C.id, 1 << m, C.r)

@@ -20,6 +20,17 @@ static void run_op(int op, int n, const int32_t *a, const int32_t *b, int32_t *o
       case 3: r = x; znorm(r); break;
       case 4: zcanon(r, x); break;
       case 5: zset_zero(r); r.l[0] = zis_zero_mod(x) ? 1 : 0; break;
+      case 6: zsqr(r, x); break;
+      case 7: zmul_sub(r, x, y, y, x); break;  // x y - y x = 0 exercises the signs; see also op 8
+      case 8: {  // x y - (x + 1) (y - 1) with normalised operands built from x and y
+        Fz<P> x1 = x, y1 = y;
+        x1.l[0] += 1;
+        y1.l[0] -= 1;
+        znorm(x1);
+        znorm(y1);
+        zmul_sub(r, x, y, x1, y1);
+        break;
+      }
       default: r = x; break;
     }
     memcpy(out + k * P::N, r.l, sizeof r.l);

@@ -137,11 +137,11 @@ def test_candidates_contain_both_analytic_picks(zkh, curve, logn):
 
 
 def test_plain_pick_matches_op_count_model(zkh):
-    # the plain pick minimises W * (10 n + 28 * 2^(c-1)) with W = ceil(255 / c) over c <= 17
-    # (windows sorted in LDS); its windows are all c wide
+    # the plain pick minimises W * (10 n + 28 * 2^(c-1)) with W = ceil(255 / c) over c <= 16
+    # (windows whose bucket bases fit LDS); its windows are all c wide
     for logn in range(8, 24, 3):
         n = 1 << logn
-        best = min(range(2, 18), key=lambda c: (math.ceil(255 / c) * (10 * n + 28 * 2 ** (c - 1)), c))
+        best = min(range(2, 17), key=lambda c: (math.ceil(255 / c) * (10 * n + 28 * 2 ** (c - 1)), c))
         assert zkh.msm_pick(0, zkh.stats_from(n, 0, 0, 254), zkh.SCHED_PLAIN)[0] == best
 
 

@@ -102,6 +102,45 @@ def test_zmul_matches_integers_at_bounds(h, field):
 
 
 @pytest.mark.parametrize("field", [0, 1, 2, 3], ids=["bn-fp", "bn-fr", "bls-fp", "bls-fr"])
+def test_zsqr_equals_zmul(h, field):
+    """The squaring kernel sums each off-diagonal product once against a doubled limb; its
+    column sums are zmul(a, a)'s, so the outputs agree limb for limb."""
+    p, bits, N, _ = FIELDS[field]
+    Rinv = pow(1 << (bits * N), -1, p)
+    rng = splitmix64(300 + field)
+    bound = 8 if field != 3 else 4
+    xs = sample_values(p, rng, 500, bound)
+    A = arr([to_limbs(x, bits, N) for x in xs])
+    S = op(h, field, 6, A)
+    M = op(h, field, 0, A, A)
+    assert (S == M).all()
+    for i, x in enumerate(xs):
+        assert value(S[i], bits) % p == x * x * Rinv % p
+
+
+@pytest.mark.parametrize("field", [0, 1, 2, 3], ids=["bn-fp", "bn-fr", "bls-fp", "bls-fr"])
+def test_zmul_sub_matches_integers(h, field):
+    """(a b - c d) / R' with one reduction: exact modulo p, normalised, within (-2p, 2p)."""
+    p, bits, N, _ = FIELDS[field]
+    Rinv = pow(1 << (bits * N), -1, p)
+    rng = splitmix64(400 + field)
+    bound = 8 if field != 3 else 4
+    xs = sample_values(p, rng, 400, bound - 1)
+    ys = sample_values(p, rng, 400, bound - 1)[::-1]
+    A = arr([to_limbs(x, bits, N) for x in xs])
+    B = arr([to_limbs(y, bits, N) for y in ys])
+    Z = op(h, field, 7, A, B)
+    for i in range(len(xs)):
+        assert value(Z[i], bits) % p == 0
+        assert is_normalised(Z[i], bits)
+    R = op(h, field, 8, A, B)
+    for i, (x, y) in enumerate(zip(xs, ys)):
+        got = value(R[i], bits)
+        assert got % p == (x * y - (x + 1) * (y - 1)) * Rinv % p, i
+        assert is_normalised(R[i], bits) and -2 * p < got < 2 * p
+
+
+@pytest.mark.parametrize("field", [0, 1, 2, 3], ids=["bn-fp", "bn-fr", "bls-fp", "bls-fr"])
 def test_zero_test_norm_and_canon(h, field):
     p, bits, N, _ = FIELDS[field]
     rng = splitmix64(7 + field)

@@ -70,6 +70,37 @@ def test_oracle_ntt_vs_golden(golden, manifest, C):
 
 
 @pytest.mark.parametrize("C", CURVES, ids=IDS)
+def test_oracle_g1_fft_vs_golden(golden, manifest, C):
+    o = Oracle(C)
+    keys = [k for k in manifest["g1fft"] if k.startswith(C.name + "_")]
+    assert keys
+    for key in keys:
+        base = key.rsplit("_", 1)[0]
+        m = int(base.rsplit("_", 1)[1])
+        P, g = golden[base + "_src"], golden[key + "_gen"]
+        assert (o.g1_fft(m, g, P) == golden[key + "_fwd"]).all(), key
+        assert (o.g1_fft(m, g, P, inverse=True) == golden[key + "_inv"]).all(), key
+
+
+@pytest.mark.parametrize("C", CURVES, ids=IDS)
+def test_oracle_g1_fft_closed_form_and_roundtrip(C):
+    """FFT of k_i G is (NTT(k))_j G (linearity in the scalar), and the inverse undoes it."""
+    o = Oracle(C)
+    m = 3
+    ks = random_scalars(77 + C.id, 1 << m, C.r)
+    P = o.proj_points(ks)
+    g = o.fft_generator(m)
+    Y = o.g1_fft(m, g, P)
+    w = C.fft_root(m)
+    for j in range(1 << m):
+        e = sum(k * pow(w, i * j, C.r) for i, k in enumerate(ks)) % C.r
+        assert C.words_to_affine(Y[j][:2 * C.lp]) == C.mul(e, C.g)
+    back = o.g1_fft(m, g, Y, inverse=True)
+    for i, k in enumerate(ks):
+        assert C.words_to_affine(back[i][:2 * C.lp]) == C.mul(k, C.g)
+
+
+@pytest.mark.parametrize("C", CURVES, ids=IDS)
 def test_oracle_large_ntt_vs_golden_digest(manifest, C):
     o = Oracle(C)
     for e in manifest["large"]:

@@ -47,6 +47,25 @@ int ntt_host(int curve, int m, const uint64_t *gen, const uint64_t *eta, int fla
 
 const zkh::FieldDesc &fr(int curve) { return curve == 0 ? zkh::BN254_FR : zkh::BLS381_FR; }
 
+// host-in / host-out G1 FFT (bn128_G1_proj.c:854-866, :890-902): 2^m projective points
+int g1_fft_host(int curve, int m, const uint64_t *gen, bool inverse, const uint64_t *src, uint64_t *tgt) {
+  if (m < 0 || m > 26) {
+    set_error("g1_fft: log size out of range");
+    return ZK_EINVAL;
+  }
+  Context *ctx;
+  ZK_TRY(with_ctx(&ctx));
+  std::lock_guard<std::recursive_mutex> lk(ctx->lock);
+  const size_t bytes = ((size_t)1 << m) * 3 * 8 * (curve == 0 ? 4 : 6);
+  DevBuf a, b;
+  ZK_TRY(a.alloc(bytes));
+  ZK_TRY(b.alloc(bytes));
+  ZK_TRY(copy_h2d(a.p, src, bytes, ctx->stream));
+  ZK_TRY(g1_fft_device(curve, m, gen, inverse, a.as<uint64_t>(), b.as<uint64_t>(), ctx->stream));
+  ZK_TRY(copy_d2h(tgt, b.p, bytes, ctx->stream));
+  return ZK_OK;
+}
+
 // bn128_poly_mont.c:492-557: K = 2^(m_tgt - m_src) shifted NTTs with eta = gen_tgt^k,
 // interleaved as tgt[i*K + k].  When gen_src = gen_tgt^K (subgroups of one domain) this is
 // one size-2^m_tgt NTT of the zero-padded input, which is what the device runs.
@@ -175,9 +194,28 @@ MSM_SYMBOL(bls12_381_G1_jac_MSM_mont_coeff_jac_out, 1, ZKG_MSM_JAC_OUT | ZKG_MSM
 MSM_SYMBOL(bls12_381_G1_jac_MSM_std_coeff_affine_out, 1, 0)
 MSM_SYMBOL(bls12_381_G1_jac_MSM_mont_coeff_affine_out, 1, ZKG_MSM_MONT)
 
+// ---------------- reference G1 FFT symbols ----------------
+#define G1_FFT_SYMBOLS(PFX, CID)                                                                       \
+  void PFX##_G1_proj_fft_forward(int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt) {     \
+    check(#PFX "_G1_proj_fft_forward", g1_fft_host(CID, m, gen, false, src, tgt));                    \
+  }                                                                                                    \
+  void PFX##_G1_proj_fft_inverse(int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt) {     \
+    check(#PFX "_G1_proj_fft_inverse", g1_fft_host(CID, m, gen, true, src, tgt));                     \
+  }
+
+G1_FFT_SYMBOLS(bn128, 0)
+G1_FFT_SYMBOLS(bls12_381, 1)
+
 // ---------------- extension API ----------------
 
 const char *zkg_last_error(void) { return last_error(); }
+
+int zkg_g1_fft(int curve, int m, const uint64_t *gen, int inverse, const uint64_t *d_src, uint64_t *d_tgt) {
+  Context *ctx;
+  ZK_TRY(with_ctx(&ctx));
+  std::lock_guard<std::recursive_mutex> lk(ctx->lock);
+  return g1_fft_device(curve, m, gen, inverse != 0, d_src, d_tgt, ctx->stream);
+}
 
 int zkg_device_info(int *num_cus, size_t *hbm_bytes) {
   Context *ctx;

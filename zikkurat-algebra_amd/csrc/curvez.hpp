@@ -87,10 +87,8 @@ ZK_HD void zxyzz_mdbl(XyzzZ<C> &r, const AffZ<C> &a) {
   znorm(t);  // X3 = M^2 - 2S
   r.x = t;
   zsub(t, s, r.x);
-  zmul(t, m, t);
-  zmul(s, w, a.y);
-  zsub(r.y, t, s);
-  znorm(r.y);
+  znorm(t);
+  zmul_sub(r.y, m, t, w, a.y);  // Y3 = M (S - X3) - W Y1, one reduction
   r.zz = v;
   r.zzz = w;
 }
@@ -119,10 +117,8 @@ ZK_HD void zxyzz_dbl(XyzzZ<C> &r, const XyzzZ<C> &p) {
   znorm(t);
   r.x = t;
   zsub(t, s, r.x);
-  zmul(t, m, t);
-  zmul(s, w, y1);
-  zsub(r.y, t, s);
-  znorm(r.y);
+  znorm(t);
+  zmul_sub(r.y, m, t, w, y1);
   zmul(r.zz, v, zz1);
   zmul(r.zzz, w, zzz1);
 }
@@ -161,11 +157,10 @@ ZK_HD void zxyzz_madd(XyzzZ<C> &acc, const AffZ<C> &a) {
   zsub(t, t, q);
   znorm(t);
   acc.x = t;              // X3 = R^2 - PPP - 2Q
-  zsub(t, q, acc.x);      // Q - X3 (one unnormalised operand)
-  zmul(t, R, t);
-  zmul(q, acc.y, ppp);
-  zsub(acc.y, t, q);
-  znorm(acc.y);           // Y3 = R (Q - X3) - Y1 PPP
+  zsub(t, q, acc.x);      // Q - X3
+  znorm(t);
+  zmul_sub(q, R, t, acc.y, ppp);
+  acc.y = q;              // Y3 = R (Q - X3) - Y1 PPP, one reduction
 }
 
 // r = p + q (XYZZ, complete)
@@ -212,13 +207,34 @@ ZK_HD void zxyzz_add(XyzzZ<C> &r, const XyzzZ<C> &p, const XyzzZ<C> &q) {
   znorm(t);
   F x3 = t;
   zsub(t, u1, x3);
-  zmul(t, R, t);
-  zmul(s1, s1, ppp);
-  zsub(r.y, t, s1);
-  znorm(r.y);
+  znorm(t);
+  zmul_sub(r.y, R, t, s1, ppp);
   r.x = x3;
   r.zz = zz;
   r.zzz = zzz;
+}
+
+// r = -p
+template <class C>
+ZK_HD void zxyzz_neg(XyzzZ<C> &r, const XyzzZ<C> &p) {
+  r = p;
+  zneg(r.y, p.y);
+  znorm(r.y);
+}
+
+// r = k p for a scalar k < 2^256 in standard form (8 little-endian u32 words): binary
+// double-and-add from the top set bit (any exact method gives the same point)
+template <class C>
+ZK_HD void zxyzz_scalar_mul(XyzzZ<C> &r, const XyzzZ<C> &p, const uint32_t (&k)[8]) {
+  zxyzz_set_inf(r);
+  int top = 255;
+  while (top >= 0 && !((k[top >> 5] >> (top & 31)) & 1u)) top--;
+  for (int i = top; i >= 0; i--) {
+    XyzzZ<C> t;
+    zxyzz_dbl(t, r);
+    if ((k[i >> 5] >> (i & 31)) & 1u) zxyzz_add(r, t, p);
+    else r = t;
+  }
 }
 
 // ---- device memory

@@ -173,9 +173,107 @@ ZK_HD void zmul(Fz<P> &r, const Fz<P> &a, const Fz<P> &b) {
   r.l[N - 1] = (int32_t)carry;
 }
 
+// (a b - c d) / R' with one reduction for four normalised operands: column sums hold 2N
+// products plus the N reduction terms, below 27 * 2^58 < 2^63; for |a|, |b|, |c|, |d| < 8p the
+// result lies in (-2p, 2p) for every field here (R' > 128 p).  Output normalised.
+template <class P>
+ZK_HD void zmul_sub(Fz<P> &r, const Fz<P> &a, const Fz<P> &b, const Fz<P> &c, const Fz<P> &d) {
+  constexpr int N = P::N, B = P::BITS;
+  constexpr uint32_t MASK = (1u << B) - 1;
+  uint32_t m[N];
+  int64_t carry = 0;
+#pragma unroll
+  for (int col = 0; col < 2 * N - 1; col++) {
+    const int lo = col < N ? 0 : col - N + 1;
+    const int hi = col < N ? col : N - 1;
+    int64_t s0 = carry, s1 = 0;
+#pragma unroll
+    for (int i = lo; i <= hi; i++) {
+      s0 += (int64_t)a.l[i] * (int64_t)b.l[col - i];
+      s1 -= (int64_t)c.l[i] * (int64_t)d.l[col - i];
+    }
+#pragma unroll
+    for (int i = lo; i <= hi; i++) {
+      if (i >= col) continue;
+      const int64_t pr = (int64_t)((uint64_t)m[i] * (uint64_t)(uint32_t)P::P[col - i]);
+      if ((i - lo) & 1) s0 += pr;
+      else s1 += pr;
+    }
+    int64_t s = s0 + s1;
+    if (col < N) {
+      const uint32_t mc = ((uint32_t)s * P::NP) & MASK;
+      m[col] = mc;
+      s += (int64_t)((uint64_t)mc * (uint64_t)(uint32_t)P::P[0]);
+    } else {
+      r.l[col - N] = (int32_t)((uint32_t)s & MASK);
+    }
+    carry = s >> B;
+  }
+  r.l[N - 1] = (int32_t)carry;
+}
+
+// a^2 / R' for a normalised a: each off-diagonal product a_i a_j (i < j) once, against the
+// doubled limb 2 a_i (|2 a_i| < 2^30), so a column holds about half of zmul's multiply-adds
+// with the same sum bound; the reduction half is zmul's.
 template <class P>
 ZK_HD void zsqr(Fz<P> &r, const Fz<P> &a) {
-  zmul(r, a, a);
+  constexpr int N = P::N, B = P::BITS;
+  constexpr uint32_t MASK = (1u << B) - 1;
+  int32_t d[N];
+#pragma unroll
+  for (int i = 0; i < N; i++) d[i] = a.l[i] * 2;
+  uint32_t m[N];
+  int64_t carry = 0;
+#pragma unroll
+  for (int c = 0; c < 2 * N - 1; c++) {
+    const int lo = c < N ? 0 : c - N + 1;
+    const int hi = c < N ? c : N - 1;
+    int64_t s0 = carry, s1 = 0;
+    int k = 0;
+#pragma unroll
+    for (int i = lo; i <= hi; i++) {
+      const int j = c - i;
+      if (i > j) continue;
+      const int64_t pr = i == j ? (int64_t)a.l[i] * (int64_t)a.l[i] : (int64_t)d[i] * (int64_t)a.l[j];
+      if (k & 1) s1 += pr;
+      else s0 += pr;
+      k++;
+    }
+#pragma unroll
+    for (int i = lo; i <= hi; i++) {
+      if (i >= c) continue;
+      const int64_t pr = (int64_t)((uint64_t)m[i] * (uint64_t)(uint32_t)P::P[c - i]);
+      if ((i - lo) & 1) s0 += pr;
+      else s1 += pr;
+    }
+    int64_t s = s0 + s1;
+    if (c < N) {
+      const uint32_t mc = ((uint32_t)s * P::NP) & MASK;
+      m[c] = mc;
+      s += (int64_t)((uint64_t)mc * (uint64_t)(uint32_t)P::P[0]);
+    } else {
+      r.l[c - N] = (int32_t)((uint32_t)s & MASK);
+    }
+    carry = s >> B;
+  }
+  r.l[N - 1] = (int32_t)carry;
+}
+
+// a^-1 (internal form) by Fermat's little theorem: a^(p-2) in the Montgomery domain, square and
+// multiply over the bits of p - 2 (p's low limb is at least 2, so only that limb changes).
+// Output normalised, < 2p; a == 0 gives 0.
+template <class P>
+ZK_HD void zinv(Fz<P> &r, const Fz<P> &a) {
+  Fz<P> acc, t;
+  zset_one(acc);
+  for (int i = P::N * P::BITS - 1; i >= 0; i--) {
+    zsqr(t, acc);
+    const int limb = i / P::BITS, bit = i % P::BITS;
+    const uint32_t e = (uint32_t)P::P[limb] - (limb == 0 ? 2u : 0u);
+    if ((e >> bit) & 1) zmul(acc, t, a);
+    else acc = t;
+  }
+  r = acc;
 }
 
 // a == 0 (mod p) for a normalised a with |a| < 8p.  The low limb of j p (j = -8..8) filters;

@@ -443,7 +443,10 @@ __device__ __forceinline__ uint32_t bucket_from(const uint32_t *__restrict__ off
   return lo;
 }
 
-template <class C>
+// PF: the next entry's point is loaded before the current entry's addition, so its latency
+// overlaps the arithmetic (18 more registers per thread).  Measured ~3% slower than letting
+// occupancy hide the latency, so it is off unless a schedule asks for it.
+template <class C, bool PF>
 __global__ __launch_bounds__(kThreads) void msm_accum_kernel(const uint32_t *__restrict__ vals,
                                                             const uint32_t *__restrict__ off, uint32_t nkeys,
                                                             int seg_log, const int32_t *__restrict__ points,
@@ -460,7 +463,21 @@ __global__ __launch_bounds__(kThreads) void msm_accum_kernel(const uint32_t *__r
   bool cont_run = off[cur] < e0;
   XyzzZ<C> acc;
   zxyzz_set_inf(acc);
+  uint32_t v = vals[e0];
+  AffZ<C> a;
+  if (PF) zaff_load(a, points + (size_t)(v & 0x7fffffffu) * AW);
   for (uint64_t e = e0; e < e1; e++) {
+    uint32_t vn = 0;
+    AffZ<C> an;
+    if (PF) {
+      if (e + 1 < e1) {
+        vn = vals[e + 1];
+        zaff_load(an, points + (size_t)(vn & 0x7fffffffu) * AW);
+      }
+    } else {
+      v = vals[e];
+      zaff_load(a, points + (size_t)(v & 0x7fffffffu) * AW);
+    }
     if (e == next) {
       flush_run<C>(acc, cont_run, seg, cur, bsum, cont);
       cont_run = false;
@@ -468,12 +485,14 @@ __global__ __launch_bounds__(kThreads) void msm_accum_kernel(const uint32_t *__r
       cur = bucket_from(off, nkeys, cur + 1, (uint32_t)e);
       next = off[cur + 1];
     }
-    const uint32_t v = vals[e];
-    AffZ<C> a;
-    zaff_load(a, points + (size_t)(v & 0x7fffffffu) * AW);
-    if (zaff_is_inf(a)) continue;
-    if (v >> 31) zneg(a.y, a.y);
-    zxyzz_madd(acc, a);
+    if (!zaff_is_inf(a)) {
+      if (v >> 31) zneg(a.y, a.y);
+      zxyzz_madd(acc, a);
+    }
+    if (PF) {
+      v = vn;
+      a = an;
+    }
   }
   flush_run<C>(acc, cont_run, seg, cur, bsum, cont);
 }
@@ -651,9 +670,10 @@ __global__ __launch_bounds__(kThreads) void scalar_mul_gen_kernel(const uint64_t
 
 static const zkh::FieldDesc &fp_desc(int curve) { return curve == 0 ? zkh::BN254_FP : zkh::BLS381_FP; }
 
-// windows up to this width are sorted in LDS; wider ones fall back to global atomics, which
-// are slower than any saving in buckets
-static constexpr int kMaxAnalyticWindow = 17;
+// windows up to 16 bits sort with the chunk's bucket bases in LDS; 17-bit windows look the
+// bases up in HBM and wider ones fall back to global atomics, both slower than the buckets
+// they save (measured: BN254 2^22 at c = 17 spends 2.9 ms sorting against 2.1 ms at c = 16)
+static constexpr int kMaxAnalyticWindow = 16;
 
 int msm_heuristic_window(int curve, long long n, int scalar_bits) {
   (void)curve;
@@ -944,9 +964,15 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   }
   ZK_HIP(hipGetLastError());
   ev.mark(st);
-  hipLaunchKernelGGL(msm_accum_kernel<C>, dim3((unsigned)((nseg + kThreads - 1) / kThreads)), blk, 0, st,
-                     vals.as<uint32_t>(), off.as<uint32_t>(), (uint32_t)nkeys, seg_log,
-                     zp, bsum.as<int32_t>(), cont.as<int32_t>());
+  const bool prefetch = user && user->prefetch == 2;
+  if (prefetch)
+    hipLaunchKernelGGL((msm_accum_kernel<C, true>), dim3((unsigned)((nseg + kThreads - 1) / kThreads)), blk, 0, st,
+                       vals.as<uint32_t>(), off.as<uint32_t>(), (uint32_t)nkeys, seg_log, zp, bsum.as<int32_t>(),
+                       cont.as<int32_t>());
+  else
+    hipLaunchKernelGGL((msm_accum_kernel<C, false>), dim3((unsigned)((nseg + kThreads - 1) / kThreads)), blk, 0,
+                       st, vals.as<uint32_t>(), off.as<uint32_t>(), (uint32_t)nkeys, seg_log, zp, bsum.as<int32_t>(),
+                       cont.as<int32_t>());
   ZK_HIP(hipGetLastError());
   ev.mark(st);
   const uint32_t limit = (user && user->fix_limit > 0) ? (uint32_t)user->fix_limit : 16u;

@@ -61,6 +61,7 @@ template <> struct ZOf<Bn254Fr> { using T = ZBn254Fr; };
 template <> struct ZOf<Bls381Fr> { using T = ZBls381Fr; };
 
 static constexpr int kZW = 9;  // LDS words per element (9 limbs of 29 bits)
+static constexpr int kBatchTileLog = 9;  // largest LDS tile (log2 elements) of a batched job
 
 // omega^e as an internal-form table entry; returns true when the table value must be negated
 template <class Z>
@@ -235,11 +236,20 @@ __global__ __launch_bounds__(kNttThreads) void ntt_batch_pass_kernel(const PassA
   int lo = 0, hi = nitems;  // last i with prefix[i] <= b
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
-    if (prefix[mid] <= b) lo = mid;
+    if (__builtin_amdgcn_readfirstlane(prefix[mid]) <= b) lo = mid;
     else hi = mid;
   }
-  const PassArgs a = items[lo];
-  ntt_pass_body<F>(a, b - prefix[lo], reinterpret_cast<int32_t *>(lds_raw));
+  // the item is the same for the whole workgroup: copy it through readfirstlane so that its
+  // fields live in scalar registers, as kernel arguments do (vector copies would cost ~40
+  // VGPRs and the occupancy with them)
+  static_assert(sizeof(PassArgs) % 4 == 0, "PassArgs is copied as words");
+  PassArgs a;
+  const uint32_t *src = reinterpret_cast<const uint32_t *>(items + lo);
+  uint32_t *dst = reinterpret_cast<uint32_t *>(&a);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(PassArgs) / 4); i++) dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
+  const uint32_t first = __builtin_amdgcn_readfirstlane(prefix[lo]);
+  ntt_pass_body<F>(a, b - first, reinterpret_cast<int32_t *>(lds_raw));
 }
 
 // ---------------------------------------------------------------------------------------
@@ -478,16 +488,41 @@ static int get_twiddles(int curve, int m, const uint64_t *gen, const uint32_t **
   return ZK_OK;
 }
 
+// coset tables in internal form: tab[k][j] = (eta^(2^(11 k)))^j R', k < 3, j < 2048; the three
+// bases are raised on the host, so each entry is at most 11 squarings away from its base
 template <class F>
-static int build_eta_tables(const zkh::Fe &eta, DevBuf &buf, hipStream_t st) {
-  ZK_TRY(buf.alloc(3 * 2048 * F::N * 4));
-  Fe<F> e = to_dev<F>(eta);
-  uint32_t *b = buf.as<uint32_t>();
-  hipLaunchKernelGGL(pow_table_kernel<F>, dim3(8), dim3(256), 0, st, b, e, 1ull, 2048);
-  hipLaunchKernelGGL(pow_table_kernel<F>, dim3(8), dim3(256), 0, st, b + 2048 * F::N, e, 2048ull, 2048);
-  hipLaunchKernelGGL(pow_table_kernel<F>, dim3(8), dim3(256), 0, st, b + 4096 * F::N, e, 1ull << 22, 2048);
+__global__ void eta_tables_kernel(uint32_t *tab, Fe<F> b0, Fe<F> b1, Fe<F> b2) {
+  using Z = typename ZOf<F>::T;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * 2048) return;
+  const int k = t >> 11, j = t & 2047;
+  Fe<F> r;
+  fpow_u64(r, k == 0 ? b0 : (k == 1 ? b1 : b2), (uint64_t)j);
+  Fz<Z> a, c;
+  zunpack(a, r.w);
+  zset_const(c, Z::K_IN);
+  zmul(a, a, c);
+  zcanon(a, a);
+  zpack(tab + (size_t)t * 8, a);
+}
+
+template <class F>
+static int launch_eta_tables(uint32_t *tab, int curve, const zkh::Fe &eta, hipStream_t st) {
+  const zkh::FieldDesc &f = fr_desc(curve);
+  zkh::Fe b1 = eta, b2;
+  for (int i = 0; i < 11; i++) b1 = zkh::sqr(f, b1);
+  b2 = b1;
+  for (int i = 0; i < 11; i++) b2 = zkh::sqr(f, b2);
+  hipLaunchKernelGGL(eta_tables_kernel<F>, dim3(24), dim3(256), 0, st, tab, to_dev<F>(eta), to_dev<F>(b1),
+                     to_dev<F>(b2));
   ZK_HIP(hipGetLastError());
-  return tab_to_internal<F>(b, 3 * 2048, st);
+  return ZK_OK;
+}
+
+template <class F>
+static int build_eta_tables(int curve, const zkh::Fe &eta, DevBuf &buf, hipStream_t st) {
+  ZK_TRY(buf.alloc(3 * 2048 * F::N * 4));
+  return launch_eta_tables<F>(buf.as<uint32_t>(), curve, eta, st);
 }
 
 NttPlan ntt_plan(int m, int smax, int tile_log) {
@@ -657,7 +692,7 @@ static int ntt_device_t(int curve, int m, const uint64_t *gen, const uint64_t *e
   int tw_shift = 0;
   if (m >= 1) ZK_TRY(get_twiddles<F>(curve, m, gen, &tw, &tw_shift, st));
   DevBuf etab;
-  if (eta) ZK_TRY(build_eta_tables<F>(zkh::load(f, eta), etab, st));
+  if (eta) ZK_TRY(build_eta_tables<F>(curve, zkh::load(f, eta), etab, st));
   uint32_t lastf[8];
   last_factor<F>(lastf, curve, m, inverse);
   const NttTuning tu = tune ? *tune : ntt_tuning_for(curve, m);
@@ -732,7 +767,11 @@ static int ntt_batch_create_t(int curve, int njobs, const NttBatchJob *jobs, Ntt
   uint64_t tmp_words = 0, ncoset = 0;
   std::vector<NttPlan> plans(njobs);
   for (int j = 0; j < njobs; j++) {
-    const NttTuning tu = ntt_tuning_for(curve, jobs[j].m);
+    // every item of a round shares the launch's LDS size (the largest item's): keep all tiles
+    // at most 2^kBatchTileLog elements so that no job lowers the others' occupancy
+    NttTuning tu = ntt_tuning_for(curve, jobs[j].m);
+    tu.smax = std::min(tu.smax, kBatchTileLog);
+    tu.tile_log = std::min(tu.tile_log, kBatchTileLog);
     plans[j] = ntt_plan(jobs[j].m, tu.smax, tu.tile_log);
     if (plans[j].npass > 1) tmp_words += ((uint64_t)1 << jobs[j].m) * F::N;
     if (jobs[j].eta) ncoset++;
@@ -756,11 +795,7 @@ static int ntt_batch_create_t(int curve, int njobs, const NttBatchJob *jobs, Ntt
     if (jobs[j].eta) {
       uint32_t *e = pl->eta.as<uint32_t>() + eoff;
       eoff += 3 * 2048 * F::N;
-      Fe<F> ed = to_dev<F>(zkh::load(f, jobs[j].eta));
-      hipLaunchKernelGGL(pow_table_kernel<F>, dim3(8), dim3(256), 0, st, e, ed, 1ull, 2048);
-      hipLaunchKernelGGL(pow_table_kernel<F>, dim3(8), dim3(256), 0, st, e + 2048 * F::N, ed, 2048ull, 2048);
-      hipLaunchKernelGGL(pow_table_kernel<F>, dim3(8), dim3(256), 0, st, e + 4096 * F::N, ed, 1ull << 22, 2048);
-      ZK_TRY(tab_to_internal<F>(e, 3 * 2048, st));
+      ZK_TRY(launch_eta_tables<F>(e, curve, zkh::load(f, jobs[j].eta), st));
       etab = e;
     }
     uint32_t lastf[8];

@@ -57,7 +57,7 @@ class ScalarStats(ctypes.Structure):
 class MsmSchedule(ctypes.Structure):
     """zkg_msm_schedule."""
     _fields_ = [("window", ctypes.c_int), ("seg_log", ctypes.c_int), ("policy", ctypes.c_int),
-                ("fix_limit", ctypes.c_int), ("reserved", ctypes.c_int * 4)]
+                ("fix_limit", ctypes.c_int), ("prefetch", ctypes.c_int), ("reserved", ctypes.c_int * 3)]
 
 
 _lib = None
@@ -122,6 +122,7 @@ def lib():
         L.zkg_mlp_train.argtypes = [vp, ctypes.c_int, fp_, fp_, ctypes.c_int, ctypes.c_float, fp_]
         L.zkg_msm_set_cost_model.argtypes = [vp]
         L.zkg_ntt_set_tuning.argtypes = [ctypes.c_int, ctypes.c_int]
+        L.zkg_g1_fft.argtypes = [ctypes.c_int, ctypes.c_int, u64p, ctypes.c_int, vp, vp]
         L.zkg_ntt_set_tuning_for.argtypes = [ctypes.c_int] * 4
         L.zkg_ntt_tune.argtypes = [ctypes.c_int] * 5 + [ip, ip, dp, ip]
         _lib = L
@@ -278,6 +279,23 @@ def asymm_ntt(curve, m_src, gen_src, src, m_tgt, gen_tgt):
     return out
 
 
+def g1_fft(curve, m, gen, src, inverse=False):
+    """G1 FFT through the reference entry point <c>_G1_proj_fft_{forward,inverse}: src is
+    (2^m, 3 L_p) projective Montgomery points; outputs normalised."""
+    src = _c(src)
+    if src.shape[0] != (1 << m):
+        raise ValueError("g1 fft: subgroup size differs from the array size")
+    out = np.zeros_like(src)
+    f = _ref_fn(curve, "G1_proj_fft_inverse" if inverse else "G1_proj_fft_forward")
+    f.argtypes = [ctypes.c_int, u64p, u64p, u64p]
+    f(m, _p(_c(gen)), _p(src), _p(out))
+    return out
+
+
+def g1_fft_device(curve, m, gen, d_src, d_tgt, inverse=False):
+    _check(lib().zkg_g1_fft(curve, m, _p(_c(gen)), int(inverse), d_src.ptr, d_tgt.ptr))
+
+
 def fft_generator(curve, m):
     out = np.zeros(4, np.uint64)
     _check(lib().zkg_fft_generator(curve, m, _p(out)))
@@ -288,15 +306,15 @@ def fft_generator(curve, m):
 # extension API (device buffers)
 
 def msm_device(curve, n, d_scalars, d_points, expo_nlimbs=4, flags=0, window=0, seg_log=0, policy=SCHED_PLAIN,
-               fix_limit=0):
+               fix_limit=0, prefetch=0):
     """d_points: DeviceArray of affine points, or PreparedPoints (flag added automatically)."""
     if isinstance(d_points, PreparedPoints):
         flags |= MSM_POINTS_INTERNAL
     size = 3 * FP_LIMBS[curve] if flags & (MSM_PROJ_OUT | MSM_JAC_OUT) else 2 * FP_LIMBS[curve]
     res = np.zeros(size, np.uint64)
     sched = None
-    if window or seg_log or policy or fix_limit:
-        sched = MsmSchedule(window, seg_log, policy, fix_limit)
+    if window or seg_log or policy or fix_limit or prefetch:
+        sched = MsmSchedule(window, seg_log, policy, fix_limit, prefetch)
     _check(lib().zkg_msm(curve, n, d_scalars.ptr, expo_nlimbs, d_points.ptr, _p(res), flags,
                          ctypes.cast(ctypes.byref(sched), vp) if sched is not None else None))
     return res
