@@ -95,6 +95,59 @@ void bn128_G1_proj_fft_inverse(int m, const uint64_t *gen, const uint64_t *src, 
 /* lib/cbits/curves/g1/proj/bls12_381_G1_proj.h:54-55 */
 void bls12_381_G1_proj_fft_forward(int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt);
 void bls12_381_G1_proj_fft_inverse(int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt);
+/* Jacobian inputs (X:Y:Z) = (X/Z^2, Y/Z^3); outputs normalised [x : y : 1], infinity (0, 1, 0)
+ * as bn128_G1_jac_normalize writes it (bn128_G1_jac.c:62-85).
+ * lib/cbits/curves/g1/jac/bn128_G1_jac.h:54-55 (impl bn128_G1_jac.c:840-950) */
+void bn128_G1_jac_fft_forward(int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt);
+void bn128_G1_jac_fft_inverse(int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt);
+/* lib/cbits/curves/g1/jac/bls12_381_G1_jac.h:54-55 */
+void bls12_381_G1_jac_fft_forward(int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt);
+void bls12_381_G1_jac_fft_inverse(int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt);
+
+/* Binary-coefficient MSMs: the sum of the points whose byte is non-zero (one row), or one such
+ * sum per row of an nrows x npoints byte matrix given as row pointers.  proj_out results are
+ * normalised like the other MSM entry points. */
+/* lib/cbits/curves/g1/proj/bn128_G1_proj.h:45 (impl bn128_G1_proj.c:684-692) */
+void bn128_G1_proj_MSM_binary_coeff_proj_out(int npoints, const uint8_t *expos, const uint64_t *grps, uint64_t *tgt);
+/* bn128_G1_proj.h:48 (impl bn128_G1_proj.c:694-698) */
+void bn128_G1_proj_MSM_binary_coeff_affine_out(int npoints, const uint8_t *expos, const uint64_t *grps, uint64_t *tgt);
+/* bn128_G1_proj.h:52 (impl bn128_G1_proj.c:716-770) */
+void bn128_G1_proj_multi_MSM_binary_coeff_proj_out(int nrows, int npoints, const uint8_t **expos, const uint64_t *grps, uint64_t *tgt);
+/* bn128_G1_proj.h:53 (impl bn128_G1_proj.c:773-782) */
+void bn128_G1_proj_multi_MSM_binary_coeff_affine_out(int nrows, int npoints, const uint8_t **expos, const uint64_t *grps, uint64_t *tgt);
+/* lib/cbits/curves/g1/proj/bls12_381_G1_proj.h:45-53 */
+void bls12_381_G1_proj_MSM_binary_coeff_proj_out(int npoints, const uint8_t *expos, const uint64_t *grps, uint64_t *tgt);
+void bls12_381_G1_proj_MSM_binary_coeff_affine_out(int npoints, const uint8_t *expos, const uint64_t *grps, uint64_t *tgt);
+void bls12_381_G1_proj_multi_MSM_binary_coeff_proj_out(int nrows, int npoints, const uint8_t **expos, const uint64_t *grps, uint64_t *tgt);
+void bls12_381_G1_proj_multi_MSM_binary_coeff_affine_out(int nrows, int npoints, const uint8_t **expos, const uint64_t *grps, uint64_t *tgt);
+
+/* Jacobian twins (lib/cbits/curves/g1/jac/bn128_G1_jac.h:45-53, impl bn128_G1_jac.c:726-830);
+ * jac_out results are normalised [x : y : 1], infinity [1 : 1 : 0]. */
+void bn128_G1_jac_MSM_binary_coeff_jac_out(int npoints, const uint8_t *expos, const uint64_t *grps, uint64_t *tgt);
+void bn128_G1_jac_MSM_binary_coeff_affine_out(int npoints, const uint8_t *expos, const uint64_t *grps, uint64_t *tgt);
+void bn128_G1_jac_multi_MSM_binary_coeff_jac_out(int nrows, int npoints, const uint8_t **expos, const uint64_t *grps, uint64_t *tgt);
+void bn128_G1_jac_multi_MSM_binary_coeff_affine_out(int nrows, int npoints, const uint8_t **expos, const uint64_t *grps, uint64_t *tgt);
+/* lib/cbits/curves/g1/jac/bls12_381_G1_jac.h:45-53 */
+void bls12_381_G1_jac_MSM_binary_coeff_jac_out(int npoints, const uint8_t *expos, const uint64_t *grps, uint64_t *tgt);
+void bls12_381_G1_jac_MSM_binary_coeff_affine_out(int npoints, const uint8_t *expos, const uint64_t *grps, uint64_t *tgt);
+void bls12_381_G1_jac_multi_MSM_binary_coeff_jac_out(int nrows, int npoints, const uint8_t **expos, const uint64_t *grps, uint64_t *tgt);
+void bls12_381_G1_jac_multi_MSM_binary_coeff_affine_out(int nrows, int npoints, const uint8_t **expos, const uint64_t *grps, uint64_t *tgt);
+
+/* Batch conversions: projective (X:Y:Z) -> affine (X/Z, Y/Z), Z = 0 -> all 0xFF bytes; affine
+ * -> (x, y, 1), the affine infinity (0,0) -> (0, 1, 0).  Jacobian: (X/Z^2, Y/Z^3), infinity
+ * [1 : 1 : 0]. */
+/* bn128_G1_proj.h:10 (impl bn128_G1_proj.c:157-166) */
+void bn128_G1_proj_batch_to_affine(int N, const uint64_t *src, uint64_t *tgt);
+/* bn128_G1_proj.h:9 (impl bn128_G1_proj.c:146-155) */
+void bn128_G1_proj_batch_from_affine(int N, const uint64_t *src, uint64_t *tgt);
+/* bls12_381_G1_proj.h:9-10 */
+void bls12_381_G1_proj_batch_to_affine(int N, const uint64_t *src, uint64_t *tgt);
+void bls12_381_G1_proj_batch_from_affine(int N, const uint64_t *src, uint64_t *tgt);
+/* bn128_G1_jac.h:9-10 (impl bn128_G1_jac.c:139-158), bls12_381_G1_jac.h:9-10 */
+void bn128_G1_jac_batch_to_affine(int N, const uint64_t *src, uint64_t *tgt);
+void bn128_G1_jac_batch_from_affine(int N, const uint64_t *src, uint64_t *tgt);
+void bls12_381_G1_jac_batch_to_affine(int N, const uint64_t *src, uint64_t *tgt);
+void bls12_381_G1_jac_batch_from_affine(int N, const uint64_t *src, uint64_t *tgt);
 
 /* =========================================================================================
  * Part 2: extension API (no reference counterpart)
@@ -195,8 +248,16 @@ int zkg_fill_random(int curve, int field, size_t n, uint64_t seed, uint64_t *d_o
  * output (input generation for tests and benchmarks) */
 int zkg_g1_scalar_mul_gen(int curve, int n, const uint64_t *d_scalars, uint64_t *d_out_affine);
 
+/* projective (jac = 0) or Jacobian (jac = 1) -> affine for n device points (Montgomery batch
+ * inversion, 16 points per thread) */
+int zkg_g1_batch_to_affine(int curve, int jac, int n, const uint64_t *d_src, uint64_t *d_tgt);
+/* binary-coefficient MSM on device buffers (d_bits: npoints bytes); flags as zkg_msm */
+int zkg_msm_binary(int curve, int npoints, const uint8_t *d_bits, const uint64_t *d_grps, uint64_t *tgt_host,
+                   int flags);
 /* G1 FFT on device buffers (2^m projective points in, normalised projective points out) */
-int zkg_g1_fft(int curve, int m, const uint64_t *gen, int inverse, const uint64_t *d_src, uint64_t *d_tgt);
+#define ZKG_G1_FFT_INVERSE 1
+#define ZKG_G1_FFT_JAC 2    /* Jacobian inputs (outputs are the same normalised words) */
+int zkg_g1_fft(int curve, int m, const uint64_t *gen, int mode, const uint64_t *d_src, uint64_t *d_tgt);
 
 /* [tau^i G]_{i<n} for tau in standard form (examples/KZG.hs:59-62), affine Montgomery output */
 int zkg_srs_generate(int curve, int n, const uint64_t *tau_std, uint64_t *d_out_affine);

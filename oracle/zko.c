@@ -683,3 +683,70 @@ void zko_g1_fft_forward(int cv, int m, const uint64_t *gen, const uint64_t *src,
 void zko_g1_fft_inverse(int cv, int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt) {
   g1_fft(cv, m, gen, src, tgt, 1);
 }
+
+/* Jacobian G1 FFT (bn128_G1_jac.c:840-950): the same recursions over Jacobian points.  A
+ * Jacobian (X:Y:Z) is the projective (X Z : Y : Z^3) of the same affine point, and both
+ * normalisations give (x, y, 1), infinity (0, 1, 0) (bn128_G1_jac.c:62-85), so the projective
+ * recursions above compute the same outputs word for word. */
+static void jac_to_proj(const curve_t *C, const uint64_t *J, uint64_t *P) {
+  const field_t *f = C->fp; int n = f->nl;
+  uint64_t z2[MAXL];
+  if (is_zero(f, J + 2 * n)) { proj_set_inf(C, P); return; }
+  f_mul(f, J, J + 2 * n, PX(P));
+  cpy(f, PY(P), J + n);
+  f_sqr(f, J + 2 * n, z2);
+  f_mul(f, z2, J + 2 * n, PZ(P));
+}
+static void g1_jac_fft(int cv, int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt, int inverse) {
+  const curve_t *C = &CURVES[cv]; int n = C->fp->nl;
+  const size_t GN = 3 * (size_t)n, N = (size_t)1 << m;
+  uint64_t *p = (uint64_t *)malloc(8 * GN * N);
+  if (!p) abort();
+  for (size_t i = 0; i < N; i++) jac_to_proj(C, src + i * GN, p + i * GN);
+  g1_fft(cv, m, gen, p, tgt, inverse);
+  free(p);
+}
+void zko_g1_jac_fft_forward(int cv, int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt) {
+  g1_jac_fft(cv, m, gen, src, tgt, 0);
+}
+void zko_g1_jac_fft_inverse(int cv, int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt) {
+  g1_jac_fft(cv, m, gen, src, tgt, 1);
+}
+
+/* ---------------- batch conversions (bn128_G1_proj.c:146-166, bn128_G1_jac.c:139-158) -------- */
+void zko_g1_batch_to_affine(int cv, int jac, int N, const uint64_t *src, uint64_t *tgt) {
+  const curve_t *C = &CURVES[cv]; int n = C->fp->nl;
+  for (int i = 0; i < N; i++) {
+    if (jac) zko_g1_jac_to_affine(cv, src + (size_t)i * 3 * n, tgt + (size_t)i * 2 * n);
+    else proj_to_affine(C, src + (size_t)i * 3 * n, tgt + (size_t)i * 2 * n);
+  }
+}
+/* affine (0,0) -> (0 : 1 : 0), Jacobian [1 : 1 : 0] (bn128_G1_jac.c:108-117, :183-187) */
+void zko_g1_batch_from_affine(int cv, int jac, int N, const uint64_t *src, uint64_t *tgt) {
+  const curve_t *C = &CURVES[cv]; const field_t *f = C->fp; int n = f->nl;
+  for (int i = 0; i < N; i++) {
+    const uint64_t *a = src + (size_t)i * 2 * n;
+    uint64_t *p = tgt + (size_t)i * 3 * n;
+    proj_from_affine(C, a, p);
+    if (jac && aff_is_inf(C, a)) cpy(f, PX(p), f->one);
+  }
+}
+
+/* ---------------- binary MSM (bn128_G1_proj.c:684-698) ----------------
+ * sum of the affine points whose coefficient byte is non-zero, by mixed additions into a
+ * projective accumulator (the reference's loop); the multi form (:716-782) returns the same
+ * group element per row, so it is restated as one such sum per row.  Outputs are normalised
+ * projective points (X/Z, Y/Z, 1), infinity (0, 1, 0). */
+void zko_msm_binary_proj(int cv, int npoints, const uint8_t *expos, const uint64_t *grps, uint64_t *tgt) {
+  const curve_t *C = &CURVES[cv]; int n = C->fp->nl;
+  uint64_t acc[3 * MAXL];
+  proj_set_inf(C, acc);
+  for (int i = 0; i < npoints; i++)
+    if (expos[i]) proj_madd(C, acc, grps + (size_t)i * 2 * n, acc);
+  zko_g1_proj_normalize(cv, acc, tgt);
+}
+void zko_multi_msm_binary_proj(int cv, int nrows, int npoints, const uint8_t *const *rows, const uint64_t *grps,
+                               uint64_t *tgt) {
+  int n = CURVES[cv].fp->nl;
+  for (int r = 0; r < nrows; r++) zko_msm_binary_proj(cv, npoints, rows[r], grps, tgt + (size_t)r * 3 * n);
+}

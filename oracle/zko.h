@@ -70,6 +70,16 @@ void zko_ntt_forward_asymmetric(int curve, int m_src, int m_tgt, const uint64_t 
  * (bn128_G1_proj.c:791-902) */
 void zko_g1_fft_forward(int curve, int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt);
 void zko_g1_fft_inverse(int curve, int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt);
+/* the same over Jacobian points (bn128_G1_jac.c:840-950); outputs (x, y, 1), infinity (0, 1, 0) */
+void zko_g1_jac_fft_forward(int curve, int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt);
+void zko_g1_jac_fft_inverse(int curve, int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt);
+/* batch conversions, projective (jac = 0) or Jacobian (jac = 1) <-> affine */
+void zko_g1_batch_to_affine(int curve, int jac, int N, const uint64_t *src, uint64_t *tgt);
+void zko_g1_batch_from_affine(int curve, int jac, int N, const uint64_t *src, uint64_t *tgt);
+/* binary-coefficient MSMs, normalised projective outputs (bn128_G1_proj.c:684-782) */
+void zko_msm_binary_proj(int curve, int npoints, const uint8_t *expos, const uint64_t *grps, uint64_t *tgt);
+void zko_multi_msm_binary_proj(int curve, int nrows, int npoints, const uint8_t *const *rows,
+                               const uint64_t *grps, uint64_t *tgt);
 /* generator of the size-2^m FFT subgroup, Montgomery form (Class/FFT.hs:60-68) */
 void zko_fft_generator(int curve, int m, uint64_t *gen);
 

@@ -132,10 +132,62 @@ class Oracle:
                 self.lib.zko_f_mul(self.c.id, 0, ptr(np.ascontiguousarray(w)), ptr(z), ptr(out[i, j * L:(j + 1) * L]))
         return out
 
-    def g1_fft(self, m, gen, src, inverse=False):
+    def g1_fft(self, m, gen, src, inverse=False, jac=False):
         out = np.zeros_like(src)
-        fn = self.lib.zko_g1_fft_inverse if inverse else self.lib.zko_g1_fft_forward
+        if jac:
+            fn = self.lib.zko_g1_jac_fft_inverse if inverse else self.lib.zko_g1_jac_fft_forward
+        else:
+            fn = self.lib.zko_g1_fft_inverse if inverse else self.lib.zko_g1_fft_forward
         fn(self.c.id, m, ptr(np.ascontiguousarray(gen)), ptr(np.ascontiguousarray(src)), ptr(out))
+        return out
+
+    def batch_to_affine(self, P, jac=False):
+        P = np.ascontiguousarray(P, dtype=np.uint64)
+        n = P.size // (3 * self.lp)
+        out = np.zeros((n, 2 * self.lp), np.uint64)
+        self.lib.zko_g1_batch_to_affine(self.c.id, int(jac), n, ptr(P), ptr(out))
+        return out
+
+    def batch_from_affine(self, A, jac=False):
+        A = np.ascontiguousarray(A, dtype=np.uint64)
+        n = A.size // (2 * self.lp)
+        out = np.zeros((n, 3 * self.lp), np.uint64)
+        self.lib.zko_g1_batch_from_affine(self.c.id, int(jac), n, ptr(A), ptr(out))
+        return out
+
+    def msm_binary(self, bits, points):
+        """Normalised projective sum of the points whose byte is non-zero."""
+        bits = np.ascontiguousarray(bits, dtype=np.uint8)
+        out = np.zeros(3 * self.lp, np.uint64)
+        self.lib.zko_msm_binary_proj(self.c.id, len(bits), bits.ctypes.data_as(ctypes.c_void_p),
+                                     ptr(np.ascontiguousarray(points)), ptr(out))
+        return out
+
+    def multi_msm_binary(self, rows, points):
+        rows = np.ascontiguousarray(rows, dtype=np.uint8)
+        nr, n = rows.shape
+        out = np.zeros((nr, 3 * self.lp), np.uint64)
+        ptrs = (ctypes.c_void_p * max(nr, 1))(*[rows.ctypes.data + r * n for r in range(nr)])
+        self.lib.zko_multi_msm_binary_proj(self.c.id, nr, n, ptrs, ptr(np.ascontiguousarray(points)), ptr(out))
+        return out
+
+    def jac_points(self, ks, zs=None):
+        """Jacobian Montgomery points k_i G with a non-trivial Z (x z^2, y z^3, z), infinity [1:1:0]."""
+        L = self.lp
+        aff = self.points_from_scalars(ks)
+        out = np.zeros((len(ks), 3 * L), np.uint64)
+        one = np.array(to_limbs(self.c.fp_to_mont(1), L), np.uint64)
+        for i, k in enumerate(ks):
+            if k == 0:
+                out[i, :L] = one
+                out[i, L:2 * L] = one
+                continue
+            zi = (zs[i] if zs else 5 + 3 * i) % self.c.p
+            z2, z3 = zi * zi % self.c.p, zi * zi * zi % self.c.p
+            for j, (w, zz) in enumerate(((aff[i, :L], z2), (aff[i, L:], z3))):
+                zm = np.array(to_limbs(self.c.fp_to_mont(zz), L), np.uint64)
+                self.lib.zko_f_mul(self.c.id, 0, ptr(np.ascontiguousarray(w)), ptr(zm), ptr(out[i, j * L:(j + 1) * L]))
+            out[i, 2 * L:] = np.array(to_limbs(self.c.fp_to_mont(zi), L), np.uint64)
         return out
 
 
@@ -190,8 +242,38 @@ class Ref:
         self.fn("Fr_mont_mul" if field == 1 else "Fp_mont_mul")(ptr(a), ptr(b), ptr(out))
         return out
 
-    def g1_fft(self, m, gen, src, inverse=False):
+    def g1_fft(self, m, gen, src, inverse=False, coord="proj"):
         out = np.zeros_like(src)
-        self.fn("G1_proj_fft_inverse" if inverse else "G1_proj_fft_forward")(
+        self.fn(f"G1_{coord}_fft_{'inverse' if inverse else 'forward'}")(
             m, ptr(np.ascontiguousarray(gen)), ptr(np.ascontiguousarray(src)), ptr(out))
         return out
+
+    def batch_to_affine(self, P, coord="proj"):
+        P = np.ascontiguousarray(P, dtype=np.uint64)
+        n = P.size // (3 * self.lp)
+        out = np.zeros((n, 2 * self.lp), np.uint64)
+        self.fn(f"G1_{coord}_batch_to_affine")(n, ptr(P), ptr(out))
+        return out
+
+    def batch_from_affine(self, A, coord="proj"):
+        A = np.ascontiguousarray(A, dtype=np.uint64)
+        n = A.size // (2 * self.lp)
+        out = np.zeros((n, 3 * self.lp), np.uint64)
+        self.fn(f"G1_{coord}_batch_from_affine")(n, ptr(A), ptr(out))
+        return out
+
+    def msm_binary(self, bits, points, coord="proj", out="affine"):
+        bits = np.ascontiguousarray(bits, dtype=np.uint8)
+        res = np.zeros(2 * self.lp if out == "affine" else 3 * self.lp, np.uint64)
+        self.fn(f"G1_{coord}_MSM_binary_coeff_{out}_out")(
+            len(bits), bits.ctypes.data_as(ctypes.c_void_p), ptr(np.ascontiguousarray(points)), ptr(res))
+        return res
+
+    def multi_msm_binary(self, rows, points, coord="proj", out="affine"):
+        rows = np.ascontiguousarray(rows, dtype=np.uint8)
+        nr, n = rows.shape
+        res = np.zeros((nr, 2 * self.lp if out == "affine" else 3 * self.lp), np.uint64)
+        ptrs = (ctypes.c_void_p * max(nr, 1))(*[rows.ctypes.data + r * n for r in range(nr)])
+        self.fn(f"G1_{coord}_multi_MSM_binary_coeff_{out}_out")(nr, n, ptrs, ptr(np.ascontiguousarray(points)),
+                                                                ptr(res))
+        return res

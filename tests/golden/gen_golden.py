@@ -44,7 +44,8 @@ def main():
     assert ref() is not None, "build the reference first: make -C oracle ref"
     out = {}
     manifest = {"generator": "tests/golden/gen_golden.py", "reference": "oracle/_ref/libzkref.so (lib/cbits sources, clang -O2)",
-                "msm": [], "ntt": [], "large": [], "g1fft": []}
+                "msm": [], "ntt": [], "large": [], "g1fft": [], "binary": [],
+                "conv": [], "g1fft_jac": []}
     for C in (BN, BLS):
         R, O = Ref(C), Oracle(C)
         for n in (1, 3, 17, 256, 1000):
@@ -95,6 +96,47 @@ def main():
                 out[f"{key}_{gname}_fwd"] = R.g1_fft(m, g, P)
                 out[f"{key}_{gname}_inv"] = R.g1_fft(m, g, P, inverse=True)
                 manifest["g1fft"].append(f"{key}_{gname}")
+        # Jacobian G1 FFT (bn128_G1_jac.c:840-950): Jacobian points with Z != 1 and an infinity
+        for m in (1, 3, 4):
+            ks = random_scalars(8500 + m + 100 * C.id, 1 << m, C.r)
+            ks[1] = 0
+            P = O.jac_points(ks)
+            key = f"{C.name}_g1fftjac_{m}"
+            g = np.array(to_limbs(C.fr_to_mont(C.fft_root(m)), 4), np.uint64)
+            out[key + "_src"], out[key + "_gen"] = P, g
+            out[key + "_fwd"] = R.g1_fft(m, g, P, coord="jac")
+            out[key + "_inv"] = R.g1_fft(m, g, P, inverse=True, coord="jac")
+            manifest["g1fft_jac"].append(key)
+        # binary and multi-binary MSMs (bn128_G1_proj.c:684-782): byte coefficients, any non-zero
+        # byte selects the point; points include an infinity, a repeated point and a negation
+        rng = np.random.default_rng(13000 + C.id)
+        for n in (1, 40, 300):
+            ks, _ = msm_inputs(C, n, 13100 + n + 100 * C.id)
+            P = O.points_from_scalars(ks)
+            bits = (rng.random(n) < 0.5).astype(np.uint8) * rng.integers(1, 256, n).astype(np.uint8)
+            if n >= 8:
+                bits[3:8] = 1  # infinity, the repeated pair and the cancelling pair selected
+            rows = (rng.random((21, n)) < 0.4).astype(np.uint8)
+            rows[0] = 0
+            rows[1] = 1
+            key = f"{C.name}_binary_{n}"
+            out[key + "_points"], out[key + "_bits"], out[key + "_rows"] = P, bits, rows
+            out[key + "_affine"] = R.msm_binary(bits, P)
+            out[key + "_jac_affine"] = R.msm_binary(bits, P, coord="jac")
+            out[key + "_multi_affine"] = R.multi_msm_binary(rows, P)
+            out[key + "_multi_jac_affine"] = R.multi_msm_binary(rows, P, coord="jac")
+            manifest["binary"].append(key)
+        # batch conversions (bn128_G1_proj.c:146-166, bn128_G1_jac.c:139-158)
+        ks = random_scalars(13500 + 100 * C.id, 37, C.r)
+        ks[0] = ks[20] = 0
+        key = f"{C.name}_conv"
+        out[key + "_proj"], out[key + "_jac"] = O.proj_points(ks), O.jac_points(ks)
+        out[key + "_affine"] = O.points_from_scalars(ks)
+        out[key + "_proj_to_affine"] = R.batch_to_affine(out[key + "_proj"])
+        out[key + "_jac_to_affine"] = R.batch_to_affine(out[key + "_jac"], coord="jac")
+        out[key + "_proj_from_affine"] = R.batch_from_affine(out[key + "_affine"])
+        out[key + "_jac_from_affine"] = R.batch_from_affine(out[key + "_affine"], coord="jac")
+        manifest["conv"].append(key)
         # large NTTs: output digests only (inputs from seeded_fr_vector, taken as Montgomery words)
         for m in (16, 20):
             X = seeded_fr_vector(9000 + m + 100 * C.id, 1 << m, C.r)

@@ -47,8 +47,71 @@ int ntt_host(int curve, int m, const uint64_t *gen, const uint64_t *eta, int fla
 
 const zkh::FieldDesc &fr(int curve) { return curve == 0 ? zkh::BN254_FR : zkh::BLS381_FR; }
 
+// host-in / host-out binary MSMs (bn128_G1_proj.c:684-782): the points go to the device once
+// (converted to the internal form), every row's coefficient bytes become 0/1 scalars, and the
+// rows run as one batch of MSMs on concurrent streams
+int binary_host(int curve, int nrows, int npoints, const uint8_t *const *rows, const uint64_t *grps, uint64_t *tgt,
+                int flags) {
+  const int lp = curve == 0 ? 4 : 6;
+  const int outw = (flags & (ZKG_MSM_PROJ_OUT | ZKG_MSM_JAC_OUT)) ? 3 * lp : 2 * lp;
+  if (nrows <= 0) return ZK_OK;
+  if (npoints <= 0) {
+    for (int r = 0; r < nrows; r++) {
+      MsmResult res;
+      msm_infinity(curve, &res);
+      ZK_TRY(msm_write_result(curve, res, tgt + (size_t)r * outw, flags));
+    }
+    return ZK_OK;
+  }
+  Context *ctx;
+  ZK_TRY(with_ctx(&ctx));
+  std::lock_guard<std::recursive_mutex> lk(ctx->lock);
+  DevBuf dp, dint, bits, sc;
+  const size_t pb = (size_t)npoints * 2 * lp * 8;
+  ZK_TRY(dp.alloc(pb));
+  ZK_TRY(copy_h2d(dp.p, grps, pb, ctx->stream));
+  ZK_TRY(dint.alloc((size_t)npoints * msm_internal_point_bytes(curve)));
+  ZK_TRY(msm_prepare_points(curve, npoints, dp.as<uint64_t>(), dint.p, ctx->stream));
+  ZK_TRY(bits.alloc((size_t)npoints));
+  ZK_TRY(sc.alloc((size_t)nrows * npoints * 8));
+  for (int r = 0; r < nrows; r++) {
+    ZK_TRY(copy_h2d(bits.p, rows[r], (size_t)npoints, ctx->stream));
+    ZK_TRY(binary_scalars(npoints, bits.as<uint8_t>(), sc.as<uint64_t>() + (size_t)r * npoints, ctx->stream));
+  }
+  ZK_HIP(hipStreamSynchronize(ctx->stream));
+  std::vector<int> curves(nrows, curve), ns(nrows, npoints);
+  std::vector<const uint64_t *> scp(nrows), pts(nrows, (const uint64_t *)dint.p);
+  std::vector<uint64_t *> outs(nrows);
+  for (int r = 0; r < nrows; r++) {
+    scp[r] = sc.as<uint64_t>() + (size_t)r * npoints;
+    outs[r] = tgt + (size_t)r * outw;
+  }
+  zkg_msm_schedule s;
+  memset(&s, 0, sizeof s);
+  s.policy = ZKG_SCHED_STATS;
+  return msm_batch(nrows, curves.data(), ns.data(), scp.data(), 1, 0, pts.data(), true, outs.data(),
+                   flags & (ZKG_MSM_PROJ_OUT | ZKG_MSM_JAC_OUT), &s, nrows > 1 ? 4 : 1, nullptr);
+}
+
+// host-in / host-out batch conversions (bn128_G1_proj.c:146-166, bn128_G1_jac.c:139-158)
+int batch_conv_host(int curve, bool jac, int n, const uint64_t *src, uint64_t *tgt, bool to_affine) {
+  if (n <= 0) return ZK_OK;
+  const int lp = curve == 0 ? 4 : 6;
+  Context *ctx;
+  ZK_TRY(with_ctx(&ctx));
+  std::lock_guard<std::recursive_mutex> lk(ctx->lock);
+  const size_t sb = (size_t)n * (to_affine ? 3 : 2) * lp * 8, tb = (size_t)n * (to_affine ? 2 : 3) * lp * 8;
+  DevBuf a, b;
+  ZK_TRY(a.alloc(sb));
+  ZK_TRY(b.alloc(tb));
+  ZK_TRY(copy_h2d(a.p, src, sb, ctx->stream));
+  if (to_affine) ZK_TRY(g1_batch_to_affine(curve, jac, n, a.as<uint64_t>(), b.as<uint64_t>(), ctx->stream));
+  else ZK_TRY(g1_batch_from_affine(curve, jac, n, a.as<uint64_t>(), b.as<uint64_t>(), ctx->stream));
+  return copy_d2h(tgt, b.p, tb, ctx->stream);
+}
+
 // host-in / host-out G1 FFT (bn128_G1_proj.c:854-866, :890-902): 2^m projective points
-int g1_fft_host(int curve, int m, const uint64_t *gen, bool inverse, const uint64_t *src, uint64_t *tgt) {
+int g1_fft_host(int curve, int m, const uint64_t *gen, bool inverse, bool jac, const uint64_t *src, uint64_t *tgt) {
   if (m < 0 || m > 26) {
     set_error("g1_fft: log size out of range");
     return ZK_EINVAL;
@@ -61,7 +124,7 @@ int g1_fft_host(int curve, int m, const uint64_t *gen, bool inverse, const uint6
   ZK_TRY(a.alloc(bytes));
   ZK_TRY(b.alloc(bytes));
   ZK_TRY(copy_h2d(a.p, src, bytes, ctx->stream));
-  ZK_TRY(g1_fft_device(curve, m, gen, inverse, a.as<uint64_t>(), b.as<uint64_t>(), ctx->stream));
+  ZK_TRY(g1_fft_device(curve, m, gen, inverse, jac, a.as<uint64_t>(), b.as<uint64_t>(), ctx->stream));
   ZK_TRY(copy_d2h(tgt, b.p, bytes, ctx->stream));
   return ZK_OK;
 }
@@ -195,26 +258,78 @@ MSM_SYMBOL(bls12_381_G1_jac_MSM_std_coeff_affine_out, 1, 0)
 MSM_SYMBOL(bls12_381_G1_jac_MSM_mont_coeff_affine_out, 1, ZKG_MSM_MONT)
 
 // ---------------- reference G1 FFT symbols ----------------
-#define G1_FFT_SYMBOLS(PFX, CID)                                                                       \
-  void PFX##_G1_proj_fft_forward(int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt) {     \
-    check(#PFX "_G1_proj_fft_forward", g1_fft_host(CID, m, gen, false, src, tgt));                    \
+#define G1_FFT_SYMBOLS(PFX, CID, CO, JAC)                                                              \
+  void PFX##_G1_##CO##_fft_forward(int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt) {   \
+    check(#PFX "_G1_" #CO "_fft_forward", g1_fft_host(CID, m, gen, false, JAC, src, tgt));           \
   }                                                                                                    \
-  void PFX##_G1_proj_fft_inverse(int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt) {     \
-    check(#PFX "_G1_proj_fft_inverse", g1_fft_host(CID, m, gen, true, src, tgt));                     \
+  void PFX##_G1_##CO##_fft_inverse(int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt) {   \
+    check(#PFX "_G1_" #CO "_fft_inverse", g1_fft_host(CID, m, gen, true, JAC, src, tgt));            \
   }
 
-G1_FFT_SYMBOLS(bn128, 0)
-G1_FFT_SYMBOLS(bls12_381, 1)
+G1_FFT_SYMBOLS(bn128, 0, proj, false)
+G1_FFT_SYMBOLS(bls12_381, 1, proj, false)
+G1_FFT_SYMBOLS(bn128, 0, jac, true)
+G1_FFT_SYMBOLS(bls12_381, 1, jac, true)
+
+// ---------------- reference binary MSM and batch conversion symbols ----------------
+#define G1_EXTRA_SYMBOLS(PFX, CID, CO, OUTF, JAC)                                                       \
+  void PFX##_G1_##CO##_MSM_binary_coeff_##CO##_out(int npoints, const uint8_t *expos, const uint64_t *grps, \
+                                                   uint64_t *tgt) {                                       \
+    check(#PFX "_G1_" #CO "_MSM_binary_coeff_" #CO "_out", binary_host(CID, 1, npoints, &expos, grps, tgt, OUTF)); \
+  }                                                                                                      \
+  void PFX##_G1_##CO##_MSM_binary_coeff_affine_out(int npoints, const uint8_t *expos, const uint64_t *grps, \
+                                                   uint64_t *tgt) {                                       \
+    check(#PFX "_G1_" #CO "_MSM_binary_coeff_affine_out", binary_host(CID, 1, npoints, &expos, grps, tgt, 0)); \
+  }                                                                                                      \
+  void PFX##_G1_##CO##_multi_MSM_binary_coeff_##CO##_out(int nrows, int npoints, const uint8_t **expos,  \
+                                                         const uint64_t *grps, uint64_t *tgt) {          \
+    check(#PFX "_G1_" #CO "_multi_MSM_binary_coeff_" #CO "_out",                                         \
+          binary_host(CID, nrows, npoints, expos, grps, tgt, OUTF));                                      \
+  }                                                                                                      \
+  void PFX##_G1_##CO##_multi_MSM_binary_coeff_affine_out(int nrows, int npoints, const uint8_t **expos,  \
+                                                         const uint64_t *grps, uint64_t *tgt) {          \
+    check(#PFX "_G1_" #CO "_multi_MSM_binary_coeff_affine_out",                                           \
+          binary_host(CID, nrows, npoints, expos, grps, tgt, 0));                                         \
+  }                                                                                                      \
+  void PFX##_G1_##CO##_batch_to_affine(int N, const uint64_t *src, uint64_t *tgt) {                      \
+    check(#PFX "_G1_" #CO "_batch_to_affine", batch_conv_host(CID, JAC, N, src, tgt, true));             \
+  }                                                                                                      \
+  void PFX##_G1_##CO##_batch_from_affine(int N, const uint64_t *src, uint64_t *tgt) {                    \
+    check(#PFX "_G1_" #CO "_batch_from_affine", batch_conv_host(CID, JAC, N, src, tgt, false));          \
+  }
+
+G1_EXTRA_SYMBOLS(bn128, 0, proj, ZKG_MSM_PROJ_OUT, false)
+G1_EXTRA_SYMBOLS(bls12_381, 1, proj, ZKG_MSM_PROJ_OUT, false)
+G1_EXTRA_SYMBOLS(bn128, 0, jac, ZKG_MSM_JAC_OUT, true)
+G1_EXTRA_SYMBOLS(bls12_381, 1, jac, ZKG_MSM_JAC_OUT, true)
 
 // ---------------- extension API ----------------
 
 const char *zkg_last_error(void) { return last_error(); }
 
-int zkg_g1_fft(int curve, int m, const uint64_t *gen, int inverse, const uint64_t *d_src, uint64_t *d_tgt) {
+int zkg_g1_batch_to_affine(int curve, int jac, int n, const uint64_t *d_src, uint64_t *d_tgt) {
   Context *ctx;
   ZK_TRY(with_ctx(&ctx));
   std::lock_guard<std::recursive_mutex> lk(ctx->lock);
-  return g1_fft_device(curve, m, gen, inverse != 0, d_src, d_tgt, ctx->stream);
+  return g1_batch_to_affine(curve, jac != 0, n, d_src, d_tgt, ctx->stream);
+}
+
+int zkg_msm_binary(int curve, int npoints, const uint8_t *d_bits, const uint64_t *d_grps, uint64_t *tgt_host,
+                   int flags) {
+  Context *ctx;
+  ZK_TRY(with_ctx(&ctx));
+  std::lock_guard<std::recursive_mutex> lk(ctx->lock);
+  MsmResult r;
+  ZK_TRY(msm_binary_device(curve, npoints, d_bits, d_grps, (flags & ZKG_MSM_POINTS_INTERNAL) != 0, ctx->stream, &r));
+  return msm_write_result(curve, r, tgt_host, flags);
+}
+
+int zkg_g1_fft(int curve, int m, const uint64_t *gen, int mode, const uint64_t *d_src, uint64_t *d_tgt) {
+  Context *ctx;
+  ZK_TRY(with_ctx(&ctx));
+  std::lock_guard<std::recursive_mutex> lk(ctx->lock);
+  return g1_fft_device(curve, m, gen, (mode & ZKG_G1_FFT_INVERSE) != 0, (mode & ZKG_G1_FFT_JAC) != 0, d_src, d_tgt,
+                       ctx->stream);
 }
 
 int zkg_device_info(int *num_cus, size_t *hbm_bytes) {

@@ -29,8 +29,9 @@ __device__ __forceinline__ uint32_t g1_bitrev(uint32_t x, int bits) {
   return bits ? (__builtin_bitreverse32(x) >> (32 - bits)) : 0u;
 }
 
-// projective external (X:Y:Z) -> XYZZ (X Z, Y Z^2, Z^2, Z^3); Z = 0 is the infinity
-template <class C>
+// projective external (X:Y:Z) -> XYZZ (X Z, Y Z^2, Z^2, Z^3); Jacobian (X:Y:Z), whose affine
+// point is (X/Z^2, Y/Z^3), -> (X, Y, Z^2, Z^3); Z = 0 is the infinity
+template <class C, bool JAC>
 __global__ __launch_bounds__(kG1Threads) void g1_proj_in_kernel(const uint32_t *__restrict__ src, uint32_t N, int m,
                                                                int bitrev_out, int32_t *__restrict__ dst) {
   using F = typename C::Fp;
@@ -51,14 +52,20 @@ __global__ __launch_bounds__(kG1Threads) void g1_proj_in_kernel(const uint32_t *
     zfrom_ext(z, p + 2 * LW);
     zsqr(q.zz, z);
     zmul(q.zzz, q.zz, z);
-    zmul(q.x, x, z);
-    zmul(q.y, y, q.zz);
+    if (JAC) {
+      q.x = x;
+      q.y = y;
+    } else {
+      zmul(q.x, x, z);
+      zmul(q.y, y, q.zz);
+    }
   }
   const uint32_t j = bitrev_out ? g1_bitrev(i, m) : i;
   zxyzz_store(dst + (size_t)j * PW, q);
 }
 
-// XYZZ -> normalised projective external words (x, y, 1), infinity (0, 1, 0)
+// XYZZ -> normalised external words (x, y, 1), infinity (0, 1, 0); the projective and the
+// Jacobian normalisations agree on these words (bn128_G1_proj.c:78-97, bn128_G1_jac.c:62-85)
 template <class C>
 __global__ __launch_bounds__(kG1Threads) void g1_proj_out_kernel(const int32_t *__restrict__ pts, uint32_t N, int m,
                                                                 int bitrev_in, uint32_t *__restrict__ dst) {
@@ -159,7 +166,7 @@ __global__ __launch_bounds__(kG1Threads) void g1_scale_kernel(int32_t *__restric
 }
 
 template <class C>
-static int g1_fft_run(int m, const uint64_t *gen, bool inverse, const uint64_t *d_src, uint64_t *d_dst,
+static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const uint64_t *d_src, uint64_t *d_dst,
                       hipStream_t st) {
   using FR = typename C::Fr;
   constexpr int ZW = ZLayout<C>::XYZZ;
@@ -183,8 +190,12 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, const uint64_t *
   };
   const dim3 blk(kG1Threads), gN((N + kG1Threads - 1) / kG1Threads), gH((halfN + kG1Threads - 1) / kG1Threads);
   if (halfN) hipLaunchKernelGGL(g1_twiddle_kernel<C>, gH, blk, 0, st, tw.as<uint32_t>(), to_fr(g), halfN);
-  hipLaunchKernelGGL(g1_proj_in_kernel<C>, gN, blk, 0, st, (const uint32_t *)d_src, N, m, inverse ? 0 : 1,
-                     pts.as<int32_t>());
+  if (jac)
+    hipLaunchKernelGGL((g1_proj_in_kernel<C, true>), gN, blk, 0, st, (const uint32_t *)d_src, N, m, inverse ? 0 : 1,
+                       pts.as<int32_t>());
+  else
+    hipLaunchKernelGGL((g1_proj_in_kernel<C, false>), gN, blk, 0, st, (const uint32_t *)d_src, N, m, inverse ? 0 : 1,
+                       pts.as<int32_t>());
   if (!inverse) {
     for (int s = 1; s <= m; s++)
       hipLaunchKernelGGL((g1_fft_stage_kernel<C, false>), gH, blk, 0, st, pts.as<int32_t>(), tw.as<uint32_t>(), m, s);
@@ -203,14 +214,14 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, const uint64_t *
   return ZK_OK;
 }
 
-int g1_fft_device(int curve, int m, const uint64_t *gen, bool inverse, const uint64_t *d_src, uint64_t *d_dst,
-                  hipStream_t st) {
+int g1_fft_device(int curve, int m, const uint64_t *gen, bool inverse, bool jac, const uint64_t *d_src,
+                  uint64_t *d_dst, hipStream_t st) {
   if (m < 0 || m > 26) {
     set_error("g1_fft: log size out of range");
     return ZK_EINVAL;
   }
-  if (curve == 0) return g1_fft_run<Bn254G1>(m, gen, inverse, d_src, d_dst, st);
-  if (curve == 1) return g1_fft_run<Bls381G1>(m, gen, inverse, d_src, d_dst, st);
+  if (curve == 0) return g1_fft_run<Bn254G1>(m, gen, inverse, jac, d_src, d_dst, st);
+  if (curve == 1) return g1_fft_run<Bls381G1>(m, gen, inverse, jac, d_src, d_dst, st);
   set_error("g1_fft: bad curve id");
   return ZK_EINVAL;
 }

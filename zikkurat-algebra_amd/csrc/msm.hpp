@@ -58,9 +58,15 @@ size_t msm_internal_point_bytes(int curve);
 size_t msm_footprint(int curve, int n, int enl, bool points_internal, const zkg_msm_schedule *sched);
 
 int g1_scalar_mul_gen(int curve, int n, const uint64_t *d_scalars, uint64_t *d_out, hipStream_t st);
+// batch conversions (external Montgomery words) and binary-coefficient MSMs
+int g1_batch_to_affine(int curve, bool jac, int n, const uint64_t *d_src, uint64_t *d_dst, hipStream_t st);
+int g1_batch_from_affine(int curve, bool jac, int n, const uint64_t *d_src, uint64_t *d_dst, hipStream_t st);
+int binary_scalars(int n, const uint8_t *d_bits, uint64_t *d_scalars, hipStream_t st);
+int msm_binary_device(int curve, int n, const uint8_t *d_bits, const uint64_t *d_points, bool points_internal,
+                      hipStream_t st, MsmResult *out);
 // FFT of 2^m projective G1 points (external Montgomery words), normalised outputs
-int g1_fft_device(int curve, int m, const uint64_t *gen, bool inverse, const uint64_t *d_src, uint64_t *d_dst,
-                  hipStream_t st);
+int g1_fft_device(int curve, int m, const uint64_t *gen, bool inverse, bool jac, const uint64_t *d_src,
+                  uint64_t *d_dst, hipStream_t st);
 int srs_generate(int curve, int n, const uint64_t *tau_std, uint64_t *d_out, hipStream_t st);
 
 struct Server;

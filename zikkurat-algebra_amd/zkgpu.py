@@ -123,6 +123,8 @@ def lib():
         L.zkg_msm_set_cost_model.argtypes = [vp]
         L.zkg_ntt_set_tuning.argtypes = [ctypes.c_int, ctypes.c_int]
         L.zkg_g1_fft.argtypes = [ctypes.c_int, ctypes.c_int, u64p, ctypes.c_int, vp, vp]
+        L.zkg_g1_batch_to_affine.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp]
+        L.zkg_msm_binary.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp, u64p, ctypes.c_int]
         L.zkg_ntt_set_tuning_for.argtypes = [ctypes.c_int] * 4
         L.zkg_ntt_tune.argtypes = [ctypes.c_int] * 5 + [ip, ip, dp, ip]
         _lib = L
@@ -159,6 +161,14 @@ class DeviceArray:
         d = cls(a.size)
         _check(lib().zkg_memcpy_h2d(d.ptr, a.ctypes.data_as(vp), 8 * a.size))
         return d
+
+    @classmethod
+    def from_bytes(cls, b):
+        """A byte array, zero-padded to whole words (for coefficient bytes of binary MSMs)."""
+        b = np.ascontiguousarray(b, dtype=np.uint8).ravel()
+        padded = np.zeros((b.size + 7) // 8 * 8, np.uint8)
+        padded[:b.size] = b
+        return cls.from_host(padded.view(np.uint64))
 
     def to_host(self, shape=None):
         out = np.empty(self.nwords, dtype=np.uint64)
@@ -279,21 +289,103 @@ def asymm_ntt(curve, m_src, gen_src, src, m_tgt, gen_tgt):
     return out
 
 
-def g1_fft(curve, m, gen, src, inverse=False):
-    """G1 FFT through the reference entry point <c>_G1_proj_fft_{forward,inverse}: src is
-    (2^m, 3 L_p) projective Montgomery points; outputs normalised."""
+def g1_fft(curve, m, gen, src, inverse=False, coord="proj"):
+    """G1 FFT through the reference entry point <c>_G1_<coord>_fft_{forward,inverse}: src is
+    (2^m, 3 L_p) projective (or Jacobian) Montgomery points; outputs normalised."""
     src = _c(src)
     if src.shape[0] != (1 << m):
         raise ValueError("g1 fft: subgroup size differs from the array size")
     out = np.zeros_like(src)
-    f = _ref_fn(curve, "G1_proj_fft_inverse" if inverse else "G1_proj_fft_forward")
+    f = _ref_fn(curve, f"G1_{coord}_fft_{'inverse' if inverse else 'forward'}")
     f.argtypes = [ctypes.c_int, u64p, u64p, u64p]
     f(m, _p(_c(gen)), _p(src), _p(out))
     return out
 
 
-def g1_fft_device(curve, m, gen, d_src, d_tgt, inverse=False):
-    _check(lib().zkg_g1_fft(curve, m, _p(_c(gen)), int(inverse), d_src.ptr, d_tgt.ptr))
+G1_FFT_INVERSE, G1_FFT_JAC = 1, 2
+
+
+def g1_fft_device(curve, m, gen, d_src, d_tgt, inverse=False, jac=False):
+    mode = (G1_FFT_INVERSE if inverse else 0) | (G1_FFT_JAC if jac else 0)
+    _check(lib().zkg_g1_fft(curve, m, _p(_c(gen)), mode, d_src.ptr, d_tgt.ptr))
+
+
+def batch_to_affine(curve, pts, coord="proj"):
+    """<c>_G1_<coord>_batch_to_affine (bn128_G1_proj.c:157-166): (N, 3 L_p) -> (N, 2 L_p),
+    infinity -> all 0xFF words."""
+    pts = _c(pts)
+    L = FP_LIMBS[curve]
+    n = pts.size // (3 * L)
+    out = np.zeros((n, 2 * L), np.uint64)
+    f = _ref_fn(curve, f"G1_{coord}_batch_to_affine")
+    f.argtypes = [ctypes.c_int, u64p, u64p]
+    f(n, _p(pts), _p(out))
+    return out
+
+
+def batch_from_affine(curve, aff, coord="proj"):
+    """<c>_G1_<coord>_batch_from_affine (bn128_G1_proj.c:146-155): (N, 2 L_p) -> (N, 3 L_p)."""
+    aff = _c(aff)
+    L = FP_LIMBS[curve]
+    n = aff.size // (2 * L)
+    out = np.zeros((n, 3 * L), np.uint64)
+    f = _ref_fn(curve, f"G1_{coord}_batch_from_affine")
+    f.argtypes = [ctypes.c_int, u64p, u64p]
+    f(n, _p(aff), _p(out))
+    return out
+
+
+def batch_to_affine_device(curve, n, d_src, d_tgt, jac=False):
+    _check(lib().zkg_g1_batch_to_affine(curve, int(jac), n, d_src.ptr, d_tgt.ptr))
+
+
+def _bytes_ptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+
+
+def msm_binary(curve, bits, points, coord="proj", out="affine"):
+    """<c>_G1_<coord>_MSM_binary_coeff_<out>_out (bn128_G1_proj.c:684-698): the sum of the
+    points whose byte in `bits` is non-zero.  out is "affine" or the coordinate system."""
+    bits = np.ascontiguousarray(bits, dtype=np.uint8)
+    points = _c(points)
+    L = FP_LIMBS[curve]
+    n = points.size // (2 * L)
+    if bits.size != n:
+        raise ValueError("msm_binary: coefficient and point counts differ")
+    res = np.zeros(2 * L if out == "affine" else 3 * L, np.uint64)
+    f = _ref_fn(curve, f"G1_{coord}_MSM_binary_coeff_{out}_out")
+    f.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_uint8), u64p, u64p]
+    f(n, _bytes_ptr(bits), _p(points), _p(res))
+    return res
+
+
+def multi_msm_binary(curve, rows, points, coord="proj", out="affine"):
+    """<c>_G1_<coord>_multi_MSM_binary_coeff_<out>_out (bn128_G1_proj.c:716-782): one binary
+    MSM per row of the (nrows, npoints) byte matrix, all over the same points."""
+    rows = np.ascontiguousarray(rows, dtype=np.uint8)
+    points = _c(points)
+    L = FP_LIMBS[curve]
+    n = points.size // (2 * L)
+    if rows.ndim != 2 or rows.shape[1] != n:
+        raise ValueError("multi_msm_binary: need an (nrows, npoints) coefficient matrix")
+    nrows = rows.shape[0]
+    w = 2 * L if out == "affine" else 3 * L
+    res = np.zeros((nrows, w), np.uint64)
+    u8p = ctypes.POINTER(ctypes.c_uint8)
+    ptrs = (u8p * max(nrows, 1))(*[ctypes.cast(rows.ctypes.data + r * n, u8p) for r in range(nrows)])
+    f = _ref_fn(curve, f"G1_{coord}_multi_MSM_binary_coeff_{out}_out")
+    f.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(u8p), u64p, u64p]
+    f(nrows, n, ptrs, _p(points), _p(res))
+    return res
+
+
+def msm_binary_device(curve, n, d_bits, d_points, flags=0):
+    """Binary MSM on device buffers: d_bits holds n bytes; flags as msm_device (output form,
+    MSM_POINTS_INTERNAL)."""
+    L = FP_LIMBS[curve]
+    res = np.zeros(3 * L if flags & (MSM_PROJ_OUT | MSM_JAC_OUT) else 2 * L, np.uint64)
+    _check(lib().zkg_msm_binary(curve, n, d_bits.ptr, d_points.ptr, _p(res), flags))
+    return res
 
 
 def fft_generator(curve, m):
