@@ -149,6 +149,22 @@ void bn128_G1_jac_batch_from_affine(int N, const uint64_t *src, uint64_t *tgt);
 void bls12_381_G1_jac_batch_to_affine(int N, const uint64_t *src, uint64_t *tgt);
 void bls12_381_G1_jac_batch_from_affine(int N, const uint64_t *src, uint64_t *tgt);
 
+/* Polynomial operations of a KZG opening (coefficients: flat Montgomery Fr, 4 limbs each).
+ * eval_at: sum a_i loc^i.  div_by_vanishing: quotient and remainder by x^n - eta, the quotient
+ * zero-filled beyond deg p - n + 1 and the remainder beyond n; the reference's asserts
+ * (nquot >= deg p - n + 1, nrem >= n, quot != NULL) abort with a message.
+ * quot_by_vanishing: the quotient, returning 1 when the remainder is zero. */
+/* lib/cbits/curves/poly/mont/bn128_poly_mont.h:6 (impl bn128_poly_mont.c:225-243) */
+void bn128_poly_mont_eval_at(int n1, const uint64_t *src1, const uint64_t *loc, uint64_t *tgt);
+/* bn128_poly_mont.h:24 (impl bn128_poly_mont.c:317-395) */
+void bn128_poly_mont_div_by_vanishing(int n1, const uint64_t *src1, int expo_n, const uint64_t *eta, int nquot, uint64_t *quot, int nrem, uint64_t *rem);
+/* bn128_poly_mont.h:25 (impl bn128_poly_mont.c:400-413) */
+uint8_t bn128_poly_mont_quot_by_vanishing(int n1, const uint64_t *src1, int expo_n, const uint64_t *eta, int nquot, uint64_t *quot);
+/* lib/cbits/curves/poly/mont/bls12_381_poly_mont.h:6, :24-25 */
+void bls12_381_poly_mont_eval_at(int n1, const uint64_t *src1, const uint64_t *loc, uint64_t *tgt);
+void bls12_381_poly_mont_div_by_vanishing(int n1, const uint64_t *src1, int expo_n, const uint64_t *eta, int nquot, uint64_t *quot, int nrem, uint64_t *rem);
+uint8_t bls12_381_poly_mont_quot_by_vanishing(int n1, const uint64_t *src1, int expo_n, const uint64_t *eta, int nquot, uint64_t *quot);
+
 /* =========================================================================================
  * Part 2: extension API (no reference counterpart)
  * ========================================================================================= */
@@ -248,6 +264,18 @@ int zkg_fill_random(int curve, int field, size_t n, uint64_t seed, uint64_t *d_o
  * output (input generation for tests and benchmarks) */
 int zkg_g1_scalar_mul_gen(int curve, int n, const uint64_t *d_scalars, uint64_t *d_out_affine);
 
+/* polynomial evaluation, degree (-1 for zero) and division by x^n - eta on device buffers;
+ * the result of zkg_poly_eval goes to host memory (one Fr element) */
+int zkg_poly_eval(int curve, int n, const uint64_t *d_coeffs, const uint64_t *loc, uint64_t *out_host);
+int zkg_poly_degree(int n, const uint64_t *d_coeffs, int *deg);
+int zkg_poly_div_by_vanishing(int curve, int n1, const uint64_t *d_src, int expo_n, const uint64_t *eta,
+                              int nquot, uint64_t *d_quot, int nrem, uint64_t *d_rem);
+/* KZG opening (examples/KZG.hs:108-114) of the polynomial with n Montgomery coefficients at z:
+ * value_host = p(z), proof_host = MSM of (p - p(z)) / (x - z) over the first n - 1 SRS points
+ * (d_srs: affine points, or the prepared form with srs_flags = ZKG_MSM_POINTS_INTERNAL);
+ * out_flags selects the proof's output form as for zkg_msm */
+int zkg_kzg_open(int curve, int n, const uint64_t *d_coeffs, const uint64_t *z, const uint64_t *d_srs,
+                 int srs_flags, uint64_t *value_host, uint64_t *proof_host, int out_flags);
 /* projective (jac = 0) or Jacobian (jac = 1) -> affine for n device points (Montgomery batch
  * inversion, 16 points per thread) */
 int zkg_g1_batch_to_affine(int curve, int jac, int n, const uint64_t *d_src, uint64_t *d_tgt);

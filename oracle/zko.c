@@ -750,3 +750,80 @@ void zko_multi_msm_binary_proj(int cv, int nrows, int npoints, const uint8_t *co
   int n = CURVES[cv].fp->nl;
   for (int r = 0; r < nrows; r++) zko_msm_binary_proj(cv, npoints, rows[r], grps, tgt + (size_t)r * 3 * n);
 }
+
+/* ---------------- polynomials of a KZG opening (bn128_poly_mont.c) ---------------- */
+static int poly_degree(int cv, int n1, const uint64_t *a) {
+  const field_t *f = F(cv, ZKO_FR);
+  for (int i = n1 - 1; i >= 0; i--)
+    if (!is_zero(f, a + 4 * (size_t)i)) return i;
+  return -1;
+}
+int zko_poly_degree(int cv, int n1, const uint64_t *a) { return poly_degree(cv, n1, a); }
+
+/* bn128_poly_mont.c:225-243: sum a_i x^i with a running power of x */
+void zko_poly_eval(int cv, int n1, const uint64_t *a, const uint64_t *x, uint64_t *tgt) {
+  const field_t *f = F(cv, ZKO_FR);
+  uint64_t run[MAXL], t[MAXL];
+  set0(f, tgt);
+  cpy(f, run, f->one);
+  for (int i = 0; i < n1; i++) {
+    f_mul(f, a + 4 * (size_t)i, run, t);
+    f_add(f, tgt, t, tgt);
+    f_mul(f, run, x, run);
+  }
+}
+
+/* bn128_poly_mont.c:317-395: q_j = a_{j+n} + eta q_{j+n} from j = D-n down to 0 (q_j = a_{j+n}
+ * at the top), r_j = a_j + eta q_j for j <= D-n and a_j above; D < n gives q = 0, r = p.
+ * Unused tails are zero.  Returns -1 where the reference's asserts would fail. */
+int zko_poly_div_by_vanishing(int cv, int n1, const uint64_t *a, int n, const uint64_t *eta, int nquot,
+                              uint64_t *quot, int nrem, uint64_t *rem) {
+  const field_t *f = F(cv, ZKO_FR);
+  const int D = poly_degree(cv, n1, a);
+  if (!quot || nquot < D - n + 1 || (rem && nrem < n) || n < 1) return -1;
+  uint64_t t[MAXL];
+  if (D < n) {
+    for (int j = 0; j < nquot; j++) set0(f, quot + 4 * (size_t)j);
+    if (rem) {
+      if (nrem < D + 1) return -1;
+      for (int j = 0; j < nrem; j++) set0(f, rem + 4 * (size_t)j);
+      memcpy(rem, a, 32 * (size_t)(D + 1));
+    }
+    return 0;
+  }
+  for (int j = D - n + 1; j < nquot; j++) set0(f, quot + 4 * (size_t)j);
+  if (rem)
+    for (int j = n; j < nrem; j++) set0(f, rem + 4 * (size_t)j);
+  for (int j = D - n; j >= 0; j--) {
+    if (j + n <= D - n) {
+      f_mul(f, quot + 4 * (size_t)(j + n), eta, t);
+      f_add(f, a + 4 * (size_t)(j + n), t, quot + 4 * (size_t)j);
+    } else {
+      cpy(f, quot + 4 * (size_t)j, a + 4 * (size_t)(j + n));
+    }
+  }
+  if (rem)
+    for (int j = 0; j < n; j++) {
+      if (j <= D - n) {
+        f_mul(f, quot + 4 * (size_t)j, eta, t);
+        f_add(f, a + 4 * (size_t)j, t, rem + 4 * (size_t)j);
+      } else {
+        cpy(f, rem + 4 * (size_t)j, a + 4 * (size_t)j);
+      }
+    }
+  return 0;
+}
+
+/* bn128_poly_mont.c:400-413: 1 when the remainder is zero, 0 otherwise, -1 on bad sizes */
+int zko_poly_quot_by_vanishing(int cv, int n1, const uint64_t *a, int n, const uint64_t *eta, int nquot,
+                               uint64_t *quot) {
+  if (n < 1) return -1;
+  uint64_t *rem = (uint64_t *)malloc(32 * (size_t)n);
+  if (!rem) abort();
+  int st = zko_poly_div_by_vanishing(cv, n1, a, n, eta, nquot, quot, n, rem);
+  int ok = 1;
+  for (int j = 0; st == 0 && j < 4 * n; j++)
+    if (rem[j]) { ok = 0; break; }
+  free(rem);
+  return st ? st : ok;
+}

@@ -80,6 +80,13 @@ void zko_g1_batch_from_affine(int curve, int jac, int N, const uint64_t *src, ui
 void zko_msm_binary_proj(int curve, int npoints, const uint8_t *expos, const uint64_t *grps, uint64_t *tgt);
 void zko_multi_msm_binary_proj(int curve, int nrows, int npoints, const uint8_t *const *rows,
                                const uint64_t *grps, uint64_t *tgt);
+/* polynomials over Fr (flat Montgomery coefficients; bn128_poly_mont.c:26-32, :225-243, :317-413) */
+int  zko_poly_degree(int curve, int n1, const uint64_t *a);
+void zko_poly_eval(int curve, int n1, const uint64_t *a, const uint64_t *x, uint64_t *tgt);
+int  zko_poly_div_by_vanishing(int curve, int n1, const uint64_t *a, int n, const uint64_t *eta, int nquot,
+                               uint64_t *quot, int nrem, uint64_t *rem);
+int  zko_poly_quot_by_vanishing(int curve, int n1, const uint64_t *a, int n, const uint64_t *eta, int nquot,
+                                uint64_t *quot);
 /* generator of the size-2^m FFT subgroup, Montgomery form (Class/FFT.hs:60-68) */
 void zko_fft_generator(int curve, int m, uint64_t *gen);
 

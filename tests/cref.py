@@ -61,6 +61,12 @@ class Oracle:
         self.lib.zko_f_mul(self.c.id, field, ptr(a), ptr(b), ptr(out))
         return out
 
+    def f_sub(self, field, a, b):
+        n = 4 if field == 1 else self.lp
+        out = np.zeros(n, np.uint64)
+        self.lib.zko_f_sub(self.c.id, field, ptr(np.ascontiguousarray(a)), ptr(np.ascontiguousarray(b)), ptr(out))
+        return out
+
     def msm_affine(self, scalars, points, enl=4, mont=False):
         n = points.shape[0]
         out = np.zeros(2 * self.lp, np.uint64)
@@ -140,6 +146,28 @@ class Oracle:
             fn = self.lib.zko_g1_fft_inverse if inverse else self.lib.zko_g1_fft_forward
         fn(self.c.id, m, ptr(np.ascontiguousarray(gen)), ptr(np.ascontiguousarray(src)), ptr(out))
         return out
+
+    def poly_eval(self, coeffs, x):
+        out = np.zeros(4, np.uint64)
+        c = np.ascontiguousarray(coeffs, dtype=np.uint64)
+        self.lib.zko_poly_eval(self.c.id, c.size // 4, ptr(c), ptr(np.ascontiguousarray(x)), ptr(out))
+        return out
+
+    def div_by_vanishing(self, coeffs, n, eta, nquot, nrem):
+        """(status, quotient, remainder); status -1 where the reference asserts."""
+        c = np.ascontiguousarray(coeffs, dtype=np.uint64)
+        q = np.zeros((max(nquot, 1), 4), np.uint64)
+        r = np.zeros((max(nrem, 1), 4), np.uint64)
+        st = self.lib.zko_poly_div_by_vanishing(self.c.id, c.size // 4, ptr(c), n, ptr(np.ascontiguousarray(eta)),
+                                                nquot, ptr(q), nrem, ptr(r) if nrem else None)
+        return st, q[:nquot], r[:nrem]
+
+    def quot_by_vanishing(self, coeffs, n, eta, nquot):
+        c = np.ascontiguousarray(coeffs, dtype=np.uint64)
+        q = np.zeros((max(nquot, 1), 4), np.uint64)
+        st = self.lib.zko_poly_quot_by_vanishing(self.c.id, c.size // 4, ptr(c), n, ptr(np.ascontiguousarray(eta)),
+                                                 nquot, ptr(q))
+        return st, q[:nquot]
 
     def batch_to_affine(self, P, jac=False):
         P = np.ascontiguousarray(P, dtype=np.uint64)
@@ -247,6 +275,28 @@ class Ref:
         self.fn(f"G1_{coord}_fft_{'inverse' if inverse else 'forward'}")(
             m, ptr(np.ascontiguousarray(gen)), ptr(np.ascontiguousarray(src)), ptr(out))
         return out
+
+    def poly_eval(self, coeffs, x):
+        out = np.zeros(4, np.uint64)
+        c = np.ascontiguousarray(coeffs, dtype=np.uint64)
+        self.fn("poly_mont_eval_at")(c.size // 4, ptr(c), ptr(np.ascontiguousarray(x)), ptr(out))
+        return out
+
+    def div_by_vanishing(self, coeffs, n, eta, nquot, nrem):
+        c = np.ascontiguousarray(coeffs, dtype=np.uint64)
+        q = np.zeros((max(nquot, 1), 4), np.uint64)
+        r = np.zeros((max(nrem, 1), 4), np.uint64)
+        self.fn("poly_mont_div_by_vanishing")(c.size // 4, ptr(c), n, ptr(np.ascontiguousarray(eta)), nquot, ptr(q),
+                                              nrem, ptr(r) if nrem else None)
+        return q[:nquot], r[:nrem]
+
+    def quot_by_vanishing(self, coeffs, n, eta, nquot):
+        c = np.ascontiguousarray(coeffs, dtype=np.uint64)
+        q = np.zeros((max(nquot, 1), 4), np.uint64)
+        f = self.fn("poly_mont_quot_by_vanishing")
+        f.restype = ctypes.c_uint8
+        ok = f(c.size // 4, ptr(c), n, ptr(np.ascontiguousarray(eta)), nquot, ptr(q))
+        return int(ok), q[:nquot]
 
     def batch_to_affine(self, P, coord="proj"):
         P = np.ascontiguousarray(P, dtype=np.uint64)

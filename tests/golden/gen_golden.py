@@ -45,7 +45,7 @@ def main():
     out = {}
     manifest = {"generator": "tests/golden/gen_golden.py", "reference": "oracle/_ref/libzkref.so (lib/cbits sources, clang -O2)",
                 "msm": [], "ntt": [], "large": [], "g1fft": [], "binary": [],
-                "conv": [], "g1fft_jac": []}
+                "conv": [], "g1fft_jac": [], "poly": [], "kzg": []}
     for C in (BN, BLS):
         R, O = Ref(C), Oracle(C)
         for n in (1, 3, 17, 256, 1000):
@@ -137,6 +137,40 @@ def main():
         out[key + "_proj_from_affine"] = R.batch_from_affine(out[key + "_affine"])
         out[key + "_jac_from_affine"] = R.batch_from_affine(out[key + "_affine"], coord="jac")
         manifest["conv"].append(key)
+        # polynomial evaluation and division by x^n - eta (bn128_poly_mont.c:225-243, :317-413)
+        one = np.array(to_limbs(C.fr_to_mont(1), 4), np.uint64)
+        for n1 in (1, 5, 64, 300):
+            cs = random_scalars(14000 + n1 + 100 * C.id, n1, C.r)
+            a = ints_to_array([C.fr_to_mont(c) for c in cs], 4)
+            if n1 >= 5:
+                a[-1] = 0  # a trailing zero coefficient: the degree is below the length
+            x = np.array(to_limbs(C.fr_to_mont(random_scalars(14500 + n1, 1, C.r)[0]), 4), np.uint64)
+            key = f"{C.name}_poly_{n1}"
+            out[key + "_coeffs"], out[key + "_x"] = a, x
+            out[key + "_eval"] = R.poly_eval(a, x)
+            for n in (1, 3, 16, 64):
+                for ename, eta in (("one", one), ("x", x)):
+                    nq, nr = n1 + 2, n + 1
+                    q, r = R.div_by_vanishing(a, n, eta, nq, nr)
+                    out[f"{key}_div_{n}_{ename}_quot"], out[f"{key}_div_{n}_{ename}_rem"] = q, r
+            manifest["poly"].append(key)
+        # KZG opening (examples/KZG.hs:108-114) over an SRS tau^i G of 16 points
+        tau = random_scalars(15000 + C.id, 1, C.r)[0]
+        srs = O.points_from_scalars([pow(tau, i, C.r) for i in range(16)])
+        cs = random_scalars(15100 + C.id, 16, C.r)
+        a = ints_to_array([C.fr_to_mont(c) for c in cs], 4)
+        z = np.array(to_limbs(C.fr_to_mont(random_scalars(15200 + C.id, 1, C.r)[0]), 4), np.uint64)
+        v = R.poly_eval(a, z)
+        pm = a.copy()
+        pm[0] = O.f_sub(1, a[0], v)
+        ok, q = R.quot_by_vanishing(pm, 1, z, 15)
+        assert ok == 1
+        key = f"{C.name}_kzg_16"
+        out[key + "_srs"], out[key + "_coeffs"], out[key + "_z"], out[key + "_tau"] = srs, a, z, np.array(
+            to_limbs(tau, 4), np.uint64)
+        out[key + "_value"] = v
+        out[key + "_proof"] = R.msm(q, srs[:15], mont=True)
+        manifest["kzg"].append(key)
         # large NTTs: output digests only (inputs from seeded_fr_vector, taken as Montgomery words)
         for m in (16, 20):
             X = seeded_fr_vector(9000 + m + 100 * C.id, 1 << m, C.r)

@@ -25,7 +25,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wno-unused-command-line-argument", f"-I{PKG.parent / 'include'}"]
 SOURCES = ["runtime.cpp", "hostfield.cpp", "abi.cpp", "ntt.hip", "msm.hip", "peak.hip", "mlp.hip", "tune.cpp",
-           "serve.cpp", "g1fft.hip", "g1extra.hip"]
+           "serve.cpp", "g1fft.hip", "g1extra.hip", "poly.hip"]
 
 
 def compile_one(src: Path, verbose: bool) -> Path:

@@ -110,6 +110,93 @@ int batch_conv_host(int curve, bool jac, int n, const uint64_t *src, uint64_t *t
   return copy_d2h(tgt, b.p, tb, ctx->stream);
 }
 
+// degree as bn128_poly_mont_degree computes it (bn128_poly_mont.c:26-32): the highest index
+// whose Montgomery words are not all zero, -1 for the zero polynomial
+int host_degree(int n1, const uint64_t *src) {
+  for (int i = n1 - 1; i >= 0; i--) {
+    const uint64_t *c = src + (size_t)i * 4;
+    if (c[0] | c[1] | c[2] | c[3]) return i;
+  }
+  return -1;
+}
+
+// host-in / host-out evaluation (bn128_poly_mont.c:225-243)
+int eval_host(int curve, int n1, const uint64_t *src, const uint64_t *loc, uint64_t *tgt) {
+  if (n1 <= 0) {
+    memset(tgt, 0, 32);
+    return ZK_OK;
+  }
+  Context *ctx;
+  ZK_TRY(with_ctx(&ctx));
+  std::lock_guard<std::recursive_mutex> lk(ctx->lock);
+  DevBuf a;
+  ZK_TRY(a.alloc((size_t)n1 * 32));
+  ZK_TRY(copy_h2d(a.p, src, (size_t)n1 * 32, ctx->stream));
+  return poly_eval_device(curve, n1, a.as<uint64_t>(), loc, tgt, ctx->stream);
+}
+
+// host-in / host-out division by x^n - eta (bn128_poly_mont.c:317-395), with the reference's
+// argument checks (its asserts) as errors and its zero filling of the unused tails
+int vanish_host(int curve, int n1, const uint64_t *src, int n, const uint64_t *eta, int nquot, uint64_t *quot,
+                int nrem, uint64_t *rem) {
+  const int D = host_degree(n1, src);
+  if (!quot) {
+    set_error("div_by_vanishing: the quotient buffer cannot be NULL");
+    return ZK_EINVAL;
+  }
+  if (nquot < D - n + 1 || (rem && nrem < n) || n < 1) {
+    set_error("div_by_vanishing: need nquot >= deg p - n + 1, nrem >= n, n >= 1");
+    return ZK_EINVAL;
+  }
+  if (D < n) {  // the quotient is zero, the remainder is p
+    memset(quot, 0, (size_t)(nquot > 0 ? nquot : 0) * 32);
+    if (rem) {
+      if (nrem < D + 1) {
+        set_error("div_by_vanishing: remainder buffer too small");
+        return ZK_EINVAL;
+      }
+      memset(rem, 0, (size_t)nrem * 32);
+      memcpy(rem, src, (size_t)(D + 1) * 32);
+    }
+    return ZK_OK;
+  }
+  const int M = D - n + 1;
+  for (int j = M; j < nquot; j++) memset(quot + (size_t)j * 4, 0, 32);
+  if (rem)
+    for (int j = n; j < nrem; j++) memset(rem + (size_t)j * 4, 0, 32);
+  Context *ctx;
+  ZK_TRY(with_ctx(&ctx));
+  std::lock_guard<std::recursive_mutex> lk(ctx->lock);
+  DevBuf a, q, r;
+  ZK_TRY(a.alloc((size_t)(D + 1) * 32));
+  ZK_TRY(q.alloc((size_t)M * 32));
+  if (rem) ZK_TRY(r.alloc((size_t)n * 32));
+  ZK_TRY(copy_h2d(a.p, src, (size_t)(D + 1) * 32, ctx->stream));
+  ZK_TRY(vanish_div_device(curve, D, a.as<uint64_t>(), n, eta, q.as<uint64_t>(), rem ? r.as<uint64_t>() : nullptr,
+                           ctx->stream));
+  ZK_TRY(copy_d2h(quot, q.p, (size_t)M * 32, ctx->stream));
+  if (rem) ZK_TRY(copy_d2h(rem, r.p, (size_t)n * 32, ctx->stream));
+  return ZK_OK;
+}
+
+// bn128_poly_mont.c:400-413: the quotient, and whether the remainder is zero
+int quot_vanish_host(int curve, int n1, const uint64_t *src, int n, const uint64_t *eta, int nquot, uint64_t *quot,
+                     uint8_t *ok) {
+  if (n < 1) {
+    set_error("quot_by_vanishing: n must be at least 1");
+    return ZK_EINVAL;
+  }
+  std::vector<uint64_t> rem((size_t)n * 4);
+  ZK_TRY(vanish_host(curve, n1, src, n, eta, nquot, quot, n, rem.data()));
+  *ok = 1;
+  for (uint64_t w : rem)
+    if (w) {
+      *ok = 0;
+      break;
+    }
+  return ZK_OK;
+}
+
 // host-in / host-out G1 FFT (bn128_G1_proj.c:854-866, :890-902): 2^m projective points
 int g1_fft_host(int curve, int m, const uint64_t *gen, bool inverse, bool jac, const uint64_t *src, uint64_t *tgt) {
   if (m < 0 || m > 26) {
@@ -303,6 +390,26 @@ G1_EXTRA_SYMBOLS(bls12_381, 1, proj, ZKG_MSM_PROJ_OUT, false)
 G1_EXTRA_SYMBOLS(bn128, 0, jac, ZKG_MSM_JAC_OUT, true)
 G1_EXTRA_SYMBOLS(bls12_381, 1, jac, ZKG_MSM_JAC_OUT, true)
 
+// ---------------- reference polynomial symbols of a KZG opening ----------------
+#define POLY_SYMBOLS(PFX, CID)                                                                         \
+  void PFX##_poly_mont_eval_at(int n1, const uint64_t *src1, const uint64_t *loc, uint64_t *tgt) {     \
+    check(#PFX "_poly_mont_eval_at", eval_host(CID, n1, src1, loc, tgt));                             \
+  }                                                                                                    \
+  void PFX##_poly_mont_div_by_vanishing(int n1, const uint64_t *src1, int expo_n, const uint64_t *eta, \
+                                        int nquot, uint64_t *quot, int nrem, uint64_t *rem) {          \
+    check(#PFX "_poly_mont_div_by_vanishing",                                                          \
+          vanish_host(CID, n1, src1, expo_n, eta, nquot, quot, nrem, rem));                            \
+  }                                                                                                    \
+  uint8_t PFX##_poly_mont_quot_by_vanishing(int n1, const uint64_t *src1, int expo_n, const uint64_t *eta, \
+                                            int nquot, uint64_t *quot) {                               \
+    uint8_t ok = 0;                                                                                    \
+    check(#PFX "_poly_mont_quot_by_vanishing", quot_vanish_host(CID, n1, src1, expo_n, eta, nquot, quot, &ok)); \
+    return ok;                                                                                         \
+  }
+
+POLY_SYMBOLS(bn128, 0)
+POLY_SYMBOLS(bls12_381, 1)
+
 // ---------------- extension API ----------------
 
 const char *zkg_last_error(void) { return last_error(); }
@@ -312,6 +419,73 @@ int zkg_g1_batch_to_affine(int curve, int jac, int n, const uint64_t *d_src, uin
   ZK_TRY(with_ctx(&ctx));
   std::lock_guard<std::recursive_mutex> lk(ctx->lock);
   return g1_batch_to_affine(curve, jac != 0, n, d_src, d_tgt, ctx->stream);
+}
+
+int zkg_poly_eval(int curve, int n, const uint64_t *d_coeffs, const uint64_t *loc, uint64_t *out_host) {
+  Context *ctx;
+  ZK_TRY(with_ctx(&ctx));
+  std::lock_guard<std::recursive_mutex> lk(ctx->lock);
+  return poly_eval_device(curve, n, d_coeffs, loc, out_host, ctx->stream);
+}
+
+int zkg_poly_degree(int n, const uint64_t *d_coeffs, int *deg) {
+  Context *ctx;
+  ZK_TRY(with_ctx(&ctx));
+  std::lock_guard<std::recursive_mutex> lk(ctx->lock);
+  return poly_degree_device(n, d_coeffs, deg, ctx->stream);
+}
+
+int zkg_poly_div_by_vanishing(int curve, int n1, const uint64_t *d_src, int expo_n, const uint64_t *eta,
+                              int nquot, uint64_t *d_quot, int nrem, uint64_t *d_rem) {
+  if (expo_n < 1 || !d_quot || (d_rem && nrem < expo_n)) {
+    set_error("div_by_vanishing: need n >= 1, a quotient buffer and nrem >= n");
+    return ZK_EINVAL;
+  }
+  Context *ctx;
+  ZK_TRY(with_ctx(&ctx));
+  std::lock_guard<std::recursive_mutex> lk(ctx->lock);
+  int D;
+  ZK_TRY(poly_degree_device(n1, d_src, &D, ctx->stream));
+  if (nquot < D - expo_n + 1) {
+    set_error("div_by_vanishing: need nquot >= deg p - n + 1");
+    return ZK_EINVAL;
+  }
+  const int M = D >= expo_n ? D - expo_n + 1 : 0;
+  if (nquot > M) ZK_HIP(hipMemsetAsync(d_quot + (size_t)M * 4, 0, (size_t)(nquot - M) * 32, ctx->stream));
+  if (d_rem) {
+    const int filled = D >= expo_n ? expo_n : D + 1;
+    if (nrem > filled) ZK_HIP(hipMemsetAsync(d_rem + (size_t)filled * 4, 0, (size_t)(nrem - filled) * 32, ctx->stream));
+    if (D < expo_n && D >= 0)
+      ZK_HIP(hipMemcpyAsync(d_rem, d_src, (size_t)(D + 1) * 32, hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  if (D >= expo_n) ZK_TRY(vanish_div_device(curve, D, d_src, expo_n, eta, d_quot, d_rem, ctx->stream));
+  ZK_HIP(hipStreamSynchronize(ctx->stream));
+  return ZK_OK;
+}
+
+// KZG opening at z of the polynomial with n Montgomery coefficients (examples/KZG.hs:108-114):
+// value p(z), and the commitment of (p - p(z)) / (x - z) to the first n - 1 SRS points
+int zkg_kzg_open(int curve, int n, const uint64_t *d_coeffs, const uint64_t *z, const uint64_t *d_srs,
+                 int srs_flags, uint64_t *value_host, uint64_t *proof_host, int out_flags) {
+  if (curve != 0 && curve != 1) {
+    set_error("kzg_open: bad curve id");
+    return ZK_EINVAL;
+  }
+  Context *ctx;
+  ZK_TRY(with_ctx(&ctx));
+  std::lock_guard<std::recursive_mutex> lk(ctx->lock);
+  ZK_TRY(poly_eval_device(curve, n, d_coeffs, z, value_host, ctx->stream));
+  MsmResult r;
+  if (n < 2) {
+    msm_infinity(curve, &r);
+  } else {
+    DevBuf q;
+    ZK_TRY(q.alloc((size_t)(n - 1) * 32));
+    ZK_TRY(vanish_div_device(curve, n - 1, d_coeffs, 1, z, q.as<uint64_t>(), nullptr, ctx->stream));
+    ZK_TRY(msm_device(curve, n - 1, q.as<uint64_t>(), 4, 1, d_srs, (srs_flags & ZKG_MSM_POINTS_INTERNAL) != 0,
+                      nullptr, ctx->stream, &r));
+  }
+  return msm_write_result(curve, r, proof_host, out_flags);
 }
 
 int zkg_msm_binary(int curve, int npoints, const uint8_t *d_bits, const uint64_t *d_grps, uint64_t *tgt_host,

@@ -64,6 +64,12 @@ int g1_batch_from_affine(int curve, bool jac, int n, const uint64_t *d_src, uint
 int binary_scalars(int n, const uint8_t *d_bits, uint64_t *d_scalars, hipStream_t st);
 int msm_binary_device(int curve, int n, const uint8_t *d_bits, const uint64_t *d_points, bool points_internal,
                       hipStream_t st, MsmResult *out);
+// polynomial evaluation (result to host memory), degree, and division by x^n - eta
+int poly_eval_device(int curve, int n, const uint64_t *d_coeffs, const uint64_t *loc, uint64_t *out_host,
+                     hipStream_t st);
+int poly_degree_device(int n, const uint64_t *d_coeffs, int *deg_host, hipStream_t st);
+int vanish_div_device(int curve, int D, const uint64_t *d_a, int n, const uint64_t *eta, uint64_t *d_quot,
+                      uint64_t *d_rem, hipStream_t st);
 // FFT of 2^m projective G1 points (external Montgomery words), normalised outputs
 int g1_fft_device(int curve, int m, const uint64_t *gen, bool inverse, bool jac, const uint64_t *d_src,
                   uint64_t *d_dst, hipStream_t st);

@@ -123,6 +123,11 @@ def lib():
         L.zkg_msm_set_cost_model.argtypes = [vp]
         L.zkg_ntt_set_tuning.argtypes = [ctypes.c_int, ctypes.c_int]
         L.zkg_g1_fft.argtypes = [ctypes.c_int, ctypes.c_int, u64p, ctypes.c_int, vp, vp]
+        L.zkg_poly_eval.argtypes = [ctypes.c_int, ctypes.c_int, vp, u64p, u64p]
+        L.zkg_poly_degree.argtypes = [ctypes.c_int, vp, ctypes.POINTER(ctypes.c_int)]
+        L.zkg_poly_div_by_vanishing.argtypes = [ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, u64p, ctypes.c_int, vp,
+                                                ctypes.c_int, vp]
+        L.zkg_kzg_open.argtypes = [ctypes.c_int, ctypes.c_int, vp, u64p, vp, ctypes.c_int, u64p, u64p, ctypes.c_int]
         L.zkg_g1_batch_to_affine.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp]
         L.zkg_msm_binary.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp, u64p, ctypes.c_int]
         L.zkg_ntt_set_tuning_for.argtypes = [ctypes.c_int] * 4
@@ -308,6 +313,68 @@ G1_FFT_INVERSE, G1_FFT_JAC = 1, 2
 def g1_fft_device(curve, m, gen, d_src, d_tgt, inverse=False, jac=False):
     mode = (G1_FFT_INVERSE if inverse else 0) | (G1_FFT_JAC if jac else 0)
     _check(lib().zkg_g1_fft(curve, m, _p(_c(gen)), mode, d_src.ptr, d_tgt.ptr))
+
+
+def poly_eval(curve, coeffs, loc):
+    """<c>_poly_mont_eval_at (bn128_poly_mont.c:225-243): sum a_i loc^i, Montgomery Fr words."""
+    coeffs = _c(coeffs)
+    out = np.zeros(4, np.uint64)
+    f = _ref_fn(curve, "poly_mont_eval_at")
+    f.argtypes = [ctypes.c_int, u64p, u64p, u64p]
+    f(coeffs.size // 4, _p(coeffs), _p(_c(loc)), _p(out))
+    return out
+
+
+def div_by_vanishing(curve, coeffs, n, eta, nquot, nrem):
+    """<c>_poly_mont_div_by_vanishing (bn128_poly_mont.c:317-395): quotient and remainder of the
+    polynomial by x^n - eta, in buffers of nquot and nrem coefficients."""
+    coeffs = _c(coeffs)
+    q = np.zeros((max(nquot, 1), 4), np.uint64)
+    r = np.zeros((max(nrem, 1), 4), np.uint64)
+    f = _ref_fn(curve, "poly_mont_div_by_vanishing")
+    f.argtypes = [ctypes.c_int, u64p, ctypes.c_int, u64p, ctypes.c_int, u64p, ctypes.c_int, u64p]
+    f(coeffs.size // 4, _p(coeffs), n, _p(_c(eta)), nquot, _p(q), nrem, _p(r) if nrem else None)
+    return q[:nquot], r[:nrem]
+
+
+def quot_by_vanishing(curve, coeffs, n, eta, nquot):
+    """<c>_poly_mont_quot_by_vanishing (bn128_poly_mont.c:400-413): (remainder is zero, quotient)."""
+    coeffs = _c(coeffs)
+    q = np.zeros((max(nquot, 1), 4), np.uint64)
+    f = _ref_fn(curve, "poly_mont_quot_by_vanishing")
+    f.argtypes = [ctypes.c_int, u64p, ctypes.c_int, u64p, ctypes.c_int, u64p]
+    f.restype = ctypes.c_uint8
+    ok = f(coeffs.size // 4, _p(coeffs), n, _p(_c(eta)), nquot, _p(q))
+    return bool(ok), q[:nquot]
+
+
+def poly_eval_device(curve, n, d_coeffs, loc):
+    out = np.zeros(4, np.uint64)
+    _check(lib().zkg_poly_eval(curve, n, d_coeffs.ptr, _p(_c(loc)), _p(out)))
+    return out
+
+
+def poly_degree_device(n, d_coeffs):
+    d = ctypes.c_int()
+    _check(lib().zkg_poly_degree(n, d_coeffs.ptr, ctypes.byref(d)))
+    return d.value
+
+
+def div_by_vanishing_device(curve, n1, d_src, n, eta, nquot, d_quot, nrem=0, d_rem=None):
+    _check(lib().zkg_poly_div_by_vanishing(curve, n1, d_src.ptr, n, _p(_c(eta)), nquot, d_quot.ptr, nrem,
+                                           d_rem.ptr if d_rem is not None else None))
+
+
+def kzg_open(curve, n, d_coeffs, z, srs, out_flags=0):
+    """KZG opening (examples/KZG.hs:108-114) of the polynomial with n Montgomery coefficients at
+    z: (p(z), commitment of (p - p(z)) / (x - z)); srs is a DeviceArray of affine points or
+    PreparedPoints."""
+    L = FP_LIMBS[curve]
+    value = np.zeros(4, np.uint64)
+    proof = np.zeros(3 * L if out_flags & (MSM_PROJ_OUT | MSM_JAC_OUT) else 2 * L, np.uint64)
+    flags = MSM_POINTS_INTERNAL if isinstance(srs, PreparedPoints) else 0
+    _check(lib().zkg_kzg_open(curve, n, d_coeffs.ptr, _p(_c(z)), srs.ptr, flags, _p(value), _p(proof), out_flags))
+    return value, proof
 
 
 def batch_to_affine(curve, pts, coord="proj"):
