@@ -351,6 +351,116 @@ def bench_g1_fft(m=12, cpu_m=7, seed=6):
     return out
 
 
+def _ref_lib():
+    ref_so = REPO / "oracle" / "_ref" / "libzkref.so"
+    return ctypes.CDLL(str(ref_so)) if ref_so.exists() else None
+
+
+def bench_next(seed=7, logn=20, cpu_log=14):
+    """The rows around the MSM, BN254, device-resident inputs: batch projective -> affine of 2^logn
+    points; a binary MSM of 2^logn points (half selected) and 64 binary rows over 2^16 points;
+    a KZG opening of a 2^logn-coefficient polynomial against a prepared SRS (evaluation, quotient
+    by x - z, MSM), with the evaluation and the quotient alone at 2^22; the SRS written to and
+    read back from a flat file.  CPU: the reference's C on 2^cpu_log-sized samples, one core."""
+    u64p = ctypes.POINTER(ctypes.c_uint64)
+    rng = np.random.default_rng(seed)
+    N = 1 << logn
+    out = {}
+    # batch_to_affine: points (x z, y z, z) from k_i G and random z
+    d_k = zkgpu.fill_random(0, 1, N, seed)
+    aff = zkgpu.scalar_mul_gen(0, d_k, N).to_host((N, 8))
+    Z = zkgpu.fill_random(0, 0, N, seed + 1)
+    d_x, d_y = zkgpu.DeviceArray.from_host(np.ascontiguousarray(aff[:, :4])), zkgpu.DeviceArray.from_host(
+        np.ascontiguousarray(aff[:, 4:]))
+    d_t, d_u = zkgpu.DeviceArray(4 * N), zkgpu.DeviceArray(4 * N)
+    zkgpu.field_op(0, 0, zkgpu.FOP["mul"], d_x, Z, d_t, N)
+    zkgpu.field_op(0, 0, zkgpu.FOP["mul"], d_y, Z, d_u, N)
+    P = np.concatenate([d_t.to_host((N, 4)), d_u.to_host((N, 4)), Z.to_host((N, 4))], axis=1)
+    d_P, d_A = zkgpu.DeviceArray.from_host(P), zkgpu.DeviceArray(8 * N)
+    tb = time_steps(lambda: zkgpu.batch_to_affine_device(0, N, d_P, d_A), 5, 1)
+    ok = bool((d_A.to_host((N, 8)) == aff).all())
+    out.update({"batch_to_affine_bn254_log": logn, "batch_to_affine_ms": round(tb * 1e3, 3),
+                "batch_to_affine_mpts": round(N / tb / 1e6, 1), "batch_to_affine_ok": ok})
+    # binary MSM and 64 binary rows
+    d_aff = zkgpu.DeviceArray.from_host(aff)
+    prep = zkgpu.PreparedPoints(0, d_aff, N)
+    bits = (rng.random(N) < 0.5).astype(np.uint8)
+    d_bits = zkgpu.DeviceArray.from_bytes(bits)
+    tbin = time_steps(lambda: zkgpu.msm_binary_device(0, N, d_bits, prep), 5, 1)
+    rows = (rng.random((64, 1 << 16)) < 0.5).astype(np.uint8)
+    d_rows = zkgpu.DeviceArray.from_bytes(rows)
+    prep16 = zkgpu.PreparedPoints(0, d_aff, 1 << 16)
+    tmul = time_steps(lambda: zkgpu.multi_msm_binary_device(0, 64, 1 << 16, d_rows, prep16), 3, 1)
+    out.update({"binary_msm_ms": round(tbin * 1e3, 3), "binary_msm_mpts": round(N / tbin / 1e6, 1),
+                "multi_binary_64x2p16_ms": round(tmul * 1e3, 3)})
+    # KZG opening against the SRS tau^i G (prepared), coefficients resident
+    tau = int(uniform_scalars(rng, 1, BN_R)[0][0])
+    d_srs = zkgpu.srs_generate(0, N, tau)
+    srs = zkgpu.PreparedPoints(0, d_srs, N)
+    d_c = zkgpu.fill_random(0, 1, N, seed + 2)
+    z = uniform_scalars(rng, 1, BN_R)[0]
+    topen = time_steps(lambda: zkgpu.kzg_open(0, N, d_c, z, srs), 5, 1)
+    out.update({"kzg_open_bn254_log": logn, "kzg_open_ms": round(topen * 1e3, 3)})
+    M = 1 << 22
+    d_big = zkgpu.fill_random(0, 1, M, seed + 3)
+    d_q = zkgpu.DeviceArray(4 * M)
+    tq = time_steps(lambda: zkgpu.div_by_vanishing_device(0, M, d_big, 1, z, M - 1, d_q), 5, 1)
+    te = time_steps(lambda: zkgpu.poly_eval_device(0, M, d_big, z), 5, 1)
+    out.update({"quot_by_x_minus_z_2p22_ms": round(tq * 1e3, 3),
+                "quot_by_x_minus_z_2p22_GBs": round(2 * 32 * M / tq / 1e9, 1),
+                "poly_eval_2p22_ms": round(te * 1e3, 3), "poly_eval_2p22_GBs": round(32 * M / te / 1e9, 1)})
+    # SRS as a flat file
+    path = Path(os.environ.get("TMPDIR", "/tmp")) / f"zkg_srs_{os.getpid()}.bin"
+    t0 = time.perf_counter()
+    zkgpu.flat_write(path, d_srs, 8, N)
+    tw = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    back, _ = zkgpu.srs_load(0, path, prepare=False)
+    tr = time.perf_counter() - t0
+    same = bool((back.to_host((N, 8)) == d_srs.to_host((N, 8))).all())
+    path.unlink()
+    out.update({"srs_file_write_GBs": round(64 * N / tw / 1e9, 2), "srs_file_read_GBs": round(64 * N / tr / 1e9, 2),
+                "srs_file_roundtrip_ok": same})
+    lib = _ref_lib()
+    if lib is not None:
+        n = 1 << cpu_log
+        cpu = {"log": cpu_log, "cores": 1}
+        src = np.ascontiguousarray(P[:n])
+        tgt = np.zeros((n, 8), np.uint64)
+        t0 = time.perf_counter()
+        lib.bn128_G1_proj_batch_to_affine(n, src.ctypes.data_as(u64p), tgt.ctypes.data_as(u64p))
+        cpu["batch_to_affine_mpts"] = round(n / (time.perf_counter() - t0) / 1e6, 4)
+        cpu["batch_to_affine_matches"] = bool((tgt == aff[:n]).all())
+        b = np.ascontiguousarray(bits[:n])
+        res = np.zeros(12, np.uint64)
+        t0 = time.perf_counter()
+        lib.bn128_G1_proj_MSM_binary_coeff_proj_out(n, b.ctypes.data_as(ctypes.c_void_p),
+                                                    np.ascontiguousarray(aff[:n]).ctypes.data_as(u64p),
+                                                    res.ctypes.data_as(u64p))
+        cpu["binary_msm_mpts"] = round(n / (time.perf_counter() - t0) / 1e6, 4)
+        # KZG opening on the CPU: eval_at, quot_by_vanishing of p - p(z) by x - z, MSM
+        c = d_c.to_host((N, 4))[:n].copy()
+        sr = d_srs.to_host((N, 8))[:n].copy()
+        v = np.zeros(4, np.uint64)
+        q = np.zeros((n, 4), np.uint64)
+        proof = np.zeros(8, np.uint64)
+        zc = np.ascontiguousarray(z)
+        t0 = time.perf_counter()
+        lib.bn128_poly_mont_eval_at(n, c.ctypes.data_as(u64p), zc.ctypes.data_as(u64p), v.ctypes.data_as(u64p))
+        lib.bn128_poly_mont_quot_by_vanishing(n, c.ctypes.data_as(u64p), 1, zc.ctypes.data_as(u64p), n,
+                                              q.ctypes.data_as(u64p))
+        lib.bn128_G1_proj_MSM_mont_coeff_affine_out(n - 1, q.ctypes.data_as(u64p), sr.ctypes.data_as(u64p),
+                                                    proof.ctypes.data_as(u64p), 4)
+        cpu["kzg_open_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+        d_c14 = zkgpu.DeviceArray.from_host(c)
+        gv, gp = zkgpu.kzg_open(0, n, d_c14, z, d_srs)
+        cpu["kzg_open_matches"] = bool((gv == v).all() and (gp == proof).all())
+        out["next_rows_cpu_ref"] = cpu
+    for x in (prep, prep16, srs):
+        x.free()
+    return out
+
+
 def bench_serve(njobs=64, qps_list=(100, 400), seed=5):
     """KZG commits on BLS12-381: iNTT of n evaluations (n = 2^14..2^21, log-uniform) then an MSM
     against a resident SRS of 2^21 points tau^i G; Poisson arrivals at each stated QPS; served
@@ -391,9 +501,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--cpu-sample-log", type=int, default=18)
-    ap.add_argument("--leg", choices=["all", "msm", "ntt", "copy", "nttbatch", "learned", "g1fft"], default="all",
+    ap.add_argument("--leg", choices=["all", "msm", "ntt", "copy", "nttbatch", "learned", "g1fft", "next"],
+                    default="all",
                     help="profiling runs: only the BN254 MSM, the 2^22 NTT pair, the HBM copy kernel, the "
-                         "batched NTTs or the learned MSM mix")
+                         "batched NTTs, the learned MSM mix, the G1 FFT or the rows around the MSM "
+                         "(conversions, binary MSMs, KZG opening, SRS files)")
     a = ap.parse_args()
     if a.leg == "msm":
         a.no_extra = a.no_cpu = True
@@ -414,9 +526,9 @@ def main():
         bw = zkgpu.microbench(zkgpu.MB_COPY)
         log(f"copy kernel: 1 GiB read + 1 GiB written per launch, {bw / 1e12:.2f} TB/s")
         return
-    if a.leg in ("ntt", "nttbatch", "learned", "g1fft"):
+    if a.leg in ("ntt", "nttbatch", "learned", "g1fft", "next"):
         fn = {"ntt": lambda: bench_ntt_pair(a.steps, a.warmup), "nttbatch": bench_ntt_batch,
-              "learned": bench_learned_msm, "g1fft": bench_g1_fft}[a.leg]
+              "learned": bench_learned_msm, "g1fft": bench_g1_fft, "next": bench_next}[a.leg]
         for k, v in fn().items():
             log(f"{k}: {v}")
         return
@@ -468,6 +580,8 @@ def main():
         log(f"ntt batch: {extra['ntt_batch_ms']} ms batched vs {extra['ntt_plain_ms']} ms plain, identical {extra['ntt_batch_bit_identical_to_plain']}")
         extra.update(bench_g1_fft())
         log(f"g1 fft: {[(k, v) for k, v in extra.items() if k.startswith('g1_fft')]}")
+        extra.update(bench_next())
+        log(f"next rows: {[(k, v) for k, v in extra.items() if k.split('_')[0] in ('batch', 'binary', 'multi', 'kzg', 'quot', 'poly', 'srs', 'next')]}")
         extra.update(bench_learned_msm())
         log(f"learned msm mix: {[(k, v) for k, v in extra.items() if k.startswith('learned_msm')]}")
         extra.update(bench_serve())

@@ -270,6 +270,15 @@ int zkg_poly_eval(int curve, int n, const uint64_t *d_coeffs, const uint64_t *lo
 int zkg_poly_degree(int n, const uint64_t *d_coeffs, int *deg);
 int zkg_poly_div_by_vanishing(int curve, int n1, const uint64_t *d_src, int expo_n, const uint64_t *eta,
                               int nquot, uint64_t *d_quot, int nrem, uint64_t *d_rem);
+/* Flat arrays on disk in the reference's format (lib/src/ZK/Algebra/Class/Flat.hs:211-233:
+ * raw little-endian limbs, no header, length = file size / element size, a size that is not a
+ * multiple of the element being an error), streamed through pinned staging buffers. */
+int zkg_flat_file_count(const char *path, size_t elem_bytes, size_t *count);
+int zkg_flat_read(const char *path, size_t elem_bytes, void *d_dst, size_t capacity, size_t *count);
+int zkg_flat_write(const char *path, const void *d_src, size_t elem_bytes, size_t count);
+/* an SRS file of affine Montgomery G1 points into a new device buffer (released with
+ * zkg_free), converted to the prepared MSM form when prepare != 0 */
+int zkg_srs_load(int curve, const char *path, int prepare, void **d_out, int *n_out);
 /* KZG opening (examples/KZG.hs:108-114) of the polynomial with n Montgomery coefficients at z:
  * value_host = p(z), proof_host = MSM of (p - p(z)) / (x - z) over the first n - 1 SRS points
  * (d_srs: affine points, or the prepared form with srs_flags = ZKG_MSM_POINTS_INTERNAL);
@@ -277,11 +286,15 @@ int zkg_poly_div_by_vanishing(int curve, int n1, const uint64_t *d_src, int expo
 int zkg_kzg_open(int curve, int n, const uint64_t *d_coeffs, const uint64_t *z, const uint64_t *d_srs,
                  int srs_flags, uint64_t *value_host, uint64_t *proof_host, int out_flags);
 /* projective (jac = 0) or Jacobian (jac = 1) -> affine for n device points (Montgomery batch
- * inversion, 16 points per thread) */
+ * inversion, 16 points per thread); the two buffers must not overlap */
 int zkg_g1_batch_to_affine(int curve, int jac, int n, const uint64_t *d_src, uint64_t *d_tgt);
 /* binary-coefficient MSM on device buffers (d_bits: npoints bytes); flags as zkg_msm */
 int zkg_msm_binary(int curve, int npoints, const uint8_t *d_bits, const uint64_t *d_grps, uint64_t *tgt_host,
                    int flags);
+/* nrows binary MSMs over one device point set; d_bits: nrows x npoints bytes, row-major;
+ * tgt_host: nrows results of the form flags selects */
+int zkg_multi_msm_binary(int curve, int nrows, int npoints, const uint8_t *d_bits, const uint64_t *d_grps,
+                         uint64_t *tgt_host, int flags);
 /* G1 FFT on device buffers (2^m projective points in, normalised projective points out) */
 #define ZKG_G1_FFT_INVERSE 1
 #define ZKG_G1_FFT_JAC 2    /* Jacobian inputs (outputs are the same normalised words) */

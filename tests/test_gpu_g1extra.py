@@ -217,3 +217,24 @@ def test_jac_g1_fft_vs_oracle(zk, C, m):
         for inv in (False, True):
             exp = o.g1_fft(m, g, P, inverse=inv, jac=True)
             assert (zk.g1_fft(C.id, m, g, P, inverse=inv, coord="jac") == exp).all(), (m, inv)
+
+
+@pytest.mark.parametrize("C", CURVES, ids=IDS)
+def test_multi_binary_msm_device(zk, C):
+    """The device-buffer form over prepared points equals the reference entry point and the
+    oracle, row by row."""
+    o = Oracle(C)
+    n, nrows = 3000, 9
+    ks = random_scalars(8800 + C.id, n, C.r)
+    P = o.points_from_scalars(ks)
+    rng = np.random.default_rng(8)
+    rows = (rng.random((nrows, n)) < 0.3).astype(np.uint8)
+    rows[2] = 0
+    d_p = zk.DeviceArray.from_host(P)
+    prep = zk.PreparedPoints(C.id, d_p, n)
+    d_bits = zk.DeviceArray.from_bytes(rows)
+    got = zk.multi_msm_binary_device(C.id, nrows, n, d_bits, prep, flags=zk.MSM_PROJ_OUT)
+    assert (got == o.multi_msm_binary(rows, P)).all()
+    aff = zk.multi_msm_binary_device(C.id, nrows, n, d_bits, d_p)
+    assert (aff == zk.multi_msm_binary(C.id, rows, P)).all()
+    prep.free()

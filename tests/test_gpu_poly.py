@@ -154,3 +154,28 @@ def test_kzg_open_closed_form(zk, C, logn):
     assert C.fr_from_mont(array_to_ints(value.reshape(1, 4))[0]) == pz
     e = (ptau - pz) * pow((tau - zi) % C.r, C.r - 2, C.r) % C.r
     assert (proof == Oracle(C).points_from_scalars([e])[0]).all()
+
+
+@pytest.mark.parametrize("C", CURVES, ids=IDS)
+def test_poly_degree_device(zk, C):
+    """Degree on the device: top coefficient non-zero (the quick path), trailing zeros (the
+    reduction over all coefficients), and the zero polynomial (-1)."""
+    n = 300001
+    a = coeffs(zk, C, n, 61)
+    assert zk.poly_degree_device(n, zk.DeviceArray.from_host(a)) == n - 1
+    a[-5:] = 0
+    assert zk.poly_degree_device(n, zk.DeviceArray.from_host(a)) == n - 6
+    a[:] = 0
+    a[12345] = 1
+    assert zk.poly_degree_device(n, zk.DeviceArray.from_host(a)) == 12345
+    a[:] = 0
+    assert zk.poly_degree_device(n, zk.DeviceArray.from_host(a)) == -1
+    assert zk.poly_degree_device(0, zk.DeviceArray.from_host(a)) == -1
+    # the device division follows the degree the reduction finds
+    b = coeffs(zk, C, 1000, 62)
+    b[-3:] = 0
+    eta = fe(C, 5)
+    st, q, r = Oracle(C).div_by_vanishing(b, 7, eta, 1000, 9)
+    d_q, d_r = zk.DeviceArray(1000 * 4), zk.DeviceArray(9 * 4)
+    zk.div_by_vanishing_device(C.id, 1000, zk.DeviceArray.from_host(b), 7, eta, 1000, d_q, 9, d_r)
+    assert st == 0 and (d_q.to_host((1000, 4)) == q).all() and (d_r.to_host((9, 4)) == r).all()

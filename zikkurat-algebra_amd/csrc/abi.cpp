@@ -47,50 +47,38 @@ int ntt_host(int curve, int m, const uint64_t *gen, const uint64_t *eta, int fla
 
 const zkh::FieldDesc &fr(int curve) { return curve == 0 ? zkh::BN254_FR : zkh::BLS381_FR; }
 
-// host-in / host-out binary MSMs (bn128_G1_proj.c:684-782): the points go to the device once
-// (converted to the internal form), every row's coefficient bytes become 0/1 scalars, and the
-// rows run as one batch of MSMs on concurrent streams
-int binary_host(int curve, int nrows, int npoints, const uint8_t *const *rows, const uint64_t *grps, uint64_t *tgt,
-                int flags) {
+// binary MSMs of nrows coefficient rows (row r at d_bits + r * npoints) over one device point
+// set, written to tgt in the form flags selects
+int multi_binary_device(Context *ctx, int curve, int nrows, int npoints, const uint8_t *d_bits,
+                        const uint64_t *d_points, bool points_internal, uint64_t *tgt, int flags) {
   const int lp = curve == 0 ? 4 : 6;
   const int outw = (flags & (ZKG_MSM_PROJ_OUT | ZKG_MSM_JAC_OUT)) ? 3 * lp : 2 * lp;
   if (nrows <= 0) return ZK_OK;
-  if (npoints <= 0) {
-    for (int r = 0; r < nrows; r++) {
-      MsmResult res;
-      msm_infinity(curve, &res);
-      ZK_TRY(msm_write_result(curve, res, tgt + (size_t)r * outw, flags));
-    }
-    return ZK_OK;
-  }
+  std::vector<MsmResult> res(nrows);
+  ZK_TRY(binary_sum_device(curve, nrows, npoints, d_bits, d_points, points_internal, ctx->stream, res.data()));
+  for (int r = 0; r < nrows; r++) ZK_TRY(msm_write_result(curve, res[r], tgt + (size_t)r * outw, flags));
+  return ZK_OK;
+}
+
+// host-in / host-out binary MSMs (bn128_G1_proj.c:684-782): the points go to the device once,
+// converted to the internal form, and the coefficient rows are packed into one byte matrix
+int binary_host(int curve, int nrows, int npoints, const uint8_t *const *rows, const uint64_t *grps, uint64_t *tgt,
+                int flags) {
+  const int lp = curve == 0 ? 4 : 6;
   Context *ctx;
   ZK_TRY(with_ctx(&ctx));
   std::lock_guard<std::recursive_mutex> lk(ctx->lock);
-  DevBuf dp, dint, bits, sc;
+  if (nrows <= 0 || npoints <= 0) return multi_binary_device(ctx, curve, nrows, npoints, nullptr, nullptr, true, tgt, flags);
+  DevBuf dp, dint, bits;
   const size_t pb = (size_t)npoints * 2 * lp * 8;
   ZK_TRY(dp.alloc(pb));
   ZK_TRY(copy_h2d(dp.p, grps, pb, ctx->stream));
   ZK_TRY(dint.alloc((size_t)npoints * msm_internal_point_bytes(curve)));
   ZK_TRY(msm_prepare_points(curve, npoints, dp.as<uint64_t>(), dint.p, ctx->stream));
-  ZK_TRY(bits.alloc((size_t)npoints));
-  ZK_TRY(sc.alloc((size_t)nrows * npoints * 8));
-  for (int r = 0; r < nrows; r++) {
-    ZK_TRY(copy_h2d(bits.p, rows[r], (size_t)npoints, ctx->stream));
-    ZK_TRY(binary_scalars(npoints, bits.as<uint8_t>(), sc.as<uint64_t>() + (size_t)r * npoints, ctx->stream));
-  }
-  ZK_HIP(hipStreamSynchronize(ctx->stream));
-  std::vector<int> curves(nrows, curve), ns(nrows, npoints);
-  std::vector<const uint64_t *> scp(nrows), pts(nrows, (const uint64_t *)dint.p);
-  std::vector<uint64_t *> outs(nrows);
-  for (int r = 0; r < nrows; r++) {
-    scp[r] = sc.as<uint64_t>() + (size_t)r * npoints;
-    outs[r] = tgt + (size_t)r * outw;
-  }
-  zkg_msm_schedule s;
-  memset(&s, 0, sizeof s);
-  s.policy = ZKG_SCHED_STATS;
-  return msm_batch(nrows, curves.data(), ns.data(), scp.data(), 1, 0, pts.data(), true, outs.data(),
-                   flags & (ZKG_MSM_PROJ_OUT | ZKG_MSM_JAC_OUT), &s, nrows > 1 ? 4 : 1, nullptr);
+  ZK_TRY(bits.alloc((size_t)nrows * npoints));
+  for (int r = 0; r < nrows; r++)
+    ZK_TRY(copy_h2d(bits.as<uint8_t>() + (size_t)r * npoints, rows[r], (size_t)npoints, ctx->stream));
+  return multi_binary_device(ctx, curve, nrows, npoints, bits.as<uint8_t>(), dint.as<uint64_t>(), true, tgt, flags);
 }
 
 // host-in / host-out batch conversions (bn128_G1_proj.c:146-166, bn128_G1_jac.c:139-158)
@@ -421,6 +409,109 @@ int zkg_g1_batch_to_affine(int curve, int jac, int n, const uint64_t *d_src, uin
   return g1_batch_to_affine(curve, jac != 0, n, d_src, d_tgt, ctx->stream);
 }
 
+// ---- flat arrays on disk (lib/src/ZK/Algebra/Class/Flat.hs:211-233): raw little-endian limbs,
+// no header, the file size giving the length ----
+
+int zkg_flat_file_count(const char *path, size_t elem_bytes, size_t *count) {
+  if (!path || !elem_bytes || !count) {
+    set_error("flat array: bad argument");
+    return ZK_EINVAL;
+  }
+  FILE *f = fopen(path, "rb");
+  if (!f) {
+    set_error("flat array: cannot open the file");
+    return ZK_EINVAL;
+  }
+  int st = ZK_OK;
+  if (fseeko(f, 0, SEEK_END) != 0) st = ZK_EINVAL;
+  const off_t size = st == ZK_OK ? ftello(f) : -1;
+  fclose(f);
+  if (size < 0) {
+    set_error("flat array: cannot size the file");
+    return ZK_EINVAL;
+  }
+  if ((size_t)size % elem_bytes) {
+    set_error("readFlatArray: the input file has invalid size");
+    return ZK_EINVAL;
+  }
+  *count = (size_t)size / elem_bytes;
+  return ZK_OK;
+}
+
+int zkg_flat_read(const char *path, size_t elem_bytes, void *d_dst, size_t capacity, size_t *count) {
+  size_t n;
+  ZK_TRY(zkg_flat_file_count(path, elem_bytes, &n));
+  if (n > capacity) {
+    set_error("flat array read: the device buffer is too small for the file");
+    return ZK_EINVAL;
+  }
+  Context *ctx;
+  ZK_TRY(with_ctx(&ctx));
+  std::lock_guard<std::recursive_mutex> lk(ctx->lock);
+  FILE *f = fopen(path, "rb");
+  if (!f) {
+    set_error("flat array: cannot open the file");
+    return ZK_EINVAL;
+  }
+  const int st = file_to_device(f, d_dst, n * elem_bytes, ctx->stream);
+  fclose(f);
+  if (st == ZK_OK && count) *count = n;
+  return st;
+}
+
+int zkg_flat_write(const char *path, const void *d_src, size_t elem_bytes, size_t count) {
+  if (!path || !elem_bytes) {
+    set_error("flat array: bad argument");
+    return ZK_EINVAL;
+  }
+  Context *ctx;
+  ZK_TRY(with_ctx(&ctx));
+  std::lock_guard<std::recursive_mutex> lk(ctx->lock);
+  FILE *f = fopen(path, "wb");
+  if (!f) {
+    set_error("flat array: cannot create the file");
+    return ZK_EINVAL;
+  }
+  int st = device_to_file(f, d_src, count * elem_bytes, ctx->stream);
+  if (fclose(f) != 0 && st == ZK_OK) {
+    set_error("flat array write: close failed");
+    st = ZK_EINVAL;
+  }
+  return st;
+}
+
+// an SRS file of affine G1 points (2 L_p limbs each) into device memory, optionally converted to
+// the prepared MSM form; *d_out is released with zkg_free
+int zkg_srs_load(int curve, const char *path, int prepare, void **d_out, int *n_out) {
+  if ((curve != 0 && curve != 1) || !d_out || !n_out) {
+    set_error("srs_load: bad argument");
+    return ZK_EINVAL;
+  }
+  const size_t elem = (size_t)(curve == 0 ? 4 : 6) * 2 * 8;
+  size_t n;
+  ZK_TRY(zkg_flat_file_count(path, elem, &n));
+  if (n > 0x7fffffff) {
+    set_error("srs_load: too many points");
+    return ZK_EINVAL;
+  }
+  void *aff = nullptr;
+  ZK_TRY(dev_alloc(&aff, n ? n * elem : 256));
+  int st = zkg_flat_read(path, elem, aff, n, nullptr);
+  if (st == ZK_OK && prepare) {
+    void *prep = nullptr;
+    st = zkg_msm_prepare_points(curve, (int)n, (const uint64_t *)aff, &prep);
+    dev_free(aff);
+    aff = prep;
+  }
+  if (st != ZK_OK) {
+    if (aff) dev_free(aff);
+    return st;
+  }
+  *d_out = aff;
+  *n_out = (int)n;
+  return ZK_OK;
+}
+
 int zkg_poly_eval(int curve, int n, const uint64_t *d_coeffs, const uint64_t *loc, uint64_t *out_host) {
   Context *ctx;
   ZK_TRY(with_ctx(&ctx));
@@ -488,14 +579,26 @@ int zkg_kzg_open(int curve, int n, const uint64_t *d_coeffs, const uint64_t *z, 
   return msm_write_result(curve, r, proof_host, out_flags);
 }
 
+int zkg_multi_msm_binary(int curve, int nrows, int npoints, const uint8_t *d_bits, const uint64_t *d_grps,
+                         uint64_t *tgt_host, int flags) {
+  if (curve != 0 && curve != 1) {
+    set_error("multi_msm_binary: bad curve id");
+    return ZK_EINVAL;
+  }
+  Context *ctx;
+  ZK_TRY(with_ctx(&ctx));
+  std::lock_guard<std::recursive_mutex> lk(ctx->lock);
+  return multi_binary_device(ctx, curve, nrows, npoints, d_bits, d_grps, (flags & ZKG_MSM_POINTS_INTERNAL) != 0,
+                             tgt_host, flags);
+}
+
 int zkg_msm_binary(int curve, int npoints, const uint8_t *d_bits, const uint64_t *d_grps, uint64_t *tgt_host,
                    int flags) {
   Context *ctx;
   ZK_TRY(with_ctx(&ctx));
   std::lock_guard<std::recursive_mutex> lk(ctx->lock);
-  MsmResult r;
-  ZK_TRY(msm_binary_device(curve, npoints, d_bits, d_grps, (flags & ZKG_MSM_POINTS_INTERNAL) != 0, ctx->stream, &r));
-  return msm_write_result(curve, r, tgt_host, flags);
+  return multi_binary_device(ctx, curve, 1, npoints, d_bits, d_grps, (flags & ZKG_MSM_POINTS_INTERNAL) != 0,
+                             tgt_host, flags);
 }
 
 int zkg_g1_fft(int curve, int m, const uint64_t *gen, int mode, const uint64_t *d_src, uint64_t *d_tgt) {

@@ -61,9 +61,10 @@ int g1_scalar_mul_gen(int curve, int n, const uint64_t *d_scalars, uint64_t *d_o
 // batch conversions (external Montgomery words) and binary-coefficient MSMs
 int g1_batch_to_affine(int curve, bool jac, int n, const uint64_t *d_src, uint64_t *d_dst, hipStream_t st);
 int g1_batch_from_affine(int curve, bool jac, int n, const uint64_t *d_src, uint64_t *d_dst, hipStream_t st);
-int binary_scalars(int n, const uint8_t *d_bits, uint64_t *d_scalars, hipStream_t st);
-int msm_binary_device(int curve, int n, const uint8_t *d_bits, const uint64_t *d_points, bool points_internal,
-                      hipStream_t st, MsmResult *out);
+// sums of the points selected by the non-zero bytes of each of nrows rows (row r at
+// d_bits + r * n); out holds nrows results
+int binary_sum_device(int curve, int nrows, int n, const uint8_t *d_bits, const uint64_t *d_points,
+                      bool points_internal, hipStream_t st, MsmResult *out);
 // polynomial evaluation (result to host memory), degree, and division by x^n - eta
 int poly_eval_device(int curve, int n, const uint64_t *d_coeffs, const uint64_t *loc, uint64_t *out_host,
                      hipStream_t st);

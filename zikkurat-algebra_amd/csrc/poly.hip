@@ -133,9 +133,10 @@ __global__ __launch_bounds__(kPolyThreads) void vanish_rem_kernel(const uint32_t
   fstore(rem + (size_t)j * FR::N, x);
 }
 
-// deg = max { i : a_i != 0 } (-1 for the zero polynomial): one atomic per wavefront
+// deg = max { i : a_i != 0 } (-1 for the zero polynomial): one atomic per block
 __global__ __launch_bounds__(kPolyThreads) void poly_degree_kernel(const uint4 *__restrict__ a, uint32_t n,
                                                                   int *__restrict__ deg) {
+  __shared__ int wave_best[kPolyThreads / 64];
   int best = -1;
   for (uint32_t i = blockIdx.x * kPolyThreads + threadIdx.x; i < n; i += gridDim.x * kPolyThreads) {
     const uint4 u = a[2 * (size_t)i], w = a[2 * (size_t)i + 1];
@@ -146,7 +147,12 @@ __global__ __launch_bounds__(kPolyThreads) void poly_degree_kernel(const uint4 *
     const int o = __shfl_xor(best, off);
     best = o > best ? o : best;
   }
-  if ((threadIdx.x & 63) == 0 && best >= 0) atomicMax(deg, best);
+  if ((threadIdx.x & 63) == 0) wave_best[threadIdx.x >> 6] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < kPolyThreads / 64; k++) best = wave_best[k] > best ? wave_best[k] : best;
+    if (best >= 0) atomicMax(deg, best);
+  }
 }
 
 __global__ void poly_degree_init_kernel(int *deg) {
@@ -219,10 +225,17 @@ int poly_degree_device(int n, const uint64_t *d_coeffs, int *deg_host, hipStream
     *deg_host = -1;
     return ZK_OK;
   }
+  uint64_t top[4];  // the usual case: the top coefficient is non-zero
+  ZK_HIP(hipMemcpyAsync(top, d_coeffs + (size_t)(n - 1) * 4, 32, hipMemcpyDeviceToHost, st));
+  ZK_HIP(hipStreamSynchronize(st));
+  if (top[0] | top[1] | top[2] | top[3]) {
+    *deg_host = n - 1;
+    return ZK_OK;
+  }
   DevBuf d;
   ZK_TRY(d.alloc(4));
   hipLaunchKernelGGL(poly_degree_init_kernel, dim3(1), dim3(64), 0, st, d.as<int>());
-  const unsigned g = grid_for((uint64_t)n) < 4096 ? grid_for((uint64_t)n) : 4096;
+  const unsigned g = grid_for((uint64_t)n) < 1024 ? grid_for((uint64_t)n) : 1024;
   hipLaunchKernelGGL(poly_degree_kernel, dim3(g), dim3(kPolyThreads), 0, st, (const uint4 *)d_coeffs, (uint32_t)n,
                      d.as<int>());
   ZK_HIP(hipGetLastError());

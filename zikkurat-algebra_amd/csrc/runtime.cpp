@@ -138,7 +138,10 @@ static int ensure_stage() {
   return ZK_OK;
 }
 
-int copy_h2d(void *dst, const void *src, size_t bytes, hipStream_t s) {
+// fill(stage, off, n) writes bytes [off, off + n) of the source into the pinned stage; the two
+// stages alternate so that filling one overlaps the copy out of the other
+template <class Fill>
+static int stage_h2d(void *dst, size_t bytes, hipStream_t s, Fill fill) {
   if (!bytes) return ZK_OK;
   std::lock_guard<std::mutex> g(g_stage_lock);
   ZK_TRY(ensure_stage());
@@ -148,7 +151,7 @@ int copy_h2d(void *dst, const void *src, size_t bytes, hipStream_t s) {
   while (off < bytes) {
     size_t n = bytes - off < kStageBytes ? bytes - off : kStageBytes;
     if (used[k]) ZK_HIP(hipEventSynchronize(g_stage_ev[k]));
-    memcpy(g_stage[k], (const char *)src + off, n);
+    ZK_TRY(fill(g_stage[k], off, n));
     ZK_HIP(hipMemcpyAsync((char *)dst + off, g_stage[k], n, hipMemcpyHostToDevice, s));
     ZK_HIP(hipEventRecord(g_stage_ev[k], s));
     used[k] = true;
@@ -159,19 +162,63 @@ int copy_h2d(void *dst, const void *src, size_t bytes, hipStream_t s) {
   return ZK_OK;
 }
 
-int copy_d2h(void *dst, const void *src, size_t bytes, hipStream_t s) {
+// drain(stage, off, n) consumes bytes [off, off + n) of the device source; the copy of the next
+// chunk into the other stage runs while the current one is drained
+template <class Drain>
+static int stage_d2h(const void *src, size_t bytes, hipStream_t s, Drain drain) {
   if (!bytes) return ZK_OK;
   std::lock_guard<std::mutex> g(g_stage_lock);
   ZK_TRY(ensure_stage());
-  size_t off = 0;
-  while (off < bytes) {
-    size_t n = bytes - off < kStageBytes ? bytes - off : kStageBytes;
-    ZK_HIP(hipMemcpyAsync(g_stage[0], (const char *)src + off, n, hipMemcpyDeviceToHost, s));
-    ZK_HIP(hipStreamSynchronize(s));
-    memcpy((char *)dst + off, g_stage[0], n);
-    off += n;
+  auto issue = [&](size_t off, int k) -> int {
+    const size_t n = bytes - off < kStageBytes ? bytes - off : kStageBytes;
+    ZK_HIP(hipMemcpyAsync(g_stage[k], (const char *)src + off, n, hipMemcpyDeviceToHost, s));
+    ZK_HIP(hipEventRecord(g_stage_ev[k], s));
+    return ZK_OK;
+  };
+  ZK_TRY(issue(0, 0));
+  int k = 0;
+  for (size_t off = 0; off < bytes; off += kStageBytes, k ^= 1) {
+    const size_t n = bytes - off < kStageBytes ? bytes - off : kStageBytes;
+    if (off + n < bytes) ZK_TRY(issue(off + n, k ^ 1));
+    ZK_HIP(hipEventSynchronize(g_stage_ev[k]));
+    ZK_TRY(drain(g_stage[k], off, n));
   }
+  ZK_HIP(hipStreamSynchronize(s));
   return ZK_OK;
+}
+
+int copy_h2d(void *dst, const void *src, size_t bytes, hipStream_t s) {
+  return stage_h2d(dst, bytes, s, [&](void *stage, size_t off, size_t n) {
+    memcpy(stage, (const char *)src + off, n);
+    return ZK_OK;
+  });
+}
+
+int copy_d2h(void *dst, const void *src, size_t bytes, hipStream_t s) {
+  return stage_d2h(src, bytes, s, [&](const void *stage, size_t off, size_t n) {
+    memcpy((char *)dst + off, stage, n);
+    return ZK_OK;
+  });
+}
+
+int file_to_device(FILE *f, void *dst, size_t bytes, hipStream_t s) {
+  return stage_h2d(dst, bytes, s, [&](void *stage, size_t, size_t n) {
+    if (fread(stage, 1, n, f) != n) {
+      set_error("flat array read: short read");
+      return ZK_EINVAL;
+    }
+    return ZK_OK;
+  });
+}
+
+int device_to_file(FILE *f, const void *src, size_t bytes, hipStream_t s) {
+  return stage_d2h(src, bytes, s, [&](const void *stage, size_t, size_t n) {
+    if (fwrite(stage, 1, n, f) != n) {
+      set_error("flat array write: short write");
+      return ZK_EINVAL;
+    }
+    return ZK_OK;
+  });
 }
 
 }  // namespace zk

@@ -1,6 +1,7 @@
 // Runtime of the library: device context, stream, caching device allocator, pinned staging,
 // error reporting.  One process drives one GPU (the current HIP device).
 #pragma once
+#include <stdio.h>
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 #include <stdint.h>
@@ -83,6 +84,9 @@ struct DevBuf {
 // staging ring, on the library stream.  Synchronous with respect to the host on return.
 int copy_h2d(void *dst, const void *src, size_t bytes, hipStream_t s);
 int copy_d2h(void *dst, const void *src, size_t bytes, hipStream_t s);
+// the same staging between an open file and device memory (raw bytes, sequential)
+int file_to_device(FILE *f, void *dst, size_t bytes, hipStream_t s);
+int device_to_file(FILE *f, const void *src, size_t bytes, hipStream_t s);
 
 // Event-timed region helper for kernels launched on the library stream.
 struct KernelTimer {

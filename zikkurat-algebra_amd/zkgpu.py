@@ -123,12 +123,19 @@ def lib():
         L.zkg_msm_set_cost_model.argtypes = [vp]
         L.zkg_ntt_set_tuning.argtypes = [ctypes.c_int, ctypes.c_int]
         L.zkg_g1_fft.argtypes = [ctypes.c_int, ctypes.c_int, u64p, ctypes.c_int, vp, vp]
+        L.zkg_flat_file_count.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+        L.zkg_flat_read.argtypes = [ctypes.c_char_p, ctypes.c_size_t, vp, ctypes.c_size_t,
+                                    ctypes.POINTER(ctypes.c_size_t)]
+        L.zkg_flat_write.argtypes = [ctypes.c_char_p, vp, ctypes.c_size_t, ctypes.c_size_t]
+        L.zkg_srs_load.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(vp),
+                                   ctypes.POINTER(ctypes.c_int)]
         L.zkg_poly_eval.argtypes = [ctypes.c_int, ctypes.c_int, vp, u64p, u64p]
         L.zkg_poly_degree.argtypes = [ctypes.c_int, vp, ctypes.POINTER(ctypes.c_int)]
         L.zkg_poly_div_by_vanishing.argtypes = [ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, u64p, ctypes.c_int, vp,
                                                 ctypes.c_int, vp]
         L.zkg_kzg_open.argtypes = [ctypes.c_int, ctypes.c_int, vp, u64p, vp, ctypes.c_int, u64p, u64p, ctypes.c_int]
         L.zkg_g1_batch_to_affine.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp]
+        L.zkg_multi_msm_binary.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, u64p, ctypes.c_int]
         L.zkg_msm_binary.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp, u64p, ctypes.c_int]
         L.zkg_ntt_set_tuning_for.argtypes = [ctypes.c_int] * 4
         L.zkg_ntt_tune.argtypes = [ctypes.c_int] * 5 + [ip, ip, dp, ip]
@@ -446,9 +453,22 @@ def multi_msm_binary(curve, rows, points, coord="proj", out="affine"):
     return res
 
 
+def multi_msm_binary_device(curve, nrows, n, d_bits, d_points, flags=0):
+    """nrows binary MSMs on device buffers (d_bits: nrows x n bytes, row-major)."""
+    if isinstance(d_points, PreparedPoints):
+        flags |= MSM_POINTS_INTERNAL
+    L = FP_LIMBS[curve]
+    w = 3 * L if flags & (MSM_PROJ_OUT | MSM_JAC_OUT) else 2 * L
+    res = np.zeros((max(nrows, 1), w), np.uint64)
+    _check(lib().zkg_multi_msm_binary(curve, nrows, n, d_bits.ptr, d_points.ptr, _p(res), flags))
+    return res[:nrows]
+
+
 def msm_binary_device(curve, n, d_bits, d_points, flags=0):
     """Binary MSM on device buffers: d_bits holds n bytes; flags as msm_device (output form,
     MSM_POINTS_INTERNAL)."""
+    if isinstance(d_points, PreparedPoints):
+        flags |= MSM_POINTS_INTERNAL
     L = FP_LIMBS[curve]
     res = np.zeros(3 * L if flags & (MSM_PROJ_OUT | MSM_JAC_OUT) else 2 * L, np.uint64)
     _check(lib().zkg_msm_binary(curve, n, d_bits.ptr, d_points.ptr, _p(res), flags))
@@ -674,6 +694,39 @@ class NttBatch:
             pass
 
 
+def flat_file_count(path, elem_words):
+    n = ctypes.c_size_t()
+    _check(lib().zkg_flat_file_count(str(path).encode(), 8 * elem_words, ctypes.byref(n)))
+    return n.value
+
+
+def flat_read(path, elem_words):
+    """readFlatArray (Flat.hs:211-226) into device memory: a DeviceArray of count * elem_words."""
+    n = flat_file_count(path, elem_words)
+    d = DeviceArray(n * elem_words)
+    got = ctypes.c_size_t()
+    _check(lib().zkg_flat_read(str(path).encode(), 8 * elem_words, d.ptr, n, ctypes.byref(got)))
+    return d
+
+
+def flat_write(path, d, elem_words, count):
+    """writeFlatArray (Flat.hs:229-233) of count elements from device memory."""
+    _check(lib().zkg_flat_write(str(path).encode(), d.ptr, 8 * elem_words, count))
+
+
+def srs_load(curve, path, prepare=True):
+    """An SRS file of affine points: PreparedPoints (prepare=True) or a DeviceArray, and n."""
+    ptr, n = vp(), ctypes.c_int()
+    _check(lib().zkg_srs_load(curve, str(path).encode(), int(prepare), ctypes.byref(ptr), ctypes.byref(n)))
+    if prepare:
+        pp = PreparedPoints(curve, None, n.value)
+        pp.ptr = ptr
+        return pp, n.value
+    d = DeviceArray.__new__(DeviceArray)
+    d.nwords, d.ptr = n.value * 2 * FP_LIMBS[curve], ptr
+    return d, n.value
+
+
 def srs_generate(curve, n, tau):
     """[tau^i G]_{i<n}, affine Montgomery, on the device (tau: standard-form integer)."""
     t = np.array([(tau >> (64 * k)) & 0xFFFFFFFFFFFFFFFF for k in range(4)], dtype=np.uint64)
@@ -688,7 +741,8 @@ class PreparedPoints:
     def __init__(self, curve, d_points, n):
         self.n = n
         self.ptr = vp()
-        _check(lib().zkg_msm_prepare_points(curve, n, d_points.ptr, ctypes.byref(self.ptr)))
+        if d_points is not None:
+            _check(lib().zkg_msm_prepare_points(curve, n, d_points.ptr, ctypes.byref(self.ptr)))
 
     def free(self):
         if self.ptr:
