@@ -145,7 +145,8 @@ def test_zero_test_norm_and_canon(h, field):
     p, bits, N, _ = FIELDS[field]
     rng = splitmix64(7 + field)
     mults = [j * p for j in range(-7, 8)]
-    near = [j * p + d for j in range(-7, 8) for d in (1, -1, 1 << bits, -(1 << bits))]
+    # the filter hits on the low limb of j p; values sharing it must still be told apart
+    near = [j * p + d for j in range(-7, 8) for d in (1, -1, 1 << bits, -(1 << bits), 1 << (bits * (N - 1)))]
     rand = sample_values(p, rng, 200, 7)
     vals = mults + near + rand
     A = arr([to_limbs(v, bits, N) for v in vals])

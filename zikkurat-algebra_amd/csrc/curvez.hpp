@@ -40,10 +40,13 @@ struct XyzzZ {
 template <class C>
 struct ZLayout {
   static constexpr int N = ZP<C>::T::N;
-  // affine point: one 128-byte line (the MSM reads points at random indices, so a record
-  // that straddled two lines would fetch both)
-  static constexpr int AFF = 32;
-  static_assert(2 * N <= AFF, "affine point must fit one 128-byte line");
+  // affine point records, read at random indices by the MSM, never straddle a line:
+  // PACKED records hold the canonical internal values bit-packed into EXT_WORDS words each
+  // (BN254: 2 x 32 bytes, half a 128-byte line, unpacked by shifts on load); otherwise the
+  // limbs as stored, padded to one 128-byte line (BLS12-381: 2 x 14 limbs)
+  static constexpr bool PACKED = 2 * ZP<C>::T::EXT_WORDS == 16;
+  static constexpr int AFF = PACKED ? 16 : 32;
+  static_assert(PACKED || 2 * N <= AFF, "affine point must fit one 128-byte line");
   static constexpr int XYZZ = 4 * N;               // 36 or 56 words: multiples of 4
 };
 
@@ -241,15 +244,41 @@ ZK_HD void zxyzz_scalar_mul(XyzzZ<C> &r, const XyzzZ<C> &p, const uint32_t (&k)[
 
 template <class C>
 __device__ __forceinline__ void zaff_load(AffZ<C> &a, const int32_t *__restrict__ p) {
-  constexpr int N = ZP<C>::T::N;
-  zload(a.x, p);
-  zload(a.y, p + N);
+  constexpr int N = ZP<C>::T::N, E = ZP<C>::T::EXT_WORDS;
+  if (ZLayout<C>::PACKED) {
+    const uint4 *q = reinterpret_cast<const uint4 *>(p);
+    uint32_t w[2 * E];
+#pragma unroll
+    for (int j = 0; j < E / 2; j++) {
+      const uint4 v = q[j];
+      w[4 * j] = v.x;
+      w[4 * j + 1] = v.y;
+      w[4 * j + 2] = v.z;
+      w[4 * j + 3] = v.w;
+    }
+    zunpack(a.x, w);
+    zunpack(a.y, w + E);
+  } else {
+    zload(a.x, p);
+    zload(a.y, p + N);
+  }
 }
+// a whole record (padding included); infinity is the all-zero record
 template <class C>
 __device__ __forceinline__ void zaff_store(int32_t *__restrict__ p, const AffZ<C> &a) {
-  constexpr int N = ZP<C>::T::N;
-  zstore(p, a.x);
-  zstore(p + N, a.y);
+  constexpr int N = ZP<C>::T::N, E = ZP<C>::T::EXT_WORDS;
+  if (ZLayout<C>::PACKED) {
+    FzP<C> t;
+    zcanon(t, a.x);
+    zpack((uint32_t *)p, t);
+    zcanon(t, a.y);
+    zpack((uint32_t *)p + E, t);
+  } else {
+    zstore(p, a.x);
+    zstore(p + N, a.y);
+#pragma unroll
+    for (int j = 2 * N; j < ZLayout<C>::AFF; j++) p[j] = 0;
+  }
 }
 template <class C>
 __device__ __forceinline__ void zxyzz_load(XyzzZ<C> &a, const int32_t *__restrict__ p) {

@@ -29,9 +29,8 @@
 
 namespace zk {
 
-// parameter packs: limbs of p, -p^-1 mod 2^BITS, conversion constants, R' mod p, and the
-// low limbs (j p) mod 2^BITS for j = -8..8 used by the zero test (values generated from the
-// primes of lib/cbits/curves/fields/mont/*_mont.c)
+// parameter packs: limbs of p, -p^-1 mod 2^BITS, conversion constants and R' mod p (values
+// generated from the primes of lib/cbits/curves/fields/mont/*_mont.c)
 struct ZBn254Fp {
   static constexpr int BITS = 29;
   static constexpr int N = 9;
@@ -41,7 +40,6 @@ struct ZBn254Fp {
   static constexpr int32_t K_IN[N] = {322215073, 442336424, 171859116, 268585440, 314135016, 244503300, 348886451, 68918589, 360451};
   static constexpr int32_t K_OUT[N] = {93261213, 451550318, 297979764, 299258347, 342016167, 297253948, 482187706, 406012155, 920183};
   static constexpr int32_t ONE[N] = {360500257, 337389400, 408039635, 21759001, 178483129, 490881230, 299191303, 86689704, 903222};
-  static constexpr uint32_t ZLOW[17] = {471340488, 345314063, 219287638, 93261213, 504105700, 378079275, 252052850, 126026425, 0, 410844487, 284818062, 158791637, 32765212, 443609699, 317583274, 191556849, 65530424};
 };
 struct ZBn254Fr {
   static constexpr int BITS = 29;
@@ -52,7 +50,6 @@ struct ZBn254Fr {
   static constexpr int32_t K_IN[N] = {268430039, 492061940, 71535269, 62181526, 323781850, 244503300, 348886451, 68918589, 360451};
   static constexpr int32_t K_OUT[N] = {268435451, 78749922, 406014571, 418196286, 342025071, 297253948, 482187706, 406012155, 920183};
   static constexpr int32_t ONE[N] = {268435287, 514263732, 86771339, 391139145, 178784091, 490881230, 299191303, 86689704, 903222};
-  static constexpr uint32_t ZLOW[17] = {536870904, 268435449, 536870906, 268435451, 536870908, 268435453, 536870910, 268435455, 0, 268435457, 2, 268435459, 4, 268435461, 6, 268435463, 8};
 };
 struct ZBls381Fr {
   static constexpr int BITS = 29;
@@ -63,7 +60,6 @@ struct ZBls381Fr {
   static constexpr int32_t K_IN[N] = {536868651, 18087, 526333248, 216203804, 294598241, 8484555, 89016444, 271483280, 1968430};
   static constexpr int32_t K_OUT[N] = {536870910, 15, 13762688, 158331904, 83842440, 133690974, 364799537, 188061856, 1582257};
   static constexpr int32_t ONE[N] = {536870842, 559, 481694080, 172907520, 250130850, 384216799, 419952827, 139714039, 5047359};
-  static constexpr uint32_t ZLOW[17] = {536870904, 536870905, 536870906, 536870907, 536870908, 536870909, 536870910, 536870911, 0, 1, 2, 3, 4, 5, 6, 7, 8};
 };
 struct ZBls381Fp {
   static constexpr int BITS = 28;
@@ -74,7 +70,6 @@ struct ZBls381Fp {
   static constexpr int32_t K_IN[N] = {135160473, 55574580, 246491222, 233514649, 211322480, 82798231, 118064104, 172912617, 65577043, 248456990, 203477795, 116089733, 75436678, 87770};
   static constexpr int32_t K_OUT[N] = {196605, 9437184, 201327222, 48192, 146467828, 91569269, 90130584, 117777780, 130963539, 91145943, 119183266, 239678912, 247724672, 89957};
   static constexpr int32_t ONE[N] = {55049400, 226492416, 176409, 13493970, 208802528, 138027152, 3630751, 228652862, 162569026, 19495856, 85318031, 1231484, 106560762, 22394};
-  static constexpr uint32_t ZLOW[17] = {174760, 152915, 131070, 109225, 87380, 65535, 43690, 21845, 0, 268413611, 268391766, 268369921, 268348076, 268326231, 268304386, 268282541, 268260696};
 };
 
 template <class P>
@@ -276,33 +271,28 @@ ZK_HD void zinv(Fz<P> &r, const Fz<P> &a) {
   r = acc;
 }
 
-// a == 0 (mod p) for a normalised a with |a| < 8p.  The low limb of j p (j = -8..8) filters;
-// the exact check runs only on a filter hit.
+// a == 0 (mod p) for a normalised a with |a| < 8p, i.e. a = j p for some j in [-8, 8].  Then
+// the low limb is j p mod 2^BITS, so (low * NP) mod 2^BITS = -j mod 2^BITS (NP = -p^-1): one
+// product filters, and only a hit (|j| <= 8) runs the exact comparison with that j p.
 template <class P>
 ZK_HD bool zis_zero_mod(const Fz<P> &a) {
   constexpr int N = P::N, B = P::BITS;
   constexpr int64_t MASK = (1ll << B) - 1;
-  const uint32_t low = (uint32_t)a.l[0];
-  bool cand = false;
+  const uint32_t t = ((uint32_t)a.l[0] * P::NP + 8u) & (uint32_t)MASK;  // 8 - j
+  if (t > 16u) return false;
+  const int64_t j = 8 - (int64_t)t;
+  int64_t c = 0;
 #pragma unroll
-  for (int j = 0; j < 17; j++) cand |= (low == P::ZLOW[j]);
-  if (!cand) return false;
-  for (int j = -8; j <= 8; j++) {
-    if (low != P::ZLOW[j + 8]) continue;
-    int64_t c = 0;
-    bool z = true;
-    for (int i = 0; i < N; i++) {
-      const int64_t d = (int64_t)a.l[i] - (int64_t)j * P::P[i] + c;
-      if (i < N - 1) {
-        if ((d & MASK) != 0) z = false;
-        c = d >> B;
-      } else if (d != 0) {
-        z = false;
-      }
+  for (int i = 0; i < N; i++) {
+    const int64_t d = (int64_t)a.l[i] - j * P::P[i] + c;
+    if (i < N - 1) {
+      if (d & MASK) return false;
+      c = d >> B;
+    } else if (d != 0) {
+      return false;
     }
-    if (z) return true;
   }
-  return false;
+  return true;
 }
 
 // canonical representative in [0, p) of a normalised a with -p < a < 2p

@@ -66,12 +66,37 @@ __device__ __forceinline__ uint32_t encode_digit(uint32_t raw, int c, uint32_t &
 }
 
 template <class C>
+__device__ __forceinline__ void point_to_internal(const uint32_t *__restrict__ pts, uint32_t i,
+                                                  int32_t *__restrict__ out) {
+  using F = typename C::Fp;
+  uint32_t w[2 * F::N];
+  const uint4 *q = reinterpret_cast<const uint4 *>(pts + (size_t)i * 2 * F::N);
+#pragma unroll
+  for (int j = 0; j < F::N / 2; j++) {
+    uint4 v = q[j];
+    w[4 * j] = v.x;
+    w[4 * j + 1] = v.y;
+    w[4 * j + 2] = v.z;
+    w[4 * j + 3] = v.w;
+  }
+  AffZ<C> a;
+  zfrom_ext(a.x, w);
+  zfrom_ext(a.y, w + F::N);
+  zaff_store(out + (size_t)i * ZLayout<C>::AFF, a);
+}
+
+// signed digits of scalar i for every window; when ext_pts is given, point i is converted to
+// the internal record in the same pass (both sweeps are bound by memory traffic)
+template <class C>
 __global__ __launch_bounds__(kThreads) void msm_digits_kernel(const uint64_t *__restrict__ sc, int enl, int mont,
                                                              uint32_t n, int c, int W, int wide,
-                                                             uint32_t *__restrict__ dig) {
+                                                             uint32_t *__restrict__ dig,
+                                                             const uint32_t *__restrict__ ext_pts,
+                                                             int32_t *__restrict__ zpts) {
   using Fr = typename C::Fr;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  if (ext_pts) point_to_internal<C>(ext_pts, i, zpts);
   uint32_t carry = 0;
   if (enl == 4) {
     uint64_t s[4];
@@ -364,30 +389,13 @@ __device__ __forceinline__ void flush_run(const XyzzZ<C> &acc, bool cont_run, ui
   zxyzz_store(dst, acc);
 }
 
-// affine points, external Montgomery words -> internal reduced-radix form (once per MSM)
+// affine points, external Montgomery words -> internal records (prepared point sets)
 template <class C>
 __global__ __launch_bounds__(kThreads) void msm_points_in_kernel(const uint32_t *__restrict__ pts, uint32_t n,
                                                                 int32_t *__restrict__ out) {
-  using F = typename C::Fp;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  uint32_t w[2 * F::N];
-  const uint4 *q = reinterpret_cast<const uint4 *>(pts + (size_t)i * 2 * F::N);
-#pragma unroll
-  for (int j = 0; j < F::N / 2; j++) {
-    uint4 v = q[j];
-    w[4 * j] = v.x;
-    w[4 * j + 1] = v.y;
-    w[4 * j + 2] = v.z;
-    w[4 * j + 3] = v.w;
-  }
-  AffZ<C> a;
-  zfrom_ext(a.x, w);
-  zfrom_ext(a.y, w + F::N);
-  int32_t *o = out + (size_t)i * ZLayout<C>::AFF;
-  zaff_store(o, a);
-#pragma unroll
-  for (int j = 2 * ZP<C>::T::N; j < ZLayout<C>::AFF; j++) o[j] = 0;
+  point_to_internal<C>(pts, i, out);
 }
 
 // final window terms: internal XYZZ -> external canonical words (4 field elements each)
@@ -500,12 +508,28 @@ __global__ __launch_bounds__(kThreads) void msm_accum_kernel(const uint32_t *__r
 // ---------------------------------------------------------------------------------------
 // 6. continuation partials
 
+// buckets whose runs span more than `limit` segments get pairwise trees (heavy[0] = count,
+// heavy[1] = most partials, heavy[2..] = buckets); known from the offsets alone, so this runs
+// before the accumulation and its counts reach the host while the accumulation runs
+__global__ __launch_bounds__(kThreads) void msm_heavy_detect_kernel(const uint32_t *__restrict__ off, uint32_t nkeys,
+                                                                   int seg_log, uint32_t *__restrict__ heavy,
+                                                                   uint32_t heavy_cap, uint32_t limit) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nkeys) return;
+  const uint32_t s = off[b], t = off[b + 1];
+  if (s == t) return;
+  const uint64_t k = ((uint64_t)(t - 1) >> seg_log) - (s >> seg_log);
+  if (k > limit) {
+    const uint32_t idx = atomicAdd(&heavy[0], 1u);
+    atomicMax(&heavy[1], (uint32_t)k);
+    if (idx < heavy_cap) heavy[2 + idx] = b;
+  }
+}
+
 template <class C>
 __global__ __launch_bounds__(kThreads) void msm_fix_kernel(const uint32_t *__restrict__ off, uint32_t nkeys,
                                                           int seg_log, int32_t *__restrict__ bsum,
-                                                          const int32_t *__restrict__ cont,
-                                                          uint32_t *__restrict__ heavy, uint32_t heavy_cap,
-                                                          uint32_t limit) {
+                                                          const int32_t *__restrict__ cont, uint32_t limit) {
   constexpr int PW = ZLayout<C>::XYZZ;
   const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nkeys) return;
@@ -514,12 +538,7 @@ __global__ __launch_bounds__(kThreads) void msm_fix_kernel(const uint32_t *__res
   const uint64_t fs = s >> seg_log, ls = (uint64_t)(t - 1) >> seg_log;
   if (ls == fs) return;
   const uint64_t k = ls - fs;
-  if (k > limit) {
-    const uint32_t idx = atomicAdd(&heavy[0], 1u);
-    atomicMax(&heavy[1], (uint32_t)k);
-    if (idx < heavy_cap) heavy[2 + idx] = b;
-    return;
-  }
+  if (k > limit) return;  // a heavy bucket
   XyzzZ<C> acc, q;
   zxyzz_load(acc, bsum + (size_t)b * PW);
   for (uint64_t j = fs + 1; j <= ls; j++) {
@@ -759,6 +778,19 @@ struct Events {
   }
 };
 
+// per host thread: a pinned 16-byte buffer and an event for the mid-MSM read-back (each
+// thread drives its own stream; both live until the process ends)
+static uint32_t *pinned_info() {
+  static thread_local uint32_t *p = nullptr;
+  if (!p && hipHostMalloc((void **)&p, 16, hipHostMallocDefault) != hipSuccess) p = nullptr;
+  return p;
+}
+static hipEvent_t info_event() {
+  static thread_local hipEvent_t e = nullptr;
+  if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) e = nullptr;
+  return e;
+}
+
 static int msm_set_lds_attributes() {
   ZK_HIP(hipFuncSetAttribute((const void *)msm_chunk_hist_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                              128 * 1024));
@@ -923,11 +955,9 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   const dim3 blk(kThreads);
   const dim3 gpts((n + kThreads - 1) / kThreads);
   const int32_t *zp = points_internal ? (const int32_t *)d_points : zpts.as<int32_t>();
-  if (!points_internal)
-    hipLaunchKernelGGL(msm_points_in_kernel<C>, gpts, blk, 0, st, (const uint32_t *)d_points, (uint32_t)n,
-                       zpts.as<int32_t>());
   hipLaunchKernelGGL(msm_digits_kernel<C>, gpts, blk, 0, st, d_scalars, enl, mont, (uint32_t)n, c, W, sch.wide,
-                     dig.as<uint32_t>());
+                     dig.as<uint32_t>(), points_internal ? (const uint32_t *)nullptr : (const uint32_t *)d_points,
+                     zpts.as<int32_t>());
   const dim3 gpw((n + kThreads - 1) / kThreads, W);
   const dim3 gkeys((unsigned)((nkeys + kThreads - 1) / kThreads));
   const size_t lds_words = ((size_t)B + 1) / 2 * 4;
@@ -963,6 +993,19 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
                        cursor.as<uint32_t>(), vals.as<uint32_t>());
   }
   ZK_HIP(hipGetLastError());
+  const uint32_t limit = (user && user->fix_limit > 0) ? (uint32_t)user->fix_limit : 16u;
+  hipLaunchKernelGGL(msm_heavy_detect_kernel, dim3((unsigned)((nkeys + kThreads - 1) / kThreads)), blk, 0, st,
+                     off.as<uint32_t>(), (uint32_t)nkeys, seg_log, heavy.as<uint32_t>(), heavy_cap, limit);
+  ZK_HIP(hipGetLastError());
+  // heavy count, most partials and sorted entries, read back while the accumulation runs
+  uint32_t *info = pinned_info();
+  if (!info) {
+    set_error("msm: pinned host buffer unavailable");
+    return ZK_ENOMEM;
+  }
+  ZK_HIP(hipMemcpyAsync(info, heavy.p, 8, hipMemcpyDeviceToHost, st));
+  ZK_HIP(hipMemcpyAsync(info + 2, off.as<uint32_t>() + nkeys, 4, hipMemcpyDeviceToHost, st));
+  ZK_HIP(hipEventRecord(info_event(), st));
   ev.mark(st);
   const bool prefetch = user && user->prefetch == 2;
   if (prefetch)
@@ -975,15 +1018,11 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
                        cont.as<int32_t>());
   ZK_HIP(hipGetLastError());
   ev.mark(st);
-  const uint32_t limit = (user && user->fix_limit > 0) ? (uint32_t)user->fix_limit : 16u;
   hipLaunchKernelGGL(msm_fix_kernel<C>, dim3((unsigned)((nkeys + kThreads - 1) / kThreads)), blk, 0, st,
-                     off.as<uint32_t>(), (uint32_t)nkeys, seg_log, bsum.as<int32_t>(), cont.as<int32_t>(),
-                     heavy.as<uint32_t>(), heavy_cap, limit);
+                     off.as<uint32_t>(), (uint32_t)nkeys, seg_log, bsum.as<int32_t>(), cont.as<int32_t>(), limit);
   ZK_HIP(hipGetLastError());
-  uint32_t hh[2], nent = 0;
-  ZK_HIP(hipMemcpyAsync(hh, heavy.p, 8, hipMemcpyDeviceToHost, st));
-  ZK_HIP(hipMemcpyAsync(&nent, off.as<uint32_t>() + nkeys, 4, hipMemcpyDeviceToHost, st));
-  ZK_HIP(hipStreamSynchronize(st));
+  ZK_HIP(hipEventSynchronize(info_event()));  // completed before the accumulation started
+  const uint32_t hh[2] = {info[0], info[1]}, nent = info[2];
   g_stats[0] = c;
   g_stats[1] = W;
   g_stats[2] = nent;
