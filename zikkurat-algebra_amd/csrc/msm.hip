@@ -644,6 +644,40 @@ __global__ __launch_bounds__(kThreads) void msm_pyramid_kernel(const int32_t *__
   if (a == 0) zxyzz_store(out + ((uint64_t)A * per + rem) * PW, p1);
 }
 
+// two pyramid levels in one pass: input [A][W][M] -> output [A+2][W][M/4] (M >= 4).  For
+// a < A the output is the sum of 4 consecutive inputs (two pair levels); array A is the odd
+// children of the first level reduced once more (B1 + B3), array A + 1 the odd children of the
+// second level (B2 + B3).  Half the launches and half the traffic of two single levels.
+template <class C>
+__global__ __launch_bounds__(kThreads) void msm_pyramid2_kernel(const int32_t *__restrict__ in,
+                                                               int32_t *__restrict__ out, uint32_t W, uint32_t M,
+                                                               uint32_t A) {
+  constexpr int PW = ZLayout<C>::XYZZ;
+  const uint32_t q = M >> 2;
+  const uint64_t per = (uint64_t)W * q;
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= per * A) return;
+  const uint32_t a = (uint32_t)(t / per);
+  const uint64_t rem = t % per;  // = w * q + i
+  const int32_t *src = in + ((uint64_t)a * W * M + 4 * rem) * PW;
+  XyzzZ<C> x, y, p0;
+  if (a == 0) {
+    zxyzz_load(x, src + PW);
+    zxyzz_load(y, src + 3 * PW);
+    zxyzz_add(x, x, y);  // B1 + B3
+    zxyzz_store(out + ((uint64_t)A * per + rem) * PW, x);
+  }
+  zxyzz_load(x, src);
+  zxyzz_load(y, src + PW);
+  zxyzz_add(p0, x, y);  // B0 + B1
+  zxyzz_load(x, src + 2 * PW);
+  zxyzz_load(y, src + 3 * PW);
+  zxyzz_add(x, x, y);  // B2 + B3
+  if (a == 0) zxyzz_store(out + ((uint64_t)(A + 1) * per + rem) * PW, x);
+  zxyzz_add(p0, p0, x);
+  zxyzz_store(out + ((uint64_t)a * per + rem) * PW, p0);
+}
+
 // ---------------------------------------------------------------------------------------
 // fixed-base scalar multiplication k * G (input generation), affine output, 0 -> (0,0)
 
@@ -778,18 +812,58 @@ struct Events {
   }
 };
 
-// per host thread: a pinned 16-byte buffer and an event for the mid-MSM read-back (each
-// thread drives its own stream; both live until the process ends)
-static uint32_t *pinned_info() {
-  static thread_local uint32_t *p = nullptr;
-  if (!p && hipHostMalloc((void **)&p, 16, hipHostMallocDefault) != hipSuccess) p = nullptr;
-  return p;
-}
-static hipEvent_t info_event() {
-  static thread_local hipEvent_t e = nullptr;
-  if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) e = nullptr;
-  return e;
-}
+// Host-side resources of one running MSM: pinned words for the mid-MSM read-back, pinned
+// result words and an event.  Taken from a pool and returned after the call, so concurrent
+// callers (batch workers, serving threads) each hold one and the pool never grows past the
+// peak concurrency.
+struct MsmStage {
+  uint32_t *info = nullptr;  // heavy count, most partials, sorted entries
+  uint32_t *res = nullptr;   // pinned result words
+  size_t res_words = 0;
+  hipEvent_t info_ev = nullptr;
+  int init() {
+    ZK_HIP(hipHostMalloc((void **)&info, 16, hipHostMallocDefault));
+    ZK_HIP(hipEventCreateWithFlags(&info_ev, hipEventDisableTiming));
+    return ZK_OK;
+  }
+  int reserve_res(size_t words) {
+    if (words <= res_words) return ZK_OK;
+    if (res) ZK_HIP(hipHostFree(res));
+    res = nullptr;
+    res_words = 0;
+    ZK_HIP(hipHostMalloc((void **)&res, words * 4, hipHostMallocDefault));
+    res_words = words;
+    return ZK_OK;
+  }
+};
+
+static std::mutex g_stage_mu;
+static std::vector<MsmStage *> g_stage_free;
+
+struct StageLease {
+  MsmStage *s = nullptr;
+  ~StageLease() {
+    if (s) {
+      std::lock_guard<std::mutex> g(g_stage_mu);
+      g_stage_free.push_back(s);
+    }
+  }
+  int acquire() {
+    {
+      std::lock_guard<std::mutex> g(g_stage_mu);
+      if (!g_stage_free.empty()) {
+        s = g_stage_free.back();
+        g_stage_free.pop_back();
+        return ZK_OK;
+      }
+    }
+    MsmStage *n = new MsmStage;
+    const int st = n->init();
+    if (st != ZK_OK) return st;  // a half-built stage is dropped (its handles stay allocated)
+    s = n;
+    return ZK_OK;
+  }
+};
 
 static int msm_set_lds_attributes() {
   ZK_HIP(hipFuncSetAttribute((const void *)msm_chunk_hist_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -998,14 +1072,13 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
                      off.as<uint32_t>(), (uint32_t)nkeys, seg_log, heavy.as<uint32_t>(), heavy_cap, limit);
   ZK_HIP(hipGetLastError());
   // heavy count, most partials and sorted entries, read back while the accumulation runs
-  uint32_t *info = pinned_info();
-  if (!info) {
-    set_error("msm: pinned host buffer unavailable");
-    return ZK_ENOMEM;
-  }
+  StageLease lease;
+  ZK_TRY(lease.acquire());
+  MsmStage &stage = *lease.s;
+  uint32_t *info = stage.info;
   ZK_HIP(hipMemcpyAsync(info, heavy.p, 8, hipMemcpyDeviceToHost, st));
   ZK_HIP(hipMemcpyAsync(info + 2, off.as<uint32_t>() + nkeys, 4, hipMemcpyDeviceToHost, st));
-  ZK_HIP(hipEventRecord(info_event(), st));
+  ZK_HIP(hipEventRecord(stage.info_ev, st));
   ev.mark(st);
   const bool prefetch = user && user->prefetch == 2;
   if (prefetch)
@@ -1021,7 +1094,7 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   hipLaunchKernelGGL(msm_fix_kernel<C>, dim3((unsigned)((nkeys + kThreads - 1) / kThreads)), blk, 0, st,
                      off.as<uint32_t>(), (uint32_t)nkeys, seg_log, bsum.as<int32_t>(), cont.as<int32_t>(), limit);
   ZK_HIP(hipGetLastError());
-  ZK_HIP(hipEventSynchronize(info_event()));  // completed before the accumulation started
+  ZK_HIP(hipEventSynchronize(stage.info_ev));  // completed before the accumulation started
   const uint32_t hh[2] = {info[0], info[1]}, nent = info[2];
   g_stats[0] = c;
   g_stats[1] = W;
@@ -1064,13 +1137,21 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   uint32_t A = 1, M = B;
   int k = 0;
   while (M > 1) {
-    const uint64_t thr = (uint64_t)A * W * (M / 2);
-    hipLaunchKernelGGL(msm_pyramid_kernel<C>, dim3((unsigned)((thr + kThreads - 1) / kThreads)), blk, 0, st, in,
-                       bufs[k], (uint32_t)W, M, A);
+    if (M >= 4) {
+      const uint64_t thr = (uint64_t)A * W * (M / 4);
+      hipLaunchKernelGGL(msm_pyramid2_kernel<C>, dim3((unsigned)((thr + kThreads - 1) / kThreads)), blk, 0, st, in,
+                         bufs[k], (uint32_t)W, M, A);
+      A += 2;
+      M >>= 2;
+    } else {
+      const uint64_t thr = (uint64_t)A * W * (M / 2);
+      hipLaunchKernelGGL(msm_pyramid_kernel<C>, dim3((unsigned)((thr + kThreads - 1) / kThreads)), blk, 0, st, in,
+                         bufs[k], (uint32_t)W, M, A);
+      A += 1;
+      M >>= 1;
+    }
     in = bufs[k];
     k ^= 1;
-    A += 1;
-    M >>= 1;
   }
   ZK_HIP(hipGetLastError());
   ev.mark(st);
@@ -1080,8 +1161,9 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   hipLaunchKernelGGL(msm_points_out_kernel<C>, dim3((nres + kThreads - 1) / kThreads), blk, 0, st, in, nres,
                      outx.as<uint32_t>());
   ZK_HIP(hipGetLastError());
-  std::vector<uint32_t> res((size_t)nres * PW);
-  ZK_HIP(hipMemcpyAsync(res.data(), outx.p, res.size() * 4, hipMemcpyDeviceToHost, st));
+  ZK_TRY(stage.reserve_res((size_t)nres * PW));
+  const uint32_t *res = stage.res;
+  ZK_HIP(hipMemcpyAsync(stage.res, outx.p, (size_t)nres * PW * 4, hipMemcpyDeviceToHost, st));
   ZK_HIP(hipStreamSynchronize(st));
   ev.mark(st);
   auto t_host1 = std::chrono::steady_clock::now();
@@ -1089,7 +1171,7 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   // host Horner: sum over (w, j) of 2^(start_w + max(j-1, 0)) * term
   const zkh::FieldDesc &f = fp_desc(C::id);
   auto get = [&](uint32_t a, uint32_t w) {
-    const uint32_t *p = res.data() + ((size_t)a * W + w) * PW;
+    const uint32_t *p = res + ((size_t)a * W + w) * PW;
     zkh::Xyzz q;
     zkh::Fe *dst[4] = {&q.x, &q.y, &q.zz, &q.zzz};
     for (int e = 0; e < 4; e++) {
