@@ -201,6 +201,49 @@ def test_learned_policy_and_batch_bit_exact(zk, C):
     net.close()
 
 
+@pytest.mark.parametrize("C", CURVES, ids=IDS)
+def test_sparse_compaction_bit_exact(zk, C):
+    """Measured sparse scalar sets (half or more in {0, 1}) run as a compacted dense MSM plus a
+    direct sum of the points with scalar 1: every mix of zeros, ones and others, Montgomery and
+    one-limb scalars, external and prepared points, equals the plain path and the closed form."""
+    n = (1 << 14) + 37
+    ks = random_scalars(90 + C.id, n, C.r)
+    o = Oracle(C)
+    P = o.points_from_scalars(ks[:4096])
+    P = np.ascontiguousarray(np.tile(P, (n // 4096 + 1, 1))[:n])
+    kk = [ks[i % 4096] for i in range(n)]
+    dP = zk.DeviceArray.from_host(P)
+    prep = zk.PreparedPoints(C.id, dP, n)
+    rng = np.random.default_rng(91)
+    others = random_scalars(92, n, C.r)
+    cases = {
+        "sparse": [int(others[i]) if rng.random() < 0.2 else int(rng.integers(0, 2)) for i in range(n)],
+        "ones_only": [1] * n,
+        "zeros_and_ones": [int(rng.integers(0, 2)) for _ in range(n)],
+        "zeros_only": [0] * n,
+        "few_others": [int(others[i]) if i % 997 == 0 else 0 for i in range(n)],
+        "one_other": [5] + [1] * (n - 1),
+    }
+    for name, ss in cases.items():
+        S = ints_to_array(ss, 4)
+        dS = zk.DeviceArray.from_host(S)
+        ref = zk.msm_device(C.id, n, dS, dP)
+        e = sum(s * k for s, k in zip(ss, kk)) % C.r
+        if e:
+            assert C.words_to_affine(ref) == C.mul(e, C.g), name
+        for pol in (zk.SCHED_STATS, zk.SCHED_LEARNED):
+            assert (zk.msm_device(C.id, n, dS, dP, policy=pol) == ref).all(), (name, pol)
+            assert (zk.msm_device(C.id, n, dS, prep, policy=pol) == ref).all(), (name, pol, "prepared")
+        Sm = ints_to_array([C.fr_to_mont(x) for x in ss], 4)
+        dSm = zk.DeviceArray.from_host(Sm)
+        assert (zk.msm_device(C.id, n, dSm, dP, flags=zk.MSM_MONT, policy=zk.SCHED_STATS) == ref).all(), name
+        if all(x < 2**64 for x in ss):
+            S1 = np.ascontiguousarray(S[:, :1])
+            got = zk.msm_device(C.id, n, zk.DeviceArray.from_host(S1), dP, expo_nlimbs=1, policy=zk.SCHED_STATS)
+            assert (got == ref).all(), (name, "one limb")
+    prep.free()
+
+
 def test_cost_model_gain_over_heuristics(zk):
     """Exhaustive tuning of a small BN254 job set; the network trained on the grid picks schedules
     for a held-out mix.  Reports (and bounds) its time against the plain path, the analytic pick

@@ -29,6 +29,8 @@
 
 #include <chrono>
 #include <mutex>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "curve.hpp"
@@ -719,6 +721,52 @@ __global__ __launch_bounds__(kThreads) void scalar_mul_gen_kernel(const uint64_t
 }
 
 // ---------------------------------------------------------------------------------------
+// sparse scalar sets: compaction of the scalars outside {0, 1}
+
+// keep[i] = scalar i is neither 0 nor 1 (u32, for the scan); one[i] = scalar i is 1
+template <class C>
+__global__ __launch_bounds__(kThreads) void msm_sparse_flags_kernel(const uint64_t *__restrict__ sc, int enl,
+                                                                   int mont, uint32_t n,
+                                                                   uint32_t *__restrict__ keep,
+                                                                   uint8_t *__restrict__ one) {
+  using Fr = typename C::Fr;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t lo, hi = 0;
+  if (enl == 4 && mont) {
+    Fe<Fr> x;
+#pragma unroll
+    for (int j = 0; j < 8; j++) x.w[j] = (uint32_t)(sc[(size_t)i * 4 + j / 2] >> (32 * (j & 1)));
+    fto_std(x, x);
+    lo = (uint64_t)x.w[0] | ((uint64_t)x.w[1] << 32);
+#pragma unroll
+    for (int j = 2; j < 8; j++) hi |= x.w[j];
+  } else {
+    lo = sc[(size_t)i * enl];
+    for (int j = 1; j < enl; j++) hi |= sc[(size_t)i * enl + j];
+  }
+  const bool small = hi == 0 && lo <= 1;
+  keep[i] = small ? 0u : 1u;
+  one[i] = (hi == 0 && lo == 1) ? 1 : 0;
+}
+
+// scalar and point i move to slot pos[i] when kept (words per scalar enl, per point pw)
+__global__ __launch_bounds__(kThreads) void msm_sparse_gather_kernel(const uint64_t *__restrict__ sc, int enl,
+                                                                    const uint32_t *__restrict__ pts, int pw,
+                                                                    uint32_t n, const uint32_t *__restrict__ keep,
+                                                                    const uint32_t *__restrict__ pos,
+                                                                    uint64_t *__restrict__ sc_out,
+                                                                    uint32_t *__restrict__ pts_out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !keep[i]) return;
+  const uint32_t k = pos[i];
+  for (int j = 0; j < enl; j++) sc_out[(size_t)k * enl + j] = sc[(size_t)i * enl + j];
+  const uint4 *src = reinterpret_cast<const uint4 *>(pts + (size_t)i * pw);
+  uint4 *dst = reinterpret_cast<uint4 *>(pts_out + (size_t)k * pw);
+  for (int j = 0; j < pw / 4; j++) dst[j] = src[j];
+}
+
+// ---------------------------------------------------------------------------------------
 // host side
 
 static const zkh::FieldDesc &fp_desc(int curve) { return curve == 0 ? zkh::BN254_FP : zkh::BLS381_FP; }
@@ -821,9 +869,17 @@ struct MsmStage {
   uint32_t *res = nullptr;   // pinned result words
   size_t res_words = 0;
   hipEvent_t info_ev = nullptr;
+  hipStream_t aux = nullptr;     // a second stream, created on first use
+  hipEvent_t aux_ev = nullptr;
   int init() {
     ZK_HIP(hipHostMalloc((void **)&info, 16, hipHostMallocDefault));
     ZK_HIP(hipEventCreateWithFlags(&info_ev, hipEventDisableTiming));
+    return ZK_OK;
+  }
+  int ensure_aux() {
+    if (aux) return ZK_OK;
+    ZK_HIP(hipEventCreateWithFlags(&aux_ev, hipEventDisableTiming));
+    ZK_HIP(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking));
     return ZK_OK;
   }
   int reserve_res(size_t words) {
@@ -964,6 +1020,11 @@ static MsmSchedule resolve_schedule(int curve, int n, int enl, const zkg_msm_sch
 }
 
 template <class C>
+static int msm_sparse_run(int n, const uint64_t *d_scalars, int enl, int mont, const uint64_t *d_points,
+                          bool points_internal, const zkg_msm_schedule *user, hipStream_t st,
+                          const zkg_scalar_stats &stats, MsmResult *out);
+
+template <class C>
 static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const uint64_t *d_points,
                    bool points_internal, const zkg_msm_schedule *user, hipStream_t st, MsmResult *out) {
   using F = typename C::Fp;
@@ -973,6 +1034,11 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   zkg_scalar_stats stats;
   const bool use_stats = user && user->policy != ZKG_SCHED_PLAIN;
   if (use_stats) ZK_TRY(msm_scalar_stats(C::id, n, d_scalars, enl, mont, st, &stats));
+  // Measured sparse scalar sets (at least half the scalars 0 or 1): the other scalars and
+  // their points are compacted into a dense MSM of their own, the points of the scalars
+  // equal to 1 are summed directly, and the zeros drop out; the two sums are added on the host.
+  if (use_stats && n >= (1 << 14) && (uint64_t)(stats.n_zero + stats.n_one) * 2 >= (uint64_t)n)
+    return msm_sparse_run<C>(n, d_scalars, enl, mont, d_points, points_internal, user, st, stats, out);
   MsmSchedule sch;
   ZK_TRY(msm_resolve(C::id, n, enl, user, use_stats ? &stats : nullptr, &sch));
   const int c = sch.c, W = sch.windows, logB = c - 1;
@@ -1212,6 +1278,78 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   }
   g_ms_phase[g_nphase++] = std::chrono::duration<double, std::milli>(t_host2 - t_host1).count();
   g_ms_total = std::chrono::duration<double, std::milli>(t_host2 - t_host0).count();
+  return ZK_OK;
+}
+
+template <class C>
+static int msm_sparse_run(int n, const uint64_t *d_scalars, int enl, int mont, const uint64_t *d_points,
+                          bool points_internal, const zkg_msm_schedule *user, hipStream_t st,
+                          const zkg_scalar_stats &stats, MsmResult *out) {
+  using F = typename C::Fp;
+  const int pw = points_internal ? ZLayout<C>::AFF : 2 * F::N;  // u32 words per point record
+  const uint64_t k = (uint64_t)n - stats.n_zero - stats.n_one;   // scalars outside {0, 1}
+  DevBuf keep, pos, ones, scan_b, sc2, pts2;
+  ZK_TRY(keep.alloc((uint64_t)n * 4));
+  ZK_TRY(pos.alloc(((uint64_t)n + 1) * 4));
+  ZK_TRY(ones.alloc((uint64_t)n));
+  const uint64_t per_block = (uint64_t)kThreads * kScanPer;
+  const uint32_t nsb = (uint32_t)(((uint64_t)n + 1 + per_block - 1) / per_block);
+  ZK_TRY(scan_b.alloc((uint64_t)nsb * 4));
+  ZK_TRY(sc2.alloc((k ? k : 1) * enl * 8));
+  ZK_TRY(pts2.alloc((k ? k : 1) * pw * 4));
+  const dim3 blk(kThreads), g((n + kThreads - 1) / kThreads);
+  hipLaunchKernelGGL(msm_sparse_flags_kernel<C>, g, blk, 0, st, d_scalars, enl, mont, (uint32_t)n,
+                     keep.as<uint32_t>(), ones.as<uint8_t>());
+  hipLaunchKernelGGL(scan_block_sums_kernel, dim3(nsb), blk, 0, st, keep.as<uint32_t>(), (size_t)n,
+                     scan_b.as<uint32_t>());
+  hipLaunchKernelGGL(scan_top_kernel, dim3(1), blk, 0, st, scan_b.as<uint32_t>(), nsb);
+  // (the scan's cursor output duplicates pos)
+  hipLaunchKernelGGL(scan_final_kernel, dim3(nsb), blk, 0, st, keep.as<uint32_t>(), (size_t)n,
+                     scan_b.as<uint32_t>(), pos.as<uint32_t>(), pos.as<uint32_t>());
+  hipLaunchKernelGGL(msm_sparse_gather_kernel, g, blk, 0, st, d_scalars, enl, (const uint32_t *)d_points, pw,
+                     (uint32_t)n, keep.as<uint32_t>(), pos.as<uint32_t>(), sc2.as<uint64_t>(), pts2.as<uint32_t>());
+  ZK_HIP(hipGetLastError());
+  // the direct sum of the ones is latency-bound: it runs on a second stream, from a second
+  // host thread, beside the dense MSM
+  StageLease lease;
+  ZK_TRY(lease.acquire());
+  ZK_TRY(lease.s->ensure_aux());
+  ZK_HIP(hipEventRecord(lease.s->aux_ev, st));
+  ZK_HIP(hipStreamWaitEvent(lease.s->aux, lease.s->aux_ev, 0));
+  MsmResult dense, ones_sum;
+  int ones_status = ZK_OK;
+  std::string ones_err;  // error messages are per thread
+  std::thread side([&] {
+    ones_status = binary_sum_device(C::id, 1, n, ones.as<uint8_t>(), d_points, points_internal, lease.s->aux,
+                                    &ones_sum);
+    if (ones_status != ZK_OK) ones_err = last_error();
+  });
+  int dense_status = ZK_OK;
+  if (k) dense_status = msm_run<C>((int)k, sc2.as<uint64_t>(), enl, mont, pts2.as<uint64_t>(), points_internal, user,
+                                   st, &dense);
+  else msm_infinity(C::id, &dense);
+  side.join();
+  ZK_TRY(dense_status);
+  if (ones_status != ZK_OK) {
+    set_error(ones_err);
+    return ones_status;
+  }
+  const zkh::FieldDesc &f = fp_desc(C::id);
+  auto load = [&](const MsmResult &r) {
+    zkh::Xyzz q;
+    q.x = zkh::load(f, r.x);
+    q.y = zkh::load(f, r.y);
+    q.zz = zkh::load(f, r.zz);
+    q.zzz = zkh::load(f, r.zzz);
+    return q;
+  };
+  const zkh::Xyzz sum = zkh::xyzz_add(f, load(dense), load(ones_sum));
+  memset(out, 0, sizeof *out);
+  out->curve = C::id;
+  zkh::store(f, out->x, sum.x);
+  zkh::store(f, out->y, sum.y);
+  zkh::store(f, out->zz, sum.zz);
+  zkh::store(f, out->zzz, sum.zzz);
   return ZK_OK;
 }
 
