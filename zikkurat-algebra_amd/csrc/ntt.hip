@@ -60,10 +60,32 @@ template <class F> struct ZOf;
 template <> struct ZOf<Bn254Fr> { using T = ZBn254Fr; };
 template <> struct ZOf<Bls381Fr> { using T = ZBls381Fr; };
 
-static constexpr int kZW = 9;  // LDS words per element (9 limbs of 29 bits)
+// LDS words per element: 9 limbs of 29 bits padded to 12, so that an element moves as three
+// 16-byte LDS accesses
+static constexpr int kZW = 12;
+// largest LDS tile: 2^11 elements of 48 bytes (96 KiB)
+static constexpr int kMaxTileLog = 11;
 static constexpr int kBatchTileLog = 9;  // largest LDS tile (log2 elements) of a batched job
 
 // omega^e as an internal-form table entry; returns true when the table value must be negated
+// an element of 8 words in HBM (32-byte aligned): two 16-byte loads / stores around the
+// (un)packing, instead of eight word accesses
+template <class Z>
+__device__ __forceinline__ void zunpack_g(Fz<Z> &r, const uint32_t *__restrict__ p) {
+  const uint4 *q = reinterpret_cast<const uint4 *>(p);
+  const uint4 v0 = q[0], v1 = q[1];
+  const uint32_t w[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+  zunpack(r, w);
+}
+template <class Z>
+__device__ __forceinline__ void zpack_g(uint32_t *__restrict__ p, const Fz<Z> &a) {
+  uint32_t w[8];
+  zpack(w, a);
+  uint4 *q = reinterpret_cast<uint4 *>(p);
+  q[0] = make_uint4(w[0], w[1], w[2], w[3]);
+  q[1] = make_uint4(w[4], w[5], w[6], w[7]);
+}
+
 template <class Z>
 __device__ __forceinline__ bool twiddle_z(Fz<Z> &w, const uint32_t *__restrict__ tw, uint64_t e, int logn,
                                           int tw_shift, int inverse) {
@@ -72,7 +94,7 @@ __device__ __forceinline__ bool twiddle_z(Fz<Z> &w, const uint32_t *__restrict__
   const uint64_t half = N >> 1;
   const bool neg = e >= half;
   if (neg) e -= half;
-  zunpack(w, tw + ((e << tw_shift) * 8));
+  zunpack_g(w, tw + ((e << tw_shift) * 8));
   return neg;
 }
 
@@ -80,11 +102,11 @@ __device__ __forceinline__ bool twiddle_z(Fz<Z> &w, const uint32_t *__restrict__
 template <class Z>
 __device__ __forceinline__ void eta_pow_z(Fz<Z> &r, const uint32_t *__restrict__ et, uint64_t n) {
   Fz<Z> a, b;
-  zunpack(a, et + (n & 2047) * 8);
-  zunpack(b, et + (2048 + ((n >> 11) & 2047)) * 8);
+  zunpack_g(a, et + (n & 2047) * 8);
+  zunpack_g(b, et + (2048 + ((n >> 11) & 2047)) * 8);
   zmul(r, a, b);
   if (n >> 22) {
-    zunpack(a, et + (4096 + ((n >> 22) & 2047)) * 8);
+    zunpack_g(a, et + (4096 + ((n >> 22) & 2047)) * 8);
     zmul(r, r, a);
   }
 }
@@ -103,13 +125,43 @@ __device__ __forceinline__ void zneg_canon(Fz<Z> &r, const Fz<Z> &t) {
 
 template <class Z>
 __device__ __forceinline__ void zlds_load(Fz<Z> &r, const int32_t *p) {
+  static_assert(Z::N <= kZW, "element exceeds its LDS slot");
+  const int4 *q = reinterpret_cast<const int4 *>(p);
+  int32_t w[kZW];
 #pragma unroll
-  for (int i = 0; i < Z::N; i++) r.l[i] = p[i];
+  for (int j = 0; j < kZW / 4; j++) {
+    const int4 v = q[j];
+    w[4 * j] = v.x;
+    w[4 * j + 1] = v.y;
+    w[4 * j + 2] = v.z;
+    w[4 * j + 3] = v.w;
+  }
+#pragma unroll
+  for (int i = 0; i < Z::N; i++) r.l[i] = w[i];
 }
 template <class Z>
 __device__ __forceinline__ void zlds_store(int32_t *p, const Fz<Z> &a) {
+  int32_t w[kZW];
 #pragma unroll
-  for (int i = 0; i < Z::N; i++) p[i] = a.l[i];
+  for (int i = 0; i < kZW; i++) w[i] = i < Z::N ? a.l[i] : 0;
+  int4 *q = reinterpret_cast<int4 *>(p);
+#pragma unroll
+  for (int j = 0; j < kZW / 4; j++) q[j] = make_int4(w[4 * j], w[4 * j + 1], w[4 * j + 2], w[4 * j + 3]);
+}
+
+// DIT butterfly (u, v) -> (u + w v, u - w v) for the stored twiddle w, whose true value is -w
+// when neg (the outputs then swap); w == nullptr is the twiddle 1
+template <class Z>
+__device__ __forceinline__ void zbutterfly(Fz<Z> &u, Fz<Z> &v, const Fz<Z> *w, bool neg) {
+  Fz<Z> t, s0, s1;
+  if (w) zmul(t, v, *w);
+  else t = v;
+  zadd(s0, u, t);
+  zsub(s1, u, t);
+  znorm(s0);
+  znorm(s1);
+  u = neg ? s1 : s0;
+  v = neg ? s0 : s1;
 }
 
 // one workgroup's share of a pass: `blk` indexes the pass's tiles
@@ -145,7 +197,7 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t blk, i
       gi = (rr << a.s) + n;
     }
     E x;
-    zunpack(x, a.in + gi * 8);
+    zunpack_g(x, a.in + gi * 8);
     if (a.eta_pre) {
       E e;
       eta_pow_z(e, a.eta_pre, gi);
@@ -155,34 +207,68 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t blk, i
   }
   __syncthreads();
 
-  // ---- radix-2 DIT stages in LDS (natural-order result)
-  for (int lh = 0; lh < a.s; lh++) {
+  // ---- DIT stages in LDS (natural-order result), two at a time: a thread takes the four
+  // elements n0 + {0, h, 2h, 3h} through stages lh and lh + 1 in registers (a radix-4 group)
+  int lh = 0;
+  while (lh < a.s) {
     const int h = 1 << lh;
-    for (int q = tid; q < (elems >> 1); q += kNttThreads) {
-      const int c = q & (T - 1);
-      const int bf = q >> a.logt;
-      const int j = bf & (h - 1);
-      const int blk = bf >> lh;
-      const int pu = ((((blk << (lh + 1)) | j) << a.logt) | c) * kZW;
-      const int pv = pu + ((h << a.logt) * kZW);
-      E u, v, t;
-      zlds_load(u, lds + pu);
-      zlds_load(v, lds + pv);
-      bool neg = false;
-      if (lh > 0) {
-        E w;
-        neg = twiddle_z(w, a.tw, (uint64_t)j << (a.logn - lh - 1), a.logn, a.tw_shift, a.inverse);
-        zmul(t, v, w);
-      } else {
-        t = v;
+    if (lh + 1 < a.s) {
+      for (int q = tid; q < (elems >> 2); q += kNttThreads) {
+        const int c = q & (T - 1);
+        const int g = q >> a.logt;
+        const int j = g & (h - 1);
+        const int blk = g >> lh;
+        const int n0 = (blk << (lh + 2)) | j;
+        const int stride = (h << a.logt) * kZW;
+        const int p0 = ((n0 << a.logt) | c) * kZW;
+        E x0, x1, x2, x3;
+        zlds_load(x0, lds + p0);
+        zlds_load(x1, lds + p0 + stride);
+        zlds_load(x2, lds + p0 + 2 * stride);
+        zlds_load(x3, lds + p0 + 3 * stride);
+        if (lh > 0) {
+          E w1;
+          const bool n1 = twiddle_z(w1, a.tw, (uint64_t)j << (a.logn - lh - 1), a.logn, a.tw_shift, a.inverse);
+          zbutterfly(x0, x1, &w1, n1);
+          zbutterfly(x2, x3, &w1, n1);
+        } else {
+          zbutterfly(x0, x1, (const E *)nullptr, false);
+          zbutterfly(x2, x3, (const E *)nullptr, false);
+        }
+        E w2;
+        bool n2 = twiddle_z(w2, a.tw, (uint64_t)j << (a.logn - lh - 2), a.logn, a.tw_shift, a.inverse);
+        if (lh > 0 || j > 0) zbutterfly(x0, x2, &w2, n2);
+        else zbutterfly(x0, x2, (const E *)nullptr, false);
+        n2 = twiddle_z(w2, a.tw, (uint64_t)(j + h) << (a.logn - lh - 2), a.logn, a.tw_shift, a.inverse);
+        zbutterfly(x1, x3, &w2, n2);
+        zlds_store(lds + p0, x0);
+        zlds_store(lds + p0 + stride, x1);
+        zlds_store(lds + p0 + 2 * stride, x2);
+        zlds_store(lds + p0 + 3 * stride, x3);
       }
-      E s0, s1;
-      zadd(s0, u, t);
-      zsub(s1, u, t);
-      znorm(s0);
-      znorm(s1);
-      zlds_store(lds + pu, neg ? s1 : s0);
-      zlds_store(lds + pv, neg ? s0 : s1);
+      lh += 2;
+    } else {
+      for (int q = tid; q < (elems >> 1); q += kNttThreads) {
+        const int c = q & (T - 1);
+        const int bf = q >> a.logt;
+        const int j = bf & (h - 1);
+        const int blk = bf >> lh;
+        const int pu = ((((blk << (lh + 1)) | j) << a.logt) | c) * kZW;
+        const int pv = pu + ((h << a.logt) * kZW);
+        E u, v;
+        zlds_load(u, lds + pu);
+        zlds_load(v, lds + pv);
+        if (lh > 0) {
+          E w;
+          const bool neg = twiddle_z(w, a.tw, (uint64_t)j << (a.logn - lh - 1), a.logn, a.tw_shift, a.inverse);
+          zbutterfly(u, v, &w, neg);
+        } else {
+          zbutterfly(u, v, (const E *)nullptr, false);
+        }
+        zlds_store(lds + pu, u);
+        zlds_store(lds + pv, v);
+      }
+      lh += 1;
     }
     __syncthreads();
   }
@@ -216,7 +302,7 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t blk, i
     }
     zcanon(x, x);
     if (neg) zneg_canon(x, x);
-    zpack(a.out + go * 8, x);
+    zpack_g(a.out + go * 8, x);
   }
 }
 
@@ -956,7 +1042,7 @@ int ntt_tune(int curve, int m, int inverse, int reps, NttTuning *best, double *b
   double bms = 1e300;
   NttTuning bt = ntt_tuning_for(curve, m);
   for (int smax = 5; smax <= 11; smax++) {
-    for (int tl = 7; tl <= 12; tl++) {
+    for (int tl = 7; tl <= kMaxTileLog; tl++) {
       const NttTuning t{smax, tl};
       std::vector<float> ts;
       for (int r = -1; r < std::max(1, reps); r++) {
