@@ -60,11 +60,9 @@ template <class F> struct ZOf;
 template <> struct ZOf<Bn254Fr> { using T = ZBn254Fr; };
 template <> struct ZOf<Bls381Fr> { using T = ZBls381Fr; };
 
-// LDS words per element: 9 limbs of 29 bits padded to 12, so that an element moves as three
-// 16-byte LDS accesses
-static constexpr int kZW = 12;
-// largest LDS tile: 2^11 elements of 48 bytes (96 KiB)
-static constexpr int kMaxTileLog = 11;
+static constexpr int kZW = 9;  // LDS words per element (9 limbs of 29 bits)
+// largest LDS tile: 2^12 elements of 36 bytes (144 KiB)
+static constexpr int kMaxTileLog = 12;
 static constexpr int kBatchTileLog = 9;  // largest LDS tile (log2 elements) of a batched job
 
 // omega^e as an internal-form table entry; returns true when the table value must be negated
@@ -126,27 +124,13 @@ __device__ __forceinline__ void zneg_canon(Fz<Z> &r, const Fz<Z> &t) {
 template <class Z>
 __device__ __forceinline__ void zlds_load(Fz<Z> &r, const int32_t *p) {
   static_assert(Z::N <= kZW, "element exceeds its LDS slot");
-  const int4 *q = reinterpret_cast<const int4 *>(p);
-  int32_t w[kZW];
 #pragma unroll
-  for (int j = 0; j < kZW / 4; j++) {
-    const int4 v = q[j];
-    w[4 * j] = v.x;
-    w[4 * j + 1] = v.y;
-    w[4 * j + 2] = v.z;
-    w[4 * j + 3] = v.w;
-  }
-#pragma unroll
-  for (int i = 0; i < Z::N; i++) r.l[i] = w[i];
+  for (int i = 0; i < Z::N; i++) r.l[i] = p[i];
 }
 template <class Z>
 __device__ __forceinline__ void zlds_store(int32_t *p, const Fz<Z> &a) {
-  int32_t w[kZW];
 #pragma unroll
-  for (int i = 0; i < kZW; i++) w[i] = i < Z::N ? a.l[i] : 0;
-  int4 *q = reinterpret_cast<int4 *>(p);
-#pragma unroll
-  for (int j = 0; j < kZW / 4; j++) q[j] = make_int4(w[4 * j], w[4 * j + 1], w[4 * j + 2], w[4 * j + 3]);
+  for (int i = 0; i < Z::N; i++) p[i] = a.l[i];
 }
 
 // DIT butterfly (u, v) -> (u + w v, u - w v) for the stored twiddle w, whose true value is -w
@@ -207,12 +191,15 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t blk, i
   }
   __syncthreads();
 
-  // ---- DIT stages in LDS (natural-order result), two at a time: a thread takes the four
-  // elements n0 + {0, h, 2h, 3h} through stages lh and lh + 1 in registers (a radix-4 group)
+  // ---- DIT stages in LDS (natural-order result).  Tiles with at least four elements per
+  // thread take two stages at a time: a thread carries the four elements n0 + {0, h, 2h, 3h}
+  // through stages lh and lh + 1 in registers (a radix-4 group); smaller tiles keep every
+  // thread busy with one radix-2 butterfly per stage.
+  const bool pairs = (elems >> 2) >= kNttThreads;
   int lh = 0;
   while (lh < a.s) {
     const int h = 1 << lh;
-    if (lh + 1 < a.s) {
+    if (pairs && lh + 1 < a.s) {
       for (int q = tid; q < (elems >> 2); q += kNttThreads) {
         const int c = q & (T - 1);
         const int g = q >> a.logt;
