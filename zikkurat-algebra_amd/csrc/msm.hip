@@ -680,6 +680,73 @@ __global__ __launch_bounds__(kThreads) void msm_pyramid2_kernel(const int32_t *_
   zxyzz_store(out + ((uint64_t)a * per + rem) * PW, p0);
 }
 
+// The last pyramid levels hold a few hundred points per window, so a launch per level (or per
+// two) costs a dependent chain of additions plus a launch for very little work.  Here one
+// workgroup per window runs every remaining level, M -> 1, each a single-level step (pairs of
+// every array, plus the odd children of array 0).  Level 0 reads `in` ([A][W][M]) and writes
+// the workgroup's LDS; later levels update the LDS in place (every item is read into
+// registers before a barrier and written after it); the last level writes [A + L][W] to
+// `out`.  Windows share no memory but `in` (read only) and disjoint points of `out`, and
+// `out` must not overlap `in`.  The host guarantees at most 2 blockDim.x items at level 0,
+// at most blockDim.x at every later level, and LDS for the level-0 output.
+static constexpr int kTailThreads = 512;
+static constexpr size_t kTailLdsBytes = 128 * 1024;
+
+template <class C>
+__device__ __forceinline__ void tail_item(XyzzZ<C> &r, const int32_t *src, uint32_t M, uint32_t A, uint32_t a,
+                                          uint32_t i, uint64_t stride) {
+  constexpr int PW = ZLayout<C>::XYZZ;
+  if (a < A) {
+    XyzzZ<C> q;
+    zxyzz_load(r, src + (a * stride + 2 * i) * PW);
+    zxyzz_load(q, src + (a * stride + 2 * i + 1) * PW);
+    zxyzz_add(r, r, q);
+  } else {
+    zxyzz_load(r, src + (2 * i + 1) * PW);  // odd child of array 0
+  }
+}
+
+template <class C>
+__global__ __launch_bounds__(kTailThreads) void msm_pyramid_tail_kernel(const int32_t *__restrict__ in,
+                                                                        int32_t *__restrict__ out, uint32_t W,
+                                                                        uint32_t M, uint32_t A) {
+  constexpr int PW = ZLayout<C>::XYZZ;
+  extern __shared__ int32_t tl_lds[];
+  const uint32_t w = blockIdx.x;
+  // level 0: global [A][W][M] (window w's arrays are M points apart by W M) -> LDS [A+1][M/2]
+  {
+    const uint32_t half = M >> 1, items = (A + 1) * half;
+    for (uint32_t j = threadIdx.x; j < items; j += blockDim.x) {
+      const uint32_t a = j / half, i = j - a * half;
+      XyzzZ<C> r;
+      tail_item<C>(r, in + (uint64_t)w * M * PW, M, A, a, i, (uint64_t)W * M);
+      if (half == 1)
+        zxyzz_store(out + ((uint64_t)a * W + w) * PW, r);
+      else
+        zxyzz_store(tl_lds + (uint64_t)j * PW, r);
+    }
+    A += 1;
+    M = half;
+  }
+  while (M > 1) {
+    __syncthreads();  // the previous level's points are in LDS
+    const uint32_t half = M >> 1, items = (A + 1) * half;
+    const uint32_t j = threadIdx.x;
+    XyzzZ<C> r;
+    const uint32_t a = j / half, i = j - a * half;
+    if (j < items) tail_item<C>(r, tl_lds, M, A, a, i, M);
+    __syncthreads();  // every read of this level is done before the in-place writes
+    if (j < items) {
+      if (half == 1)
+        zxyzz_store(out + ((uint64_t)a * W + w) * PW, r);
+      else
+        zxyzz_store(tl_lds + (uint64_t)j * PW, r);
+    }
+    A += 1;
+    M = half;
+  }
+}
+
 // ---------------------------------------------------------------------------------------
 // fixed-base scalar multiplication k * G (input generation), affine output, 0 -> (0,0)
 
@@ -928,6 +995,10 @@ static int msm_set_lds_attributes() {
                              128 * 1024));
   ZK_HIP(hipFuncSetAttribute((const void *)msm_chunk_scatter_lds_kernel,
                              hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024));
+  ZK_HIP(hipFuncSetAttribute((const void *)msm_pyramid_tail_kernel<Bn254G1>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)kTailLdsBytes));
+  ZK_HIP(hipFuncSetAttribute((const void *)msm_pyramid_tail_kernel<Bls381G1>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)kTailLdsBytes));
   return ZK_OK;
 }
 
@@ -1203,6 +1274,21 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   uint32_t A = 1, M = B;
   int k = 0;
   while (M > 1) {
+    // the remaining levels in one launch once they fit one workgroup per window (level 0 at
+    // most two items per thread, later levels one; level 0's output in LDS)
+    const uint32_t items0 = (A + 1) * (M / 2), items1 = (A + 2) * (M / 4);
+    const size_t tail_lds = (size_t)items0 * ZW * 4;
+    if (items0 <= 2u * kTailThreads && items1 <= (uint32_t)kTailThreads && tail_lds <= kTailLdsBytes) {
+      int levels = 0;
+      while ((M >> levels) > 1) levels++;
+      const unsigned tb = std::min((std::max((items0 + 1) / 2, items1) + 63) & ~63u, (unsigned)kTailThreads);
+      hipLaunchKernelGGL(msm_pyramid_tail_kernel<C>, dim3((unsigned)W), dim3(tb), tail_lds, st, in, bufs[k],
+                         (uint32_t)W, M, A);
+      in = bufs[k];
+      A += levels;
+      M = 1;
+      break;
+    }
     if (M >= 4) {
       const uint64_t thr = (uint64_t)A * W * (M / 4);
       hipLaunchKernelGGL(msm_pyramid2_kernel<C>, dim3((unsigned)((thr + kThreads - 1) / kThreads)), blk, 0, st, in,
