@@ -27,6 +27,7 @@
 // its digits over most of its buckets.
 #include <string.h>
 
+#include <algorithm>
 #include <chrono>
 #include <mutex>
 #include <string>
@@ -183,16 +184,6 @@ __global__ __launch_bounds__(kThreads) void msm_stats_kernel(const uint64_t *__r
 // ---------------------------------------------------------------------------------------
 // 2-4. counting sort by (window, bucket)
 
-__global__ __launch_bounds__(kThreads) void msm_count_kernel(const uint32_t *__restrict__ dig, uint32_t n, int logB,
-                                                            uint32_t *__restrict__ count) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t w = blockIdx.y;
-  if (i >= n) return;
-  const uint32_t code = dig[(size_t)w * n + i];
-  if (code == kZeroDigit) return;
-  atomicAdd(&count[(w << logB) + (code >> 1)], 1u);
-}
-
 static constexpr int kScanPer = 16;  // elements per thread in the block scan
 
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *sh, uint32_t *total) {
@@ -237,7 +228,7 @@ __global__ __launch_bounds__(kThreads) void scan_top_kernel(uint32_t *__restrict
   }
 }
 
-// out[i] = exclusive prefix of in; out[len] = total
+// out[i] = exclusive prefix of in; out[len] = total (cursor, when given, gets a copy)
 __global__ __launch_bounds__(kThreads) void scan_final_kernel(const uint32_t *__restrict__ in, size_t len,
                                                              const uint32_t *__restrict__ bsums,
                                                              uint32_t *__restrict__ out, uint32_t *__restrict__ cursor) {
@@ -254,7 +245,7 @@ __global__ __launch_bounds__(kThreads) void scan_final_kernel(const uint32_t *__
   for (int k = 0; k < kScanPer; k++) {
     if (base + k < len) {
       out[base + k] = run;
-      cursor[base + k] = run;
+      if (cursor) cursor[base + k] = run;
     } else if (base + k == len) {
       out[len] = run;
     }
@@ -262,121 +253,119 @@ __global__ __launch_bounds__(kThreads) void scan_final_kernel(const uint32_t *__
   }
 }
 
-__global__ __launch_bounds__(kThreads) void msm_scatter_kernel(const uint32_t *__restrict__ dig, uint32_t n, int logB,
-                                                              uint32_t *__restrict__ cursor, uint32_t *__restrict__ vals) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t w = blockIdx.y;
-  if (i >= n) return;
-  const uint32_t code = dig[(size_t)w * n + i];
-  if (code == kZeroDigit) return;
-  const uint32_t key = (w << logB) + (code >> 1);
-  const uint32_t pos = atomicAdd(&cursor[key], 1u);
-  vals[pos] = i | (code << 31);
-}
-
-// LDS-staged counting sort for windows of up to 17 bits.  Workgroup (chunk, window) counts
-// the digits of at most 65535 consecutive points in LDS with packed 16-bit counters (two
-// buckets per u32 word, 2^(c-1) buckets <= 64 Ki -> <= 128 KiB of LDS), so one bucket's
-// count in a chunk never exceeds 16 bits.
+// Two-level counting sort by (window, bucket).  A window's B = P 2^L buckets form P
+// partitions of 2^L consecutive buckets.
+//   level 1  workgroup (chunk, window) counts its chunk's digits per partition in LDS; a scan
+//            over (window, partition, chunk) gives every chunk its run in each partition, and
+//            the chunk places (entry, low bucket bits) words there: runs of ~kPChunk / P
+//            consecutive 8-byte words, so HBM sees whole lines, not one 4-byte word per bucket.
+//   level 2  workgroup (window, partition) counts its partition's words per low bucket, writes
+//            the bucket offsets, and places the entries in bucket order.
+// Positions within a bucket come from LDS atomics, so their order varies from run to run;
+// the bucket sums do not depend on it.
 static constexpr int kSortThreads = 1024;
-static constexpr uint32_t kChunk = 32768;  // < 2^16: a chunk's bucket count fits 16 bits
+static constexpr uint32_t kPChunk = 65536;  // points per level-1 workgroup
 
-__global__ __launch_bounds__(kSortThreads) void msm_chunk_hist_kernel(const uint32_t *__restrict__ dig, uint32_t n,
-                                                                      int logB, uint16_t *__restrict__ hist) {
-  extern __shared__ uint32_t cnt[];  // 2^(logB-1) words (2^logB packed u16 counters)
-  const uint32_t B = 1u << logB, words = (B + 1) >> 1;
+// LDS counter increment returning the counter's previous value plus this lane's rank.  The
+// lanes that share the first active lane's key use one atomic for all of them when there are
+// enough of them (skewed digit sets put most lanes of a wave on one bucket).
+__device__ __forceinline__ uint32_t lds_rank(uint32_t *cnt, uint32_t key, bool active) {
+  const uint64_t act = __ballot(active);
+  if (act == 0) return 0;
+  const int leader = __ffsll((unsigned long long)act) - 1;
+  const uint32_t lkey = __shfl((int)key, leader);
+  const bool mine = active && key == lkey;
+  const uint64_t same = __ballot(mine);
+  if (__popcll(same) >= 8) {
+    const int lane = __lane_id();
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&cnt[lkey], (uint32_t)__popcll(same));
+    base = (uint32_t)__shfl((int)base, leader);
+    if (mine) return base + (uint32_t)__popcll(same & ((1ull << lane) - 1));
+    return active ? atomicAdd(&cnt[key], 1u) : 0u;
+  }
+  return active ? atomicAdd(&cnt[key], 1u) : 0u;
+}
+
+// phist[(w P + p) nchunks + chunk] = digits of the chunk in partition p of window w
+__global__ __launch_bounds__(kSortThreads) void msm_part_hist_kernel(const uint32_t *__restrict__ dig, uint32_t n,
+                                                                     int L, uint32_t P,
+                                                                     uint32_t *__restrict__ phist) {
+  extern __shared__ uint32_t cnt[];  // P counters
   const uint32_t chunk = blockIdx.x, w = blockIdx.y, nchunks = gridDim.x;
-  for (uint32_t j = threadIdx.x; j < words; j += kSortThreads) cnt[j] = 0;
+  for (uint32_t j = threadIdx.x; j < P; j += kSortThreads) cnt[j] = 0;
   __syncthreads();
-  const uint32_t i0 = chunk * kChunk;
-  uint32_t i1 = i0 + kChunk;
-  if (i1 > n) i1 = n;
+  const uint32_t i0 = chunk * kPChunk;
+  const uint32_t i1 = n - i0 < kPChunk ? n : i0 + kPChunk;
   const uint32_t *d = dig + (size_t)w * n;
   for (uint32_t i = i0 + threadIdx.x; i < i1; i += kSortThreads) {
     const uint32_t code = d[i];
-    if (code == kZeroDigit) continue;
-    const uint32_t b = code >> 1;
-    atomicAdd(&cnt[b >> 1], 1u << (16 * (b & 1)));
+    const bool active = code != kZeroDigit;
+    lds_rank(cnt, active ? (code >> 1) >> L : 0u, active);
   }
   __syncthreads();
-  uint16_t *h = hist + ((size_t)w * nchunks + chunk) * B;
-  for (uint32_t b = threadIdx.x; b < B; b += kSortThreads) h[b] = (uint16_t)(cnt[b >> 1] >> (16 * (b & 1)));
+  for (uint32_t p = threadIdx.x; p < P; p += kSortThreads) phist[((size_t)w * P + p) * nchunks + chunk] = cnt[p];
 }
 
-// count[w B + b] = sum over chunks
-__global__ __launch_bounds__(kThreads) void msm_bucket_total_kernel(const uint16_t *__restrict__ hist, uint32_t W,
-                                                                   uint32_t logB, uint32_t nchunks,
-                                                                   uint32_t *__restrict__ count) {
-  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t B = 1u << logB;
-  if (t >= (uint64_t)W * B) return;
-  const uint32_t w = (uint32_t)(t >> logB), b = (uint32_t)(t & (B - 1));
+// tmp[ppos...] = entry (point index | sign << 31) | low bucket bits << 32
+__global__ __launch_bounds__(kSortThreads) void msm_part_scatter_kernel(const uint32_t *__restrict__ dig, uint32_t n,
+                                                                        int L, uint32_t P,
+                                                                        const uint32_t *__restrict__ ppos,
+                                                                        uint64_t *__restrict__ tmp) {
+  extern __shared__ uint32_t cur[];  // P cursors
+  const uint32_t chunk = blockIdx.x, w = blockIdx.y, nchunks = gridDim.x;
+  for (uint32_t p = threadIdx.x; p < P; p += kSortThreads) cur[p] = ppos[((size_t)w * P + p) * nchunks + chunk];
+  __syncthreads();
+  const uint32_t i0 = chunk * kPChunk;
+  const uint32_t i1 = n - i0 < kPChunk ? n : i0 + kPChunk;
+  const uint32_t *d = dig + (size_t)w * n;
+  const uint32_t lmask = (1u << L) - 1;
+  for (uint32_t i = i0 + threadIdx.x; i < i1; i += kSortThreads) {
+    const uint32_t code = d[i];
+    const bool active = code != kZeroDigit;
+    const uint32_t b = code >> 1;
+    const uint32_t pos = lds_rank(cur, active ? b >> L : 0u, active);
+    if (active) tmp[pos] = (uint64_t)(i | (code << 31)) | ((uint64_t)(b & lmask) << 32);
+  }
+}
+
+// Workgroup g = w P + p sorts tmp[ppos[g nchunks], ppos[(g + 1) nchunks]) by the low bucket
+// bits into vals and writes off[g 2^L + lo] (global bucket w B + p 2^L + lo); the last
+// workgroup also writes off[nparts 2^L] = the total.
+__global__ __launch_bounds__(kThreads) void msm_part_sort_kernel(const uint64_t *__restrict__ tmp,
+                                                                 const uint32_t *__restrict__ ppos, uint32_t nchunks,
+                                                                 int L, uint32_t *__restrict__ off,
+                                                                 uint32_t *__restrict__ vals) {
+  extern __shared__ uint32_t lc[];  // 2^L counters, then cursors
+  __shared__ uint32_t sh[kThreads];
+  const uint32_t g = blockIdx.x, NL = 1u << L, t = threadIdx.x;
+  const uint32_t start = ppos[(size_t)g * nchunks], end = ppos[(size_t)(g + 1) * nchunks];
+  for (uint32_t j = t; j < NL; j += kThreads) lc[j] = 0;
+  __syncthreads();
+  for (uint32_t e = start + t; e < end; e += kThreads) lds_rank(lc, (uint32_t)(tmp[e] >> 32), true);
+  __syncthreads();
+  // exclusive scan of the counters: thread t owns `per` consecutive ones
+  const uint32_t per = NL > kThreads ? NL / kThreads : 1u;
+  const uint32_t j0 = t * per;
   uint32_t s = 0;
-  for (uint32_t c = 0; c < nchunks; c++) s += hist[((size_t)w * nchunks + c) * B + b];
-  count[t] = s;
-}
-
-// pre[w][chunk][b] = off[w B + b] + sum of the bucket's counts in earlier chunks
-__global__ __launch_bounds__(kThreads) void msm_chunk_prefix_kernel(const uint16_t *__restrict__ hist,
-                                                                   const uint32_t *__restrict__ off, uint32_t W,
-                                                                   uint32_t logB, uint32_t nchunks,
-                                                                   uint32_t *__restrict__ pre) {
-  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t B = 1u << logB;
-  if (t >= (uint64_t)W * B) return;
-  const uint32_t w = (uint32_t)(t >> logB), b = (uint32_t)(t & (B - 1));
-  uint32_t run = off[t];
-  for (uint32_t c = 0; c < nchunks; c++) {
-    const size_t k = ((size_t)w * nchunks + c) * B + b;
-    pre[k] = run;
-    run += hist[k];
+  for (uint32_t k = 0; k < per; k++)
+    if (j0 + k < NL) s += lc[j0 + k];
+  uint32_t tot;
+  uint32_t run = start + block_exclusive_scan(s, sh, &tot);
+  for (uint32_t k = 0; k < per; k++) {
+    if (j0 + k < NL) {
+      const uint32_t cnt = lc[j0 + k];
+      lc[j0 + k] = run;
+      off[(size_t)g * NL + j0 + k] = run;
+      run += cnt;
+    }
   }
-}
-
-__global__ __launch_bounds__(kSortThreads) void msm_chunk_scatter_kernel(const uint32_t *__restrict__ dig, uint32_t n,
-                                                                         int logB, const uint32_t *__restrict__ pre,
-                                                                         uint32_t *__restrict__ vals) {
-  extern __shared__ uint32_t rank[];
-  const uint32_t B = 1u << logB, words = (B + 1) >> 1;
-  const uint32_t chunk = blockIdx.x, w = blockIdx.y, nchunks = gridDim.x;
-  for (uint32_t j = threadIdx.x; j < words; j += kSortThreads) rank[j] = 0;
+  if (g == gridDim.x - 1 && t == 0) off[(size_t)gridDim.x * NL] = end;
   __syncthreads();
-  const uint32_t i0 = chunk * kChunk;
-  uint32_t i1 = i0 + kChunk;
-  if (i1 > n) i1 = n;
-  const uint32_t *d = dig + (size_t)w * n;
-  const uint32_t *pb = pre + ((size_t)w * nchunks + chunk) * B;
-  for (uint32_t i = i0 + threadIdx.x; i < i1; i += kSortThreads) {
-    const uint32_t code = d[i];
-    if (code == kZeroDigit) continue;
-    const uint32_t b = code >> 1;
-    const uint32_t sh = 16 * (b & 1);
-    const uint32_t r = (atomicAdd(&rank[b >> 1], 1u << sh) >> sh) & 0xffffu;
-    vals[pb[b] + r] = i | (code << 31);
-  }
-}
-
-// windows of up to 16 bits: the chunk's absolute bucket bases fit in LDS (2^15 u32 = 128 KiB)
-// and the LDS atomic returns the final position directly
-__global__ __launch_bounds__(kSortThreads) void msm_chunk_scatter_lds_kernel(const uint32_t *__restrict__ dig,
-                                                                             uint32_t n, int logB,
-                                                                             const uint32_t *__restrict__ pre,
-                                                                             uint32_t *__restrict__ vals) {
-  extern __shared__ uint32_t base[];
-  const uint32_t B = 1u << logB;
-  const uint32_t chunk = blockIdx.x, w = blockIdx.y, nchunks = gridDim.x;
-  const uint32_t *pb = pre + ((size_t)w * nchunks + chunk) * B;
-  for (uint32_t j = threadIdx.x; j < B; j += kSortThreads) base[j] = pb[j];
-  __syncthreads();
-  const uint32_t i0 = chunk * kChunk;
-  uint32_t i1 = i0 + kChunk;
-  if (i1 > n) i1 = n;
-  const uint32_t *d = dig + (size_t)w * n;
-  for (uint32_t i = i0 + threadIdx.x; i < i1; i += kSortThreads) {
-    const uint32_t code = d[i];
-    if (code == kZeroDigit) continue;
-    const uint32_t pos = atomicAdd(&base[code >> 1], 1u);
-    vals[pos] = i | (code << 31);
+  for (uint32_t e = start + t; e < end; e += kThreads) {
+    const uint64_t v = tmp[e];
+    const uint32_t pos = lds_rank(lc, (uint32_t)(v >> 32), true);
+    vals[pos] = (uint32_t)v;
   }
 }
 
@@ -989,12 +978,6 @@ struct StageLease {
 };
 
 static int msm_set_lds_attributes() {
-  ZK_HIP(hipFuncSetAttribute((const void *)msm_chunk_hist_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             128 * 1024));
-  ZK_HIP(hipFuncSetAttribute((const void *)msm_chunk_scatter_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             128 * 1024));
-  ZK_HIP(hipFuncSetAttribute((const void *)msm_chunk_scatter_lds_kernel,
-                             hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024));
   ZK_HIP(hipFuncSetAttribute((const void *)msm_pyramid_tail_kernel<Bn254G1>,
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)kTailLdsBytes));
   ZK_HIP(hipFuncSetAttribute((const void *)msm_pyramid_tail_kernel<Bls381G1>,
@@ -1090,6 +1073,15 @@ static MsmSchedule resolve_schedule(int curve, int n, int enl, const zkg_msm_sch
   return s;
 }
 
+// sort partitions per window: about 4096 digits per partition for uniform scalars, at most
+// 2^10 partitions (level-1 LDS counters) and 2^12 buckets per partition (level-2 counters)
+static int sort_partition_bits(int n, int logB) {
+  int logP = 0;
+  while ((1ull << (logP + 12)) < (uint64_t)n) logP++;
+  logP = std::min(logP, std::min(logB, 10));
+  return std::max(logP, logB - 12);
+}
+
 template <class C>
 static int msm_sparse_run(int n, const uint64_t *d_scalars, int enl, int mont, const uint64_t *d_points,
                           bool points_internal, const zkg_msm_schedule *user, hipStream_t st,
@@ -1123,10 +1115,18 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   const int seg_log = sch.seg_log;
   const uint64_t nseg = (total >> seg_log) + 1;
 
-  DevBuf dig, vals, count, off, cursor, bsum, cont, heavy, pyrA, pyrB, scan_b, hist, pre_big, zpts, outx;
+  const int logP = sort_partition_bits(n, logB);
+  const int L = logB - logP;
+  const uint32_t P = 1u << logP;
+  const uint32_t nchunks = (uint32_t)(((uint64_t)n + kPChunk - 1) / kPChunk);
+  const uint64_t nph = (uint64_t)W * P * nchunks;  // level-1 counts, scanned in place of positions
+
+  DevBuf dig, vals, tmp, phist, ppos, off, bsum, cont, heavy, pyrA, pyrB, scan_b, zpts, outx;
   ZK_TRY(dig.alloc(total * 4));
   ZK_TRY(vals.alloc(total * 4));
-  ZK_TRY(count.alloc((nkeys + 1) * 4));
+  ZK_TRY(tmp.alloc(total * 8));
+  ZK_TRY(phist.alloc(nph * 4));
+  ZK_TRY(ppos.alloc((nph + 1) * 4));
   ZK_TRY(off.alloc((nkeys + 1) * 4));
   ZK_TRY(bsum.alloc(nkeys * ZW * 4));
   ZK_TRY(cont.alloc(nseg * ZW * 4));
@@ -1137,30 +1137,16 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   ZK_TRY(pyrA.alloc(nkeys * ZW * 4));
   ZK_TRY(pyrB.alloc(nkeys * ZW * 4));
   const uint64_t per_block = (uint64_t)kThreads * kScanPer;
-  const uint64_t nscan = nkeys + 1;
-  const uint32_t nsb = (uint32_t)((nscan + per_block - 1) / per_block);
+  const uint32_t nsb = (uint32_t)((nph + 1 + per_block - 1) / per_block);
   ZK_TRY(scan_b.alloc((uint64_t)nsb * 4));
-  const bool lds_sort = logB <= 16;
-  const uint32_t nchunks = (uint32_t)((n + kChunk - 1) / kChunk);
-  if (lds_sort) {
-    ZK_TRY(hist.alloc((uint64_t)W * nchunks * B * 2));
-    // the chunk prefixes (u32) live in the pyramid buffer, idle until the reduction, when it is
-    // large enough
-    if ((uint64_t)W * nchunks * B * 4 > nkeys * ZW * 4) ZK_TRY(pre_big.alloc((uint64_t)W * nchunks * B * 4));
-    static std::once_flag attr_once;
-    int attr_status = ZK_OK;
-    std::call_once(attr_once, [&] {
-      attr_status = msm_set_lds_attributes();
-    });
-    ZK_TRY(attr_status);
-  } else {
-    ZK_TRY(cursor.alloc((nkeys + 1) * 4));
-  }
+  static std::once_flag attr_once;
+  int attr_status = ZK_OK;
+  std::call_once(attr_once, [&] { attr_status = msm_set_lds_attributes(); });
+  ZK_TRY(attr_status);
 
   Events ev;
   auto t_host0 = std::chrono::steady_clock::now();
   ev.mark(st);
-  if (!lds_sort) ZK_HIP(hipMemsetAsync(count.p, 0, (nkeys + 1) * 4, st));
   ZK_HIP(hipMemsetAsync(bsum.p, 0, nkeys * ZW * 4, st));
   ZK_HIP(hipMemsetAsync(heavy.p, 0, 8, st));
   const dim3 blk(kThreads);
@@ -1169,40 +1155,18 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   hipLaunchKernelGGL(msm_digits_kernel<C>, gpts, blk, 0, st, d_scalars, enl, mont, (uint32_t)n, c, W, sch.wide,
                      dig.as<uint32_t>(), points_internal ? (const uint32_t *)nullptr : (const uint32_t *)d_points,
                      zpts.as<int32_t>());
-  const dim3 gpw((n + kThreads - 1) / kThreads, W);
-  const dim3 gkeys((unsigned)((nkeys + kThreads - 1) / kThreads));
-  const size_t lds_words = ((size_t)B + 1) / 2 * 4;
-  if (lds_sort) {
-    const dim3 gch(nchunks, W);
-    hipLaunchKernelGGL(msm_chunk_hist_kernel, gch, dim3(kSortThreads), lds_words, st, dig.as<uint32_t>(),
-                       (uint32_t)n, logB, hist.as<uint16_t>());
-    hipLaunchKernelGGL(msm_bucket_total_kernel, gkeys, blk, 0, st, hist.as<uint16_t>(), (uint32_t)W,
-                       (uint32_t)logB, nchunks, count.as<uint32_t>());
-    ZK_HIP(hipMemsetAsync(count.as<uint32_t>() + nkeys, 0, 4, st));
-  } else {
-    hipLaunchKernelGGL(msm_count_kernel, gpw, blk, 0, st, dig.as<uint32_t>(), (uint32_t)n, logB,
-                       count.as<uint32_t>());
-  }
-  hipLaunchKernelGGL(scan_block_sums_kernel, dim3(nsb), blk, 0, st, count.as<uint32_t>(), (size_t)nscan,
+  const dim3 gch(nchunks, W);
+  hipLaunchKernelGGL(msm_part_hist_kernel, gch, dim3(kSortThreads), (size_t)P * 4, st, dig.as<uint32_t>(),
+                     (uint32_t)n, L, P, phist.as<uint32_t>());
+  hipLaunchKernelGGL(scan_block_sums_kernel, dim3(nsb), blk, 0, st, phist.as<uint32_t>(), (size_t)nph,
                      scan_b.as<uint32_t>());
   hipLaunchKernelGGL(scan_top_kernel, dim3(1), blk, 0, st, scan_b.as<uint32_t>(), nsb);
-  uint32_t *cur_or_unused = lds_sort ? pyrB.as<uint32_t>() : cursor.as<uint32_t>();
-  hipLaunchKernelGGL(scan_final_kernel, dim3(nsb), blk, 0, st, count.as<uint32_t>(), (size_t)nkeys,
-                     scan_b.as<uint32_t>(), off.as<uint32_t>(), cur_or_unused);
-  if (lds_sort) {
-    uint32_t *pre = pre_big.p ? pre_big.as<uint32_t>() : pyrA.as<uint32_t>();
-    hipLaunchKernelGGL(msm_chunk_prefix_kernel, gkeys, blk, 0, st, hist.as<uint16_t>(), off.as<uint32_t>(),
-                       (uint32_t)W, (uint32_t)logB, nchunks, pre);
-    if (logB <= 15)
-      hipLaunchKernelGGL(msm_chunk_scatter_lds_kernel, dim3(nchunks, W), dim3(kSortThreads), (size_t)B * 4, st,
-                         dig.as<uint32_t>(), (uint32_t)n, logB, pre, vals.as<uint32_t>());
-    else
-      hipLaunchKernelGGL(msm_chunk_scatter_kernel, dim3(nchunks, W), dim3(kSortThreads), lds_words, st,
-                         dig.as<uint32_t>(), (uint32_t)n, logB, pre, vals.as<uint32_t>());
-  } else {
-    hipLaunchKernelGGL(msm_scatter_kernel, gpw, blk, 0, st, dig.as<uint32_t>(), (uint32_t)n, logB,
-                       cursor.as<uint32_t>(), vals.as<uint32_t>());
-  }
+  hipLaunchKernelGGL(scan_final_kernel, dim3(nsb), blk, 0, st, phist.as<uint32_t>(), (size_t)nph,
+                     scan_b.as<uint32_t>(), ppos.as<uint32_t>(), (uint32_t *)nullptr);
+  hipLaunchKernelGGL(msm_part_scatter_kernel, gch, dim3(kSortThreads), (size_t)P * 4, st, dig.as<uint32_t>(),
+                     (uint32_t)n, L, P, ppos.as<uint32_t>(), tmp.as<uint64_t>());
+  hipLaunchKernelGGL(msm_part_sort_kernel, dim3((unsigned)(W * P)), blk, ((size_t)1 << L) * 4, st,
+                     tmp.as<uint64_t>(), ppos.as<uint32_t>(), nchunks, L, off.as<uint32_t>(), vals.as<uint32_t>());
   ZK_HIP(hipGetLastError());
   const uint32_t limit = (user && user->fix_limit > 0) ? (uint32_t)user->fix_limit : 16u;
   hipLaunchKernelGGL(msm_heavy_detect_kernel, dim3((unsigned)((nkeys + kThreads - 1) / kThreads)), blk, 0, st,
@@ -1389,9 +1353,8 @@ static int msm_sparse_run(int n, const uint64_t *d_scalars, int enl, int mont, c
   hipLaunchKernelGGL(scan_block_sums_kernel, dim3(nsb), blk, 0, st, keep.as<uint32_t>(), (size_t)n,
                      scan_b.as<uint32_t>());
   hipLaunchKernelGGL(scan_top_kernel, dim3(1), blk, 0, st, scan_b.as<uint32_t>(), nsb);
-  // (the scan's cursor output duplicates pos)
   hipLaunchKernelGGL(scan_final_kernel, dim3(nsb), blk, 0, st, keep.as<uint32_t>(), (size_t)n,
-                     scan_b.as<uint32_t>(), pos.as<uint32_t>(), pos.as<uint32_t>());
+                     scan_b.as<uint32_t>(), pos.as<uint32_t>(), (uint32_t *)nullptr);
   hipLaunchKernelGGL(msm_sparse_gather_kernel, g, blk, 0, st, d_scalars, enl, (const uint32_t *)d_points, pw,
                      (uint32_t)n, keep.as<uint32_t>(), pos.as<uint32_t>(), sc2.as<uint64_t>(), pts2.as<uint32_t>());
   ZK_HIP(hipGetLastError());
@@ -1479,8 +1442,11 @@ size_t msm_footprint(int curve, int n, int enl, bool points_internal, const zkg_
   const uint64_t W = s.windows, B = 1ull << (s.c - 1), nk = W * B, total = W * (uint64_t)n;
   const uint64_t zw = curve == 0 ? ZLayout<Bn254G1>::XYZZ : ZLayout<Bls381G1>::XYZZ;
   const uint64_t aw = curve == 0 ? ZLayout<Bn254G1>::AFF : ZLayout<Bls381G1>::AFF;
-  const uint64_t nchunks = (n + kChunk - 1) / kChunk;
-  uint64_t b = total * 8 + nk * 16 + 3 * nk * zw * 4 + ((total >> s.seg_log) + 1) * zw * 4 + W * nchunks * B * 6;
+  const uint64_t nchunks = ((uint64_t)n + kPChunk - 1) / kPChunk;
+  const uint64_t nph = W * ((uint64_t)1 << sort_partition_bits(n, s.c - 1)) * nchunks;
+  // digits, entries and level-1 words; offsets and heavy list; bucket sums and two pyramid
+  // buffers; continuation partials; level-1 counts and positions
+  uint64_t b = total * 16 + nk * 8 + 3 * nk * zw * 4 + ((total >> s.seg_log) + 1) * zw * 4 + nph * 8;
   if (!points_internal) b += (uint64_t)n * aw * 4;
   return (size_t)b;
 }
