@@ -186,19 +186,20 @@ __global__ __launch_bounds__(kThreads) void msm_stats_kernel(const uint64_t *__r
 
 static constexpr int kScanPer = 16;  // elements per thread in the block scan
 
+template <int NT = kThreads>
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *sh, uint32_t *total) {
-  // Hillis-Steele over 256 threads in LDS
+  // Hillis-Steele over the NT threads of the workgroup in LDS
   const int t = threadIdx.x;
   sh[t] = v;
   __syncthreads();
-  for (int o = 1; o < kThreads; o <<= 1) {
+  for (int o = 1; o < NT; o <<= 1) {
     uint32_t x = t >= o ? sh[t - o] : 0u;
     __syncthreads();
     sh[t] += x;
     __syncthreads();
   }
   const uint32_t incl = sh[t];
-  *total = sh[kThreads - 1];
+  *total = sh[NT - 1];
   __syncthreads();
   return incl - v;
 }
@@ -264,7 +265,12 @@ __global__ __launch_bounds__(kThreads) void scan_final_kernel(const uint32_t *__
 // Positions within a bucket come from LDS atomics, so their order varies from run to run;
 // the bucket sums do not depend on it.
 static constexpr int kSortThreads = 1024;
-static constexpr uint32_t kPChunk = 65536;  // points per level-1 workgroup
+static constexpr uint32_t kPChunk = 32768;  // points per level-1 workgroup
+static constexpr int kPUnroll = 8;          // digits loaded ahead per thread (latency hiding)
+static constexpr size_t kPartStageBytes = 16384;  // LDS for one partition's level-1 words
+// dynamic LDS of the staged level-1 placement: 3 counters per partition (at most 2^10
+// partitions) and one chunk of words
+static constexpr size_t kScatterStageLds = (3 * 1024 + kPChunk) * 4;
 
 // LDS counter increment returning the counter's previous value plus this lane's rank.  The
 // lanes that share the first active lane's key use one atomic for all of them when there are
@@ -298,20 +304,50 @@ __global__ __launch_bounds__(kSortThreads) void msm_part_hist_kernel(const uint3
   const uint32_t i0 = chunk * kPChunk;
   const uint32_t i1 = n - i0 < kPChunk ? n : i0 + kPChunk;
   const uint32_t *d = dig + (size_t)w * n;
-  for (uint32_t i = i0 + threadIdx.x; i < i1; i += kSortThreads) {
-    const uint32_t code = d[i];
-    const bool active = code != kZeroDigit;
-    lds_rank(cnt, active ? (code >> 1) >> L : 0u, active);
+  for (uint32_t i = i0 + threadIdx.x; i < i1; i += kSortThreads * kPUnroll) {
+    uint32_t code[kPUnroll];
+#pragma unroll
+    for (int u = 0; u < kPUnroll; u++) {
+      const uint32_t k = i + u * kSortThreads;
+      code[u] = k < i1 ? d[k] : kZeroDigit;
+    }
+#pragma unroll
+    for (int u = 0; u < kPUnroll; u++) {
+      const bool active = code[u] != kZeroDigit;
+      lds_rank(cnt, active ? (code[u] >> 1) >> L : 0u, active);
+    }
   }
   __syncthreads();
   for (uint32_t p = threadIdx.x; p < P; p += kSortThreads) phist[((size_t)w * P + p) * nchunks + chunk] = cnt[p];
 }
 
-// tmp[ppos...] = entry (point index | sign << 31) | low bucket bits << 32
+// Level-1 words.  Narrow (u32): point index in the low ib bits, the sign above it, the low
+// bucket bits above that (for ib + 1 + L <= 32, which covers 2^20 points with up to 11 low
+// bits).  Wide (u64): the final entry (point index | sign << 31) low, the low bucket bits high.
+template <class T> struct PartWord;
+template <> struct PartWord<uint32_t> {
+  __device__ static uint32_t pack(uint32_t i, uint32_t sign, uint32_t lo, int ib) {
+    return i | (sign << ib) | (lo << (ib + 1));
+  }
+  __device__ static uint32_t lo(uint32_t v, int ib) { return v >> (ib + 1); }
+  __device__ static uint32_t entry(uint32_t v, int ib) {
+    return (v & ((1u << ib) - 1)) | (((v >> ib) & 1u) << 31);
+  }
+};
+template <> struct PartWord<uint64_t> {
+  __device__ static uint64_t pack(uint32_t i, uint32_t sign, uint32_t lo, int) {
+    return (uint64_t)(i | (sign << 31)) | ((uint64_t)lo << 32);
+  }
+  __device__ static uint32_t lo(uint64_t v, int) { return (uint32_t)(v >> 32); }
+  __device__ static uint32_t entry(uint64_t v, int) { return (uint32_t)v; }
+};
+
+// tmp[ppos...] = level-1 word of every nonzero digit of the chunk
+template <class T>
 __global__ __launch_bounds__(kSortThreads) void msm_part_scatter_kernel(const uint32_t *__restrict__ dig, uint32_t n,
-                                                                        int L, uint32_t P,
+                                                                        int L, uint32_t P, int ib,
                                                                         const uint32_t *__restrict__ ppos,
-                                                                        uint64_t *__restrict__ tmp) {
+                                                                        T *__restrict__ tmp) {
   extern __shared__ uint32_t cur[];  // P cursors
   const uint32_t chunk = blockIdx.x, w = blockIdx.y, nchunks = gridDim.x;
   for (uint32_t p = threadIdx.x; p < P; p += kSortThreads) cur[p] = ppos[((size_t)w * P + p) * nchunks + chunk];
@@ -320,29 +356,107 @@ __global__ __launch_bounds__(kSortThreads) void msm_part_scatter_kernel(const ui
   const uint32_t i1 = n - i0 < kPChunk ? n : i0 + kPChunk;
   const uint32_t *d = dig + (size_t)w * n;
   const uint32_t lmask = (1u << L) - 1;
-  for (uint32_t i = i0 + threadIdx.x; i < i1; i += kSortThreads) {
-    const uint32_t code = d[i];
-    const bool active = code != kZeroDigit;
-    const uint32_t b = code >> 1;
-    const uint32_t pos = lds_rank(cur, active ? b >> L : 0u, active);
-    if (active) tmp[pos] = (uint64_t)(i | (code << 31)) | ((uint64_t)(b & lmask) << 32);
+  for (uint32_t i = i0 + threadIdx.x; i < i1; i += kSortThreads * kPUnroll) {
+    uint32_t code[kPUnroll];
+#pragma unroll
+    for (int u = 0; u < kPUnroll; u++) {
+      const uint32_t k = i + u * kSortThreads;
+      code[u] = k < i1 ? d[k] : kZeroDigit;
+    }
+#pragma unroll
+    for (int u = 0; u < kPUnroll; u++) {
+      const bool active = code[u] != kZeroDigit;
+      const uint32_t b = code[u] >> 1;
+      const uint32_t pos = lds_rank(cur, active ? b >> L : 0u, active);
+      if (active) tmp[pos] = PartWord<T>::pack(i + u * kSortThreads, code[u] & 1u, b & lmask, ib);
+    }
+  }
+}
+
+// Narrow words with the chunk staged in LDS: the words are first placed in partition order in
+// LDS, then each wave copies whole partition runs, so every global store instruction writes
+// consecutive words (direct placement writes one word per run per instruction, and the runs'
+// lines leave L2 half written).  LDS: 3 P counters and kPChunk words.
+__global__ __launch_bounds__(kSortThreads) void msm_part_scatter_staged_kernel(
+    const uint32_t *__restrict__ dig, uint32_t n, int L, uint32_t P, int ib, const uint32_t *__restrict__ phist,
+    const uint32_t *__restrict__ ppos, uint32_t *__restrict__ tmp) {
+  extern __shared__ uint32_t sm[];
+  __shared__ uint32_t sh[kSortThreads];
+  uint32_t *lcur = sm, *gbase = sm + P, *lcnt = sm + 2 * P, *stage = sm + 3 * P;
+  const uint32_t chunk = blockIdx.x, w = blockIdx.y, nchunks = gridDim.x, t = threadIdx.x;
+  // P <= kSortThreads: thread t owns partition t
+  const size_t k = ((size_t)w * P + t) * nchunks + chunk;
+  const uint32_t c = t < P ? phist[k] : 0u;
+  uint32_t tot;
+  const uint32_t ex = block_exclusive_scan<kSortThreads>(c, sh, &tot);
+  if (t < P) {
+    lcur[t] = ex;
+    gbase[t] = ppos[k];
+    lcnt[t] = c;
+  }
+  __syncthreads();
+  const uint32_t i0 = chunk * kPChunk;
+  const uint32_t i1 = n - i0 < kPChunk ? n : i0 + kPChunk;
+  const uint32_t *d = dig + (size_t)w * n;
+  const uint32_t lmask = (1u << L) - 1;
+  for (uint32_t i = i0 + t; i < i1; i += kSortThreads * kPUnroll) {
+    uint32_t code[kPUnroll];
+#pragma unroll
+    for (int u = 0; u < kPUnroll; u++) {
+      const uint32_t q = i + u * kSortThreads;
+      code[u] = q < i1 ? d[q] : kZeroDigit;
+    }
+#pragma unroll
+    for (int u = 0; u < kPUnroll; u++) {
+      const bool active = code[u] != kZeroDigit;
+      const uint32_t b = code[u] >> 1;
+      const uint32_t pos = lds_rank(lcur, active ? b >> L : 0u, active);
+      if (active) stage[pos] = PartWord<uint32_t>::pack(i + u * kSortThreads, code[u] & 1u, b & lmask, ib);
+    }
+  }
+  __syncthreads();
+  // lcur[p] is now the end of run p in the stage
+  const uint32_t lane = t & 63, wave = t >> 6;
+  for (uint32_t p = wave; p < P; p += kSortThreads / 64) {
+    const uint32_t cnt = lcnt[p], ls = lcur[p] - cnt, gs = gbase[p];
+    for (uint32_t j = lane; j < cnt; j += 64) tmp[gs + j] = stage[ls + j];
   }
 }
 
 // Workgroup g = w P + p sorts tmp[ppos[g nchunks], ppos[(g + 1) nchunks]) by the low bucket
 // bits into vals and writes off[g 2^L + lo] (global bucket w B + p 2^L + lo); the last
-// workgroup also writes off[nparts 2^L] = the total.
-__global__ __launch_bounds__(kThreads) void msm_part_sort_kernel(const uint64_t *__restrict__ tmp,
+// workgroup also writes off[nparts 2^L] = the total.  A partition of at most `stage` words is
+// read from HBM once into LDS and its entries are ordered in LDS, then written with
+// consecutive stores; a larger one (skewed digits) is read twice and placed directly.
+template <class T>
+__global__ __launch_bounds__(kThreads) void msm_part_sort_kernel(const T *__restrict__ tmp,
                                                                  const uint32_t *__restrict__ ppos, uint32_t nchunks,
-                                                                 int L, uint32_t *__restrict__ off,
+                                                                 int L, int ib, uint32_t stage,
+                                                                 uint32_t *__restrict__ off,
                                                                  uint32_t *__restrict__ vals) {
-  extern __shared__ uint32_t lc[];  // 2^L counters, then cursors
+  extern __shared__ uint32_t lc[];  // 2^L counters, then cursors; the staged words; staged entries
   __shared__ uint32_t sh[kThreads];
   const uint32_t g = blockIdx.x, NL = 1u << L, t = threadIdx.x;
+  T *words = reinterpret_cast<T *>(lc + (NL < 2 ? 2 : NL));
+  uint32_t *ents = reinterpret_cast<uint32_t *>(words + stage);
   const uint32_t start = ppos[(size_t)g * nchunks], end = ppos[(size_t)(g + 1) * nchunks];
+  const bool staged = end - start <= stage;
   for (uint32_t j = t; j < NL; j += kThreads) lc[j] = 0;
   __syncthreads();
-  for (uint32_t e = start + t; e < end; e += kThreads) lds_rank(lc, (uint32_t)(tmp[e] >> 32), true);
+  for (uint32_t e = start + t; e < end; e += kThreads * 4) {
+    T v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const uint32_t k = e + u * kThreads;
+      v[u] = k < end ? tmp[k] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const uint32_t k = e + u * kThreads;
+      if (staged && k < end) words[k - start] = v[u];
+      lds_rank(lc, PartWord<T>::lo(v[u], ib), k < end);
+    }
+  }
   __syncthreads();
   // exclusive scan of the counters: thread t owns `per` consecutive ones
   const uint32_t per = NL > kThreads ? NL / kThreads : 1u;
@@ -362,10 +476,28 @@ __global__ __launch_bounds__(kThreads) void msm_part_sort_kernel(const uint64_t 
   }
   if (g == gridDim.x - 1 && t == 0) off[(size_t)gridDim.x * NL] = end;
   __syncthreads();
-  for (uint32_t e = start + t; e < end; e += kThreads) {
-    const uint64_t v = tmp[e];
-    const uint32_t pos = lds_rank(lc, (uint32_t)(v >> 32), true);
-    vals[pos] = (uint32_t)v;
+  for (uint32_t e = start + t; e < end; e += kThreads * 4) {
+    T v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const uint32_t k = e + u * kThreads;
+      v[u] = k < end ? (staged ? words[k - start] : tmp[k]) : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const bool active = e + u * kThreads < end;
+      const uint32_t pos = lds_rank(lc, PartWord<T>::lo(v[u], ib), active);
+      if (active) {
+        if (staged)
+          ents[pos - start] = PartWord<T>::entry(v[u], ib);
+        else
+          vals[pos] = PartWord<T>::entry(v[u], ib);
+      }
+    }
+  }
+  if (staged) {
+    __syncthreads();
+    for (uint32_t e = t; e < end - start; e += kThreads) vals[start + e] = ents[e];
   }
 }
 
@@ -978,6 +1110,8 @@ struct StageLease {
 };
 
 static int msm_set_lds_attributes() {
+  ZK_HIP(hipFuncSetAttribute((const void *)msm_part_scatter_staged_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)kScatterStageLds));
   ZK_HIP(hipFuncSetAttribute((const void *)msm_pyramid_tail_kernel<Bn254G1>,
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)kTailLdsBytes));
   ZK_HIP(hipFuncSetAttribute((const void *)msm_pyramid_tail_kernel<Bls381G1>,
@@ -1073,11 +1207,11 @@ static MsmSchedule resolve_schedule(int curve, int n, int enl, const zkg_msm_sch
   return s;
 }
 
-// sort partitions per window: about 4096 digits per partition for uniform scalars, at most
+// sort partitions per window: about 2048 digits per partition for uniform scalars, at most
 // 2^10 partitions (level-1 LDS counters) and 2^12 buckets per partition (level-2 counters)
 static int sort_partition_bits(int n, int logB) {
   int logP = 0;
-  while ((1ull << (logP + 12)) < (uint64_t)n) logP++;
+  while ((1ull << (logP + 11)) < (uint64_t)n) logP++;
   logP = std::min(logP, std::min(logB, 10));
   return std::max(logP, logB - 12);
 }
@@ -1124,7 +1258,11 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   DevBuf dig, vals, tmp, phist, ppos, off, bsum, cont, heavy, pyrA, pyrB, scan_b, zpts, outx;
   ZK_TRY(dig.alloc(total * 4));
   ZK_TRY(vals.alloc(total * 4));
-  ZK_TRY(tmp.alloc(total * 8));
+  // level-1 words are 32-bit when the point index, the sign and the low bucket bits fit
+  int ib = 0;
+  while ((1ull << ib) < (uint64_t)n) ib++;
+  const bool narrow = ib < 31 && ib + 1 + L <= 32;
+  ZK_TRY(tmp.alloc(total * (narrow ? 4 : 8)));
   ZK_TRY(phist.alloc(nph * 4));
   ZK_TRY(ppos.alloc((nph + 1) * 4));
   ZK_TRY(off.alloc((nkeys + 1) * 4));
@@ -1163,10 +1301,23 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   hipLaunchKernelGGL(scan_top_kernel, dim3(1), blk, 0, st, scan_b.as<uint32_t>(), nsb);
   hipLaunchKernelGGL(scan_final_kernel, dim3(nsb), blk, 0, st, phist.as<uint32_t>(), (size_t)nph,
                      scan_b.as<uint32_t>(), ppos.as<uint32_t>(), (uint32_t *)nullptr);
-  hipLaunchKernelGGL(msm_part_scatter_kernel, gch, dim3(kSortThreads), (size_t)P * 4, st, dig.as<uint32_t>(),
-                     (uint32_t)n, L, P, ppos.as<uint32_t>(), tmp.as<uint64_t>());
-  hipLaunchKernelGGL(msm_part_sort_kernel, dim3((unsigned)(W * P)), blk, ((size_t)1 << L) * 4, st,
-                     tmp.as<uint64_t>(), ppos.as<uint32_t>(), nchunks, L, off.as<uint32_t>(), vals.as<uint32_t>());
+  const size_t cnt_bytes = ((size_t)1 << std::max(L, 1)) * 4;
+  if (narrow) {
+    const uint32_t stage = kPartStageBytes / 4;
+    hipLaunchKernelGGL(msm_part_scatter_staged_kernel, gch, dim3(kSortThreads), kScatterStageLds, st,
+                       dig.as<uint32_t>(), (uint32_t)n, L, P, ib, phist.as<uint32_t>(), ppos.as<uint32_t>(),
+                       tmp.as<uint32_t>());
+    hipLaunchKernelGGL(msm_part_sort_kernel<uint32_t>, dim3((unsigned)(W * P)), blk,
+                       cnt_bytes + kPartStageBytes + (size_t)stage * 4, st, tmp.as<uint32_t>(), ppos.as<uint32_t>(),
+                       nchunks, L, ib, stage, off.as<uint32_t>(), vals.as<uint32_t>());
+  } else {
+    const uint32_t stage = kPartStageBytes / 8;
+    hipLaunchKernelGGL(msm_part_scatter_kernel<uint64_t>, gch, dim3(kSortThreads), (size_t)P * 4, st,
+                       dig.as<uint32_t>(), (uint32_t)n, L, P, ib, ppos.as<uint32_t>(), tmp.as<uint64_t>());
+    hipLaunchKernelGGL(msm_part_sort_kernel<uint64_t>, dim3((unsigned)(W * P)), blk,
+                       cnt_bytes + kPartStageBytes + (size_t)stage * 4, st, tmp.as<uint64_t>(), ppos.as<uint32_t>(),
+                       nchunks, L, ib, stage, off.as<uint32_t>(), vals.as<uint32_t>());
+  }
   ZK_HIP(hipGetLastError());
   const uint32_t limit = (user && user->fix_limit > 0) ? (uint32_t)user->fix_limit : 16u;
   hipLaunchKernelGGL(msm_heavy_detect_kernel, dim3((unsigned)((nkeys + kThreads - 1) / kThreads)), blk, 0, st,
