@@ -57,6 +57,26 @@ def test_msm_vs_oracle(zk, C, n):
 
 
 @pytest.mark.parametrize("C", CURVES, ids=IDS)
+@pytest.mark.parametrize("n", [9, 4099, 20000])
+def test_msm_scalars_above_r(zk, C, n):
+    """Standard-form scalars are 256-bit integers, not field elements: values at and above r
+    (r, r + 1, 2r - 1, 5r + 7, 2^256 - 1 and uniform 256-bit values) against the oracle, which
+    multiplies by the integers themselves, and the closed form (s mod r) summed over k_i G."""
+    o = Oracle(C)
+    ks = random_scalars(300 + n, n, C.r)
+    rng = np.random.default_rng(n)
+    ss = [int.from_bytes(rng.bytes(32), "little") for _ in range(n)]
+    top = (1 << 256) - 1
+    for j, v in enumerate([C.r, C.r + 1, 2 * C.r - 1, min(5 * C.r + 7, top), top, 0, 1, C.r - 1]):
+        ss[j] = v
+    P = make_points(C, ks)
+    S = ints_to_array(ss, 4)
+    exp = o.msm_affine(S, P)
+    assert (zk.msm_std(C.id, S, P) == exp).all()
+    assert C.words_to_affine(exp) == C.mul(sum(s * k for s, k in zip(ss, ks)) % C.r, C.g)
+
+
+@pytest.mark.parametrize("C", CURVES, ids=IDS)
 def test_all_reference_msm_symbols(zk, C):
     """Every exported MSM entry point (proj/jac x std/mont x affine/proj/jac out)."""
     o = Oracle(C)

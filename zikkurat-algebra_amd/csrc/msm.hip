@@ -92,7 +92,7 @@ __device__ __forceinline__ void point_to_internal(const uint32_t *__restrict__ p
 // the internal record in the same pass (both sweeps are bound by memory traffic)
 template <class C>
 __global__ __launch_bounds__(kThreads) void msm_digits_kernel(const uint64_t *__restrict__ sc, int enl, int mont,
-                                                             uint32_t n, int c, int W, int wide,
+                                                             int reduce, uint32_t n, int c, int W, int wide,
                                                              uint32_t *__restrict__ dig,
                                                              const uint32_t *__restrict__ ext_pts,
                                                              int32_t *__restrict__ zpts) {
@@ -112,6 +112,29 @@ __global__ __launch_bounds__(kThreads) void msm_digits_kernel(const uint64_t *__
       fto_std(x, x);
 #pragma unroll
       for (int j = 0; j < 4; j++) s[j] = (uint64_t)x.w[2 * j] | ((uint64_t)x.w[2 * j + 1] << 32);
+    }
+    if (reduce) {
+      // s < 2^256 < 6 r: at most five subtractions of r
+      uint64_t r[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) r[j] = (uint64_t)Fr::P[2 * j] | ((uint64_t)Fr::P[2 * j + 1] << 32);
+      for (int k = 0; k < 5; k++) {
+        bool ge = true;
+        for (int j = 3; j >= 0; j--) {
+          if (s[j] != r[j]) {
+            ge = s[j] > r[j];
+            break;
+          }
+        }
+        if (!ge) break;
+        uint64_t borrow = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const uint64_t d = s[j] - r[j] - borrow;
+          borrow = (s[j] < r[j] || (s[j] == r[j] && borrow)) ? 1 : 0;
+          s[j] = d;
+        }
+      }
     }
     int bit = 0;
     for (int w = 0; w < W; w++) {
@@ -1182,6 +1205,12 @@ int msm_resolve(int curve, int n, int enl, const zkg_msm_schedule *user, const z
   c = c < 1 ? 1 : (c > 20 ? 20 : c);
   s.seg_log = seg < 1 ? 1 : (seg > 12 ? 12 : seg);
   s.bits = st ? st->max_bits : 64 * enl;
+  // BN254's G1 has prime order r, so s P = (s mod r) P for every curve point: 4-limb scalars
+  // reduced mod r need windows for 254 bits (plus the carry), not 256
+  if (!st && curve == 0 && enl == 4) {
+    s.bits = 254;
+    s.reduce = 1;
+  }
   msm_window_layout(s.bits, c, st != nullptr, &s.windows, &s.c, &s.wide);
   *out = s;
   return ZK_OK;
@@ -1290,7 +1319,8 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   const dim3 blk(kThreads);
   const dim3 gpts((n + kThreads - 1) / kThreads);
   const int32_t *zp = points_internal ? (const int32_t *)d_points : zpts.as<int32_t>();
-  hipLaunchKernelGGL(msm_digits_kernel<C>, gpts, blk, 0, st, d_scalars, enl, mont, (uint32_t)n, c, W, sch.wide,
+  hipLaunchKernelGGL(msm_digits_kernel<C>, gpts, blk, 0, st, d_scalars, enl, mont, sch.reduce, (uint32_t)n, c, W,
+                     sch.wide,
                      dig.as<uint32_t>(), points_internal ? (const uint32_t *)nullptr : (const uint32_t *)d_points,
                      zpts.as<int32_t>());
   const dim3 gch(nchunks, W);
