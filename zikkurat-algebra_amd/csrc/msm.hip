@@ -834,6 +834,7 @@ __global__ __launch_bounds__(kThreads) void msm_pyramid2_kernel(const int32_t *_
 // `out` must not overlap `in`.  The host guarantees at most 2 blockDim.x items at level 0,
 // at most blockDim.x at every later level, and LDS for the level-0 output.
 static constexpr int kTailThreads = 512;
+static constexpr uint64_t kPyr2MinThreads = 65536;
 static constexpr size_t kTailLdsBytes = 128 * 1024;
 
 template <class C>
@@ -1434,7 +1435,10 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
       M = 1;
       break;
     }
-    if (M >= 4) {
+    // two levels per launch while that still fills the GPU (one thread per 4 points, 3-4
+    // dependent additions each); below, a level per launch (one addition per thread) has the
+    // shorter chain
+    if (M >= 4 && (uint64_t)A * W * (M / 4) >= kPyr2MinThreads) {
       const uint64_t thr = (uint64_t)A * W * (M / 4);
       hipLaunchKernelGGL(msm_pyramid2_kernel<C>, dim3((unsigned)((thr + kThreads - 1) / kThreads)), blk, 0, st, in,
                          bufs[k], (uint32_t)W, M, A);
