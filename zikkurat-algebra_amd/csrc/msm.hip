@@ -291,6 +291,7 @@ static constexpr int kSortThreads = 1024;
 static constexpr uint32_t kPChunk = 32768;  // points per level-1 workgroup
 static constexpr int kPUnroll = 8;          // digits loaded ahead per thread (latency hiding)
 static constexpr size_t kPartStageBytes = 16384;  // LDS for one partition's level-1 words
+static constexpr uint32_t kBigPart = 16384;       // larger partitions are sorted run by run
 // dynamic LDS of the staged level-1 placement: 3 counters per partition (at most 2^10
 // partitions) and one chunk of words
 static constexpr size_t kScatterStageLds = (3 * 1024 + kPChunk) * 4;
@@ -450,19 +451,27 @@ __global__ __launch_bounds__(kSortThreads) void msm_part_scatter_staged_kernel(
 // bits into vals and writes off[g 2^L + lo] (global bucket w B + p 2^L + lo); the last
 // workgroup also writes off[nparts 2^L] = the total.  A partition of at most `stage` words is
 // read from HBM once into LDS and its entries are ordered in LDS, then written with
-// consecutive stores; a larger one (skewed digits) is read twice and placed directly.
+// consecutive stores; a larger one is read twice and placed directly.  A partition of more
+// than kBigPart words (skewed digits: one bucket holding a large share of a window) is only
+// listed in big[1..] (big[0] = count) for the run-parallel kernels below.
 template <class T>
 __global__ __launch_bounds__(kThreads) void msm_part_sort_kernel(const T *__restrict__ tmp,
                                                                  const uint32_t *__restrict__ ppos, uint32_t nchunks,
                                                                  int L, int ib, uint32_t stage,
                                                                  uint32_t *__restrict__ off,
-                                                                 uint32_t *__restrict__ vals) {
+                                                                 uint32_t *__restrict__ vals,
+                                                                 uint32_t *__restrict__ big) {
   extern __shared__ uint32_t lc[];  // 2^L counters, then cursors; the staged words; staged entries
   __shared__ uint32_t sh[kThreads];
   const uint32_t g = blockIdx.x, NL = 1u << L, t = threadIdx.x;
   T *words = reinterpret_cast<T *>(lc + (NL < 2 ? 2 : NL));
   uint32_t *ents = reinterpret_cast<uint32_t *>(words + stage);
   const uint32_t start = ppos[(size_t)g * nchunks], end = ppos[(size_t)(g + 1) * nchunks];
+  if (g == gridDim.x - 1 && t == 0) off[(size_t)gridDim.x * NL] = end;
+  if (end - start > kBigPart) {  // sorted run by run by the big-partition kernels
+    if (t == 0) big[1 + atomicAdd(&big[0], 1u)] = g;
+    return;
+  }
   const bool staged = end - start <= stage;
   for (uint32_t j = t; j < NL; j += kThreads) lc[j] = 0;
   __syncthreads();
@@ -497,7 +506,6 @@ __global__ __launch_bounds__(kThreads) void msm_part_sort_kernel(const T *__rest
       run += cnt;
     }
   }
-  if (g == gridDim.x - 1 && t == 0) off[(size_t)gridDim.x * NL] = end;
   __syncthreads();
   for (uint32_t e = start + t; e < end; e += kThreads * 4) {
     T v[4];
@@ -521,6 +529,97 @@ __global__ __launch_bounds__(kThreads) void msm_part_sort_kernel(const T *__rest
   if (staged) {
     __syncthreads();
     for (uint32_t e = t; e < end - start; e += kThreads) vals[start + e] = ents[e];
+  }
+}
+
+// Big partitions, sorted by their level-1 runs (one per chunk) in parallel: count every run
+// by its low bucket bits, scan each partition over (low bits, run) into bucket offsets and
+// per-run bases, then place every run.  rcnt[(i nchunks + chunk) 2^L + lo] belongs to the
+// i-th listed partition.  Persistent grids: the item count is read on the device.
+template <class T>
+__global__ __launch_bounds__(kThreads) void msm_big_count_kernel(const T *__restrict__ tmp,
+                                                                 const uint32_t *__restrict__ ppos, uint32_t nchunks,
+                                                                 int L, int ib, const uint32_t *__restrict__ big,
+                                                                 uint32_t *__restrict__ rcnt) {
+  extern __shared__ uint32_t lc[];
+  const uint32_t NL = 1u << L, t = threadIdx.x;
+  const uint32_t items = big[0] * nchunks;
+  for (uint32_t it = blockIdx.x; it < items; it += gridDim.x) {
+    const uint32_t bi = it / nchunks, ch = it - bi * nchunks;
+    const size_t r = (size_t)big[1 + bi] * nchunks + ch;
+    const uint32_t s = ppos[r], e = ppos[r + 1];
+    for (uint32_t j = t; j < NL; j += kThreads) lc[j] = 0;
+    __syncthreads();
+    for (uint32_t k = s + t; k < e; k += kThreads * 4) {
+      T v[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) v[u] = k + u * kThreads < e ? tmp[k + u * kThreads] : T(0);
+#pragma unroll
+      for (int u = 0; u < 4; u++) lds_rank(lc, PartWord<T>::lo(v[u], ib), k + u * kThreads < e);
+    }
+    __syncthreads();
+    for (uint32_t j = t; j < NL; j += kThreads) rcnt[((size_t)bi * nchunks + ch) * NL + j] = lc[j];
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void msm_big_scan_kernel(const uint32_t *__restrict__ ppos, uint32_t nchunks,
+                                                                int L, const uint32_t *__restrict__ big,
+                                                                uint32_t *__restrict__ rcnt,
+                                                                uint32_t *__restrict__ off) {
+  __shared__ uint32_t sh[kThreads];
+  const uint32_t NL = 1u << L, t = threadIdx.x, nbig = big[0];
+  const uint32_t per = NL > kThreads ? NL / kThreads : 1u, j0 = t * per;
+  for (uint32_t bi = blockIdx.x; bi < nbig; bi += gridDim.x) {
+    const uint32_t g = big[1 + bi];
+    const uint32_t start = ppos[(size_t)g * nchunks];
+    uint32_t *rc = rcnt + (size_t)bi * nchunks * NL;
+    uint32_t s = 0;
+    for (uint32_t k = 0; k < per; k++)
+      if (j0 + k < NL)
+        for (uint32_t ch = 0; ch < nchunks; ch++) s += rc[(size_t)ch * NL + j0 + k];
+    uint32_t tot;
+    uint32_t run = start + block_exclusive_scan(s, sh, &tot);
+    for (uint32_t k = 0; k < per; k++) {
+      if (j0 + k < NL) {
+        off[(size_t)g * NL + j0 + k] = run;
+        for (uint32_t ch = 0; ch < nchunks; ch++) {
+          const uint32_t c = rc[(size_t)ch * NL + j0 + k];
+          rc[(size_t)ch * NL + j0 + k] = run;
+          run += c;
+        }
+      }
+    }
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kThreads) void msm_big_place_kernel(const T *__restrict__ tmp,
+                                                                 const uint32_t *__restrict__ ppos, uint32_t nchunks,
+                                                                 int L, int ib, const uint32_t *__restrict__ big,
+                                                                 const uint32_t *__restrict__ rcnt,
+                                                                 uint32_t *__restrict__ vals) {
+  extern __shared__ uint32_t lc[];
+  const uint32_t NL = 1u << L, t = threadIdx.x;
+  const uint32_t items = big[0] * nchunks;
+  for (uint32_t it = blockIdx.x; it < items; it += gridDim.x) {
+    const uint32_t bi = it / nchunks, ch = it - bi * nchunks;
+    const size_t r = (size_t)big[1 + bi] * nchunks + ch;
+    const uint32_t s = ppos[r], e = ppos[r + 1];
+    for (uint32_t j = t; j < NL; j += kThreads) lc[j] = rcnt[((size_t)bi * nchunks + ch) * NL + j];
+    __syncthreads();
+    for (uint32_t k = s + t; k < e; k += kThreads * 4) {
+      T v[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) v[u] = k + u * kThreads < e ? tmp[k + u * kThreads] : T(0);
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const bool active = k + u * kThreads < e;
+        const uint32_t pos = lds_rank(lc, PartWord<T>::lo(v[u], ib), active);
+        if (active) vals[pos] = PartWord<T>::entry(v[u], ib);
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -1285,7 +1384,7 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   const uint32_t nchunks = (uint32_t)(((uint64_t)n + kPChunk - 1) / kPChunk);
   const uint64_t nph = (uint64_t)W * P * nchunks;  // level-1 counts, scanned in place of positions
 
-  DevBuf dig, vals, tmp, phist, ppos, off, bsum, cont, heavy, pyrA, pyrB, scan_b, zpts, outx;
+  DevBuf dig, vals, tmp, phist, ppos, big, rcnt, off, bsum, cont, heavy, pyrA, pyrB, scan_b, zpts, outx;
   ZK_TRY(dig.alloc(total * 4));
   ZK_TRY(vals.alloc(total * 4));
   // level-1 words are 32-bit when the point index, the sign and the low bucket bits fit
@@ -1295,6 +1394,10 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   ZK_TRY(tmp.alloc(total * (narrow ? 4 : 8)));
   ZK_TRY(phist.alloc(nph * 4));
   ZK_TRY(ppos.alloc((nph + 1) * 4));
+  // big partitions hold more than kBigPart entries each, so there are at most total / kBigPart
+  const uint64_t nbig_cap = std::min<uint64_t>((uint64_t)W * P, total / kBigPart + 1);
+  ZK_TRY(big.alloc(((uint64_t)W * P + 1) * 4));
+  ZK_TRY(rcnt.alloc(nbig_cap * nchunks * ((uint64_t)1 << L) * 4));
   ZK_TRY(off.alloc((nkeys + 1) * 4));
   ZK_TRY(bsum.alloc(nkeys * ZW * 4));
   ZK_TRY(cont.alloc(nseg * ZW * 4));
@@ -1317,6 +1420,7 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   ev.mark(st);
   ZK_HIP(hipMemsetAsync(bsum.p, 0, nkeys * ZW * 4, st));
   ZK_HIP(hipMemsetAsync(heavy.p, 0, 8, st));
+  ZK_HIP(hipMemsetAsync(big.p, 0, 4, st));
   const dim3 blk(kThreads);
   const dim3 gpts((n + kThreads - 1) / kThreads);
   const int32_t *zp = points_internal ? (const int32_t *)d_points : zpts.as<int32_t>();
@@ -1333,21 +1437,30 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   hipLaunchKernelGGL(scan_final_kernel, dim3(nsb), blk, 0, st, phist.as<uint32_t>(), (size_t)nph,
                      scan_b.as<uint32_t>(), ppos.as<uint32_t>(), (uint32_t *)nullptr);
   const size_t cnt_bytes = ((size_t)1 << std::max(L, 1)) * 4;
+  // level 2: partitions of at most kBigPart words, one workgroup each; larger ones run by run
+  auto level2 = [&](auto word) {
+    using T = decltype(word);
+    const uint32_t stage = (uint32_t)(kPartStageBytes / sizeof(T));
+    hipLaunchKernelGGL(msm_part_sort_kernel<T>, dim3((unsigned)(W * P)), blk,
+                       cnt_bytes + kPartStageBytes + (size_t)stage * 4, st, tmp.as<T>(), ppos.as<uint32_t>(), nchunks,
+                       L, ib, stage, off.as<uint32_t>(), vals.as<uint32_t>(), big.as<uint32_t>());
+    const unsigned gb = (unsigned)std::min<uint64_t>((uint64_t)nbig_cap * nchunks, 1024);
+    hipLaunchKernelGGL(msm_big_count_kernel<T>, dim3(gb), blk, cnt_bytes, st, tmp.as<T>(), ppos.as<uint32_t>(),
+                       nchunks, L, ib, big.as<uint32_t>(), rcnt.as<uint32_t>());
+    hipLaunchKernelGGL(msm_big_scan_kernel, dim3((unsigned)std::min<uint64_t>(nbig_cap, 256)), blk, 0, st,
+                       ppos.as<uint32_t>(), nchunks, L, big.as<uint32_t>(), rcnt.as<uint32_t>(), off.as<uint32_t>());
+    hipLaunchKernelGGL(msm_big_place_kernel<T>, dim3(gb), blk, cnt_bytes, st, tmp.as<T>(), ppos.as<uint32_t>(),
+                       nchunks, L, ib, big.as<uint32_t>(), rcnt.as<uint32_t>(), vals.as<uint32_t>());
+  };
   if (narrow) {
-    const uint32_t stage = kPartStageBytes / 4;
     hipLaunchKernelGGL(msm_part_scatter_staged_kernel, gch, dim3(kSortThreads), kScatterStageLds, st,
                        dig.as<uint32_t>(), (uint32_t)n, L, P, ib, phist.as<uint32_t>(), ppos.as<uint32_t>(),
                        tmp.as<uint32_t>());
-    hipLaunchKernelGGL(msm_part_sort_kernel<uint32_t>, dim3((unsigned)(W * P)), blk,
-                       cnt_bytes + kPartStageBytes + (size_t)stage * 4, st, tmp.as<uint32_t>(), ppos.as<uint32_t>(),
-                       nchunks, L, ib, stage, off.as<uint32_t>(), vals.as<uint32_t>());
+    level2(uint32_t(0));
   } else {
-    const uint32_t stage = kPartStageBytes / 8;
     hipLaunchKernelGGL(msm_part_scatter_kernel<uint64_t>, gch, dim3(kSortThreads), (size_t)P * 4, st,
                        dig.as<uint32_t>(), (uint32_t)n, L, P, ib, ppos.as<uint32_t>(), tmp.as<uint64_t>());
-    hipLaunchKernelGGL(msm_part_sort_kernel<uint64_t>, dim3((unsigned)(W * P)), blk,
-                       cnt_bytes + kPartStageBytes + (size_t)stage * 4, st, tmp.as<uint64_t>(), ppos.as<uint32_t>(),
-                       nchunks, L, ib, stage, off.as<uint32_t>(), vals.as<uint32_t>());
+    level2(uint64_t(0));
   }
   ZK_HIP(hipGetLastError());
   const uint32_t limit = (user && user->fix_limit > 0) ? (uint32_t)user->fix_limit : 16u;
@@ -1628,10 +1741,14 @@ size_t msm_footprint(int curve, int n, int enl, bool points_internal, const zkg_
   const uint64_t zw = curve == 0 ? ZLayout<Bn254G1>::XYZZ : ZLayout<Bls381G1>::XYZZ;
   const uint64_t aw = curve == 0 ? ZLayout<Bn254G1>::AFF : ZLayout<Bls381G1>::AFF;
   const uint64_t nchunks = ((uint64_t)n + kPChunk - 1) / kPChunk;
-  const uint64_t nph = W * ((uint64_t)1 << sort_partition_bits(n, s.c - 1)) * nchunks;
+  const int logP = sort_partition_bits(n, s.c - 1), L = s.c - 1 - logP;
+  const uint64_t nph = W * ((uint64_t)1 << logP) * nchunks;
+  const uint64_t nbig = std::min<uint64_t>(W << logP, total / kBigPart + 1);
   // digits, entries and level-1 words; offsets and heavy list; bucket sums and two pyramid
-  // buffers; continuation partials; level-1 counts and positions
-  uint64_t b = total * 16 + nk * 8 + 3 * nk * zw * 4 + ((total >> s.seg_log) + 1) * zw * 4 + nph * 8;
+  // buffers; continuation partials; level-1 counts and positions, big-partition list and run
+  // counts
+  uint64_t b = total * 16 + nk * 8 + 3 * nk * zw * 4 + ((total >> s.seg_log) + 1) * zw * 4 + nph * 8 +
+               (W << logP) * 4 + nbig * nchunks * ((uint64_t)1 << L) * 4;
   if (!points_internal) b += (uint64_t)n * aw * 4;
   return (size_t)b;
 }
