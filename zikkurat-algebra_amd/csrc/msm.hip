@@ -869,8 +869,14 @@ __global__ __launch_bounds__(kThreads) void msm_heavy_final_kernel(const uint32_
 // ---------------------------------------------------------------------------------------
 // 7. bucket pyramid: input [A][W][M] -> output [A+1][W][M/2]
 
+// waves per SIMD the pyramid kernels are compiled for.  Left free, the compiler gives a
+// BLS12-381 addition 256 VGPRs plus AGPRs (one wave per SIMD); held to two waves it spills a
+// little and the pyramid is 15% faster (0.57 against 0.67 ms at 2^20).  BN254 fits two waves
+// without a bound, and forcing three (168 VGPRs, spills) measured slower.
+template <class C> struct PyrWaves { static constexpr int v = C::id == 0 ? 1 : 2; };
+
 template <class C>
-__global__ __launch_bounds__(kThreads) void msm_pyramid_kernel(const int32_t *__restrict__ in,
+__global__ __launch_bounds__(kThreads, PyrWaves<C>::v) void msm_pyramid_kernel(const int32_t *__restrict__ in,
                                                               int32_t *__restrict__ out, uint32_t W, uint32_t M,
                                                               uint32_t A) {
   constexpr int PW = ZLayout<C>::XYZZ;
@@ -881,12 +887,12 @@ __global__ __launch_bounds__(kThreads) void msm_pyramid_kernel(const int32_t *__
   const uint32_t a = (uint32_t)(t / per);
   const uint64_t rem = t % per;  // = w * half + i
   const uint64_t src = (uint64_t)a * W * M + 2 * rem;
-  XyzzZ<C> p0, p1, s;
+  XyzzZ<C> p0, p1;
   zxyzz_load(p0, in + src * PW);
   zxyzz_load(p1, in + (src + 1) * PW);
-  zxyzz_add(s, p0, p1);
-  zxyzz_store(out + ((uint64_t)a * per + rem) * PW, s);
-  if (a == 0) zxyzz_store(out + ((uint64_t)A * per + rem) * PW, p1);
+  if (a == 0) zxyzz_store(out + ((uint64_t)A * per + rem) * PW, p1);  // before the sum: p1 dies in it
+  zxyzz_add(p0, p0, p1);
+  zxyzz_store(out + ((uint64_t)a * per + rem) * PW, p0);
 }
 
 // two pyramid levels in one pass: input [A][W][M] -> output [A+2][W][M/4] (M >= 4).  For
@@ -894,7 +900,7 @@ __global__ __launch_bounds__(kThreads) void msm_pyramid_kernel(const int32_t *__
 // children of the first level reduced once more (B1 + B3), array A + 1 the odd children of the
 // second level (B2 + B3).  Half the launches and half the traffic of two single levels.
 template <class C>
-__global__ __launch_bounds__(kThreads) void msm_pyramid2_kernel(const int32_t *__restrict__ in,
+__global__ __launch_bounds__(kThreads, PyrWaves<C>::v) void msm_pyramid2_kernel(const int32_t *__restrict__ in,
                                                                int32_t *__restrict__ out, uint32_t W, uint32_t M,
                                                                uint32_t A) {
   constexpr int PW = ZLayout<C>::XYZZ;
