@@ -170,3 +170,23 @@ def test_msm_schedules_agree(zk, C):
         for seg in (1, 3, 5, 8):
             res = zk.msm_device(C.id, n, dS, dP, window=c, seg_log=seg)
             assert C.words_to_affine(res) == exp, (c, seg)
+
+
+@pytest.mark.parametrize("C", CURVES, ids=IDS)
+def test_msm_wide_sort_words(zk, C):
+    """2^22 + 3 points at a 20-bit window: the point index (23 bits), the sign and the low
+    bucket bits no longer fit one 32-bit word, so the sort's first level writes 64-bit words.
+    The result must equal the same MSM at c = 16 (32-bit words) bit for bit, and a skewed
+    scalar set (every third scalar 1) must too."""
+    n = (1 << 22) + 3
+    dK = zk.fill_random(C.id, 1, n, 71)
+    dP = zk.scalar_mul_gen(C.id, dK, n)
+    dS = zk.fill_random(C.id, 1, n, 72)
+    narrow = zk.msm_device(C.id, n, dS, dP, window=16)
+    wide = zk.msm_device(C.id, n, dS, dP, window=20)
+    assert (wide == narrow).all()
+    s = dS.to_host((n, 4))
+    s[::3] = 0
+    s[::3, 0] = 1
+    dS2 = zk.DeviceArray.from_host(s)
+    assert (zk.msm_device(C.id, n, dS2, dP, window=20) == zk.msm_device(C.id, n, dS2, dP, window=16)).all()
