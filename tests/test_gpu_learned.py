@@ -257,9 +257,6 @@ def test_cost_model_gain_over_heuristics(zk):
     rep = costmodel.evaluate(held, net)
     summary = {k: v for k, v in rep.items() if k != "rows"}
     print(json.dumps(summary))
-    out = REPO / "gpurun_out"
-    if out.is_dir():
-        (out / "cost_model_gain_test.json").write_text(json.dumps(summary, indent=1))
     # the network's picks beat the plain path and stay near the exhaustive optimum
     assert rep["mlp"]["total_ms"] < rep["plain"]["total_ms"]
     assert rep["mlp"]["geomean_vs_best"] < 1.3

@@ -1,4 +1,4 @@
-// Microbenchmarks that measure the rooflines the hot paths are judged against, and
+// Microbenchmarks that measure the rooflines the hot paths are measured against, and
 // elementwise field kernels used to check the device field layer against the oracle.
 //
 //   imul   : v_mad_u64_u32 throughput (32x32->64 multiply-add), the MSM/NTT bound
