@@ -68,9 +68,69 @@ __device__ __forceinline__ uint32_t encode_digit(uint32_t raw, int c, uint32_t &
   return raw ? ((raw - 1) << 1) : kZeroDigit;
 }
 
+// GLV split of BN254 scalars (Gallant, Lambert, Vanstone, CRYPTO 2001).  BN254's G1 has prime
+// order r and the endomorphism phi(x, y) = (beta x, y) = lambda P, with beta and lambda
+// primitive cube roots of unity in Fp and Fr.  A scalar k < r is split as
+// k = k1 + lambda k2 (mod r): c1 = floor(k G1 / 2^256), c2 = floor(k G2 / 2^256),
+// k1 = k - c1 a1 - c2 a2, k2 = c1 |b1| - c2 b2, where (a1, b1), (a2, b2) is a short basis of
+// {(x, y) : x + lambda y = 0 mod r} (extended Euclid on r and lambda; det = r) and
+// G1 = round(2^256 b2 / r), G2 = round(-2^256 b1 / r).  The floor errors lie in
+// (-1.125, 0.125), so |k1|, |k2| < 1.125 a1 < 2^127: both fit 128-bit two's complement, and
+// signed-digit windows over 128 bits cover them.
+struct Bn254Glv {
+  static constexpr uint64_t G1[3] = {0x5398fd0300ff6565ull, 0x4ccef014a773d2d2ull, 0x2ull};
+  static constexpr uint64_t G2[2] = {0xd91d232ec7e0b3d7ull, 0x2ull};
+  static constexpr uint64_t A1[2] = {0x8211bbeb7d4f1128ull, 0x6f4d8248eeb859fcull};
+  static constexpr uint64_t A2 = 0x89d3256894d213e3ull;  // = |b1|
+  static constexpr uint64_t B2[2] = {0x0be4e1541221250bull, 0x6f4d8248eeb859fdull};
+  // beta R' mod p (R' = 2^261), the internal form of beta: zmul(x R', beta R') = beta x R'
+  static constexpr int32_t BETA[9] = {416069521, 391846970, 193189602, 244240497, 310558434,
+                                      69340804,  535036287, 352865610, 1478937};
+};
+
+// (k G) >> 256 for k < 2^256 and a constant G of NG 64-bit limbs, low 128 bits of the quotient
+template <int NG>
+__device__ __forceinline__ unsigned __int128 mul_shift256(const uint64_t k[4], const uint64_t (&g)[NG]) {
+  using u128 = unsigned __int128;
+  uint64_t prod[4 + NG] = {};
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    uint64_t carry = 0;
+#pragma unroll
+    for (int j = 0; j < NG; j++) {
+      const u128 t = (u128)k[i] * g[j] + prod[i + j] + carry;
+      prod[i + j] = (uint64_t)t;
+      carry = (uint64_t)(t >> 64);
+    }
+    prod[i + NG] = carry;
+  }
+  return (u128)prod[4] | ((u128)prod[5] << 64);
+}
+
+// k < r (4 limbs) -> |k1|, |k2| as two 64-bit limbs each, with their signs
+__device__ __forceinline__ void glv_split(const uint64_t k[4], uint64_t m1[2], bool &neg1, uint64_t m2[2],
+                                          bool &neg2) {
+  using G = Bn254Glv;
+  using u128 = unsigned __int128;
+  const u128 c1 = mul_shift256<3>(k, G::G1);
+  const u128 c2 = mul_shift256<2>(k, G::G2);  // < 2^64
+  const u128 a1 = (u128)G::A1[0] | ((u128)G::A1[1] << 64);
+  const u128 b2 = (u128)G::B2[0] | ((u128)G::B2[1] << 64);
+  u128 k1 = ((u128)k[0] | ((u128)k[1] << 64)) - c1 * a1 - c2 * (u128)G::A2;  // mod 2^128
+  u128 k2 = c1 * (u128)G::A2 - c2 * b2;
+  neg1 = (uint64_t)(k1 >> 127) != 0;
+  neg2 = (uint64_t)(k2 >> 127) != 0;
+  if (neg1) k1 = (u128)0 - k1;
+  if (neg2) k2 = (u128)0 - k2;
+  m1[0] = (uint64_t)k1;
+  m1[1] = (uint64_t)(k1 >> 64);
+  m2[0] = (uint64_t)k2;
+  m2[1] = (uint64_t)(k2 >> 64);
+}
+
 template <class C>
 __device__ __forceinline__ void point_to_internal(const uint32_t *__restrict__ pts, uint32_t i,
-                                                  int32_t *__restrict__ out) {
+                                                  int32_t *__restrict__ out, int32_t *__restrict__ phi_out) {
   using F = typename C::Fp;
   uint32_t w[2 * F::N];
   const uint4 *q = reinterpret_cast<const uint4 *>(pts + (size_t)i * 2 * F::N);
@@ -86,20 +146,31 @@ __device__ __forceinline__ void point_to_internal(const uint32_t *__restrict__ p
   zfrom_ext(a.x, w);
   zfrom_ext(a.y, w + F::N);
   zaff_store(out + (size_t)i * ZLayout<C>::AFF, a);
+  if constexpr (C::id == 0) {
+    if (phi_out) {  // phi(P) = (beta x, y); the affine infinity (0, 0) maps to itself
+      Fz<ZBn254Fp> beta;
+      zset_const(beta, Bn254Glv::BETA);
+      zmul(a.x, a.x, beta);
+      zaff_store(phi_out + (size_t)i * ZLayout<C>::AFF, a);
+    }
+  }
 }
 
-// signed digits of scalar i for every window; when ext_pts is given, point i is converted to
-// the internal record in the same pass (both sweeps are bound by memory traffic)
+// signed digits of scalar i for every window (row w at dig + w * ns); when ext_pts is given,
+// point i is converted to the internal record in the same pass (both sweeps are bound by
+// memory traffic).  With glv (BN254, 4-limb scalars reduced mod r) scalar i becomes k1 for
+// point i and k2 for point n + i = phi(P_i), whose record is written too (ns = 2n).
 template <class C>
 __global__ __launch_bounds__(kThreads) void msm_digits_kernel(const uint64_t *__restrict__ sc, int enl, int mont,
-                                                             int reduce, uint32_t n, int c, int W, int wide,
-                                                             uint32_t *__restrict__ dig,
+                                                             int reduce, int glv, uint32_t n, uint32_t ns, int c,
+                                                             int W, int wide, uint32_t *__restrict__ dig,
                                                              const uint32_t *__restrict__ ext_pts,
                                                              int32_t *__restrict__ zpts) {
   using Fr = typename C::Fr;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  if (ext_pts) point_to_internal<C>(ext_pts, i, zpts);
+  if (ext_pts)
+    point_to_internal<C>(ext_pts, i, zpts, glv ? zpts + (size_t)n * ZLayout<C>::AFF : (int32_t *)nullptr);
   uint32_t carry = 0;
   if (enl == 4) {
     uint64_t s[4];
@@ -136,11 +207,29 @@ __global__ __launch_bounds__(kThreads) void msm_digits_kernel(const uint64_t *__
         }
       }
     }
+    if (C::id == 0 && glv) {
+      uint64_t m1[2], m2[2];
+      bool n1, n2;
+      glv_split(s, m1, n1, m2, n2);
+      uint32_t carry2 = 0;
+      int bit = 0;
+      for (int w = 0; w < W; w++) {
+        const int cw = w < wide ? c : c - 1;
+        uint32_t d1 = encode_digit(window_bits(m1, 2, bit, cw) + carry, cw, carry);
+        uint32_t d2 = encode_digit(window_bits(m2, 2, bit, cw) + carry2, cw, carry2);
+        if (n1 && d1 != kZeroDigit) d1 ^= 1u;  // digits of -|k1|: flip the sign bit
+        if (n2 && d2 != kZeroDigit) d2 ^= 1u;
+        dig[(size_t)w * ns + i] = d1;
+        dig[(size_t)w * ns + n + i] = d2;
+        bit += cw;
+      }
+      return;
+    }
     int bit = 0;
     for (int w = 0; w < W; w++) {
       const int cw = w < wide ? c : c - 1;
       const uint32_t raw = window_bits(s, 4, bit, cw) + carry;
-      dig[(size_t)w * n + i] = encode_digit(raw, cw, carry);
+      dig[(size_t)w * ns + i] = encode_digit(raw, cw, carry);
       bit += cw;
     }
   } else {
@@ -149,7 +238,7 @@ __global__ __launch_bounds__(kThreads) void msm_digits_kernel(const uint64_t *__
     for (int w = 0; w < W; w++) {
       const int cw = w < wide ? c : c - 1;
       const uint32_t raw = window_bits(s, enl, bit, cw) + carry;
-      dig[(size_t)w * n + i] = encode_digit(raw, cw, carry);
+      dig[(size_t)w * ns + i] = encode_digit(raw, cw, carry);
       bit += cw;
     }
   }
@@ -640,7 +729,7 @@ __global__ __launch_bounds__(kThreads) void msm_points_in_kernel(const uint32_t 
                                                                 int32_t *__restrict__ out) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  point_to_internal<C>(pts, i, out);
+  point_to_internal<C>(pts, i, out, nullptr);
 }
 
 // final window terms: internal XYZZ -> external canonical words (4 field elements each)
@@ -1336,6 +1425,21 @@ void msm_window_layout(int bits, int c_req, bool balanced, int *W, int *c, int *
   *wide = total - w * (cm - 1);
 }
 
+// The GLV split applies to BN254 under the plain schedule (4-limb scalars, reduced mod r) when
+// the points arrive as affine words, so that phi(P_i) = (beta x_i, y_i) can be written next to
+// P_i in the conversion pass (a prepared point set holds P_i only).  The problem becomes 2n
+// points with scalars of at most 127 bits: windows over 128 bits, the window width chosen for
+// 2n points (or the caller's).  Entries stay ~16 n at c = 16 while the buckets and the host's
+// doubling chain halve.
+static void glv_schedule(int curve, int n, const zkg_msm_schedule *user, bool points_internal, MsmSchedule *s) {
+  if (curve != 0 || !s->reduce || points_internal) return;
+  int c = (user && user->window > 0) ? user->window : msm_heuristic_window(curve, 2 * n, 127);
+  c = c < 1 ? 1 : (c > 20 ? 20 : c);
+  s->bits = 127;
+  s->glv = 1;
+  msm_window_layout(127, c, false, &s->windows, &s->c, &s->wide);
+}
+
 static MsmSchedule resolve_schedule(int curve, int n, int enl, const zkg_msm_schedule *user) {
   MsmSchedule s;
   msm_resolve(curve, n, enl, user, nullptr, &s);
@@ -1373,21 +1477,24 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
     return msm_sparse_run<C>(n, d_scalars, enl, mont, d_points, points_internal, user, st, stats, out);
   MsmSchedule sch;
   ZK_TRY(msm_resolve(C::id, n, enl, user, use_stats ? &stats : nullptr, &sch));
+  glv_schedule(C::id, n, user, points_internal, &sch);
+  const bool glv = sch.glv != 0;
+  const uint32_t np = glv ? 2u * (uint32_t)n : (uint32_t)n;  // points of the bucket problem
   const int c = sch.c, W = sch.windows, logB = c - 1;
   const uint32_t B = 1u << logB;
   const uint64_t nkeys = (uint64_t)W * B;
-  if (nkeys >= (1ull << 32) || (uint64_t)W * n >= (1ull << 32)) {
+  if (nkeys >= (1ull << 32) || (uint64_t)W * np >= (1ull << 32) || np >= (1u << 31)) {
     set_error("msm: problem too large for 32-bit sort keys");
     return ZK_EINVAL;
   }
-  const uint64_t total = (uint64_t)W * n;
+  const uint64_t total = (uint64_t)W * np;
   const int seg_log = sch.seg_log;
   const uint64_t nseg = (total >> seg_log) + 1;
 
-  const int logP = sort_partition_bits(n, logB);
+  const int logP = sort_partition_bits((int)np, logB);
   const int L = logB - logP;
   const uint32_t P = 1u << logP;
-  const uint32_t nchunks = (uint32_t)(((uint64_t)n + kPChunk - 1) / kPChunk);
+  const uint32_t nchunks = (uint32_t)(((uint64_t)np + kPChunk - 1) / kPChunk);
   const uint64_t nph = (uint64_t)W * P * nchunks;  // level-1 counts, scanned in place of positions
 
   DevBuf dig, vals, tmp, phist, ppos, big, rcnt, off, bsum, cont, heavy, pyrA, pyrB, scan_b, zpts, outx;
@@ -1395,7 +1502,7 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   ZK_TRY(vals.alloc(total * 4));
   // level-1 words are 32-bit when the point index, the sign and the low bucket bits fit
   int ib = 0;
-  while ((1ull << ib) < (uint64_t)n) ib++;
+  while ((1ull << ib) < (uint64_t)np) ib++;
   const bool narrow = ib < 31 && ib + 1 + L <= 32;
   ZK_TRY(tmp.alloc(total * (narrow ? 4 : 8)));
   ZK_TRY(phist.alloc(nph * 4));
@@ -1407,7 +1514,7 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   ZK_TRY(off.alloc((nkeys + 1) * 4));
   ZK_TRY(bsum.alloc(nkeys * ZW * 4));
   ZK_TRY(cont.alloc(nseg * ZW * 4));
-  if (!points_internal) ZK_TRY(zpts.alloc((uint64_t)n * AW * 4));
+  if (!points_internal) ZK_TRY(zpts.alloc((uint64_t)np * AW * 4));
   // every bucket may be heavy (small seg_log, few entries per segment)
   const uint32_t heavy_cap = (uint32_t)nkeys;
   ZK_TRY(heavy.alloc(((uint64_t)heavy_cap + 2) * 4));
@@ -1430,13 +1537,13 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   const dim3 blk(kThreads);
   const dim3 gpts((n + kThreads - 1) / kThreads);
   const int32_t *zp = points_internal ? (const int32_t *)d_points : zpts.as<int32_t>();
-  hipLaunchKernelGGL(msm_digits_kernel<C>, gpts, blk, 0, st, d_scalars, enl, mont, sch.reduce, (uint32_t)n, c, W,
-                     sch.wide,
+  hipLaunchKernelGGL(msm_digits_kernel<C>, gpts, blk, 0, st, d_scalars, enl, mont, sch.reduce, (int)glv, (uint32_t)n,
+                     np, c, W, sch.wide,
                      dig.as<uint32_t>(), points_internal ? (const uint32_t *)nullptr : (const uint32_t *)d_points,
                      zpts.as<int32_t>());
   const dim3 gch(nchunks, W);
   hipLaunchKernelGGL(msm_part_hist_kernel, gch, dim3(kSortThreads), (size_t)P * 4, st, dig.as<uint32_t>(),
-                     (uint32_t)n, L, P, phist.as<uint32_t>());
+                     np, L, P, phist.as<uint32_t>());
   hipLaunchKernelGGL(scan_block_sums_kernel, dim3(nsb), blk, 0, st, phist.as<uint32_t>(), (size_t)nph,
                      scan_b.as<uint32_t>());
   hipLaunchKernelGGL(scan_top_kernel, dim3(1), blk, 0, st, scan_b.as<uint32_t>(), nsb);
@@ -1460,12 +1567,12 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   };
   if (narrow) {
     hipLaunchKernelGGL(msm_part_scatter_staged_kernel, gch, dim3(kSortThreads), kScatterStageLds, st,
-                       dig.as<uint32_t>(), (uint32_t)n, L, P, ib, phist.as<uint32_t>(), ppos.as<uint32_t>(),
+                       dig.as<uint32_t>(), np, L, P, ib, phist.as<uint32_t>(), ppos.as<uint32_t>(),
                        tmp.as<uint32_t>());
     level2(uint32_t(0));
   } else {
     hipLaunchKernelGGL(msm_part_scatter_kernel<uint64_t>, gch, dim3(kSortThreads), (size_t)P * 4, st,
-                       dig.as<uint32_t>(), (uint32_t)n, L, P, ib, ppos.as<uint32_t>(), tmp.as<uint64_t>());
+                       dig.as<uint32_t>(), np, L, P, ib, ppos.as<uint32_t>(), tmp.as<uint64_t>());
     level2(uint64_t(0));
   }
   ZK_HIP(hipGetLastError());
@@ -1742,12 +1849,14 @@ int msm_prepare_points(int curve, int n, const uint64_t *d_ext, void *d_int, hip
 
 // device bytes one MSM of n points allocates (admission control of the serving loop)
 size_t msm_footprint(int curve, int n, int enl, bool points_internal, const zkg_msm_schedule *sched) {
-  const MsmSchedule s = resolve_schedule(curve, n, enl, sched);
-  const uint64_t W = s.windows, B = 1ull << (s.c - 1), nk = W * B, total = W * (uint64_t)n;
+  MsmSchedule s = resolve_schedule(curve, n, enl, sched);
+  glv_schedule(curve, n, sched, points_internal, &s);
+  const uint64_t np = s.glv ? 2 * (uint64_t)n : (uint64_t)n;
+  const uint64_t W = s.windows, B = 1ull << (s.c - 1), nk = W * B, total = W * np;
   const uint64_t zw = curve == 0 ? ZLayout<Bn254G1>::XYZZ : ZLayout<Bls381G1>::XYZZ;
   const uint64_t aw = curve == 0 ? ZLayout<Bn254G1>::AFF : ZLayout<Bls381G1>::AFF;
-  const uint64_t nchunks = ((uint64_t)n + kPChunk - 1) / kPChunk;
-  const int logP = sort_partition_bits(n, s.c - 1), L = s.c - 1 - logP;
+  const uint64_t nchunks = (np + kPChunk - 1) / kPChunk;
+  const int logP = sort_partition_bits((int)np, s.c - 1), L = s.c - 1 - logP;
   const uint64_t nph = W * ((uint64_t)1 << logP) * nchunks;
   const uint64_t nbig = std::min<uint64_t>(W << logP, total / kBigPart + 1);
   // digits, entries and level-1 words; offsets and heavy list; bucket sums and two pyramid
@@ -1755,7 +1864,7 @@ size_t msm_footprint(int curve, int n, int enl, bool points_internal, const zkg_
   // counts
   uint64_t b = total * 16 + nk * 8 + 3 * nk * zw * 4 + ((total >> s.seg_log) + 1) * zw * 4 + nph * 8 +
                (W << logP) * 4 + nbig * nchunks * ((uint64_t)1 << L) * 4;
-  if (!points_internal) b += (uint64_t)n * aw * 4;
+  if (!points_internal) b += np * aw * 4;
   return (size_t)b;
 }
 

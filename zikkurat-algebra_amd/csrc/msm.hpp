@@ -26,6 +26,7 @@ struct MsmSchedule {
   int seg_log = 5;    // log2 sorted entries per accumulation thread
   int bits = 0;       // scalar bits the windows cover
   int reduce = 0;     // scalars are reduced mod r before the digits (BN254: r P = 0 on the curve)
+  int glv = 0;        // BN254 GLV split: 2n points (P_i, phi(P_i)) with scalars below 2^127
   int start(int w) const { return w < wide ? w * c : wide * c + (w - wide) * (c - 1); }
 };
 // the window layout for a requested width (balanced or uniform, see above)
