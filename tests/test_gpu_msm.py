@@ -190,3 +190,41 @@ def test_msm_wide_sort_words(zk, C):
     s[::3, 0] = 1
     dS2 = zk.DeviceArray.from_host(s)
     assert (zk.msm_device(C.id, n, dS2, dP, window=20) == zk.msm_device(C.id, n, dS2, dP, window=16)).all()
+
+
+def test_msm_bn254_glv_extremes(zk):
+    """BN254 under the plain schedule splits k = k1 + lambda k2 (mod r) with |k1|, |k2| < 2^127
+    and sums k1 P + k2 phi(P), phi(x, y) = (beta x, y).  Scalars chosen at the split's extremes
+    (largest |k1| and |k2| of a random search, each sign, k = lambda and r - lambda where one
+    half is zero, r - 1) must give the oracle's integer MSM.  The decomposition below only picks
+    inputs; the check is the oracle and the closed form."""
+    C = BN
+    lam = 0x30644E72E131A029048B6E193FD84104CC37A73FEC2BC5E9B8CA0B2D36636F23
+    assert pow(lam, 3, C.r) == 1 and lam != 1
+    a1, b1 = 147946756881789319000765030803803410728, -9931322734385697763
+    a2, b2 = 9931322734385697763, 147946756881789319010696353538189108491
+    g1 = ((b2 << 256) + C.r // 2) // C.r
+    g2 = ((-b1 << 256) + C.r // 2) // C.r
+
+    def split(k):
+        c1, c2 = (k * g1) >> 256, (k * g2) >> 256
+        return k - c1 * a1 - c2 * a2, -c1 * b1 - c2 * b2
+
+    rng = np.random.default_rng(9)
+    pool = [int.from_bytes(rng.bytes(32), "little") % C.r for _ in range(20000)]
+    sp = [(k,) + split(k) for k in pool]
+    pick = set()
+    for idx, sign in ((1, 1), (1, -1), (2, 1), (2, -1)):
+        pick.add(max(sp, key=lambda t: sign * t[idx])[0])
+    ss = sorted(pick) + [lam, C.r - lam, C.r - 1, 1, 0, (C.r - 1) // 2]
+    for k in ss:
+        k1, k2 = split(k)
+        assert (k1 + lam * k2 - k) % C.r == 0 and abs(k1) < 2**127 and abs(k2) < 2**127
+    n = 3000
+    ss = (ss * (n // len(ss) + 1))[:n]
+    ks = random_scalars(4242, n, C.r)
+    P = make_points(C, ks)
+    S = ints_to_array(ss, 4)
+    exp = Oracle(C).msm_affine(S, P)
+    assert (zk.msm_std(C.id, S, P) == exp).all()
+    assert C.words_to_affine(exp) == C.mul(sum(s * k for s, k in zip(ss, ks)) % C.r, C.g)
