@@ -1381,8 +1381,13 @@ int msm_scalar_stats(int curve, int n, const uint64_t *d_scalars, int enl, int m
 // Schedule of one MSM.  Without statistics the windows cover the declared scalar width
 // (64 * enl bits, plus one for the signed-digit carry); with them they cover max_bits + 1.
 int msm_resolve(int curve, int n, int enl, const zkg_msm_schedule *user, const zkg_scalar_stats *st,
-                MsmSchedule *out) {
+                MsmSchedule *out, bool glv_ok) {
   MsmSchedule s;
+  // GLV split (BN254, 4-limb scalars reduced mod r, points given as affine words): 2n points
+  // P_i and phi(P_i) with scalars below 2^127, so windows over 128 bits; entries stay ~16 n at
+  // c = 16 while the buckets and the host's doubling chain halve.  Measured scalars of at most
+  // 128 bits are already short and keep the plain layout.
+  const bool glv = glv_ok && curve == 0 && enl == 4 && (!st || st->max_bits > 128);
   const int fr_bits = curve == 0 ? 254 : 255;
   const int bits_for_cost = enl * 64 < fr_bits ? enl * 64 : fr_bits;
   const int policy = user ? user->policy : ZKG_SCHED_PLAIN;
@@ -1395,7 +1400,10 @@ int msm_resolve(int curve, int n, int enl, const zkg_msm_schedule *user, const z
       if (seg == 0) seg = ms;
     }
   }
-  if (c == 0) c = st ? msm_stats_window(curve, *st) : msm_heuristic_window(curve, n, bits_for_cost);
+  if (c == 0) {
+    if (glv) c = msm_heuristic_window(curve, 2ll * n, 127);
+    else c = st ? msm_stats_window(curve, *st) : msm_heuristic_window(curve, n, bits_for_cost);
+  }
   if (seg == 0) seg = 5;
   c = c < 1 ? 1 : (c > 20 ? 20 : c);
   s.seg_log = seg < 1 ? 1 : (seg > 12 ? 12 : seg);
@@ -1405,6 +1413,11 @@ int msm_resolve(int curve, int n, int enl, const zkg_msm_schedule *user, const z
   if (!st && curve == 0 && enl == 4) {
     s.bits = 254;
     s.reduce = 1;
+  }
+  if (glv) {
+    s.bits = 127;
+    s.reduce = 1;
+    s.glv = 1;
   }
   msm_window_layout(s.bits, c, st != nullptr, &s.windows, &s.c, &s.wide);
   *out = s;
@@ -1425,24 +1438,9 @@ void msm_window_layout(int bits, int c_req, bool balanced, int *W, int *c, int *
   *wide = total - w * (cm - 1);
 }
 
-// The GLV split applies to BN254 under the plain schedule (4-limb scalars, reduced mod r) when
-// the points arrive as affine words, so that phi(P_i) = (beta x_i, y_i) can be written next to
-// P_i in the conversion pass (a prepared point set holds P_i only).  The problem becomes 2n
-// points with scalars of at most 127 bits: windows over 128 bits, the window width chosen for
-// 2n points (or the caller's).  Entries stay ~16 n at c = 16 while the buckets and the host's
-// doubling chain halve.
-static void glv_schedule(int curve, int n, const zkg_msm_schedule *user, bool points_internal, MsmSchedule *s) {
-  if (curve != 0 || !s->reduce || points_internal) return;
-  int c = (user && user->window > 0) ? user->window : msm_heuristic_window(curve, 2 * n, 127);
-  c = c < 1 ? 1 : (c > 20 ? 20 : c);
-  s->bits = 127;
-  s->glv = 1;
-  msm_window_layout(127, c, false, &s->windows, &s->c, &s->wide);
-}
-
-static MsmSchedule resolve_schedule(int curve, int n, int enl, const zkg_msm_schedule *user) {
+static MsmSchedule resolve_schedule(int curve, int n, int enl, const zkg_msm_schedule *user, bool glv_ok) {
   MsmSchedule s;
-  msm_resolve(curve, n, enl, user, nullptr, &s);
+  msm_resolve(curve, n, enl, user, nullptr, &s, glv_ok);
   return s;
 }
 
@@ -1476,8 +1474,7 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   if (use_stats && n >= (1 << 14) && (uint64_t)(stats.n_zero + stats.n_one) * 2 >= (uint64_t)n)
     return msm_sparse_run<C>(n, d_scalars, enl, mont, d_points, points_internal, user, st, stats, out);
   MsmSchedule sch;
-  ZK_TRY(msm_resolve(C::id, n, enl, user, use_stats ? &stats : nullptr, &sch));
-  glv_schedule(C::id, n, user, points_internal, &sch);
+  ZK_TRY(msm_resolve(C::id, n, enl, user, use_stats ? &stats : nullptr, &sch, !points_internal));
   const bool glv = sch.glv != 0;
   const uint32_t np = glv ? 2u * (uint32_t)n : (uint32_t)n;  // points of the bucket problem
   const int c = sch.c, W = sch.windows, logB = c - 1;
@@ -1849,8 +1846,7 @@ int msm_prepare_points(int curve, int n, const uint64_t *d_ext, void *d_int, hip
 
 // device bytes one MSM of n points allocates (admission control of the serving loop)
 size_t msm_footprint(int curve, int n, int enl, bool points_internal, const zkg_msm_schedule *sched) {
-  MsmSchedule s = resolve_schedule(curve, n, enl, sched);
-  glv_schedule(curve, n, sched, points_internal, &s);
+  const MsmSchedule s = resolve_schedule(curve, n, enl, sched, !points_internal);
   const uint64_t np = s.glv ? 2 * (uint64_t)n : (uint64_t)n;
   const uint64_t W = s.windows, B = 1ull << (s.c - 1), nk = W * B, total = W * np;
   const uint64_t zw = curve == 0 ? ZLayout<Bn254G1>::XYZZ : ZLayout<Bls381G1>::XYZZ;

@@ -44,8 +44,11 @@ int msm_scalar_stats(int curve, int n, const uint64_t *d_scalars, int enl, int m
 bool cost_model_pick(int curve, const zkg_scalar_stats &st, int *c, int *seg_log);
 void msm_features(int curve, const zkg_scalar_stats &st, int c, int seg_log, float *x);
 int msm_candidates(int curve, const zkg_scalar_stats &st, int max, int *cs, int *segs);
+// glv_ok: the points arrive as affine words, so a BN254 job may take the GLV split (2n points
+// P_i and phi(P_i), scalars below 2^127): under the plain schedule always, under the
+// statistics policies when the measured scalars are wider than 128 bits
 int msm_resolve(int curve, int n, int enl, const zkg_msm_schedule *user, const zkg_scalar_stats *st,
-                MsmSchedule *out);
+                MsmSchedule *out, bool glv_ok = false);
 
 void msm_infinity(int curve, MsmResult *r);
 int msm_write_result(int curve, const MsmResult &r, uint64_t *tgt, int flags);
