@@ -31,6 +31,7 @@ static void run_op(int op, int n, const int32_t *a, const int32_t *b, int32_t *o
         zmul_sub(r, x, y, x1, y1);
         break;
       }
+      case 9: zinv(r, x); break;
       default: r = x; break;
     }
     memcpy(out + k * P::N, r.l, sizeof r.l);

@@ -102,6 +102,27 @@ def test_zmul_matches_integers_at_bounds(h, field):
 
 
 @pytest.mark.parametrize("field", [0, 1, 2, 3], ids=["bn-fp", "bn-fr", "bls-fp", "bls-fr"])
+def test_zinv_matches_integers(h, field):
+    """zinv of a normalised internal value v = x R' is x^-1 R' (Fermat in the Montgomery
+    domain), so v zinv(v) = R'^2 mod p; zero and multiples of p give zero.  Inputs: the bound
+    values (|v| < 8p, negatives), 1, p - 1, small and random values."""
+    p, bits, N, _ = FIELDS[field]
+    Rp = 1 << (bits * N)
+    rng = splitmix64(700 + field)
+    xs = sample_values(p, rng, 120, 8 if field != 3 else 4) + [1, 2, p - 1, p + 1, -1, 3 * p, 0, -2 * p]
+    A = arr([to_limbs(x, bits, N) for x in xs])
+    R = op(h, field, 9, A, A)
+    for i, x in enumerate(xs):
+        got = value(R[i], bits)
+        assert is_normalised(R[i], bits)
+        assert -p < got < 2 * p, (i, got / p)
+        if x % p == 0:
+            assert got % p == 0, i
+        else:
+            assert got * x % p == Rp * Rp % p, i
+
+
+@pytest.mark.parametrize("field", [0, 1, 2, 3], ids=["bn-fp", "bn-fr", "bls-fp", "bls-fr"])
 def test_zsqr_equals_zmul(h, field):
     """The squaring kernel sums each off-diagonal product once against a doubled limb; its
     column sums are zmul(a, a)'s, so the outputs agree limb for limb."""
