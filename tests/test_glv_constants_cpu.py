@@ -1,5 +1,5 @@
-"""The BN254 GLV constants of the MSM digit kernel (zikkurat-algebra_amd/csrc/msm.hip,
-struct Bn254Glv), read from the source and checked against what the mathematics fixes:
+"""The BN254 GLV constants used by the MSM digit kernel and the G1 FFT
+(zikkurat-algebra_amd/csrc/glv.hpp, struct Bn254Glv), read from the source and checked against what the mathematics fixes:
 beta and lambda are primitive cube roots of unity in Fp and Fr with (beta x, y) = lambda (x, y)
 on G1; (a1, b1) and (a2, b2) lie in the lattice {(x, y) : x + lambda y = 0 mod r} with
 determinant r; G1, G2 are the stated roundings; the internal form of beta is beta 2^261 mod p
@@ -11,7 +11,7 @@ from pathlib import Path
 
 from zkmath import BN
 
-SRC = Path(__file__).resolve().parent.parent / "zikkurat-algebra_amd" / "csrc" / "msm.hip"
+SRC = Path(__file__).resolve().parent.parent / "zikkurat-algebra_amd" / "csrc" / "glv.hpp"
 
 
 def constants():

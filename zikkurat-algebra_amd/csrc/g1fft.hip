@@ -16,6 +16,7 @@
 
 #include "curve.hpp"
 #include "curvez.hpp"
+#include "glv.hpp"
 #include "field.hpp"
 #include "hostfield.hpp"
 #include "msm.hpp"
@@ -112,6 +113,14 @@ __global__ __launch_bounds__(kG1Threads) void g1_twiddle_kernel(uint32_t *__rest
   fstore(tab + (size_t)j * FR::N, r);
 }
 
+// k P: BN254 through the GLV endomorphism (exact for every point of its prime-order G1);
+// BLS12-381, whose G1 has a cofactor, by double-and-add on k
+template <class C>
+__device__ __forceinline__ void scalar_mul(XyzzZ<C> &r, const XyzzZ<C> &p, const uint32_t (&k)[8]) {
+  if constexpr (C::id == 0) zxyzz_scalar_mul_glv(r, p, k);
+  else zxyzz_scalar_mul(r, p, k);
+}
+
 // one radix-2 stage over blocks of 2^s points (DIT: u + w v, u - w v; DIF: u + v, w (u - v))
 template <class C, bool DIF>
 __global__ __launch_bounds__(kG1Threads) void g1_fft_stage_kernel(int32_t *__restrict__ pts,
@@ -130,7 +139,7 @@ __global__ __launch_bounds__(kG1Threads) void g1_fft_stage_kernel(int32_t *__res
   for (int j = 0; j < 8; j++) k[j] = kp[j];
   if (!DIF) {
     if (pos == 0) w = v;  // twiddle 1
-    else zxyzz_scalar_mul(w, v, k);
+    else scalar_mul<C>(w, v, k);
     zxyzz_add(a, u, w);
     zxyzz_neg(w, w);
     zxyzz_add(v, u, w);
@@ -142,7 +151,7 @@ __global__ __launch_bounds__(kG1Threads) void g1_fft_stage_kernel(int32_t *__res
     zxyzz_add(w, u, v);
     if (pos != 0) {
       XyzzZ<C> r;
-      zxyzz_scalar_mul(r, w, k);
+      scalar_mul<C>(r, w, k);
       w = r;
     }
     zxyzz_store(pts + (size_t)i0 * PW, a);
@@ -161,7 +170,7 @@ __global__ __launch_bounds__(kG1Threads) void g1_scale_kernel(int32_t *__restric
   for (int j = 0; j < 8; j++) k[j] = kst.w[j];
   XyzzZ<C> p, r;
   zxyzz_load(p, pts + (size_t)i * PW);
-  zxyzz_scalar_mul(r, p, k);
+  scalar_mul<C>(r, p, k);
   zxyzz_store(pts + (size_t)i * PW, r);
 }
 

@@ -1,0 +1,102 @@
+// GLV endomorphism of BN254's G1: constants and the scalar split, shared by the MSM digit
+// pass and the G1 FFT.
+#pragma once
+#include <stdint.h>
+
+#include "curvez.hpp"
+
+namespace zk {
+
+// GLV split of BN254 scalars (Gallant, Lambert, Vanstone, CRYPTO 2001).  BN254's G1 has prime
+// order r and the endomorphism phi(x, y) = (beta x, y) = lambda P, with beta and lambda
+// primitive cube roots of unity in Fp and Fr.  A scalar k < r is split as
+// k = k1 + lambda k2 (mod r): c1 = floor(k G1 / 2^256), c2 = floor(k G2 / 2^256),
+// k1 = k - c1 a1 - c2 a2, k2 = c1 |b1| - c2 b2, where (a1, b1), (a2, b2) is a short basis of
+// {(x, y) : x + lambda y = 0 mod r} (extended Euclid on r and lambda; det = r) and
+// G1 = round(2^256 b2 / r), G2 = round(-2^256 b1 / r).  The floor errors lie in
+// (-1.125, 0.125), so |k1|, |k2| < 1.125 a1 < 2^127: both fit 128-bit two's complement, and
+// signed-digit windows over 128 bits cover them.
+struct Bn254Glv {
+  static constexpr uint64_t G1[3] = {0x5398fd0300ff6565ull, 0x4ccef014a773d2d2ull, 0x2ull};
+  static constexpr uint64_t G2[2] = {0xd91d232ec7e0b3d7ull, 0x2ull};
+  static constexpr uint64_t A1[2] = {0x8211bbeb7d4f1128ull, 0x6f4d8248eeb859fcull};
+  static constexpr uint64_t A2 = 0x89d3256894d213e3ull;  // = |b1|
+  static constexpr uint64_t B2[2] = {0x0be4e1541221250bull, 0x6f4d8248eeb859fdull};
+  // beta R' mod p (R' = 2^261), the internal form of beta: zmul(x R', beta R') = beta x R'
+  static constexpr int32_t BETA[9] = {416069521, 391846970, 193189602, 244240497, 310558434,
+                                      69340804,  535036287, 352865610, 1478937};
+};
+
+// (k G) >> 256 for k < 2^256 and a constant G of NG 64-bit limbs, low 128 bits of the quotient
+template <int NG>
+__device__ __forceinline__ unsigned __int128 mul_shift256(const uint64_t k[4], const uint64_t (&g)[NG]) {
+  using u128 = unsigned __int128;
+  uint64_t prod[4 + NG] = {};
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    uint64_t carry = 0;
+#pragma unroll
+    for (int j = 0; j < NG; j++) {
+      const u128 t = (u128)k[i] * g[j] + prod[i + j] + carry;
+      prod[i + j] = (uint64_t)t;
+      carry = (uint64_t)(t >> 64);
+    }
+    prod[i + NG] = carry;
+  }
+  return (u128)prod[4] | ((u128)prod[5] << 64);
+}
+
+// k < r (4 limbs) -> |k1|, |k2| as two 64-bit limbs each, with their signs
+__device__ __forceinline__ void glv_split(const uint64_t k[4], uint64_t m1[2], bool &neg1, uint64_t m2[2],
+                                          bool &neg2) {
+  using G = Bn254Glv;
+  using u128 = unsigned __int128;
+  const u128 c1 = mul_shift256<3>(k, G::G1);
+  const u128 c2 = mul_shift256<2>(k, G::G2);  // < 2^64
+  const u128 a1 = (u128)G::A1[0] | ((u128)G::A1[1] << 64);
+  const u128 b2 = (u128)G::B2[0] | ((u128)G::B2[1] << 64);
+  u128 k1 = ((u128)k[0] | ((u128)k[1] << 64)) - c1 * a1 - c2 * (u128)G::A2;  // mod 2^128
+  u128 k2 = c1 * (u128)G::A2 - c2 * b2;
+  neg1 = (uint64_t)(k1 >> 127) != 0;
+  neg2 = (uint64_t)(k2 >> 127) != 0;
+  if (neg1) k1 = (u128)0 - k1;
+  if (neg2) k2 = (u128)0 - k2;
+  m1[0] = (uint64_t)k1;
+  m1[1] = (uint64_t)(k1 >> 64);
+  m2[0] = (uint64_t)k2;
+  m2[1] = (uint64_t)(k2 >> 64);
+}
+
+// k P for a BN254 point (XYZZ) and a standard-form scalar k < r (8 words): the GLV split and one
+// joint double-and-add over at most 127 bits (Straus-Shamir) with P, phi(P) and their sum, the
+// signs of k1, k2 folded into them; phi(X, Y, ZZ, ZZZ) = (beta X, Y, ZZ, ZZZ) since x = X / ZZ.
+// About 127 doublings and 96 additions against 254 and 127 for double-and-add on k.
+template <class C>
+__device__ __forceinline__ void zxyzz_scalar_mul_glv(XyzzZ<C> &r, const XyzzZ<C> &p, const uint32_t (&k)[8]) {
+  uint64_t k64[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) k64[j] = (uint64_t)k[2 * j] | ((uint64_t)k[2 * j + 1] << 32);
+  uint64_t m1[2], m2[2];
+  bool n1, n2;
+  glv_split(k64, m1, n1, m2, n2);
+  XyzzZ<C> q0 = p, q1 = p, q01;
+  Fz<ZBn254Fp> beta;
+  zset_const(beta, Bn254Glv::BETA);
+  zmul(q1.x, p.x, beta);
+  if (n1) zxyzz_neg(q0, q0);
+  if (n2) zxyzz_neg(q1, q1);
+  zxyzz_add(q01, q0, q1);
+  auto bit = [&](const uint64_t (&m)[2], int i) -> uint32_t { return (uint32_t)(m[i >> 6] >> (i & 63)) & 1u; };
+  int top = 127;
+  while (top >= 0 && !(bit(m1, top) | bit(m2, top))) top--;
+  zxyzz_set_inf(r);
+  for (int i = top; i >= 0; i--) {
+    XyzzZ<C> t;
+    zxyzz_dbl(t, r);
+    const uint32_t b1 = bit(m1, i), b2 = bit(m2, i);
+    if (b1 | b2) zxyzz_add(r, t, b1 ? (b2 ? q01 : q0) : q1);
+    else r = t;
+  }
+}
+
+}  // namespace zk

@@ -36,6 +36,7 @@
 
 #include "curve.hpp"
 #include "curvez.hpp"
+#include "glv.hpp"
 #include "hostfield.hpp"
 #include "msm.hpp"
 #include "runtime.hpp"
@@ -66,66 +67,6 @@ __device__ __forceinline__ uint32_t encode_digit(uint32_t raw, int c, uint32_t &
   }
   carry = 0;
   return raw ? ((raw - 1) << 1) : kZeroDigit;
-}
-
-// GLV split of BN254 scalars (Gallant, Lambert, Vanstone, CRYPTO 2001).  BN254's G1 has prime
-// order r and the endomorphism phi(x, y) = (beta x, y) = lambda P, with beta and lambda
-// primitive cube roots of unity in Fp and Fr.  A scalar k < r is split as
-// k = k1 + lambda k2 (mod r): c1 = floor(k G1 / 2^256), c2 = floor(k G2 / 2^256),
-// k1 = k - c1 a1 - c2 a2, k2 = c1 |b1| - c2 b2, where (a1, b1), (a2, b2) is a short basis of
-// {(x, y) : x + lambda y = 0 mod r} (extended Euclid on r and lambda; det = r) and
-// G1 = round(2^256 b2 / r), G2 = round(-2^256 b1 / r).  The floor errors lie in
-// (-1.125, 0.125), so |k1|, |k2| < 1.125 a1 < 2^127: both fit 128-bit two's complement, and
-// signed-digit windows over 128 bits cover them.
-struct Bn254Glv {
-  static constexpr uint64_t G1[3] = {0x5398fd0300ff6565ull, 0x4ccef014a773d2d2ull, 0x2ull};
-  static constexpr uint64_t G2[2] = {0xd91d232ec7e0b3d7ull, 0x2ull};
-  static constexpr uint64_t A1[2] = {0x8211bbeb7d4f1128ull, 0x6f4d8248eeb859fcull};
-  static constexpr uint64_t A2 = 0x89d3256894d213e3ull;  // = |b1|
-  static constexpr uint64_t B2[2] = {0x0be4e1541221250bull, 0x6f4d8248eeb859fdull};
-  // beta R' mod p (R' = 2^261), the internal form of beta: zmul(x R', beta R') = beta x R'
-  static constexpr int32_t BETA[9] = {416069521, 391846970, 193189602, 244240497, 310558434,
-                                      69340804,  535036287, 352865610, 1478937};
-};
-
-// (k G) >> 256 for k < 2^256 and a constant G of NG 64-bit limbs, low 128 bits of the quotient
-template <int NG>
-__device__ __forceinline__ unsigned __int128 mul_shift256(const uint64_t k[4], const uint64_t (&g)[NG]) {
-  using u128 = unsigned __int128;
-  uint64_t prod[4 + NG] = {};
-#pragma unroll
-  for (int i = 0; i < 4; i++) {
-    uint64_t carry = 0;
-#pragma unroll
-    for (int j = 0; j < NG; j++) {
-      const u128 t = (u128)k[i] * g[j] + prod[i + j] + carry;
-      prod[i + j] = (uint64_t)t;
-      carry = (uint64_t)(t >> 64);
-    }
-    prod[i + NG] = carry;
-  }
-  return (u128)prod[4] | ((u128)prod[5] << 64);
-}
-
-// k < r (4 limbs) -> |k1|, |k2| as two 64-bit limbs each, with their signs
-__device__ __forceinline__ void glv_split(const uint64_t k[4], uint64_t m1[2], bool &neg1, uint64_t m2[2],
-                                          bool &neg2) {
-  using G = Bn254Glv;
-  using u128 = unsigned __int128;
-  const u128 c1 = mul_shift256<3>(k, G::G1);
-  const u128 c2 = mul_shift256<2>(k, G::G2);  // < 2^64
-  const u128 a1 = (u128)G::A1[0] | ((u128)G::A1[1] << 64);
-  const u128 b2 = (u128)G::B2[0] | ((u128)G::B2[1] << 64);
-  u128 k1 = ((u128)k[0] | ((u128)k[1] << 64)) - c1 * a1 - c2 * (u128)G::A2;  // mod 2^128
-  u128 k2 = c1 * (u128)G::A2 - c2 * b2;
-  neg1 = (uint64_t)(k1 >> 127) != 0;
-  neg2 = (uint64_t)(k2 >> 127) != 0;
-  if (neg1) k1 = (u128)0 - k1;
-  if (neg2) k2 = (u128)0 - k2;
-  m1[0] = (uint64_t)k1;
-  m1[1] = (uint64_t)(k1 >> 64);
-  m2[0] = (uint64_t)k2;
-  m2[1] = (uint64_t)(k2 >> 64);
 }
 
 template <class C>
