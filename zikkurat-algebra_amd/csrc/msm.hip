@@ -185,7 +185,12 @@ __global__ __launch_bounds__(kThreads) void msm_digits_kernel(const uint64_t *__
   }
 }
 
-// scalar statistics: stats[0] = largest bit length, stats[1] = zeros, stats[2] = ones
+// scalar statistics: stats[0] = largest bit length, stats[1] = zeros, stats[2] = ones.  A
+// grid-stride loop over at most kStatsBlocks blocks: each thread folds its scalars locally,
+// the block folds in LDS, and one set of global atomics per block reaches the three words
+// (one block per 256 scalars put ~50 000 same-address atomics on them at 2^22: 0.5 ms).
+static constexpr unsigned kStatsBlocks = 1024;
+
 template <class C>
 __global__ __launch_bounds__(kThreads) void msm_stats_kernel(const uint64_t *__restrict__ sc, int enl, int mont,
                                                             uint32_t n, uint32_t *__restrict__ stats) {
@@ -197,10 +202,9 @@ __global__ __launch_bounds__(kThreads) void msm_stats_kernel(const uint64_t *__r
     s_one = 0;
   }
   __syncthreads();
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  int bits = -1;
-  if (i < n) {
-    bits = 0;
+  uint32_t bmax = 0, zc = 0, oc = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    int bits = 0;
     if (mont && enl == 4) {
       Fe<Fr> x;
 #pragma unroll
@@ -219,16 +223,16 @@ __global__ __launch_bounds__(kThreads) void msm_stats_kernel(const uint64_t *__r
         if (v) bits = 64 * j + 64 - __clzll(v);
       }
     }
+    bmax = bmax > (uint32_t)bits ? bmax : (uint32_t)bits;
+    zc += bits == 0;
+    oc += bits == 1;
   }
-  const uint64_t zmask = __ballot(bits == 0), omask = __ballot(bits == 1);
-  if ((threadIdx.x & 63) == 0) {
-    if (zmask) atomicAdd(&s_zero, (uint32_t)__popcll(zmask));
-    if (omask) atomicAdd(&s_one, (uint32_t)__popcll(omask));
-  }
-  if (bits > 0) atomicMax(&s_max, (uint32_t)bits);
+  if (bmax) atomicMax(&s_max, bmax);
+  if (zc) atomicAdd(&s_zero, zc);
+  if (oc) atomicAdd(&s_one, oc);
   __syncthreads();
   if (threadIdx.x == 0) {
-    atomicMax(&stats[0], s_max);
+    if (s_max) atomicMax(&stats[0], s_max);
     if (s_zero) atomicAdd(&stats[1], s_zero);
     if (s_one) atomicAdd(&stats[2], s_one);
   }
@@ -1304,7 +1308,7 @@ int msm_scalar_stats(int curve, int n, const uint64_t *d_scalars, int enl, int m
   DevBuf d;
   ZK_TRY(d.alloc(16));
   ZK_HIP(hipMemsetAsync(d.p, 0, 16, st));
-  const dim3 g((n + kThreads - 1) / kThreads), b(kThreads);
+  const dim3 g(std::min<unsigned>((unsigned)((n + kThreads - 1) / kThreads), kStatsBlocks)), b(kThreads);
   if (curve == 0)
     hipLaunchKernelGGL(msm_stats_kernel<Bn254G1>, g, b, 0, st, d_scalars, enl, mont, (uint32_t)n, d.as<uint32_t>());
   else
@@ -1412,7 +1416,10 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   // Measured sparse scalar sets (at least half the scalars 0 or 1): the other scalars and
   // their points are compacted into a dense MSM of their own, the points of the scalars
   // equal to 1 are summed directly, and the zeros drop out; the two sums are added on the host.
-  if (use_stats && n >= (1 << 14) && (uint64_t)(stats.n_zero + stats.n_one) * 2 >= (uint64_t)n)
+  // Zeros and ones cost the plain pipeline little beyond their digits, so the compaction pays
+  // only while its own passes are small next to the MSM: measured faster up to 2^20 points,
+  // slower from 2^21 (BN254 2^22, 80% in {0, 1}: 2.82 ms compacted against 2.52 ms).
+  if (use_stats && n >= (1 << 14) && n < (1 << 21) && (uint64_t)(stats.n_zero + stats.n_one) * 2 >= (uint64_t)n)
     return msm_sparse_run<C>(n, d_scalars, enl, mont, d_points, points_internal, user, st, stats, out);
   MsmSchedule sch;
   ZK_TRY(msm_resolve(C::id, n, enl, user, use_stats ? &stats : nullptr, &sch, !points_internal));
