@@ -543,9 +543,12 @@ def main():
     log(f"bn254 2^{a.logn} closed form: {ok_bn}")
     t = time_steps(bn.run, a.steps, a.warmup)
     value = bn.n / t / 1e6
-    phases = zkgpu.last_timing()[1]
-    log(f"bn254 msm: {t * 1e3:.3f} ms/step, {value:.1f} Mpts/s, phases(ms) {['%.3f' % x for x in phases]}")
+    # per-phase HIP events in calls of their own (their records idle the stream between phases)
+    zkgpu.set_phase_timing(True)
     t_acc = kernel_ms(bn.run, 5, 1)
+    phases = zkgpu.last_timing()[1]
+    zkgpu.set_phase_timing(False)
+    log(f"bn254 msm: {t * 1e3:.3f} ms/step, {value:.1f} Mpts/s, phases(ms) {['%.3f' % x for x in phases]}")
     # algorithmic work of the accumulation kernel: one mixed XYZZ add (8M + 2S) per nonzero digit
     st = zkgpu.last_msm_stats()
     c_used, entries = st["window"], st["entries"]

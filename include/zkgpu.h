@@ -361,8 +361,13 @@ int zkg_msm_set_cost_model(void *mlp);
  * 2..5 = Montgomery multiplications per second (BN254 Fp, BN254 Fr, BLS Fp, BLS Fr) */
 int zkg_microbench(int which, double *result);
 
-/* timing of the last zkg_msm / zkg_ntt call on the library stream (HIP events), per phase */
+/* timing of the calling thread's last MSM: total host wall clock, and per phase the HIP-event
+ * durations of sort, accumulation, continuation fix, pyramid and read-back (recorded only
+ * while zkg_msm_phase_timing is on) followed by the host combine */
 int zkg_last_timing(double *ms_total, double *ms_phases, int max_phases, int *nphases);
+/* 1: later MSMs (any thread) record HIP events between their phases; 0 (the default): they
+ * record none, since each event record idles the stream a few microseconds */
+int zkg_msm_phase_timing(int on);
 /* schedule facts of the last MSM: {window c, windows W, sorted (nonzero-digit) entries, heavy buckets} */
 int zkg_msm_last_stats(long long *out4);
 

@@ -158,6 +158,28 @@ def test_msm_closed_form(zk, C, logn, dist):
 
 
 @pytest.mark.parametrize("C", CURVES, ids=IDS)
+def test_msm_phase_timing_switch(zk, C):
+    """Phase events are recorded only while phase timing is on (five device phases and the host
+    combine); the switch never changes the result."""
+    n = 1 << 14
+    ks = random_scalars(300, n, C.r)
+    ss = random_scalars(301, n, C.r)
+    dP = device_points(zk, C, ks)
+    dS = zk.DeviceArray.from_host(ints_to_array(ss, 4))
+    off = zk.msm_device(C.id, n, dS, dP)
+    assert len(zk.last_timing()[1]) == 1
+    zk.set_phase_timing(True)
+    try:
+        on = zk.msm_device(C.id, n, dS, dP)
+        phases = zk.last_timing()[1]
+    finally:
+        zk.set_phase_timing(False)
+    assert (on == off).all()
+    assert len(phases) == 6 and min(phases) >= 0
+    assert C.words_to_affine(off) == C.mul(sum(s * k for s, k in zip(ss, ks)) % C.r, C.g)
+
+
+@pytest.mark.parametrize("C", CURVES, ids=IDS)
 def test_msm_schedules_agree(zk, C):
     """Window width and segment length change the schedule, never the result."""
     n = 5000

@@ -807,6 +807,8 @@ int zkg_last_timing(double *ms_total, double *ms_phases, int max_phases, int *np
   return msm_last_timing(ms_total, ms_phases, max_phases, nphases);
 }
 
+int zkg_msm_phase_timing(int on) { return msm_set_phase_timing(on); }
+
 int zkg_msm_last_stats(long long *out4) { return msm_last_stats(out4); }
 
 // ---------------- schedule selection ----------------

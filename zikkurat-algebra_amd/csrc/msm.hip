@@ -16,11 +16,14 @@
 //              [t 2^s, (t+1) 2^s), finds their buckets from the offsets (binary search, then
 //              a walk) and folds each run with mixed XYZZ + affine adds;
 //              a run that began in an earlier segment is written as a continuation partial
-//   6 fix      each bucket adds its continuation partials (heavy buckets: pairwise tree)
+//   6 fix      each bucket adds its continuation partials (heavy buckets: pairwise tree);
+//              an empty bucket gets the infinity the accumulation did not write
 //   7 pyramid  bucket reduction sum_b (b+1) B_b without serial running sums: level k pairs
 //              neighbours (P'_i = P_2i + P_2i+1) and keeps the odd children Q^(k) = P_2i+1,
 //              which are pairwise reduced alongside; then sum_b (b+1) B_b =
-//              Sum(B) + sum_k 2^(k-1) U_k with U_k the reduced Q^(k) (depth c-1 additions)
+//              Sum(B) + sum_k 2^(k-1) U_k with U_k the reduced Q^(k) (depth c-1 additions);
+//              wide levels two per launch, the narrow ones per aligned block in LDS (one
+//              launch for all levels of a block), the last ones one workgroup per window
 //   8 host     the window terms are combined by one Horner pass on the host
 //              (sum_w 2^(start_w) (Sum_w + sum_k 2^(k-1) U_wk), ~255 doublings) and normalised.
 // Window widths are balanced (c or c-1 bits each, see MsmSchedule), so every window spreads
@@ -28,6 +31,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <mutex>
 #include <string>
@@ -351,12 +355,21 @@ __device__ __forceinline__ uint32_t lds_rank(uint32_t *cnt, uint32_t key, bool a
   return active ? atomicAdd(&cnt[key], 1u) : 0u;
 }
 
-// phist[(w P + p) nchunks + chunk] = digits of the chunk in partition p of window w
+// phist[(w P + p) nchunks + chunk] = digits of the chunk in partition p of window w.
+// Workgroup (0, 0) also clears the heavy-bucket counters (heavy[0..1]) and the big-partition
+// count (big[0]) that later kernels of the same MSM count into, in place of two fill launches.
 __global__ __launch_bounds__(kSortThreads) void msm_part_hist_kernel(const uint32_t *__restrict__ dig, uint32_t n,
                                                                      int L, uint32_t P,
-                                                                     uint32_t *__restrict__ phist) {
+                                                                     uint32_t *__restrict__ phist,
+                                                                     uint32_t *__restrict__ heavy,
+                                                                     uint32_t *__restrict__ big) {
   extern __shared__ uint32_t cnt[];  // P counters
   const uint32_t chunk = blockIdx.x, w = blockIdx.y, nchunks = gridDim.x;
+  if (chunk == 0 && w == 0 && threadIdx.x == 0) {
+    heavy[0] = 0;
+    heavy[1] = 0;
+    big[0] = 0;
+  }
   for (uint32_t j = threadIdx.x; j < P; j += kSortThreads) cnt[j] = 0;
   __syncthreads();
   const uint32_t i0 = chunk * kPChunk;
@@ -788,20 +801,22 @@ __global__ __launch_bounds__(kThreads) void msm_accum_kernel(const uint32_t *__r
 // 6. continuation partials
 
 // buckets whose runs span more than `limit` segments get pairwise trees (heavy[0] = count,
-// heavy[1] = most partials, heavy[2..] = buckets); known from the offsets alone, so this runs
-// before the accumulation and its counts reach the host while the accumulation runs
+// heavy[1] = most partials, heavy[2] = sorted entries, heavy[3..] = buckets); known from the
+// offsets alone, so this runs before the accumulation and the three counts reach the host (one
+// copy) while the accumulation runs
 __global__ __launch_bounds__(kThreads) void msm_heavy_detect_kernel(const uint32_t *__restrict__ off, uint32_t nkeys,
                                                                    int seg_log, uint32_t *__restrict__ heavy,
                                                                    uint32_t heavy_cap, uint32_t limit) {
   const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nkeys) return;
+  if (b == 0) heavy[2] = off[nkeys];
   const uint32_t s = off[b], t = off[b + 1];
   if (s == t) return;
   const uint64_t k = ((uint64_t)(t - 1) >> seg_log) - (s >> seg_log);
   if (k > limit) {
     const uint32_t idx = atomicAdd(&heavy[0], 1u);
     atomicMax(&heavy[1], (uint32_t)k);
-    if (idx < heavy_cap) heavy[2 + idx] = b;
+    if (idx < heavy_cap) heavy[3 + idx] = b;
   }
 }
 
@@ -813,12 +828,16 @@ __global__ __launch_bounds__(kThreads) void msm_fix_kernel(const uint32_t *__res
   const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nkeys) return;
   const uint32_t s = off[b], t = off[b + 1];
-  if (s == t) return;
+  XyzzZ<C> acc, q;
+  if (s == t) {  // no entry: the accumulation wrote nothing here
+    zxyzz_set_inf(acc);
+    zxyzz_store(bsum + (size_t)b * PW, acc);
+    return;
+  }
   const uint64_t fs = s >> seg_log, ls = (uint64_t)(t - 1) >> seg_log;
   if (ls == fs) return;
   const uint64_t k = ls - fs;
   if (k > limit) return;  // a heavy bucket
-  XyzzZ<C> acc, q;
   zxyzz_load(acc, bsum + (size_t)b * PW);
   for (uint64_t j = fs + 1; j <= ls; j++) {
     zxyzz_load(q, cont + j * PW);
@@ -837,7 +856,7 @@ __global__ __launch_bounds__(kThreads) void msm_heavy_round_kernel(const uint32_
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t h = (uint32_t)(t / pairs_per_bucket), j = (uint32_t)(t % pairs_per_bucket);
   if (h >= heavy[0]) return;
-  const uint32_t b = heavy[2 + h];
+  const uint32_t b = heavy[3 + h];
   const uint64_t fs = off[b] >> seg_log, ls = (uint64_t)(off[b + 1] - 1) >> seg_log;
   const uint64_t K = ls - fs, base = fs + 1;
   const uint64_t i0 = (uint64_t)j << (r + 1), i1 = i0 + (1ull << r);
@@ -891,7 +910,7 @@ __global__ __launch_bounds__(kThreads) void msm_heavy_final_kernel(const uint32_
   constexpr int PW = ZLayout<C>::XYZZ;
   const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
   if (h >= heavy[0]) return;
-  const uint32_t b = heavy[2 + h];
+  const uint32_t b = heavy[3 + h];
   const uint64_t fs = off[b] >> seg_log;
   XyzzZ<C> a, q;
   zxyzz_load(a, bsum + (size_t)b * PW);
@@ -963,22 +982,25 @@ __global__ __launch_bounds__(kThreads, PyrWaves<C>::v) void msm_pyramid2_kernel(
   zxyzz_store(out + ((uint64_t)a * per + rem) * PW, p0);
 }
 
-// The last pyramid levels hold a few hundred points per window, so a launch per level (or per
-// two) costs a dependent chain of additions plus a launch for very little work.  Here one
-// workgroup per window runs every remaining level, M -> 1, each a single-level step (pairs of
-// every array, plus the odd children of array 0).  Level 0 reads `in` ([A][W][M]) and writes
-// the workgroup's LDS; later levels update the LDS in place (every item is read into
-// registers before a barrier and written after it); the last level writes [A + L][W] to
-// `out`.  Windows share no memory but `in` (read only) and disjoint points of `out`, and
-// `out` must not overlap `in`.  The host guarantees at most 2 blockDim.x items at level 0,
-// at most blockDim.x at every later level, and LDS for the level-0 output.
+// Once a level holds a few thousand points per window, a launch per level costs a dependent
+// chain of additions plus a launch for little work.  Here each workgroup owns an aligned
+// block of Mb points of every array of one window (grid: M / Mb blocks x W windows) and runs
+// the log2(Mb) levels that reduce the block to one point, each a single-level step (pairs of
+// every array, plus the odd children of array 0): the levels of one block use only that
+// block's points.  With Mb = M this is the tail of the pyramid, one workgroup per window.
+// Level 0 reads `in` ([A][W][M]) and writes the workgroup's LDS; later levels update the LDS
+// in place (every item is read into registers before a barrier and written after it); the
+// last level writes point q of each of the A + log2(Mb) arrays of window w to `out`
+// ([A + log2 Mb][W][M / Mb]).  Workgroups share no memory but `in` (read only) and disjoint
+// points of `out`, and `out` must not overlap `in`.  The host guarantees at most 2 blockDim.x
+// items at level 0, at most blockDim.x at every later level, and LDS for the level-0 output.
 static constexpr int kTailThreads = 512;
 static constexpr uint64_t kPyr2MinThreads = 65536;
 static constexpr size_t kTailLdsBytes = 128 * 1024;
 
 template <class C>
-__device__ __forceinline__ void tail_item(XyzzZ<C> &r, const int32_t *src, uint32_t M, uint32_t A, uint32_t a,
-                                          uint32_t i, uint64_t stride) {
+__device__ __forceinline__ void tail_item(XyzzZ<C> &r, const int32_t *src, uint32_t A, uint32_t a, uint32_t i,
+                                          uint64_t stride) {
   constexpr int PW = ZLayout<C>::XYZZ;
   if (a < A) {
     XyzzZ<C> q;
@@ -993,41 +1015,44 @@ __device__ __forceinline__ void tail_item(XyzzZ<C> &r, const int32_t *src, uint3
 template <class C>
 __global__ __launch_bounds__(kTailThreads) void msm_pyramid_tail_kernel(const int32_t *__restrict__ in,
                                                                         int32_t *__restrict__ out, uint32_t W,
-                                                                        uint32_t M, uint32_t A) {
+                                                                        uint32_t M, uint32_t A, uint32_t Mb) {
   constexpr int PW = ZLayout<C>::XYZZ;
   extern __shared__ int32_t tl_lds[];
-  const uint32_t w = blockIdx.x;
-  // level 0: global [A][W][M] (window w's arrays are M points apart by W M) -> LDS [A+1][M/2]
+  const uint32_t q = blockIdx.x, w = blockIdx.y, nb = M / Mb;
+  // output point q of array a (window w)
+  auto dst = [&](uint32_t a) { return out + (((uint64_t)a * W + w) * nb + q) * PW; };
+  // level 0: global [A][W][M] (window w's arrays are W M points apart) -> LDS [A+1][Mb/2]
   {
-    const uint32_t half = M >> 1, items = (A + 1) * half;
+    const uint32_t half = Mb >> 1, items = (A + 1) * half;
+    const int32_t *src = in + ((uint64_t)w * M + (uint64_t)q * Mb) * PW;
     for (uint32_t j = threadIdx.x; j < items; j += blockDim.x) {
       const uint32_t a = j / half, i = j - a * half;
       XyzzZ<C> r;
-      tail_item<C>(r, in + (uint64_t)w * M * PW, M, A, a, i, (uint64_t)W * M);
+      tail_item<C>(r, src, A, a, i, (uint64_t)W * M);
       if (half == 1)
-        zxyzz_store(out + ((uint64_t)a * W + w) * PW, r);
+        zxyzz_store(dst(a), r);
       else
         zxyzz_store(tl_lds + (uint64_t)j * PW, r);
     }
     A += 1;
-    M = half;
+    Mb = half;
   }
-  while (M > 1) {
+  while (Mb > 1) {
     __syncthreads();  // the previous level's points are in LDS
-    const uint32_t half = M >> 1, items = (A + 1) * half;
+    const uint32_t half = Mb >> 1, items = (A + 1) * half;
     const uint32_t j = threadIdx.x;
     XyzzZ<C> r;
     const uint32_t a = j / half, i = j - a * half;
-    if (j < items) tail_item<C>(r, tl_lds, M, A, a, i, M);
+    if (j < items) tail_item<C>(r, tl_lds, A, a, i, Mb);
     __syncthreads();  // every read of this level is done before the in-place writes
     if (j < items) {
       if (half == 1)
-        zxyzz_store(out + ((uint64_t)a * W + w) * PW, r);
+        zxyzz_store(dst(a), r);
       else
         zxyzz_store(tl_lds + (uint64_t)j * PW, r);
     }
     A += 1;
-    M = half;
+    Mb = half;
   }
 }
 
@@ -1195,19 +1220,29 @@ int msm_last_timing(double *ms_total, double *ms_phases, int max_phases, int *np
   return ZK_OK;
 }
 
+// Phase events are recorded only while phase timing is on: each record puts a timestamp
+// barrier in the stream, which idles the GPU for a few microseconds before the next kernel.
+static std::atomic<int> g_phase_timing{0};
+
+int msm_set_phase_timing(int on) {
+  g_phase_timing.store(on ? 1 : 0);
+  return ZK_OK;
+}
+
 struct Events {
   hipEvent_t e[8];
-  int n = 0;
-  bool ok = true;
+  int n = 0, made = 0;
+  bool ok = false;
   Events() {
-    for (int i = 0; i < 8; i++)
-      if (hipEventCreate(&e[i]) != hipSuccess) ok = false;
+    if (!g_phase_timing.load()) return;
+    while (made < 8 && hipEventCreate(&e[made]) == hipSuccess) made++;
+    ok = made == 8;
   }
   ~Events() {
-    for (int i = 0; i < 8; i++) (void)hipEventDestroy(e[i]);
+    for (int i = 0; i < made; i++) (void)hipEventDestroy(e[i]);
   }
   void mark(hipStream_t st) {
-    if (n < 8) (void)hipEventRecord(e[n++], st);
+    if (ok && n < 8) (void)hipEventRecord(e[n++], st);
   }
 };
 
@@ -1462,7 +1497,7 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   if (!points_internal) ZK_TRY(zpts.alloc((uint64_t)np * AW * 4));
   // every bucket may be heavy (small seg_log, few entries per segment)
   const uint32_t heavy_cap = (uint32_t)nkeys;
-  ZK_TRY(heavy.alloc(((uint64_t)heavy_cap + 2) * 4));
+  ZK_TRY(heavy.alloc(((uint64_t)heavy_cap + 3) * 4));
   ZK_TRY(pyrA.alloc(nkeys * ZW * 4));
   ZK_TRY(pyrB.alloc(nkeys * ZW * 4));
   const uint64_t per_block = (uint64_t)kThreads * kScanPer;
@@ -1476,9 +1511,6 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   Events ev;
   auto t_host0 = std::chrono::steady_clock::now();
   ev.mark(st);
-  ZK_HIP(hipMemsetAsync(bsum.p, 0, nkeys * ZW * 4, st));
-  ZK_HIP(hipMemsetAsync(heavy.p, 0, 8, st));
-  ZK_HIP(hipMemsetAsync(big.p, 0, 4, st));
   const dim3 blk(kThreads);
   const dim3 gpts((n + kThreads - 1) / kThreads);
   const int32_t *zp = points_internal ? (const int32_t *)d_points : zpts.as<int32_t>();
@@ -1488,7 +1520,7 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
                      zpts.as<int32_t>());
   const dim3 gch(nchunks, W);
   hipLaunchKernelGGL(msm_part_hist_kernel, gch, dim3(kSortThreads), (size_t)P * 4, st, dig.as<uint32_t>(),
-                     np, L, P, phist.as<uint32_t>());
+                     np, L, P, phist.as<uint32_t>(), heavy.as<uint32_t>(), big.as<uint32_t>());
   hipLaunchKernelGGL(scan_block_sums_kernel, dim3(nsb), blk, 0, st, phist.as<uint32_t>(), (size_t)nph,
                      scan_b.as<uint32_t>());
   hipLaunchKernelGGL(scan_top_kernel, dim3(1), blk, 0, st, scan_b.as<uint32_t>(), nsb);
@@ -1530,8 +1562,7 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   ZK_TRY(lease.acquire());
   MsmStage &stage = *lease.s;
   uint32_t *info = stage.info;
-  ZK_HIP(hipMemcpyAsync(info, heavy.p, 8, hipMemcpyDeviceToHost, st));
-  ZK_HIP(hipMemcpyAsync(info + 2, off.as<uint32_t>() + nkeys, 4, hipMemcpyDeviceToHost, st));
+  ZK_HIP(hipMemcpyAsync(info, heavy.p, 12, hipMemcpyDeviceToHost, st));
   ZK_HIP(hipEventRecord(stage.info_ev, st));
   ev.mark(st);
   const bool prefetch = user && user->prefetch == 2;
@@ -1599,22 +1630,35 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
       int levels = 0;
       while ((M >> levels) > 1) levels++;
       const unsigned tb = std::min((std::max((items0 + 1) / 2, items1) + 63) & ~63u, (unsigned)kTailThreads);
-      hipLaunchKernelGGL(msm_pyramid_tail_kernel<C>, dim3((unsigned)W), dim3(tb), tail_lds, st, in, bufs[k],
-                         (uint32_t)W, M, A);
+      hipLaunchKernelGGL(msm_pyramid_tail_kernel<C>, dim3(1u, (unsigned)W), dim3(tb), tail_lds, st, in, bufs[k],
+                         (uint32_t)W, M, A, M);
       in = bufs[k];
       A += levels;
       M = 1;
       break;
     }
+    // the largest block of Mb points per array whose level 0 is one item per thread
+    uint32_t Mb = M / 2;
+    while (Mb >= 4 && ((A + 1) * (Mb / 2) > (uint32_t)kTailThreads ||
+                       (size_t)(A + 1) * (Mb / 2) * ZW * 4 > kTailLdsBytes))
+      Mb >>= 1;
     // two levels per launch while that still fills the GPU (one thread per 4 points, 3-4
-    // dependent additions each); below, a level per launch (one addition per thread) has the
-    // shorter chain
+    // dependent additions each); below, the levels of each Mb-point block in one launch (one
+    // addition per thread per level, no launch between levels), or a level per launch
     if (M >= 4 && (uint64_t)A * W * (M / 4) >= kPyr2MinThreads) {
       const uint64_t thr = (uint64_t)A * W * (M / 4);
       hipLaunchKernelGGL(msm_pyramid2_kernel<C>, dim3((unsigned)((thr + kThreads - 1) / kThreads)), blk, 0, st, in,
                          bufs[k], (uint32_t)W, M, A);
       A += 2;
       M >>= 2;
+    } else if (Mb >= 4) {
+      const uint32_t items = (A + 1) * (Mb / 2);
+      int levels = 0;
+      while ((Mb >> levels) > 1) levels++;
+      hipLaunchKernelGGL(msm_pyramid_tail_kernel<C>, dim3(M / Mb, (unsigned)W), dim3((items + 63) & ~63u),
+                         (size_t)items * ZW * 4, st, in, bufs[k], (uint32_t)W, M, A, Mb);
+      A += levels;
+      M /= Mb;
     } else {
       const uint64_t thr = (uint64_t)A * W * (M / 2);
       hipLaunchKernelGGL(msm_pyramid_kernel<C>, dim3((unsigned)((thr + kThreads - 1) / kThreads)), blk, 0, st, in,

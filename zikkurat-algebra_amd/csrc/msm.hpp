@@ -99,6 +99,7 @@ int msm_batch(int njobs, const int *curves, const int *ns, const uint64_t *const
               const zkg_msm_schedule *sched, int nstreams, double *wall_ms);
 
 int msm_last_timing(double *ms_total, double *ms_phases, int max_phases, int *nphases);
+int msm_set_phase_timing(int on);
 int msm_last_stats(long long *out4);
 
 }  // namespace zk

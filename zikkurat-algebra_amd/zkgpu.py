@@ -84,6 +84,7 @@ def lib():
         L.zkg_microbench.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
         L.zkg_last_timing.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                       ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.zkg_msm_phase_timing.argtypes = [ctypes.c_int]
         L.zkg_ntt_batch_create.argtypes = [ctypes.c_int, ctypes.c_int, vp, ctypes.POINTER(vp)]
         L.zkg_ntt_batch_run.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
         L.zkg_ntt_batch_destroy.argtypes = [vp]
@@ -840,6 +841,12 @@ def last_timing():
     n = ctypes.c_int()
     _check(lib().zkg_last_timing(ctypes.byref(tot), ph, 8, ctypes.byref(n)))
     return tot.value, list(ph)[:n.value]
+
+
+def set_phase_timing(on):
+    """Record HIP events between the phases of later MSMs (off by default: each event record
+    idles the stream a few microseconds)."""
+    _check(lib().zkg_msm_phase_timing(1 if on else 0))
 
 
 def last_msm_stats():
