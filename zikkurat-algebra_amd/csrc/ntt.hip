@@ -134,16 +134,22 @@ __device__ __forceinline__ void zlds_store(int32_t *p, const Fz<Z> &a) {
 }
 
 // DIT butterfly (u, v) -> (u + w v, u - w v) for the stored twiddle w, whose true value is -w
-// when neg (the outputs then swap); w == nullptr is the twiddle 1
-template <class Z>
+// when neg (the outputs then swap); w == nullptr is the twiddle 1.  NORM = false leaves the
+// outputs as limb-wise sums / differences: the first stage of a radix-4 group takes normalised
+// inputs, so its outputs are sums or differences of two normalised values, which zmul accepts
+// against a normalised twiddle; the group's second stage (limbs below 4 * 2^29 < 2^31)
+// normalises before the store.
+template <bool NORM = true, class Z>
 __device__ __forceinline__ void zbutterfly(Fz<Z> &u, Fz<Z> &v, const Fz<Z> *w, bool neg) {
   Fz<Z> t, s0, s1;
   if (w) zmul(t, v, *w);
   else t = v;
   zadd(s0, u, t);
   zsub(s1, u, t);
-  znorm(s0);
-  znorm(s1);
+  if (NORM) {
+    znorm(s0);
+    znorm(s1);
+  }
   u = neg ? s1 : s0;
   v = neg ? s0 : s1;
 }
@@ -216,11 +222,11 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t blk, i
         if (lh > 0) {
           E w1;
           const bool n1 = twiddle_z(w1, a.tw, (uint64_t)j << (a.logn - lh - 1), a.logn, a.tw_shift, a.inverse);
-          zbutterfly(x0, x1, &w1, n1);
-          zbutterfly(x2, x3, &w1, n1);
+          zbutterfly<false>(x0, x1, &w1, n1);
+          zbutterfly<false>(x2, x3, &w1, n1);
         } else {
-          zbutterfly(x0, x1, (const E *)nullptr, false);
-          zbutterfly(x2, x3, (const E *)nullptr, false);
+          zbutterfly<false>(x0, x1, (const E *)nullptr, false);
+          zbutterfly<false>(x2, x3, (const E *)nullptr, false);
         }
         E w2;
         bool n2 = twiddle_z(w2, a.tw, (uint64_t)j << (a.logn - lh - 2), a.logn, a.tw_shift, a.inverse);
