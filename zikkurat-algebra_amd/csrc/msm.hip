@@ -1629,7 +1629,9 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
     if (items0 <= 2u * kTailThreads && items1 <= (uint32_t)kTailThreads && tail_lds <= kTailLdsBytes) {
       int levels = 0;
       while ((M >> levels) > 1) levels++;
-      const unsigned tb = std::min((std::max((items0 + 1) / 2, items1) + 63) & ~63u, (unsigned)kTailThreads);
+      // one item per thread at level 0 when that fits one workgroup (each round is one more
+      // addition of latency), else two
+      const unsigned tb = std::min((items0 + 63) & ~63u, (unsigned)kTailThreads);
       hipLaunchKernelGGL(msm_pyramid_tail_kernel<C>, dim3(1u, (unsigned)W), dim3(tb), tail_lds, st, in, bufs[k],
                          (uint32_t)W, M, A, M);
       in = bufs[k];
