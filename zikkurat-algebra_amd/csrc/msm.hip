@@ -951,7 +951,9 @@ __global__ __launch_bounds__(kThreads, PyrWaves<C>::v) void msm_pyramid_kernel(c
 // two pyramid levels in one pass: input [A][W][M] -> output [A+2][W][M/4] (M >= 4).  For
 // a < A the output is the sum of 4 consecutive inputs (two pair levels); array A is the odd
 // children of the first level reduced once more (B1 + B3), array A + 1 the odd children of the
-// second level (B2 + B3).  Half the launches and half the traffic of two single levels.
+// second level (B2 + B3).  Half the launches and half the traffic of two single levels.  The
+// B1 + B3 terms have threads of their own (the last W M/4 of the grid), so no thread runs
+// more than three additions, two of them independent.
 template <class C>
 __global__ __launch_bounds__(kThreads, PyrWaves<C>::v) void msm_pyramid2_kernel(const int32_t *__restrict__ in,
                                                                int32_t *__restrict__ out, uint32_t W, uint32_t M,
@@ -960,17 +962,19 @@ __global__ __launch_bounds__(kThreads, PyrWaves<C>::v) void msm_pyramid2_kernel(
   const uint32_t q = M >> 2;
   const uint64_t per = (uint64_t)W * q;
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= per * A) return;
+  if (t >= per * (A + 1)) return;
   const uint32_t a = (uint32_t)(t / per);
   const uint64_t rem = t % per;  // = w * q + i
-  const int32_t *src = in + ((uint64_t)a * W * M + 4 * rem) * PW;
   XyzzZ<C> x, y, p0;
-  if (a == 0) {
+  if (a == A) {  // odd children of array 0's first level, reduced once more
+    const int32_t *src = in + 4 * rem * PW;
     zxyzz_load(x, src + PW);
     zxyzz_load(y, src + 3 * PW);
     zxyzz_add(x, x, y);  // B1 + B3
     zxyzz_store(out + ((uint64_t)A * per + rem) * PW, x);
+    return;
   }
+  const int32_t *src = in + ((uint64_t)a * W * M + 4 * rem) * PW;
   zxyzz_load(x, src);
   zxyzz_load(y, src + PW);
   zxyzz_add(p0, x, y);  // B0 + B1
@@ -1648,7 +1652,7 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
     // dependent additions each); below, the levels of each Mb-point block in one launch (one
     // addition per thread per level, no launch between levels), or a level per launch
     if (M >= 4 && (uint64_t)A * W * (M / 4) >= kPyr2MinThreads) {
-      const uint64_t thr = (uint64_t)A * W * (M / 4);
+      const uint64_t thr = (uint64_t)(A + 1) * W * (M / 4);
       hipLaunchKernelGGL(msm_pyramid2_kernel<C>, dim3((unsigned)((thr + kThreads - 1) / kThreads)), blk, 0, st, in,
                          bufs[k], (uint32_t)W, M, A);
       A += 2;
