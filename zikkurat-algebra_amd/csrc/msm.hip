@@ -277,20 +277,9 @@ __global__ __launch_bounds__(kThreads) void scan_block_sums_kernel(const uint32_
   if (threadIdx.x == 0) bsums[blockIdx.x] = tot;
 }
 
-__global__ __launch_bounds__(kThreads) void scan_top_kernel(uint32_t *__restrict__ bsums, uint32_t nb) {
-  __shared__ uint32_t sh[kThreads];
-  uint32_t carry = 0;
-  for (uint32_t base = 0; base < nb; base += kThreads) {
-    const uint32_t i = base + threadIdx.x;
-    const uint32_t v = i < nb ? bsums[i] : 0u;
-    uint32_t tot;
-    const uint32_t ex = block_exclusive_scan(v, sh, &tot);
-    if (i < nb) bsums[i] = ex + carry;
-    carry += tot;
-  }
-}
-
-// out[i] = exclusive prefix of in; out[len] = total (cursor, when given, gets a copy)
+// out[i] = exclusive prefix of in; out[len] = total (cursor, when given, gets a copy).  bsums
+// holds the per-block totals of scan_block_sums_kernel; each block adds up the totals of the
+// blocks before it (a few hundred words at most), so no separate launch scans them.
 __global__ __launch_bounds__(kThreads) void scan_final_kernel(const uint32_t *__restrict__ in, size_t len,
                                                              const uint32_t *__restrict__ bsums,
                                                              uint32_t *__restrict__ out, uint32_t *__restrict__ cursor) {
@@ -302,8 +291,10 @@ __global__ __launch_bounds__(kThreads) void scan_final_kernel(const uint32_t *__
     v[k] = base + k < len ? in[base + k] : 0u;
     s += v[k];
   }
-  uint32_t tot;
-  uint32_t run = block_exclusive_scan(s, sh, &tot) + bsums[blockIdx.x];
+  uint32_t pre = 0, before, tot;
+  for (uint32_t i = threadIdx.x; i < blockIdx.x; i += kThreads) pre += bsums[i];
+  (void)block_exclusive_scan(pre, sh, &before);
+  uint32_t run = block_exclusive_scan(s, sh, &tot) + before;
   for (int k = 0; k < kScanPer; k++) {
     if (base + k < len) {
       out[base + k] = run;
@@ -640,13 +631,39 @@ __global__ __launch_bounds__(kThreads) void msm_big_scan_kernel(const uint32_t *
   }
 }
 
+// Buckets whose runs span more than `limit` segments get pairwise trees (heavy[0] = count,
+// heavy[1] = most partials, heavy[2] = sorted entries, heavy[3..] = buckets).  Known from the
+// offsets alone, so the last sort kernel finds them and the three counts reach the host (one
+// copy) while the accumulation runs.
+__device__ __forceinline__ void heavy_detect(const uint32_t *__restrict__ off, uint32_t nkeys, uint32_t b,
+                                             int seg_log, uint32_t *__restrict__ heavy, uint32_t heavy_cap,
+                                             uint32_t limit) {
+  if (b == 0) heavy[2] = off[nkeys];
+  const uint32_t s = off[b], t = off[b + 1];
+  if (s == t) return;
+  const uint64_t k = ((uint64_t)(t - 1) >> seg_log) - (s >> seg_log);
+  if (k > limit) {
+    const uint32_t idx = atomicAdd(&heavy[0], 1u);
+    atomicMax(&heavy[1], (uint32_t)k);
+    if (idx < heavy_cap) heavy[3 + idx] = b;
+  }
+}
+
+// Places the big partitions' entries run by run; every bucket offset is final by now (the
+// partition sort and msm_big_scan_kernel wrote them), so the grid also runs heavy_detect over
+// all buckets first.
 template <class T>
 __global__ __launch_bounds__(kThreads) void msm_big_place_kernel(const T *__restrict__ tmp,
                                                                  const uint32_t *__restrict__ ppos, uint32_t nchunks,
                                                                  int L, int ib, const uint32_t *__restrict__ big,
                                                                  const uint32_t *__restrict__ rcnt,
-                                                                 uint32_t *__restrict__ vals) {
+                                                                 uint32_t *__restrict__ vals,
+                                                                 const uint32_t *__restrict__ off, uint32_t nkeys,
+                                                                 int seg_log, uint32_t *__restrict__ heavy,
+                                                                 uint32_t heavy_cap, uint32_t limit) {
   extern __shared__ uint32_t lc[];
+  for (uint32_t b = blockIdx.x * kThreads + threadIdx.x; b < nkeys; b += gridDim.x * kThreads)
+    heavy_detect(off, nkeys, b, seg_log, heavy, heavy_cap, limit);
   const uint32_t NL = 1u << L, t = threadIdx.x;
   const uint32_t items = big[0] * nchunks;
   for (uint32_t it = blockIdx.x; it < items; it += gridDim.x) {
@@ -799,26 +816,6 @@ __global__ __launch_bounds__(kThreads) void msm_accum_kernel(const uint32_t *__r
 
 // ---------------------------------------------------------------------------------------
 // 6. continuation partials
-
-// buckets whose runs span more than `limit` segments get pairwise trees (heavy[0] = count,
-// heavy[1] = most partials, heavy[2] = sorted entries, heavy[3..] = buckets); known from the
-// offsets alone, so this runs before the accumulation and the three counts reach the host (one
-// copy) while the accumulation runs
-__global__ __launch_bounds__(kThreads) void msm_heavy_detect_kernel(const uint32_t *__restrict__ off, uint32_t nkeys,
-                                                                   int seg_log, uint32_t *__restrict__ heavy,
-                                                                   uint32_t heavy_cap, uint32_t limit) {
-  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nkeys) return;
-  if (b == 0) heavy[2] = off[nkeys];
-  const uint32_t s = off[b], t = off[b + 1];
-  if (s == t) return;
-  const uint64_t k = ((uint64_t)(t - 1) >> seg_log) - (s >> seg_log);
-  if (k > limit) {
-    const uint32_t idx = atomicAdd(&heavy[0], 1u);
-    atomicMax(&heavy[1], (uint32_t)k);
-    if (idx < heavy_cap) heavy[3 + idx] = b;
-  }
-}
 
 template <class C>
 __global__ __launch_bounds__(kThreads) void msm_fix_kernel(const uint32_t *__restrict__ off, uint32_t nkeys,
@@ -1527,9 +1524,9 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
                      np, L, P, phist.as<uint32_t>(), heavy.as<uint32_t>(), big.as<uint32_t>());
   hipLaunchKernelGGL(scan_block_sums_kernel, dim3(nsb), blk, 0, st, phist.as<uint32_t>(), (size_t)nph,
                      scan_b.as<uint32_t>());
-  hipLaunchKernelGGL(scan_top_kernel, dim3(1), blk, 0, st, scan_b.as<uint32_t>(), nsb);
   hipLaunchKernelGGL(scan_final_kernel, dim3(nsb), blk, 0, st, phist.as<uint32_t>(), (size_t)nph,
                      scan_b.as<uint32_t>(), ppos.as<uint32_t>(), (uint32_t *)nullptr);
+  const uint32_t limit = (user && user->fix_limit > 0) ? (uint32_t)user->fix_limit : 16u;
   const size_t cnt_bytes = ((size_t)1 << std::max(L, 1)) * 4;
   // level 2: partitions of at most kBigPart words, one workgroup each; larger ones run by run
   auto level2 = [&](auto word) {
@@ -1544,7 +1541,8 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
     hipLaunchKernelGGL(msm_big_scan_kernel, dim3((unsigned)std::min<uint64_t>(nbig_cap, 256)), blk, 0, st,
                        ppos.as<uint32_t>(), nchunks, L, big.as<uint32_t>(), rcnt.as<uint32_t>(), off.as<uint32_t>());
     hipLaunchKernelGGL(msm_big_place_kernel<T>, dim3(gb), blk, cnt_bytes, st, tmp.as<T>(), ppos.as<uint32_t>(),
-                       nchunks, L, ib, big.as<uint32_t>(), rcnt.as<uint32_t>(), vals.as<uint32_t>());
+                       nchunks, L, ib, big.as<uint32_t>(), rcnt.as<uint32_t>(), vals.as<uint32_t>(),
+                       off.as<uint32_t>(), (uint32_t)nkeys, seg_log, heavy.as<uint32_t>(), heavy_cap, limit);
   };
   if (narrow) {
     hipLaunchKernelGGL(msm_part_scatter_staged_kernel, gch, dim3(kSortThreads), kScatterStageLds, st,
@@ -1556,10 +1554,6 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
                        dig.as<uint32_t>(), np, L, P, ib, ppos.as<uint32_t>(), tmp.as<uint64_t>());
     level2(uint64_t(0));
   }
-  ZK_HIP(hipGetLastError());
-  const uint32_t limit = (user && user->fix_limit > 0) ? (uint32_t)user->fix_limit : 16u;
-  hipLaunchKernelGGL(msm_heavy_detect_kernel, dim3((unsigned)((nkeys + kThreads - 1) / kThreads)), blk, 0, st,
-                     off.as<uint32_t>(), (uint32_t)nkeys, seg_log, heavy.as<uint32_t>(), heavy_cap, limit);
   ZK_HIP(hipGetLastError());
   // heavy count, most partials and sorted entries, read back while the accumulation runs
   StageLease lease;
@@ -1758,7 +1752,6 @@ static int msm_sparse_run(int n, const uint64_t *d_scalars, int enl, int mont, c
                      keep.as<uint32_t>(), ones.as<uint8_t>());
   hipLaunchKernelGGL(scan_block_sums_kernel, dim3(nsb), blk, 0, st, keep.as<uint32_t>(), (size_t)n,
                      scan_b.as<uint32_t>());
-  hipLaunchKernelGGL(scan_top_kernel, dim3(1), blk, 0, st, scan_b.as<uint32_t>(), nsb);
   hipLaunchKernelGGL(scan_final_kernel, dim3(nsb), blk, 0, st, keep.as<uint32_t>(), (size_t)n,
                      scan_b.as<uint32_t>(), pos.as<uint32_t>(), (uint32_t *)nullptr);
   hipLaunchKernelGGL(msm_sparse_gather_kernel, g, blk, 0, st, d_scalars, enl, (const uint32_t *)d_points, pw,
