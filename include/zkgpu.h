@@ -64,6 +64,23 @@ void bls12_381_G1_jac_MSM_mont_coeff_jac_out(int npoints, const uint64_t *expos,
 void bls12_381_G1_jac_MSM_std_coeff_affine_out(int npoints, const uint64_t *expos, const uint64_t *grps, uint64_t *tgt, int expo_nlimbs);
 void bls12_381_G1_jac_MSM_mont_coeff_affine_out(int npoints, const uint64_t *expos, const uint64_t *grps, uint64_t *tgt, int expo_nlimbs);
 
+/* The reference's slow MSM, sum_i scl(expos[i], grps[i]) one point at a time, its own test
+ * oracle.  Same contract as *_MSM_std_coeff_proj_out / *_jac_out (normalised output); here it
+ * runs the same bucket MSM.  The .c files define the *_proj_out_ / *_jac_out_ names, the
+ * headers declare *_projc_out_ / *_jacc_out_ (never defined there); both are exported. */
+/* impl bn128_G1_proj.c:611-620 (declared bn128_G1_proj.h:50 as ..._projc_out_slow_reference) */
+void bn128_G1_proj_MSM_std_coeff_proj_out_slow_reference(int npoints, const uint64_t *expos, const uint64_t *grps, uint64_t *tgt, int expo_nlimbs);
+void bn128_G1_proj_MSM_std_coeff_projc_out_slow_reference(int npoints, const uint64_t *expos, const uint64_t *grps, uint64_t *tgt, int expo_nlimbs);
+/* impl bn128_G1_jac.c:660-669 (declared bn128_G1_jac.h:50 as ..._jacc_out_slow_reference) */
+void bn128_G1_jac_MSM_std_coeff_jac_out_slow_reference(int npoints, const uint64_t *expos, const uint64_t *grps, uint64_t *tgt, int expo_nlimbs);
+void bn128_G1_jac_MSM_std_coeff_jacc_out_slow_reference(int npoints, const uint64_t *expos, const uint64_t *grps, uint64_t *tgt, int expo_nlimbs);
+/* impl bls12_381_G1_proj.c:612-621 (declared bls12_381_G1_proj.h:50) */
+void bls12_381_G1_proj_MSM_std_coeff_proj_out_slow_reference(int npoints, const uint64_t *expos, const uint64_t *grps, uint64_t *tgt, int expo_nlimbs);
+void bls12_381_G1_proj_MSM_std_coeff_projc_out_slow_reference(int npoints, const uint64_t *expos, const uint64_t *grps, uint64_t *tgt, int expo_nlimbs);
+/* impl bls12_381_G1_jac.c:660-669 (declared bls12_381_G1_jac.h:50) */
+void bls12_381_G1_jac_MSM_std_coeff_jac_out_slow_reference(int npoints, const uint64_t *expos, const uint64_t *grps, uint64_t *tgt, int expo_nlimbs);
+void bls12_381_G1_jac_MSM_std_coeff_jacc_out_slow_reference(int npoints, const uint64_t *expos, const uint64_t *grps, uint64_t *tgt, int expo_nlimbs);
+
 /* NTT over Fr: src and tgt are 2^m Montgomery Fr elements (4 u64 each); gen is the
  * Montgomery-form generator of the size-2^m subgroup (the *forward* one for the inverse too);
  * eta is the Montgomery-form coset shift.  src may equal tgt. */

@@ -5,7 +5,8 @@ oracle/_ref/libzkref.so; this script calls its entry points on seeded inputs and
 and outputs (small cases) or output SHA-256 digests (large cases, inputs regenerated from the
 seed by tests/zkmath.py:seeded_fr_vector).  Run from the repository root:
 
-    make -C oracle ref && python tests/golden/gen_golden.py
+    make -C oracle ref && python tests/golden/gen_golden.py            # everything
+    make -C oracle ref && python tests/golden/gen_golden.py --large    # only missing large entries
 """
 from __future__ import annotations
 
@@ -172,26 +173,78 @@ def main():
         out[key + "_proof"] = R.msm(q, srs[:15], mont=True)
         manifest["kzg"].append(key)
         # large NTTs: output digests only (inputs from seeded_fr_vector, taken as Montgomery words)
-        for m in (16, 20):
-            X = seeded_fr_vector(9000 + m + 100 * C.id, 1 << m, C.r)
-            g = np.array(to_limbs(C.fr_to_mont(C.fft_root(m)), 4), np.uint64)
-            manifest["large"].append({"curve": C.name, "kind": "ntt_fwd", "m": m, "seed": 9000 + m + 100 * C.id,
-                                      "sha256": sha(R.ntt(m, g, X)), "inv_sha256": sha(R.ntt(m, g, X, inverse=True))})
+        for m in LARGE_NTT_LOGS:
+            manifest["large"].append(large_ntt_entry(C, m))
         print(C.name, "done", flush=True)
     # large MSM: reference output for points k_i G and scalars s_i from seeded vectors
-    for C, logn in ((BN, 14), (BN, 16), (BLS, 14)):
-        n = 1 << logn
-        kseed, sseed = 11000 + logn + 100 * C.id, 12000 + logn + 100 * C.id
-        K = seeded_fr_vector(kseed, n, C.r)
-        S = seeded_fr_vector(sseed, n, C.r)
-        P = Oracle(C).points_from_scalars([int(sum(int(w) << (64 * i) for i, w in enumerate(row))) for row in K])
-        res = Ref(C).msm(S, P)
-        manifest["large"].append({"curve": C.name, "kind": "msm_std_affine", "logn": logn, "kseed": kseed,
-                                  "sseed": sseed, "result": [int(x) for x in res], "points_sha256": sha(P)})
+    for C, logn in LARGE_MSMS:
+        manifest["large"].append(large_msm_entry(C, logn))
         print(C.name, "msm", logn, flush=True)
     np.savez_compressed(HERE / "golden_small.npz", **out)
     (HERE / "manifest.json").write_text(json.dumps(manifest, indent=1))
 
 
+# digests of the reference NTT at these sizes (2^22: the BN254 forward + inverse workload)
+LARGE_NTT_LOGS = (16, 20, 22)
+# reference MSM results at these sizes (BN254 2^16: the plain single-MSM workload; 2^20: the
+# BN254 and BLS12-381 MSMs of the headline and of the BLS12-381 timing workload)
+LARGE_MSMS = ((BN, 14), (BN, 16), (BLS, 14), (BN, 20), (BLS, 20))
+
+
+def large_ntt_entry(C, m):
+    R = Ref(C)
+    X = seeded_fr_vector(9000 + m + 100 * C.id, 1 << m, C.r)
+    g = np.array(to_limbs(C.fr_to_mont(C.fft_root(m)), 4), np.uint64)
+    return {"curve": C.name, "kind": "ntt_fwd", "m": m, "seed": 9000 + m + 100 * C.id,
+            "sha256": sha(R.ntt(m, g, X)), "inv_sha256": sha(R.ntt(m, g, X, inverse=True))}
+
+
+def _points_chunk(args):
+    cid, rows = args
+    C = BN if cid == BN.id else BLS
+    return Oracle(C).points_from_scalars([int.from_bytes(r.tobytes(), "little") for r in rows])
+
+
+def large_msm_entry(C, logn, workers=8):
+    """Points k_i G (the oracle's scalar multiplication, in worker processes) and the reference's
+    MSM over them; the points are stored as a digest (the tests rebuild them on the device)."""
+    import multiprocessing as mp
+    n = 1 << logn
+    kseed, sseed = 11000 + logn + 100 * C.id, 12000 + logn + 100 * C.id
+    K = seeded_fr_vector(kseed, n, C.r)
+    S = seeded_fr_vector(sseed, n, C.r)
+    chunks = [(C.id, K[i:i + 4096]) for i in range(0, n, 4096)]
+    if len(chunks) > 1:
+        with mp.get_context("fork").Pool(workers) as pool:
+            P = np.concatenate(pool.map(_points_chunk, chunks))
+    else:
+        P = _points_chunk(chunks[0])
+    res = Ref(C).msm(S, P)
+    return {"curve": C.name, "kind": "msm_std_affine", "logn": logn, "kseed": kseed, "sseed": sseed,
+            "result": [int(x) for x in res], "points_sha256": sha(P)}
+
+
+def add_missing_large():
+    """Adds the large entries the manifest lacks (same seeds and rules as a full run), leaving the
+    small fixtures untouched."""
+    assert ref() is not None, "build the reference first: make -C oracle ref"
+    path = HERE / "manifest.json"
+    manifest = json.loads(path.read_text())
+    have = {(e["curve"], e["kind"], e.get("m", e.get("logn"))) for e in manifest["large"]}
+    for C in (BN, BLS):
+        for m in LARGE_NTT_LOGS:
+            if (C.name, "ntt_fwd", m) not in have:
+                manifest["large"].append(large_ntt_entry(C, m))
+                print(C.name, "ntt", m, flush=True)
+    for C, logn in LARGE_MSMS:
+        if (C.name, "msm_std_affine", logn) not in have:
+            manifest["large"].append(large_msm_entry(C, logn))
+            print(C.name, "msm", logn, flush=True)
+    path.write_text(json.dumps(manifest, indent=1))
+
+
 if __name__ == "__main__":
-    main()
+    if "--large" in sys.argv[1:]:
+        add_missing_large()
+    else:
+        main()

@@ -332,6 +332,19 @@ MSM_SYMBOL(bls12_381_G1_jac_MSM_mont_coeff_jac_out, 1, ZKG_MSM_JAC_OUT | ZKG_MSM
 MSM_SYMBOL(bls12_381_G1_jac_MSM_std_coeff_affine_out, 1, 0)
 MSM_SYMBOL(bls12_381_G1_jac_MSM_mont_coeff_affine_out, 1, ZKG_MSM_MONT)
 
+// The reference's own slow MSM (one windowed scalar multiplication per point, summed; its test
+// oracle): same contract as the proj/jac output forms above, so it runs the same MSM here.  The
+// .c files define the *_proj_out_ / *_jac_out_ names; the headers declare *_projc_out_ /
+// *_jacc_out_, which the reference never defines, so both spellings are exported.
+MSM_SYMBOL(bn128_G1_proj_MSM_std_coeff_proj_out_slow_reference, 0, ZKG_MSM_PROJ_OUT)
+MSM_SYMBOL(bn128_G1_proj_MSM_std_coeff_projc_out_slow_reference, 0, ZKG_MSM_PROJ_OUT)
+MSM_SYMBOL(bn128_G1_jac_MSM_std_coeff_jac_out_slow_reference, 0, ZKG_MSM_JAC_OUT)
+MSM_SYMBOL(bn128_G1_jac_MSM_std_coeff_jacc_out_slow_reference, 0, ZKG_MSM_JAC_OUT)
+MSM_SYMBOL(bls12_381_G1_proj_MSM_std_coeff_proj_out_slow_reference, 1, ZKG_MSM_PROJ_OUT)
+MSM_SYMBOL(bls12_381_G1_proj_MSM_std_coeff_projc_out_slow_reference, 1, ZKG_MSM_PROJ_OUT)
+MSM_SYMBOL(bls12_381_G1_jac_MSM_std_coeff_jac_out_slow_reference, 1, ZKG_MSM_JAC_OUT)
+MSM_SYMBOL(bls12_381_G1_jac_MSM_std_coeff_jacc_out_slow_reference, 1, ZKG_MSM_JAC_OUT)
+
 // ---------------- reference G1 FFT symbols ----------------
 #define G1_FFT_SYMBOLS(PFX, CID, CO, JAC)                                                              \
   void PFX##_G1_##CO##_fft_forward(int m, const uint64_t *gen, const uint64_t *src, uint64_t *tgt) {   \

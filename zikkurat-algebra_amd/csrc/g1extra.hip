@@ -299,6 +299,7 @@ static int binary_sum_run(int nrows, int n, const uint8_t *d_bits, const uint64_
   using F = typename C::Fp;
   constexpr int PW = ZLayout<C>::XYZZ;
   DevBuf prep, part, res;
+  StreamSyncGuard drain(st);  // the buffers outlive the queued kernels on every return
   const int32_t *pts = (const int32_t *)d_points;
   if (!points_internal) {
     ZK_TRY(prep.alloc((size_t)n * msm_internal_point_bytes(C::id)));
@@ -326,6 +327,7 @@ static int binary_sum_run(int nrows, int n, const uint8_t *d_bits, const uint64_
   std::vector<uint32_t> w((size_t)nrows * 4 * F::N);
   ZK_HIP(hipMemcpyAsync(w.data(), res.p, w.size() * 4, hipMemcpyDeviceToHost, st));
   ZK_HIP(hipStreamSynchronize(st));
+  drain.release();
   for (int r = 0; r < nrows; r++) {
     MsmResult &o = out[r];
     memset(&o, 0, sizeof o);

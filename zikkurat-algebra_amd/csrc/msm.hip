@@ -1478,7 +1478,9 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   const uint32_t nchunks = (uint32_t)(((uint64_t)np + kPChunk - 1) / kPChunk);
   const uint64_t nph = (uint64_t)W * P * nchunks;  // level-1 counts, scanned in place of positions
 
-  DevBuf dig, vals, tmp, phist, ppos, big, rcnt, off, bsum, cont, heavy, pyrA, pyrB, scan_b, zpts, outx;
+  // every buffer the queued kernels touch lives until the stream is drained (the guard below)
+  DevBuf dig, vals, tmp, phist, ppos, big, rcnt, off, bsum, cont, heavy, pyrA, pyrB, scan_b, zpts, outx, own;
+  StreamSyncGuard drain(st);
   ZK_TRY(dig.alloc(total * 4));
   ZK_TRY(vals.alloc(total * 4));
   // level-1 words are 32-bit when the point index, the sign and the low bucket bits fit
@@ -1591,7 +1593,6 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
     const uint32_t nh = hh[0], kmax = hh[1];
     // segments holding sorted entries; every continuation partial lies below this bound
     const uint32_t nseg_used = (uint32_t)(((uint64_t)nent + (1ull << seg_log) - 1) >> seg_log);
-    DevBuf own;
     for (int r = 0; (1ull << r) < kmax; r++) {
       const uint32_t pairs = (uint32_t)((kmax + (2ull << r) - 1) >> (r + 1));
       const uint64_t thr = (uint64_t)nh * pairs;
@@ -1681,6 +1682,7 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   const uint32_t *res = stage.res;
   ZK_HIP(hipMemcpyAsync(stage.res, outx.p, (size_t)nres * PW * 4, hipMemcpyDeviceToHost, st));
   ZK_HIP(hipStreamSynchronize(st));
+  drain.release();
   ev.mark(st);
   auto t_host1 = std::chrono::steady_clock::now();
 
@@ -1739,6 +1741,9 @@ static int msm_sparse_run(int n, const uint64_t *d_scalars, int enl, int mont, c
   const int pw = points_internal ? ZLayout<C>::AFF : 2 * F::N;  // u32 words per point record
   const uint64_t k = (uint64_t)n - stats.n_zero - stats.n_one;   // scalars outside {0, 1}
   DevBuf keep, pos, ones, scan_b, sc2, pts2;
+  // drains the stream on every return: with no dense part (k = 0) nothing else waits for the
+  // gather kernels that read these buffers
+  StreamSyncGuard drain(st);
   ZK_TRY(keep.alloc((uint64_t)n * 4));
   ZK_TRY(pos.alloc(((uint64_t)n + 1) * 4));
   ZK_TRY(ones.alloc((uint64_t)n));

@@ -513,23 +513,34 @@ static void last_factor(uint32_t out[8], int curve, int m, bool inverse) {
   internal_words<F>(out, v);
 }
 
+// tab[j] = g^j R' (internal form, canonical) for j < cnt, through lo (2048 entries) and hi
+// (ceil(cnt / 2048) entries) powers
+template <class F>
+static int launch_twiddles(uint32_t *tab, uint32_t *lo, uint32_t *hi, uint64_t cnt, const zkh::Fe &g,
+                           hipStream_t st) {
+  const uint64_t hicnt = (cnt + 2047) / 2048;
+  Fe<F> gd = to_dev<F>(g);
+  hipLaunchKernelGGL(pow_table_kernel<F>, dim3(2048 / 256), dim3(256), 0, st, lo, gd, 1ull, 2048);
+  hipLaunchKernelGGL(pow_table_kernel<F>, dim3((hicnt + 255) / 256), dim3(256), 0, st, hi, gd, 2048ull,
+                     (int)hicnt);
+  hipLaunchKernelGGL(twiddle_table_kernel<F>, dim3((cnt + 255) / 256), dim3(256), 0, st, tab,
+                     (const uint32_t *)lo, (const uint32_t *)hi, cnt);
+  ZK_HIP(hipGetLastError());
+  return tab_to_internal<F>(tab, cnt, st);  // the pass kernels read w R'
+}
+
 template <class F>
 static int build_twiddles(int curve, int m, const zkh::Fe &g, DevBuf &buf, hipStream_t st) {
   const uint64_t cnt = m >= 1 ? (1ull << (m - 1)) : 1;
   ZK_TRY(buf.alloc(cnt * F::N * 4));
   DevBuf lo, hi;
+  StreamSyncGuard drain(st);  // lo/hi outlive the queued table kernels on every return
   ZK_TRY(lo.alloc(2048 * F::N * 4));
   const uint64_t hicnt = (cnt + 2047) / 2048;
   ZK_TRY(hi.alloc(hicnt * F::N * 4));
-  Fe<F> gd = to_dev<F>(g);
-  hipLaunchKernelGGL(pow_table_kernel<F>, dim3(2048 / 256), dim3(256), 0, st, lo.as<uint32_t>(), gd, 1ull, 2048);
-  hipLaunchKernelGGL(pow_table_kernel<F>, dim3((hicnt + 255) / 256), dim3(256), 0, st, hi.as<uint32_t>(), gd,
-                     2048ull, (int)hicnt);
-  hipLaunchKernelGGL(twiddle_table_kernel<F>, dim3((cnt + 255) / 256), dim3(256), 0, st, buf.as<uint32_t>(),
-                     lo.as<uint32_t>(), hi.as<uint32_t>(), cnt);
-  ZK_HIP(hipGetLastError());
-  ZK_TRY(tab_to_internal<F>(buf.as<uint32_t>(), cnt, st));  // the pass kernels read w R'
+  ZK_TRY(launch_twiddles<F>(buf.as<uint32_t>(), lo.as<uint32_t>(), hi.as<uint32_t>(), cnt, g, st));
   ZK_HIP(hipStreamSynchronize(st));  // lo/hi are released on return
+  drain.release();
   (void)curve;
   return ZK_OK;
 }
@@ -670,6 +681,7 @@ static int ntt_network(int curve, int m, const uint64_t *gen, const uint32_t *d_
   const uint64_t N = 1ull << m;
   const uint64_t half = N >> 1;
   DevBuf gp;
+  StreamSyncGuard drain(st);  // gp outlives the queued stages on every return
   ZK_TRY(gp.alloc((half ? half : 1) * F::N * 4));
   zkh::Fe g = zkh::load(f, gen);
   zkh::Fe first = zkh::one(f);
@@ -710,6 +722,7 @@ static int ntt_network(int curve, int m, const uint64_t *gen, const uint32_t *d_
   }
   ZK_HIP(hipGetLastError());
   ZK_HIP(hipStreamSynchronize(st));  // gp is released on return
+  drain.release();
   return ZK_OK;
 }
 
@@ -748,7 +761,8 @@ static int ntt_device_t(int curve, int m, const uint64_t *gen, const uint64_t *e
     if (d_in != d_out) ZK_HIP(hipMemcpyAsync(d_out, d_in, F::N * 4, hipMemcpyDeviceToDevice, st));
     return ZK_OK;
   }
-  DevBuf tmp;
+  DevBuf tmp, alias, etab;
+  StreamSyncGuard drain(st);  // the buffers outlive the queued kernels on every return
   ZK_TRY(tmp.alloc(N * F::N * 4));
   const zkh::FieldDesc &f = fr_desc(curve);
   const bool ok = ntt_generator_ok(curve, m, gen);
@@ -759,7 +773,6 @@ static int ntt_device_t(int curve, int m, const uint64_t *gen, const uint64_t *e
       set_error("shifted NTT with a generator of order != 2^m is not supported");
       return ZK_ENOTROOT;
     }
-    DevBuf alias;
     if (d_in == d_out) {
       ZK_TRY(alias.alloc(N * F::N * 4));
       ZK_HIP(hipMemcpyAsync(alias.p, d_in, N * F::N * 4, hipMemcpyDeviceToDevice, st));
@@ -770,7 +783,6 @@ static int ntt_device_t(int curve, int m, const uint64_t *gen, const uint64_t *e
   const uint32_t *tw = nullptr;
   int tw_shift = 0;
   if (m >= 1) ZK_TRY(get_twiddles<F>(curve, m, gen, &tw, &tw_shift, st));
-  DevBuf etab;
   if (eta) ZK_TRY(build_eta_tables<F>(curve, zkh::load(f, eta), etab, st));
   uint32_t lastf[8];
   last_factor<F>(lastf, curve, m, inverse);
@@ -780,8 +792,8 @@ static int ntt_device_t(int curve, int m, const uint64_t *gen, const uint64_t *e
   const uint32_t *eo = (eta && inverse) ? etab.as<uint32_t>() : nullptr;
   ZK_TRY(ntt_fast<F>(plan, d_in, d_out, tmp.as<uint32_t>(), tw, tw_shift, inverse, ep, eo,
                      lastf, st));
-  if (eta) ZK_HIP(hipStreamSynchronize(st));  // eta tables are released on return
-  ZK_HIP(hipStreamSynchronize(st));           // tmp is released on return
+  ZK_HIP(hipStreamSynchronize(st));  // tmp and the eta tables are released on return
+  drain.release();
   return ZK_OK;
 }
 

@@ -80,6 +80,22 @@ struct DevBuf {
   T *as() const { return reinterpret_cast<T *>(p); }
 };
 
+// Synchronises a stream when it leaves scope unless released.  dev_free returns a block to the
+// cache at once, with no record of the stream still using it, so no DevBuf may be destroyed
+// while work queued on a stream still reads or writes it: a function that queues work and can
+// return early declares this guard after its DevBufs (destructors run in reverse order, so the
+// stream is drained before any of them is freed) and releases it after its own final sync.
+struct StreamSyncGuard {
+  hipStream_t s;
+  explicit StreamSyncGuard(hipStream_t st) : s(st) {}
+  StreamSyncGuard(const StreamSyncGuard &) = delete;
+  StreamSyncGuard &operator=(const StreamSyncGuard &) = delete;
+  ~StreamSyncGuard() {
+    if (s) (void)hipStreamSynchronize(s);
+  }
+  void release() { s = nullptr; }
+};
+
 // Host -> device / device -> host copies of caller (pageable) memory through a pinned
 // staging ring, on the library stream.  Synchronous with respect to the host on return.
 int copy_h2d(void *dst, const void *src, size_t bytes, hipStream_t s);

@@ -224,6 +224,19 @@ def msm_ref_symbol(curve, coord, kind, out, scalars, points, expo_nlimbs=4):
     return res
 
 
+def msm_slow_reference(curve, coord, scalars, points, expo_nlimbs=4, header_spelling=False):
+    """<c>_G1_<coord>_MSM_std_coeff_<coord>_out_slow_reference (bn128_G1_proj.c:611-620), or the
+    headers' spelling <coord>c_out (bn128_G1_proj.h:50); normalised proj/jac output words."""
+    scalars, points = _c(scalars), _c(points)
+    n = points.shape[0] if points.ndim == 2 else points.size // (2 * FP_LIMBS[curve])
+    res = np.zeros(3 * FP_LIMBS[curve], np.uint64)
+    out = coord + ("c" if header_spelling else "")
+    f = _ref_fn(curve, f"G1_{coord}_MSM_std_coeff_{out}_out_slow_reference")
+    f.argtypes = [ctypes.c_int, u64p, u64p, u64p, ctypes.c_int]
+    f(n, _p(scalars), _p(points), _p(res), expo_nlimbs)
+    return res
+
+
 def msm_std(curve, scalars, points, expo_nlimbs=4):
     """Sum s_i P_i (standard-form scalars), affine output; infinity = all 0xFF words."""
     scalars, points = _c(scalars), _c(points)
