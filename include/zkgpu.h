@@ -242,6 +242,14 @@ int zkg_msm_prepare_points(int curve, int npoints, const uint64_t *d_grps, void 
  * gen/eta are host pointers. */
 int zkg_ntt(int curve, int m, const uint64_t *gen, const uint64_t *eta, int flags, const uint64_t *d_src,
             uint64_t *d_tgt);
+/* zkg_ntt without any device allocation.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Scratch (pass vector, twiddles rebuilt on every call,
+ * coset tables) lives in one fixed file-scope device array of 12.3 MiB per GPU, so m <= 18
+ * (status 1 above), and the generator must have order 2^m (status 4 otherwise).  Every
+ * call on a GPU shares that array: calls that overlap on one GPU race.  Synchronous. */
+int zkg_ntt_noalloc(int curve, int m, const uint64_t *gen, const uint64_t *eta, int flags, const uint64_t *d_src,
+                    uint64_t *d_tgt);
 /* Batched NTTs of uneven sizes over the standard FFT domains (generator of order 2^m from
  * lib/src/ZK/Algebra/Class/FFT.hs:60-68).  create() builds twiddle/coset tables and descriptor
  * tables and captures one launch per pass round into a hipGraph; run() replays it

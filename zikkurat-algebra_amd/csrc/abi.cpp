@@ -702,6 +702,14 @@ int zkg_ntt(int curve, int m, const uint64_t *gen, const uint64_t *eta, int flag
   return ntt_device(curve, m, gen, eta, flags & 1, d_src, d_tgt, ctx->stream, nullptr);
 }
 
+int zkg_ntt_noalloc(int curve, int m, const uint64_t *gen, const uint64_t *eta, int flags, const uint64_t *d_src,
+                    uint64_t *d_tgt) {
+  Context *ctx;
+  ZK_TRY(with_ctx(&ctx));
+  std::lock_guard<std::recursive_mutex> lk(ctx->lock);
+  return ntt_device_noalloc(curve, m, gen, eta, flags & 1, d_src, d_tgt, ctx->stream);
+}
+
 int zkg_ntt_batch_create(int curve, int njobs, const zkg_ntt_job *jobs, void **plan) {
   Context *ctx;
   ZK_TRY(with_ctx(&ctx));

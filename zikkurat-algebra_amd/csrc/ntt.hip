@@ -1098,4 +1098,96 @@ int ntt_device(int curve, int m, const uint64_t *gen, const uint64_t *eta, int f
   return ntt_device_t<Bls381Fr>(curve, m, gen, eta, flags, (const uint32_t *)d_in, (uint32_t *)d_out, st, tune);
 }
 
+// ---------------------------------------------------------------------------------------
+// NTT without allocation, for callers that cannot allocate device memory.  The pass scratch
+// vector, the twiddle table (rebuilt on every call, never cached) and the coset tables live in
+// one file-scope device array, `scratch`, of a fixed size: transforms of at most 2^18 elements
+// (12.3 MiB, held on every GPU the library runs on until the process ends).  Every call on a
+// GPU shares that GPU's array and leaves the last call's contents in it, so calls that overlap
+// on one GPU race.  Generators of order 2^m only (the fast path): others would need the
+// reference network's powers table beside an aliasing copy.
+
+static constexpr int kNoAllocMaxLog = 18;
+static constexpr size_t kNaWords = 8;  // u32 words per stored Fr element
+static constexpr size_t kNaTmp = 0;                                                  // 2^18 elements
+static constexpr size_t kNaTw = kNaTmp + (kNaWords << kNoAllocMaxLog);               // 2^17 twiddles
+static constexpr size_t kNaLo = kNaTw + (kNaWords << (kNoAllocMaxLog - 1));          // 2048 powers
+static constexpr size_t kNaHi = kNaLo + kNaWords * 2048;                             // 64 powers
+static constexpr size_t kNaEta = kNaHi + kNaWords * ((1u << (kNoAllocMaxLog - 1)) / 2048);  // 3 x 2048
+static constexpr size_t kNaTotal = kNaEta + kNaWords * 3 * 2048;
+static_assert(kNaTw % 64 == 0 && kNaLo % 64 == 0 && kNaHi % 64 == 0 && kNaEta % 64 == 0,
+              "scratch regions stay 256-byte aligned");
+
+static __device__ __align__(256) uint32_t scratch[kNaTotal];
+
+// device address of `scratch` on each device (the host-side name is not a device address)
+static std::mutex g_scratch_mu;
+static std::vector<uint32_t *> g_scratch_ptr;
+
+static int scratch_on_current_device(uint32_t **out) {
+  int dev = 0;
+  ZK_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((size_t)dev >= g_scratch_ptr.size()) g_scratch_ptr.resize((size_t)dev + 1, nullptr);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    ZK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) {
+      set_error("ntt_noalloc: no device address for the scratch array");
+      return ZK_EHIP;
+    }
+    g_scratch_ptr[dev] = (uint32_t *)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return ZK_OK;
+}
+
+template <class F>
+static int ntt_noalloc_t(int curve, int m, const uint64_t *gen, const uint64_t *eta, int flags, const uint32_t *d_in,
+                         uint32_t *d_out, hipStream_t st, uint32_t *scr) {
+  const bool inverse = flags & NTT_INVERSE;
+  if (m == 0 && !eta) {
+    if (d_in != d_out) ZK_HIP(hipMemcpyAsync(d_out, d_in, F::N * 4, hipMemcpyDeviceToDevice, st));
+    ZK_HIP(hipStreamSynchronize(st));
+    return ZK_OK;
+  }
+  if (!ntt_generator_ok(curve, m, gen)) {
+    set_error("ntt_noalloc: the generator must have order 2^m");
+    return ZK_ENOTROOT;
+  }
+  const zkh::FieldDesc &f = fr_desc(curve);
+  const uint32_t *tw = nullptr;
+  if (m >= 1) {
+    ZK_TRY(launch_twiddles<F>(scr + kNaTw, scr + kNaLo, scr + kNaHi, 1ull << (m - 1), zkh::load(f, gen), st));
+    tw = scr + kNaTw;
+  }
+  if (eta) ZK_TRY(launch_eta_tables<F>(scr + kNaEta, curve, zkh::load(f, eta), st));
+  uint32_t lastf[8];
+  last_factor<F>(lastf, curve, m, inverse);
+  const NttTuning tu = ntt_tuning_for(curve, m);
+  const NttPlan plan = ntt_plan(m, tu.smax, tu.tile_log);
+  const uint32_t *ep = (eta && !inverse) ? scr + kNaEta : nullptr;
+  const uint32_t *eo = (eta && inverse) ? scr + kNaEta : nullptr;
+  ZK_TRY(ntt_fast<F>(plan, d_in, d_out, scr + kNaTmp, tw, 0, inverse, ep, eo, lastf, st));
+  ZK_HIP(hipStreamSynchronize(st));
+  return ZK_OK;
+}
+
+int ntt_device_noalloc(int curve, int m, const uint64_t *gen, const uint64_t *eta, int flags, const void *d_in,
+                       void *d_out, hipStream_t st) {
+  if (curve != 0 && curve != 1) {
+    set_error("ntt_noalloc: bad curve id");
+    return ZK_EINVAL;
+  }
+  if (m < 0 || m > kNoAllocMaxLog) {
+    set_error("ntt_noalloc: log size must be 0..18 (its fixed scratch holds 2^18 elements)");
+    return ZK_EINVAL;
+  }
+  uint32_t *scr = nullptr;
+  ZK_TRY(scratch_on_current_device(&scr));
+  if (curve == 0)
+    return ntt_noalloc_t<Bn254Fr>(curve, m, gen, eta, flags, (const uint32_t *)d_in, (uint32_t *)d_out, st, scr);
+  return ntt_noalloc_t<Bls381Fr>(curve, m, gen, eta, flags, (const uint32_t *)d_in, (uint32_t *)d_out, st, scr);
+}
+
 }  // namespace zk

@@ -41,6 +41,11 @@ bool ntt_generator_ok(int curve, int m, const uint64_t *gen);
 int ntt_device(int curve, int m, const uint64_t *gen, const uint64_t *eta, int flags, const void *d_in,
                void *d_out, hipStream_t st, const NttTuning *tune);
 
+// The same transform with no device allocation: scratch in a fixed file-scope device array
+// (m <= 18, generators of order 2^m); calls that overlap on one GPU race.
+int ntt_device_noalloc(int curve, int m, const uint64_t *gen, const uint64_t *eta, int flags, const void *d_in,
+                       void *d_out, hipStream_t st);
+
 struct NttBatchJob {
   int m;
   int inverse;

@@ -78,6 +78,7 @@ def lib():
         L.zkg_memcpy_d2h.argtypes = [vp, vp, ctypes.c_size_t]
         L.zkg_msm.argtypes = [ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, vp, u64p, ctypes.c_int, vp]
         L.zkg_ntt.argtypes = [ctypes.c_int, ctypes.c_int, u64p, u64p, ctypes.c_int, vp, vp]
+        L.zkg_ntt_noalloc.argtypes = [ctypes.c_int, ctypes.c_int, u64p, u64p, ctypes.c_int, vp, vp]
         L.zkg_fft_generator.argtypes = [ctypes.c_int, ctypes.c_int, u64p]
         L.zkg_field_op.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_size_t, vp, vp, vp]
         L.zkg_g1_scalar_mul_gen.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp]
@@ -671,6 +672,13 @@ def ntt_tune(curve, m, inverse=False, reps=3, install=True):
 def ntt_device(curve, m, gen, d_src, d_tgt, inverse=False, eta=None):
     _check(lib().zkg_ntt(curve, m, _p(_c(gen)), _p(_c(eta)) if eta is not None else None,
                          NTT_INVERSE if inverse else NTT_FORWARD, d_src.ptr, d_tgt.ptr))
+
+
+def ntt_device_noalloc(curve, m, gen, d_src, d_tgt, inverse=False, eta=None):
+    """zkg_ntt_noalloc: the same transform with no device allocation (m <= 18, order-2^m gen)."""
+    gen = _c(np.asarray(gen, np.uint64))
+    e = _p(_c(np.asarray(eta, np.uint64))) if eta is not None else None
+    _check(lib().zkg_ntt_noalloc(curve, m, _p(gen), e, 1 if inverse else 0, d_src.ptr, d_tgt.ptr))
 
 
 class _NttJob(ctypes.Structure):
