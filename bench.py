@@ -27,7 +27,10 @@ import numpy as np
 REPO = Path(__file__).resolve().parent
 sys.path.insert(0, str(REPO / "zikkurat-algebra_amd"))
 
-import zkgpu  # noqa: E402
+if __name__ == "__main__" and "--cpu-worker" in sys.argv:
+    zkgpu = None  # a CPU baseline worker process: the reference C only, no GPU library
+else:
+    import zkgpu  # noqa: E402
 
 BN_R = 21888242871839275222246405745257275088548364400416034343698204186575808495617
 BN_P = 21888242871839275222246405745257275088696311157297823662689037894645226208583
@@ -161,18 +164,53 @@ def kernel_ms(fn, steps, phase):
     return tot / steps
 
 
-def cpu_baseline(case, sample_log):
-    """The reference C (oracle/_ref) on a bounded prefix of the same inputs, one core."""
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), logical CPUs and the CPUs this process may use."""
+    name = platform.machine()
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                name = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return name, os.cpu_count() or 1, avail
+
+
+def _ref_msm_fn():
     ref_so = REPO / "oracle" / "_ref" / "libzkref.so"
-    kind = "reference"
     if ref_so.exists():
-        lib = ctypes.CDLL(str(ref_so))
-        fn = lib.bn128_G1_proj_MSM_std_coeff_affine_out
-    else:
-        lib = ctypes.CDLL(str(REPO / "oracle" / "libzkoracle.so"))
-        kind = "port"
-        def fn(n, e, g, t, enl):
-            return lib.zko_msm_std_affine(0, n, e, g, t, enl)
+        return ctypes.CDLL(str(ref_so)).bn128_G1_proj_MSM_std_coeff_affine_out, "reference"
+    lib = ctypes.CDLL(str(REPO / "oracle" / "libzkoracle.so"))
+
+    def fn(n, e, g, t, enl):
+        return lib.zko_msm_std_affine(0, n, e, g, t, enl)
+    return fn, "port"
+
+
+def cpu_worker(path):
+    """One CPU baseline process: the reference MSM over the sample saved at `path`; prints its
+    wall time and result words as JSON."""
+    d = np.load(path)
+    sc, pts = np.ascontiguousarray(d["scalars"]), np.ascontiguousarray(d["points"])
+    fn, _ = _ref_msm_fn()
+    out = np.zeros(8, np.uint64)
+    u64p = ctypes.POINTER(ctypes.c_uint64)
+    t0 = time.perf_counter()
+    fn(len(sc), sc.ctypes.data_as(u64p), pts.ctypes.data_as(u64p), out.ctypes.data_as(u64p), 4)
+    print(json.dumps({"s": time.perf_counter() - t0, "out": [int(x) for x in out]}), flush=True)
+
+
+def cpu_baseline(case, sample_log, max_procs=16):
+    """The reference C (oracle/_ref) on a bounded prefix of the same inputs: one core, and k
+    independent copies on k cores at once (the reference has no parallelism inside one MSM).
+    The reference picks its window as round(log2 n - 3.5) (bn128_G1_proj.c:600): 15 bits at 2^18
+    against 17 at 2^20, so its per-point cost at 2^20 is (ceil(256/17) (1 + 2^17/2^20)) /
+    (ceil(256/15) (1 + 2^15/2^18)) of the sample's, which `per_point_2p20_factor` states."""
+    import subprocess
+    import tempfile
+    fn, kind = _ref_msm_fn()
     n = 1 << sample_log
     pts = case.d_points.to_host((case.n, 8))[:n].copy()
     sc = np.ascontiguousarray(case.s[:n])
@@ -181,12 +219,32 @@ def cpu_baseline(case, sample_log):
     t0 = time.perf_counter()
     fn(n, sc.ctypes.data_as(u64p), pts.ctypes.data_as(u64p), out.ctypes.data_as(u64p), 4)
     dt = time.perf_counter() - t0
-    # the CPU result must equal the GPU's on the same sample
-    gpu = zkgpu.msm_std(0, sc, pts)
-    return {"value": round(n / dt / 1e6, 5), "unit": "Mpoints/s", "cores": 1, "kind": kind,
-            "sample": f"BN254 G1 MSM n=2^{sample_log} (first 2^{sample_log} points/scalars of the "
-                      f"benchmark inputs), single thread, {dt:.2f} s; CPU: {platform.processor() or platform.machine()}",
-            "matches_gpu": bool((gpu == out).all())}
+    gpu = zkgpu.msm_std(0, sc, pts)  # the CPU result must equal the GPU's on the same sample
+    name, ncpu, avail = cpu_model()
+    c_s = int(np.floor(np.log2(n) - 3.5 + 0.5))
+    c_20 = int(np.floor(20 - 3.5 + 0.5))
+    cost = lambda c, lg: -(-256 // c) * (1 + 2 ** c / 2 ** lg)
+    res = {"value": round(n / dt / 1e6, 5), "unit": "Mpoints/s", "cores": 1, "kind": kind,
+           "sample": f"BN254 G1 MSM n=2^{sample_log} (first 2^{sample_log} points/scalars of the benchmark "
+                     f"inputs), single thread, {dt:.2f} s, reference window c={c_s} (c={c_20} at 2^20)",
+           "cpu": name, "host_logical_cpus": ncpu, "cpus_available": avail,
+           "per_point_2p20_factor": round(cost(c_20, 20) / cost(c_s, sample_log), 4),
+           "matches_gpu": bool((gpu == out).all())}
+    # k copies on k cores: separate processes (no GPU use), all started together
+    k = max(1, min(avail, max_procs))
+    with tempfile.TemporaryDirectory() as td:
+        path = Path(td) / "sample.npz"
+        np.savez(path, scalars=sc, points=pts)
+        t0 = time.perf_counter()
+        procs = [subprocess.Popen([sys.executable, str(Path(__file__).resolve()), "--cpu-worker", str(path)],
+                                  stdout=subprocess.PIPE, text=True) for _ in range(k)]
+        outs = [json.loads(p.communicate()[0].strip().splitlines()[-1]) for p in procs]
+        wall = time.perf_counter() - t0
+    same = all(o["out"] == [int(x) for x in out] for o in outs)
+    res["all_cores"] = {"value": round(k * n / wall / 1e6, 5), "unit": "Mpoints/s", "cores": k,
+                        "jobs": f"{k} independent copies of the sample, one process each, {wall:.2f} s wall",
+                        "matches": same}
+    return res
 
 
 def bench_ntt_pair(steps, warmup, peak_imul=None, m=22):
@@ -606,4 +664,7 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if "--cpu-worker" in sys.argv:
+        cpu_worker(sys.argv[sys.argv.index("--cpu-worker") + 1])
+    else:
+        main()

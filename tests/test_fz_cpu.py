@@ -161,6 +161,39 @@ def test_zmul_sub_matches_integers(h, field):
         assert is_normalised(R[i], bits) and -2 * p < got < 2 * p
 
 
+@pytest.mark.parametrize("field", [0, 2], ids=["bn-fp", "bls-fp"])
+def test_difference_operands(h, field):
+    """The curve formulas feed limb-wise differences of two normalised values (P = U2 - X1,
+    R = S2 - Y1, Q - X3) to zsqr, zmul, zmul_sub and the zero test without normalising them.
+    Differences of a value in (-p, 2p) and one in (-7p, 5p), the widest the formulas make, and
+    limbs at the ends of their ranges."""
+    p, bits, N, _ = FIELDS[field]
+    Rinv = pow(1 << (bits * N), -1, p)
+    rng = splitmix64(800 + field)
+    us = [random_below(rng, 3 * p, 8) - p for _ in range(300)] + [2 * p - 1, -(p - 1), 0, 2 * p - 1]
+    xs = [random_below(rng, 12 * p, 8) - 7 * p for _ in range(300)] + [-(7 * p - 1), 5 * p - 1, 5 * p - 1, 0]
+    U = arr([to_limbs(u, bits, N) for u in us])
+    X = arr([to_limbs(x, bits, N) for x in xs])
+    D = arr(U.astype(np.int64) - X.astype(np.int64))
+    ds = [u - x for u, x in zip(us, xs)]
+    assert all(value(D[i], bits) == ds[i] for i in range(len(ds)))
+    assert max(abs(int(v)) for v in D[:, :-1].ravel()) < (1 << bits)
+    S = op(h, field, 6, D)
+    M = op(h, field, 0, D, arr([to_limbs(u, bits, N) for u in us]))
+    E = op(h, field, 10, D, D[::-1].copy())
+    Z = op(h, field, 5, D)
+    for i, d in enumerate(ds):
+        for row, want in ((S[i], d * d), (M[i], d * us[i]), (E[i], d * ds[::-1][i] - ds[::-1][i] ** 2)):
+            got = value(row, bits)
+            assert got % p == want * Rinv % p, i
+            assert is_normalised(row, bits) and -2 * p < got < 2 * p, (i, got / p)
+        assert bool(Z[i, 0]) == (d % p == 0), i
+    # exact multiples of p written as differences are recognised as zero
+    Zm = op(h, field, 5, arr(arr([to_limbs(j * p + 3, bits, N) for j in range(-3, 4)]).astype(np.int64)
+                             - arr([to_limbs(3, bits, N)] * 7).astype(np.int64)))
+    assert Zm[:, 0].all()
+
+
 @pytest.mark.parametrize("field", [0, 1, 2, 3], ids=["bn-fp", "bn-fr", "bls-fp", "bls-fr"])
 def test_zero_test_norm_and_canon(h, field):
     p, bits, N, _ = FIELDS[field]

@@ -137,13 +137,14 @@ ZK_HD void zxyzz_madd(XyzzZ<C> &acc, const AffZ<C> &a) {
     zset_one(acc.zzz);
     return;
   }
+  // P and R stay limb-wise differences of two normalised values (limbs below 2^29 in
+  // magnitude): valid operands of zsqr, zmul and zmul_sub, and of the zero test, which reads
+  // the value, not the limb ranges
   F P, R, pp, ppp, q, t;
   zmul(P, a.x, acc.zz);
-  zsub(P, P, acc.x);
-  znorm(P);  // P = U2 - X1
+  zsub(P, P, acc.x);  // P = U2 - X1
   zmul(R, a.y, acc.zzz);
-  zsub(R, R, acc.y);
-  znorm(R);  // R = S2 - Y1
+  zsub(R, R, acc.y);  // R = S2 - Y1
   if (zis_zero_mod(P)) {
     if (zis_zero_mod(R)) zxyzz_mdbl(acc, a);
     else zxyzz_set_inf(acc);
@@ -160,8 +161,7 @@ ZK_HD void zxyzz_madd(XyzzZ<C> &acc, const AffZ<C> &a) {
   zsub(t, t, q);
   znorm(t);
   acc.x = t;              // X3 = R^2 - PPP - 2Q
-  zsub(t, q, acc.x);      // Q - X3
-  znorm(t);
+  zsub(t, q, acc.x);      // Q - X3, a difference of normalised values
   zmul_sub(q, R, t, acc.y, ppp);
   acc.y = q;              // Y3 = R (Q - X3) - Y1 PPP, one reduction
 }
@@ -181,12 +181,10 @@ ZK_HD void zxyzz_add(XyzzZ<C> &r, const XyzzZ<C> &p, const XyzzZ<C> &q) {
   F u1, s1, P, R, pp, ppp, t, zz, zzz;
   zmul(u1, p.x, q.zz);
   zmul(P, q.x, p.zz);
-  zsub(P, P, u1);
-  znorm(P);
+  zsub(P, P, u1);  // differences of normalised values, as in zxyzz_madd
   zmul(s1, p.y, q.zzz);
   zmul(R, q.y, p.zzz);
   zsub(R, R, s1);
-  znorm(R);
   if (zis_zero_mod(P)) {
     if (zis_zero_mod(R)) {
       XyzzZ<C> pc = p;
@@ -210,7 +208,6 @@ ZK_HD void zxyzz_add(XyzzZ<C> &r, const XyzzZ<C> &p, const XyzzZ<C> &q) {
   znorm(t);
   F x3 = t;
   zsub(t, u1, x3);
-  znorm(t);
   zmul_sub(r.y, R, t, s1, ppp);
   r.x = x3;
   r.zz = zz;
