@@ -522,7 +522,10 @@ def bench_next(seed=7, logn=20, cpu_log=14):
 def bench_serve(njobs=64, qps_list=(100, 400), seed=5):
     """KZG commits on BLS12-381: iNTT of n evaluations (n = 2^14..2^21, log-uniform) then an MSM
     against a resident SRS of 2^21 points tau^i G; Poisson arrivals at each stated QPS; served
-    (4 workers, continuous batching) against the plain path (one job at a time)."""
+    (4 workers, continuous batching) against the plain path (one job at a time), and the served
+    form with the optional size rule (no job starts beside one of 2^21 points).  Before each
+    trace a warm-up runs four jobs of every size at once, so that every form finds its device
+    buffers of every size in the allocator's cache."""
     rng = np.random.default_rng(seed)
     srs_n = 1 << 21
     tau = int(uniform_scalars(rng, 1, BLS_R)[0][0])
@@ -531,12 +534,15 @@ def bench_serve(njobs=64, qps_list=(100, 400), seed=5):
     logns = [int(x) for x in rng.integers(14, 22, size=njobs)]
     values = [uniform_scalars(rng, 1 << m, BLS_R) for m in logns]  # canonical words = valid Montgomery elements
     out = {"serve_jobs": njobs, "serve_srs_log": 21, "serve_sizes": "2^14..2^21 log-uniform"}
+    warm = [m for m in range(21, 13, -1) for _ in range(4)]
+    warm_vals = [uniform_scalars(rng, 1 << m, BLS_R) for m in range(21, 13, -1) for _ in range(4)]
     for qps in qps_list:
         arrivals = list(np.cumsum(rng.exponential(1.0 / qps, size=njobs)) - 0.0)
         res = {}
-        for name, workers in (("plain", 1), ("served", 4)):
+        for name, workers, excl in (("plain", 1, 0), ("served", 4, 0), ("served_size_rule_2p21", 4, 21)):
             srv = zkgpu.Server(1, srs, 48 << 30, workers)
-            srv.run_trace(logns[:4], [0.0] * 4, values[:4])  # warm-up (twiddles, allocator)
+            srv.set_exclusive_log(excl)
+            srv.run_trace(warm, [0.0] * len(warm), warm_vals)  # warm-up (twiddles, allocator)
             outs, lat, wall = srv.run_trace(logns, arrivals, values)
             peak, conc = srv.stats()
             srv.close()
@@ -546,7 +552,9 @@ def bench_serve(njobs=64, qps_list=(100, 400), seed=5):
                                 "commits_per_s": round(njobs / wall, 1), "max_concurrent": conc,
                                 "peak_admitted_GB": round(peak / 2**30, 2)})
         same = all((a == b_).all() for a, b_ in zip(res["plain"][0], res["served"][0]))
-        out[f"serve_qps{qps}"] = {"plain": res["plain"][1], "served": res["served"][1], "bit_identical": same}
+        same = same and all((a == b_).all() for a, b_ in zip(res["plain"][0], res["served_size_rule_2p21"][0]))
+        out[f"serve_qps{qps}"] = {"plain": res["plain"][1], "served": res["served"][1],
+                                  "served_size_rule_2p21": res["served_size_rule_2p21"][1], "bit_identical": same}
     return out
 
 

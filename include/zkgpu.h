@@ -335,6 +335,9 @@ int zkg_srs_generate(int curve, int n, const uint64_t *tau_std, uint64_t *d_out_
 int zkg_serve_create(int curve, const void *d_srs_prepared, int srs_n, size_t hbm_budget, int nworkers, void **server);
 int zkg_serve_submit(void *server, int logn, const uint64_t *values, uint64_t *out_affine, long long *job_id);
 int zkg_serve_drain(void *server);
+/* optional size rule: no job starts while a job of at least 2^logn points runs (default 0,
+ * off: every job is admitted by footprint only) */
+int zkg_serve_set_exclusive_log(void *server, int logn);
 int zkg_serve_job_times(void *server, long long id, double *queue_ms, double *run_ms, double *latency_ms);
 int zkg_serve_stats(void *server, size_t *peak_bytes, int *max_concurrent);
 /* submits job k at arrival_s[k] seconds after the call starts, waits for all of them */
@@ -347,7 +350,9 @@ int zkg_serve_destroy(void *server);
 /* scalar statistics on the device (flags: ZKG_MSM_MONT when the scalars are Montgomery Fr) */
 int zkg_msm_scalar_stats(int curve, int npoints, const uint64_t *d_expos, int expo_nlimbs, int flags,
                          zkg_scalar_stats *out);
-/* cost-model input row for one (job, candidate schedule); ZKG_MSM_NFEAT floats */
+/* cost-model input row for one (job, candidate schedule); ZKG_MSM_NFEAT floats.  These three
+ * describe a job with 4-limb scalars over affine (unprepared) points, whose BN254 form takes the
+ * GLV split when its scalars are wider than 128 bits (2n points, 127-bit halves). */
 #define ZKG_MSM_NFEAT 11
 int zkg_msm_features(int curve, const zkg_scalar_stats *st, int window, int seg_log, float *out);
 /* candidate schedules searched by the autotuner and the cost model; returns the count */

@@ -98,16 +98,30 @@ def test_window_layout_covers_exactly():
             assert layout(bits, cm) == (W, cm)  # the effective width is its own layout
 
 
-def test_features_encode_window_counts(zkh):
-    # W = ceil((max_bits + 1) / c): 254-bit scalars at c = 16 need 16 windows, 64-bit ones 5
-    for bits, c, W, cm in [(254, 16, 16, 16), (64, 16, 5, 13), (63, 16, 4, 16), (0, 8, 1, 1), (255, 13, 20, 13),
-                           (254, 14, 19, 14), (254, 17, 15, 17)]:
-        assert layout(bits, c) == (W, cm)
-        st = zkh.stats_from(1 << 16, 0, 0, bits)
-        x = zkh.msm_features(0, st, c, 5)
-        assert math.isclose(x[6] * 64, W, abs_tol=1e-4)
+def shape(curve, bits, n):
+    """The bucket problem a job runs as: BN254 scalars wider than 128 bits take the GLV split
+    (2n points, 127-bit halves, two digits per window from each full scalar)."""
+    if curve == 0 and bits > 128:
+        return 127, 2 * n, 2
+    return bits, n, 1
+
+
+@pytest.mark.parametrize("curve", [0, 1])
+def test_features_encode_window_counts(zkh, curve):
+    # W = ceil((bits + 1) / c): 255-bit BLS12-381 scalars at c = 16 need 16 windows, 64-bit
+    # ones 5; BN254 scalars over 128 bits run as 2n points of 127 bits (8 windows at c = 16)
+    for bits, c in [(254, 16), (64, 16), (63, 16), (0, 8), (255, 13), (254, 14), (254, 17), (128, 16), (129, 16)]:
+        n = 1 << 16
+        b, pts, per = shape(curve, bits, n)
+        W, cm = layout(b, c)
+        st = zkh.stats_from(n, 0, 0, bits)
+        x = zkh.msm_features(curve, st, c, 5)
+        assert math.isclose(x[6] * 64, W, abs_tol=1e-4), (bits, c)
         assert math.isclose(x[4] * 20, cm, abs_tol=1e-5) and math.isclose(x[5] * 8, 5, abs_tol=1e-5)
-        assert math.isclose(x[1] * 24, 16, abs_tol=1e-4)
+        assert math.isclose(x[1] * 24, math.log2(pts), abs_tol=1e-4)
+        assert math.isclose(x[2] * 256, b, abs_tol=1e-3)
+        assert math.isclose(2 ** (x[7] * 32) - 1, W * per * n, rel_tol=1e-4)
+    assert layout(254, 16) == (16, 16) and layout(127, 16) == (8, 16) and layout(127, 17) == (8, 16)
 
 
 def test_features_sparse_counts_entries(zkh):
@@ -130,19 +144,24 @@ def test_candidates_contain_both_analytic_picks(zkh, curve, logn):
         cands = zkh.msm_candidates(curve, st)
         assert len(cands) == len(set(cands)) > 0
         assert all(2 <= c <= 18 and 2 <= s <= 7 for c, s in cands)
-        assert all(layout(bits, c)[1] == c for c, _ in cands)  # listed by effective width
+        b = shape(curve, bits, n)[0]
+        assert all(layout(b, c)[1] == c for c, _ in cands)  # listed by effective width
         assert tuple(zkh.msm_pick(curve, st, zkh.SCHED_STATS)) in cands
         cp, sp = zkh.msm_pick(curve, st, zkh.SCHED_PLAIN)
         assert min(c for c, _ in cands) <= cp <= max(c for c, _ in cands) and sp == 5
 
 
 def test_plain_pick_matches_op_count_model(zkh):
-    # the plain pick minimises W * (10 n + 28 * 2^(c-1)) with W = ceil(255 / c) over c <= 16
-    # (windows whose bucket bases fit LDS); its windows are all c wide
+    # the plain pick minimises W * (10 n' + 28 * 2^(c-1)) over c <= 16 (windows whose bucket
+    # bases fit LDS): BLS12-381 with W = ceil(256 / c) over n points, uniform c-bit windows;
+    # BN254 through the GLV split, W = ceil(128 / c) over 2n points, reported as the balanced
+    # width the job runs with
     for logn in range(8, 24, 3):
         n = 1 << logn
-        best = min(range(2, 17), key=lambda c: (math.ceil(255 / c) * (10 * n + 28 * 2 ** (c - 1)), c))
-        assert zkh.msm_pick(0, zkh.stats_from(n, 0, 0, 254), zkh.SCHED_PLAIN)[0] == best
+        best = min(range(2, 17), key=lambda c: (math.ceil(256 / c) * (10 * n + 28 * 2 ** (c - 1)), c))
+        assert zkh.msm_pick(1, zkh.stats_from(n, 0, 0, 255), zkh.SCHED_PLAIN)[0] == best
+        best = min(range(2, 17), key=lambda c: (math.ceil(128 / c) * (20 * n + 28 * 2 ** (c - 1)), c))
+        assert zkh.msm_pick(0, zkh.stats_from(n, 0, 0, 254), zkh.SCHED_PLAIN)[0] == layout(127, best)[1]
 
 
 def test_stats_pick_narrow_scalars_use_fewer_windows(zkh):

@@ -52,6 +52,39 @@ def test_served_commitments(zk, C):
         assert C.words_to_affine(outs[k]) == C.mul(ft, C.g)
 
 
+def test_large_jobs_run_alone(zk):
+    """With the size rule at 2^10 points, no job starts beside a job of 2^10 or 2^11 points (so
+    with only such jobs at most one runs at a time); every result equals the job run alone."""
+    C = BN
+    srs_n = 1 << 12
+    d_srs = zk.srs_generate(C.id, srs_n, 777)
+    srs = zk.PreparedPoints(C.id, d_srs, srs_n)
+    lognes = [10, 11, 10, 11, 10]
+    vals = [ints_to_array([C.fr_to_mont(v) for v in random_scalars(600 + k, 1 << m, C.r)], 4)
+            for k, m in enumerate(lognes)]
+    alone = zk.Server(C.id, srs, 1 << 34, 1)
+    ref = []
+    for m, v in zip(lognes, vals):
+        ref.append(alone.submit(m, v)[1])
+        alone.drain()
+    alone.close()
+    big = zk.Server(C.id, srs, 1 << 34, 4)
+    big.set_exclusive_log(10)
+    outs = [big.submit(m, v)[1] for m, v in zip(lognes, vals)]
+    big.drain()
+    assert big.stats()[1] == 1
+    big.close()
+    assert all((a == b).all() for a, b in zip(ref, outs))
+    small = zk.Server(C.id, srs, 1 << 34, 4)
+    small.set_exclusive_log(12)
+    for _ in range(3):
+        outs = [small.submit(m, v)[1] for m, v in zip(lognes, vals)]
+        small.drain()
+        assert all((a == b).all() for a, b in zip(ref, outs))
+    small.close()
+    srs.free()
+
+
 def test_admission_respects_budget(zk):
     C = BLS
     srs_n = 1 << 12

@@ -809,6 +809,7 @@ int zkg_serve_submit(void *server, int logn, const uint64_t *values, uint64_t *o
   return serve_submit((Server *)server, logn, values, out_affine, job_id);
 }
 int zkg_serve_drain(void *server) { return serve_drain((Server *)server); }
+int zkg_serve_set_exclusive_log(void *server, int logn) { return serve_set_exclusive_log((Server *)server, logn); }
 int zkg_serve_job_times(void *server, long long id, double *queue_ms, double *run_ms, double *latency_ms) {
   return serve_job_times((Server *)server, id, queue_ms, run_ms, latency_ms);
 }
@@ -859,7 +860,7 @@ int zkg_msm_features(int curve, const zkg_scalar_stats *st, int window, int seg_
     set_error("msm_features: bad curve, window or seg_log");
     return ZK_EINVAL;
   }
-  msm_features(curve, *st, window, seg_log, out);
+  msm_features(curve, *st, window, seg_log, true, out);
   return ZK_OK;
 }
 
@@ -868,7 +869,7 @@ int zkg_msm_candidates(int curve, const zkg_scalar_stats *st, int max, int *wind
     set_error("msm_candidates: bad arguments");
     return -1;
   }
-  return msm_candidates(curve, *st, max, windows, seg_logs);
+  return msm_candidates(curve, *st, true, max, windows, seg_logs);
 }
 
 int zkg_msm_pick(int curve, const zkg_scalar_stats *st, int policy, zkg_msm_schedule *out) {
@@ -880,7 +881,7 @@ int zkg_msm_pick(int curve, const zkg_scalar_stats *st, int policy, zkg_msm_sche
   memset(&u, 0, sizeof u);
   u.policy = policy;
   MsmSchedule s;
-  ZK_TRY(msm_resolve(curve, (int)st->n, 4, &u, policy == ZKG_SCHED_PLAIN ? nullptr : st, &s));
+  ZK_TRY(msm_resolve(curve, (int)st->n, 4, &u, policy == ZKG_SCHED_PLAIN ? nullptr : st, &s, true));
   memset(out, 0, sizeof *out);
   out->window = s.c;
   out->seg_log = s.seg_log;

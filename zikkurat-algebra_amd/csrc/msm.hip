@@ -1376,7 +1376,7 @@ int msm_resolve(int curve, int n, int enl, const zkg_msm_schedule *user, const z
   int seg = (user && user->seg_log > 0) ? user->seg_log : 0;
   if (st && (c == 0 || seg == 0) && policy == ZKG_SCHED_LEARNED) {
     int mc = 0, ms = 0;
-    if (cost_model_pick(curve, *st, &mc, &ms)) {
+    if (cost_model_pick(curve, *st, glv_ok && enl == 4, &mc, &ms)) {
       if (c == 0) c = mc;
       if (seg == 0) seg = ms;
     }

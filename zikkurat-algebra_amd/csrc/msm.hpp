@@ -40,10 +40,11 @@ int msm_heuristic_window(int curve, long long n, int scalar_bits);
 int msm_stats_window(int curve, const zkg_scalar_stats &st);
 int msm_scalar_stats(int curve, int n, const uint64_t *d_scalars, int enl, int mont, hipStream_t st,
                      zkg_scalar_stats *out);
-// cost-model choice for a job (false when no model is installed)
-bool cost_model_pick(int curve, const zkg_scalar_stats &st, int *c, int *seg_log);
-void msm_features(int curve, const zkg_scalar_stats &st, int c, int seg_log, float *x);
-int msm_candidates(int curve, const zkg_scalar_stats &st, int max, int *cs, int *segs);
+// cost-model choice for a job (false when no model is installed).  glv_ok as for msm_resolve:
+// the candidates and features then describe the GLV shape the job runs with.
+bool cost_model_pick(int curve, const zkg_scalar_stats &st, bool glv_ok, int *c, int *seg_log);
+void msm_features(int curve, const zkg_scalar_stats &st, int c, int seg_log, bool glv_ok, float *x);
+int msm_candidates(int curve, const zkg_scalar_stats &st, bool glv_ok, int max, int *cs, int *segs);
 // glv_ok: the points arrive as affine words, so a BN254 job may take the GLV split (2n points
 // P_i and phi(P_i), scalars below 2^127): under the plain schedule always, under the
 // statistics policies when the measured scalars are wider than 128 bits
@@ -84,6 +85,7 @@ int srs_generate(int curve, int n, const uint64_t *tau_std, uint64_t *d_out, hip
 struct Server;
 int serve_create(int curve, const void *d_srs_internal, int srs_n, size_t budget, int nworkers, Server **out);
 int serve_submit(Server *s, int logn, const uint64_t *values, uint64_t *out, long long *id);
+int serve_set_exclusive_log(Server *s, int logn);
 int serve_drain(Server *s);
 int serve_job_times(Server *s, long long id, double *queue_ms, double *run_ms, double *latency_ms);
 int serve_stats(Server *s, size_t *peak_bytes, int *max_concurrent);

@@ -29,21 +29,41 @@ namespace zk {
 
 static_assert(ZKG_MSM_NFEAT == 11, "feature count");
 
-// Cost-model input row.  Every feature is a plain function of the job and the candidate; the
-// last one is the analytic cost of msm_stats_window, so the network learns a correction to it.
-void msm_features(int curve, const zkg_scalar_stats &st, int c_req, int seg_log, float *x) {
+// The bucket problem a job runs as (msm_resolve): with the GLV split (BN254, 4-limb scalars
+// wider than 128 bits, points given as affine words) 2n points P_i, phi(P_i) with scalar halves
+// below 2^127, each full scalar giving a digit per window in both halves; otherwise n points
+// with windows over the measured bits.
+struct MsmShape {
+  int bits;
+  double points, per_full;  // bucket-problem points; nonzero digits per window of a full scalar
+};
+static MsmShape msm_shape(const zkg_scalar_stats &st, bool glv) {
+  const double n = st.n > 0 ? (double)st.n : 1.0;
+  if (glv) return MsmShape{127, 2.0 * n, 2.0};
+  return MsmShape{st.max_bits, n, 1.0};
+}
+static bool msm_glv_shape(int curve, const zkg_scalar_stats &st, bool glv_ok) {
+  return glv_ok && curve == 0 && st.max_bits > 128;
+}
+
+// Cost-model input row.  Every feature is a plain function of the job and the candidate as it
+// runs (the GLV shape included); the last one is the analytic cost of msm_stats_window, so the
+// network learns a correction to it.
+void msm_features(int curve, const zkg_scalar_stats &st, int c_req, int seg_log, bool glv_ok, float *x) {
+  const bool glv = msm_glv_shape(curve, st, glv_ok);
+  const MsmShape sh = msm_shape(st, glv);
   int w, c, wide;
-  msm_window_layout(st.max_bits, c_req, true, &w, &c, &wide);
+  msm_window_layout(sh.bits, c_req, true, &w, &c, &wide);
   const double n = st.n > 0 ? (double)st.n : 1.0;
   const double nfull = (double)(st.n - st.n_zero - st.n_one), ones = (double)st.n_one;
   const double W = (double)w;
-  const double entries = W * nfull + ones;
+  const double entries = W * sh.per_full * nfull + ones;
   const double buckets = W * (double)(1ll << (c - 1));
   const double threads = entries / (double)(1ll << seg_log) + 1.0;
-  const double cost = 10.0 * entries + 3.0 * W * n + 28.0 * buckets;
+  const double cost = 10.0 * entries + 3.0 * W * sh.points + 28.0 * buckets;
   x[0] = (float)curve;
-  x[1] = (float)(log2(n) / 24.0);
-  x[2] = (float)(st.max_bits / 256.0);
+  x[1] = (float)(log2(sh.points) / 24.0);
+  x[2] = (float)(sh.bits / 256.0);
   x[3] = (float)((st.n_zero + st.n_one) / n);
   x[4] = (float)(c / 20.0);
   x[5] = (float)(seg_log / 8.0);
@@ -54,17 +74,21 @@ void msm_features(int curve, const zkg_scalar_stats &st, int c_req, int seg_log,
   x[10] = (float)(log2(cost + 1.0) / 48.0);
 }
 
-// Windows around both analytic choices (declared width and measured scalars) up to 18 bits,
-// seg_log 2..7; requested widths with the same balanced layout on these scalars are one
-// candidate (listed by their effective width).
-int msm_candidates(int curve, const zkg_scalar_stats &st, int max, int *cs, int *segs) {
+// Windows around both analytic choices (declared width and measured scalars; with the GLV
+// split, the choice for 2n points of 127 bits) up to 18 bits, seg_log 2..7; requested widths
+// with the same balanced layout on the bits the job runs with are one candidate (listed by
+// their effective width).
+int msm_candidates(int curve, const zkg_scalar_stats &st, bool glv_ok, int max, int *cs, int *segs) {
+  const bool glv = msm_glv_shape(curve, st, glv_ok);
+  const MsmShape sh = msm_shape(st, glv);
   const int fr_bits = curve == 0 ? 254 : 255;
-  const int hp = msm_heuristic_window(curve, st.n, fr_bits), hs = msm_stats_window(curve, st);
+  const int hp = glv ? msm_heuristic_window(curve, 2ll * st.n, 127) : msm_heuristic_window(curve, st.n, fr_bits);
+  const int hs = glv ? hp : msm_stats_window(curve, st);
   const int lo = std::max(2, std::min(hp, hs) - 5), hi = std::min(18, std::max(hp, hs) + 3);
   int k = 0, last = -1;
   for (int c = lo; c <= hi; c++) {
     int w, cm, wide;
-    msm_window_layout(st.max_bits, c, true, &w, &cm, &wide);
+    msm_window_layout(sh.bits, c, true, &w, &cm, &wide);
     if (cm == last) continue;
     last = cm;
     for (int s = 2; s <= 7; s++) {
@@ -82,7 +106,7 @@ int msm_candidates(int curve, const zkg_scalar_stats &st, int max, int *cs, int 
 
 static std::mutex g_model_mu;
 static Mlp *g_model = nullptr;
-static std::map<std::tuple<int, long long, int, int, int>, std::pair<int, int>> g_pick_cache;
+static std::map<std::tuple<int, long long, int, int, int, bool>, std::pair<int, int>> g_pick_cache;
 
 int cost_model_set(Mlp *m) {
   std::lock_guard<std::mutex> lk(g_model_mu);
@@ -103,12 +127,12 @@ void cost_model_forget(Mlp *m) {
   }
 }
 
-bool cost_model_pick(int curve, const zkg_scalar_stats &st, int *c, int *seg_log) {
+bool cost_model_pick(int curve, const zkg_scalar_stats &st, bool glv_ok, int *c, int *seg_log) {
   std::lock_guard<std::mutex> lk(g_model_mu);
   if (!g_model) return false;
-  // jobs with the same size, width and (to 1/64) share of zeros and ones share a decision
+  // jobs with the same size, width, shape and (to 1/64) share of zeros and ones share a decision
   const auto key = std::make_tuple(curve, st.n, st.max_bits, (int)(64.0 * st.n_zero / std::max(1ll, st.n)),
-                                   (int)(64.0 * st.n_one / std::max(1ll, st.n)));
+                                   (int)(64.0 * st.n_one / std::max(1ll, st.n)), glv_ok);
   auto it = g_pick_cache.find(key);
   if (it != g_pick_cache.end()) {
     *c = it->second.first;
@@ -116,10 +140,11 @@ bool cost_model_pick(int curve, const zkg_scalar_stats &st, int *c, int *seg_log
     return true;
   }
   int cs[256], ss[256];
-  const int k = msm_candidates(curve, st, 256, cs, ss);
+  const int k = msm_candidates(curve, st, glv_ok, 256, cs, ss);
   if (k == 0) return false;
   std::vector<float> X((size_t)k * ZKG_MSM_NFEAT), y(k);
-  for (int i = 0; i < k; i++) msm_features(curve, st, cs[i], ss[i], X.data() + (size_t)i * ZKG_MSM_NFEAT);
+  for (int i = 0; i < k; i++)
+    msm_features(curve, st, cs[i], ss[i], glv_ok, X.data() + (size_t)i * ZKG_MSM_NFEAT);
   if (mlp_forward(g_model, k, X.data(), y.data()) != ZK_OK) return false;
   const int best = (int)(std::min_element(y.begin(), y.end()) - y.begin());
   *c = cs[best];

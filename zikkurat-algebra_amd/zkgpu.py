@@ -95,6 +95,7 @@ def lib():
         L.zkg_serve_create.argtypes = [ctypes.c_int, vp, ctypes.c_int, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(vp)]
         L.zkg_serve_submit.argtypes = [vp, ctypes.c_int, u64p, u64p, ctypes.POINTER(ctypes.c_longlong)]
         L.zkg_serve_drain.argtypes = [vp]
+        L.zkg_serve_set_exclusive_log.argtypes = [vp, ctypes.c_int]
         L.zkg_serve_job_times.argtypes = [vp, ctypes.c_longlong] + [ctypes.POINTER(ctypes.c_double)] * 3
         L.zkg_serve_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_int)]
         L.zkg_serve_run_trace.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),
@@ -797,6 +798,10 @@ class Server:
 
     def drain(self):
         _check(lib().zkg_serve_drain(self.h))
+
+    def set_exclusive_log(self, logn):
+        """No job starts while one of at least 2^logn points runs (0: admission by footprint only)."""
+        _check(lib().zkg_serve_set_exclusive_log(self.h, logn))
 
     def times(self, jid):
         q, r, l = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
