@@ -11,8 +11,22 @@
 //
 // Device form: points in XYZZ reduced-radix coordinates (one record per point, in HBM),
 // twiddles as standard-form scalars; each butterfly multiplies one point by its twiddle
-// (double-and-add over the scalar's bits) and adds / subtracts.  One launch per stage.
+// (double-and-add over the scalar's bits) and adds / subtracts.
+//
+// A scalar multiplication is a chain of ~127 (BN254, GLV) or ~255 dependent doublings, so a
+// stage of a small transform is one chain long however few threads it has, and one launch per
+// radix-2 stage makes the transform m chains long.  Grouped stages cut that depth: r radix-2
+// stages s0+1 .. s0+r map each group of 2^r points (spaced 2^s0 apart) by a 2^r x 2^r matrix
+// whose entries are +-(products of twiddles along the unique network path), so every scalar
+// multiplication of the group can run at once, followed (DIT) or preceded and followed (DIF)
+// by additions.  Entries equal up to sign are shared: (4^r - 1) / 3 scalar multiplications per
+// group instead of r 2^(r-1), in one chain instead of r.  The stage plan takes the largest
+// r <= 4 whose thread count still fits about one round of the GPU, so large transforms keep
+// the cheaper radix-2 stages.
 #include <string.h>
+
+#include <algorithm>
+#include <vector>
 
 #include "curve.hpp"
 #include "curvez.hpp"
@@ -174,6 +188,248 @@ __global__ __launch_bounds__(kG1Threads) void g1_scale_kernel(int32_t *__restric
   zxyzz_store(pts + (size_t)i * PW, r);
 }
 
+// ---- grouped stages ---------------------------------------------------------------------
+//
+// Group G = (ghi, p0), p0 < 2^s0: points b + q 2^s0, q < 2^r, with b = ghi 2^(s0+r) + p0.  In
+// the DIT network (stages s0+1 .. s0+r, t = 0 .. r-1) input q reaches output q' through
+// stage t on the multiplied side exactly when bit t of q is set, with twiddle
+// gen^((p0 + (q' mod 2^t) 2^s0) 2^(m - s0 - t - 1)) and sign -1 when bit t of q' is set too:
+//     y_q' = x_0 + sum_{q >= 1} (-1)^popcount(q & q') M(q, q' mod 2^h(q)) x_q,
+// h(q) the top bit of q.  In the DIF network (t = r-1 .. 0, twiddles of gen^-1) the roles of
+// q and q' swap:
+//     y_q' = sum_{L < 2^h(q')} M'(q', L) sum_{q mod 2^h(q') = L} (-1)^popcount(q & q') x_q,
+// and y_0 = sum_q x_q.  Slot k of a group, k < (4^r - 1) / 3, is the pair (index with top bit
+// h, L < 2^h): k = (4^h - 1) / 3 + ((index - 2^h) << h | L); slot (4^r - 1) / 3 holds x_0
+// (DIT) or y_0 before the final scale (DIF).
+
+__host__ __device__ __forceinline__ uint32_t g1_kcount(int r) { return ((1u << (2 * r)) - 1) / 3; }
+
+__device__ __forceinline__ void g1_slot_decode(uint32_t k, int &h, uint32_t &idx, uint32_t &L) {
+  uint32_t base = 0;
+  h = 0;
+  while (k >= base + (1u << (2 * h))) {
+    base += 1u << (2 * h);
+    h++;
+  }
+  const uint32_t i = k - base;
+  idx = (1u << h) | (i >> h);
+  L = i & ((1u << h) - 1);
+}
+__device__ __forceinline__ uint32_t g1_slot(int h, uint32_t idx, uint32_t L) {
+  return ((1u << (2 * h)) - 1) / 3 + (((idx - (1u << h)) << h) | L);
+}
+
+// M(idx, L) = product over set bits t of idx of tw[(p0 + (L mod 2^t) 2^s0) << (m - s0 - t - 1)]
+// (Montgomery powers of the stage generator; every index is below N / 2), times `extra` when
+// has_extra; returned in standard form for the scalar multiplication
+template <class C>
+__device__ __forceinline__ void g1_path_coeff(Fe<typename C::Fr> &c, const uint32_t *__restrict__ twm, int m, int s0,
+                                              uint32_t p0, uint32_t idx, uint32_t L, int h, bool has_extra,
+                                              const Fe<typename C::Fr> &extra) {
+  using FR = typename C::Fr;
+  bool have = false;
+  for (int t = 0; t <= h; t++) {
+    if (!((idx >> t) & 1u)) continue;
+    const uint64_t pos = p0 + ((uint64_t)(L & ((1u << t) - 1)) << s0);
+    Fe<FR> w;
+    fload(w, twm + (pos << (m - s0 - t - 1)) * FR::N);
+    if (have) fmul(c, c, w);
+    else c = w;
+    have = true;
+  }
+  if (!have) fset_one(c);
+  if (has_extra) fmul(c, c, extra);
+  fto_std(c, c);
+}
+
+// One thread per slot; slot k of group G lives at scr[G (K + 1) + k].  (Splitting a BN254 slot
+// into its two GLV halves on two threads doubles the doublings and measured 1.75x slower.)
+
+// Sum of the nterm terms term(t) of one output over 8 lanes: lane j adds terms j, j + 8, ...,
+// then three xor-shuffle rounds combine the 8 partial sums (a chain of nterm / 8 + 3 additions
+// instead of nterm).  All 8 lanes of an output group must call it.
+template <class C, class Term>
+__device__ __forceinline__ void g1_sum8(XyzzZ<C> &acc, uint32_t nterm, uint32_t lane, Term term) {
+  constexpr int N = ZP<C>::T::N;
+  XyzzZ<C> t, o;
+  zxyzz_set_inf(acc);
+  for (uint32_t i = lane; i < nterm; i += 8) {
+    term(i, t);
+    zxyzz_add(acc, acc, t);
+  }
+#pragma unroll
+  for (int off = 4; off >= 1; off >>= 1) {
+#pragma unroll
+    for (int l = 0; l < N; l++) {
+      o.x.l[l] = __shfl_xor(acc.x.l[l], off, 8);
+      o.y.l[l] = __shfl_xor(acc.y.l[l], off, 8);
+      o.zz.l[l] = __shfl_xor(acc.zz.l[l], off, 8);
+      o.zzz.l[l] = __shfl_xor(acc.zzz.l[l], off, 8);
+    }
+    zxyzz_add(acc, acc, o);
+  }
+}
+
+// DIT: slot k < K of group G = M(q, L) x_q; slot K = x_0
+template <class C>
+__global__ __launch_bounds__(kG1Threads) void g1_group_dit_mul_kernel(const int32_t *__restrict__ pts,
+                                                                     int32_t *__restrict__ scr,
+                                                                     const uint32_t *__restrict__ twm, int m, int s0,
+                                                                     int r, uint32_t ngroups) {
+  using FR = typename C::Fr;
+  constexpr int PW = ZLayout<C>::XYZZ;
+  const uint32_t K = g1_kcount(r);
+  const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= (uint64_t)ngroups * (K + 1)) return;
+  const uint32_t G = (uint32_t)(tid / (K + 1)), k = (uint32_t)(tid % (K + 1));
+  const uint32_t p0 = G & ((1u << s0) - 1), ghi = G >> s0;
+  const uint64_t b = ((uint64_t)ghi << (s0 + r)) | p0;
+  XyzzZ<C> x;
+  if (k == K) {
+    zxyzz_load(x, pts + b * PW);
+    zxyzz_store(scr + tid * PW, x);
+    return;
+  }
+  int h;
+  uint32_t q, L;
+  g1_slot_decode(k, h, q, L);
+  Fe<FR> c;
+  g1_path_coeff<C>(c, twm, m, s0, p0, q, L, h, false, c);
+  zxyzz_load(x, pts + (b + ((uint64_t)q << s0)) * PW);
+  XyzzZ<C> y;
+  scalar_mul<C>(y, x, c.w);
+  zxyzz_store(scr + tid * PW, y);
+}
+
+// DIT: y_q' = x_0 + sum_q (-1)^popcount(q & q') slot(q, q' mod 2^h(q)), 8 lanes per output
+template <class C>
+__global__ __launch_bounds__(kG1Threads) void g1_group_dit_sum_kernel(int32_t *__restrict__ pts,
+                                                                     const int32_t *__restrict__ scr, int s0, int r,
+                                                                     uint32_t ngroups) {
+  constexpr int PW = ZLayout<C>::XYZZ;
+  const uint32_t K = g1_kcount(r), Q = 1u << r;
+  const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t out = tid >> 3;
+  const uint32_t lane = (uint32_t)(tid & 7);
+  const bool live = out < (uint64_t)ngroups * Q;  // whole 8-lane groups are live or not
+  const uint32_t G = live ? (uint32_t)(out / Q) : 0u, qq = live ? (uint32_t)(out % Q) : 0u;
+  const int32_t *g = scr + (uint64_t)G * (K + 1) * PW;
+  // term 0: x_0; term q >= 1: slot (q, q' mod 2^h(q)) with its sign
+  const uint32_t nterm = live ? Q : 0u;
+  XyzzZ<C> acc;
+  g1_sum8<C>(acc, nterm, lane, [&](uint32_t q, XyzzZ<C> &t) {
+    if (q == 0) {
+      zxyzz_load(t, g + (uint64_t)K * PW);
+      return;
+    }
+    const int h = 31 - __clz(q);
+    zxyzz_load(t, g + (uint64_t)g1_slot(h, q, qq & ((1u << h) - 1)) * PW);
+    if (__popc(q & qq) & 1) zxyzz_neg(t, t);
+  });
+  if (live && lane == 0) {
+    const uint32_t p0 = G & ((1u << s0) - 1), ghi = G >> s0;
+    const uint64_t b = ((uint64_t)ghi << (s0 + r)) | p0;
+    zxyzz_store(pts + (b + ((uint64_t)qq << s0)) * PW, acc);
+  }
+}
+
+// DIF: slot k < K = M'(q', L) (sum over q mod 2^h = L of (-1)^popcount(q & q') x_q), times
+// `fin` (N^-1) on the last group; slot K = fin (sum of all x_q)
+template <class C>
+__global__ __launch_bounds__(kG1Threads) void g1_group_dif_mul_kernel(const int32_t *__restrict__ pts,
+                                                                     int32_t *__restrict__ scr,
+                                                                     const uint32_t *__restrict__ twm, int m, int s0,
+                                                                     int r, uint32_t ngroups, int has_fin,
+                                                                     Fe<typename C::Fr> fin) {
+  using FR = typename C::Fr;
+  constexpr int PW = ZLayout<C>::XYZZ;
+  const uint32_t K = g1_kcount(r), Q = 1u << r;
+  const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= (uint64_t)ngroups * (K + 1)) return;
+  const uint32_t G = (uint32_t)(tid / (K + 1)), k = (uint32_t)(tid % (K + 1));
+  const uint32_t p0 = G & ((1u << s0) - 1), ghi = G >> s0;
+  const uint64_t b = ((uint64_t)ghi << (s0 + r)) | p0;
+  XyzzZ<C> s, t;
+  int h = r;
+  uint32_t qq = 0, L = 0;
+  if (k < K) g1_slot_decode(k, h, qq, L);
+  // the inputs q with q mod 2^h = L (all of them for y_0)
+  const uint32_t nterm = k < K ? (Q >> h) : Q;
+  for (uint32_t j = 0; j < nterm; j++) {
+    const uint32_t q = k < K ? (L | (j << h)) : j;
+    zxyzz_load(t, pts + (b + ((uint64_t)q << s0)) * PW);
+    if (__popc(q & qq) & 1) zxyzz_neg(t, t);
+    if (j == 0) s = t;
+    else zxyzz_add(s, s, t);
+  }
+  Fe<FR> c;
+  if (k < K) {
+    g1_path_coeff<C>(c, twm, m, s0, p0, qq, L, h, has_fin != 0, fin);
+  } else if (has_fin) {
+    c = fin;
+    fto_std(c, c);
+  } else {
+    zxyzz_store(scr + tid * PW, s);
+    return;
+  }
+  XyzzZ<C> y;
+  scalar_mul<C>(y, s, c.w);
+  zxyzz_store(scr + tid * PW, y);
+}
+
+// DIF: y_0 = slot K; y_q' = sum_{L < 2^h(q')} slot(q', L); 8 lanes per output
+template <class C>
+__global__ __launch_bounds__(kG1Threads) void g1_group_dif_sum_kernel(int32_t *__restrict__ pts,
+                                                                     const int32_t *__restrict__ scr, int s0, int r,
+                                                                     uint32_t ngroups) {
+  constexpr int PW = ZLayout<C>::XYZZ;
+  const uint32_t K = g1_kcount(r), Q = 1u << r;
+  const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t out = tid >> 3;
+  const uint32_t lane = (uint32_t)(tid & 7);
+  const bool live = out < (uint64_t)ngroups * Q;
+  const uint32_t G = live ? (uint32_t)(out / Q) : 0u, qq = live ? (uint32_t)(out % Q) : 0u;
+  const int32_t *g = scr + (uint64_t)G * (K + 1) * PW;
+  const int h = qq ? 31 - __clz(qq) : 0;
+  const uint32_t nterm = live ? (qq ? (1u << h) : 1u) : 0u;
+  XyzzZ<C> acc;
+  g1_sum8<C>(acc, nterm, lane, [&](uint32_t L, XyzzZ<C> &t) {
+    const uint32_t k = qq ? g1_slot(h, qq, L) : K;
+    zxyzz_load(t, g + (uint64_t)k * PW);
+  });
+  if (live && lane == 0) {
+    const uint32_t p0 = G & ((1u << s0) - 1), ghi = G >> s0;
+    const uint64_t b = ((uint64_t)ghi << (s0 + r)) | p0;
+    zxyzz_store(pts + (b + ((uint64_t)qq << s0)) * PW, acc);
+  }
+}
+
+// tab[j] = g^j in Montgomery form, j < cnt
+template <class C>
+__global__ __launch_bounds__(kG1Threads) void g1_twiddle_mont_kernel(uint32_t *__restrict__ tab,
+                                                                    Fe<typename C::Fr> g, uint32_t cnt) {
+  using FR = typename C::Fr;
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= cnt) return;
+  Fe<FR> r;
+  fpow_u64(r, g, j);
+  fstore(tab + (size_t)j * FR::N, r);
+}
+
+// Stage plan: group sizes r summing to m, each the largest r <= 4 (and <= what remains) whose
+// launch, ngroups (K_r + 1) threads, stays within about one round of resident threads; r = 1
+// keeps the radix-2 kernel
+static void g1_stage_plan(int m, uint64_t round_threads, std::vector<int> &rs) {
+  rs.clear();
+  int left = m;
+  while (left > 0) {
+    int r = std::min(4, left);
+    while (r > 1 && ((1ull << m) >> r) * (g1_kcount(r) + 1) > round_threads) r--;
+    rs.push_back(r);
+    left -= r;
+  }
+}
+
 template <class C>
 static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const uint64_t *d_src, uint64_t *d_dst,
                       hipStream_t st) {
@@ -189,37 +445,91 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
     }
     g = zkh::inv(fr, g);
   }
-  DevBuf pts, tw;
+  DevBuf pts, tw, twm, scr;
+  StreamSyncGuard drain(st);  // the buffers outlive the queued kernels on every return
   ZK_TRY(pts.alloc((size_t)N * ZW * 4));
   ZK_TRY(tw.alloc((size_t)(halfN ? halfN : 1) * 32));
+  Context *ctx = nullptr;
+  ZK_TRY(context(&ctx));
+  std::vector<int> plan;
+  g1_stage_plan(m, (uint64_t)ctx->num_cus * 4 * 3 * 64, plan);
+  uint64_t scr_pts = 0;
+  bool grouped = false;
+  for (int r : plan)
+    if (r > 1) {
+      grouped = true;
+      scr_pts = std::max<uint64_t>(scr_pts, (uint64_t)(N >> r) * (g1_kcount(r) + 1));
+    }
+  if (grouped) {
+    ZK_TRY(twm.alloc((size_t)(halfN ? halfN : 1) * 32));
+    ZK_TRY(scr.alloc(scr_pts * ZW * 4));
+  }
   auto to_fr = [&](const zkh::Fe &x) {
     Fe<FR> d;
     for (int j = 0; j < 8; j++) d.w[j] = (uint32_t)(x.w[j / 2] >> (32 * (j & 1)));
     return d;
   };
   const dim3 blk(kG1Threads), gN((N + kG1Threads - 1) / kG1Threads), gH((halfN + kG1Threads - 1) / kG1Threads);
+  // Montgomery form of g for the path products (to_fr gives the words of the Montgomery value)
   if (halfN) hipLaunchKernelGGL(g1_twiddle_kernel<C>, gH, blk, 0, st, tw.as<uint32_t>(), to_fr(g), halfN);
+  if (halfN && grouped)
+    hipLaunchKernelGGL(g1_twiddle_mont_kernel<C>, gH, blk, 0, st, twm.as<uint32_t>(), to_fr(g), halfN);
   if (jac)
     hipLaunchKernelGGL((g1_proj_in_kernel<C, true>), gN, blk, 0, st, (const uint32_t *)d_src, N, m, inverse ? 0 : 1,
                        pts.as<int32_t>());
   else
     hipLaunchKernelGGL((g1_proj_in_kernel<C, false>), gN, blk, 0, st, (const uint32_t *)d_src, N, m, inverse ? 0 : 1,
                        pts.as<int32_t>());
+  auto grid = [](uint64_t threads) { return dim3((unsigned)((threads + kG1Threads - 1) / kG1Threads)); };
   if (!inverse) {
-    for (int s = 1; s <= m; s++)
-      hipLaunchKernelGGL((g1_fft_stage_kernel<C, false>), gH, blk, 0, st, pts.as<int32_t>(), tw.as<uint32_t>(), m, s);
-  } else {
-    for (int s = m; s >= 1; s--)
-      hipLaunchKernelGGL((g1_fft_stage_kernel<C, true>), gH, blk, 0, st, pts.as<int32_t>(), tw.as<uint32_t>(), m, s);
-    if (m > 0) {
-      const zkh::Fe ninv = zkh::to_std(fr, zkh::inv(fr, zkh::from_u64(fr, N)));
+    int s0 = 0;
+    for (int r : plan) {
+      if (r == 1) {
+        hipLaunchKernelGGL((g1_fft_stage_kernel<C, false>), gH, blk, 0, st, pts.as<int32_t>(), tw.as<uint32_t>(), m,
+                           s0 + 1);
+      } else {
+        const uint32_t ng = N >> r;
+        hipLaunchKernelGGL(g1_group_dit_mul_kernel<C>, grid((uint64_t)ng * (g1_kcount(r) + 1)), blk,
+                           0, st,
+                           (const int32_t *)pts.as<int32_t>(), scr.as<int32_t>(), twm.as<uint32_t>(), m, s0, r, ng);
+        hipLaunchKernelGGL(g1_group_dit_sum_kernel<C>, grid(((uint64_t)ng << r) * 8), blk, 0, st, pts.as<int32_t>(),
+                           (const int32_t *)scr.as<int32_t>(), s0, r, ng);
+      }
+      s0 += r;
+    }
+  } else if (m > 0) {
+    // N^-1 folds into the last group when it is a grouped stage, else a scaling pass follows
+    const zkh::Fe ninv_m = zkh::inv(fr, zkh::from_u64(fr, N));
+    int s0 = m;
+    bool folded = false;
+    for (size_t i = plan.size(); i-- > 0;) {
+      const int r = plan[i];
+      s0 -= r;
+      if (r == 1) {
+        hipLaunchKernelGGL((g1_fft_stage_kernel<C, true>), gH, blk, 0, st, pts.as<int32_t>(), tw.as<uint32_t>(), m,
+                           s0 + 1);
+      } else {
+        const uint32_t ng = N >> r;
+        const int fin = s0 == 0 ? 1 : 0;
+        folded = folded || fin;
+        hipLaunchKernelGGL(g1_group_dif_mul_kernel<C>, grid((uint64_t)ng * (g1_kcount(r) + 1)), blk,
+                           0, st,
+                           (const int32_t *)pts.as<int32_t>(), scr.as<int32_t>(), twm.as<uint32_t>(), m, s0, r, ng,
+                           fin, to_fr(ninv_m));
+        hipLaunchKernelGGL(g1_group_dif_sum_kernel<C>, grid(((uint64_t)ng << r) * 8), blk, 0, st, pts.as<int32_t>(),
+                           (const int32_t *)scr.as<int32_t>(), s0, r, ng);
+      }
+    }
+    if (!folded) {
+      const zkh::Fe ninv = zkh::to_std(fr, ninv_m);
       hipLaunchKernelGGL(g1_scale_kernel<C>, gN, blk, 0, st, pts.as<int32_t>(), N, to_fr(ninv));
     }
   }
   hipLaunchKernelGGL(g1_proj_out_kernel<C>, gN, blk, 0, st, pts.as<int32_t>(), N, m, inverse ? 1 : 0,
                      (uint32_t *)d_dst);
   ZK_HIP(hipGetLastError());
-  ZK_HIP(hipStreamSynchronize(st));  // pts / tw are released on return
+  ZK_HIP(hipStreamSynchronize(st));  // pts / tw / twm / scr are released on return
+  drain.release();
   return ZK_OK;
 }
 
