@@ -221,6 +221,26 @@ def test_zero_test_norm_and_canon(h, field):
 
 
 @pytest.mark.parametrize("field", [0, 1, 2, 3], ids=["bn-fp", "bn-fr", "bls-fp", "bls-fr"])
+def test_reduce_small_then_canon(h, field):
+    """zreduce_small (the forward NTT's last step in place of a product by one) takes a normalised
+    value with |a| < 64 p into (-p, 2p), and zcanon then gives a mod p: multiples of p and their
+    neighbours (where the quotient estimate is closest to a rounding edge) and random values."""
+    p, bits, N, _ = FIELDS[field]
+    rng = splitmix64(11 + field)
+    edge = [j * p + d for j in range(-63, 64) for d in (0, 1, -1, 1 << bits, -(1 << bits), (1 << (bits * (N - 2))) - 1)]
+    vals = edge + sample_values(p, rng, 300, 63) + [64 * p - 1, -(64 * p - 1)]
+    A = arr([to_limbs(v, bits, N) for v in vals])
+    R = op(h, field, 11, A)
+    for i, v in enumerate(vals):
+        assert is_normalised(R[i], bits), v
+        w = value(R[i], bits)
+        assert -p < w < 2 * p and (w - v) % p == 0, v
+    Cc = op(h, field, 4, R)
+    for i, v in enumerate(vals):
+        assert value(Cc[i], bits) == v % p
+
+
+@pytest.mark.parametrize("field", [0, 1, 2, 3], ids=["bn-fp", "bn-fr", "bls-fp", "bls-fr"])
 def test_external_conversion_roundtrip(h, field):
     p, bits, N, W = FIELDS[field]
     Rp, R = 1 << (bits * N), 1 << (32 * W)

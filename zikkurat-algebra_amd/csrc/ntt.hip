@@ -44,7 +44,8 @@ struct PassArgs {
   const uint32_t *eta_pre;  // forward shifted: multiply input i by eta^i (first pass only)
   const uint32_t *eta_post; // inverse shifted: multiply output i by eta^i (last pass only)
   uint32_t ninv[8];         // last pass factor, internal form: N^-1 R' (inverse) or R' (forward)
-  int logn, s, logc, logr, logt, last, inverse, tw_shift, has_ninv;
+  int logn, s, logc, logr, logt, last, inverse, tw_shift;
+  int has_ninv;             // last pass: the factor is not one (inverse, or a coset post-scale)
   int nsplit;               // number of passes before this (last pass: digit reversal)
   int split[6];             // log sizes of the earlier passes
 };
@@ -285,13 +286,17 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t blk, i
       go = (((r << a.s) + k) << a.logc) | col;
     } else {
       go = (rho0 + c) + ((uint64_t)k << a.logr);
-      E f = ninv;
-      if (a.eta_post) {
-        E e;
-        eta_pow_z(e, a.eta_post, go);
-        zmul(f, f, e);
+      if (a.has_ninv) {
+        E f = ninv;
+        if (a.eta_post) {
+          E e;
+          eta_pow_z(e, a.eta_post, go);
+          zmul(f, f, e);
+        }
+        zmul(x, x, f);
+      } else {
+        zreduce_small(x, x);  // forward transform: the factor is one
       }
-      zmul(x, x, f);
     }
     zcanon(x, x);
     if (neg) zneg_canon(x, x);
@@ -655,7 +660,7 @@ static int ntt_fast(const NttPlan &plan, const uint32_t *d_in, uint32_t *d_out, 
     a.eta_pre = p == 0 ? eta_pre : nullptr;
     a.eta_post = last ? eta_post : nullptr;
     if (last) {
-      a.has_ninv = 1;
+      a.has_ninv = inverse || eta_post;
       for (int j = 0; j < 8; j++) a.ninv[j] = last_fac[j];
     }
     int tl = plan.tile_log - a.s;
@@ -909,7 +914,7 @@ static int ntt_batch_create_t(int curve, int njobs, const NttBatchJob *jobs, Ntt
       a.eta_pre = (p == 0 && !inverse) ? etab : nullptr;
       a.eta_post = (last && inverse) ? etab : nullptr;
       if (last) {
-        a.has_ninv = 1;
+        a.has_ninv = inverse || a.eta_post;
         for (int k = 0; k < 8; k++) a.ninv[k] = lastf[k];
       }
       int tl = pj.tile_log - a.s;
