@@ -356,9 +356,21 @@ def bench_learned_msm(njobs=24, seed=3, nstreams=4):
                                                           nstreams=nstreams)[0])
     t_learned_batch, out_lb = timed(lambda: zkgpu.msm_batch(curves, ns, dS, dP, policy=zkgpu.SCHED_LEARNED,
                                                             nstreams=nstreams)[0])
+    # the captured form: one hipGraph of every job's pipeline, replayed (plan built once)
+    graphs = {}
+    for name, pol in (("plain", zkgpu.SCHED_PLAIN), ("learned", zkgpu.SCHED_LEARNED)):
+        zkgpu.synchronize()
+        t0 = time.perf_counter()
+        plan = zkgpu.MsmBatch(curves, ns, dS, dP, policy=pol, nstreams=nstreams)
+        t_create = time.perf_counter() - t0
+        t_run, out = timed(lambda: plan.run()[0])
+        plan.close()
+        graphs[name] = (t_create, t_run, out)
     zkgpu.Mlp.uninstall()
-    same = all((a == b).all() for o in (out_seq, out_pb, out_lb) for a, b in zip(o, ref))
+    same = all((a == b).all() for o in (out_seq, out_pb, out_lb, graphs["plain"][2], graphs["learned"][2])
+               for a, b in zip(o, ref))
     pts_total = sum(ns)
+    t_lg = graphs["learned"][1]
     return {"learned_msm_jobs": njobs, "learned_msm_points": pts_total,
             "learned_msm_mix": "bn254+bls12-381, 2^14..2^22 log-uniform, uniform/u64/sparse scalars",
             "learned_msm_plain_ms": round(t_plain * 1e3, 3),
@@ -366,6 +378,11 @@ def bench_learned_msm(njobs=24, seed=3, nstreams=4):
             "learned_msm_plain_batched_ms": round(t_plain_batch * 1e3, 3),
             "learned_msm_learned_batched_ms": round(t_learned_batch * 1e3, 3),
             "learned_msm_batched_learned_vs_plain": round(t_plain / t_learned_batch, 3),
+            "learned_msm_graph_plain_ms": round(graphs["plain"][1] * 1e3, 3),
+            "learned_msm_graph_learned_ms": round(t_lg * 1e3, 3),
+            "learned_msm_graph_create_ms": round(graphs["learned"][0] * 1e3, 1),
+            "learned_msm_graph_learned_vs_plain": round(t_plain / t_lg, 3),
+            "learned_msm_graph_learned_mpts": round(pts_total / t_lg / 1e6, 1),
             "learned_msm_plain_mpts": round(pts_total / t_plain / 1e6, 1),
             "learned_msm_learned_batched_mpts": round(pts_total / t_learned_batch / 1e6, 1),
             "learned_msm_streams": nstreams, "learned_msm_model": Path(str(stem)).name,

@@ -370,6 +370,18 @@ int zkg_msm_tune(int curve, int npoints, const uint64_t *d_expos, int expo_nlimb
 int zkg_msm_batch(int njobs, const int *curves, const int *npoints, const uint64_t *const *d_expos, int expo_nlimbs,
                   const uint64_t *const *d_grps, uint64_t *const *outs, int flags, const zkg_msm_schedule *sched,
                   int nstreams, double *wall_ms);
+/* The same job list as one captured plan: create() fits each job's schedule (sched->policy, on
+ * the scalars present then), allocates every job's buffers and captures every job's pipeline
+ * into one hipGraph whose nstreams branches run concurrently; run() replays it once, waits, and
+ * writes outs[k] (flags as at create).  The device buffers must stay valid for the plan's
+ * lifetime; run() rereads their contents, and returns ZKG_EINVAL if a scalar is wider than the
+ * windows fitted at create (statistics policies).  Results are those of one zkg_msm per job.
+ * The reference runs such jobs one zkg_msm-like call at a time (bn128_G1_proj.c:506-586). */
+int zkg_msm_batch_create(int njobs, const int *curves, const int *npoints, const uint64_t *const *d_expos,
+                         int expo_nlimbs, const uint64_t *const *d_grps, int flags, const zkg_msm_schedule *sched,
+                         int nstreams, void **plan);
+int zkg_msm_batch_run(void *plan, uint64_t *const *outs, double *wall_ms);
+int zkg_msm_batch_destroy(void *plan);
 
 /* Dense fp32 MLP (ReLU hidden layers, linear output, MSE loss, Adam) run by the library's
  * own HIP kernels: the schedule cost model.  dims has nlayers+1 entries; parameters are

@@ -4,6 +4,7 @@
 // result back before returning, like the single-threaded C it replaces.
 #include <string.h>
 
+#include <memory>
 #include <mutex>
 #include <vector>
 
@@ -914,6 +915,39 @@ int zkg_msm_batch(int njobs, const int *curves, const int *npoints, const uint64
   return msm_batch(njobs, curves, npoints, d_expos, expo_nlimbs, flags & ZKG_MSM_MONT, d_grps,
                    (flags & ZKG_MSM_POINTS_INTERNAL) != 0, outs, flags & (ZKG_MSM_PROJ_OUT | ZKG_MSM_JAC_OUT), sched,
                    nstreams, wall_ms);
+}
+
+struct MsmBatchPlan {
+  MsmGraph *g = nullptr;
+  int out_flags = 0;
+};
+int zkg_msm_batch_create(int njobs, const int *curves, const int *npoints, const uint64_t *const *d_expos,
+                         int expo_nlimbs, const uint64_t *const *d_grps, int flags, const zkg_msm_schedule *sched,
+                         int nstreams, void **plan) {
+  *plan = nullptr;
+  for (int k = 0; k < njobs; k++) ZK_TRY(check_msm_args(curves[k], npoints[k], expo_nlimbs, flags));
+  Context *ctx;
+  ZK_TRY(with_ctx(&ctx));
+  std::unique_ptr<MsmBatchPlan> pl(new MsmBatchPlan);
+  pl->out_flags = flags & (ZKG_MSM_PROJ_OUT | ZKG_MSM_JAC_OUT);
+  ZK_TRY(msm_graph_create(njobs, curves, npoints, d_expos, expo_nlimbs, flags & ZKG_MSM_MONT, d_grps,
+                          (flags & ZKG_MSM_POINTS_INTERNAL) != 0, sched, nstreams, &pl->g));
+  *plan = pl.release();
+  return ZK_OK;
+}
+int zkg_msm_batch_run(void *plan, uint64_t *const *outs, double *wall_ms) {
+  MsmBatchPlan *pl = (MsmBatchPlan *)plan;
+  if (!pl || !pl->g) {
+    set_error("msm_batch_run: no plan");
+    return ZK_EINVAL;
+  }
+  return msm_graph_run(pl->g, outs, pl->out_flags, wall_ms);
+}
+int zkg_msm_batch_destroy(void *plan) {
+  MsmBatchPlan *pl = (MsmBatchPlan *)plan;
+  if (pl) msm_graph_destroy(pl->g);
+  delete pl;
+  return ZK_OK;
 }
 
 // ---------------- cost-model network ----------------

@@ -33,6 +33,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -60,6 +61,16 @@ __device__ __forceinline__ uint32_t window_bits(const uint64_t *__restrict__ s, 
   uint64_t v = s[limb] >> off;
   if (off + c > 64 && limb + 1 < enl) v |= s[limb + 1] << (64 - off);
   return (uint32_t)(v & ((1ull << c) - 1));
+}
+
+// true when s has a set bit at position `from` or above
+__device__ __forceinline__ bool bits_from(const uint64_t *__restrict__ s, int enl, int from) {
+  uint64_t any = 0;
+  for (int j = 0; j < enl; j++) {
+    if (64 * (j + 1) <= from) continue;
+    any |= 64 * j >= from ? s[j] : s[j] >> (from - 64 * j);
+  }
+  return any != 0;
 }
 
 __device__ __forceinline__ uint32_t encode_digit(uint32_t raw, int c, uint32_t &carry) {
@@ -104,11 +115,14 @@ __device__ __forceinline__ void point_to_internal(const uint32_t *__restrict__ p
 // signed digits of scalar i for every window (row w at dig + w * ns); when ext_pts is given,
 // point i is converted to the internal record in the same pass (both sweeps are bound by
 // memory traffic).  With glv (BN254, 4-limb scalars reduced mod r) scalar i becomes k1 for
-// point i and k2 for point n + i = phi(P_i), whose record is written too (ns = 2n).
+// point i and k2 for point n + i = phi(P_i), whose record is written too (ns = 2n).  With ovf,
+// a scalar the windows do not cover (a carry out of the top window or a bit above it: the
+// schedule was fitted to other scalars) sets *ovf.
 template <class C>
 __global__ __launch_bounds__(kThreads) void msm_digits_kernel(const uint64_t *__restrict__ sc, int enl, int mont,
                                                              int reduce, int glv, uint32_t n, uint32_t ns, int c,
-                                                             int W, int wide, uint32_t *__restrict__ dig,
+                                                             int W, int wide, uint32_t *__restrict__ ovf,
+                                                             uint32_t *__restrict__ dig,
                                                              const uint32_t *__restrict__ ext_pts,
                                                              int32_t *__restrict__ zpts) {
   using Fr = typename C::Fr;
@@ -177,6 +191,7 @@ __global__ __launch_bounds__(kThreads) void msm_digits_kernel(const uint64_t *__
       dig[(size_t)w * ns + i] = encode_digit(raw, cw, carry);
       bit += cw;
     }
+    if (ovf && (carry || bits_from(s, 4, bit))) atomicOr(ovf, 1u);
   } else {
     const uint64_t *s = sc + (size_t)i * enl;
     int bit = 0;
@@ -186,6 +201,7 @@ __global__ __launch_bounds__(kThreads) void msm_digits_kernel(const uint64_t *__
       dig[(size_t)w * ns + i] = encode_digit(raw, cw, carry);
       bit += cw;
     }
+    if (ovf && (carry || bits_from(s, enl, bit))) atomicOr(ovf, 1u);
   }
 }
 
@@ -916,6 +932,52 @@ __global__ __launch_bounds__(kThreads) void msm_heavy_final_kernel(const uint32_
   zxyzz_store(bsum + (size_t)b * PW, a);
 }
 
+// Heavy buckets without a host round trip (captured pipelines): workgroup g takes listed buckets
+// g, g + gridDim.x, ...; its threads sum the bucket's continuation partials j = tid, tid + 256,
+// ... serially, a tree in LDS adds the 256 partial sums, and thread 0 adds the total into the
+// bucket sum.  heavy[0] (the count) is read on the device, so the launch size is fixed.
+static constexpr uint32_t kHeavyTreeBlocks = 1024;
+
+template <class C>
+__global__ __launch_bounds__(kThreads) void msm_heavy_tree_kernel(const uint32_t *__restrict__ off,
+                                                                 const uint32_t *__restrict__ heavy,
+                                                                 uint32_t heavy_cap, int seg_log,
+                                                                 int32_t *__restrict__ bsum,
+                                                                 const int32_t *__restrict__ cont) {
+  constexpr int PW = ZLayout<C>::XYZZ;
+  extern __shared__ int32_t part[];  // kThreads XYZZ points
+  const uint32_t nh = min(heavy[0], heavy_cap);
+  const uint32_t t = threadIdx.x;
+  for (uint32_t h = blockIdx.x; h < nh; h += gridDim.x) {
+    const uint32_t b = heavy[3 + h];
+    const uint64_t fs = off[b] >> seg_log, ls = (uint64_t)(off[b + 1] - 1) >> seg_log;
+    XyzzZ<C> acc, q;
+    zxyzz_set_inf(acc);
+    for (uint64_t j = fs + 1 + t; j <= ls; j += kThreads) {
+      zxyzz_load(q, cont + j * PW);
+      zxyzz_add(acc, acc, q);
+    }
+    zxyzz_store(part + t * PW, acc);
+    __syncthreads();
+    for (uint32_t s = kThreads / 2; s > 0; s >>= 1) {
+      if (t < s) {
+        zxyzz_load(acc, part + t * PW);
+        zxyzz_load(q, part + (t + s) * PW);
+        zxyzz_add(acc, acc, q);
+        zxyzz_store(part + t * PW, acc);
+      }
+      __syncthreads();
+    }
+    if (t == 0) {
+      zxyzz_load(acc, bsum + (size_t)b * PW);
+      zxyzz_load(q, part);
+      zxyzz_add(acc, acc, q);
+      zxyzz_store(bsum + (size_t)b * PW, acc);
+    }
+    __syncthreads();  // part is reused by the next bucket
+  }
+}
+
 // ---------------------------------------------------------------------------------------
 // 7. bucket pyramid: input [A][W][M] -> output [A+1][W][M/2]
 
@@ -1439,158 +1501,161 @@ static int msm_sparse_run(int n, const uint64_t *d_scalars, int enl, int mont, c
                           bool points_internal, const zkg_msm_schedule *user, hipStream_t st,
                           const zkg_scalar_stats &stats, MsmResult *out);
 
+// ---------------------------------------------------------------------------------------
+// One dense MSM pipeline: its schedule, launch sizes and device buffers.  msm_run drives it
+// with one host read between launches (the heavy-bucket count, fetched while the accumulation
+// runs); the captured batch (msm_graph_*) queues the same launches with the heavy buckets
+// reduced by one device-driven kernel, so nothing from the digits to the result copy waits
+// on the host.
+
 template <class C>
-static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const uint64_t *d_points,
-                   bool points_internal, const zkg_msm_schedule *user, hipStream_t st, MsmResult *out) {
+struct MsmPipe {
   using F = typename C::Fp;
-  constexpr int PW = 4 * F::N;               // u32 words per external XYZZ point (host side)
-  constexpr int ZW = ZLayout<C>::XYZZ;       // words per internal XYZZ point (device buffers)
-  constexpr int AW = ZLayout<C>::AFF;        // words per internal affine point
-  zkg_scalar_stats stats;
-  const bool use_stats = user && user->policy != ZKG_SCHED_PLAIN;
-  if (use_stats) ZK_TRY(msm_scalar_stats(C::id, n, d_scalars, enl, mont, st, &stats));
-  // Measured sparse scalar sets (at least half the scalars 0 or 1): the other scalars and
-  // their points are compacted into a dense MSM of their own, the points of the scalars
-  // equal to 1 are summed directly, and the zeros drop out; the two sums are added on the host.
-  // Zeros and ones cost the plain pipeline little beyond their digits, so the compaction pays
-  // only while its own passes are small next to the MSM: measured faster up to 2^20 points,
-  // slower from 2^21 (BN254 2^22, 80% in {0, 1}: 2.82 ms compacted against 2.52 ms).
-  if (use_stats && n >= (1 << 14) && n < (1 << 21) && (uint64_t)(stats.n_zero + stats.n_one) * 2 >= (uint64_t)n)
-    return msm_sparse_run<C>(n, d_scalars, enl, mont, d_points, points_internal, user, st, stats, out);
+  static constexpr int PW = 4 * F::N;          // u32 words per external XYZZ point (host side)
+  static constexpr int ZW = ZLayout<C>::XYZZ;  // words per internal XYZZ point (device buffers)
+  static constexpr int AW = ZLayout<C>::AFF;   // words per internal affine point
+  int n = 0, enl = 4, mont = 0;
+  const uint64_t *d_scalars = nullptr, *d_points = nullptr;
+  bool points_internal = false, prefetch = false;
+  uint32_t *ovf = nullptr;  // set by the digits pass when a scalar exceeds sch.bits (captured plans)
   MsmSchedule sch;
-  ZK_TRY(msm_resolve(C::id, n, enl, user, use_stats ? &stats : nullptr, &sch, !points_internal));
-  const bool glv = sch.glv != 0;
-  const uint32_t np = glv ? 2u * (uint32_t)n : (uint32_t)n;  // points of the bucket problem
-  const int c = sch.c, W = sch.windows, logB = c - 1;
-  const uint32_t B = 1u << logB;
-  const uint64_t nkeys = (uint64_t)W * B;
-  if (nkeys >= (1ull << 32) || (uint64_t)W * np >= (1ull << 32) || np >= (1u << 31)) {
-    set_error("msm: problem too large for 32-bit sort keys");
-    return ZK_EINVAL;
-  }
-  const uint64_t total = (uint64_t)W * np;
-  const int seg_log = sch.seg_log;
-  const uint64_t nseg = (total >> seg_log) + 1;
-
-  const int logP = sort_partition_bits((int)np, logB);
-  const int L = logB - logP;
-  const uint32_t P = 1u << logP;
-  const uint32_t nchunks = (uint32_t)(((uint64_t)np + kPChunk - 1) / kPChunk);
-  const uint64_t nph = (uint64_t)W * P * nchunks;  // level-1 counts, scanned in place of positions
-
-  // every buffer the queued kernels touch lives until the stream is drained (the guard below)
+  uint32_t limit = 16;
+  uint32_t np = 0, B = 0, P = 0, nchunks = 0, nsb = 0, heavy_cap = 0, A = 0, nres = 0;
+  int c = 0, W = 0, logB = 0, seg_log = 0, L = 0, ib = 0;
+  int pyr_final = -1;  // buffer holding the pyramid's output: -1 bsum, 0 pyrA, 1 pyrB
+  bool narrow = false;
+  uint64_t nkeys = 0, total = 0, nseg = 0, nph = 0, nbig_cap = 0;
   DevBuf dig, vals, tmp, phist, ppos, big, rcnt, off, bsum, cont, heavy, pyrA, pyrB, scan_b, zpts, outx, own;
-  StreamSyncGuard drain(st);
-  ZK_TRY(dig.alloc(total * 4));
-  ZK_TRY(vals.alloc(total * 4));
-  // level-1 words are 32-bit when the point index, the sign and the low bucket bits fit
-  int ib = 0;
-  while ((1ull << ib) < (uint64_t)np) ib++;
-  const bool narrow = ib < 31 && ib + 1 + L <= 32;
-  ZK_TRY(tmp.alloc(total * (narrow ? 4 : 8)));
-  ZK_TRY(phist.alloc(nph * 4));
-  ZK_TRY(ppos.alloc((nph + 1) * 4));
-  // big partitions hold more than kBigPart entries each, so there are at most total / kBigPart
-  const uint64_t nbig_cap = std::min<uint64_t>((uint64_t)W * P, total / kBigPart + 1);
-  ZK_TRY(big.alloc(((uint64_t)W * P + 1) * 4));
-  ZK_TRY(rcnt.alloc(nbig_cap * nchunks * ((uint64_t)1 << L) * 4));
-  ZK_TRY(off.alloc((nkeys + 1) * 4));
-  ZK_TRY(bsum.alloc(nkeys * ZW * 4));
-  ZK_TRY(cont.alloc(nseg * ZW * 4));
-  if (!points_internal) ZK_TRY(zpts.alloc((uint64_t)np * AW * 4));
-  // every bucket may be heavy (small seg_log, few entries per segment)
-  const uint32_t heavy_cap = (uint32_t)nkeys;
-  ZK_TRY(heavy.alloc(((uint64_t)heavy_cap + 3) * 4));
-  ZK_TRY(pyrA.alloc(nkeys * ZW * 4));
-  ZK_TRY(pyrB.alloc(nkeys * ZW * 4));
-  const uint64_t per_block = (uint64_t)kThreads * kScanPer;
-  const uint32_t nsb = (uint32_t)((nph + 1 + per_block - 1) / per_block);
-  ZK_TRY(scan_b.alloc((uint64_t)nsb * 4));
-  static std::once_flag attr_once;
-  int attr_status = ZK_OK;
-  std::call_once(attr_once, [&] { attr_status = msm_set_lds_attributes(); });
-  ZK_TRY(attr_status);
 
-  Events ev;
-  auto t_host0 = std::chrono::steady_clock::now();
-  ev.mark(st);
-  const dim3 blk(kThreads);
-  const dim3 gpts((n + kThreads - 1) / kThreads);
-  const int32_t *zp = points_internal ? (const int32_t *)d_points : zpts.as<int32_t>();
-  hipLaunchKernelGGL(msm_digits_kernel<C>, gpts, blk, 0, st, d_scalars, enl, mont, sch.reduce, (int)glv, (uint32_t)n,
-                     np, c, W, sch.wide,
-                     dig.as<uint32_t>(), points_internal ? (const uint32_t *)nullptr : (const uint32_t *)d_points,
-                     zpts.as<int32_t>());
-  const dim3 gch(nchunks, W);
-  hipLaunchKernelGGL(msm_part_hist_kernel, gch, dim3(kSortThreads), (size_t)P * 4, st, dig.as<uint32_t>(),
-                     np, L, P, phist.as<uint32_t>(), heavy.as<uint32_t>(), big.as<uint32_t>());
-  hipLaunchKernelGGL(scan_block_sums_kernel, dim3(nsb), blk, 0, st, phist.as<uint32_t>(), (size_t)nph,
-                     scan_b.as<uint32_t>());
-  hipLaunchKernelGGL(scan_final_kernel, dim3(nsb), blk, 0, st, phist.as<uint32_t>(), (size_t)nph,
-                     scan_b.as<uint32_t>(), ppos.as<uint32_t>(), (uint32_t *)nullptr);
-  const uint32_t limit = (user && user->fix_limit > 0) ? (uint32_t)user->fix_limit : 16u;
-  const size_t cnt_bytes = ((size_t)1 << std::max(L, 1)) * 4;
-  // level 2: partitions of at most kBigPart words, one workgroup each; larger ones run by run
-  auto level2 = [&](auto word) {
-    using T = decltype(word);
-    const uint32_t stage = (uint32_t)(kPartStageBytes / sizeof(T));
-    hipLaunchKernelGGL(msm_part_sort_kernel<T>, dim3((unsigned)(W * P)), blk,
-                       cnt_bytes + kPartStageBytes + (size_t)stage * 4, st, tmp.as<T>(), ppos.as<uint32_t>(), nchunks,
-                       L, ib, stage, off.as<uint32_t>(), vals.as<uint32_t>(), big.as<uint32_t>());
-    const unsigned gb = (unsigned)std::min<uint64_t>((uint64_t)nbig_cap * nchunks, 1024);
-    hipLaunchKernelGGL(msm_big_count_kernel<T>, dim3(gb), blk, cnt_bytes, st, tmp.as<T>(), ppos.as<uint32_t>(),
-                       nchunks, L, ib, big.as<uint32_t>(), rcnt.as<uint32_t>());
-    hipLaunchKernelGGL(msm_big_scan_kernel, dim3((unsigned)std::min<uint64_t>(nbig_cap, 256)), blk, 0, st,
-                       ppos.as<uint32_t>(), nchunks, L, big.as<uint32_t>(), rcnt.as<uint32_t>(), off.as<uint32_t>());
-    hipLaunchKernelGGL(msm_big_place_kernel<T>, dim3(gb), blk, cnt_bytes, st, tmp.as<T>(), ppos.as<uint32_t>(),
-                       nchunks, L, ib, big.as<uint32_t>(), rcnt.as<uint32_t>(), vals.as<uint32_t>(),
-                       off.as<uint32_t>(), (uint32_t)nkeys, seg_log, heavy.as<uint32_t>(), heavy_cap, limit);
-  };
-  if (narrow) {
-    hipLaunchKernelGGL(msm_part_scatter_staged_kernel, gch, dim3(kSortThreads), kScatterStageLds, st,
-                       dig.as<uint32_t>(), np, L, P, ib, phist.as<uint32_t>(), ppos.as<uint32_t>(),
-                       tmp.as<uint32_t>());
-    level2(uint32_t(0));
-  } else {
-    hipLaunchKernelGGL(msm_part_scatter_kernel<uint64_t>, gch, dim3(kSortThreads), (size_t)P * 4, st,
-                       dig.as<uint32_t>(), np, L, P, ib, ppos.as<uint32_t>(), tmp.as<uint64_t>());
-    level2(uint64_t(0));
+  const int32_t *zp() const { return points_internal ? (const int32_t *)d_points : zpts.as<int32_t>(); }
+  const int32_t *pyr_out() const {
+    return pyr_final < 0 ? bsum.as<int32_t>() : (pyr_final == 0 ? pyrA.as<int32_t>() : pyrB.as<int32_t>());
   }
-  ZK_HIP(hipGetLastError());
-  // heavy count, most partials and sorted entries, read back while the accumulation runs
-  StageLease lease;
-  ZK_TRY(lease.acquire());
-  MsmStage &stage = *lease.s;
-  uint32_t *info = stage.info;
-  ZK_HIP(hipMemcpyAsync(info, heavy.p, 12, hipMemcpyDeviceToHost, st));
-  ZK_HIP(hipEventRecord(stage.info_ev, st));
-  ev.mark(st);
-  const bool prefetch = user && user->prefetch == 2;
-  if (prefetch)
-    hipLaunchKernelGGL((msm_accum_kernel<C, true>), dim3((unsigned)((nseg + kThreads - 1) / kThreads)), blk, 0, st,
-                       vals.as<uint32_t>(), off.as<uint32_t>(), (uint32_t)nkeys, seg_log, zp, bsum.as<int32_t>(),
-                       cont.as<int32_t>());
-  else
-    hipLaunchKernelGGL((msm_accum_kernel<C, false>), dim3((unsigned)((nseg + kThreads - 1) / kThreads)), blk, 0,
-                       st, vals.as<uint32_t>(), off.as<uint32_t>(), (uint32_t)nkeys, seg_log, zp, bsum.as<int32_t>(),
-                       cont.as<int32_t>());
-  ZK_HIP(hipGetLastError());
-  ev.mark(st);
-  hipLaunchKernelGGL(msm_fix_kernel<C>, dim3((unsigned)((nkeys + kThreads - 1) / kThreads)), blk, 0, st,
-                     off.as<uint32_t>(), (uint32_t)nkeys, seg_log, bsum.as<int32_t>(), cont.as<int32_t>(), limit);
-  ZK_HIP(hipGetLastError());
-  ZK_HIP(hipEventSynchronize(stage.info_ev));  // completed before the accumulation started
-  const uint32_t hh[2] = {info[0], info[1]}, nent = info[2];
-  g_stats[0] = c;
-  g_stats[1] = W;
-  g_stats[2] = nent;
-  g_stats[3] = hh[0];
-  if (hh[0] > heavy_cap) {
-    set_error("msm: too many heavy buckets");
-    return ZK_EINVAL;
+
+  int setup(int n_, const uint64_t *sc, int enl_, int mont_, const uint64_t *pts, bool internal,
+            const MsmSchedule &s, const zkg_msm_schedule *user) {
+    n = n_;
+    d_scalars = sc;
+    enl = enl_;
+    mont = mont_;
+    d_points = pts;
+    points_internal = internal;
+    sch = s;
+    prefetch = user && user->prefetch == 2;
+    limit = (user && user->fix_limit > 0) ? (uint32_t)user->fix_limit : 16u;
+    np = sch.glv ? 2u * (uint32_t)n : (uint32_t)n;  // points of the bucket problem
+    c = sch.c;
+    W = sch.windows;
+    logB = c - 1;
+    B = 1u << logB;
+    nkeys = (uint64_t)W * B;
+    if (nkeys >= (1ull << 32) || (uint64_t)W * np >= (1ull << 32) || np >= (1u << 31)) {
+      set_error("msm: problem too large for 32-bit sort keys");
+      return ZK_EINVAL;
+    }
+    total = (uint64_t)W * np;
+    seg_log = sch.seg_log;
+    nseg = (total >> seg_log) + 1;
+    const int logP = sort_partition_bits((int)np, logB);
+    L = logB - logP;
+    P = 1u << logP;
+    nchunks = (uint32_t)(((uint64_t)np + kPChunk - 1) / kPChunk);
+    nph = (uint64_t)W * P * nchunks;  // level-1 counts, scanned in place of positions
+    ZK_TRY(dig.alloc(total * 4));
+    ZK_TRY(vals.alloc(total * 4));
+    // level-1 words are 32-bit when the point index, the sign and the low bucket bits fit
+    ib = 0;
+    while ((1ull << ib) < (uint64_t)np) ib++;
+    narrow = ib < 31 && ib + 1 + L <= 32;
+    ZK_TRY(tmp.alloc(total * (narrow ? 4 : 8)));
+    ZK_TRY(phist.alloc(nph * 4));
+    ZK_TRY(ppos.alloc((nph + 1) * 4));
+    // big partitions hold more than kBigPart entries each, so there are at most total / kBigPart
+    nbig_cap = std::min<uint64_t>((uint64_t)W * P, total / kBigPart + 1);
+    ZK_TRY(big.alloc(((uint64_t)W * P + 1) * 4));
+    ZK_TRY(rcnt.alloc(nbig_cap * nchunks * ((uint64_t)1 << L) * 4));
+    ZK_TRY(off.alloc((nkeys + 1) * 4));
+    ZK_TRY(bsum.alloc(nkeys * ZW * 4));
+    ZK_TRY(cont.alloc(nseg * ZW * 4));
+    if (!points_internal) ZK_TRY(zpts.alloc((uint64_t)np * AW * 4));
+    // every bucket may be heavy (small seg_log, few entries per segment)
+    heavy_cap = (uint32_t)nkeys;
+    ZK_TRY(heavy.alloc(((uint64_t)heavy_cap + 3) * 4));
+    ZK_TRY(pyrA.alloc(nkeys * ZW * 4));
+    ZK_TRY(pyrB.alloc(nkeys * ZW * 4));
+    const uint64_t per_block = (uint64_t)kThreads * kScanPer;
+    nsb = (uint32_t)((nph + 1 + per_block - 1) / per_block);
+    ZK_TRY(scan_b.alloc((uint64_t)nsb * 4));
+    pyramid(nullptr, false);  // A arrays of W points come out of the pyramid
+    nres = A * (uint32_t)W;
+    ZK_TRY(outx.alloc((uint64_t)nres * PW * 4));
+    return ZK_OK;
   }
-  if (hh[0]) {
-    const uint32_t nh = hh[0], kmax = hh[1];
+
+  // digits, level-1 histogram and scan, level-1 scatter, level-2 sort (bucket offsets, sorted
+  // entries, heavy buckets listed)
+  void launch_sort(hipStream_t st) {
+    const dim3 blk(kThreads);
+    const dim3 gpts((n + kThreads - 1) / kThreads);
+    hipLaunchKernelGGL(msm_digits_kernel<C>, gpts, blk, 0, st, d_scalars, enl, mont, sch.reduce, sch.glv,
+                       (uint32_t)n, np, c, W, sch.wide, ovf, dig.as<uint32_t>(),
+                       points_internal ? (const uint32_t *)nullptr : (const uint32_t *)d_points, zpts.as<int32_t>());
+    const dim3 gch(nchunks, W);
+    hipLaunchKernelGGL(msm_part_hist_kernel, gch, dim3(kSortThreads), (size_t)P * 4, st, dig.as<uint32_t>(), np, L,
+                       P, phist.as<uint32_t>(), heavy.as<uint32_t>(), big.as<uint32_t>());
+    hipLaunchKernelGGL(scan_block_sums_kernel, dim3(nsb), blk, 0, st, phist.as<uint32_t>(), (size_t)nph,
+                       scan_b.as<uint32_t>());
+    hipLaunchKernelGGL(scan_final_kernel, dim3(nsb), blk, 0, st, phist.as<uint32_t>(), (size_t)nph,
+                       scan_b.as<uint32_t>(), ppos.as<uint32_t>(), (uint32_t *)nullptr);
+    const size_t cnt_bytes = ((size_t)1 << std::max(L, 1)) * 4;
+    // level 2: partitions of at most kBigPart words, one workgroup each; larger ones run by run
+    auto level2 = [&](auto word) {
+      using T = decltype(word);
+      const uint32_t stage = (uint32_t)(kPartStageBytes / sizeof(T));
+      hipLaunchKernelGGL(msm_part_sort_kernel<T>, dim3((unsigned)(W * P)), blk,
+                         cnt_bytes + kPartStageBytes + (size_t)stage * 4, st, tmp.as<T>(), ppos.as<uint32_t>(),
+                         nchunks, L, ib, stage, off.as<uint32_t>(), vals.as<uint32_t>(), big.as<uint32_t>());
+      const unsigned gb = (unsigned)std::min<uint64_t>((uint64_t)nbig_cap * nchunks, 1024);
+      hipLaunchKernelGGL(msm_big_count_kernel<T>, dim3(gb), blk, cnt_bytes, st, tmp.as<T>(), ppos.as<uint32_t>(),
+                         nchunks, L, ib, big.as<uint32_t>(), rcnt.as<uint32_t>());
+      hipLaunchKernelGGL(msm_big_scan_kernel, dim3((unsigned)std::min<uint64_t>(nbig_cap, 256)), blk, 0, st,
+                         ppos.as<uint32_t>(), nchunks, L, big.as<uint32_t>(), rcnt.as<uint32_t>(), off.as<uint32_t>());
+      hipLaunchKernelGGL(msm_big_place_kernel<T>, dim3(gb), blk, cnt_bytes, st, tmp.as<T>(), ppos.as<uint32_t>(),
+                         nchunks, L, ib, big.as<uint32_t>(), rcnt.as<uint32_t>(), vals.as<uint32_t>(),
+                         off.as<uint32_t>(), (uint32_t)nkeys, seg_log, heavy.as<uint32_t>(), heavy_cap, limit);
+    };
+    if (narrow) {
+      hipLaunchKernelGGL(msm_part_scatter_staged_kernel, gch, dim3(kSortThreads), kScatterStageLds, st,
+                         dig.as<uint32_t>(), np, L, P, ib, phist.as<uint32_t>(), ppos.as<uint32_t>(),
+                         tmp.as<uint32_t>());
+      level2(uint32_t(0));
+    } else {
+      hipLaunchKernelGGL(msm_part_scatter_kernel<uint64_t>, gch, dim3(kSortThreads), (size_t)P * 4, st,
+                         dig.as<uint32_t>(), np, L, P, ib, ppos.as<uint32_t>(), tmp.as<uint64_t>());
+      level2(uint64_t(0));
+    }
+  }
+
+  void launch_accum(hipStream_t st) {
+    const dim3 blk(kThreads), g((unsigned)((nseg + kThreads - 1) / kThreads));
+    if (prefetch)
+      hipLaunchKernelGGL((msm_accum_kernel<C, true>), g, blk, 0, st, vals.as<uint32_t>(), off.as<uint32_t>(),
+                         (uint32_t)nkeys, seg_log, zp(), bsum.as<int32_t>(), cont.as<int32_t>());
+    else
+      hipLaunchKernelGGL((msm_accum_kernel<C, false>), g, blk, 0, st, vals.as<uint32_t>(), off.as<uint32_t>(),
+                         (uint32_t)nkeys, seg_log, zp(), bsum.as<int32_t>(), cont.as<int32_t>());
+  }
+
+  void launch_fix(hipStream_t st) {
+    hipLaunchKernelGGL(msm_fix_kernel<C>, dim3((unsigned)((nkeys + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+                       st, off.as<uint32_t>(), (uint32_t)nkeys, seg_log, bsum.as<int32_t>(), cont.as<int32_t>(),
+                       limit);
+  }
+
+  // heavy buckets, host-driven: nh buckets with at most kmax continuation partials each,
+  // pairwise rounds then the final addition into the bucket sums
+  int launch_heavy_rounds(hipStream_t st, uint32_t nh, uint32_t kmax, uint32_t nent) {
+    const dim3 blk(kThreads);
     // segments holding sorted entries; every continuation partial lies below this bound
     const uint32_t nseg_used = (uint32_t)(((uint64_t)nent + (1ull << seg_log) - 1) >> seg_log);
     for (int r = 0; (1ull << r) < kmax; r++) {
@@ -1612,112 +1677,200 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
     }
     hipLaunchKernelGGL(msm_heavy_final_kernel<C>, dim3((nh + kThreads - 1) / kThreads), blk, 0, st,
                        off.as<uint32_t>(), heavy.as<uint32_t>(), seg_log, bsum.as<int32_t>(), cont.as<int32_t>());
-    ZK_HIP(hipGetLastError());
+    return ZK_OK;
   }
-  ev.mark(st);
-  // pyramid
-  const int32_t *in = bsum.as<int32_t>();
-  int32_t *bufs[2] = {pyrA.as<int32_t>(), pyrB.as<int32_t>()};
-  uint32_t A = 1, M = B;
-  int k = 0;
-  while (M > 1) {
-    // the remaining levels in one launch once they fit one workgroup per window (level 0 at
-    // most two items per thread, later levels one; level 0's output in LDS)
-    const uint32_t items0 = (A + 1) * (M / 2), items1 = (A + 2) * (M / 4);
-    const size_t tail_lds = (size_t)items0 * ZW * 4;
-    if (items0 <= 2u * kTailThreads && items1 <= (uint32_t)kTailThreads && tail_lds <= kTailLdsBytes) {
-      int levels = 0;
-      while ((M >> levels) > 1) levels++;
-      // one item per thread at level 0 when that fits one workgroup (each round is one more
-      // addition of latency), else two
-      const unsigned tb = std::min((items0 + 63) & ~63u, (unsigned)kTailThreads);
-      hipLaunchKernelGGL(msm_pyramid_tail_kernel<C>, dim3(1u, (unsigned)W), dim3(tb), tail_lds, st, in, bufs[k],
-                         (uint32_t)W, M, A, M);
+
+  // heavy buckets, device-driven: one workgroup per listed bucket (grid-stride), whatever
+  // their number and size; exits at once when there is none
+  void launch_heavy_tree(hipStream_t st) {
+    const unsigned g = (unsigned)std::min<uint64_t>(heavy_cap, kHeavyTreeBlocks);
+    hipLaunchKernelGGL(msm_heavy_tree_kernel<C>, dim3(g), dim3(kThreads), (size_t)kThreads * ZW * 4, st,
+                       off.as<uint32_t>(), heavy.as<uint32_t>(), heavy_cap, seg_log, bsum.as<int32_t>(),
+                       cont.as<int32_t>());
+  }
+
+  // Bucket pyramid over the W windows of B bucket sums; launch = false only works out the
+  // number A of output arrays and the buffer that holds them.
+  void pyramid(hipStream_t st, bool launch) {
+    const dim3 blk(kThreads);
+    const int32_t *in = bsum.as<int32_t>();
+    int32_t *bufs[2] = {pyrA.as<int32_t>(), pyrB.as<int32_t>()};
+    int fin = -1;
+    uint32_t a = 1, M = B;
+    int k = 0;
+    while (M > 1) {
+      // the remaining levels in one launch once they fit one workgroup per window (level 0 at
+      // most two items per thread, later levels one; level 0's output in LDS)
+      const uint32_t items0 = (a + 1) * (M / 2), items1 = (a + 2) * (M / 4);
+      const size_t tail_lds = (size_t)items0 * ZW * 4;
+      if (items0 <= 2u * kTailThreads && items1 <= (uint32_t)kTailThreads && tail_lds <= kTailLdsBytes) {
+        int levels = 0;
+        while ((M >> levels) > 1) levels++;
+        // one item per thread at level 0 when that fits one workgroup (each round is one more
+        // addition of latency), else two
+        const unsigned tb = std::min((items0 + 63) & ~63u, (unsigned)kTailThreads);
+        if (launch)
+          hipLaunchKernelGGL(msm_pyramid_tail_kernel<C>, dim3(1u, (unsigned)W), dim3(tb), tail_lds, st, in, bufs[k],
+                             (uint32_t)W, M, a, M);
+        in = bufs[k];
+        fin = k;
+        a += levels;
+        M = 1;
+        break;
+      }
+      // the largest block of Mb points per array whose level 0 is one item per thread
+      uint32_t Mb = M / 2;
+      while (Mb >= 4 && ((a + 1) * (Mb / 2) > (uint32_t)kTailThreads ||
+                         (size_t)(a + 1) * (Mb / 2) * ZW * 4 > kTailLdsBytes))
+        Mb >>= 1;
+      // two levels per launch while that still fills the GPU (one thread per 4 points, 3-4
+      // dependent additions each); below, the levels of each Mb-point block in one launch (one
+      // addition per thread per level, no launch between levels), or a level per launch
+      if (M >= 4 && (uint64_t)a * W * (M / 4) >= kPyr2MinThreads) {
+        const uint64_t thr = (uint64_t)(a + 1) * W * (M / 4);
+        if (launch)
+          hipLaunchKernelGGL(msm_pyramid2_kernel<C>, dim3((unsigned)((thr + kThreads - 1) / kThreads)), blk, 0, st,
+                             in, bufs[k], (uint32_t)W, M, a);
+        a += 2;
+        M >>= 2;
+      } else if (Mb >= 4) {
+        const uint32_t items = (a + 1) * (Mb / 2);
+        int levels = 0;
+        while ((Mb >> levels) > 1) levels++;
+        if (launch)
+          hipLaunchKernelGGL(msm_pyramid_tail_kernel<C>, dim3(M / Mb, (unsigned)W), dim3((items + 63) & ~63u),
+                             (size_t)items * ZW * 4, st, in, bufs[k], (uint32_t)W, M, a, Mb);
+        a += levels;
+        M /= Mb;
+      } else {
+        const uint64_t thr = (uint64_t)a * W * (M / 2);
+        if (launch)
+          hipLaunchKernelGGL(msm_pyramid_kernel<C>, dim3((unsigned)((thr + kThreads - 1) / kThreads)), blk, 0, st,
+                             in, bufs[k], (uint32_t)W, M, a);
+        a += 1;
+        M >>= 1;
+      }
       in = bufs[k];
-      A += levels;
-      M = 1;
-      break;
+      fin = k;
+      k ^= 1;
     }
-    // the largest block of Mb points per array whose level 0 is one item per thread
-    uint32_t Mb = M / 2;
-    while (Mb >= 4 && ((A + 1) * (Mb / 2) > (uint32_t)kTailThreads ||
-                       (size_t)(A + 1) * (Mb / 2) * ZW * 4 > kTailLdsBytes))
-      Mb >>= 1;
-    // two levels per launch while that still fills the GPU (one thread per 4 points, 3-4
-    // dependent additions each); below, the levels of each Mb-point block in one launch (one
-    // addition per thread per level, no launch between levels), or a level per launch
-    if (M >= 4 && (uint64_t)A * W * (M / 4) >= kPyr2MinThreads) {
-      const uint64_t thr = (uint64_t)(A + 1) * W * (M / 4);
-      hipLaunchKernelGGL(msm_pyramid2_kernel<C>, dim3((unsigned)((thr + kThreads - 1) / kThreads)), blk, 0, st, in,
-                         bufs[k], (uint32_t)W, M, A);
-      A += 2;
-      M >>= 2;
-    } else if (Mb >= 4) {
-      const uint32_t items = (A + 1) * (Mb / 2);
-      int levels = 0;
-      while ((Mb >> levels) > 1) levels++;
-      hipLaunchKernelGGL(msm_pyramid_tail_kernel<C>, dim3(M / Mb, (unsigned)W), dim3((items + 63) & ~63u),
-                         (size_t)items * ZW * 4, st, in, bufs[k], (uint32_t)W, M, A, Mb);
-      A += levels;
-      M /= Mb;
-    } else {
-      const uint64_t thr = (uint64_t)A * W * (M / 2);
-      hipLaunchKernelGGL(msm_pyramid_kernel<C>, dim3((unsigned)((thr + kThreads - 1) / kThreads)), blk, 0, st, in,
-                         bufs[k], (uint32_t)W, M, A);
-      A += 1;
-      M >>= 1;
-    }
-    in = bufs[k];
-    k ^= 1;
+    A = a;
+    pyr_final = fin;
   }
+
+  // the A arrays of W points ([0] = Sum_w, [j] = U_{w, j-1}) as external canonical words
+  void launch_out(hipStream_t st) {
+    hipLaunchKernelGGL(msm_points_out_kernel<C>, dim3((nres + kThreads - 1) / kThreads), dim3(kThreads), 0, st,
+                       pyr_out(), nres, outx.as<uint32_t>());
+  }
+
+  // host Horner: sum over (w, j) of 2^(start_w + max(j-1, 0)) * term
+  void combine(const uint32_t *res, MsmResult *out) const {
+    const zkh::FieldDesc &f = fp_desc(C::id);
+    auto get = [&](uint32_t a, uint32_t w) {
+      const uint32_t *p = res + ((size_t)a * W + w) * PW;
+      zkh::Xyzz q;
+      zkh::Fe *dst[4] = {&q.x, &q.y, &q.zz, &q.zzz};
+      for (int e = 0; e < 4; e++) {
+        *dst[e] = zkh::zero();
+        for (int j = 0; j < F::N; j++) dst[e]->w[j / 2] |= (uint64_t)p[e * F::N + j] << (32 * (j & 1));
+      }
+      return q;
+    };
+    const int emax = sch.start(W - 1) + (int)(A > 1 ? A - 2 : 0);
+    std::vector<std::vector<uint32_t>> at(emax + 1);  // encoded (a << 16) | w
+    for (uint32_t w = 0; w < (uint32_t)W; w++) {
+      const int s0 = sch.start((int)w);
+      at[s0].push_back((0u << 16) | w);
+      for (uint32_t a = 1; a < A; a++) at[s0 + (a - 1)].push_back((a << 16) | w);
+    }
+    zkh::Xyzz acc = zkh::xyzz_inf();
+    for (int e = emax; e >= 0; e--) {
+      acc = zkh::xyzz_dbl(f, acc);
+      for (uint32_t code : at[e]) acc = zkh::xyzz_add(f, acc, get(code >> 16, code & 0xffff));
+    }
+    memset(out, 0, sizeof *out);
+    out->curve = C::id;
+    zkh::store(f, out->x, acc.x);
+    zkh::store(f, out->y, acc.y);
+    zkh::store(f, out->zz, acc.zz);
+    zkh::store(f, out->zzz, acc.zzz);
+  }
+};
+
+static int msm_lds_attributes_once() {
+  static std::once_flag attr_once;
+  static int attr_status = ZK_OK;
+  std::call_once(attr_once, [&] { attr_status = msm_set_lds_attributes(); });
+  return attr_status;
+}
+
+template <class C>
+static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const uint64_t *d_points,
+                   bool points_internal, const zkg_msm_schedule *user, hipStream_t st, MsmResult *out) {
+  zkg_scalar_stats stats;
+  const bool use_stats = user && user->policy != ZKG_SCHED_PLAIN;
+  if (use_stats) ZK_TRY(msm_scalar_stats(C::id, n, d_scalars, enl, mont, st, &stats));
+  // Measured sparse scalar sets (at least half the scalars 0 or 1): the other scalars and
+  // their points are compacted into a dense MSM of their own, the points of the scalars
+  // equal to 1 are summed directly, and the zeros drop out; the two sums are added on the host.
+  // Zeros and ones cost the plain pipeline little beyond their digits, so the compaction pays
+  // only while its own passes are small next to the MSM: measured faster up to 2^20 points,
+  // slower from 2^21 (BN254 2^22, 80% in {0, 1}: 2.82 ms compacted against 2.52 ms).
+  if (use_stats && n >= (1 << 14) && n < (1 << 21) && (uint64_t)(stats.n_zero + stats.n_one) * 2 >= (uint64_t)n)
+    return msm_sparse_run<C>(n, d_scalars, enl, mont, d_points, points_internal, user, st, stats, out);
+  MsmSchedule sch;
+  ZK_TRY(msm_resolve(C::id, n, enl, user, use_stats ? &stats : nullptr, &sch, !points_internal));
+  // every buffer the queued kernels touch lives until the stream is drained (the guard below)
+  MsmPipe<C> p;
+  StreamSyncGuard drain(st);
+  ZK_TRY(p.setup(n, d_scalars, enl, mont, d_points, points_internal, sch, user));
+  ZK_TRY(msm_lds_attributes_once());
+
+  Events ev;
+  auto t_host0 = std::chrono::steady_clock::now();
+  ev.mark(st);
+  p.launch_sort(st);
+  ZK_HIP(hipGetLastError());
+  // heavy count, most partials and sorted entries, read back while the accumulation runs
+  StageLease lease;
+  ZK_TRY(lease.acquire());
+  MsmStage &stage = *lease.s;
+  uint32_t *info = stage.info;
+  ZK_HIP(hipMemcpyAsync(info, p.heavy.p, 12, hipMemcpyDeviceToHost, st));
+  ZK_HIP(hipEventRecord(stage.info_ev, st));
+  ev.mark(st);
+  p.launch_accum(st);
   ZK_HIP(hipGetLastError());
   ev.mark(st);
-  // A arrays of W points: [0] = Sum_w, [j] = U_{w, j-1}; converted to external canonical words
-  const uint32_t nres = A * (uint32_t)W;
-  ZK_TRY(outx.alloc((uint64_t)nres * PW * 4));
-  hipLaunchKernelGGL(msm_points_out_kernel<C>, dim3((nres + kThreads - 1) / kThreads), blk, 0, st, in, nres,
-                     outx.as<uint32_t>());
+  p.launch_fix(st);
   ZK_HIP(hipGetLastError());
-  ZK_TRY(stage.reserve_res((size_t)nres * PW));
-  const uint32_t *res = stage.res;
-  ZK_HIP(hipMemcpyAsync(stage.res, outx.p, (size_t)nres * PW * 4, hipMemcpyDeviceToHost, st));
+  ZK_HIP(hipEventSynchronize(stage.info_ev));  // completed before the accumulation started
+  const uint32_t hh[2] = {info[0], info[1]}, nent = info[2];
+  g_stats[0] = p.c;
+  g_stats[1] = p.W;
+  g_stats[2] = nent;
+  g_stats[3] = hh[0];
+  if (hh[0] > p.heavy_cap) {
+    set_error("msm: too many heavy buckets");
+    return ZK_EINVAL;
+  }
+  if (hh[0]) ZK_TRY(p.launch_heavy_rounds(st, hh[0], hh[1], nent));
+  ZK_HIP(hipGetLastError());
+  ev.mark(st);
+  p.pyramid(st, true);
+  ZK_HIP(hipGetLastError());
+  ev.mark(st);
+  p.launch_out(st);
+  ZK_HIP(hipGetLastError());
+  ZK_TRY(stage.reserve_res((size_t)p.nres * MsmPipe<C>::PW));
+  ZK_HIP(hipMemcpyAsync(stage.res, p.outx.p, (size_t)p.nres * MsmPipe<C>::PW * 4, hipMemcpyDeviceToHost, st));
   ZK_HIP(hipStreamSynchronize(st));
   drain.release();
   ev.mark(st);
   auto t_host1 = std::chrono::steady_clock::now();
-
-  // host Horner: sum over (w, j) of 2^(start_w + max(j-1, 0)) * term
-  const zkh::FieldDesc &f = fp_desc(C::id);
-  auto get = [&](uint32_t a, uint32_t w) {
-    const uint32_t *p = res + ((size_t)a * W + w) * PW;
-    zkh::Xyzz q;
-    zkh::Fe *dst[4] = {&q.x, &q.y, &q.zz, &q.zzz};
-    for (int e = 0; e < 4; e++) {
-      *dst[e] = zkh::zero();
-      for (int j = 0; j < F::N; j++) dst[e]->w[j / 2] |= (uint64_t)p[e * F::N + j] << (32 * (j & 1));
-    }
-    return q;
-  };
-  const int emax = sch.start(W - 1) + (int)(A > 1 ? A - 2 : 0);
-  std::vector<std::vector<uint32_t>> at(emax + 1);  // encoded (a << 16) | w
-  for (uint32_t w = 0; w < (uint32_t)W; w++) {
-    const int s0 = sch.start((int)w);
-    at[s0].push_back((0u << 16) | w);
-    for (uint32_t a = 1; a < A; a++) at[s0 + (a - 1)].push_back((a << 16) | w);
-  }
-  zkh::Xyzz acc = zkh::xyzz_inf();
-  for (int e = emax; e >= 0; e--) {
-    acc = zkh::xyzz_dbl(f, acc);
-    for (uint32_t code : at[e]) acc = zkh::xyzz_add(f, acc, get(code >> 16, code & 0xffff));
-  }
+  p.combine(stage.res, out);
   auto t_host2 = std::chrono::steady_clock::now();
-
-  memset(out, 0, sizeof *out);
-  out->curve = C::id;
-  zkh::store(f, out->x, acc.x);
-  zkh::store(f, out->y, acc.y);
-  zkh::store(f, out->zz, acc.zz);
-  zkh::store(f, out->zzz, acc.zzz);
 
   // timing: sort, accumulate, fix, pyramid, d2h, host combine
   g_nphase = 0;
@@ -1911,5 +2064,187 @@ int g1_scalar_mul_gen(int curve, int n, const uint64_t *d_scalars, uint64_t *d_o
   ZK_HIP(hipStreamSynchronize(st));
   return ZK_OK;
 }
+
+// ---------------------------------------------------------------------------------------
+// Captured MSM batch.  create() fits every job's schedule (its policy, on the scalars present
+// then), allocates every job's buffers and queues every job's pipeline once into a hipGraph:
+// the jobs are dealt, largest first, to `nstreams` streams forked from and joined back into one
+// capture stream, so the graph's branches run concurrently.  run() launches the graph once,
+// waits for it, and combines the jobs' window sums on the host.  Nothing inside a replay waits
+// on the host: heavy buckets go through msm_heavy_tree_kernel, and every launch size is fixed
+// at capture.  A replay with scalars the fitted windows do not cover (a wider bit length than
+// at create) is caught by the digits pass and reported as an error.
+
+__global__ void msm_flag_clear_kernel(uint32_t *flag) { *flag = 0; }
+
+struct MsmGraphJob {
+  virtual ~MsmGraphJob() {
+    if (res) (void)hipHostFree(res);
+  }
+  virtual int setup(int n, const uint64_t *sc, int enl, int mont, const uint64_t *pts, bool internal,
+                    const zkg_msm_schedule *user, uint32_t *ovf, hipStream_t st) = 0;
+  virtual void enqueue(hipStream_t st) = 0;
+  virtual void combine(MsmResult *out) const = 0;
+  int curve = 0;
+  double work = 0;           // relative cost, for dealing jobs to streams
+  uint32_t *res = nullptr;   // pinned result words
+};
+
+template <class C>
+struct MsmGraphJobT : MsmGraphJob {
+  MsmPipe<C> p;
+  int setup(int n, const uint64_t *sc, int enl, int mont, const uint64_t *pts, bool internal,
+            const zkg_msm_schedule *user, uint32_t *ovf, hipStream_t st) override {
+    curve = C::id;
+    zkg_scalar_stats stats;
+    const bool use_stats = user && user->policy != ZKG_SCHED_PLAIN;
+    if (use_stats) ZK_TRY(msm_scalar_stats(C::id, n, sc, enl, mont, st, &stats));
+    MsmSchedule sch;
+    ZK_TRY(msm_resolve(C::id, n, enl, user, use_stats ? &stats : nullptr, &sch, !internal));
+    ZK_TRY(p.setup(n, sc, enl, mont, pts, internal, sch, user));
+    p.ovf = use_stats ? ovf : nullptr;  // declared-width windows cover every scalar
+    ZK_HIP(hipHostMalloc((void **)&res, (size_t)p.nres * MsmPipe<C>::PW * 4, hipHostMallocDefault));
+    work = (double)p.total * (C::id == 1 ? 2.0 : 1.0);
+    return ZK_OK;
+  }
+  void enqueue(hipStream_t st) override {
+    p.launch_sort(st);
+    p.launch_accum(st);
+    p.launch_fix(st);
+    p.launch_heavy_tree(st);
+    p.pyramid(st, true);
+    p.launch_out(st);
+    (void)hipMemcpyAsync(res, p.outx.p, (size_t)p.nres * MsmPipe<C>::PW * 4, hipMemcpyDeviceToHost, st);
+  }
+  void combine(MsmResult *out) const override { p.combine(res, out); }
+};
+
+// a job of no points: the point at infinity, no device work
+struct MsmGraphJobEmpty : MsmGraphJob {
+  int setup(int, const uint64_t *, int, int, const uint64_t *, bool, const zkg_msm_schedule *, uint32_t *,
+            hipStream_t) override {
+    return ZK_OK;
+  }
+  void enqueue(hipStream_t) override {}
+  void combine(MsmResult *out) const override { msm_infinity(curve, out); }
+};
+
+struct MsmGraph {
+  std::vector<MsmGraphJob *> jobs;
+  hipStream_t origin = nullptr;
+  std::vector<hipStream_t> lanes;
+  std::vector<hipEvent_t> events;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  DevBuf flag;
+  uint32_t *flag_host = nullptr;
+  ~MsmGraph() {
+    if (origin) (void)hipStreamSynchronize(origin);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (hipEvent_t e : events) (void)hipEventDestroy(e);
+    for (hipStream_t s : lanes) (void)hipStreamDestroy(s);
+    if (origin) (void)hipStreamDestroy(origin);
+    if (flag_host) (void)hipHostFree(flag_host);
+    for (MsmGraphJob *j : jobs) delete j;
+  }
+};
+
+int msm_graph_create(int njobs, const int *curves, const int *ns, const uint64_t *const *d_scalars, int enl, int mont,
+                     const uint64_t *const *d_points, bool points_internal, const zkg_msm_schedule *sched,
+                     int nstreams, MsmGraph **out) {
+  *out = nullptr;
+  if (njobs <= 0) {
+    set_error("msm graph: no jobs");
+    return ZK_EINVAL;
+  }
+  std::unique_ptr<MsmGraph> g(new MsmGraph);
+  ZK_HIP(hipStreamCreateWithFlags(&g->origin, hipStreamNonBlocking));
+  ZK_TRY(g->flag.alloc(4));
+  ZK_HIP(hipHostMalloc((void **)&g->flag_host, 4, hipHostMallocDefault));
+  ZK_TRY(msm_lds_attributes_once());
+  zkg_msm_schedule s;
+  memset(&s, 0, sizeof s);
+  s.policy = sched ? sched->policy : ZKG_SCHED_PLAIN;
+  s.fix_limit = sched ? sched->fix_limit : 0;
+  for (int k = 0; k < njobs; k++) {
+    if (ns[k] < 0 || (curves[k] != 0 && curves[k] != 1)) {
+      set_error("msm graph: every job needs a curve id 0/1 and a point count >= 0");
+      return ZK_EINVAL;
+    }
+    MsmGraphJob *j = ns[k] == 0       ? (MsmGraphJob *)new MsmGraphJobEmpty
+                     : curves[k] == 0 ? (MsmGraphJob *)new MsmGraphJobT<Bn254G1>
+                                      : (MsmGraphJob *)new MsmGraphJobT<Bls381G1>;
+    j->curve = curves[k];
+    g->jobs.push_back(j);
+    ZK_TRY(j->setup(ns[k], d_scalars[k], enl, mont, d_points[k], points_internal, &s, g->flag.as<uint32_t>(),
+                    g->origin));
+  }
+  nstreams = std::max(1, std::min(nstreams, 16));
+  g->lanes.resize(nstreams, nullptr);
+  for (auto &l : g->lanes) ZK_HIP(hipStreamCreateWithFlags(&l, hipStreamNonBlocking));
+  g->events.resize(nstreams + 1, nullptr);
+  for (auto &e : g->events) ZK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  // largest first, each to the stream with the least work so far
+  std::vector<int> order(njobs);
+  for (int k = 0; k < njobs; k++) order[k] = k;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return g->jobs[a]->work > g->jobs[b]->work; });
+  std::vector<double> load(nstreams, 0.0);
+  ZK_HIP(hipStreamSynchronize(g->origin));  // the statistics passes above
+  ZK_HIP(hipStreamBeginCapture(g->origin, hipStreamCaptureModeThreadLocal));
+  hipLaunchKernelGGL(msm_flag_clear_kernel, dim3(1), dim3(1), 0, g->origin, g->flag.as<uint32_t>());
+  (void)hipEventRecord(g->events[nstreams], g->origin);
+  for (hipStream_t l : g->lanes) (void)hipStreamWaitEvent(l, g->events[nstreams], 0);
+  for (int k : order) {
+    const int li = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    g->jobs[k]->enqueue(g->lanes[li]);
+    load[li] += g->jobs[k]->work;
+  }
+  for (int li = 0; li < nstreams; li++) {
+    (void)hipEventRecord(g->events[li], g->lanes[li]);
+    (void)hipStreamWaitEvent(g->origin, g->events[li], 0);
+  }
+  (void)hipMemcpyAsync(g->flag_host, g->flag.p, 4, hipMemcpyDeviceToHost, g->origin);
+  const hipError_t le = hipGetLastError();
+  const hipError_t ce = hipStreamEndCapture(g->origin, &g->graph);
+  if (le != hipSuccess || ce != hipSuccess) {
+    set_error(std::string("msm graph: capture failed: ") + hipGetErrorString(le != hipSuccess ? le : ce));
+    return ZK_EHIP;
+  }
+  ZK_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+  *out = g.release();
+  return ZK_OK;
+}
+
+int msm_graph_run(MsmGraph *g, uint64_t *const *outs, int out_flags, double *wall_ms) {
+  const auto t0 = std::chrono::steady_clock::now();
+  ZK_HIP(hipGraphLaunch(g->exec, g->origin));
+  ZK_HIP(hipStreamSynchronize(g->origin));
+  if (*g->flag_host) {
+    set_error("msm graph: a scalar is wider than the windows fitted at create; create the plan again");
+    return ZK_EINVAL;
+  }
+  // host Horner of every job, spread over a few threads
+  const int nj = (int)g->jobs.size();
+  std::vector<int> status(nj, ZK_OK);
+  std::atomic<int> next{0};
+  auto worker = [&] {
+    for (int k; (k = next.fetch_add(1)) < nj;) {
+      MsmResult r;
+      g->jobs[k]->combine(&r);
+      status[k] = msm_write_result(g->jobs[k]->curve, r, outs[k], out_flags);
+    }
+  };
+  std::vector<std::thread> th;
+  const int nt = std::min(nj, 8);
+  for (int t = 1; t < nt; t++) th.emplace_back(worker);
+  worker();
+  for (auto &t : th) t.join();
+  for (int k = 0; k < nj; k++) ZK_TRY(status[k]);
+  if (wall_ms) *wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return ZK_OK;
+}
+
+void msm_graph_destroy(MsmGraph *g) { delete g; }
 
 }  // namespace zk

@@ -100,6 +100,15 @@ int msm_batch(int njobs, const int *curves, const int *ns, const uint64_t *const
               const uint64_t *const *d_points, bool points_internal, uint64_t *const *outs, int out_flags,
               const zkg_msm_schedule *sched, int nstreams, double *wall_ms);
 
+// captured batch of MSMs (one hipGraph over nstreams concurrent branches); run() replays it
+// and writes every job's result to outs[k] (host words, msm_write_result flags)
+struct MsmGraph;
+int msm_graph_create(int njobs, const int *curves, const int *ns, const uint64_t *const *d_scalars, int enl, int mont,
+                     const uint64_t *const *d_points, bool points_internal, const zkg_msm_schedule *sched,
+                     int nstreams, MsmGraph **out);
+int msm_graph_run(MsmGraph *g, uint64_t *const *outs, int out_flags, double *wall_ms);
+void msm_graph_destroy(MsmGraph *g);
+
 int msm_last_timing(double *ms_total, double *ms_phases, int max_phases, int *nphases);
 int msm_set_phase_timing(int on);
 int msm_last_stats(long long *out4);

@@ -114,6 +114,11 @@ def lib():
         L.zkg_msm_batch.argtypes = [ctypes.c_int, ip, ip, ctypes.POINTER(vp), ctypes.c_int,
                                     ctypes.POINTER(vp), ctypes.POINTER(u64p), ctypes.c_int,
                                     ctypes.POINTER(MsmSchedule), ctypes.c_int, dp]
+        L.zkg_msm_batch_create.argtypes = [ctypes.c_int, ip, ip, ctypes.POINTER(vp), ctypes.c_int,
+                                           ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(MsmSchedule), ctypes.c_int,
+                                           ctypes.POINTER(vp)]
+        L.zkg_msm_batch_run.argtypes = [vp, ctypes.POINTER(u64p), dp]
+        L.zkg_msm_batch_destroy.argtypes = [vp]
         L.zkg_mlp_create.argtypes = [ctypes.c_int, ip, ctypes.c_uint, ctypes.POINTER(vp)]
         L.zkg_mlp_destroy.argtypes = [vp]
         L.zkg_mlp_nparams.argtypes = [vp]
@@ -184,6 +189,13 @@ class DeviceArray:
         padded = np.zeros((b.size + 7) // 8 * 8, np.uint8)
         padded[:b.size] = b
         return cls.from_host(padded.view(np.uint64))
+
+    def copy_from_host(self, a):
+        """Overwrite the buffer's first a.size words (a plan bound to this buffer rereads them)."""
+        a = _c(a)
+        if a.size > self.nwords:
+            raise ValueError("copy_from_host: more words than the buffer holds")
+        _check(lib().zkg_memcpy_h2d(self.ptr, a.ctypes.data_as(vp), 8 * a.size))
 
     def to_host(self, shape=None):
         out = np.empty(self.nwords, dtype=np.uint64)
@@ -587,6 +599,47 @@ def msm_batch(curves, ns, d_scalars, d_points, policy=SCHED_PLAIN, nstreams=4, e
     _check(lib().zkg_msm_batch(k, C, N, S, expo_nlimbs, P, O, flags, ctypes.byref(sched), nstreams,
                                ctypes.byref(wall)))
     return outs, wall.value
+
+
+class MsmBatch:
+    """The msm_batch job list as one captured plan (zkg_msm_batch_*): create fits the schedules
+    and captures the pipelines into a hipGraph; run() replays it and returns (results, wall ms)."""
+
+    def __init__(self, curves, ns, d_scalars, d_points, policy=SCHED_PLAIN, nstreams=4, expo_nlimbs=4, flags=0):
+        if any(isinstance(p, PreparedPoints) for p in d_points):
+            if not all(isinstance(p, PreparedPoints) for p in d_points):
+                raise ValueError("MsmBatch: mix of prepared and plain point sets")
+            flags |= MSM_POINTS_INTERNAL
+        k = len(ns)
+        if isinstance(curves, int):
+            curves = [curves] * k
+        wide = flags & (MSM_PROJ_OUT | MSM_JAC_OUT)
+        self.outs = [np.zeros((3 if wide else 2) * FP_LIMBS[c], np.uint64) for c in curves]
+        self._O = (u64p * k)(*[_p(o) for o in self.outs])
+        C = (ctypes.c_int * k)(*curves)
+        N = (ctypes.c_int * k)(*ns)
+        S = (vp * k)(*[d.ptr for d in d_scalars])
+        P = (vp * k)(*[d.ptr for d in d_points])
+        sched = MsmSchedule(0, 0, policy)
+        self.plan = vp()
+        _check(lib().zkg_msm_batch_create(k, C, N, S, expo_nlimbs, P, flags, ctypes.byref(sched), nstreams,
+                                          ctypes.byref(self.plan)))
+
+    def run(self):
+        wall = ctypes.c_double()
+        _check(lib().zkg_msm_batch_run(self.plan, self._O, ctypes.byref(wall)))
+        return [o.copy() for o in self.outs], wall.value
+
+    def close(self):
+        if self.plan:
+            lib().zkg_msm_batch_destroy(self.plan)
+            self.plan = vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _fp(a):
