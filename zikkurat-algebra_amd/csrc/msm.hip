@@ -16,8 +16,9 @@
 //              [t 2^s, (t+1) 2^s), finds their buckets from the offsets (binary search, then
 //              a walk) and folds each run with mixed XYZZ + affine adds;
 //              a run that began in an earlier segment is written as a continuation partial
-//   6 fix      each bucket adds its continuation partials (heavy buckets: pairwise tree);
-//              an empty bucket gets the infinity the accumulation did not write
+//   6 fix      each bucket adds its continuation partials serially, an empty bucket gets the
+//              infinity the accumulation did not write; heavy buckets (many partials) are
+//              folded by workgroups in LDS, several workgroups per bucket
 //   7 pyramid  bucket reduction sum_b (b+1) B_b without serial running sums: level k pairs
 //              neighbours (P'_i = P_2i + P_2i+1) and keeps the odd children Q^(k) = P_2i+1,
 //              which are pairwise reduced alongside; then sum_b (b+1) B_b =
@@ -647,10 +648,10 @@ __global__ __launch_bounds__(kThreads) void msm_big_scan_kernel(const uint32_t *
   }
 }
 
-// Buckets whose runs span more than `limit` segments get pairwise trees (heavy[0] = count,
-// heavy[1] = most partials, heavy[2] = sorted entries, heavy[3..] = buckets).  Known from the
-// offsets alone, so the last sort kernel finds them and the three counts reach the host (one
-// copy) while the accumulation runs.
+// Buckets whose runs span more than `limit` segments are heavy (heavy[0] = count, heavy[1] =
+// most partials, heavy[2] = sorted entries, heavy[3..] = buckets).  Known from the offsets
+// alone, so the last sort kernel lists them before the accumulation; the heavy-bucket kernels
+// read the list on the device.
 __device__ __forceinline__ void heavy_detect(const uint32_t *__restrict__ off, uint32_t nkeys, uint32_t b,
                                              int seg_log, uint32_t *__restrict__ heavy, uint32_t heavy_cap,
                                              uint32_t limit) {
@@ -859,122 +860,99 @@ __global__ __launch_bounds__(kThreads) void msm_fix_kernel(const uint32_t *__res
   zxyzz_store(bsum + (size_t)b * PW, acc);
 }
 
-// one pairwise round over the continuation partials of heavy buckets
-template <class C>
-__global__ __launch_bounds__(kThreads) void msm_heavy_round_kernel(const uint32_t *__restrict__ off,
-                                                                  const uint32_t *__restrict__ heavy, int seg_log,
-                                                                  int32_t *__restrict__ cont, int r,
-                                                                  uint32_t pairs_per_bucket) {
-  constexpr int PW = ZLayout<C>::XYZZ;
-  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t h = (uint32_t)(t / pairs_per_bucket), j = (uint32_t)(t % pairs_per_bucket);
-  if (h >= heavy[0]) return;
-  const uint32_t b = heavy[3 + h];
-  const uint64_t fs = off[b] >> seg_log, ls = (uint64_t)(off[b + 1] - 1) >> seg_log;
-  const uint64_t K = ls - fs, base = fs + 1;
-  const uint64_t i0 = (uint64_t)j << (r + 1), i1 = i0 + (1ull << r);
-  if (i1 >= K) return;
-  XyzzZ<C> a, q;
-  zxyzz_load(a, cont + (base + i0) * PW);
-  zxyzz_load(q, cont + (base + i1) * PW);
-  zxyzz_add(a, a, q);
-  zxyzz_store(cont + (base + i0) * PW, a);
-}
+// Heavy buckets (more than fix_limit continuation partials), reduced on the device in two
+// launches of G workgroups whatever their number and size, with no host round trip.  With nh
+// listed buckets and nh <= G, each bucket gets Q = G / nh workgroups: workgroup (h, q) sums the
+// bucket's partials q 256 + t, (q + Q) 256 + t, ... (t its thread), folds the 256 sums in LDS
+// and writes one point to hsum[q nh + h]; the combine kernel then folds each bucket's Q points
+// the same way and adds the total into the bucket sum.  With nh > G, one workgroup per bucket
+// (grid-stride) adds straight into the bucket sum and the combine kernel does nothing.  Both
+// exit at once when no bucket is heavy.  A bucket of K partials costs a chain of about
+// K / (Q 256) + 8 additions, then Q / 256 + 8.
+static constexpr uint32_t kHeavyBlocks = 1024;
 
-// segment -> bucket holding its first entry (the owner of a continuation partial)
-__global__ __launch_bounds__(kThreads) void msm_seg_owner_kernel(const uint32_t *__restrict__ off, uint32_t nkeys,
-                                                                int seg_log, uint32_t nseg,
-                                                                uint32_t *__restrict__ own) {
-  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nseg) return;
-  own[j] = bucket_of(off, nkeys, (uint32_t)((uint64_t)j << seg_log));
-}
-
-// The same pairwise round with one thread per segment: the work per round is bounded by the
-// segment count whatever the spread of partial counts over heavy buckets.
+// the sum of the workgroup's acc values, in thread 0 (part: kThreads XYZZ points of LDS)
 template <class C>
-__global__ __launch_bounds__(kThreads) void msm_heavy_round_seg_kernel(const uint32_t *__restrict__ off,
-                                                                      const uint32_t *__restrict__ own,
-                                                                      uint32_t nseg, int seg_log, uint32_t limit,
-                                                                      int32_t *__restrict__ cont, int r) {
+__device__ __forceinline__ void wg_fold(XyzzZ<C> &acc, int32_t *part) {
   constexpr int PW = ZLayout<C>::XYZZ;
-  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nseg) return;
-  const uint32_t b = own[j];
-  const uint64_t fs = off[b] >> seg_log;
-  if (j <= fs) return;  // not a continuation partial
-  const uint64_t ls = (uint64_t)(off[b + 1] - 1) >> seg_log, K = ls - fs;
-  if (K <= limit) return;  // summed serially by the fix kernel
-  const uint64_t i = j - fs - 1;
-  if (i & ((2ull << r) - 1)) return;
-  if (i + (1ull << r) >= K) return;
-  XyzzZ<C> a, q;
-  zxyzz_load(a, cont + (uint64_t)j * PW);
-  zxyzz_load(q, cont + ((uint64_t)j + (1ull << r)) * PW);
-  zxyzz_add(a, a, q);
-  zxyzz_store(cont + (uint64_t)j * PW, a);
+  const uint32_t t = threadIdx.x;
+  zxyzz_store(part + t * PW, acc);
+  __syncthreads();
+  for (uint32_t s = kThreads / 2; s > 0; s >>= 1) {
+    if (t < s) {
+      XyzzZ<C> q;
+      zxyzz_load(acc, part + t * PW);
+      zxyzz_load(q, part + (t + s) * PW);
+      zxyzz_add(acc, acc, q);
+      zxyzz_store(part + t * PW, acc);
+    }
+    __syncthreads();
+  }
+  if (t == 0) zxyzz_load(acc, part);
+  __syncthreads();  // part is reused by the caller's next item
 }
 
 template <class C>
-__global__ __launch_bounds__(kThreads) void msm_heavy_final_kernel(const uint32_t *__restrict__ off,
-                                                                  const uint32_t *__restrict__ heavy, int seg_log,
-                                                                  int32_t *__restrict__ bsum,
-                                                                  const int32_t *__restrict__ cont) {
+__device__ __forceinline__ void bsum_add(int32_t *__restrict__ bsum, uint32_t b, const XyzzZ<C> &v) {
   constexpr int PW = ZLayout<C>::XYZZ;
-  const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
-  if (h >= heavy[0]) return;
-  const uint32_t b = heavy[3 + h];
-  const uint64_t fs = off[b] >> seg_log;
-  XyzzZ<C> a, q;
+  XyzzZ<C> a;
   zxyzz_load(a, bsum + (size_t)b * PW);
-  zxyzz_load(q, cont + (fs + 1) * PW);
-  zxyzz_add(a, a, q);
+  zxyzz_add(a, a, v);
   zxyzz_store(bsum + (size_t)b * PW, a);
 }
 
-// Heavy buckets without a host round trip (captured pipelines): workgroup g takes listed buckets
-// g, g + gridDim.x, ...; its threads sum the bucket's continuation partials j = tid, tid + 256,
-// ... serially, a tree in LDS adds the 256 partial sums, and thread 0 adds the total into the
-// bucket sum.  heavy[0] (the count) is read on the device, so the launch size is fixed.
-static constexpr uint32_t kHeavyTreeBlocks = 1024;
-
 template <class C>
-__global__ __launch_bounds__(kThreads) void msm_heavy_tree_kernel(const uint32_t *__restrict__ off,
+__global__ __launch_bounds__(kThreads) void msm_heavy_part_kernel(const uint32_t *__restrict__ off,
                                                                  const uint32_t *__restrict__ heavy,
-                                                                 uint32_t heavy_cap, int seg_log,
+                                                                 uint32_t heavy_cap, uint32_t G, int seg_log,
                                                                  int32_t *__restrict__ bsum,
-                                                                 const int32_t *__restrict__ cont) {
+                                                                 const int32_t *__restrict__ cont,
+                                                                 int32_t *__restrict__ hsum) {
   constexpr int PW = ZLayout<C>::XYZZ;
-  extern __shared__ int32_t part[];  // kThreads XYZZ points
+  extern __shared__ int32_t part[];
   const uint32_t nh = min(heavy[0], heavy_cap);
-  const uint32_t t = threadIdx.x;
-  for (uint32_t h = blockIdx.x; h < nh; h += gridDim.x) {
+  if (nh == 0) return;
+  const bool direct = nh > G;
+  const uint32_t Q = direct ? 1u : G / nh, items = direct ? nh : nh * Q;
+  const uint64_t stride = (uint64_t)Q * kThreads;
+  for (uint32_t it = blockIdx.x; it < items; it += gridDim.x) {
+    const uint32_t h = it % nh, q = it / nh;
     const uint32_t b = heavy[3 + h];
     const uint64_t fs = off[b] >> seg_log, ls = (uint64_t)(off[b + 1] - 1) >> seg_log;
-    XyzzZ<C> acc, q;
+    XyzzZ<C> acc, x;
     zxyzz_set_inf(acc);
-    for (uint64_t j = fs + 1 + t; j <= ls; j += kThreads) {
-      zxyzz_load(q, cont + j * PW);
-      zxyzz_add(acc, acc, q);
+    for (uint64_t j = fs + 1 + (uint64_t)q * kThreads + threadIdx.x; j <= ls; j += stride) {
+      zxyzz_load(x, cont + j * PW);
+      zxyzz_add(acc, acc, x);
     }
-    zxyzz_store(part + t * PW, acc);
-    __syncthreads();
-    for (uint32_t s = kThreads / 2; s > 0; s >>= 1) {
-      if (t < s) {
-        zxyzz_load(acc, part + t * PW);
-        zxyzz_load(q, part + (t + s) * PW);
-        zxyzz_add(acc, acc, q);
-        zxyzz_store(part + t * PW, acc);
-      }
-      __syncthreads();
+    wg_fold<C>(acc, part);
+    if (threadIdx.x == 0) {
+      if (direct) bsum_add<C>(bsum, b, acc);
+      else zxyzz_store(hsum + (size_t)it * PW, acc);
     }
-    if (t == 0) {
-      zxyzz_load(acc, bsum + (size_t)b * PW);
-      zxyzz_load(q, part);
-      zxyzz_add(acc, acc, q);
-      zxyzz_store(bsum + (size_t)b * PW, acc);
+  }
+}
+
+template <class C>
+__global__ __launch_bounds__(kThreads) void msm_heavy_combine_kernel(const uint32_t *__restrict__ off,
+                                                                    const uint32_t *__restrict__ heavy,
+                                                                    uint32_t heavy_cap, uint32_t G,
+                                                                    int32_t *__restrict__ bsum,
+                                                                    const int32_t *__restrict__ hsum) {
+  constexpr int PW = ZLayout<C>::XYZZ;
+  extern __shared__ int32_t part[];
+  const uint32_t nh = min(heavy[0], heavy_cap);
+  if (nh == 0 || nh > G) return;
+  const uint32_t Q = G / nh;
+  for (uint32_t h = blockIdx.x; h < nh; h += gridDim.x) {
+    XyzzZ<C> acc, x;
+    zxyzz_set_inf(acc);
+    for (uint32_t q = threadIdx.x; q < Q; q += kThreads) {
+      zxyzz_load(x, hsum + ((size_t)q * nh + h) * PW);
+      zxyzz_add(acc, acc, x);
     }
-    __syncthreads();  // part is reused by the next bucket
+    wg_fold<C>(acc, part);
+    if (threadIdx.x == 0) bsum_add<C>(bsum, heavy[3 + h], acc);
   }
 }
 
@@ -1309,20 +1287,18 @@ struct Events {
   }
 };
 
-// Host-side resources of one running MSM: pinned words for the mid-MSM read-back, pinned
-// result words and an event.  Taken from a pool and returned after the call, so concurrent
+// Host-side resources of one running MSM: pinned words for the statistics read-back, pinned
+// result words and a side stream.  Taken from a pool and returned after the call, so concurrent
 // callers (batch workers, serving threads) each hold one and the pool never grows past the
 // peak concurrency.
 struct MsmStage {
   uint32_t *info = nullptr;  // heavy count, most partials, sorted entries
   uint32_t *res = nullptr;   // pinned result words
   size_t res_words = 0;
-  hipEvent_t info_ev = nullptr;
   hipStream_t aux = nullptr;     // a second stream, created on first use
   hipEvent_t aux_ev = nullptr;
   int init() {
     ZK_HIP(hipHostMalloc((void **)&info, 16, hipHostMallocDefault));
-    ZK_HIP(hipEventCreateWithFlags(&info_ev, hipEventDisableTiming));
     return ZK_OK;
   }
   int ensure_aux() {
@@ -1502,11 +1478,9 @@ static int msm_sparse_run(int n, const uint64_t *d_scalars, int enl, int mont, c
                           const zkg_scalar_stats &stats, MsmResult *out);
 
 // ---------------------------------------------------------------------------------------
-// One dense MSM pipeline: its schedule, launch sizes and device buffers.  msm_run drives it
-// with one host read between launches (the heavy-bucket count, fetched while the accumulation
-// runs); the captured batch (msm_graph_*) queues the same launches with the heavy buckets
-// reduced by one device-driven kernel, so nothing from the digits to the result copy waits
-// on the host.
+// One dense MSM pipeline: its schedule, launch sizes and device buffers.  msm_run queues its
+// launches on a stream and waits once, for the result words; the captured batch (msm_graph_*)
+// records the same launches into a graph.  No launch size depends on the data.
 
 template <class C>
 struct MsmPipe {
@@ -1525,7 +1499,8 @@ struct MsmPipe {
   int pyr_final = -1;  // buffer holding the pyramid's output: -1 bsum, 0 pyrA, 1 pyrB
   bool narrow = false;
   uint64_t nkeys = 0, total = 0, nseg = 0, nph = 0, nbig_cap = 0;
-  DevBuf dig, vals, tmp, phist, ppos, big, rcnt, off, bsum, cont, heavy, pyrA, pyrB, scan_b, zpts, outx, own;
+  uint32_t hblocks = 0;  // workgroups of the heavy-bucket launches
+  DevBuf dig, vals, tmp, phist, ppos, big, rcnt, off, bsum, cont, heavy, hsum, pyrA, pyrB, scan_b, zpts, outx;
 
   const int32_t *zp() const { return points_internal ? (const int32_t *)d_points : zpts.as<int32_t>(); }
   const int32_t *pyr_out() const {
@@ -1581,6 +1556,8 @@ struct MsmPipe {
     // every bucket may be heavy (small seg_log, few entries per segment)
     heavy_cap = (uint32_t)nkeys;
     ZK_TRY(heavy.alloc(((uint64_t)heavy_cap + 3) * 4));
+    hblocks = (uint32_t)std::min<uint64_t>(heavy_cap, kHeavyBlocks);
+    ZK_TRY(hsum.alloc((uint64_t)hblocks * ZW * 4));
     ZK_TRY(pyrA.alloc(nkeys * ZW * 4));
     ZK_TRY(pyrB.alloc(nkeys * ZW * 4));
     const uint64_t per_block = (uint64_t)kThreads * kScanPer;
@@ -1652,41 +1629,14 @@ struct MsmPipe {
                        limit);
   }
 
-  // heavy buckets, host-driven: nh buckets with at most kmax continuation partials each,
-  // pairwise rounds then the final addition into the bucket sums
-  int launch_heavy_rounds(hipStream_t st, uint32_t nh, uint32_t kmax, uint32_t nent) {
-    const dim3 blk(kThreads);
-    // segments holding sorted entries; every continuation partial lies below this bound
-    const uint32_t nseg_used = (uint32_t)(((uint64_t)nent + (1ull << seg_log) - 1) >> seg_log);
-    for (int r = 0; (1ull << r) < kmax; r++) {
-      const uint32_t pairs = (uint32_t)((kmax + (2ull << r) - 1) >> (r + 1));
-      const uint64_t thr = (uint64_t)nh * pairs;
-      if (thr <= nseg_used) {
-        hipLaunchKernelGGL(msm_heavy_round_kernel<C>, dim3((unsigned)((thr + kThreads - 1) / kThreads)), blk, 0,
-                           st, off.as<uint32_t>(), heavy.as<uint32_t>(), seg_log, cont.as<int32_t>(), r, pairs);
-      } else {
-        // a few buckets with very many partials: a bucket-by-pair grid would be nh * pairs
-        if (!own.p) {
-          ZK_TRY(own.alloc((uint64_t)nseg_used * 4));
-          hipLaunchKernelGGL(msm_seg_owner_kernel, dim3((nseg_used + kThreads - 1) / kThreads), blk, 0, st,
-                             off.as<uint32_t>(), (uint32_t)nkeys, seg_log, nseg_used, own.as<uint32_t>());
-        }
-        hipLaunchKernelGGL(msm_heavy_round_seg_kernel<C>, dim3((nseg_used + kThreads - 1) / kThreads), blk, 0, st,
-                           off.as<uint32_t>(), own.as<uint32_t>(), nseg_used, seg_log, limit, cont.as<int32_t>(), r);
-      }
-    }
-    hipLaunchKernelGGL(msm_heavy_final_kernel<C>, dim3((nh + kThreads - 1) / kThreads), blk, 0, st,
-                       off.as<uint32_t>(), heavy.as<uint32_t>(), seg_log, bsum.as<int32_t>(), cont.as<int32_t>());
-    return ZK_OK;
-  }
-
-  // heavy buckets, device-driven: one workgroup per listed bucket (grid-stride), whatever
-  // their number and size; exits at once when there is none
-  void launch_heavy_tree(hipStream_t st) {
-    const unsigned g = (unsigned)std::min<uint64_t>(heavy_cap, kHeavyTreeBlocks);
-    hipLaunchKernelGGL(msm_heavy_tree_kernel<C>, dim3(g), dim3(kThreads), (size_t)kThreads * ZW * 4, st,
-                       off.as<uint32_t>(), heavy.as<uint32_t>(), heavy_cap, seg_log, bsum.as<int32_t>(),
-                       cont.as<int32_t>());
+  // heavy buckets: the two device-driven launches (they exit at once when none is listed)
+  void launch_heavy(hipStream_t st) {
+    const size_t lds = (size_t)kThreads * ZW * 4;
+    hipLaunchKernelGGL(msm_heavy_part_kernel<C>, dim3(hblocks), dim3(kThreads), lds, st, off.as<uint32_t>(),
+                       heavy.as<uint32_t>(), heavy_cap, hblocks, seg_log, bsum.as<int32_t>(), cont.as<int32_t>(),
+                       hsum.as<int32_t>());
+    hipLaunchKernelGGL(msm_heavy_combine_kernel<C>, dim3(hblocks), dim3(kThreads), lds, st, off.as<uint32_t>(),
+                       heavy.as<uint32_t>(), heavy_cap, hblocks, bsum.as<int32_t>(), hsum.as<int32_t>());
   }
 
   // Bucket pyramid over the W windows of B bucket sums; launch = false only works out the
@@ -1832,30 +1782,18 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   ev.mark(st);
   p.launch_sort(st);
   ZK_HIP(hipGetLastError());
-  // heavy count, most partials and sorted entries, read back while the accumulation runs
+  // heavy count, most partials and sorted entries (statistics, read after the MSM)
   StageLease lease;
   ZK_TRY(lease.acquire());
   MsmStage &stage = *lease.s;
   uint32_t *info = stage.info;
   ZK_HIP(hipMemcpyAsync(info, p.heavy.p, 12, hipMemcpyDeviceToHost, st));
-  ZK_HIP(hipEventRecord(stage.info_ev, st));
   ev.mark(st);
   p.launch_accum(st);
   ZK_HIP(hipGetLastError());
   ev.mark(st);
   p.launch_fix(st);
-  ZK_HIP(hipGetLastError());
-  ZK_HIP(hipEventSynchronize(stage.info_ev));  // completed before the accumulation started
-  const uint32_t hh[2] = {info[0], info[1]}, nent = info[2];
-  g_stats[0] = p.c;
-  g_stats[1] = p.W;
-  g_stats[2] = nent;
-  g_stats[3] = hh[0];
-  if (hh[0] > p.heavy_cap) {
-    set_error("msm: too many heavy buckets");
-    return ZK_EINVAL;
-  }
-  if (hh[0]) ZK_TRY(p.launch_heavy_rounds(st, hh[0], hh[1], nent));
+  p.launch_heavy(st);
   ZK_HIP(hipGetLastError());
   ev.mark(st);
   p.pyramid(st, true);
@@ -1869,6 +1807,10 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   drain.release();
   ev.mark(st);
   auto t_host1 = std::chrono::steady_clock::now();
+  g_stats[0] = p.c;
+  g_stats[1] = p.W;
+  g_stats[2] = info[2];
+  g_stats[3] = std::min(info[0], p.heavy_cap);
   p.combine(stage.res, out);
   auto t_host2 = std::chrono::steady_clock::now();
 
@@ -2071,8 +2013,7 @@ int g1_scalar_mul_gen(int curve, int n, const uint64_t *d_scalars, uint64_t *d_o
 // the jobs are dealt, largest first, to `nstreams` streams forked from and joined back into one
 // capture stream, so the graph's branches run concurrently.  run() launches the graph once,
 // waits for it, and combines the jobs' window sums on the host.  Nothing inside a replay waits
-// on the host: heavy buckets go through msm_heavy_tree_kernel, and every launch size is fixed
-// at capture.  A replay with scalars the fitted windows do not cover (a wider bit length than
+// on the host: every launch size is fixed at capture.  A replay with scalars the fitted windows do not cover (a wider bit length than
 // at create) is caught by the digits pass and reported as an error.
 
 __global__ void msm_flag_clear_kernel(uint32_t *flag) { *flag = 0; }
@@ -2111,7 +2052,7 @@ struct MsmGraphJobT : MsmGraphJob {
     p.launch_sort(st);
     p.launch_accum(st);
     p.launch_fix(st);
-    p.launch_heavy_tree(st);
+    p.launch_heavy(st);
     p.pyramid(st, true);
     p.launch_out(st);
     (void)hipMemcpyAsync(res, p.outx.p, (size_t)p.nres * MsmPipe<C>::PW * 4, hipMemcpyDeviceToHost, st);
