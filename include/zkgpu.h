@@ -235,7 +235,9 @@ int zkg_msm(int curve, int npoints, const uint64_t *d_expos, int expo_nlimbs, co
             uint64_t *tgt_host, int flags, const zkg_msm_schedule *sched);
 
 /* Converts npoints affine points once into the device-internal reduced-radix form (for
- * bases reused across many MSMs, e.g. a KZG SRS); free with zkg_free. */
+ * bases reused across many MSMs, e.g. a KZG SRS); free with zkg_free.  BN254 sets also hold
+ * phi(P_i) = (beta x_i, y_i) beside each P_i (128 bytes per point), so MSMs over the set or
+ * any prefix of it take the GLV split; BLS12-381 sets take 128 bytes per point. */
 int zkg_msm_prepare_points(int curve, int npoints, const uint64_t *d_grps, void **d_prepared);
 
 /* NTT on device buffers (flags: 0 forward, 1 inverse); eta NULL = no coset shift.
@@ -351,8 +353,8 @@ int zkg_serve_destroy(void *server);
 int zkg_msm_scalar_stats(int curve, int npoints, const uint64_t *d_expos, int expo_nlimbs, int flags,
                          zkg_scalar_stats *out);
 /* cost-model input row for one (job, candidate schedule); ZKG_MSM_NFEAT floats.  These three
- * describe a job with 4-limb scalars over affine (unprepared) points, whose BN254 form takes the
- * GLV split when its scalars are wider than 128 bits (2n points, 127-bit halves). */
+ * describe a job with 4-limb scalars, whose BN254 form takes the GLV split when its scalars are
+ * wider than 128 bits (2n points, 127-bit halves). */
 #define ZKG_MSM_NFEAT 11
 int zkg_msm_features(int curve, const zkg_scalar_stats *st, int window, int seg_log, float *out);
 /* candidate schedules searched by the autotuner and the cost model; returns the count */

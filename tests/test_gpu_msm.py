@@ -250,3 +250,29 @@ def test_msm_bn254_glv_extremes(zk):
     exp = Oracle(C).msm_affine(S, P)
     assert (zk.msm_std(C.id, S, P) == exp).all()
     assert C.words_to_affine(exp) == C.mul(sum(s * k for s, k in zip(ss, ks)) % C.r, C.g)
+
+
+@pytest.mark.parametrize("C", [BN, BLS], ids=["bn254", "bls12-381"])
+def test_prepared_prefixes_match_affine(zk, C):
+    """A prepared set of N points (BN254: P_i and phi(P_i) side by side, so the GLV split applies)
+    used for MSMs over its first n <= N points, with the infinity among them, equals the MSM over
+    the same affine words, and the oracle: plain and statistics schedules, Montgomery and
+    standard scalars."""
+    N = 5000
+    ks = random_scalars(5150 + C.id, N, C.r)
+    P = make_points(C, ks)
+    P[17] = 0  # the affine infinity (0, 0)
+    P[N - 1] = 0
+    dP = zk.DeviceArray.from_host(P)
+    prep = zk.PreparedPoints(C.id, dP, N)
+    o = Oracle(C)
+    for n in (N, 4096, 1000, 18, 1):
+        S = ints_to_array(random_scalars(5160 + n, n, C.r), 4)
+        dS = zk.DeviceArray.from_host(S)
+        exp = o.msm_affine(S, P[:n])
+        for pol in (zk.SCHED_PLAIN, zk.SCHED_STATS):
+            a = zk.msm_device(C.id, n, dS, prep, policy=pol)
+            b = zk.msm_device(C.id, n, dS, dP, policy=pol)
+            assert (a == exp).all() and (b == exp).all(), (n, pol)
+        dS.free()
+    prep.free()

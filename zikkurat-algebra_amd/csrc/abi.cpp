@@ -75,7 +75,7 @@ int binary_host(int curve, int nrows, int npoints, const uint8_t *const *rows, c
   ZK_TRY(dp.alloc(pb));
   ZK_TRY(copy_h2d(dp.p, grps, pb, ctx->stream));
   ZK_TRY(dint.alloc((size_t)npoints * msm_internal_point_bytes(curve)));
-  ZK_TRY(msm_prepare_points(curve, npoints, dp.as<uint64_t>(), dint.p, ctx->stream));
+  ZK_TRY(msm_prepare_points(curve, npoints, dp.as<uint64_t>(), dint.p, ctx->stream, false));
   ZK_TRY(bits.alloc((size_t)nrows * npoints));
   for (int r = 0; r < nrows; r++)
     ZK_TRY(copy_h2d(bits.as<uint8_t>() + (size_t)r * npoints, rows[r], (size_t)npoints, ctx->stream));

@@ -48,6 +48,10 @@ struct ZLayout {
   static constexpr int AFF = PACKED ? 16 : 32;
   static_assert(PACKED || 2 * N <= AFF, "affine point must fit one 128-byte line");
   static constexpr int XYZZ = 4 * N;               // 36 or 56 words: multiples of 4
+  // records per point in an internal point set: BN254 keeps P_i and phi(P_i) = (beta x_i, y_i)
+  // side by side (records 2i and 2i + 1, one 128-byte line), so any prefix of a prepared set
+  // is a prepared set and the GLV split applies to it; BLS12-381 keeps P_i alone
+  static constexpr int STRIDE = ZP<C>::T::N == 9 ? 2 : 1;
 };
 
 template <class C>

@@ -163,12 +163,12 @@ __device__ __forceinline__ void tree_sum_lds(int32_t *red, XyzzZ<C> &acc, uint32
 }
 
 // partial[(row, block)] = sum of the selected points of the block's range of the row;
-// pts: prepared affine records (ZLayout<C>::AFF words each)
+// pts: prepared affine records (ZLayout<C>::AFF words each, STRIDE records per point)
 template <class C>
 __global__ __launch_bounds__(kSumThreads) void binary_sum_kernel(const uint8_t *__restrict__ bits, uint32_t n,
                                                                 uint32_t per_block, const int32_t *__restrict__ pts,
                                                                 int32_t *__restrict__ partial) {
-  constexpr int AW = ZLayout<C>::AFF, PW = ZLayout<C>::XYZZ;
+  constexpr int AW = ZLayout<C>::AFF * ZLayout<C>::STRIDE, PW = ZLayout<C>::XYZZ;
   const uint32_t kPerWave = per_block / kSumWaves;  // a multiple of 64
   __shared__ uint32_t queue[kSumWaves][128];
   __shared__ int32_t red[kSumThreads * PW];
@@ -303,7 +303,7 @@ static int binary_sum_run(int nrows, int n, const uint8_t *d_bits, const uint64_
   const int32_t *pts = (const int32_t *)d_points;
   if (!points_internal) {
     ZK_TRY(prep.alloc((size_t)n * msm_internal_point_bytes(C::id)));
-    ZK_TRY(msm_prepare_points(C::id, n, d_points, prep.p, st));
+    ZK_TRY(msm_prepare_points(C::id, n, d_points, prep.p, st, false));
     pts = prep.as<int32_t>();
   }
   // per_block: a multiple of 64 per wavefront, at least kSumMinPerBlock, at most kSumMaxBlocks

@@ -85,9 +85,11 @@ __device__ __forceinline__ uint32_t encode_digit(uint32_t raw, int c, uint32_t &
   return raw ? ((raw - 1) << 1) : kZeroDigit;
 }
 
+// point i (external affine words) -> its internal record (record i STRIDE of out); with phi
+// (BN254) also the record of phi(P_i) next to it
 template <class C>
 __device__ __forceinline__ void point_to_internal(const uint32_t *__restrict__ pts, uint32_t i,
-                                                  int32_t *__restrict__ out, int32_t *__restrict__ phi_out) {
+                                                  int32_t *__restrict__ out, bool phi) {
   using F = typename C::Fp;
   uint32_t w[2 * F::N];
   const uint4 *q = reinterpret_cast<const uint4 *>(pts + (size_t)i * 2 * F::N);
@@ -102,21 +104,30 @@ __device__ __forceinline__ void point_to_internal(const uint32_t *__restrict__ p
   AffZ<C> a;
   zfrom_ext(a.x, w);
   zfrom_ext(a.y, w + F::N);
-  zaff_store(out + (size_t)i * ZLayout<C>::AFF, a);
-  if constexpr (C::id == 0) {
-    if (phi_out) {  // phi(P) = (beta x, y); the affine infinity (0, 0) maps to itself
+  constexpr int AW = ZLayout<C>::AFF, S = ZLayout<C>::STRIDE;
+  zaff_store(out + (size_t)i * S * AW, a);
+  if constexpr (C::id == 0 && S == 2) {
+    if (phi) {  // phi(P) = (beta x, y); the affine infinity (0, 0) maps to itself
       Fz<ZBn254Fp> beta;
       zset_const(beta, Bn254Glv::BETA);
       zmul(a.x, a.x, beta);
-      zaff_store(phi_out + (size_t)i * ZLayout<C>::AFF, a);
+      zaff_store(out + ((size_t)i * S + 1) * AW, a);
     }
   }
+}
+
+// record of entry idx (a digit position): positions below n are the points P_i, positions
+// n + i (GLV) the points phi(P_i)
+template <class C>
+__device__ __forceinline__ size_t point_record(uint32_t idx, uint32_t n) {
+  if constexpr (ZLayout<C>::STRIDE == 2) return idx < n ? 2 * (size_t)idx : 2 * (size_t)(idx - n) + 1;
+  else return idx;
 }
 
 // signed digits of scalar i for every window (row w at dig + w * ns); when ext_pts is given,
 // point i is converted to the internal record in the same pass (both sweeps are bound by
 // memory traffic).  With glv (BN254, 4-limb scalars reduced mod r) scalar i becomes k1 for
-// point i and k2 for point n + i = phi(P_i), whose record is written too (ns = 2n).  With ovf,
+// point i and k2 for position n + i, phi(P_i), whose record is written too (ns = 2n).  With ovf,
 // a scalar the windows do not cover (a carry out of the top window or a bit above it: the
 // schedule was fitted to other scalars) sets *ovf.
 template <class C>
@@ -129,8 +140,7 @@ __global__ __launch_bounds__(kThreads) void msm_digits_kernel(const uint64_t *__
   using Fr = typename C::Fr;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  if (ext_pts)
-    point_to_internal<C>(ext_pts, i, zpts, glv ? zpts + (size_t)n * ZLayout<C>::AFF : (int32_t *)nullptr);
+  if (ext_pts) point_to_internal<C>(ext_pts, i, zpts, glv != 0);
   uint32_t carry = 0;
   if (enl == 4) {
     uint64_t s[4];
@@ -718,10 +728,10 @@ __device__ __forceinline__ void flush_run(const XyzzZ<C> &acc, bool cont_run, ui
 // affine points, external Montgomery words -> internal records (prepared point sets)
 template <class C>
 __global__ __launch_bounds__(kThreads) void msm_points_in_kernel(const uint32_t *__restrict__ pts, uint32_t n,
-                                                                int32_t *__restrict__ out) {
+                                                                int32_t *__restrict__ out, int phi) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  point_to_internal<C>(pts, i, out, nullptr);
+  point_to_internal<C>(pts, i, out, phi != 0);
 }
 
 // final window terms: internal XYZZ -> external canonical words (4 field elements each)
@@ -784,7 +794,8 @@ template <class C, bool PF>
 __global__ __launch_bounds__(kThreads) void msm_accum_kernel(const uint32_t *__restrict__ vals,
                                                             const uint32_t *__restrict__ off, uint32_t nkeys,
                                                             int seg_log, const int32_t *__restrict__ points,
-                                                            int32_t *__restrict__ bsum, int32_t *__restrict__ cont) {
+                                                            uint32_t npts, int32_t *__restrict__ bsum,
+                                                            int32_t *__restrict__ cont) {
   constexpr int AW = ZLayout<C>::AFF;
   const uint32_t E = off[nkeys];
   const uint64_t seg = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -799,18 +810,18 @@ __global__ __launch_bounds__(kThreads) void msm_accum_kernel(const uint32_t *__r
   zxyzz_set_inf(acc);
   uint32_t v = vals[e0];
   AffZ<C> a;
-  if (PF) zaff_load(a, points + (size_t)(v & 0x7fffffffu) * AW);
+  if (PF) zaff_load(a, points + point_record<C>(v & 0x7fffffffu, npts) * AW);
   for (uint64_t e = e0; e < e1; e++) {
     uint32_t vn = 0;
     AffZ<C> an;
     if (PF) {
       if (e + 1 < e1) {
         vn = vals[e + 1];
-        zaff_load(an, points + (size_t)(vn & 0x7fffffffu) * AW);
+        zaff_load(an, points + point_record<C>(vn & 0x7fffffffu, npts) * AW);
       }
     } else {
       v = vals[e];
-      zaff_load(a, points + (size_t)(v & 0x7fffffffu) * AW);
+      zaff_load(a, points + point_record<C>(v & 0x7fffffffu, npts) * AW);
     }
     if (e == next) {
       flush_run<C>(acc, cont_run, seg, cur, bsum, cont);
@@ -1402,8 +1413,8 @@ int msm_scalar_stats(int curve, int n, const uint64_t *d_scalars, int enl, int m
 int msm_resolve(int curve, int n, int enl, const zkg_msm_schedule *user, const zkg_scalar_stats *st,
                 MsmSchedule *out, bool glv_ok) {
   MsmSchedule s;
-  // GLV split (BN254, 4-limb scalars reduced mod r, points given as affine words): 2n points
-  // P_i and phi(P_i) with scalars below 2^127, so windows over 128 bits; entries stay ~16 n at
+  // GLV split (BN254, 4-limb scalars reduced mod r; affine words get their phi(P_i) records in
+  // the digit pass, prepared sets hold them): 2n points P_i and phi(P_i) with scalars below 2^127, so windows over 128 bits; entries stay ~16 n at
   // c = 16 while the buckets and the host's doubling chain halve.  Measured scalars of at most
   // 128 bits are already short and keep the plain layout.
   const bool glv = glv_ok && curve == 0 && enl == 4 && (!st || st->max_bits > 128);
@@ -1552,7 +1563,7 @@ struct MsmPipe {
     ZK_TRY(off.alloc((nkeys + 1) * 4));
     ZK_TRY(bsum.alloc(nkeys * ZW * 4));
     ZK_TRY(cont.alloc(nseg * ZW * 4));
-    if (!points_internal) ZK_TRY(zpts.alloc((uint64_t)np * AW * 4));
+    if (!points_internal) ZK_TRY(zpts.alloc((uint64_t)n * ZLayout<C>::STRIDE * AW * 4));
     // every bucket may be heavy (small seg_log, few entries per segment)
     heavy_cap = (uint32_t)nkeys;
     ZK_TRY(heavy.alloc(((uint64_t)heavy_cap + 3) * 4));
@@ -1617,10 +1628,10 @@ struct MsmPipe {
     const dim3 blk(kThreads), g((unsigned)((nseg + kThreads - 1) / kThreads));
     if (prefetch)
       hipLaunchKernelGGL((msm_accum_kernel<C, true>), g, blk, 0, st, vals.as<uint32_t>(), off.as<uint32_t>(),
-                         (uint32_t)nkeys, seg_log, zp(), bsum.as<int32_t>(), cont.as<int32_t>());
+                         (uint32_t)nkeys, seg_log, zp(), (uint32_t)n, bsum.as<int32_t>(), cont.as<int32_t>());
     else
       hipLaunchKernelGGL((msm_accum_kernel<C, false>), g, blk, 0, st, vals.as<uint32_t>(), off.as<uint32_t>(),
-                         (uint32_t)nkeys, seg_log, zp(), bsum.as<int32_t>(), cont.as<int32_t>());
+                         (uint32_t)nkeys, seg_log, zp(), (uint32_t)n, bsum.as<int32_t>(), cont.as<int32_t>());
   }
 
   void launch_fix(hipStream_t st) {
@@ -1770,7 +1781,7 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   if (use_stats && n >= (1 << 14) && n < (1 << 21) && (uint64_t)(stats.n_zero + stats.n_one) * 2 >= (uint64_t)n)
     return msm_sparse_run<C>(n, d_scalars, enl, mont, d_points, points_internal, user, st, stats, out);
   MsmSchedule sch;
-  ZK_TRY(msm_resolve(C::id, n, enl, user, use_stats ? &stats : nullptr, &sch, !points_internal));
+  ZK_TRY(msm_resolve(C::id, n, enl, user, use_stats ? &stats : nullptr, &sch, true));
   // every buffer the queued kernels touch lives until the stream is drained (the guard below)
   MsmPipe<C> p;
   StreamSyncGuard drain(st);
@@ -1833,7 +1844,8 @@ static int msm_sparse_run(int n, const uint64_t *d_scalars, int enl, int mont, c
                           bool points_internal, const zkg_msm_schedule *user, hipStream_t st,
                           const zkg_scalar_stats &stats, MsmResult *out) {
   using F = typename C::Fp;
-  const int pw = points_internal ? ZLayout<C>::AFF : 2 * F::N;  // u32 words per point record
+  // u32 words per point (internal: its records, P_i and phi(P_i) for BN254)
+  const int pw = points_internal ? ZLayout<C>::AFF * ZLayout<C>::STRIDE : 2 * F::N;
   const uint64_t k = (uint64_t)n - stats.n_zero - stats.n_one;   // scalars outside {0, 1}
   DevBuf keep, pos, ones, scan_b, sc2, pts2;
   // drains the stream on every return: with no dense part (k = 0) nothing else waits for the
@@ -1915,29 +1927,30 @@ int msm_device(int curve, int n, const uint64_t *d_scalars, int enl, int mont, c
 }
 
 template <class C>
-static int msm_prepare_points_t(int n, const uint64_t *d_ext, void *d_int, hipStream_t st) {
+static int msm_prepare_points_t(int n, const uint64_t *d_ext, void *d_int, bool phi, hipStream_t st) {
   hipLaunchKernelGGL(msm_points_in_kernel<C>, dim3((n + kThreads - 1) / kThreads), dim3(kThreads), 0, st,
-                     (const uint32_t *)d_ext, (uint32_t)n, (int32_t *)d_int);
+                     (const uint32_t *)d_ext, (uint32_t)n, (int32_t *)d_int, (int)phi);
   ZK_HIP(hipGetLastError());
   ZK_HIP(hipStreamSynchronize(st));
   return ZK_OK;
 }
 
 size_t msm_internal_point_bytes(int curve) {
-  return curve == 0 ? ZLayout<Bn254G1>::AFF * 4 : ZLayout<Bls381G1>::AFF * 4;
+  return curve == 0 ? ZLayout<Bn254G1>::AFF * ZLayout<Bn254G1>::STRIDE * 4
+                    : ZLayout<Bls381G1>::AFF * ZLayout<Bls381G1>::STRIDE * 4;
 }
 
-int msm_prepare_points(int curve, int n, const uint64_t *d_ext, void *d_int, hipStream_t st) {
+int msm_prepare_points(int curve, int n, const uint64_t *d_ext, void *d_int, hipStream_t st, bool phi) {
   if (n <= 0) return ZK_OK;
-  if (curve == 0) return msm_prepare_points_t<Bn254G1>(n, d_ext, d_int, st);
-  if (curve == 1) return msm_prepare_points_t<Bls381G1>(n, d_ext, d_int, st);
+  if (curve == 0) return msm_prepare_points_t<Bn254G1>(n, d_ext, d_int, phi, st);
+  if (curve == 1) return msm_prepare_points_t<Bls381G1>(n, d_ext, d_int, phi, st);
   set_error("msm_prepare_points: bad curve id");
   return ZK_EINVAL;
 }
 
 // device bytes one MSM of n points allocates (admission control of the serving loop)
 size_t msm_footprint(int curve, int n, int enl, bool points_internal, const zkg_msm_schedule *sched) {
-  const MsmSchedule s = resolve_schedule(curve, n, enl, sched, !points_internal);
+  const MsmSchedule s = resolve_schedule(curve, n, enl, sched, true);
   const uint64_t np = s.glv ? 2 * (uint64_t)n : (uint64_t)n;
   const uint64_t W = s.windows, B = 1ull << (s.c - 1), nk = W * B, total = W * np;
   const uint64_t zw = curve == 0 ? ZLayout<Bn254G1>::XYZZ : ZLayout<Bls381G1>::XYZZ;
@@ -1951,7 +1964,7 @@ size_t msm_footprint(int curve, int n, int enl, bool points_internal, const zkg_
   // counts
   uint64_t b = total * 16 + nk * 8 + 3 * nk * zw * 4 + ((total >> s.seg_log) + 1) * zw * 4 + nph * 8 +
                (W << logP) * 4 + nbig * nchunks * ((uint64_t)1 << L) * 4;
-  if (!points_internal) b += np * aw * 4;
+  if (!points_internal) b += (uint64_t)n * msm_internal_point_bytes(curve);
   return (size_t)b;
 }
 
@@ -2041,7 +2054,7 @@ struct MsmGraphJobT : MsmGraphJob {
     const bool use_stats = user && user->policy != ZKG_SCHED_PLAIN;
     if (use_stats) ZK_TRY(msm_scalar_stats(C::id, n, sc, enl, mont, st, &stats));
     MsmSchedule sch;
-    ZK_TRY(msm_resolve(C::id, n, enl, user, use_stats ? &stats : nullptr, &sch, !internal));
+    ZK_TRY(msm_resolve(C::id, n, enl, user, use_stats ? &stats : nullptr, &sch, true));
     ZK_TRY(p.setup(n, sc, enl, mont, pts, internal, sch, user));
     p.ovf = use_stats ? ovf : nullptr;  // declared-width windows cover every scalar
     ZK_HIP(hipHostMalloc((void **)&res, (size_t)p.nres * MsmPipe<C>::PW * 4, hipHostMallocDefault));

@@ -45,9 +45,10 @@ int msm_scalar_stats(int curve, int n, const uint64_t *d_scalars, int enl, int m
 bool cost_model_pick(int curve, const zkg_scalar_stats &st, bool glv_ok, int *c, int *seg_log);
 void msm_features(int curve, const zkg_scalar_stats &st, int c, int seg_log, bool glv_ok, float *x);
 int msm_candidates(int curve, const zkg_scalar_stats &st, bool glv_ok, int max, int *cs, int *segs);
-// glv_ok: the points arrive as affine words, so a BN254 job may take the GLV split (2n points
-// P_i and phi(P_i), scalars below 2^127): under the plain schedule always, under the
-// statistics policies when the measured scalars are wider than 128 bits
+// glv_ok: a BN254 job may take the GLV split (2n points P_i and phi(P_i), scalars below 2^127;
+// affine words are converted with phi in the digit pass, prepared sets carry phi): under the
+// plain schedule always, under the statistics policies when the measured scalars are wider
+// than 128 bits
 int msm_resolve(int curve, int n, int enl, const zkg_msm_schedule *user, const zkg_scalar_stats *st,
                 MsmSchedule *out, bool glv_ok = false);
 
@@ -59,7 +60,10 @@ int msm_write_result(int curve, const MsmResult &r, uint64_t *tgt, int flags);
 // points_internal: d_points already holds the reduced-radix form written by msm_prepare_points
 int msm_device(int curve, int n, const uint64_t *d_scalars, int enl, int mont, const uint64_t *d_points,
                bool points_internal, const zkg_msm_schedule *sched, hipStream_t st, MsmResult *out);
-int msm_prepare_points(int curve, int n, const uint64_t *d_ext, void *d_int, hipStream_t st);
+// external affine points -> internal records (msm_internal_point_bytes each); BN254 records
+// come in (P_i, phi(P_i)) pairs, the phi half written only when phi is set (MSMs take the GLV
+// split on a prepared set with it; a binary sum never reads it)
+int msm_prepare_points(int curve, int n, const uint64_t *d_ext, void *d_int, hipStream_t st, bool phi = true);
 size_t msm_internal_point_bytes(int curve);
 size_t msm_footprint(int curve, int n, int enl, bool points_internal, const zkg_msm_schedule *sched);
 
