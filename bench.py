@@ -297,9 +297,11 @@ def bench_ntt_batch(njobs=256, seed=4):
     plain()
     zkgpu.synchronize()
     t_plain = time.perf_counter() - t0
+    # the first plan of the process also pays HIP's one-time costs (a new hardware queue, the
+    # first graph instantiation); a later plan is what a long-lived caller pays per batch
     t0 = time.perf_counter()
     plan = zkgpu.NttBatch(0, jobs)
-    t_create = time.perf_counter() - t0
+    t_first_create = time.perf_counter() - t0
     plan.run()
     reps = 3
     t0 = time.perf_counter()
@@ -307,10 +309,20 @@ def bench_ntt_batch(njobs=256, seed=4):
     t_batch = (time.perf_counter() - t0) / reps
     same = all(bool((j[4].to_host() == p.to_host()).all()) for j, p in zip(jobs, plain_out))
     plan.close()
+    # one shot: a fresh plan built and replayed once, against the 256 plain calls
+    t0 = time.perf_counter()
+    plan = zkgpu.NttBatch(0, jobs)
+    t_create = time.perf_counter() - t0
+    plan.run()
+    t_oneshot = time.perf_counter() - t0
+    plan.close()
     bytes_moved = 64 * total
     return {"ntt_batch_jobs": njobs, "ntt_batch_elements": int(total),
             "ntt_batch_ms": round(t_batch * 1e3, 3), "ntt_batch_device_ms": round(min(dev_ms), 3),
-            "ntt_plain_ms": round(t_plain * 1e3, 3), "ntt_batch_create_ms": round(t_create * 1e3, 1),
+            "ntt_plain_ms": round(t_plain * 1e3, 3), "ntt_batch_create_ms": round(t_create * 1e3, 2),
+            "ntt_batch_first_create_ms": round(t_first_create * 1e3, 1),
+            "ntt_batch_oneshot_ms": round(t_oneshot * 1e3, 3),
+            "ntt_batch_oneshot_vs_plain": round(t_plain / t_oneshot, 3),
             "ntt_batch_GBps": round(bytes_moved / t_batch / 1e9, 1), "ntt_plain_GBps": round(bytes_moved / t_plain / 1e9, 1),
             "ntt_batched_vs_plain": round(t_plain / t_batch, 3), "ntt_batch_bit_identical_to_plain": same}
 

@@ -147,3 +147,26 @@ def test_extreme_inputs_closed_form(zk, C, m):
             num = c * (pow(eta, N, C.r) - 1) % C.r
             den = (eta * pow(gr, k, C.r) - 1) % C.r
             assert got[k] == mont(num * pow(den, -1, C.r))
+
+
+@pytest.mark.parametrize("C", CURVES, ids=["bn254", "bls12-381"])
+@pytest.mark.parametrize("m", [1, 5, 9, 12, 16, 20])
+def test_in_place_equals_out_of_place(zk, C, m):
+    """zkg_ntt with the source buffer as target (the first pass then gathers into a scratch
+    vector instead of the target) gives the out-of-place words: forward, inverse and both coset
+    forms, and the no-allocation entry point likewise where it applies."""
+    N = 1 << m
+    _, X = rand_vec(C, m, 300 + m)
+    g = gen_arr(C, m)
+    eta = np.array(to_limbs(C.fr_to_mont(5), 4), np.uint64)
+    for inverse in (False, True):
+        for e in (None, eta):
+            dX = zk.DeviceArray.from_host(X)
+            dY = zk.DeviceArray(N * 4)
+            zk.ntt_device(C.id, m, g, dX, dY, inverse=inverse, eta=e)
+            zk.ntt_device(C.id, m, g, dX, dX, inverse=inverse, eta=e)
+            assert (dX.to_host((N, 4)) == dY.to_host((N, 4))).all(), (inverse, e is not None)
+            if m <= 18:
+                dZ = zk.DeviceArray.from_host(X)
+                zk.ntt_device_noalloc(C.id, m, g, dZ, dZ, inverse=inverse, eta=e)
+                assert (dZ.to_host((N, 4)) == dY.to_host((N, 4))).all(), ("noalloc", inverse, e is not None)

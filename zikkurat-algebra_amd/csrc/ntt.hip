@@ -10,12 +10,13 @@
 // ntt_network_* (the reference's own butterfly network, stage by stage).
 //
 // Fast path: the transform is factored N = N_1 N_2 ... N_P (each N_p <= 2^11).  Pass p views
-// the array as [R_p][N_p][C_p] (R_p = N_1..N_{p-1}, C_p = N_{p+1}..N_P), loads a tile of T
-// columns into LDS in bit-reversed order, runs log2(N_p) radix-2 DIT stages in LDS with
-// twiddles omega^(N/N_p * j), multiplies by the four-step twiddle omega^(R_p c k) and writes
-// the tile back in place.  The last pass (C_P = 1) writes rows to their digit-reversed
-// offsets, which leaves the output in natural order.  All passes read and write HBM
-// coalesced in T*32-byte runs.
+// the array as [R_p][N_p][C_p] (R_p = N_1..N_{p-1}, C_p = N_{p+1}..N_P).  The passes run in
+// the transposed order (PassArgs): the first to run (p = P, C_P = 1) gathers its rows from
+// the digit-reversed input offsets into the output vector; each later pass loads a tile of T
+// columns, multiplies by the four-step twiddle omega^(R_p c n), runs log2(N_p) radix-2 DIT
+// stages in LDS (bit-reversed placement, twiddles omega^(N/N_p * j)) and writes the tile back
+// in place, so the output ends in natural order with no scratch vector.  All passes read and
+// write HBM coalesced in T*32-byte runs.
 #include <string.h>
 
 #include <algorithm>
@@ -37,17 +38,33 @@ __device__ __forceinline__ uint32_t bitrev(uint32_t x, int bits) {
   return bits ? (__builtin_bitreverse32(x) >> (32 - bits)) : 0u;
 }
 
+// One pass of the multi-pass transform.  With passes of s_0, ..., s_{P-1} stages (sum m), pass
+// p views the vector as [R_p][N_p][C_p] (R_p = 2^(s_0 + ... + s_{p-1}), N_p = 2^(s_p)).  The
+// classic decomposition runs p = 0, 1, ..., P-1, each an N_p-point DFT along the middle axis
+// followed by the four-step twiddle omega^(R_p c k), and writes its output in digit-reversed
+// order.  The DFT matrix is symmetric, so the transposed product runs the same passes in the
+// opposite order, each twiddle before its DFT, and the digit reversal moves to the input:
+//   gather pass (p = P-1, first to run)  reads x at rho + (n << logr), writes row rev(rho)
+//                                        (rows of 2^s contiguous elements);
+//   passes p = P-2 ... 0                 DFT, write back in place.
+// A twiddle is diagonal in the element's offset, so pass p's twiddle omega^(R_p c n) is applied
+// by the pass that runs before it, at its store (the offset decomposed in pass p's view): the
+// product leaves a value below 2p, ready for the canonical store.  Every pass after the first
+// works in place on the output vector, so an out-of-place transform needs no scratch vector.
+// The last pass to run (p = 0, or the gather pass when P = 1) writes natural order and applies
+// N^-1 and the coset post-scale.
 struct PassArgs {
   const uint32_t *in;
   uint32_t *out;
   const uint32_t *tw;       // omega^j table
-  const uint32_t *eta_pre;  // forward shifted: multiply input i by eta^i (first pass only)
-  const uint32_t *eta_post; // inverse shifted: multiply output i by eta^i (last pass only)
-  uint32_t ninv[8];         // last pass factor, internal form: N^-1 R' (inverse) or R' (forward)
-  int logn, s, logc, logr, logt, last, inverse, tw_shift;
-  int has_ninv;             // last pass: the factor is not one (inverse, or a coset post-scale)
-  int nsplit;               // number of passes before this (last pass: digit reversal)
-  int split[6];             // log sizes of the earlier passes
+  const uint32_t *eta_pre;  // forward shifted: multiply input i by eta^i (gather pass only)
+  const uint32_t *eta_post; // inverse shifted: multiply output i by eta^i (final pass only)
+  uint32_t ninv[8];         // final pass factor, internal form: N^-1 R' (inverse) or R' (forward)
+  int logn, s, logc, logr, logt, gather, fin, inverse, tw_shift;
+  int nx_s, nx_logc, nx_logr;  // the next pass's view (not fin): its twiddle is applied at the store
+  int has_ninv;             // final pass: the factor is not one (inverse, or a coset post-scale)
+  int nsplit;               // gather pass: number of digits of rho (passes p < P-1)
+  int split[6];             // their log sizes, s_0 ... s_{P-2}
 };
 
 // ---- reduced-radix butterflies (fz.hpp).  Data stays in the reference's Montgomery form
@@ -155,7 +172,7 @@ __device__ __forceinline__ void zbutterfly(Fz<Z> &u, Fz<Z> &v, const Fz<Z> *w, b
   v = neg ? s0 : s1;
 }
 
-// one workgroup's share of a pass: `blk` indexes the pass's tiles
+// one workgroup's share of a pass: `blk` indexes the pass's tiles (see PassArgs)
 template <class F>
 __device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t blk, int32_t *lds) {
   using Z = typename ZOf<F>::T;
@@ -164,8 +181,9 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t blk, i
   const int elems = S << a.logt;
   const int tid = threadIdx.x;
 
+  // tile: T consecutive columns c0.. of row r (other passes), or T consecutive rho0.. (gather)
   uint64_t r = 0, c0 = 0, rho0 = 0;
-  if (!a.last) {
+  if (!a.gather) {
     const int lg = a.logc - a.logt;
     r = blk >> lg;
     c0 = (blk & ((1ull << lg) - 1)) << a.logt;
@@ -173,20 +191,12 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t blk, i
     rho0 = blk << a.logt;
   }
 
-  // ---- load (coalesced over c), optional coset pre-scale, bit-reversed LDS placement
+  // ---- load (coalesced over the tile's T consecutive positions), coset pre-scale (gather
+  // pass), bit-reversed LDS placement
   for (int idx = tid; idx < elems; idx += kNttThreads) {
     const uint32_t n = idx >> a.logt, c = idx & (T - 1);
-    uint64_t gi;
-    if (!a.last) {
-      gi = (((r << a.s) + n) << a.logc) | (c0 + c);
-    } else {
-      uint64_t rho = rho0 + c, rr = 0;
-      for (int i = 0; i < a.nsplit; i++) {
-        rr = (rr << a.split[i]) | (rho & ((1ull << a.split[i]) - 1));
-        rho >>= a.split[i];
-      }
-      gi = (rr << a.s) + n;
-    }
+    const uint64_t gi = a.gather ? (rho0 + c) + ((uint64_t)n << a.logr)
+                                 : ((((r << a.s) + n) << a.logc) | (c0 + c));
     E x;
     zunpack_g(x, a.in + gi * 8);
     if (a.eta_pre) {
@@ -267,36 +277,45 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t blk, i
     __syncthreads();
   }
 
-  // ---- four-step twiddle (or N^-1 and coset post-scale), canonical store
+  // ---- store: row rev(rho) (gather pass) or in place, times the next pass's twiddle
+  // omega^(c n R) of the offset in that pass's [R][N][C] view; the final pass multiplies by
+  // N^-1 and the coset post-scale (or reduces: the factor is one); canonical words
   E ninv;
 #pragma unroll
   for (int j = 0; j < Z::N; j++) ninv.l[j] = 0;
-  if (a.last) zunpack(ninv, a.ninv);  // internal-form factor: N^-1 R' (inverse) or R' (forward)
+  if (a.fin && a.has_ninv) zunpack(ninv, a.ninv);  // internal-form factor N^-1 R' (or R' with a post-scale)
   for (int idx = tid; idx < elems; idx += kNttThreads) {
     const uint32_t k = idx >> a.logt, c = idx & (T - 1);
     E x;
     zlds_load(x, lds + idx * kZW);
     uint64_t go;
-    bool neg = false;
-    if (!a.last) {
-      const uint64_t col = c0 + c;
-      E w;
-      neg = twiddle_z(w, a.tw, (col * k) << a.logr, a.logn, a.tw_shift, a.inverse);
-      zmul(x, x, w);
-      go = (((r << a.s) + k) << a.logc) | col;
-    } else {
-      go = (rho0 + c) + ((uint64_t)k << a.logr);
-      if (a.has_ninv) {
-        E f = ninv;
-        if (a.eta_post) {
-          E e;
-          eta_pow_z(e, a.eta_post, go);
-          zmul(f, f, e);
-        }
-        zmul(x, x, f);
-      } else {
-        zreduce_small(x, x);  // forward transform: the factor is one
+    if (a.gather) {
+      uint64_t rho = rho0 + c, rr = 0;
+      for (int i = 0; i < a.nsplit; i++) {
+        rr = (rr << a.split[i]) | (rho & ((1ull << a.split[i]) - 1));
+        rho >>= a.split[i];
       }
+      go = (rr << a.s) + k;
+    } else {
+      go = (((r << a.s) + k) << a.logc) | (c0 + c);
+    }
+    bool neg = false;
+    if (!a.fin) {
+      const uint64_t col = go & ((1ull << a.nx_logc) - 1);
+      const uint64_t nn = (go >> a.nx_logc) & ((1ull << a.nx_s) - 1);
+      E w;
+      neg = twiddle_z(w, a.tw, (col * nn) << a.nx_logr, a.logn, a.tw_shift, a.inverse);
+      zmul(x, x, w);
+    } else if (a.has_ninv) {
+      E f = ninv;
+      if (a.eta_post) {
+        E e;
+        eta_pow_z(e, a.eta_post, go);
+        zmul(f, f, e);
+      }
+      zmul(x, x, f);
+    } else {
+      zreduce_small(x, x);  // |x| < 64 p after at most 11 stages
     }
     zcanon(x, x);
     if (neg) zneg_canon(x, x);
@@ -601,15 +620,42 @@ __global__ void eta_tables_kernel(uint32_t *tab, Fe<F> b0, Fe<F> b1, Fe<F> b2) {
   zpack(tab + (size_t)t * 8, a);
 }
 
+// the same tables for many cosets in one launch: bases[3 i + k] are coset i's three bases, its
+// tables at tab + i * 3 * 2048 * 8 words
 template <class F>
-static int launch_eta_tables(uint32_t *tab, int curve, const zkh::Fe &eta, hipStream_t st) {
+__global__ void eta_tables_multi_kernel(uint32_t *tab, const Fe<F> *__restrict__ bases) {
+  using Z = typename ZOf<F>::T;
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;  // grid: exactly 3 * 2048 per coset
+  const uint64_t coset = t / (3 * 2048);
+  const int rem = (int)(t - coset * 3 * 2048), k = rem >> 11, j = rem & 2047;
+  Fe<F> r;
+  fpow_u64(r, bases[coset * 3 + k], (uint64_t)j);
+  Fz<Z> a, c;
+  zunpack(a, r.w);
+  zset_const(c, Z::K_IN);
+  zmul(a, a, c);
+  zcanon(a, a);
+  zpack(tab + t * 8, a);
+}
+
+// eta, eta^2048, eta^(2^22): the bases of a coset's three tables (eta_pow_z)
+template <class F>
+static void eta_bases(int curve, const zkh::Fe &eta, std::vector<Fe<F>> *out) {
   const zkh::FieldDesc &f = fr_desc(curve);
   zkh::Fe b1 = eta, b2;
   for (int i = 0; i < 11; i++) b1 = zkh::sqr(f, b1);
   b2 = b1;
   for (int i = 0; i < 11; i++) b2 = zkh::sqr(f, b2);
-  hipLaunchKernelGGL(eta_tables_kernel<F>, dim3(24), dim3(256), 0, st, tab, to_dev<F>(eta), to_dev<F>(b1),
-                     to_dev<F>(b2));
+  out->push_back(to_dev<F>(eta));
+  out->push_back(to_dev<F>(b1));
+  out->push_back(to_dev<F>(b2));
+}
+
+template <class F>
+static int launch_eta_tables(uint32_t *tab, int curve, const zkh::Fe &eta, hipStream_t st) {
+  std::vector<Fe<F>> b;
+  eta_bases<F>(curve, eta, &b);
+  hipLaunchKernelGGL(eta_tables_kernel<F>, dim3(24), dim3(256), 0, st, tab, b[0], b[1], b[2]);
   ZK_HIP(hipGetLastError());
   return ZK_OK;
 }
@@ -637,44 +683,81 @@ NttPlan ntt_plan(int m, int smax, int tile_log) {
   return p;
 }
 
+// the passes of one transform in execution order (gather pass first, see PassArgs); `buf`
+// receives the gather pass's output and holds the vector in place until the final pass writes
+// d_out (buf == d_out for an out-of-place transform, a scratch vector when d_in == d_out).
+// groups[q] = workgroups of pass q, lds_bytes[q] its dynamic LDS.
 template <class F>
-static int ntt_fast(const NttPlan &plan, const uint32_t *d_in, uint32_t *d_out, uint32_t *d_tmp,
-                    const uint32_t *tw, int tw_shift, int inverse, const uint32_t *eta_pre,
-                    const uint32_t *eta_post, const uint32_t *last_fac, hipStream_t st) {
-  const int m = plan.m;
-  int logr = 0;
-  for (int p = 0; p < plan.npass; p++) {
+static int ntt_passes(const NttPlan &plan, const uint32_t *d_in, uint32_t *buf, uint32_t *d_out, const uint32_t *tw,
+                      int tw_shift, int inverse, const uint32_t *eta_pre, const uint32_t *eta_post,
+                      const uint32_t *last_fac, PassArgs *out, uint64_t *groups, size_t *lds_bytes) {
+  const int m = plan.m, P = plan.npass;
+  if (P > 7) {
+    set_error("ntt: at most 7 passes");
+    return ZK_EINVAL;
+  }
+  int logr_of[8];
+  logr_of[0] = 0;
+  for (int p = 0; p < P; p++) logr_of[p + 1] = logr_of[p] + plan.s[p];
+  for (int q = 0; q < P; q++) {
+    const int p = P - 1 - q;  // the classic pass this one transposes
     PassArgs a;
     memset(&a, 0, sizeof a);
-    const bool last = p == plan.npass - 1;
-    a.in = p == 0 ? d_in : d_tmp;
-    a.out = last ? d_out : d_tmp;
+    a.gather = q == 0;
+    a.fin = p == 0;
+    a.in = q == 0 ? d_in : buf;
+    a.out = a.fin ? d_out : buf;
     a.tw = tw;
     a.tw_shift = tw_shift;
     a.logn = m;
     a.s = plan.s[p];
-    a.logr = logr;
-    a.logc = m - logr - a.s;
-    a.last = last;
+    a.logr = logr_of[p];
+    a.logc = m - a.logr - a.s;
     a.inverse = inverse;
-    a.eta_pre = p == 0 ? eta_pre : nullptr;
-    a.eta_post = last ? eta_post : nullptr;
-    if (last) {
+    a.eta_pre = a.gather ? eta_pre : nullptr;
+    a.eta_post = a.fin ? eta_post : nullptr;
+    if (a.fin) {
       a.has_ninv = inverse || eta_post;
       for (int j = 0; j < 8; j++) a.ninv[j] = last_fac[j];
+    } else {
+      a.nx_s = plan.s[p - 1];
+      a.nx_logr = logr_of[p - 1];
+      a.nx_logc = m - a.nx_logr - a.nx_s;
     }
     int tl = plan.tile_log - a.s;
     if (tl < 0) tl = 0;
-    const int avail = last ? logr : a.logc;
+    const int avail = a.gather ? a.logr : a.logc;
     if (tl > avail) tl = avail;
     a.logt = tl;
-    a.nsplit = p;
-    for (int i = 0; i < p; i++) a.split[i] = plan.s[i];
-    const uint64_t groups = last ? (1ull << (logr - tl)) : ((1ull << logr) << (a.logc - tl));
-    const size_t lds = ((size_t)1 << (a.s + tl)) * kZW * 4;
-    hipLaunchKernelGGL(ntt_pass_kernel<F>, dim3((unsigned)groups), dim3(kNttThreads), lds, st, a);
+    if (a.gather) {
+      a.nsplit = p;
+      for (int i = 0; i < p; i++) a.split[i] = plan.s[i];
+    }
+    out[q] = a;
+    groups[q] = a.gather ? (1ull << (a.logr - tl)) : ((1ull << a.logr) << (a.logc - tl));
+    lds_bytes[q] = ((size_t)1 << (a.s + tl)) * kZW * 4;
+  }
+  return ZK_OK;
+}
+
+// d_tmp: a scratch vector, used only when d_in == d_out and the plan has several passes
+template <class F>
+static int ntt_fast(const NttPlan &plan, const uint32_t *d_in, uint32_t *d_out, uint32_t *d_tmp,
+                    const uint32_t *tw, int tw_shift, int inverse, const uint32_t *eta_pre,
+                    const uint32_t *eta_post, const uint32_t *last_fac, hipStream_t st) {
+  uint32_t *buf = (d_in == d_out && plan.npass > 1) ? d_tmp : d_out;
+  if (!buf) {
+    set_error("ntt: an in-place multi-pass transform needs a scratch vector");
+    return ZK_EINVAL;
+  }
+  PassArgs args[8];
+  uint64_t groups[8];
+  size_t lds[8];
+  ZK_TRY(ntt_passes<F>(plan, d_in, buf, d_out, tw, tw_shift, inverse, eta_pre, eta_post, last_fac, args, groups,
+                       lds));
+  for (int q = 0; q < plan.npass; q++) {
+    hipLaunchKernelGGL(ntt_pass_kernel<F>, dim3((unsigned)groups[q]), dim3(kNttThreads), lds[q], st, args[q]);
     ZK_HIP(hipGetLastError());
-    logr += a.s;
   }
   return ZK_OK;
 }
@@ -768,10 +851,10 @@ static int ntt_device_t(int curve, int m, const uint64_t *gen, const uint64_t *e
   }
   DevBuf tmp, alias, etab;
   StreamSyncGuard drain(st);  // the buffers outlive the queued kernels on every return
-  ZK_TRY(tmp.alloc(N * F::N * 4));
   const zkh::FieldDesc &f = fr_desc(curve);
   const bool ok = ntt_generator_ok(curve, m, gen);
   if (!ok) {
+    ZK_TRY(tmp.alloc(N * F::N * 4));
     // reference network (exact for any generator); coset scaling through the fast kernels is
     // not available there, so shifted transforms scale with a separate elementwise pass
     if (eta) {
@@ -793,6 +876,7 @@ static int ntt_device_t(int curve, int m, const uint64_t *gen, const uint64_t *e
   last_factor<F>(lastf, curve, m, inverse);
   const NttTuning tu = tune ? *tune : ntt_tuning_for(curve, m);
   NttPlan plan = ntt_plan(m, tu.smax, tu.tile_log);
+  if (d_in == d_out && plan.npass > 1) ZK_TRY(tmp.alloc(N * F::N * 4));  // in place: the gather pass needs it
   const uint32_t *ep = (eta && !inverse) ? etab.as<uint32_t>() : nullptr;
   const uint32_t *eo = (eta && inverse) ? etab.as<uint32_t>() : nullptr;
   ZK_TRY(ntt_fast<F>(plan, d_in, d_out, tmp.as<uint32_t>(), tw, tw_shift, inverse, ep, eo,
@@ -826,7 +910,7 @@ struct NttBatchPlan {
   hipStream_t st = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  DevBuf tmp, eta, items, prefix;
+  DevBuf eta, eta_bases, items, prefix;
   uint64_t bytes_moved = 0;  // 2 * 32 * sum(N) (one read + one write of every vector)
   ~NttBatchPlan() {
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -859,8 +943,8 @@ static int ntt_batch_create_t(int curve, int njobs, const NttBatchJob *jobs, Ntt
   const uint32_t *tw = nullptr;
   int shift0 = 0;
   ZK_TRY(get_twiddles<F>(curve, mmax, gw, &tw, &shift0, st));
-  // scratch for multi-pass jobs and coset tables
-  uint64_t tmp_words = 0, ncoset = 0;
+  // coset tables (the passes need no scratch: every job is out of place)
+  uint64_t ncoset = 0;
   std::vector<NttPlan> plans(njobs);
   for (int j = 0; j < njobs; j++) {
     // every item of a round shares the launch's LDS size (the largest item's): keep all tiles
@@ -869,69 +953,48 @@ static int ntt_batch_create_t(int curve, int njobs, const NttBatchJob *jobs, Ntt
     tu.smax = std::min(tu.smax, kBatchTileLog);
     tu.tile_log = std::min(tu.tile_log, kBatchTileLog);
     plans[j] = ntt_plan(jobs[j].m, tu.smax, tu.tile_log);
-    if (plans[j].npass > 1) tmp_words += ((uint64_t)1 << jobs[j].m) * F::N;
     if (jobs[j].eta) ncoset++;
     pl->bytes_moved += 64ull << jobs[j].m;
   }
-  if (tmp_words) ZK_TRY(pl->tmp.alloc(tmp_words * 4));
-  if (ncoset) ZK_TRY(pl->eta.alloc(ncoset * 3 * 2048 * F::N * 4));
+  std::vector<Fe<F>> bases;  // read by an asynchronous copy: lives until the sync below
+  if (ncoset) {
+    // every coset's three power tables in one launch (bases from the host)
+    ZK_TRY(pl->eta.alloc(ncoset * 3 * 2048 * F::N * 4));
+    bases.reserve(ncoset * 3);
+    for (int j = 0; j < njobs; j++)
+      if (jobs[j].eta) eta_bases<F>(curve, zkh::load(f, jobs[j].eta), &bases);
+    ZK_TRY(pl->eta_bases.alloc(bases.size() * sizeof(Fe<F>)));
+    ZK_HIP(hipMemcpyAsync(pl->eta_bases.p, bases.data(), bases.size() * sizeof(Fe<F>), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(eta_tables_multi_kernel<F>, dim3((unsigned)(ncoset * 3 * 2048 / 256)), dim3(256), 0, st,
+                       pl->eta.as<uint32_t>(), pl->eta_bases.as<Fe<F>>());
+  }
   std::vector<std::vector<PassArgs>> rounds;
   std::vector<std::vector<uint32_t>> blocks;
-  uint64_t toff = 0, eoff = 0;
+  uint64_t eoff = 0;
   for (int j = 0; j < njobs; j++) {
     const NttPlan &pj = plans[j];
     const int m = jobs[j].m;
     const bool inverse = jobs[j].inverse;
-    uint32_t *tmpj = nullptr;
-    if (pj.npass > 1) {
-      tmpj = pl->tmp.as<uint32_t>() + toff;
-      toff += ((uint64_t)1 << m) * F::N;
-    }
     const uint32_t *etab = nullptr;
     if (jobs[j].eta) {
-      uint32_t *e = pl->eta.as<uint32_t>() + eoff;
+      etab = pl->eta.as<uint32_t>() + eoff;
       eoff += 3 * 2048 * F::N;
-      ZK_TRY(launch_eta_tables<F>(e, curve, zkh::load(f, jobs[j].eta), st));
-      etab = e;
     }
     uint32_t lastf[8];
     last_factor<F>(lastf, curve, m, inverse);
-    int logr = 0;
-    for (int p = 0; p < pj.npass; p++) {
-      PassArgs a;
-      memset(&a, 0, sizeof a);
-      const bool last = p == pj.npass - 1;
-      a.in = p == 0 ? (const uint32_t *)jobs[j].d_src : tmpj;
-      a.out = last ? (uint32_t *)jobs[j].d_tgt : tmpj;
-      a.tw = tw;
-      a.tw_shift = shift0 + (mmax - m);
-      a.logn = m;
-      a.s = pj.s[p];
-      a.logr = logr;
-      a.logc = m - logr - a.s;
-      a.last = last;
-      a.inverse = inverse;
-      a.eta_pre = (p == 0 && !inverse) ? etab : nullptr;
-      a.eta_post = (last && inverse) ? etab : nullptr;
-      if (last) {
-        a.has_ninv = inverse || a.eta_post;
-        for (int k = 0; k < 8; k++) a.ninv[k] = lastf[k];
+    PassArgs args[8];
+    uint64_t groups[8];
+    size_t lds[8];
+    ZK_TRY(ntt_passes<F>(pj, (const uint32_t *)jobs[j].d_src, (uint32_t *)jobs[j].d_tgt, (uint32_t *)jobs[j].d_tgt,
+                         tw, shift0 + (mmax - m), inverse, (!inverse) ? etab : nullptr, inverse ? etab : nullptr,
+                         lastf, args, groups, lds));
+    for (int q = 0; q < pj.npass; q++) {
+      if ((int)rounds.size() <= q) {
+        rounds.resize(q + 1);
+        blocks.resize(q + 1);
       }
-      int tl = pj.tile_log - a.s;
-      if (tl < 0) tl = 0;
-      const int avail = last ? logr : a.logc;
-      if (tl > avail) tl = avail;
-      a.logt = tl;
-      a.nsplit = p;
-      for (int i = 0; i < p; i++) a.split[i] = pj.s[i];
-      const uint64_t groups = last ? (1ull << (logr - tl)) : ((1ull << logr) << (a.logc - tl));
-      if ((int)rounds.size() <= p) {
-        rounds.resize(p + 1);
-        blocks.resize(p + 1);
-      }
-      rounds[p].push_back(a);
-      blocks[p].push_back((uint32_t)groups);
-      logr += a.s;
+      rounds[q].push_back(args[q]);
+      blocks[q].push_back((uint32_t)groups[q]);
     }
   }
   ZK_HIP(hipGetLastError());

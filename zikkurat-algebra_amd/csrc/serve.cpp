@@ -90,9 +90,10 @@ static void staged_copy(void *dst, const void *src, size_t bytes) {
   for (auto &t : th) t.join();
 }
 
+// values and coefficients (the out-of-place inverse NTT needs no scratch vector) plus the MSM
 static size_t job_footprint(int curve, int logn) {
   const size_t n = (size_t)1 << logn;
-  return 3 * n * 32 + msm_footprint(curve, (int)n, 4, true, nullptr);
+  return 2 * n * 32 + msm_footprint(curve, (int)n, 4, true, nullptr);
 }
 
 static int run_job(Server *s, ServeJob *j, hipStream_t st, void *pinned) {
