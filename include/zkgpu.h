@@ -210,7 +210,9 @@ typedef struct {
   int window;        /* c: signed-digit window width */
   int seg_log;       /* log2 of sorted entries per accumulation thread */
   int policy;        /* ZKG_SCHED_* */
-  int fix_limit;     /* buckets spanning more segments are summed by a pairwise tree (0 = 16) */
+  int fix_limit;     /* buckets spanning more segments are summed by workgroup folds (0 = 16, or
+                        with more than 1024 buckets twice the mean segments per bucket of dense
+                        scalars, clamped to 16..64) */
   int prefetch;      /* accumulation loads the next point early: 0 = default (off), 1 = off, 2 = on */
   int reserved[3];
 } zkg_msm_schedule;

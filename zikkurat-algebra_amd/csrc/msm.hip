@@ -1528,13 +1528,29 @@ struct MsmPipe {
     points_internal = internal;
     sch = s;
     prefetch = user && user->prefetch == 2;
-    limit = (user && user->fix_limit > 0) ? (uint32_t)user->fix_limit : 16u;
     np = sch.glv ? 2u * (uint32_t)n : (uint32_t)n;  // points of the bucket problem
     c = sch.c;
     W = sch.windows;
     logB = c - 1;
     B = 1u << logB;
     nkeys = (uint64_t)W * B;
+    // Buckets spanning more than `limit` segments go to the heavy-bucket kernels; the others
+    // add their partials one after another in the fix kernel (latency ~ one addition each).
+    // The heavy kernels give each bucket one or more workgroups while there are at most
+    // kHeavyBlocks of them, so with at most that many buckets in all, 16 suits every digit
+    // set.  With more buckets and all of them long (narrow windows over many points: a 13-bit
+    // window over 2^21 points puts ~22 segments in each of 20 000 buckets), the workgroups
+    // fold bucket after bucket: 7 ms against ~0.15 ms for the serial fix.  There the limit is
+    // twice the mean segments per bucket of a dense digit set, at most 64 (a longer serial
+    // chain costs more than a workgroup fold).
+    if (user && user->fix_limit > 0) {
+      limit = (uint32_t)user->fix_limit;
+    } else if (nkeys <= kHeavyBlocks) {
+      limit = 16;
+    } else {
+      const uint64_t mean = ((uint64_t)W * np + (nkeys << sch.seg_log) - 1) / (nkeys << sch.seg_log);
+      limit = (uint32_t)std::max<uint64_t>(16, std::min<uint64_t>(2 * mean, 64));
+    }
     if (nkeys >= (1ull << 32) || (uint64_t)W * np >= (1ull << 32) || np >= (1u << 31)) {
       set_error("msm: problem too large for 32-bit sort keys");
       return ZK_EINVAL;
