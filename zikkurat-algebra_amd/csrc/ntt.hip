@@ -79,6 +79,13 @@ template <> struct ZOf<Bn254Fr> { using T = ZBn254Fr; };
 template <> struct ZOf<Bls381Fr> { using T = ZBls381Fr; };
 
 static constexpr int kZW = 9;  // LDS words per element (9 limbs of 29 bits)
+// LDS word offset of tile element e: 9 words each, one pad word after every 32 elements.  The
+// bit-reversed placement of the load and the radix-4 groups of the first stages hit 4 to 32 of
+// the 32 banks' words at once with a plain 9-word stride; the pad word spreads them (at most
+// two-way in every phase for tiles of 2^7 to 2^10 elements).
+__device__ __forceinline__ int lds_slot(int e) { return e * kZW + (e >> 5); }
+// LDS bytes of a tile of 2^log elements
+static inline size_t lds_tile_bytes(int log) { return (((size_t)1 << log) * kZW + (((size_t)1 << log) >> 5)) * 4; }
 // largest LDS tile: 2^12 elements of 36 bytes (144 KiB)
 static constexpr int kMaxTileLog = 12;
 static constexpr int kBatchTileLog = 9;  // largest LDS tile (log2 elements) of a batched job
@@ -112,6 +119,17 @@ __device__ __forceinline__ bool twiddle_z(Fz<Z> &w, const uint32_t *__restrict__
   if (neg) e -= half;
   zunpack_g(w, tw + ((e << tw_shift) * 8));
   return neg;
+}
+
+// Butterfly twiddle omega^(+-e) for 0 <= e < N/2, stored value w and true value w (forward) or
+// -w (inverse, INV: the outputs then swap).  The table holds omega^j for j <= N/2 (entry N/2 is
+// -1), so omega^-e = -omega^(N/2 - e) for every e in [0, N/2): the sign depends only on the
+// direction, and no lane selects between swapped and unswapped outputs.
+template <bool INV, class Z>
+__device__ __forceinline__ void twiddle_bf(Fz<Z> &w, const uint32_t *__restrict__ tw, uint64_t e, int logn,
+                                           int tw_shift) {
+  if (INV) e = (1ull << (logn - 1)) - e;
+  zunpack_g(w, tw + ((e << tw_shift) * 8));
 }
 
 // eta^n (internal form, normalised, < 2p) from three internal-form tables
@@ -157,8 +175,8 @@ __device__ __forceinline__ void zlds_store(int32_t *p, const Fz<Z> &a) {
 // inputs, so its outputs are sums or differences of two normalised values, which zmul accepts
 // against a normalised twiddle; the group's second stage (limbs below 4 * 2^29 < 2^31)
 // normalises before the store.
-template <bool NORM = true, class Z>
-__device__ __forceinline__ void zbutterfly(Fz<Z> &u, Fz<Z> &v, const Fz<Z> *w, bool neg) {
+template <bool NORM, bool NEG, class Z>
+__device__ __forceinline__ void zbutterfly(Fz<Z> &u, Fz<Z> &v, const Fz<Z> *w) {
   Fz<Z> t, s0, s1;
   if (w) zmul(t, v, *w);
   else t = v;
@@ -168,12 +186,12 @@ __device__ __forceinline__ void zbutterfly(Fz<Z> &u, Fz<Z> &v, const Fz<Z> *w, b
     znorm(s0);
     znorm(s1);
   }
-  u = neg ? s1 : s0;
-  v = neg ? s0 : s1;
+  u = NEG ? s1 : s0;
+  v = NEG ? s0 : s1;
 }
 
-// one workgroup's share of a pass: `blk` indexes the pass's tiles (see PassArgs)
-template <class F>
+// one workgroup's share of a pass: `blk` indexes the pass's tiles (see PassArgs); INV = a.inverse
+template <class F, bool INV>
 __device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t blk, int32_t *lds) {
   using Z = typename ZOf<F>::T;
   using E = Fz<Z>;
@@ -204,7 +222,7 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t blk, i
       eta_pow_z(e, a.eta_pre, gi);
       zmul(x, x, e);
     }
-    zlds_store(lds + ((bitrev(n, a.s) << a.logt) | c) * kZW, x);
+    zlds_store(lds + lds_slot((bitrev(n, a.s) << a.logt) | c), x);
   }
   __syncthreads();
 
@@ -223,32 +241,32 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t blk, i
         const int j = g & (h - 1);
         const int blk = g >> lh;
         const int n0 = (blk << (lh + 2)) | j;
-        const int stride = (h << a.logt) * kZW;
-        const int p0 = ((n0 << a.logt) | c) * kZW;
+        const int e0 = (n0 << a.logt) | c, es = h << a.logt;
+        const int p0 = lds_slot(e0), p1 = lds_slot(e0 + es), p2 = lds_slot(e0 + 2 * es), p3 = lds_slot(e0 + 3 * es);
         E x0, x1, x2, x3;
         zlds_load(x0, lds + p0);
-        zlds_load(x1, lds + p0 + stride);
-        zlds_load(x2, lds + p0 + 2 * stride);
-        zlds_load(x3, lds + p0 + 3 * stride);
+        zlds_load(x1, lds + p1);
+        zlds_load(x2, lds + p2);
+        zlds_load(x3, lds + p3);
         if (lh > 0) {
           E w1;
-          const bool n1 = twiddle_z(w1, a.tw, (uint64_t)j << (a.logn - lh - 1), a.logn, a.tw_shift, a.inverse);
-          zbutterfly<false>(x0, x1, &w1, n1);
-          zbutterfly<false>(x2, x3, &w1, n1);
+          twiddle_bf<INV>(w1, a.tw, (uint64_t)j << (a.logn - lh - 1), a.logn, a.tw_shift);
+          zbutterfly<false, INV>(x0, x1, &w1);
+          zbutterfly<false, INV>(x2, x3, &w1);
         } else {
-          zbutterfly<false>(x0, x1, (const E *)nullptr, false);
-          zbutterfly<false>(x2, x3, (const E *)nullptr, false);
+          zbutterfly<false, false>(x0, x1, (const E *)nullptr);
+          zbutterfly<false, false>(x2, x3, (const E *)nullptr);
         }
         E w2;
-        bool n2 = twiddle_z(w2, a.tw, (uint64_t)j << (a.logn - lh - 2), a.logn, a.tw_shift, a.inverse);
-        if (lh > 0 || j > 0) zbutterfly(x0, x2, &w2, n2);
-        else zbutterfly(x0, x2, (const E *)nullptr, false);
-        n2 = twiddle_z(w2, a.tw, (uint64_t)(j + h) << (a.logn - lh - 2), a.logn, a.tw_shift, a.inverse);
-        zbutterfly(x1, x3, &w2, n2);
+        twiddle_bf<INV>(w2, a.tw, (uint64_t)j << (a.logn - lh - 2), a.logn, a.tw_shift);
+        if (lh > 0 || j > 0) zbutterfly<true, INV>(x0, x2, &w2);
+        else zbutterfly<true, false>(x0, x2, (const E *)nullptr);
+        twiddle_bf<INV>(w2, a.tw, (uint64_t)(j + h) << (a.logn - lh - 2), a.logn, a.tw_shift);
+        zbutterfly<true, INV>(x1, x3, &w2);
         zlds_store(lds + p0, x0);
-        zlds_store(lds + p0 + stride, x1);
-        zlds_store(lds + p0 + 2 * stride, x2);
-        zlds_store(lds + p0 + 3 * stride, x3);
+        zlds_store(lds + p1, x1);
+        zlds_store(lds + p2, x2);
+        zlds_store(lds + p3, x3);
       }
       lh += 2;
     } else {
@@ -257,17 +275,17 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t blk, i
         const int bf = q >> a.logt;
         const int j = bf & (h - 1);
         const int blk = bf >> lh;
-        const int pu = ((((blk << (lh + 1)) | j) << a.logt) | c) * kZW;
-        const int pv = pu + ((h << a.logt) * kZW);
+        const int eu = (((blk << (lh + 1)) | j) << a.logt) | c;
+        const int pu = lds_slot(eu), pv = lds_slot(eu + (h << a.logt));
         E u, v;
         zlds_load(u, lds + pu);
         zlds_load(v, lds + pv);
         if (lh > 0) {
           E w;
-          const bool neg = twiddle_z(w, a.tw, (uint64_t)j << (a.logn - lh - 1), a.logn, a.tw_shift, a.inverse);
-          zbutterfly(u, v, &w, neg);
+          twiddle_bf<INV>(w, a.tw, (uint64_t)j << (a.logn - lh - 1), a.logn, a.tw_shift);
+          zbutterfly<true, INV>(u, v, &w);
         } else {
-          zbutterfly(u, v, (const E *)nullptr, false);
+          zbutterfly<true, false>(u, v, (const E *)nullptr);
         }
         zlds_store(lds + pu, u);
         zlds_store(lds + pv, v);
@@ -287,7 +305,7 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t blk, i
   for (int idx = tid; idx < elems; idx += kNttThreads) {
     const uint32_t k = idx >> a.logt, c = idx & (T - 1);
     E x;
-    zlds_load(x, lds + idx * kZW);
+    zlds_load(x, lds + lds_slot(idx));
     uint64_t go;
     if (a.gather) {
       uint64_t rho = rho0 + c, rr = 0;
@@ -326,7 +344,8 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t blk, i
 template <class F>
 __global__ __launch_bounds__(kNttThreads) void ntt_pass_kernel(PassArgs a) {
   extern __shared__ uint4 lds_raw[];
-  ntt_pass_body<F>(a, blockIdx.x, reinterpret_cast<int32_t *>(lds_raw));
+  if (a.inverse) ntt_pass_body<F, true>(a, blockIdx.x, reinterpret_cast<int32_t *>(lds_raw));
+  else ntt_pass_body<F, false>(a, blockIdx.x, reinterpret_cast<int32_t *>(lds_raw));
 }
 
 // batched pass: item i owns blocks [prefix[i], prefix[i+1]) of the launch
@@ -352,7 +371,8 @@ __global__ __launch_bounds__(kNttThreads) void ntt_batch_pass_kernel(const PassA
 #pragma unroll
   for (int i = 0; i < (int)(sizeof(PassArgs) / 4); i++) dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
   const uint32_t first = __builtin_amdgcn_readfirstlane(prefix[lo]);
-  ntt_pass_body<F>(a, b - first, reinterpret_cast<int32_t *>(lds_raw));
+  if (a.inverse) ntt_pass_body<F, true>(a, b - first, reinterpret_cast<int32_t *>(lds_raw));
+  else ntt_pass_body<F, false>(a, b - first, reinterpret_cast<int32_t *>(lds_raw));
 }
 
 // ---------------------------------------------------------------------------------------
@@ -555,7 +575,7 @@ static int launch_twiddles(uint32_t *tab, uint32_t *lo, uint32_t *hi, uint64_t c
 
 template <class F>
 static int build_twiddles(int curve, int m, const zkh::Fe &g, DevBuf &buf, hipStream_t st) {
-  const uint64_t cnt = m >= 1 ? (1ull << (m - 1)) : 1;
+  const uint64_t cnt = (m >= 1 ? (1ull << (m - 1)) : 1) + 1;  // omega^j, j <= N/2
   ZK_TRY(buf.alloc(cnt * F::N * 4));
   DevBuf lo, hi;
   StreamSyncGuard drain(st);  // lo/hi outlive the queued table kernels on every return
@@ -735,7 +755,7 @@ static int ntt_passes(const NttPlan &plan, const uint32_t *d_in, uint32_t *buf, 
     }
     out[q] = a;
     groups[q] = a.gather ? (1ull << (a.logr - tl)) : ((1ull << a.logr) << (a.logc - tl));
-    lds_bytes[q] = ((size_t)1 << (a.s + tl)) * kZW * 4;
+    lds_bytes[q] = lds_tile_bytes(a.s + tl);
   }
   return ZK_OK;
 }
@@ -1016,7 +1036,7 @@ static int ntt_batch_create_t(int curve, int njobs, const NttBatchJob *jobs, Ntt
       h_items.push_back(rounds[r][i]);
       h_prefix.push_back(acc);
       acc += blocks[r][i];
-      const size_t l = ((size_t)1 << (rounds[r][i].s + rounds[r][i].logt)) * kZW * 4;
+      const size_t l = lds_tile_bytes(rounds[r][i].s + rounds[r][i].logt);
       if (l > lds) lds = l;
     }
     h_prefix.push_back(acc);
@@ -1178,10 +1198,10 @@ int ntt_device(int curve, int m, const uint64_t *gen, const uint64_t *eta, int f
 static constexpr int kNoAllocMaxLog = 18;
 static constexpr size_t kNaWords = 8;  // u32 words per stored Fr element
 static constexpr size_t kNaTmp = 0;                                                  // 2^18 elements
-static constexpr size_t kNaTw = kNaTmp + (kNaWords << kNoAllocMaxLog);               // 2^17 twiddles
-static constexpr size_t kNaLo = kNaTw + (kNaWords << (kNoAllocMaxLog - 1));          // 2048 powers
-static constexpr size_t kNaHi = kNaLo + kNaWords * 2048;                             // 64 powers
-static constexpr size_t kNaEta = kNaHi + kNaWords * ((1u << (kNoAllocMaxLog - 1)) / 2048);  // 3 x 2048
+static constexpr size_t kNaTw = kNaTmp + (kNaWords << kNoAllocMaxLog);               // 2^17 + 1 twiddles
+static constexpr size_t kNaLo = kNaTw + kNaWords * ((1u << (kNoAllocMaxLog - 1)) + 8);   // 2048 powers
+static constexpr size_t kNaHi = kNaLo + kNaWords * 2048;                             // 65 powers
+static constexpr size_t kNaEta = kNaHi + kNaWords * ((1u << (kNoAllocMaxLog - 1)) / 2048 + 8);  // 3 x 2048
 static constexpr size_t kNaTotal = kNaEta + kNaWords * 3 * 2048;
 static_assert(kNaTw % 64 == 0 && kNaLo % 64 == 0 && kNaHi % 64 == 0 && kNaEta % 64 == 0,
               "scratch regions stay 256-byte aligned");
@@ -1226,7 +1246,7 @@ static int ntt_noalloc_t(int curve, int m, const uint64_t *gen, const uint64_t *
   const zkh::FieldDesc &f = fr_desc(curve);
   const uint32_t *tw = nullptr;
   if (m >= 1) {
-    ZK_TRY(launch_twiddles<F>(scr + kNaTw, scr + kNaLo, scr + kNaHi, 1ull << (m - 1), zkh::load(f, gen), st));
+    ZK_TRY(launch_twiddles<F>(scr + kNaTw, scr + kNaLo, scr + kNaHi, (1ull << (m - 1)) + 1, zkh::load(f, gen), st));
     tw = scr + kNaTw;
   }
   if (eta) ZK_TRY(launch_eta_tables<F>(scr + kNaEta, curve, zkh::load(f, eta), st));
