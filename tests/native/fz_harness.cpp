@@ -74,6 +74,10 @@ static void run_curve(int op, int n, const int32_t *a, const int32_t *b, int32_t
     } else if (op == 1) {
       memcpy(&q, b + k * 4 * N, sizeof q);
       zxyzz_add(r, p, q);
+    } else if (op == 3) {  // scalar multiplication: b holds 8-word standard-form scalars
+      uint32_t sc[8];
+      memcpy(sc, b + k * 8, sizeof sc);
+      zxyzz_scalar_mul(r, p, sc);
     } else {
       zxyzz_dbl(r, p);
     }

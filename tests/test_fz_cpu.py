@@ -309,3 +309,24 @@ def test_xyzz_formulas(h, C):
         h.fzh_curve(C.id, 2, n, A.ctypes.data_as(i32p), None, out.ctypes.data_as(i32p))
         for i in range(n):
             assert decode_xyzz(C, out[i]) == C.add(P[i], P[i]), ("dbl", lazy, i)
+
+
+@pytest.mark.parametrize("C", [BN, BLS], ids=["bn254", "bls12-381"])
+def test_xyzz_scalar_mul(h, C):
+    """zxyzz_scalar_mul (the G1 FFT's BLS12-381 path: double-and-add over the scalar's bits)
+    equals k P for scalars 0, 1, 2, r - 1, 2^255 + 5 and random ones, on points given in lazy
+    XYZZ form (ZZ, ZZZ != 1) and on infinity."""
+    rng = splitmix64(950 + C.id)
+    ks = [random_below(rng, C.r, 4) for _ in range(10)]
+    P = [C.mul(k, C.g) for k in ks]
+    S = [0, 1, 2, C.r - 1, (1 << 255) + 5] + [random_below(rng, 1 << 256, 4) for _ in range(5)]
+    A = xyzz_rows(C, P, rng, True)
+    A[-1, :] = 0  # the last point: infinity (all-zero XYZZ)
+    P[-1] = None
+    Sw = arr([[(s >> (32 * j)) & 0xFFFFFFFF for j in range(8)] for s in S]).astype(np.uint32).view(np.int32)
+    out = np.zeros_like(A)
+    h.fzh_curve(C.id, 3, len(P), A.ctypes.data_as(i32p), np.ascontiguousarray(Sw).ctypes.data_as(i32p),
+                out.ctypes.data_as(i32p))
+    for i, (pt, k) in enumerate(zip(P, S)):
+        exp = None if pt is None else C.mul(k % C.r, pt)
+        assert decode_xyzz(C, out[i]) == exp, i
