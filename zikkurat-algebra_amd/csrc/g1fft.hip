@@ -114,15 +114,18 @@ __global__ __launch_bounds__(kG1Threads) void g1_proj_out_kernel(const int32_t *
   zto_ext(o + 2 * LW, one);
 }
 
-// tab[j] = g^j in standard form (8 u32 words), j < cnt
+// tab[j] = g^j in standard form (8 u32 words), j < cnt; with tabm, also the Montgomery form
+// there (the grouped stages' path products) from the same power
 template <class C>
-__global__ __launch_bounds__(kG1Threads) void g1_twiddle_kernel(uint32_t *__restrict__ tab, Fe<typename C::Fr> g,
+__global__ __launch_bounds__(kG1Threads) void g1_twiddle_kernel(uint32_t *__restrict__ tab,
+                                                               uint32_t *__restrict__ tabm, Fe<typename C::Fr> g,
                                                                uint32_t cnt) {
   using FR = typename C::Fr;
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= cnt) return;
   Fe<FR> r;
   fpow_u64(r, g, j);
+  if (tabm) fstore(tabm + (size_t)j * FR::N, r);
   fto_std(r, r);
   fstore(tab + (size_t)j * FR::N, r);
 }
@@ -404,17 +407,6 @@ __global__ __launch_bounds__(kG1Threads) void g1_group_dif_sum_kernel(int32_t *_
   }
 }
 
-// tab[j] = g^j in Montgomery form, j < cnt
-template <class C>
-__global__ __launch_bounds__(kG1Threads) void g1_twiddle_mont_kernel(uint32_t *__restrict__ tab,
-                                                                    Fe<typename C::Fr> g, uint32_t cnt) {
-  using FR = typename C::Fr;
-  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= cnt) return;
-  Fe<FR> r;
-  fpow_u64(r, g, j);
-  fstore(tab + (size_t)j * FR::N, r);
-}
 
 // Stage plan: group sizes r summing to m, each the largest r <= 4 (and <= what remains) whose
 // launch, ngroups (K_r + 1) threads, stays within about one round of resident threads; r = 1
@@ -471,9 +463,9 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
   };
   const dim3 blk(kG1Threads), gN((N + kG1Threads - 1) / kG1Threads), gH((halfN + kG1Threads - 1) / kG1Threads);
   // Montgomery form of g for the path products (to_fr gives the words of the Montgomery value)
-  if (halfN) hipLaunchKernelGGL(g1_twiddle_kernel<C>, gH, blk, 0, st, tw.as<uint32_t>(), to_fr(g), halfN);
-  if (halfN && grouped)
-    hipLaunchKernelGGL(g1_twiddle_mont_kernel<C>, gH, blk, 0, st, twm.as<uint32_t>(), to_fr(g), halfN);
+  if (halfN)
+    hipLaunchKernelGGL(g1_twiddle_kernel<C>, gH, blk, 0, st, tw.as<uint32_t>(),
+                       grouped ? twm.as<uint32_t>() : (uint32_t *)nullptr, to_fr(g), halfN);
   if (jac)
     hipLaunchKernelGGL((g1_proj_in_kernel<C, true>), gN, blk, 0, st, (const uint32_t *)d_src, N, m, inverse ? 0 : 1,
                        pts.as<int32_t>());
