@@ -4,6 +4,7 @@
 #include <string.h>
 
 #include "../../zikkurat-algebra_amd/csrc/curvez.hpp"
+#include "../../zikkurat-algebra_amd/csrc/glv.hpp"
 
 using namespace zk;
 
@@ -78,6 +79,24 @@ static void run_curve(int op, int n, const int32_t *a, const int32_t *b, int32_t
       uint32_t sc[8];
       memcpy(sc, b + k * 8, sizeof sc);
       zxyzz_scalar_mul(r, p, sc);
+    } else if (op == 4 || op == 5) {  // BN254 GLV forms (scalars below r): 1-bit joint, 2-bit joint windows
+      uint32_t sc[8];
+      memcpy(sc, b + k * 8, sizeof sc);
+      if constexpr (C::id == 0) {
+        if (op == 4) {
+          zxyzz_scalar_mul_glv(r, p, sc);
+        } else {
+          int32_t tab[kGlvW2Entries * ZLayout<C>::XYZZ];
+          zxyzz_scalar_mul_glv_w2(r, p, sc, tab, 0, 1);
+        }
+      } else {
+        zxyzz_set_inf(r);
+      }
+    } else if (op == 6) {  // 2-bit windows on k over a 3-entry host table (stride 1)
+      uint32_t sc[8];
+      memcpy(sc, b + k * 8, sizeof sc);
+      int32_t tab[3 * ZLayout<C>::XYZZ];
+      zxyzz_scalar_mul_w2(r, p, sc, tab, 0, 1);
     } else {
       zxyzz_dbl(r, p);
     }

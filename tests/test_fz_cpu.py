@@ -312,10 +312,12 @@ def test_xyzz_formulas(h, C):
 
 
 @pytest.mark.parametrize("C", [BN, BLS], ids=["bn254", "bls12-381"])
-def test_xyzz_scalar_mul(h, C):
-    """zxyzz_scalar_mul (the G1 FFT's BLS12-381 path: double-and-add over the scalar's bits)
-    equals k P for scalars 0, 1, 2, r - 1, 2^255 + 5 and random ones, on points given in lazy
-    XYZZ form (ZZ, ZZZ != 1) and on infinity."""
+@pytest.mark.parametrize("op", [3, 6], ids=["binary", "window2"])
+def test_xyzz_scalar_mul(h, C, op):
+    """zxyzz_scalar_mul (double-and-add over the scalar's bits) and zxyzz_scalar_mul_w2 (2-bit
+    windows over a (P, 2P, 3P) table: the G1 FFT's BLS12-381 butterflies) equal k P for scalars
+    0, 1, 2, r - 1, 2^255 + 5 and random ones, on points given in lazy XYZZ form (ZZ, ZZZ != 1)
+    and on infinity."""
     rng = splitmix64(950 + C.id)
     ks = [random_below(rng, C.r, 4) for _ in range(10)]
     P = [C.mul(k, C.g) for k in ks]
@@ -325,8 +327,39 @@ def test_xyzz_scalar_mul(h, C):
     P[-1] = None
     Sw = arr([[(s >> (32 * j)) & 0xFFFFFFFF for j in range(8)] for s in S]).astype(np.uint32).view(np.int32)
     out = np.zeros_like(A)
-    h.fzh_curve(C.id, 3, len(P), A.ctypes.data_as(i32p), np.ascontiguousarray(Sw).ctypes.data_as(i32p),
+    h.fzh_curve(C.id, op, len(P), A.ctypes.data_as(i32p), np.ascontiguousarray(Sw).ctypes.data_as(i32p),
                 out.ctypes.data_as(i32p))
     for i, (pt, k) in enumerate(zip(P, S)):
         exp = None if pt is None else C.mul(k % C.r, pt)
         assert decode_xyzz(C, out[i]) == exp, i
+
+
+def test_glv_scalar_mul_forms(h):
+    """The two GLV forms of BN254 scalar multiplication (csrc/glv.hpp), the G1 FFT's butterflies:
+    the 1-bit joint double-and-add and the 2-bit joint windows over a per-thread point table
+    (here one host table, stride 1) both equal k P, for scalars at the split's edges (0, 1, small
+    values whose top window is partial, r - 1, the cube roots of unity lambda, lambda^2 mod r and
+    their neighbours, where one half of the split is 0 or +-1) and random scalars below r, on
+    points in lazy XYZZ form and on infinity."""
+    C = BN
+    rng = splitmix64(960)
+    lam = pow(5, (C.r - 1) // 3, C.r)
+    assert lam != 1 and pow(lam, 3, C.r) == 1
+    lams = [lam, lam * lam % C.r]
+    S = [0, 1, 2, 3, 4, 5, 15, 16, 17, 63, 64, C.r - 1, C.r - 2, (1 << 127) - 1, 1 << 127]
+    for l in lams:
+        S += [l, l + 1, l - 1, C.r - l, 2 * l % C.r]
+    S += [random_below(rng, C.r, 4) for _ in range(40 - len(S))]
+    ks = [random_below(rng, C.r, 4) for _ in S]
+    P = [C.mul(k, C.g) for k in ks]
+    A = xyzz_rows(C, P, rng, True)
+    A[-1, :] = 0  # the last point: infinity
+    P[-1] = None
+    Sw = np.ascontiguousarray(arr([[(s >> (32 * j)) & 0xFFFFFFFF for j in range(8)] for s in S])
+                              .astype(np.uint32).view(np.int32))
+    for op in (4, 5):
+        out = np.zeros_like(A)
+        h.fzh_curve(C.id, op, len(P), A.ctypes.data_as(i32p), Sw.ctypes.data_as(i32p), out.ctypes.data_as(i32p))
+        for i, (pt, k) in enumerate(zip(P, S)):
+            exp = None if pt is None else C.mul(k, pt)
+            assert decode_xyzz(C, out[i]) == exp, (op, i, k)

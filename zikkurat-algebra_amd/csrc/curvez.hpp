@@ -241,6 +241,68 @@ ZK_HD void zxyzz_scalar_mul(XyzzZ<C> &r, const XyzzZ<C> &p, const uint32_t (&k)[
   }
 }
 
+// Per-thread point tables of the windowed scalar multiplications, structure of arrays: word w
+// of entry e of thread t at tab[(e * PW + w) * stride + t] (stride >= the threads sharing the
+// buffer), so lanes that pick the same entry read consecutive words.
+template <class C>
+ZK_HD void zxyzz_tab_store(int32_t *__restrict__ tab, int e, uint64_t t, uint64_t stride,
+                                              const XyzzZ<C> &a) {
+  constexpr int N = ZP<C>::T::N, PW = ZLayout<C>::XYZZ;
+  int32_t *q = tab + (uint64_t)e * PW * stride + t;
+#pragma unroll
+  for (int l = 0; l < N; l++) {
+    q[(uint64_t)l * stride] = a.x.l[l];
+    q[(uint64_t)(N + l) * stride] = a.y.l[l];
+    q[(uint64_t)(2 * N + l) * stride] = a.zz.l[l];
+    q[(uint64_t)(3 * N + l) * stride] = a.zzz.l[l];
+  }
+}
+template <class C>
+ZK_HD void zxyzz_tab_load(XyzzZ<C> &a, const int32_t *__restrict__ tab, int e, uint64_t t,
+                                             uint64_t stride) {
+  constexpr int N = ZP<C>::T::N, PW = ZLayout<C>::XYZZ;
+  const int32_t *q = tab + (uint64_t)e * PW * stride + t;
+#pragma unroll
+  for (int l = 0; l < N; l++) {
+    a.x.l[l] = q[(uint64_t)l * stride];
+    a.y.l[l] = q[(uint64_t)(N + l) * stride];
+    a.zz.l[l] = q[(uint64_t)(2 * N + l) * stride];
+    a.zzz.l[l] = q[(uint64_t)(3 * N + l) * stride];
+  }
+}
+
+// The same product by 2-bit windows over a 3-entry table (p, 2p, 3p) of thread t: a step is
+// two doublings and one addition, ~127 additions for a 255-bit scalar where the binary form,
+// whose wavefronts add at almost every bit (one lane with a set bit suffices), pays ~255.
+template <class C>
+ZK_HD void zxyzz_scalar_mul_w2(XyzzZ<C> &r, const XyzzZ<C> &p, const uint32_t (&k)[8], int32_t *__restrict__ tab,
+                               uint64_t t, uint64_t stride) {
+  {
+    XyzzZ<C> e2, e3;
+    zxyzz_dbl(e2, p);
+    zxyzz_add(e3, e2, p);
+    zxyzz_tab_store<C>(tab, 0, t, stride, p);
+    zxyzz_tab_store<C>(tab, 1, t, stride, e2);
+    zxyzz_tab_store<C>(tab, 2, t, stride, e3);
+  }
+  auto digit = [&](int s) -> uint32_t { return (k[s >> 4] >> (2 * (s & 15))) & 3u; };
+  int top = 127;
+  while (top >= 0 && digit(top) == 0) top--;
+  if (top < 0) {
+    zxyzz_set_inf(r);
+    return;
+  }
+  zxyzz_tab_load<C>(r, tab, (int)digit(top) - 1, t, stride);
+  for (int s = top - 1; s >= 0; s--) {
+    const uint32_t d = digit(s);
+    XyzzZ<C> e, u;
+    if (d) zxyzz_tab_load<C>(e, tab, (int)d - 1, t, stride);  // in flight during the doublings
+    zxyzz_dbl(u, r);
+    zxyzz_dbl(r, u);
+    if (d) zxyzz_add(r, r, e);
+  }
+}
+
 // ---- device memory
 
 template <class C>

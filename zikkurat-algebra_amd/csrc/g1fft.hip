@@ -138,6 +138,20 @@ __device__ __forceinline__ void scalar_mul(XyzzZ<C> &r, const XyzzZ<C> &p, const
   else zxyzz_scalar_mul(r, p, k);
 }
 
+// the same in the grouped kernels, where thread t of a launch owns g1_tab_entries<C>() point-table
+// entries of tab (stride = the table's threads): BN254 by 2-bit joint GLV windows, BLS12-381 by
+// 2-bit windows on k
+template <class C>
+__host__ __device__ constexpr int g1_tab_entries() {
+  return C::id == 0 ? kGlvW2Entries : 3;
+}
+template <class C>
+__device__ __forceinline__ void group_scalar_mul(XyzzZ<C> &r, const XyzzZ<C> &p, const uint32_t (&k)[8],
+                                                 int32_t *__restrict__ tab, uint64_t t, uint64_t stride) {
+  if constexpr (C::id == 0) zxyzz_scalar_mul_glv_w2(r, p, k, tab, t, stride);
+  else zxyzz_scalar_mul_w2(r, p, k, tab, t, stride);
+}
+
 // one radix-2 stage over blocks of 2^s points (DIT: u + w v, u - w v; DIF: u + v, w (u - v))
 template <class C, bool DIF>
 __global__ __launch_bounds__(kG1Threads) void g1_fft_stage_kernel(int32_t *__restrict__ pts,
@@ -278,7 +292,8 @@ template <class C>
 __global__ __launch_bounds__(kG1Threads) void g1_group_dit_mul_kernel(const int32_t *__restrict__ pts,
                                                                      int32_t *__restrict__ scr,
                                                                      const uint32_t *__restrict__ twm, int m, int s0,
-                                                                     int r, uint32_t ngroups) {
+                                                                     int r, uint32_t ngroups, int32_t *__restrict__ tab,
+                                                                     uint64_t stride) {
   using FR = typename C::Fr;
   constexpr int PW = ZLayout<C>::XYZZ;
   const uint32_t K = g1_kcount(r);
@@ -300,7 +315,7 @@ __global__ __launch_bounds__(kG1Threads) void g1_group_dit_mul_kernel(const int3
   g1_path_coeff<C>(c, twm, m, s0, p0, q, L, h, false, c);
   zxyzz_load(x, pts + (b + ((uint64_t)q << s0)) * PW);
   XyzzZ<C> y;
-  scalar_mul<C>(y, x, c.w);
+  group_scalar_mul<C>(y, x, c.w, tab, tid, stride);
   zxyzz_store(scr + tid * PW, y);
 }
 
@@ -343,7 +358,8 @@ __global__ __launch_bounds__(kG1Threads) void g1_group_dif_mul_kernel(const int3
                                                                      int32_t *__restrict__ scr,
                                                                      const uint32_t *__restrict__ twm, int m, int s0,
                                                                      int r, uint32_t ngroups, int has_fin,
-                                                                     Fe<typename C::Fr> fin) {
+                                                                     Fe<typename C::Fr> fin, int32_t *__restrict__ tab,
+                                                                     uint64_t stride) {
   using FR = typename C::Fr;
   constexpr int PW = ZLayout<C>::XYZZ;
   const uint32_t K = g1_kcount(r), Q = 1u << r;
@@ -376,7 +392,7 @@ __global__ __launch_bounds__(kG1Threads) void g1_group_dif_mul_kernel(const int3
     return;
   }
   XyzzZ<C> y;
-  scalar_mul<C>(y, s, c.w);
+  group_scalar_mul<C>(y, s, c.w, tab, tid, stride);
   zxyzz_store(scr + tid * PW, y);
 }
 
@@ -437,7 +453,7 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
     }
     g = zkh::inv(fr, g);
   }
-  DevBuf pts, tw, twm, scr;
+  DevBuf pts, tw, twm, scr, tab;
   StreamSyncGuard drain(st);  // the buffers outlive the queued kernels on every return
   ZK_TRY(pts.alloc((size_t)N * ZW * 4));
   ZK_TRY(tw.alloc((size_t)(halfN ? halfN : 1) * 32));
@@ -452,9 +468,13 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
       grouped = true;
       scr_pts = std::max<uint64_t>(scr_pts, (uint64_t)(N >> r) * (g1_kcount(r) + 1));
     }
+  // grouped launches: g1_tab_entries table points per thread (BN254: 15, 47.5 MB at 2^12 and at
+  // most ~0.43 GB where the stage plan's thread cap binds; BLS12-381: 3)
+  const uint64_t tab_stride = scr_pts;
   if (grouped) {
     ZK_TRY(twm.alloc((size_t)(halfN ? halfN : 1) * 32));
     ZK_TRY(scr.alloc(scr_pts * ZW * 4));
+    ZK_TRY(tab.alloc(tab_stride * g1_tab_entries<C>() * ZW * 4));
   }
   auto to_fr = [&](const zkh::Fe &x) {
     Fe<FR> d;
@@ -483,7 +503,8 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
         const uint32_t ng = N >> r;
         hipLaunchKernelGGL(g1_group_dit_mul_kernel<C>, grid((uint64_t)ng * (g1_kcount(r) + 1)), blk,
                            0, st,
-                           (const int32_t *)pts.as<int32_t>(), scr.as<int32_t>(), twm.as<uint32_t>(), m, s0, r, ng);
+                           (const int32_t *)pts.as<int32_t>(), scr.as<int32_t>(), twm.as<uint32_t>(), m, s0, r, ng,
+                           tab.as<int32_t>(), tab_stride);
         hipLaunchKernelGGL(g1_group_dit_sum_kernel<C>, grid(((uint64_t)ng << r) * 8), blk, 0, st, pts.as<int32_t>(),
                            (const int32_t *)scr.as<int32_t>(), s0, r, ng);
       }
@@ -507,7 +528,7 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
         hipLaunchKernelGGL(g1_group_dif_mul_kernel<C>, grid((uint64_t)ng * (g1_kcount(r) + 1)), blk,
                            0, st,
                            (const int32_t *)pts.as<int32_t>(), scr.as<int32_t>(), twm.as<uint32_t>(), m, s0, r, ng,
-                           fin, to_fr(ninv_m));
+                           fin, to_fr(ninv_m), tab.as<int32_t>(), tab_stride);
         hipLaunchKernelGGL(g1_group_dif_sum_kernel<C>, grid(((uint64_t)ng << r) * 8), blk, 0, st, pts.as<int32_t>(),
                            (const int32_t *)scr.as<int32_t>(), s0, r, ng);
       }
@@ -520,7 +541,7 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
   hipLaunchKernelGGL(g1_proj_out_kernel<C>, gN, blk, 0, st, pts.as<int32_t>(), N, m, inverse ? 1 : 0,
                      (uint32_t *)d_dst);
   ZK_HIP(hipGetLastError());
-  ZK_HIP(hipStreamSynchronize(st));  // pts / tw / twm / scr are released on return
+  ZK_HIP(hipStreamSynchronize(st));  // pts / tw / twm / scr / tab are released on return
   drain.release();
   return ZK_OK;
 }

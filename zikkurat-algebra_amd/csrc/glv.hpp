@@ -29,7 +29,7 @@ struct Bn254Glv {
 
 // (k G) >> 256 for k < 2^256 and a constant G of NG 64-bit limbs, low 128 bits of the quotient
 template <int NG>
-__device__ __forceinline__ unsigned __int128 mul_shift256(const uint64_t k[4], const uint64_t (&g)[NG]) {
+ZK_HD unsigned __int128 mul_shift256(const uint64_t k[4], const uint64_t (&g)[NG]) {
   using u128 = unsigned __int128;
   uint64_t prod[4 + NG] = {};
 #pragma unroll
@@ -47,8 +47,7 @@ __device__ __forceinline__ unsigned __int128 mul_shift256(const uint64_t k[4], c
 }
 
 // k < r (4 limbs) -> |k1|, |k2| as two 64-bit limbs each, with their signs
-__device__ __forceinline__ void glv_split(const uint64_t k[4], uint64_t m1[2], bool &neg1, uint64_t m2[2],
-                                          bool &neg2) {
+ZK_HD void glv_split(const uint64_t k[4], uint64_t m1[2], bool &neg1, uint64_t m2[2], bool &neg2) {
   using G = Bn254Glv;
   using u128 = unsigned __int128;
   const u128 c1 = mul_shift256<3>(k, G::G1);
@@ -72,7 +71,7 @@ __device__ __forceinline__ void glv_split(const uint64_t k[4], uint64_t m1[2], b
 // signs of k1, k2 folded into them; phi(X, Y, ZZ, ZZZ) = (beta X, Y, ZZ, ZZZ) since x = X / ZZ.
 // About 127 doublings and 96 additions against 254 and 127 for double-and-add on k.
 template <class C>
-__device__ __forceinline__ void zxyzz_scalar_mul_glv(XyzzZ<C> &r, const XyzzZ<C> &p, const uint32_t (&k)[8]) {
+ZK_HD void zxyzz_scalar_mul_glv(XyzzZ<C> &r, const XyzzZ<C> &p, const uint32_t (&k)[8]) {
   uint64_t k64[4];
 #pragma unroll
   for (int j = 0; j < 4; j++) k64[j] = (uint64_t)k[2 * j] | ((uint64_t)k[2 * j + 1] << 32);
@@ -96,6 +95,75 @@ __device__ __forceinline__ void zxyzz_scalar_mul_glv(XyzzZ<C> &r, const XyzzZ<C>
     const uint32_t b1 = bit(m1, i), b2 = bit(m2, i);
     if (b1 | b2) zxyzz_add(r, t, b1 ? (b2 ? q01 : q0) : q1);
     else r = t;
+  }
+}
+
+constexpr int kGlvW2Entries = 15;
+
+// The same product k P with the joint digits of k1, k2 taken two bits at a time, for callers
+// that can give each thread kGlvW2Entries table entries in device memory (thread t of a launch,
+// stride >= the launch's threads).  A step is two doublings and one addition of
+// E(i, j) = i Q0 + j Q1 (i, j < 4, not both zero; Q0 = +-P, Q1 = +-phi(P) with the signs of k1,
+// k2; entry 4 j + i - 1): at most 63 additions after the first entry.  In a wavefront the 1-bit
+// form adds at almost every one of its ~127 steps (a step adds as soon as one lane's bit pair is
+// nonzero), so this saves ~60 additions for 12 on the table (one doubling, two additions, three
+// phi, nine sums).  phi(j P) = phi(P) j, so E(0, j) comes from E(j, 0) by phi and the sign.
+template <class C>
+ZK_HD void zxyzz_scalar_mul_glv_w2(XyzzZ<C> &r, const XyzzZ<C> &p, const uint32_t (&k)[8],
+                                                        int32_t *__restrict__ tab, uint64_t t, uint64_t stride) {
+  uint64_t k64[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) k64[j] = (uint64_t)k[2 * j] | ((uint64_t)k[2 * j + 1] << 32);
+  uint64_t m1[2], m2[2];
+  bool n1, n2;
+  glv_split(k64, m1, n1, m2, n2);
+  // E(i, 0) = i Q0, i = 1..3; the sums read them back from the table (fewer live registers)
+  {
+    XyzzZ<C> e1 = p, e2, e3;
+    if (n1) zxyzz_neg(e1, e1);
+    zxyzz_dbl(e2, e1);
+    zxyzz_add(e3, e2, e1);
+    zxyzz_tab_store<C>(tab, 0, t, stride, e1);
+    zxyzz_tab_store<C>(tab, 1, t, stride, e2);
+    zxyzz_tab_store<C>(tab, 2, t, stride, e3);
+  }
+  Fz<ZBn254Fp> beta;
+  zset_const(beta, Bn254Glv::BETA);
+#pragma unroll 1
+  for (int j = 1; j <= 3; j++) {
+    // E(0, j) = phi(E(j, 0)), negated when the signs of k1 and k2 differ; E(i, j) = E(i, 0) + E(0, j)
+    XyzzZ<C> b;
+    zxyzz_tab_load<C>(b, tab, j - 1, t, stride);
+    zmul(b.x, b.x, beta);
+    if (n1 != n2) zxyzz_neg(b, b);
+    zxyzz_tab_store<C>(tab, 4 * j - 1, t, stride, b);
+#pragma unroll 1
+    for (int i = 1; i <= 3; i++) {
+      XyzzZ<C> a, s;
+      zxyzz_tab_load<C>(a, tab, i - 1, t, stride);
+      zxyzz_add(s, a, b);
+      zxyzz_tab_store<C>(tab, 4 * j + i - 1, t, stride, s);
+    }
+  }
+  auto digit = [&](int s) -> uint32_t {  // entry + 1 of window s (0: both digits zero)
+    const int b = 2 * s;
+    const uint32_t d1 = (uint32_t)(m1[b >> 6] >> (b & 63)) & 3u, d2 = (uint32_t)(m2[b >> 6] >> (b & 63)) & 3u;
+    return 4 * d2 + d1;
+  };
+  int top = 63;
+  while (top >= 0 && digit(top) == 0) top--;
+  if (top < 0) {
+    zxyzz_set_inf(r);
+    return;
+  }
+  zxyzz_tab_load<C>(r, tab, (int)digit(top) - 1, t, stride);
+  for (int s = top - 1; s >= 0; s--) {
+    const uint32_t d = digit(s);
+    XyzzZ<C> e, u;
+    if (d) zxyzz_tab_load<C>(e, tab, (int)d - 1, t, stride);  // in flight during the doublings
+    zxyzz_dbl(u, r);
+    zxyzz_dbl(r, u);
+    if (d) zxyzz_add(r, r, e);
   }
 }
 

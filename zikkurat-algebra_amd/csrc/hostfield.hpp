@@ -63,25 +63,29 @@ inline bool geq_p_n(const FieldDesc &f, const uint64_t *t) {
   }
   return true;
 }
+// r = t - p if t >= p (t < 2p, plus a carry word hi of 0 or 1), else t; no data-dependent branch
 template <int N>
-inline void sub_p_n(const FieldDesc &f, uint64_t *t) {
-  uint64_t br = 0;
+inline void reduce_once_n(const FieldDesc &f, Fe &r, const uint64_t *t, uint64_t hi) {
+  uint64_t s[N], br = 0;
   for (int i = 0; i < N; i++) {
-    u128 d = (u128)t[i] - f.p[i] - br;
-    t[i] = (uint64_t)d;
+    const u128 d = (u128)t[i] - f.p[i] - br;
+    s[i] = (uint64_t)d;
     br = (uint64_t)(d >> 64) & 1;
   }
+  const uint64_t keep = 0 - (br & (hi ^ 1));  // all ones: t < p (the borrow is not covered by hi)
+  for (int i = 0; i < N; i++) r.w[i] = (t[i] & keep) | (s[i] & ~keep);
 }
 template <int N>
 inline Fe add_n(const FieldDesc &f, const Fe &a, const Fe &b) {
   Fe r = zero();
+  uint64_t t[N];
   u128 c = 0;
   for (int i = 0; i < N; i++) {
     c += (u128)a.w[i] + b.w[i];
-    r.w[i] = (uint64_t)c;
+    t[i] = (uint64_t)c;
     c >>= 64;
   }
-  if (c || geq_p_n<N>(f, r.w)) sub_p_n<N>(f, r.w);
+  reduce_once_n<N>(f, r, t, (uint64_t)c);
   return r;
 }
 template <int N>
@@ -89,48 +93,43 @@ inline Fe sub_n(const FieldDesc &f, const Fe &a, const Fe &b) {
   Fe r = zero();
   uint64_t br = 0;
   for (int i = 0; i < N; i++) {
-    u128 d = (u128)a.w[i] - b.w[i] - br;
+    const u128 d = (u128)a.w[i] - b.w[i] - br;
     r.w[i] = (uint64_t)d;
     br = (uint64_t)(d >> 64) & 1;
   }
-  if (br) {
-    u128 c = 0;
-    for (int i = 0; i < N; i++) {
-      c += (u128)r.w[i] + f.p[i];
-      r.w[i] = (uint64_t)c;
-      c >>= 64;
-    }
+  const uint64_t mask = 0 - br;  // a < b: add p back
+  u128 c = 0;
+  for (int i = 0; i < N; i++) {
+    c += (u128)r.w[i] + (f.p[i] & mask);
+    r.w[i] = (uint64_t)c;
+    c >>= 64;
   }
   return r;
 }
-// CIOS with 64-bit limbs, fully unrolled for N limbs
+// Montgomery product by CIOS, the multiplication and reduction of each outer step in one
+// inner loop.  Every modulus here has a top word below 2^63 - 1, so the running value stays
+// below 2p in N words without a carry word ("no-carry" CIOS); one branch-free subtraction
+// ends it.
 template <int N>
 inline Fe mul_n(const FieldDesc &f, const Fe &a, const Fe &b) {
-  uint64_t t[N + 2] = {0};
+  uint64_t t[N] = {0};
   for (int i = 0; i < N; i++) {
-    u128 c = 0;
-    for (int j = 0; j < N; j++) {
-      c += (u128)a.w[i] * b.w[j] + t[j];
-      t[j] = (uint64_t)c;
-      c >>= 64;
-    }
-    u128 s = (u128)t[N] + c;
-    t[N] = (uint64_t)s;
-    t[N + 1] = (uint64_t)(s >> 64);
-    const uint64_t m = t[0] * f.np;
-    c = ((u128)m * f.p[0] + t[0]) >> 64;
+    u128 A = (u128)a.w[i] * b.w[0] + t[0];
+    const uint64_t t0 = (uint64_t)A;
+    const uint64_t m = t0 * f.np;
+    u128 C = ((u128)m * f.p[0] + t0) >> 64;
+    A >>= 64;
     for (int j = 1; j < N; j++) {
-      c += (u128)m * f.p[j] + t[j];
-      t[j - 1] = (uint64_t)c;
-      c >>= 64;
+      A += (u128)a.w[i] * b.w[j] + t[j];
+      C += (u128)m * f.p[j] + (uint64_t)A;
+      t[j - 1] = (uint64_t)C;
+      A >>= 64;
+      C >>= 64;
     }
-    s = (u128)t[N] + c;
-    t[N - 1] = (uint64_t)s;
-    t[N] = t[N + 1] + (uint64_t)(s >> 64);
+    t[N - 1] = (uint64_t)C + (uint64_t)A;
   }
   Fe r = zero();
-  for (int i = 0; i < N; i++) r.w[i] = t[i];
-  if (t[N] || geq_p_n<N>(f, r.w)) sub_p_n<N>(f, r.w);
+  reduce_once_n<N>(f, r, t, 0);
   return r;
 }
 
