@@ -245,8 +245,7 @@ ZK_HD void zxyzz_scalar_mul(XyzzZ<C> &r, const XyzzZ<C> &p, const uint32_t (&k)[
 // of entry e of thread t at tab[(e * PW + w) * stride + t] (stride >= the threads sharing the
 // buffer), so lanes that pick the same entry read consecutive words.
 template <class C>
-ZK_HD void zxyzz_tab_store(int32_t *__restrict__ tab, int e, uint64_t t, uint64_t stride,
-                                              const XyzzZ<C> &a) {
+ZK_HD void zxyzz_tab_store(int32_t *__restrict__ tab, int e, uint64_t t, uint64_t stride, const XyzzZ<C> &a) {
   constexpr int N = ZP<C>::T::N, PW = ZLayout<C>::XYZZ;
   int32_t *q = tab + (uint64_t)e * PW * stride + t;
 #pragma unroll
@@ -259,7 +258,7 @@ ZK_HD void zxyzz_tab_store(int32_t *__restrict__ tab, int e, uint64_t t, uint64_
 }
 template <class C>
 ZK_HD void zxyzz_tab_load(XyzzZ<C> &a, const int32_t *__restrict__ tab, int e, uint64_t t,
-                                             uint64_t stride) {
+                          uint64_t stride) {
   constexpr int N = ZP<C>::T::N, PW = ZLayout<C>::XYZZ;
   const int32_t *q = tab + (uint64_t)e * PW * stride + t;
 #pragma unroll
@@ -331,16 +330,30 @@ template <class C>
 __device__ __forceinline__ void zaff_store(int32_t *__restrict__ p, const AffZ<C> &a) {
   constexpr int N = ZP<C>::T::N, E = ZP<C>::T::EXT_WORDS;
   if (ZLayout<C>::PACKED) {
+    // packed in registers, then 16-byte stores (records are 64-byte aligned): a quarter of the
+    // store instructions of word stores, each lane's record in its own line
     FzP<C> t;
+    uint32_t w[2 * E];
     zcanon(t, a.x);
-    zpack((uint32_t *)p, t);
+    zpack(w, t);
     zcanon(t, a.y);
-    zpack((uint32_t *)p + E, t);
-  } else {
-    zstore(p, a.x);
-    zstore(p + N, a.y);
+    zpack(w + E, t);
+    uint4 *q = reinterpret_cast<uint4 *>(p);
 #pragma unroll
-    for (int j = 2 * N; j < ZLayout<C>::AFF; j++) p[j] = 0;
+    for (int j = 0; j < E / 2; j++) q[j] = make_uint4(w[4 * j], w[4 * j + 1], w[4 * j + 2], w[4 * j + 3]);
+  } else {
+    constexpr int AW = ZLayout<C>::AFF;
+    int32_t w[AW];
+#pragma unroll
+    for (int j = 0; j < N; j++) {
+      w[j] = a.x.l[j];
+      w[N + j] = a.y.l[j];
+    }
+#pragma unroll
+    for (int j = 2 * N; j < AW; j++) w[j] = 0;
+    int4 *q = reinterpret_cast<int4 *>(p);
+#pragma unroll
+    for (int j = 0; j < AW / 4; j++) q[j] = make_int4(w[4 * j], w[4 * j + 1], w[4 * j + 2], w[4 * j + 3]);
   }
 }
 template <class C>

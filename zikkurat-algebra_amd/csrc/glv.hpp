@@ -109,8 +109,8 @@ constexpr int kGlvW2Entries = 15;
 // nonzero), so this saves ~60 additions for 12 on the table (one doubling, two additions, three
 // phi, nine sums).  phi(j P) = phi(P) j, so E(0, j) comes from E(j, 0) by phi and the sign.
 template <class C>
-ZK_HD void zxyzz_scalar_mul_glv_w2(XyzzZ<C> &r, const XyzzZ<C> &p, const uint32_t (&k)[8],
-                                                        int32_t *__restrict__ tab, uint64_t t, uint64_t stride) {
+ZK_HD void zxyzz_scalar_mul_glv_w2(XyzzZ<C> &r, const XyzzZ<C> &p, const uint32_t (&k)[8], int32_t *__restrict__ tab,
+                                   uint64_t t, uint64_t stride) {
   uint64_t k64[4];
 #pragma unroll
   for (int j = 0; j < 4; j++) k64[j] = (uint64_t)k[2 * j] | ((uint64_t)k[2 * j + 1] << 32);
