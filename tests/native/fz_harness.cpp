@@ -33,6 +33,7 @@ static void run_op(int op, int n, const int32_t *a, const int32_t *b, int32_t *o
         break;
       }
       case 9: zinv(r, x); break;
+      case 12: zinv_fermat(r, x); break;
       case 10: zmul_sub(r, x, y, y, y); break;  // x y - y y, both operands possibly unnormalised
       case 11: zreduce_small(r, x); break;
       default: r = x; break;

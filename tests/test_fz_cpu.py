@@ -102,16 +102,20 @@ def test_zmul_matches_integers_at_bounds(h, field):
 
 
 @pytest.mark.parametrize("field", [0, 1, 2, 3], ids=["bn-fp", "bn-fr", "bls-fp", "bls-fr"])
-def test_zinv_matches_integers(h, field):
-    """zinv of a normalised internal value v = x R' is x^-1 R' (Fermat in the Montgomery
-    domain), so v zinv(v) = R'^2 mod p; zero and multiples of p give zero.  Inputs: the bound
-    values (|v| < 8p, negatives), 1, p - 1, small and random values."""
+@pytest.mark.parametrize("code", [9, 12], ids=["divsteps", "fermat"])
+def test_zinv_matches_integers(h, field, code):
+    """zinv (batched division steps) and zinv_fermat (a^(p-2)) of a normalised internal value
+    v = x R' are x^-1 R', so v zinv(v) = R'^2 mod p; zero and multiples of p give zero.  Inputs:
+    the bound values (|v| < 8p, negatives), 1, p - 1, 2^k and 2^k - 1 (long runs of even or odd
+    steps), small and random values."""
     p, bits, N, _ = FIELDS[field]
     Rp = 1 << (bits * N)
     rng = splitmix64(700 + field)
-    xs = sample_values(p, rng, 120, 8 if field != 3 else 4) + [1, 2, p - 1, p + 1, -1, 3 * p, 0, -2 * p]
+    xs = sample_values(p, rng, 400 if code == 9 else 120, 8 if field != 3 else 4)
+    xs += [1, 2, p - 1, p + 1, -1, 3 * p, 0, -2 * p, (p - 1) // 2, (p + 1) // 2]
+    xs += [1 << k for k in range(0, p.bit_length(), 37)] + [(1 << k) - 1 for k in range(2, p.bit_length(), 41)]
     A = arr([to_limbs(x, bits, N) for x in xs])
-    R = op(h, field, 9, A, A)
+    R = op(h, field, code, A, A)
     for i, x in enumerate(xs):
         got = value(R[i], bits)
         assert is_normalised(R[i], bits)

@@ -29,47 +29,55 @@
 
 namespace zk {
 
-// parameter packs: limbs of p, -p^-1 mod 2^BITS, conversion constants and R' mod p (values
-// generated from the primes of lib/cbits/curves/fields/mont/*_mont.c)
+// parameter packs: limbs of p, -p^-1 mod 2^BITS, conversion constants, R' mod p and R'^3 mod p
+// (values generated from the primes of lib/cbits/curves/fields/mont/*_mont.c)
 struct ZBn254Fp {
   static constexpr int BITS = 29;
   static constexpr int N = 9;
+  static constexpr int PBITS = 254;  // bit length of p
   static constexpr int EXT_WORDS = 8;
   static constexpr uint32_t NP = 75916169u;
   static constexpr int32_t P[N] = {410844487, 17064118, 477274959, 47522512, 361093496, 47923392, 10936641, 240920116, 3171406};
   static constexpr int32_t K_IN[N] = {322215073, 442336424, 171859116, 268585440, 314135016, 244503300, 348886451, 68918589, 360451};
   static constexpr int32_t K_OUT[N] = {93261213, 451550318, 297979764, 299258347, 342016167, 297253948, 482187706, 406012155, 920183};
   static constexpr int32_t ONE[N] = {360500257, 337389400, 408039635, 21759001, 178483129, 490881230, 299191303, 86689704, 903222};
+  static constexpr int32_t R3[N] = {237179570, 381705378, 197673865, 484389131, 296595421, 53368462, 493544588, 282381599, 2540242};  // R'^3 mod p
 };
 struct ZBn254Fr {
   static constexpr int BITS = 29;
   static constexpr int N = 9;
+  static constexpr int PBITS = 254;  // bit length of p
   static constexpr int EXT_WORDS = 8;
   static constexpr uint32_t NP = 268435455u;
   static constexpr int32_t P[N] = {268435457, 521120927, 240919632, 131109107, 361091715, 47923392, 10936641, 240920116, 3171406};
   static constexpr int32_t K_IN[N] = {268430039, 492061940, 71535269, 62181526, 323781850, 244503300, 348886451, 68918589, 360451};
   static constexpr int32_t K_OUT[N] = {268435451, 78749922, 406014571, 418196286, 342025071, 297253948, 482187706, 406012155, 920183};
   static constexpr int32_t ONE[N] = {268435287, 514263732, 86771339, 391139145, 178784091, 490881230, 299191303, 86689704, 903222};
+  static constexpr int32_t R3[N] = {2088370, 399706979, 438698222, 156269639, 345420443, 184590952, 364266987, 530565949, 1813142};  // R'^3 mod p
 };
 struct ZBls381Fr {
   static constexpr int BITS = 29;
   static constexpr int N = 9;
+  static constexpr int PBITS = 255;  // bit length of p
   static constexpr int EXT_WORDS = 8;
   static constexpr uint32_t NP = 536870911u;
   static constexpr int32_t P[N] = {1, 536870904, 529989567, 457704959, 494949691, 201589968, 354471143, 174404527, 7597479};
   static constexpr int32_t K_IN[N] = {536868651, 18087, 526333248, 216203804, 294598241, 8484555, 89016444, 271483280, 1968430};
   static constexpr int32_t K_OUT[N] = {536870910, 15, 13762688, 158331904, 83842440, 133690974, 364799537, 188061856, 1582257};
   static constexpr int32_t ONE[N] = {536870842, 559, 481694080, 172907520, 250130850, 384216799, 419952827, 139714039, 5047359};
+  static constexpr int32_t R3[N] = {433522269, 2153349, 370290243, 249241592, 265969956, 85437554, 317105442, 433409754, 619787};  // R'^3 mod p
 };
 struct ZBls381Fp {
   static constexpr int BITS = 28;
   static constexpr int N = 14;
+  static constexpr int PBITS = 381;  // bit length of p
   static constexpr int EXT_WORDS = 12;
   static constexpr uint32_t NP = 268238845u;
   static constexpr int32_t P[N] = {268413611, 267386879, 67108793, 268430101, 103030443, 168782607, 258420946, 255349035, 74926980, 79351158, 195540547, 241804465, 32127359, 106513};
   static constexpr int32_t K_IN[N] = {135160473, 55574580, 246491222, 233514649, 211322480, 82798231, 118064104, 172912617, 65577043, 248456990, 203477795, 116089733, 75436678, 87770};
   static constexpr int32_t K_OUT[N] = {196605, 9437184, 201327222, 48192, 146467828, 91569269, 90130584, 117777780, 130963539, 91145943, 119183266, 239678912, 247724672, 89957};
   static constexpr int32_t ONE[N] = {55049400, 226492416, 176409, 13493970, 208802528, 138027152, 3630751, 228652862, 162569026, 19495856, 85318031, 1231484, 106560762, 22394};
+  static constexpr int32_t R3[N] = {33011856, 43306061, 166965026, 190555500, 213237964, 202217308, 210348229, 113862836, 36074218, 152955081, 55988902, 139032520, 69516217, 82347};  // R'^3 mod p
 };
 
 template <class P>
@@ -260,9 +268,9 @@ ZK_HD void zsqr(Fz<P> &r, const Fz<P> &a) {
 // every exponent bit costs a squaring and every window a product: about 254 + 50 + 8
 // products for a 254-bit p, against 254 + 127 for square-and-multiply.  The window value is
 // the same in every lane, and the switch keeps the table in registers.  Output normalised,
-// < 2p; a == 0 gives 0.
+// < 2p; a == 0 gives 0.  (zinv below is the divstep form; this one stays as its cross-check.)
 template <class P>
-ZK_HD void zinv(Fz<P> &r, const Fz<P> &a) {
+ZK_HD void zinv_fermat(Fz<P> &r, const Fz<P> &a) {
   constexpr int N = P::N, B = P::BITS;
   uint32_t e[N];  // p - 2
   int32_t borrow = 2;
@@ -333,6 +341,110 @@ ZK_HD void zinv(Fz<P> &r, const Fz<P> &a) {
     i = j - 1;
   }
   r = acc;
+}
+
+// a^-1 (internal form) by batched division steps (Bernstein and Yang, "Fast constant-time gcd
+// computation and modular inversion", CHES 2019).  On f = p (odd) and g = A, the canonical
+// value of a (A = x R'), a division step is
+//   delta > 0 and g odd: (delta, f, g) <- (1 - delta, g, (g - f) / 2)
+//   g odd:               (delta, f, g) <- (1 + delta, f, (g + f) / 2)
+//   g even:              (delta, f, g) <- (1 + delta, f, g / 2)
+// starting from delta = 1; for f^2 + 4 g^2 <= 5 2^(2 b) (b = PBITS) g reaches 0 within
+// floor((49 b + 57) / 17) steps (the paper's Theorem 11.2), and then f = +-gcd = +-1.  d and e
+// track f = d A, g = e A (mod p) from d = 0, e = 1, so A^-1 = f d.
+// Batches of BITS steps: the steps depend only on delta and the parities of g, so they run on
+// the low 32 bits of f and g (one bit of validity is used up per step) and yield an integer
+// matrix T with |T00| + |T01|, |T10| + |T11| <= 2^BITS such that 2^BITS (f', g') = T (f, g).
+// The batch then applies T to the full limbs (the low limb of the result is zero: the division
+// by 2^BITS drops it) and to (d, e) modulo p, with k p added, k = -(T d)_0 / p mod 2^BITS, so
+// that the division is exact (the Montgomery step).  |d|, |e| grow by at most p per batch; the
+// last product by R'^3 reduces them: zmul(f d, R'^3) = A^-1 R'^2 = x^-1 R'.  A batch is about
+// BITS x 20 32-bit operations plus 4 N multiply-adds per pair, roughly 20 batches for a 254-bit
+// p against ~310 products for zinv_fermat: about a fifth of the instructions.  Every lane runs
+// until its own g is zero (at most the theorem's bound).  Output normalised, < 2p; a == 0
+// gives 0.
+template <class P>
+ZK_HD void zinv(Fz<P> &r, const Fz<P> &a) {
+  constexpr int N = P::N, B = P::BITS;
+  constexpr int32_t MASK = (1 << B) - 1;
+  constexpr int kMaxBatches = ((49 * P::PBITS + 57) / 17 + B - 1) / B;
+  Fz<P> f, g, d, e, t;
+  zreduce_small(t, a);
+  zcanon(g, t);
+  zset_const(f, P::P);
+  zset_zero(d);
+  zset_zero(e);
+  e.l[0] = 1;
+  int32_t delta = 1;
+  for (int batch = 0; batch < kMaxBatches && !zis_raw_zero(g); batch++) {
+    // B steps on the low 32 bits of f and g
+    // (unsigned words: after j steps only their low 32 - j bits are still exact, and a step
+    // reads bit 0 only)
+    uint32_t fl = (uint32_t)f.l[0] | ((uint32_t)f.l[1] << B);
+    uint32_t gl = (uint32_t)g.l[0] | ((uint32_t)g.l[1] << B);
+    int32_t u = 1, v = 0, q = 0, w = 1;
+#pragma unroll 4
+    for (int j = 0; j < B; j++) {
+      const uint32_t odd = 0u - (gl & 1u);                    // all ones when g is odd
+      const uint32_t sw = odd & (0u - (uint32_t)(delta > 0));  // all ones: swap to (g, -f)
+      const int32_t ss = (int32_t)sw, so = (int32_t)odd;
+      const uint32_t f0 = fl;
+      const int32_t u0 = u, v0 = v;
+      fl ^= (fl ^ gl) & sw;
+      gl ^= (gl ^ (0u - f0)) & sw;
+      u ^= (u ^ q) & ss;
+      v ^= (v ^ w) & ss;
+      q ^= (q ^ -u0) & ss;
+      w ^= (w ^ -v0) & ss;
+      delta = (delta ^ ss) - ss;  // -delta on a swap
+      gl += fl & odd;
+      q += u & so;
+      w += v & so;
+      gl >>= 1;  // exact: g + f is even
+      u *= 2;
+      v *= 2;
+      delta += 1;
+    }
+    // (f, g) <- T (f, g) / 2^B and (d, e) <- (T (d, e) + (kd, ke) p) / 2^B, column by column
+    int64_t cf = (int64_t)u * f.l[0] + (int64_t)v * g.l[0];
+    int64_t cg = (int64_t)q * f.l[0] + (int64_t)w * g.l[0];
+    int64_t cd = (int64_t)u * d.l[0] + (int64_t)v * e.l[0];
+    int64_t ce = (int64_t)q * d.l[0] + (int64_t)w * e.l[0];
+    const int64_t kd = (int64_t)(((uint32_t)cd * P::NP) & (uint32_t)MASK);
+    const int64_t ke = (int64_t)(((uint32_t)ce * P::NP) & (uint32_t)MASK);
+    cd += kd * P::P[0];
+    ce += ke * P::P[0];
+    cf >>= B;
+    cg >>= B;
+    cd >>= B;
+    ce >>= B;
+#pragma unroll
+    for (int i = 1; i < N; i++) {
+      cf += (int64_t)u * f.l[i] + (int64_t)v * g.l[i];
+      cg += (int64_t)q * f.l[i] + (int64_t)w * g.l[i];
+      cd += (int64_t)u * d.l[i] + (int64_t)v * e.l[i] + kd * P::P[i];
+      ce += (int64_t)q * d.l[i] + (int64_t)w * e.l[i] + ke * P::P[i];
+      f.l[i - 1] = (int32_t)(cf & MASK);
+      g.l[i - 1] = (int32_t)(cg & MASK);
+      d.l[i - 1] = (int32_t)(cd & MASK);
+      e.l[i - 1] = (int32_t)(ce & MASK);
+      cf >>= B;
+      cg >>= B;
+      cd >>= B;
+      ce >>= B;
+    }
+    f.l[N - 1] = (int32_t)cf;
+    g.l[N - 1] = (int32_t)cg;
+    d.l[N - 1] = (int32_t)cd;
+    e.l[N - 1] = (int32_t)ce;
+  }
+  if (f.l[N - 1] < 0) {  // f = -1
+    zneg(d, d);
+    znorm(d);
+  }
+  Fz<P> k3;
+  zset_const(k3, P::R3);
+  zmul(r, d, k3);
 }
 
 // a == 0 (mod p) for a normalised a with |a| < 8p, i.e. a = j p for some j in [-8, 8].  Then
