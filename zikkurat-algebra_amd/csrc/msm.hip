@@ -737,9 +737,12 @@ __global__ __launch_bounds__(kThreads) void msm_points_in_kernel(const uint32_t 
 // final window terms: internal XYZZ -> external canonical words (4 field elements each)
 template <class C>
 __global__ __launch_bounds__(kThreads) void msm_points_out_kernel(const int32_t *__restrict__ in, uint32_t cnt,
-                                                                 uint32_t *__restrict__ out) {
+                                                                 uint32_t *__restrict__ out,
+                                                                 const uint32_t *__restrict__ info_src,
+                                                                 uint32_t *__restrict__ info_dst) {
   using F = typename C::Fp;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (info_dst && i < 3) info_dst[i] = info_src[i];  // the heavy-bucket statistics words
   if (i >= cnt) return;
   XyzzZ<C> p;
   zxyzz_load(p, in + (size_t)i * ZLayout<C>::XYZZ);
@@ -1305,11 +1308,13 @@ struct Events {
 struct MsmStage {
   uint32_t *info = nullptr;  // heavy count, most partials, sorted entries
   uint32_t *res = nullptr;   // pinned result words
+  uint32_t *info_dev = nullptr, *res_dev = nullptr;  // their device addresses (kernels write them)
   size_t res_words = 0;
   hipStream_t aux = nullptr;     // a second stream, created on first use
   hipEvent_t aux_ev = nullptr;
   int init() {
     ZK_HIP(hipHostMalloc((void **)&info, 16, hipHostMallocDefault));
+    ZK_HIP(hipHostGetDevicePointer((void **)&info_dev, info, 0));
     return ZK_OK;
   }
   int ensure_aux() {
@@ -1321,9 +1326,10 @@ struct MsmStage {
   int reserve_res(size_t words) {
     if (words <= res_words) return ZK_OK;
     if (res) ZK_HIP(hipHostFree(res));
-    res = nullptr;
+    res = res_dev = nullptr;
     res_words = 0;
     ZK_HIP(hipHostMalloc((void **)&res, words * 4, hipHostMallocDefault));
+    ZK_HIP(hipHostGetDevicePointer((void **)&res_dev, res, 0));
     res_words = words;
     return ZK_OK;
   }
@@ -1735,10 +1741,13 @@ struct MsmPipe {
     pyr_final = fin;
   }
 
-  // the A arrays of W points ([0] = Sum_w, [j] = U_{w, j-1}) as external canonical words
-  void launch_out(hipStream_t st) {
+  // the A arrays of W points ([0] = Sum_w, [j] = U_{w, j-1}) as external canonical words, into
+  // outx or straight into dst (pinned host words, by their device address); with info_dst also
+  // the three heavy-bucket statistics words
+  void launch_out(hipStream_t st, uint32_t *dst = nullptr, uint32_t *info_dst = nullptr) {
     hipLaunchKernelGGL(msm_points_out_kernel<C>, dim3((nres + kThreads - 1) / kThreads), dim3(kThreads), 0, st,
-                       pyr_out(), nres, outx.as<uint32_t>());
+                       pyr_out(), nres, dst ? dst : outx.as<uint32_t>(), (const uint32_t *)heavy.as<uint32_t>(),
+                       info_dst);
   }
 
   // host Horner: sum over (w, j) of 2^(start_w + max(j-1, 0)) * term
@@ -1814,7 +1823,7 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   ZK_TRY(lease.acquire());
   MsmStage &stage = *lease.s;
   uint32_t *info = stage.info;
-  ZK_HIP(hipMemcpyAsync(info, p.heavy.p, 12, hipMemcpyDeviceToHost, st));
+  ZK_TRY(stage.reserve_res((size_t)p.nres * MsmPipe<C>::PW));
   ev.mark(st);
   p.launch_accum(st);
   ZK_HIP(hipGetLastError());
@@ -1826,10 +1835,9 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   p.pyramid(st, true);
   ZK_HIP(hipGetLastError());
   ev.mark(st);
-  p.launch_out(st);
+  // the result words and the statistics go straight to pinned host memory: no copy launches
+  p.launch_out(st, stage.res_dev, stage.info_dev);
   ZK_HIP(hipGetLastError());
-  ZK_TRY(stage.reserve_res((size_t)p.nres * MsmPipe<C>::PW));
-  ZK_HIP(hipMemcpyAsync(stage.res, p.outx.p, (size_t)p.nres * MsmPipe<C>::PW * 4, hipMemcpyDeviceToHost, st));
   ZK_HIP(hipStreamSynchronize(st));
   drain.release();
   ev.mark(st);
@@ -1841,7 +1849,7 @@ static int msm_run(int n, const uint64_t *d_scalars, int enl, int mont, const ui
   p.combine(stage.res, out);
   auto t_host2 = std::chrono::steady_clock::now();
 
-  // timing: sort, accumulate, fix, pyramid, d2h, host combine
+  // timing: sort, accumulate, fix, pyramid, output (into pinned host memory, and the wait), host combine
   g_nphase = 0;
   if (ev.ok) {
     for (int i = 0; i + 1 < ev.n; i++) {
