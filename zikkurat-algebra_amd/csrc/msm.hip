@@ -1228,6 +1228,42 @@ void msm_infinity(int curve, MsmResult *r) {
   r->curve = curve;
 }
 
+// x^-1 R for the Montgomery form x R (x != 0, canonical 64-bit words) of a base-field element,
+// by the kernels' division-step inversion run on the host (measured 2.7 against 8.9 us for
+// the host's a^(p-2) on BN254, 4.6 against 38 us on BLS12-381): unpacked, x R is an internal
+// value u, zinv gives u^-1 R'^2, and one product by R^2 / R' (= zmul(R, R), R = K_OUT) gives
+// u^-1 R^2 = x^-1 R.
+template <class P>
+static zkh::Fe host_inv_mont(const zkh::Fe &a) {
+  constexpr int E = P::EXT_WORDS;
+  uint32_t w[E];
+  for (int j = 0; j < E; j++) w[j] = (uint32_t)(a.w[j / 2] >> (32 * (j & 1)));
+  Fz<P> u, inv, k, t;
+  zunpack(u, w);
+  zinv(inv, u);
+  zset_const(k, P::K_OUT);
+  zmul(t, k, k);
+  zmul(inv, inv, t);
+  zcanon(inv, inv);
+  zpack(w, inv);
+  zkh::Fe r = zkh::zero();
+  for (int j = 0; j < E; j++) r.w[j / 2] |= (uint64_t)w[j] << (32 * (j & 1));
+  return r;
+}
+
+// affine Montgomery words of an XYZZ point (all 0xFF for infinity), as zkh::xyzz_to_affine
+static void msm_to_affine(int curve, const zkh::FieldDesc &f, const zkh::Xyzz &p, uint64_t *out) {
+  if (zkh::xyzz_is_inf(f, p)) {
+    memset(out, 0xff, 16 * f.nl);
+    return;
+  }
+  const zkh::Fe zzzi = curve == 0 ? host_inv_mont<ZBn254Fp>(p.zzz) : host_inv_mont<ZBls381Fp>(p.zzz);
+  const zkh::Fe zi = zkh::mul(f, zzzi, p.zz);  // Z^2 / Z^3
+  const zkh::Fe zzi = zkh::sqr(f, zi);
+  zkh::store(f, out, zkh::mul(f, p.x, zzi));
+  zkh::store(f, out + f.nl, zkh::mul(f, p.y, zzzi));
+}
+
 int msm_write_result(int curve, const MsmResult &r, uint64_t *tgt, int flags) {
   const zkh::FieldDesc &f = fp_desc(curve);
   const int L = f.nl;
@@ -1237,7 +1273,7 @@ int msm_write_result(int curve, const MsmResult &r, uint64_t *tgt, int flags) {
   p.zz = zkh::load(f, r.zz);
   p.zzz = zkh::load(f, r.zzz);
   if (!(flags & (ZKG_MSM_PROJ_OUT | ZKG_MSM_JAC_OUT))) {
-    zkh::xyzz_to_affine(f, p, tgt);
+    msm_to_affine(curve, f, p, tgt);
     return ZK_OK;
   }
   const zkh::Fe one = zkh::one(f), zero = zkh::zero();
@@ -1250,7 +1286,7 @@ int msm_write_result(int curve, const MsmResult &r, uint64_t *tgt, int flags) {
     return ZK_OK;
   }
   uint64_t aff[12];
-  zkh::xyzz_to_affine(f, p, aff);
+  msm_to_affine(curve, f, p, aff);
   memcpy(tgt, aff, 16 * L);
   zkh::store(f, tgt + 2 * L, one);
   return ZK_OK;
