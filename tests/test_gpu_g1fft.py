@@ -60,3 +60,33 @@ def test_g1_fft_closed_form_and_roundtrip(zk, C, m):
     back = d_back.to_host(P.shape)
     ref = Oracle(C).points_from_scalars(ks)
     assert (back[:, :2 * L] == ref).all()
+
+
+@pytest.mark.parametrize("C,m", [(BN, 14), (BN, 16), (BLS, 12)], ids=["bn254-2^14", "bn254-2^16", "bls12-381-2^12"])
+def test_g1_fft_closed_form_large(zk, C, m):
+    """The stage plans of larger transforms (2^14: three groups of four stages and one of two;
+    2^16: five groups of three stages and one radix-2 stage, the table stride set by the
+    largest group launch; BLS12-381 2^12: the 2-bit windows on k) against the same closed form,
+    with the points k_i G built on the device as (x, y, 1); the inverse must return them."""
+    N = 1 << m
+    L = C.lp
+    ks = random_scalars(700 + m, N, C.r)
+    d_k = zk.DeviceArray.from_host(ints_to_array(ks, 4))
+    aff = zk.scalar_mul_gen(C.id, d_k, N).to_host((N, 2 * L))
+    one = np.array(to_limbs(C.fp_to_mont(1), L), np.uint64)
+    P = np.ascontiguousarray(np.concatenate([aff, np.tile(one, (N, 1))], axis=1))
+    g = zk.fft_generator(C.id, m)
+    d_src = zk.DeviceArray.from_host(P)
+    d_y, d_back = zk.DeviceArray(P.size), zk.DeviceArray(P.size)
+    zk.g1_fft_device(C.id, m, g, d_src, d_y)
+    Y = d_y.to_host(P.shape)
+    K = ints_to_array([C.fr_to_mont(k) for k in ks], 4)
+    E = zk.ntt(C.id, m, g, K)
+    e_std = ints_to_array([C.fr_from_mont(x) for x in array_to_ints(E)], 4)
+    G = zk.scalar_mul_gen(C.id, zk.DeviceArray.from_host(e_std), N).to_host((N, 2 * L))
+    inf = ~G.any(axis=1)
+    assert (Y[~inf, :2 * L] == G[~inf]).all()
+    assert (Y[~inf, 2 * L:] == one).all()
+    assert not Y[inf, :L].any() and not Y[inf, 2 * L:].any()
+    zk.g1_fft_device(C.id, m, g, d_y, d_back, inverse=True)
+    assert (d_back.to_host(P.shape) == P).all()
