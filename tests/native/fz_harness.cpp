@@ -87,16 +87,16 @@ static void run_curve(int op, int n, const int32_t *a, const int32_t *b, int32_t
         if (op == 4) {
           zxyzz_scalar_mul_glv(r, p, sc);
         } else {
-          int32_t tab[kGlvW2Entries * ZLayout<C>::XYZZ];
-          zxyzz_scalar_mul_glv_w2(r, p, sc, tab, 0, 1);
+          alignas(16) uint32_t tab[kGlvW2Entries * ZLayout<C>::XYZZ];
+          zxyzz_scalar_mul_glv_w2(r, p, sc, tab);
         }
       } else {
         zxyzz_set_inf(r);
       }
-    } else if (op == 6) {  // 2-bit windows on k over a 3-entry host table (stride 1)
+    } else if (op == 6) {  // 2-bit windows on k over a 3-entry host table
       uint32_t sc[8];
       memcpy(sc, b + k * 8, sizeof sc);
-      int32_t tab[3 * ZLayout<C>::XYZZ];
+      uint32_t tab[3 * ZLayout<C>::XYZZ];
       zxyzz_scalar_mul_w2(r, p, sc, tab, 0, 1);
     } else {
       zxyzz_dbl(r, p);

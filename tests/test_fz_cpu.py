@@ -341,7 +341,7 @@ def test_xyzz_scalar_mul(h, C, op):
 def test_glv_scalar_mul_forms(h):
     """The two GLV forms of BN254 scalar multiplication (csrc/glv.hpp), the G1 FFT's butterflies:
     the 1-bit joint double-and-add and the 2-bit joint windows over a per-thread point table
-    (here one host table, stride 1) both equal k P, for scalars at the split's edges (0, 1, small
+    (here one host table) both equal k P, for scalars at the split's edges (0, 1, small
     values whose top window is partial, r - 1, the cube roots of unity lambda, lambda^2 mod r and
     their neighbours, where one half of the split is 0 or +-1) and random scalars below r, on
     points in lazy XYZZ form and on infinity."""

@@ -138,18 +138,21 @@ __device__ __forceinline__ void scalar_mul(XyzzZ<C> &r, const XyzzZ<C> &p, const
   else zxyzz_scalar_mul(r, p, k);
 }
 
-// the same in the grouped kernels, where thread t of a launch owns g1_tab_entries<C>() point-table
-// entries of tab (stride = the table's threads): BN254 by 2-bit joint GLV windows, BLS12-381 by
-// 2-bit windows on k
+// the same in the grouped kernels, where thread t of a launch (stride = the table's threads)
+// owns g1_tab_entries<C>() point-table entries of tab: BN254 by 2-bit joint GLV windows over its
+// thread-major block (at t g1_tab_entries XYZZ words), BLS12-381 by 2-bit windows on k over the
+// word-major table
 template <class C>
 __host__ __device__ constexpr int g1_tab_entries() {
   return C::id == 0 ? kGlvW2Entries : 3;
 }
 template <class C>
 __device__ __forceinline__ void group_scalar_mul(XyzzZ<C> &r, const XyzzZ<C> &p, const uint32_t (&k)[8],
-                                                 int32_t *__restrict__ tab, uint64_t t, uint64_t stride) {
-  if constexpr (C::id == 0) zxyzz_scalar_mul_glv_w2(r, p, k, tab, t, stride);
-  else zxyzz_scalar_mul_w2(r, p, k, tab, t, stride);
+                                                 uint32_t *__restrict__ tab, uint64_t t, uint64_t stride) {
+  if constexpr (C::id == 0)
+    zxyzz_scalar_mul_glv_w2(r, p, k, tab + t * (uint64_t)(g1_tab_entries<C>() * ZLayout<C>::XYZZ));
+  else
+    zxyzz_scalar_mul_w2(r, p, k, tab, t, stride);
 }
 
 // one radix-2 stage over blocks of 2^s points (DIT: u + w v, u - w v; DIF: u + v, w (u - v))
@@ -292,7 +295,7 @@ template <class C>
 __global__ __launch_bounds__(kG1Threads) void g1_group_dit_mul_kernel(const int32_t *__restrict__ pts,
                                                                      int32_t *__restrict__ scr,
                                                                      const uint32_t *__restrict__ twm, int m, int s0,
-                                                                     int r, uint32_t ngroups, int32_t *__restrict__ tab,
+                                                                     int r, uint32_t ngroups, uint32_t *__restrict__ tab,
                                                                      uint64_t stride) {
   using FR = typename C::Fr;
   constexpr int PW = ZLayout<C>::XYZZ;
@@ -358,7 +361,7 @@ __global__ __launch_bounds__(kG1Threads) void g1_group_dif_mul_kernel(const int3
                                                                      int32_t *__restrict__ scr,
                                                                      const uint32_t *__restrict__ twm, int m, int s0,
                                                                      int r, uint32_t ngroups, int has_fin,
-                                                                     Fe<typename C::Fr> fin, int32_t *__restrict__ tab,
+                                                                     Fe<typename C::Fr> fin, uint32_t *__restrict__ tab,
                                                                      uint64_t stride) {
   using FR = typename C::Fr;
   constexpr int PW = ZLayout<C>::XYZZ;
@@ -470,11 +473,10 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
     }
   // grouped launches: g1_tab_entries table points per thread (BN254: 15, 47.5 MB at 2^12 and at
   // most ~0.43 GB where the stage plan's thread cap binds; BLS12-381: 3)
-  const uint64_t tab_stride = scr_pts;
   if (grouped) {
     ZK_TRY(twm.alloc((size_t)(halfN ? halfN : 1) * 32));
     ZK_TRY(scr.alloc(scr_pts * ZW * 4));
-    ZK_TRY(tab.alloc(tab_stride * g1_tab_entries<C>() * ZW * 4));
+    ZK_TRY(tab.alloc(scr_pts * g1_tab_entries<C>() * ZW * 4));
   }
   auto to_fr = [&](const zkh::Fe &x) {
     Fe<FR> d;
@@ -504,7 +506,7 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
         hipLaunchKernelGGL(g1_group_dit_mul_kernel<C>, grid((uint64_t)ng * (g1_kcount(r) + 1)), blk,
                            0, st,
                            (const int32_t *)pts.as<int32_t>(), scr.as<int32_t>(), twm.as<uint32_t>(), m, s0, r, ng,
-                           tab.as<int32_t>(), tab_stride);
+                           tab.as<uint32_t>(), scr_pts);
         hipLaunchKernelGGL(g1_group_dit_sum_kernel<C>, grid(((uint64_t)ng << r) * 8), blk, 0, st, pts.as<int32_t>(),
                            (const int32_t *)scr.as<int32_t>(), s0, r, ng);
       }
@@ -528,7 +530,7 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
         hipLaunchKernelGGL(g1_group_dif_mul_kernel<C>, grid((uint64_t)ng * (g1_kcount(r) + 1)), blk,
                            0, st,
                            (const int32_t *)pts.as<int32_t>(), scr.as<int32_t>(), twm.as<uint32_t>(), m, s0, r, ng,
-                           fin, to_fr(ninv_m), tab.as<int32_t>(), tab_stride);
+                           fin, to_fr(ninv_m), tab.as<uint32_t>(), scr_pts);
         hipLaunchKernelGGL(g1_group_dif_sum_kernel<C>, grid(((uint64_t)ng << r) * 8), blk, 0, st, pts.as<int32_t>(),
                            (const int32_t *)scr.as<int32_t>(), s0, r, ng);
       }

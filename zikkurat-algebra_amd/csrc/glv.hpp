@@ -101,16 +101,16 @@ ZK_HD void zxyzz_scalar_mul_glv(XyzzZ<C> &r, const XyzzZ<C> &p, const uint32_t (
 constexpr int kGlvW2Entries = 15;
 
 // The same product k P with the joint digits of k1, k2 taken two bits at a time, for callers
-// that can give each thread kGlvW2Entries table entries in device memory (thread t of a launch,
-// stride >= the launch's threads).  A step is two doublings and one addition of
+// that can give each thread kGlvW2Entries thread-major table entries in device memory (its
+// block tt, kGlvW2Entries * XYZZ words).  A step is two doublings and one addition of
 // E(i, j) = i Q0 + j Q1 (i, j < 4, not both zero; Q0 = +-P, Q1 = +-phi(P) with the signs of k1,
 // k2; entry 4 j + i - 1): at most 63 additions after the first entry.  In a wavefront the 1-bit
 // form adds at almost every one of its ~127 steps (a step adds as soon as one lane's bit pair is
 // nonzero), so this saves ~60 additions for 12 on the table (one doubling, two additions, three
 // phi, nine sums).  phi(j P) = phi(P) j, so E(0, j) comes from E(j, 0) by phi and the sign.
 template <class C>
-ZK_HD void zxyzz_scalar_mul_glv_w2(XyzzZ<C> &r, const XyzzZ<C> &p, const uint32_t (&k)[8], int32_t *__restrict__ tab,
-                                   uint64_t t, uint64_t stride) {
+ZK_HD void zxyzz_scalar_mul_glv_w2(XyzzZ<C> &r, const XyzzZ<C> &p, const uint32_t (&k)[8],
+                                   uint32_t *__restrict__ tt) {
   uint64_t k64[4];
 #pragma unroll
   for (int j = 0; j < 4; j++) k64[j] = (uint64_t)k[2 * j] | ((uint64_t)k[2 * j + 1] << 32);
@@ -123,9 +123,9 @@ ZK_HD void zxyzz_scalar_mul_glv_w2(XyzzZ<C> &r, const XyzzZ<C> &p, const uint32_
     if (n1) zxyzz_neg(e1, e1);
     zxyzz_dbl(e2, e1);
     zxyzz_add(e3, e2, e1);
-    zxyzz_tab_store<C>(tab, 0, t, stride, e1);
-    zxyzz_tab_store<C>(tab, 1, t, stride, e2);
-    zxyzz_tab_store<C>(tab, 2, t, stride, e3);
+    zxyzz_tab_store<C>(tt, 0, e1);
+    zxyzz_tab_store<C>(tt, 1, e2);
+    zxyzz_tab_store<C>(tt, 2, e3);
   }
   Fz<ZBn254Fp> beta;
   zset_const(beta, Bn254Glv::BETA);
@@ -133,16 +133,16 @@ ZK_HD void zxyzz_scalar_mul_glv_w2(XyzzZ<C> &r, const XyzzZ<C> &p, const uint32_
   for (int j = 1; j <= 3; j++) {
     // E(0, j) = phi(E(j, 0)), negated when the signs of k1 and k2 differ; E(i, j) = E(i, 0) + E(0, j)
     XyzzZ<C> b;
-    zxyzz_tab_load<C>(b, tab, j - 1, t, stride);
+    zxyzz_tab_load<C>(b, tt, j - 1);
     zmul(b.x, b.x, beta);
     if (n1 != n2) zxyzz_neg(b, b);
-    zxyzz_tab_store<C>(tab, 4 * j - 1, t, stride, b);
+    zxyzz_tab_store<C>(tt, 4 * j - 1, b);
 #pragma unroll 1
     for (int i = 1; i <= 3; i++) {
       XyzzZ<C> a, s;
-      zxyzz_tab_load<C>(a, tab, i - 1, t, stride);
+      zxyzz_tab_load<C>(a, tt, i - 1);
       zxyzz_add(s, a, b);
-      zxyzz_tab_store<C>(tab, 4 * j + i - 1, t, stride, s);
+      zxyzz_tab_store<C>(tt, 4 * j + i - 1, s);
     }
   }
   auto digit = [&](int s) -> uint32_t {  // entry + 1 of window s (0: both digits zero)
@@ -156,11 +156,11 @@ ZK_HD void zxyzz_scalar_mul_glv_w2(XyzzZ<C> &r, const XyzzZ<C> &p, const uint32_
     zxyzz_set_inf(r);
     return;
   }
-  zxyzz_tab_load<C>(r, tab, (int)digit(top) - 1, t, stride);
+  zxyzz_tab_load<C>(r, tt, (int)digit(top) - 1);
   for (int s = top - 1; s >= 0; s--) {
     const uint32_t d = digit(s);
     XyzzZ<C> e, u;
-    if (d) zxyzz_tab_load<C>(e, tab, (int)d - 1, t, stride);  // in flight during the doublings
+    if (d) zxyzz_tab_load<C>(e, tt, (int)d - 1);  // in flight during the doublings
     zxyzz_dbl(u, r);
     zxyzz_dbl(r, u);
     if (d) zxyzz_add(r, r, e);
