@@ -62,12 +62,15 @@ def test_g1_fft_closed_form_and_roundtrip(zk, C, m):
     assert (back[:, :2 * L] == ref).all()
 
 
-@pytest.mark.parametrize("C,m", [(BN, 14), (BN, 16), (BLS, 12)], ids=["bn254-2^14", "bn254-2^16", "bls12-381-2^12"])
+@pytest.mark.parametrize("C,m", [(BN, 14), (BN, 16), (BN, 18), (BLS, 12), (BLS, 15)],
+                         ids=["bn254-2^14", "bn254-2^16", "bn254-2^18", "bls12-381-2^12", "bls12-381-2^15"])
 def test_g1_fft_closed_form_large(zk, C, m):
     """The stage plans of larger transforms (2^14: three groups of four stages and one of two;
     2^16: five groups of three stages and one radix-2 stage, the table stride set by the
-    largest group launch; BLS12-381 2^12: the 2-bit windows on k) against the same closed form,
-    with the points k_i G built on the device as (x, y, 1); the inverse must return them."""
+    largest group launch; 2^18: radix-2 stages only, whose threads stride over the butterflies,
+    and the inverse's separate N^-1 pass; BLS12-381: the 2-bit windows on k) against the same
+    closed form, with the points k_i G built on the device as (x, y, 1); the inverse must
+    return them."""
     N = 1 << m
     L = C.lp
     ks = random_scalars(700 + m, N, C.r)

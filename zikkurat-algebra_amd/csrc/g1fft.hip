@@ -130,24 +130,17 @@ __global__ __launch_bounds__(kG1Threads) void g1_twiddle_kernel(uint32_t *__rest
   fstore(tab + (size_t)j * FR::N, r);
 }
 
-// k P: BN254 through the GLV endomorphism (exact for every point of its prime-order G1);
-// BLS12-381, whose G1 has a cofactor, by double-and-add on k
-template <class C>
-__device__ __forceinline__ void scalar_mul(XyzzZ<C> &r, const XyzzZ<C> &p, const uint32_t (&k)[8]) {
-  if constexpr (C::id == 0) zxyzz_scalar_mul_glv(r, p, k);
-  else zxyzz_scalar_mul(r, p, k);
-}
-
-// the same in the grouped kernels, where thread t of a launch (stride = the table's threads)
-// owns g1_tab_entries<C>() point-table entries of tab: BN254 by 2-bit joint GLV windows over its
-// thread-major block (at t g1_tab_entries XYZZ words), BLS12-381 by 2-bit windows on k over the
+// k P for the twiddle products of every kernel below, where thread t of a launch (stride = the
+// table's threads) owns g1_tab_entries<C>() point-table entries of tab: BN254 by 2-bit joint GLV
+// windows over its thread-major block (at t g1_tab_entries XYZZ words; exact for every point of
+// its prime-order G1), BLS12-381, whose G1 has a cofactor, by 2-bit windows on k over the
 // word-major table
 template <class C>
 __host__ __device__ constexpr int g1_tab_entries() {
   return C::id == 0 ? kGlvW2Entries : 3;
 }
 template <class C>
-__device__ __forceinline__ void group_scalar_mul(XyzzZ<C> &r, const XyzzZ<C> &p, const uint32_t (&k)[8],
+__device__ __forceinline__ void table_scalar_mul(XyzzZ<C> &r, const XyzzZ<C> &p, const uint32_t (&k)[8],
                                                  uint32_t *__restrict__ tab, uint64_t t, uint64_t stride) {
   if constexpr (C::id == 0)
     zxyzz_scalar_mul_glv_w2(r, p, k, tab + t * (uint64_t)(g1_tab_entries<C>() * ZLayout<C>::XYZZ));
@@ -155,57 +148,65 @@ __device__ __forceinline__ void group_scalar_mul(XyzzZ<C> &r, const XyzzZ<C> &p,
     zxyzz_scalar_mul_w2(r, p, k, tab, t, stride);
 }
 
-// one radix-2 stage over blocks of 2^s points (DIT: u + w v, u - w v; DIF: u + v, w (u - v))
+// one radix-2 stage over blocks of 2^s points (DIT: u + w v, u - w v; DIF: u + v, w (u - v)),
+// nthr threads striding over the N/2 butterflies (thread t owns table block t of tab, stride
+// nthr)
 template <class C, bool DIF>
 __global__ __launch_bounds__(kG1Threads) void g1_fft_stage_kernel(int32_t *__restrict__ pts,
-                                                                 const uint32_t *__restrict__ tw, int m, int s) {
+                                                                 const uint32_t *__restrict__ tw, int m, int s,
+                                                                 uint32_t *__restrict__ tab, uint32_t nthr) {
   constexpr int PW = ZLayout<C>::XYZZ;
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (1u << (m - 1))) return;
-  const uint32_t half = 1u << (s - 1), pos = t & (half - 1), grp = t >> (s - 1);
-  const uint32_t i0 = (grp << s) | pos, i1 = i0 + half;
-  XyzzZ<C> u, v, w, a;
-  zxyzz_load(u, pts + (size_t)i0 * PW);
-  zxyzz_load(v, pts + (size_t)i1 * PW);
-  uint32_t k[8];
-  const uint32_t *kp = tw + ((size_t)pos << (m - s)) * 8;
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= nthr) return;
+  for (uint32_t t = tid; t < (1u << (m - 1)); t += nthr) {
+    const uint32_t half = 1u << (s - 1), pos = t & (half - 1), grp = t >> (s - 1);
+    const uint32_t i0 = (grp << s) | pos, i1 = i0 + half;
+    XyzzZ<C> u, v, w, a;
+    zxyzz_load(u, pts + (size_t)i0 * PW);
+    zxyzz_load(v, pts + (size_t)i1 * PW);
+    uint32_t k[8];
+    const uint32_t *kp = tw + ((size_t)pos << (m - s)) * 8;
 #pragma unroll
-  for (int j = 0; j < 8; j++) k[j] = kp[j];
-  if (!DIF) {
-    if (pos == 0) w = v;  // twiddle 1
-    else scalar_mul<C>(w, v, k);
-    zxyzz_add(a, u, w);
-    zxyzz_neg(w, w);
-    zxyzz_add(v, u, w);
-    zxyzz_store(pts + (size_t)i0 * PW, a);
-    zxyzz_store(pts + (size_t)i1 * PW, v);
-  } else {
-    zxyzz_add(a, u, v);
-    zxyzz_neg(v, v);
-    zxyzz_add(w, u, v);
-    if (pos != 0) {
-      XyzzZ<C> r;
-      scalar_mul<C>(r, w, k);
-      w = r;
+    for (int j = 0; j < 8; j++) k[j] = kp[j];
+    if (!DIF) {
+      if (pos == 0) w = v;  // twiddle 1
+      else table_scalar_mul<C>(w, v, k, tab, tid, nthr);
+      zxyzz_add(a, u, w);
+      zxyzz_neg(w, w);
+      zxyzz_add(v, u, w);
+      zxyzz_store(pts + (size_t)i0 * PW, a);
+      zxyzz_store(pts + (size_t)i1 * PW, v);
+    } else {
+      zxyzz_add(a, u, v);
+      zxyzz_neg(v, v);
+      zxyzz_add(w, u, v);
+      if (pos != 0) {
+        XyzzZ<C> r;
+        table_scalar_mul<C>(r, w, k, tab, tid, nthr);
+        w = r;
+      }
+      zxyzz_store(pts + (size_t)i0 * PW, a);
+      zxyzz_store(pts + (size_t)i1 * PW, w);
     }
-    zxyzz_store(pts + (size_t)i0 * PW, a);
-    zxyzz_store(pts + (size_t)i1 * PW, w);
   }
 }
 
-// pts[i] = k pts[i]
+// pts[i] = k pts[i], nthr threads striding over the N points (tables as above)
 template <class C>
-__global__ __launch_bounds__(kG1Threads) void g1_scale_kernel(int32_t *__restrict__ pts, uint32_t N, Fe<typename C::Fr> kst) {
+__global__ __launch_bounds__(kG1Threads) void g1_scale_kernel(int32_t *__restrict__ pts, uint32_t N, Fe<typename C::Fr> kst,
+                                                             uint32_t *__restrict__ tab, uint32_t nthr) {
   constexpr int PW = ZLayout<C>::XYZZ;
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= nthr) return;
   uint32_t k[8];
 #pragma unroll
   for (int j = 0; j < 8; j++) k[j] = kst.w[j];
-  XyzzZ<C> p, r;
-  zxyzz_load(p, pts + (size_t)i * PW);
-  scalar_mul<C>(r, p, k);
-  zxyzz_store(pts + (size_t)i * PW, r);
+  for (uint32_t i = tid; i < N; i += nthr) {
+    XyzzZ<C> p, r;
+    zxyzz_load(p, pts + (size_t)i * PW);
+    table_scalar_mul<C>(r, p, k, tab, tid, nthr);
+    zxyzz_store(pts + (size_t)i * PW, r);
+  }
 }
 
 // ---- grouped stages ---------------------------------------------------------------------
@@ -318,7 +319,7 @@ __global__ __launch_bounds__(kG1Threads) void g1_group_dit_mul_kernel(const int3
   g1_path_coeff<C>(c, twm, m, s0, p0, q, L, h, false, c);
   zxyzz_load(x, pts + (b + ((uint64_t)q << s0)) * PW);
   XyzzZ<C> y;
-  group_scalar_mul<C>(y, x, c.w, tab, tid, stride);
+  table_scalar_mul<C>(y, x, c.w, tab, tid, stride);
   zxyzz_store(scr + tid * PW, y);
 }
 
@@ -395,7 +396,7 @@ __global__ __launch_bounds__(kG1Threads) void g1_group_dif_mul_kernel(const int3
     return;
   }
   XyzzZ<C> y;
-  group_scalar_mul<C>(y, s, c.w, tab, tid, stride);
+  table_scalar_mul<C>(y, s, c.w, tab, tid, stride);
   zxyzz_store(scr + tid * PW, y);
 }
 
@@ -463,7 +464,8 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
   Context *ctx = nullptr;
   ZK_TRY(context(&ctx));
   std::vector<int> plan;
-  g1_stage_plan(m, (uint64_t)ctx->num_cus * 4 * 3 * 64, plan);
+  const uint64_t round_threads = (uint64_t)ctx->num_cus * 4 * 3 * 64;
+  g1_stage_plan(m, round_threads, plan);
   uint64_t scr_pts = 0;
   bool grouped = false;
   for (int r : plan)
@@ -471,12 +473,15 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
       grouped = true;
       scr_pts = std::max<uint64_t>(scr_pts, (uint64_t)(N >> r) * (g1_kcount(r) + 1));
     }
-  // grouped launches: g1_tab_entries table points per thread (BN254: 15, 47.5 MB at 2^12 and at
-  // most ~0.43 GB where the stage plan's thread cap binds; BLS12-381: 3)
+  // g1_tab_entries table points per thread of every launch that multiplies (BN254: 15, 47.5 MB
+  // at 2^12; the radix-2 stages and the scaling pass stride over their points with at most one
+  // round of threads, so the table stays below ~0.43 GB; BLS12-381: 3)
+  const uint32_t tab_threads = (uint32_t)std::max<uint64_t>(scr_pts, std::min<uint64_t>(N, round_threads));
+  const uint32_t stage_threads = std::min<uint32_t>(halfN, tab_threads), scale_threads = std::min<uint32_t>(N, tab_threads);
+  ZK_TRY(tab.alloc((size_t)tab_threads * g1_tab_entries<C>() * ZW * 4));
   if (grouped) {
     ZK_TRY(twm.alloc((size_t)(halfN ? halfN : 1) * 32));
     ZK_TRY(scr.alloc(scr_pts * ZW * 4));
-    ZK_TRY(tab.alloc(scr_pts * g1_tab_entries<C>() * ZW * 4));
   }
   auto to_fr = [&](const zkh::Fe &x) {
     Fe<FR> d;
@@ -499,14 +504,14 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
     int s0 = 0;
     for (int r : plan) {
       if (r == 1) {
-        hipLaunchKernelGGL((g1_fft_stage_kernel<C, false>), gH, blk, 0, st, pts.as<int32_t>(), tw.as<uint32_t>(), m,
-                           s0 + 1);
+        hipLaunchKernelGGL((g1_fft_stage_kernel<C, false>), grid(stage_threads), blk, 0, st, pts.as<int32_t>(),
+                           tw.as<uint32_t>(), m, s0 + 1, tab.as<uint32_t>(), stage_threads);
       } else {
         const uint32_t ng = N >> r;
         hipLaunchKernelGGL(g1_group_dit_mul_kernel<C>, grid((uint64_t)ng * (g1_kcount(r) + 1)), blk,
                            0, st,
                            (const int32_t *)pts.as<int32_t>(), scr.as<int32_t>(), twm.as<uint32_t>(), m, s0, r, ng,
-                           tab.as<uint32_t>(), scr_pts);
+                           tab.as<uint32_t>(), tab_threads);
         hipLaunchKernelGGL(g1_group_dit_sum_kernel<C>, grid(((uint64_t)ng << r) * 8), blk, 0, st, pts.as<int32_t>(),
                            (const int32_t *)scr.as<int32_t>(), s0, r, ng);
       }
@@ -521,8 +526,8 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
       const int r = plan[i];
       s0 -= r;
       if (r == 1) {
-        hipLaunchKernelGGL((g1_fft_stage_kernel<C, true>), gH, blk, 0, st, pts.as<int32_t>(), tw.as<uint32_t>(), m,
-                           s0 + 1);
+        hipLaunchKernelGGL((g1_fft_stage_kernel<C, true>), grid(stage_threads), blk, 0, st, pts.as<int32_t>(),
+                           tw.as<uint32_t>(), m, s0 + 1, tab.as<uint32_t>(), stage_threads);
       } else {
         const uint32_t ng = N >> r;
         const int fin = s0 == 0 ? 1 : 0;
@@ -530,14 +535,15 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
         hipLaunchKernelGGL(g1_group_dif_mul_kernel<C>, grid((uint64_t)ng * (g1_kcount(r) + 1)), blk,
                            0, st,
                            (const int32_t *)pts.as<int32_t>(), scr.as<int32_t>(), twm.as<uint32_t>(), m, s0, r, ng,
-                           fin, to_fr(ninv_m), tab.as<uint32_t>(), scr_pts);
+                           fin, to_fr(ninv_m), tab.as<uint32_t>(), tab_threads);
         hipLaunchKernelGGL(g1_group_dif_sum_kernel<C>, grid(((uint64_t)ng << r) * 8), blk, 0, st, pts.as<int32_t>(),
                            (const int32_t *)scr.as<int32_t>(), s0, r, ng);
       }
     }
     if (!folded) {
       const zkh::Fe ninv = zkh::to_std(fr, ninv_m);
-      hipLaunchKernelGGL(g1_scale_kernel<C>, gN, blk, 0, st, pts.as<int32_t>(), N, to_fr(ninv));
+      hipLaunchKernelGGL(g1_scale_kernel<C>, grid(scale_threads), blk, 0, st, pts.as<int32_t>(), N, to_fr(ninv),
+                         tab.as<uint32_t>(), scale_threads);
     }
   }
   hipLaunchKernelGGL(g1_proj_out_kernel<C>, gN, blk, 0, st, pts.as<int32_t>(), N, m, inverse ? 1 : 0,
