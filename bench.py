@@ -407,26 +407,37 @@ def bench_learned_msm(njobs=24, seed=3, nstreams=4):
             "learned_msm_bit_identical": bool(same)}
 
 
-def bench_g1_fft(m=12, cpu_m=7, seed=6):
-    """BN254 G1 FFT (bn128_G1_proj_fft_forward / _inverse) of 2^m points k_i G given as
-    projective (x, y, 1), on the 2^m subgroup generator; the inverse of the forward transform
-    must return the input bit for bit.  CPU: the reference's own C on 2^cpu_m of the points."""
+def _g1_fft_times(curve, m, seed):
+    """Forward and inverse G1 FFT of 2^m points k_i G given as projective (x, y, 1), on the 2^m
+    subgroup generator: (forward s, inverse s, round trip bit-exact, the input words)."""
     N = 1 << m
+    L = 4 if curve == 0 else 6
+    R, Pm = (BN_R, BN_P) if curve == 0 else (BLS_R, BLS_P)
     rng = np.random.default_rng(seed)
-    K = uniform_scalars(rng, N, BN_R)
-    dK = zkgpu.DeviceArray.from_host(K)
-    aff = zkgpu.scalar_mul_gen(0, dK, N).to_host((N, 8))
-    one = np.array([((1 << 256) % BN_P >> (64 * i)) & 0xFFFFFFFFFFFFFFFF for i in range(4)], np.uint64)
+    K = uniform_scalars(rng, N, R)
+    aff = zkgpu.scalar_mul_gen(curve, zkgpu.DeviceArray.from_host(K), N).to_host((N, 2 * L))
+    one = np.array([((1 << (64 * L)) % Pm >> (64 * i)) & 0xFFFFFFFFFFFFFFFF for i in range(L)], np.uint64)
     P = np.concatenate([aff, np.tile(one, (N, 1))], axis=1)
-    g = zkgpu.fft_generator(0, m)
+    g = zkgpu.fft_generator(curve, m)
     dP, dY, dB = zkgpu.DeviceArray.from_host(P), zkgpu.DeviceArray(P.size), zkgpu.DeviceArray(P.size)
-    fwd = lambda: zkgpu.g1_fft_device(0, m, g, dP, dY)
-    inv = lambda: zkgpu.g1_fft_device(0, m, g, dY, dB, inverse=True)
-    tf = time_steps(fwd, 2, 1)
-    ti = time_steps(inv, 2, 1)
-    ok = bool((dB.to_host(P.shape) == P).all())
+    tf = time_steps(lambda: zkgpu.g1_fft_device(curve, m, g, dP, dY), 2, 1)
+    ti = time_steps(lambda: zkgpu.g1_fft_device(curve, m, g, dY, dB, inverse=True), 2, 1)
+    return tf, ti, bool((dB.to_host(P.shape) == P).all()), P
+
+
+def bench_g1_fft(m=12, cpu_m=7, seed=6, big_m=16):
+    """BN254 G1 FFT (bn128_G1_proj_fft_forward / _inverse) of 2^m points; the inverse of the
+    forward transform must return the input bit for bit.  Also BN254 at 2^big_m (throughput-bound
+    stage plans) and BLS12-381 at 2^m.  CPU: the reference's own C on 2^cpu_m of the points."""
+    N = 1 << m
+    tf, ti, ok, P = _g1_fft_times(0, m, seed)
     out = {"g1_fft_bn254_log": m, "g1_fft_fwd_ms": round(tf * 1e3, 2), "g1_fft_inv_ms": round(ti * 1e3, 2),
            "g1_fft_fwd_mpts": round(N / tf / 1e6, 4), "g1_fft_roundtrip_ok": ok}
+    tfb, tib, okb, _ = _g1_fft_times(0, big_m, seed + 1)
+    tfl, til, okl, _ = _g1_fft_times(1, m, seed + 2)
+    out.update({f"g1_fft_bn254_2p{big_m}_fwd_ms": round(tfb * 1e3, 2), f"g1_fft_bn254_2p{big_m}_inv_ms": round(tib * 1e3, 2),
+                f"g1_fft_bls12_381_2p{m}_fwd_ms": round(tfl * 1e3, 2), f"g1_fft_bls12_381_2p{m}_inv_ms": round(til * 1e3, 2),
+                "g1_fft_more_roundtrips_ok": okb and okl})
     ref_so = REPO / "oracle" / "_ref" / "libzkref.so"
     if ref_so.exists():
         lib = ctypes.CDLL(str(ref_so))
