@@ -65,6 +65,10 @@ struct PassArgs {
   int has_ninv;             // final pass: the factor is not one (inverse, or a coset post-scale)
   int nsplit;               // gather pass: number of digits of rho (passes p < P-1)
   int split[6];             // their log sizes, s_0 ... s_{P-2}
+  // gather pass of an inverse transform (or null): omega^(e) N^-1 for the next pass's twiddle
+  // exponents e (multiples of 2^nx_logr below N/2), so that N^-1 rides on that product and the
+  // final pass only reduces
+  const uint32_t *tw_ninv;
 };
 
 // ---- reduced-radix butterflies (fz.hpp).  Data stays in the reference's Montgomery form
@@ -322,7 +326,15 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t blk, i
       const uint64_t col = go & ((1ull << a.nx_logc) - 1);
       const uint64_t nn = (go >> a.nx_logc) & ((1ull << a.nx_s) - 1);
       E w;
-      neg = twiddle_z(w, a.tw, (col * nn) << a.nx_logr, a.logn, a.tw_shift, a.inverse);
+      if (a.tw_ninv) {
+        const uint64_t N = 1ull << a.logn, half = N >> 1;
+        uint64_t e = (N - ((col * nn) << a.nx_logr)) & (N - 1);  // inverse: omega^-e
+        neg = e >= half;
+        if (neg) e -= half;
+        zunpack_g(w, a.tw_ninv + (e >> a.nx_logr) * 8);
+      } else {
+        neg = twiddle_z(w, a.tw, (col * nn) << a.nx_logr, a.logn, a.tw_shift, a.inverse);
+      }
       zmul(x, x, w);
     } else if (a.has_ninv) {
       E f = ninv;
@@ -622,6 +634,73 @@ static int get_twiddles(int curve, int m, const uint64_t *gen, const uint32_t **
   return ZK_OK;
 }
 
+// out[i] = omega^(i 2^step) N^-1 (internal form, canonical), i < cnt, from the twiddle table tw
+// (entries omega^j R' of a table shifted by tw_shift) and N^-1 R' (fac)
+template <class F>
+__global__ void ninv_twiddle_kernel(uint32_t *__restrict__ out, const uint32_t *__restrict__ tw, int step_shift,
+                                    uint64_t cnt, Fe<F> fac) {
+  using Z = typename ZOf<F>::T;
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= cnt) return;
+  Fz<Z> w, f;
+  zunpack_g(w, tw + ((i << step_shift) * 8));
+  zunpack(f, fac.w);
+  zmul(w, w, f);
+  zcanon(w, w);
+  zpack_g(out + i * 8, w);
+}
+
+struct NinvEntry {
+  int curve, m, step;
+  zkh::Fe gen;
+  DevBuf buf;
+};
+static std::vector<NinvEntry *> g_ninv;  // under g_tw_lock
+static constexpr uint64_t kNinvMaxEntries = 1ull << 18;
+
+// the N^-1-scaled twiddles of an inverse transform's gather pass (see PassArgs::tw_ninv), cached
+// per (curve, m, generator, step); null (no fold) when the table would exceed kNinvMaxEntries
+template <class F>
+static int get_ninv_twiddles(int curve, int m, const uint64_t *gen, int step, const uint32_t *tw, int tw_shift,
+                             const uint32_t *ninv_words, const uint32_t **out, hipStream_t st) {
+  *out = nullptr;
+  if (m < 1 || step >= m) return ZK_OK;
+  const uint64_t cnt = (1ull << (m - 1)) >> step;
+  if (cnt > kNinvMaxEntries) return ZK_OK;
+  const zkh::FieldDesc &f = fr_desc(curve);
+  const zkh::Fe g = zkh::load(f, gen);
+  std::lock_guard<std::mutex> lk(g_tw_lock);
+  for (NinvEntry *e : g_ninv)
+    if (e->curve == curve && e->m == m && e->step == step && zkh::is_eq(f, e->gen, g)) {
+      *out = e->buf.as<uint32_t>();
+      return ZK_OK;
+    }
+  NinvEntry *e = new NinvEntry();
+  e->curve = curve;
+  e->m = m;
+  e->step = step;
+  e->gen = g;
+  if (e->buf.alloc(cnt * F::N * 4) != ZK_OK) {
+    delete e;
+    return ZK_ENOMEM;
+  }
+  Fe<F> fac;
+  for (int j = 0; j < F::N; j++) fac.w[j] = ninv_words[j];
+  hipLaunchKernelGGL(ninv_twiddle_kernel<F>, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, st,
+                     e->buf.as<uint32_t>(), tw, step + tw_shift, cnt, fac);
+  hipError_t err = hipGetLastError();
+  if (err == hipSuccess) err = hipStreamSynchronize(st);  // complete before other streams may use it
+  if (err != hipSuccess) {
+    (void)hipStreamSynchronize(st);
+    delete e;
+    set_error(std::string("ntt: ") + hipGetErrorString(err));
+    return ZK_EHIP;
+  }
+  g_ninv.push_back(e);
+  *out = e->buf.as<uint32_t>();
+  return ZK_OK;
+}
+
 // coset tables in internal form: tab[k][j] = (eta^(2^(11 k)))^j R', k < 3, j < 2048; the three
 // bases are raised on the host, so each entry is at most 11 squarings away from its base
 template <class F>
@@ -710,7 +789,8 @@ NttPlan ntt_plan(int m, int smax, int tile_log) {
 template <class F>
 static int ntt_passes(const NttPlan &plan, const uint32_t *d_in, uint32_t *buf, uint32_t *d_out, const uint32_t *tw,
                       int tw_shift, int inverse, const uint32_t *eta_pre, const uint32_t *eta_post,
-                      const uint32_t *last_fac, PassArgs *out, uint64_t *groups, size_t *lds_bytes) {
+                      const uint32_t *last_fac, PassArgs *out, uint64_t *groups, size_t *lds_bytes,
+                      const uint32_t *tw_ninv = nullptr) {
   const int m = plan.m, P = plan.npass;
   if (P > 7) {
     set_error("ntt: at most 7 passes");
@@ -736,8 +816,12 @@ static int ntt_passes(const NttPlan &plan, const uint32_t *d_in, uint32_t *buf, 
     a.inverse = inverse;
     a.eta_pre = a.gather ? eta_pre : nullptr;
     a.eta_post = a.fin ? eta_post : nullptr;
+    // with tw_ninv (inverse, several passes) N^-1 rides on the gather pass's store twiddle and
+    // last_fac is one
+    const bool folded = tw_ninv && inverse && P > 1;
+    if (a.gather && folded) a.tw_ninv = tw_ninv;
     if (a.fin) {
-      a.has_ninv = inverse || eta_post;
+      a.has_ninv = (inverse && !folded) || eta_post;
       for (int j = 0; j < 8; j++) a.ninv[j] = last_fac[j];
     } else {
       a.nx_s = plan.s[p - 1];
@@ -764,7 +848,8 @@ static int ntt_passes(const NttPlan &plan, const uint32_t *d_in, uint32_t *buf, 
 template <class F>
 static int ntt_fast(const NttPlan &plan, const uint32_t *d_in, uint32_t *d_out, uint32_t *d_tmp,
                     const uint32_t *tw, int tw_shift, int inverse, const uint32_t *eta_pre,
-                    const uint32_t *eta_post, const uint32_t *last_fac, hipStream_t st) {
+                    const uint32_t *eta_post, const uint32_t *last_fac, hipStream_t st,
+                    const uint32_t *tw_ninv = nullptr) {
   uint32_t *buf = (d_in == d_out && plan.npass > 1) ? d_tmp : d_out;
   if (!buf) {
     set_error("ntt: an in-place multi-pass transform needs a scratch vector");
@@ -774,7 +859,7 @@ static int ntt_fast(const NttPlan &plan, const uint32_t *d_in, uint32_t *d_out, 
   uint64_t groups[8];
   size_t lds[8];
   ZK_TRY(ntt_passes<F>(plan, d_in, buf, d_out, tw, tw_shift, inverse, eta_pre, eta_post, last_fac, args, groups,
-                       lds));
+                       lds, tw_ninv));
   for (int q = 0; q < plan.npass; q++) {
     hipLaunchKernelGGL(ntt_pass_kernel<F>, dim3((unsigned)groups[q]), dim3(kNttThreads), lds[q], st, args[q]);
     ZK_HIP(hipGetLastError());
@@ -899,8 +984,17 @@ static int ntt_device_t(int curve, int m, const uint64_t *gen, const uint64_t *e
   if (d_in == d_out && plan.npass > 1) ZK_TRY(tmp.alloc(N * F::N * 4));  // in place: the gather pass needs it
   const uint32_t *ep = (eta && !inverse) ? etab.as<uint32_t>() : nullptr;
   const uint32_t *eo = (eta && inverse) ? etab.as<uint32_t>() : nullptr;
+  // inverse over several passes: N^-1 rides on the gather pass's store twiddle (its next pass
+  // views the vector with logr = s_0 + ... + s_{P-3}), and the final pass only reduces
+  const uint32_t *tw_ninv = nullptr;
+  if (inverse && plan.npass > 1) {
+    int step = 0;
+    for (int i = 0; i + 2 < plan.npass; i++) step += plan.s[i];
+    ZK_TRY(get_ninv_twiddles<F>(curve, m, gen, step, tw, tw_shift, lastf, &tw_ninv, st));
+    if (tw_ninv) last_factor<F>(lastf, curve, m, false);  // one: only a coset post-scale remains
+  }
   ZK_TRY(ntt_fast<F>(plan, d_in, d_out, tmp.as<uint32_t>(), tw, tw_shift, inverse, ep, eo,
-                     lastf, st));
+                     lastf, st, tw_ninv));
   ZK_HIP(hipStreamSynchronize(st));  // tmp and the eta tables are released on return
   drain.release();
   return ZK_OK;
