@@ -1096,12 +1096,24 @@ static int ntt_batch_create_t(int curve, int njobs, const NttBatchJob *jobs, Ntt
     }
     uint32_t lastf[8];
     last_factor<F>(lastf, curve, m, inverse);
+    // inverse over several passes: N^-1 on the gather pass's store twiddle, as for one transform
+    const uint32_t *tw_ninv = nullptr;
+    if (inverse && pj.npass > 1) {
+      int step = 0;
+      for (int i = 0; i + 2 < pj.npass; i++) step += pj.s[i];
+      zkh::Fe gm = gmax;
+      for (int i = m; i < mmax; i++) gm = zkh::sqr(f, gm);
+      uint64_t gmw[4];
+      zkh::store(f, gmw, gm);
+      ZK_TRY(get_ninv_twiddles<F>(curve, m, gmw, step, tw, shift0 + (mmax - m), lastf, &tw_ninv, st));
+      if (tw_ninv) last_factor<F>(lastf, curve, m, false);
+    }
     PassArgs args[8];
     uint64_t groups[8];
     size_t lds[8];
     ZK_TRY(ntt_passes<F>(pj, (const uint32_t *)jobs[j].d_src, (uint32_t *)jobs[j].d_tgt, (uint32_t *)jobs[j].d_tgt,
                          tw, shift0 + (mmax - m), inverse, (!inverse) ? etab : nullptr, inverse ? etab : nullptr,
-                         lastf, args, groups, lds));
+                         lastf, args, groups, lds, tw_ninv));
     for (int q = 0; q < pj.npass; q++) {
       if ((int)rounds.size() <= q) {
         rounds.resize(q + 1);
