@@ -10,8 +10,9 @@
 // normalised point is unique, so the outputs match bit for bit.
 //
 // Device form: points in XYZZ reduced-radix coordinates (one record per point, in HBM),
-// twiddles as standard-form scalars; each butterfly multiplies one point by its twiddle
-// (double-and-add over the scalar's bits) and adds / subtracts.
+// twiddles as standard-form scalars; each butterfly multiplies one point by its twiddle (2-bit
+// windows over a per-thread table of small multiples: the joint GLV digits for BN254, the
+// scalar's own for BLS12-381; see table_scalar_mul) and adds / subtracts.
 //
 // A scalar multiplication is a chain of ~127 (BN254, GLV) or ~255 dependent doublings, so a
 // stage of a small transform is one chain long however few threads it has, and one launch per
