@@ -98,6 +98,25 @@ static void run_curve(int op, int n, const int32_t *a, const int32_t *b, int32_t
       memcpy(sc, b + k * 8, sizeof sc);
       uint32_t tab[3 * ZLayout<C>::XYZZ];
       zxyzz_scalar_mul_w2(r, p, sc, tab, 0, 1);
+    } else if (op == 7 || op == 8) {  // the windows above and below the G1 FFT split point, apart, added
+      uint32_t sc[8];
+      memcpy(sc, b + k * 8, sizeof sc);
+      XyzzZ<C> hi, lo;
+      if (op == 7) {
+        if constexpr (C::id == 0) {
+          alignas(16) uint32_t tab[kGlvW2Entries * ZLayout<C>::XYZZ];
+          zxyzz_scalar_mul_glv_w2(hi, p, sc, tab, 44, 64);
+          zxyzz_scalar_mul_glv_w2(lo, p, sc, tab, 0, 44);
+        } else {
+          zxyzz_set_inf(hi);
+          zxyzz_set_inf(lo);
+        }
+      } else {
+        uint32_t tab[3 * ZLayout<C>::XYZZ];
+        zxyzz_scalar_mul_w2(hi, p, sc, tab, 0, 1, 88, 128);
+        zxyzz_scalar_mul_w2(lo, p, sc, tab, 0, 1, 0, 88);
+      }
+      zxyzz_add(r, hi, lo);
     } else {
       zxyzz_dbl(r, p);
     }

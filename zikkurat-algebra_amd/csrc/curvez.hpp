@@ -315,9 +315,11 @@ ZK_HD void zxyzz_soa_load(XyzzZ<C> &a, const uint32_t *__restrict__ tab, int e, 
 // The same product by 2-bit windows over a 3-entry word-major table (p, 2p, 3p) of thread t: a
 // step is two doublings and one addition, ~127 additions for a 255-bit scalar where the binary
 // form, whose wavefronts add at almost every bit (one lane with a set bit suffices), pays ~255.
+// With [wlo, whi) only those windows are added (the doublings run down to window 0), so the
+// results over a split of [0, 128) sum to k p.
 template <class C>
 ZK_HD void zxyzz_scalar_mul_w2(XyzzZ<C> &r, const XyzzZ<C> &p, const uint32_t (&k)[8], uint32_t *__restrict__ tab,
-                               uint64_t t, uint64_t stride) {
+                               uint64_t t, uint64_t stride, int wlo = 0, int whi = 128) {
   {
     XyzzZ<C> e2, e3;
     zxyzz_dbl(e2, p);
@@ -327,15 +329,15 @@ ZK_HD void zxyzz_scalar_mul_w2(XyzzZ<C> &r, const XyzzZ<C> &p, const uint32_t (&
     zxyzz_soa_store<C>(tab, 2, t, stride, e3);
   }
   auto digit = [&](int s) -> uint32_t { return (k[s >> 4] >> (2 * (s & 15))) & 3u; };
-  int top = 127;
-  while (top >= 0 && digit(top) == 0) top--;
-  if (top < 0) {
+  int top = whi - 1;
+  while (top >= wlo && digit(top) == 0) top--;
+  if (top < wlo) {
     zxyzz_set_inf(r);
     return;
   }
   zxyzz_soa_load<C>(r, tab, (int)digit(top) - 1, t, stride);
   for (int s = top - 1; s >= 0; s--) {
-    const uint32_t d = digit(s);
+    const uint32_t d = s >= wlo ? digit(s) : 0u;
     XyzzZ<C> e, u;
     if (d) zxyzz_soa_load<C>(e, tab, (int)d - 1, t, stride);  // in flight during the doublings
     zxyzz_dbl(u, r);

@@ -40,6 +40,8 @@
 namespace zk {
 
 static constexpr int kG1Threads = 128;
+// dynamic LDS of a split group launch: two workgroups (four wavefronts) per CU at most
+static constexpr size_t kG1SplitLds = 64 * 1024;
 
 __device__ __forceinline__ uint32_t g1_bitrev(uint32_t x, int bits) {
   return bits ? (__builtin_bitreverse32(x) >> (32 - bits)) : 0u;
@@ -140,13 +142,24 @@ template <class C>
 __host__ __device__ constexpr int g1_tab_entries() {
   return C::id == 0 ? kGlvW2Entries : 3;
 }
+// part 0: the whole product; parts 1 and 2: the windows above and below a split point, whose
+// results sum to it.  The high part still doubles down to window 0, so the point balances the
+// two chains: BN254 windows [44, 64) (128 doublings, ~20 additions) against [0, 44) (88, ~44);
+// BLS12-381 [88, 128) against [0, 88).
+template <class C>
+__host__ __device__ constexpr int g1_split_window() {
+  return C::id == 0 ? 44 : 88;
+}
 template <class C>
 __device__ __forceinline__ void table_scalar_mul(XyzzZ<C> &r, const XyzzZ<C> &p, const uint32_t (&k)[8],
-                                                 uint32_t *__restrict__ tab, uint64_t t, uint64_t stride) {
+                                                 uint32_t *__restrict__ tab, uint64_t t, uint64_t stride,
+                                                 int part = 0) {
+  constexpr int S = g1_split_window<C>();
   if constexpr (C::id == 0)
-    zxyzz_scalar_mul_glv_w2(r, p, k, tab + t * (uint64_t)(g1_tab_entries<C>() * ZLayout<C>::XYZZ));
+    zxyzz_scalar_mul_glv_w2(r, p, k, tab + t * (uint64_t)(g1_tab_entries<C>() * ZLayout<C>::XYZZ),
+                            part == 1 ? S : 0, part == 2 ? S : 64);
   else
-    zxyzz_scalar_mul_w2(r, p, k, tab, t, stride);
+    zxyzz_scalar_mul_w2(r, p, k, tab, t, stride, part == 1 ? S : 0, part == 2 ? S : 128);
 }
 
 // one radix-2 stage over blocks of 2^s points (DIT: u + w v, u - w v; DIF: u + v, w (u - v)),
@@ -292,24 +305,36 @@ __device__ __forceinline__ void g1_sum8(XyzzZ<C> &acc, uint32_t nterm, uint32_t 
   }
 }
 
+// Split launches (small transforms, where one thread per slot leaves most SIMDs idle): threads
+// [0, T0) take the high windows of slot tid and threads [T0, 2 T0) the low windows of slot
+// tid - T0 (T0 = ngroups (K + 1)), in separate wavefronts, so the longest chain is about a third
+// shorter; the halves land in planes 0 and 1 of scr and the sum kernels add them.  The launch
+// reserves LDS it does not use so that no SIMD holds two of these wavefronts (a wavefront
+// sharing its SIMD runs its chain at most half as fast).  Without the split, threads [0, T0)
+// take whole products into plane 0.
+
 // DIT: slot k < K of group G = M(q, L) x_q; slot K = x_0
 template <class C>
 __global__ __launch_bounds__(kG1Threads) void g1_group_dit_mul_kernel(const int32_t *__restrict__ pts,
                                                                      int32_t *__restrict__ scr,
                                                                      const uint32_t *__restrict__ twm, int m, int s0,
                                                                      int r, uint32_t ngroups, uint32_t *__restrict__ tab,
-                                                                     uint64_t stride) {
+                                                                     uint64_t stride, int split) {
   using FR = typename C::Fr;
   constexpr int PW = ZLayout<C>::XYZZ;
   const uint32_t K = g1_kcount(r);
+  const uint64_t T0 = (uint64_t)ngroups * (K + 1);
   const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= (uint64_t)ngroups * (K + 1)) return;
-  const uint32_t G = (uint32_t)(tid / (K + 1)), k = (uint32_t)(tid % (K + 1));
+  if (tid >= (split ? 2 : 1) * T0) return;
+  const int part = split ? (tid < T0 ? 1 : 2) : 0;
+  const uint64_t slot = tid < T0 ? tid : tid - T0;
+  const uint32_t G = (uint32_t)(slot / (K + 1)), k = (uint32_t)(slot % (K + 1));
   const uint32_t p0 = G & ((1u << s0) - 1), ghi = G >> s0;
   const uint64_t b = ((uint64_t)ghi << (s0 + r)) | p0;
   XyzzZ<C> x;
-  if (k == K) {
-    zxyzz_load(x, pts + b * PW);
+  if (k == K) {  // x_0 in plane 0, infinity in plane 1
+    if (part == 2) zxyzz_set_inf(x);
+    else zxyzz_load(x, pts + b * PW);
     zxyzz_store(scr + tid * PW, x);
     return;
   }
@@ -320,7 +345,7 @@ __global__ __launch_bounds__(kG1Threads) void g1_group_dit_mul_kernel(const int3
   g1_path_coeff<C>(c, twm, m, s0, p0, q, L, h, false, c);
   zxyzz_load(x, pts + (b + ((uint64_t)q << s0)) * PW);
   XyzzZ<C> y;
-  table_scalar_mul<C>(y, x, c.w, tab, tid, stride);
+  table_scalar_mul<C>(y, x, c.w, tab, tid, stride, part);
   zxyzz_store(scr + tid * PW, y);
 }
 
@@ -328,25 +353,35 @@ __global__ __launch_bounds__(kG1Threads) void g1_group_dit_mul_kernel(const int3
 template <class C>
 __global__ __launch_bounds__(kG1Threads) void g1_group_dit_sum_kernel(int32_t *__restrict__ pts,
                                                                      const int32_t *__restrict__ scr, int s0, int r,
-                                                                     uint32_t ngroups) {
+                                                                     uint32_t ngroups, int split) {
   constexpr int PW = ZLayout<C>::XYZZ;
   const uint32_t K = g1_kcount(r), Q = 1u << r;
+  const uint64_t T0 = (uint64_t)ngroups * (K + 1);
   const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t out = tid >> 3;
   const uint32_t lane = (uint32_t)(tid & 7);
   const bool live = out < (uint64_t)ngroups * Q;  // whole 8-lane groups are live or not
   const uint32_t G = live ? (uint32_t)(out / Q) : 0u, qq = live ? (uint32_t)(out % Q) : 0u;
   const int32_t *g = scr + (uint64_t)G * (K + 1) * PW;
+  // slot k: plane 0, plus plane 1 when the products were split
+  auto slot = [&](uint32_t k, XyzzZ<C> &t) {
+    zxyzz_load(t, g + (uint64_t)k * PW);
+    if (split) {
+      XyzzZ<C> u;
+      zxyzz_load(u, g + (T0 + k) * PW);
+      zxyzz_add(t, t, u);
+    }
+  };
   // term 0: x_0; term q >= 1: slot (q, q' mod 2^h(q)) with its sign
   const uint32_t nterm = live ? Q : 0u;
   XyzzZ<C> acc;
   g1_sum8<C>(acc, nterm, lane, [&](uint32_t q, XyzzZ<C> &t) {
     if (q == 0) {
-      zxyzz_load(t, g + (uint64_t)K * PW);
+      slot(K, t);
       return;
     }
     const int h = 31 - __clz(q);
-    zxyzz_load(t, g + (uint64_t)g1_slot(h, q, qq & ((1u << h) - 1)) * PW);
+    slot(g1_slot(h, q, qq & ((1u << h) - 1)), t);
     if (__popc(q & qq) & 1) zxyzz_neg(t, t);
   });
   if (live && lane == 0) {
@@ -364,13 +399,16 @@ __global__ __launch_bounds__(kG1Threads) void g1_group_dif_mul_kernel(const int3
                                                                      const uint32_t *__restrict__ twm, int m, int s0,
                                                                      int r, uint32_t ngroups, int has_fin,
                                                                      Fe<typename C::Fr> fin, uint32_t *__restrict__ tab,
-                                                                     uint64_t stride) {
+                                                                     uint64_t stride, int split) {
   using FR = typename C::Fr;
   constexpr int PW = ZLayout<C>::XYZZ;
   const uint32_t K = g1_kcount(r), Q = 1u << r;
+  const uint64_t T0 = (uint64_t)ngroups * (K + 1);
   const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= (uint64_t)ngroups * (K + 1)) return;
-  const uint32_t G = (uint32_t)(tid / (K + 1)), k = (uint32_t)(tid % (K + 1));
+  if (tid >= (split ? 2 : 1) * T0) return;
+  const int part = split ? (tid < T0 ? 1 : 2) : 0;
+  const uint64_t slot = tid < T0 ? tid : tid - T0;
+  const uint32_t G = (uint32_t)(slot / (K + 1)), k = (uint32_t)(slot % (K + 1));
   const uint32_t p0 = G & ((1u << s0) - 1), ghi = G >> s0;
   const uint64_t b = ((uint64_t)ghi << (s0 + r)) | p0;
   XyzzZ<C> s, t;
@@ -392,12 +430,13 @@ __global__ __launch_bounds__(kG1Threads) void g1_group_dif_mul_kernel(const int3
   } else if (has_fin) {
     c = fin;
     fto_std(c, c);
-  } else {
+  } else {  // the sum in plane 0, infinity in plane 1
+    if (part == 2) zxyzz_set_inf(s);
     zxyzz_store(scr + tid * PW, s);
     return;
   }
   XyzzZ<C> y;
-  table_scalar_mul<C>(y, s, c.w, tab, tid, stride);
+  table_scalar_mul<C>(y, s, c.w, tab, tid, stride, part);
   zxyzz_store(scr + tid * PW, y);
 }
 
@@ -405,9 +444,10 @@ __global__ __launch_bounds__(kG1Threads) void g1_group_dif_mul_kernel(const int3
 template <class C>
 __global__ __launch_bounds__(kG1Threads) void g1_group_dif_sum_kernel(int32_t *__restrict__ pts,
                                                                      const int32_t *__restrict__ scr, int s0, int r,
-                                                                     uint32_t ngroups) {
+                                                                     uint32_t ngroups, int split) {
   constexpr int PW = ZLayout<C>::XYZZ;
   const uint32_t K = g1_kcount(r), Q = 1u << r;
+  const uint64_t T0 = (uint64_t)ngroups * (K + 1);
   const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t out = tid >> 3;
   const uint32_t lane = (uint32_t)(tid & 7);
@@ -420,6 +460,11 @@ __global__ __launch_bounds__(kG1Threads) void g1_group_dif_sum_kernel(int32_t *_
   g1_sum8<C>(acc, nterm, lane, [&](uint32_t L, XyzzZ<C> &t) {
     const uint32_t k = qq ? g1_slot(h, qq, L) : K;
     zxyzz_load(t, g + (uint64_t)k * PW);
+    if (split) {  // plus plane 1
+      XyzzZ<C> u;
+      zxyzz_load(u, g + (T0 + k) * PW);
+      zxyzz_add(t, t, u);
+    }
   });
   if (live && lane == 0) {
     const uint32_t p0 = G & ((1u << s0) - 1), ghi = G >> s0;
@@ -467,12 +512,18 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
   std::vector<int> plan;
   const uint64_t round_threads = (uint64_t)ctx->num_cus * 4 * 3 * 64;
   g1_stage_plan(m, round_threads, plan);
+  // a group launch splits each product over two threads while the doubled launch still gets
+  // one wavefront per SIMD (the chains are latency-bound there).  BLS12-381's mul kernels hold
+  // one wavefront per SIMD by their registers; BN254's hold two, and measured launches of more
+  // than about half as many wavefronts as SIMDs got paired (2^12: 688 wavefronts, 23% slower)
+  const uint64_t split_cap = (uint64_t)ctx->num_cus * 4 * 64 / (C::id == 0 ? 2 : 1);
+  auto group_split = [&](int r) { return 2 * (uint64_t)(N >> r) * (g1_kcount(r) + 1) <= split_cap ? 1 : 0; };
   uint64_t scr_pts = 0;
   bool grouped = false;
   for (int r : plan)
     if (r > 1) {
       grouped = true;
-      scr_pts = std::max<uint64_t>(scr_pts, (uint64_t)(N >> r) * (g1_kcount(r) + 1));
+      scr_pts = std::max<uint64_t>(scr_pts, (uint64_t)(N >> r) * (g1_kcount(r) + 1) * (group_split(r) ? 2 : 1));
     }
   // g1_tab_entries table points per thread of every launch that multiplies (BN254: 15, 47.5 MB
   // at 2^12; the radix-2 stages and the scaling pass stride over their points with at most one
@@ -509,12 +560,13 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
                            tw.as<uint32_t>(), m, s0 + 1, tab.as<uint32_t>(), stage_threads);
       } else {
         const uint32_t ng = N >> r;
-        hipLaunchKernelGGL(g1_group_dit_mul_kernel<C>, grid((uint64_t)ng * (g1_kcount(r) + 1)), blk,
-                           0, st,
+        const int sp = group_split(r);
+        hipLaunchKernelGGL(g1_group_dit_mul_kernel<C>, grid((uint64_t)ng * (g1_kcount(r) + 1) * (sp ? 2 : 1)), blk,
+                           sp ? kG1SplitLds : 0, st,
                            (const int32_t *)pts.as<int32_t>(), scr.as<int32_t>(), twm.as<uint32_t>(), m, s0, r, ng,
-                           tab.as<uint32_t>(), tab_threads);
+                           tab.as<uint32_t>(), tab_threads, sp);
         hipLaunchKernelGGL(g1_group_dit_sum_kernel<C>, grid(((uint64_t)ng << r) * 8), blk, 0, st, pts.as<int32_t>(),
-                           (const int32_t *)scr.as<int32_t>(), s0, r, ng);
+                           (const int32_t *)scr.as<int32_t>(), s0, r, ng, sp);
       }
       s0 += r;
     }
@@ -533,12 +585,13 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
         const uint32_t ng = N >> r;
         const int fin = s0 == 0 ? 1 : 0;
         folded = folded || fin;
-        hipLaunchKernelGGL(g1_group_dif_mul_kernel<C>, grid((uint64_t)ng * (g1_kcount(r) + 1)), blk,
-                           0, st,
+        const int sp = group_split(r);
+        hipLaunchKernelGGL(g1_group_dif_mul_kernel<C>, grid((uint64_t)ng * (g1_kcount(r) + 1) * (sp ? 2 : 1)), blk,
+                           sp ? kG1SplitLds : 0, st,
                            (const int32_t *)pts.as<int32_t>(), scr.as<int32_t>(), twm.as<uint32_t>(), m, s0, r, ng,
-                           fin, to_fr(ninv_m), tab.as<uint32_t>(), tab_threads);
+                           fin, to_fr(ninv_m), tab.as<uint32_t>(), tab_threads, sp);
         hipLaunchKernelGGL(g1_group_dif_sum_kernel<C>, grid(((uint64_t)ng << r) * 8), blk, 0, st, pts.as<int32_t>(),
-                           (const int32_t *)scr.as<int32_t>(), s0, r, ng);
+                           (const int32_t *)scr.as<int32_t>(), s0, r, ng, sp);
       }
     }
     if (!folded) {

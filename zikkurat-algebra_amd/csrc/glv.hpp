@@ -108,9 +108,13 @@ constexpr int kGlvW2Entries = 15;
 // form adds at almost every one of its ~127 steps (a step adds as soon as one lane's bit pair is
 // nonzero), so this saves ~60 additions for 12 on the table (one doubling, two additions, three
 // phi, nine sums).  phi(j P) = phi(P) j, so E(0, j) comes from E(j, 0) by phi and the sign.
+// With [wlo, whi) only the windows in that range are added (the doublings run down to window
+// 0): the sum of the results over a split of [0, 64) is k P, so two threads can take the high
+// and the low windows of one product, the high one 64 doublings and ~32 additions long
+// instead of 128 and ~63.
 template <class C>
 ZK_HD void zxyzz_scalar_mul_glv_w2(XyzzZ<C> &r, const XyzzZ<C> &p, const uint32_t (&k)[8],
-                                   uint32_t *__restrict__ tt) {
+                                   uint32_t *__restrict__ tt, int wlo = 0, int whi = 64) {
   uint64_t k64[4];
 #pragma unroll
   for (int j = 0; j < 4; j++) k64[j] = (uint64_t)k[2 * j] | ((uint64_t)k[2 * j + 1] << 32);
@@ -150,15 +154,15 @@ ZK_HD void zxyzz_scalar_mul_glv_w2(XyzzZ<C> &r, const XyzzZ<C> &p, const uint32_
     const uint32_t d1 = (uint32_t)(m1[b >> 6] >> (b & 63)) & 3u, d2 = (uint32_t)(m2[b >> 6] >> (b & 63)) & 3u;
     return 4 * d2 + d1;
   };
-  int top = 63;
-  while (top >= 0 && digit(top) == 0) top--;
-  if (top < 0) {
+  int top = whi - 1;
+  while (top >= wlo && digit(top) == 0) top--;
+  if (top < wlo) {
     zxyzz_set_inf(r);
     return;
   }
   zxyzz_tab_load<C>(r, tt, (int)digit(top) - 1);
   for (int s = top - 1; s >= 0; s--) {
-    const uint32_t d = digit(s);
+    const uint32_t d = s >= wlo ? digit(s) : 0u;
     XyzzZ<C> e, u;
     if (d) zxyzz_tab_load<C>(e, tt, (int)d - 1);  // in flight during the doublings
     zxyzz_dbl(u, r);
