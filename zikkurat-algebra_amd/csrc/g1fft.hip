@@ -27,6 +27,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "curve.hpp"
@@ -40,8 +41,19 @@
 namespace zk {
 
 static constexpr int kG1Threads = 128;
-// dynamic LDS of a split group launch: two workgroups (four wavefronts) per CU at most
-static constexpr size_t kG1SplitLds = 64 * 1024;
+// split group launches of BN254 (whose mul kernels could hold two wavefronts per SIMD):
+// workgroups of four wavefronts with LDS reserved so that a CU holds one.  BLS12-381's hold one
+// wavefront per SIMD by their registers and launch as the other kernels.
+static constexpr int kG1SplitThreads = 256;
+static constexpr size_t kG1SplitLds = 96 * 1024;
+template <class C>
+__host__ __device__ constexpr int g1_split_threads() {
+  return C::id == 0 ? kG1SplitThreads : kG1Threads;
+}
+template <class C>
+__host__ __device__ constexpr size_t g1_split_lds() {
+  return C::id == 0 ? kG1SplitLds : 0;
+}
 
 __device__ __forceinline__ uint32_t g1_bitrev(uint32_t x, int bits) {
   return bits ? (__builtin_bitreverse32(x) >> (32 - bits)) : 0u;
@@ -309,13 +321,15 @@ __device__ __forceinline__ void g1_sum8(XyzzZ<C> &acc, uint32_t nterm, uint32_t 
 // [0, T0) take the high windows of slot tid and threads [T0, 2 T0) the low windows of slot
 // tid - T0 (T0 = ngroups (K + 1)), in separate wavefronts, so the longest chain is about a third
 // shorter; the halves land in planes 0 and 1 of scr and the sum kernels add them.  The launch
-// reserves LDS it does not use so that no SIMD holds two of these wavefronts (a wavefront
-// sharing its SIMD runs its chain at most half as fast).  Without the split, threads [0, T0)
-// take whole products into plane 0.
+// keeps one wavefront per SIMD (a wavefront sharing its SIMD runs its chain at most half as
+// fast): BN254 through workgroups of four wavefronts and LDS reserved so that a CU holds one,
+// BLS12-381 through its register count.  Without the split, threads [0, T0) take whole products
+// into plane 0.  (With 128-thread workgroups and two per CU, BN254 2^12 measured 4.1 against
+// 2.6 ms: the launch's wavefronts were paired on SIMDs.)
 
 // DIT: slot k < K of group G = M(q, L) x_q; slot K = x_0
 template <class C>
-__global__ __launch_bounds__(kG1Threads) void g1_group_dit_mul_kernel(const int32_t *__restrict__ pts,
+__global__ __launch_bounds__(kG1SplitThreads) void g1_group_dit_mul_kernel(const int32_t *__restrict__ pts,
                                                                      int32_t *__restrict__ scr,
                                                                      const uint32_t *__restrict__ twm, int m, int s0,
                                                                      int r, uint32_t ngroups, uint32_t *__restrict__ tab,
@@ -394,7 +408,7 @@ __global__ __launch_bounds__(kG1Threads) void g1_group_dit_sum_kernel(int32_t *_
 // DIF: slot k < K = M'(q', L) (sum over q mod 2^h = L of (-1)^popcount(q & q') x_q), times
 // `fin` (N^-1) on the last group; slot K = fin (sum of all x_q)
 template <class C>
-__global__ __launch_bounds__(kG1Threads) void g1_group_dif_mul_kernel(const int32_t *__restrict__ pts,
+__global__ __launch_bounds__(kG1SplitThreads) void g1_group_dif_mul_kernel(const int32_t *__restrict__ pts,
                                                                      int32_t *__restrict__ scr,
                                                                      const uint32_t *__restrict__ twm, int m, int s0,
                                                                      int r, uint32_t ngroups, int has_fin,
@@ -512,11 +526,21 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
   std::vector<int> plan;
   const uint64_t round_threads = (uint64_t)ctx->num_cus * 4 * 3 * 64;
   g1_stage_plan(m, round_threads, plan);
-  // a group launch splits each product over two threads while the doubled launch still gets
-  // one wavefront per SIMD (the chains are latency-bound there).  BLS12-381's mul kernels hold
-  // one wavefront per SIMD by their registers; BN254's hold two, and measured launches of more
-  // than about half as many wavefronts as SIMDs got paired (2^12: 688 wavefronts, 23% slower)
-  const uint64_t split_cap = (uint64_t)ctx->num_cus * 4 * 64 / (C::id == 0 ? 2 : 1);
+  // a group launch splits each product over two threads while the doubled launch still gets a
+  // SIMD per wavefront (the chains are latency-bound there)
+  const uint64_t split_cap = (uint64_t)ctx->num_cus * 4 * 64;
+  if (g1_split_lds<C>() > 0) {
+    static std::once_flag lds_once;  // the split launches reserve more than the default 64 KiB
+    static hipError_t lds_err = hipSuccess;
+    std::call_once(lds_once, [] {
+      lds_err = hipFuncSetAttribute((const void *)g1_group_dit_mul_kernel<C>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)g1_split_lds<C>());
+      if (lds_err == hipSuccess)
+        lds_err = hipFuncSetAttribute((const void *)g1_group_dif_mul_kernel<C>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)g1_split_lds<C>());
+    });
+    ZK_HIP(lds_err);
+  }
   auto group_split = [&](int r) { return 2 * (uint64_t)(N >> r) * (g1_kcount(r) + 1) <= split_cap ? 1 : 0; };
   uint64_t scr_pts = 0;
   bool grouped = false;
@@ -552,6 +576,7 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
     hipLaunchKernelGGL((g1_proj_in_kernel<C, false>), gN, blk, 0, st, (const uint32_t *)d_src, N, m, inverse ? 0 : 1,
                        pts.as<int32_t>());
   auto grid = [](uint64_t threads) { return dim3((unsigned)((threads + kG1Threads - 1) / kG1Threads)); };
+  auto grid_n = [](uint64_t threads, int per) { return dim3((unsigned)((threads + per - 1) / per)); };
   if (!inverse) {
     int s0 = 0;
     for (int r : plan) {
@@ -561,8 +586,10 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
       } else {
         const uint32_t ng = N >> r;
         const int sp = group_split(r);
-        hipLaunchKernelGGL(g1_group_dit_mul_kernel<C>, grid((uint64_t)ng * (g1_kcount(r) + 1) * (sp ? 2 : 1)), blk,
-                           sp ? kG1SplitLds : 0, st,
+        hipLaunchKernelGGL(g1_group_dit_mul_kernel<C>,
+                           sp ? grid_n((uint64_t)ng * (g1_kcount(r) + 1) * 2, g1_split_threads<C>())
+                              : grid((uint64_t)ng * (g1_kcount(r) + 1)),
+                           sp ? dim3(g1_split_threads<C>()) : blk, sp ? g1_split_lds<C>() : 0, st,
                            (const int32_t *)pts.as<int32_t>(), scr.as<int32_t>(), twm.as<uint32_t>(), m, s0, r, ng,
                            tab.as<uint32_t>(), tab_threads, sp);
         hipLaunchKernelGGL(g1_group_dit_sum_kernel<C>, grid(((uint64_t)ng << r) * 8), blk, 0, st, pts.as<int32_t>(),
@@ -586,8 +613,10 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
         const int fin = s0 == 0 ? 1 : 0;
         folded = folded || fin;
         const int sp = group_split(r);
-        hipLaunchKernelGGL(g1_group_dif_mul_kernel<C>, grid((uint64_t)ng * (g1_kcount(r) + 1) * (sp ? 2 : 1)), blk,
-                           sp ? kG1SplitLds : 0, st,
+        hipLaunchKernelGGL(g1_group_dif_mul_kernel<C>,
+                           sp ? grid_n((uint64_t)ng * (g1_kcount(r) + 1) * 2, g1_split_threads<C>())
+                              : grid((uint64_t)ng * (g1_kcount(r) + 1)),
+                           sp ? dim3(g1_split_threads<C>()) : blk, sp ? g1_split_lds<C>() : 0, st,
                            (const int32_t *)pts.as<int32_t>(), scr.as<int32_t>(), twm.as<uint32_t>(), m, s0, r, ng,
                            fin, to_fr(ninv_m), tab.as<uint32_t>(), tab_threads, sp);
         hipLaunchKernelGGL(g1_group_dif_sum_kernel<C>, grid(((uint64_t)ng << r) * 8), blk, 0, st, pts.as<int32_t>(),
