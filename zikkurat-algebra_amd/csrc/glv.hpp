@@ -110,8 +110,9 @@ constexpr int kGlvW2Entries = 15;
 // phi, nine sums).  phi(j P) = phi(P) j, so E(0, j) comes from E(j, 0) by phi and the sign.
 // With [wlo, whi) only the windows in that range are added (the doublings run down to window
 // 0): the sum of the results over a split of [0, 64) is k P, so two threads can take the high
-// and the low windows of one product, the high one 64 doublings and ~32 additions long
-// instead of 128 and ~63.
+// and the low windows of one product.  The high part keeps all ~128 doublings and adds only
+// its own windows; the low part starts at its top window, so it is 2 wlo-window doublings
+// shorter: split at window 44, they are ~128 doublings + ~20 additions and ~88 + ~44.
 template <class C>
 ZK_HD void zxyzz_scalar_mul_glv_w2(XyzzZ<C> &r, const XyzzZ<C> &p, const uint32_t (&k)[8],
                                    uint32_t *__restrict__ tt, int wlo = 0, int whi = 64) {
