@@ -87,7 +87,7 @@ static void run_curve(int op, int n, const int32_t *a, const int32_t *b, int32_t
         if (op == 4) {
           zxyzz_scalar_mul_glv(r, p, sc);
         } else {
-          alignas(16) uint32_t tab[kGlvW2Entries * ZLayout<C>::XYZZ];
+          alignas(16) uint32_t tab[kGlvW2Entries * ZLayout<C>::JACC];
           zxyzz_scalar_mul_glv_w2(r, p, sc, tab);
         }
       } else {
@@ -96,7 +96,7 @@ static void run_curve(int op, int n, const int32_t *a, const int32_t *b, int32_t
     } else if (op == 6) {  // 2-bit windows on k over a 3-entry host table
       uint32_t sc[8];
       memcpy(sc, b + k * 8, sizeof sc);
-      uint32_t tab[3 * ZLayout<C>::XYZZ];
+      uint32_t tab[3 * 5 * ZLayout<C>::N];
       zxyzz_scalar_mul_w2(r, p, sc, tab, 0, 1);
     } else if (op == 7 || op == 8) {  // the windows above and below the G1 FFT split point, apart, added
       uint32_t sc[8];
@@ -104,19 +104,38 @@ static void run_curve(int op, int n, const int32_t *a, const int32_t *b, int32_t
       XyzzZ<C> hi, lo;
       if (op == 7) {
         if constexpr (C::id == 0) {
-          alignas(16) uint32_t tab[kGlvW2Entries * ZLayout<C>::XYZZ];
-          zxyzz_scalar_mul_glv_w2(hi, p, sc, tab, 44, 64);
-          zxyzz_scalar_mul_glv_w2(lo, p, sc, tab, 0, 44);
+          alignas(16) uint32_t tab[kGlvW2Entries * ZLayout<C>::JACC];
+          zxyzz_scalar_mul_glv_w2(hi, p, sc, tab, 40, 64);
+          zxyzz_scalar_mul_glv_w2(lo, p, sc, tab, 0, 40);
         } else {
           zxyzz_set_inf(hi);
           zxyzz_set_inf(lo);
         }
       } else {
-        uint32_t tab[3 * ZLayout<C>::XYZZ];
-        zxyzz_scalar_mul_w2(hi, p, sc, tab, 0, 1, 88, 128);
-        zxyzz_scalar_mul_w2(lo, p, sc, tab, 0, 1, 0, 88);
+        uint32_t tab[3 * 5 * ZLayout<C>::N];
+        zxyzz_scalar_mul_w2(hi, p, sc, tab, 0, 1, 80, 128);
+        zxyzz_scalar_mul_w2(lo, p, sc, tab, 0, 1, 0, 80);
       }
       zxyzz_add(r, hi, lo);
+    } else if (op == 9 || op == 10 || op == 11) {  // Jacobian steps, XYZZ in and out: p + q (q cached), 2p, 4p + q
+      JacZ<C> jp, jq;
+      JacCZ<C> cq;
+      zjac_from_xyzz(jp, p);
+      if (op != 10) {
+        memcpy(&q, b + k * 4 * N, sizeof q);
+        zjac_from_xyzz(jq, q);
+        zjac_cache(cq, jq);
+      }
+      if (op == 9) {
+        zjac_add(jp, jp, cq);
+      } else if (op == 10) {
+        zjac_dbl(jp, jp);
+      } else {
+        zjac_dbl(jp, jp);
+        zjac_dbl(jp, jp);
+        zjac_add(jp, jp, cq);
+      }
+      zjac_to_xyzz(r, jp);
     } else {
       zxyzz_dbl(r, p);
     }

@@ -316,6 +316,37 @@ def test_xyzz_formulas(h, C):
 
 
 @pytest.mark.parametrize("C", [BN, BLS], ids=["bn254", "bls12-381"])
+def test_jacobian_formulas(h, C):
+    """The Jacobian steps of the windowed scalar multiplications (csrc/curvez.hpp): XYZZ points
+    in lazy form converted to Jacobian, P + Q with Q a cached entry, 2P and 4P + Q (the outputs
+    fed back in place, as the chains do), converted back to XYZZ, equal the affine results;
+    the pairs include P = Q (the addition doubles), P = -Q (infinity), 4P = -Q, and infinity
+    on either side."""
+    rng = splitmix64(910 + C.id)
+    ks = [random_below(rng, C.r, 4) for _ in range(24)]
+    P = [C.mul(k, C.g) for k in ks]
+    Q = [C.mul(k + 7, C.g) for k in ks]
+    Q[0], Q[1] = P[0], C.neg(P[1])
+    Q[2] = C.neg(C.mul(4, P[2]))
+    for lazy in (False, True):
+        A = xyzz_rows(C, P, rng, lazy)
+        B = xyzz_rows(C, Q, rng, lazy)
+        A[-1, :] = 0  # infinity on the left
+        B[-2, :] = 0  # and on the right
+        Pl, Ql = list(P), list(Q)
+        Pl[-1] = None
+        Ql[-2] = None
+        n = len(P)
+        out = np.zeros_like(A)
+        expect = {9: lambda a, b: C.add(a, b), 10: lambda a, b: C.add(a, a),
+                  11: lambda a, b: C.add(C.mul(4, a) if a is not None else None, b)}
+        for op in (9, 10, 11):
+            h.fzh_curve(C.id, op, n, A.ctypes.data_as(i32p), B.ctypes.data_as(i32p), out.ctypes.data_as(i32p))
+            for i in range(n):
+                assert decode_xyzz(C, out[i]) == expect[op](Pl[i], Ql[i]), (op, lazy, i)
+
+
+@pytest.mark.parametrize("C", [BN, BLS], ids=["bn254", "bls12-381"])
 @pytest.mark.parametrize("op", [3, 6, 8], ids=["binary", "window2", "window2-halves"])
 def test_xyzz_scalar_mul(h, C, op):
     """zxyzz_scalar_mul (double-and-add over the scalar's bits) and zxyzz_scalar_mul_w2 (2-bit

@@ -12,7 +12,7 @@
 // Device form: points in XYZZ reduced-radix coordinates (one record per point, in HBM),
 // twiddles as standard-form scalars; each butterfly multiplies one point by its twiddle (2-bit
 // windows over a per-thread table of small multiples: the joint GLV digits for BN254, the
-// scalar's own for BLS12-381; see table_scalar_mul) and adds / subtracts.
+// scalar's own for BLS12-381; a Jacobian chain, see table_scalar_mul) and adds / subtracts.
 //
 // A scalar multiplication is a chain of ~127 (BN254, GLV) or ~255 dependent doublings, so a
 // stage of a small transform is one chain long however few threads it has, and one launch per
@@ -146,21 +146,27 @@ __global__ __launch_bounds__(kG1Threads) void g1_twiddle_kernel(uint32_t *__rest
 }
 
 // k P for the twiddle products of every kernel below, where thread t of a launch (stride = the
-// table's threads) owns g1_tab_entries<C>() point-table entries of tab: BN254 by 2-bit joint GLV
-// windows over its thread-major block (at t g1_tab_entries XYZZ words; exact for every point of
-// its prime-order G1), BLS12-381, whose G1 has a cofactor, by 2-bit windows on k over the
-// word-major table
+// table's threads) owns g1_tab_entries<C>() cached Jacobian point-table entries of tab: BN254
+// by 2-bit joint GLV windows over its thread-major block (at t g1_tab_words words; exact for
+// every point of its prime-order G1), BLS12-381, whose G1 has a cofactor, by 2-bit windows on
+// k over the word-major table.  The chains run in Jacobian coordinates.
 template <class C>
 __host__ __device__ constexpr int g1_tab_entries() {
   return C::id == 0 ? kGlvW2Entries : 3;
 }
+// u32 words of one thread's table
+template <class C>
+__host__ __device__ constexpr int g1_tab_words() {
+  return C::id == 0 ? kGlvW2Entries * ZLayout<C>::JACC : 3 * 5 * ZLayout<C>::N;
+}
 // part 0: the whole product; parts 1 and 2: the windows above and below a split point, whose
 // results sum to it.  The high part still doubles down to window 0, so the point balances the
-// two chains: BN254 windows [44, 64) (128 doublings, ~20 additions) against [0, 44) (88, ~44);
-// BLS12-381 [88, 128) against [0, 88).
+// two chains: BN254 windows [40, 64) (128 doublings, ~24 additions) against [0, 40) (80, ~40);
+// BLS12-381 [80, 128) (256, ~48) against [0, 80) (160, ~80).  Measured at 2^12 against 38, 42
+// and 45 (76, 84 and 90): 1-5% faster.
 template <class C>
 __host__ __device__ constexpr int g1_split_window() {
-  return C::id == 0 ? 44 : 88;
+  return C::id == 0 ? 40 : 80;
 }
 template <class C>
 __device__ __forceinline__ void table_scalar_mul(XyzzZ<C> &r, const XyzzZ<C> &p, const uint32_t (&k)[8],
@@ -168,8 +174,7 @@ __device__ __forceinline__ void table_scalar_mul(XyzzZ<C> &r, const XyzzZ<C> &p,
                                                  int part = 0) {
   constexpr int S = g1_split_window<C>();
   if constexpr (C::id == 0)
-    zxyzz_scalar_mul_glv_w2(r, p, k, tab + t * (uint64_t)(g1_tab_entries<C>() * ZLayout<C>::XYZZ),
-                            part == 1 ? S : 0, part == 2 ? S : 64);
+    zxyzz_scalar_mul_glv_w2(r, p, k, tab + t * (uint64_t)g1_tab_words<C>(), part == 1 ? S : 0, part == 2 ? S : 64);
   else
     zxyzz_scalar_mul_w2(r, p, k, tab, t, stride, part == 1 ? S : 0, part == 2 ? S : 128);
 }
@@ -549,12 +554,13 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
       grouped = true;
       scr_pts = std::max<uint64_t>(scr_pts, (uint64_t)(N >> r) * (g1_kcount(r) + 1) * (group_split(r) ? 2 : 1));
     }
-  // g1_tab_entries table points per thread of every launch that multiplies (BN254: 15, 47.5 MB
-  // at 2^12; the radix-2 stages and the scaling pass stride over their points with at most one
-  // round of threads, so the table stays below ~0.43 GB; BLS12-381: 3)
+  // g1_tab_words per thread of every launch that multiplies (BN254: 15 cached entries, 2880
+  // bytes, 127 MB at 2^12 with split products; the radix-2 stages and the scaling pass stride
+  // over their points with at most one round of threads, so the table stays below ~0.57 GB;
+  // BLS12-381: 3 entries, 840 bytes)
   const uint32_t tab_threads = (uint32_t)std::max<uint64_t>(scr_pts, std::min<uint64_t>(N, round_threads));
   const uint32_t stage_threads = std::min<uint32_t>(halfN, tab_threads), scale_threads = std::min<uint32_t>(N, tab_threads);
-  ZK_TRY(tab.alloc((size_t)tab_threads * g1_tab_entries<C>() * ZW * 4));
+  ZK_TRY(tab.alloc((size_t)tab_threads * g1_tab_words<C>() * 4));
   if (grouped) {
     ZK_TRY(twm.alloc((size_t)(halfN ? halfN : 1) * 32));
     ZK_TRY(scr.alloc(scr_pts * ZW * 4));

@@ -102,17 +102,19 @@ constexpr int kGlvW2Entries = 15;
 
 // The same product k P with the joint digits of k1, k2 taken two bits at a time, for callers
 // that can give each thread kGlvW2Entries thread-major table entries in device memory (its
-// block tt, kGlvW2Entries * XYZZ words).  A step is two doublings and one addition of
-// E(i, j) = i Q0 + j Q1 (i, j < 4, not both zero; Q0 = +-P, Q1 = +-phi(P) with the signs of k1,
-// k2; entry 4 j + i - 1): at most 63 additions after the first entry.  In a wavefront the 1-bit
-// form adds at almost every one of its ~127 steps (a step adds as soon as one lane's bit pair is
-// nonzero), so this saves ~60 additions for 12 on the table (one doubling, two additions, three
-// phi, nine sums).  phi(j P) = phi(P) j, so E(0, j) comes from E(j, 0) by phi and the sign.
+// block tt, kGlvW2Entries * ZLayout<C>::JACC words).  A step is two doublings and one addition
+// of E(i, j) = i Q0 + j Q1 (i, j < 4, not both zero; Q0 = +-P, Q1 = +-phi(P) with the signs of
+// k1, k2; entry 4 j + i - 1): at most 63 additions after the first entry.  In a wavefront the
+// 1-bit form adds at almost every one of its ~127 steps (a step adds as soon as one lane's bit
+// pair is nonzero), so this saves ~60 additions for 12 on the table (one doubling, two
+// additions, three phi, nine sums).  phi(j P) = phi(P) j, so E(0, j) comes from E(j, 0) by phi
+// and the sign; phi(X, Y, Z) = (beta X, Y, Z) in Jacobian coordinates too.  The accumulator is
+// Jacobian and the entries are cached Jacobian points (curvez.hpp).
 // With [wlo, whi) only the windows in that range are added (the doublings run down to window
 // 0): the sum of the results over a split of [0, 64) is k P, so two threads can take the high
 // and the low windows of one product.  The high part keeps all ~128 doublings and adds only
 // its own windows; the low part starts at its top window, so it is 2 wlo-window doublings
-// shorter: split at window 44, they are ~128 doublings + ~20 additions and ~88 + ~44.
+// shorter: split at window 40, they are ~128 doublings + ~24 additions and ~80 + ~40.
 template <class C>
 ZK_HD void zxyzz_scalar_mul_glv_w2(XyzzZ<C> &r, const XyzzZ<C> &p, const uint32_t (&k)[8],
                                    uint32_t *__restrict__ tt, int wlo = 0, int whi = 64) {
@@ -124,30 +126,39 @@ ZK_HD void zxyzz_scalar_mul_glv_w2(XyzzZ<C> &r, const XyzzZ<C> &p, const uint32_
   glv_split(k64, m1, n1, m2, n2);
   // E(i, 0) = i Q0, i = 1..3; the sums read them back from the table (fewer live registers)
   {
-    XyzzZ<C> e1 = p, e2, e3;
-    if (n1) zxyzz_neg(e1, e1);
-    zxyzz_dbl(e2, e1);
-    zxyzz_add(e3, e2, e1);
-    zxyzz_tab_store<C>(tt, 0, e1);
-    zxyzz_tab_store<C>(tt, 1, e2);
-    zxyzz_tab_store<C>(tt, 2, e3);
+    JacZ<C> e1, e2, e3;
+    JacCZ<C> c;
+    zjac_from_xyzz(e1, p);
+    zjac_cache(c, e1);
+    if (n1) zjac_neg(c);
+    zjac_uncache(e1, c);
+    zjac_tab_store<C>(tt, 0, c);
+    zjac_dbl(e2, e1);
+    zjac_add(e3, e2, c);
+    zjac_cache(c, e2);
+    zjac_tab_store<C>(tt, 1, c);
+    zjac_cache(c, e3);
+    zjac_tab_store<C>(tt, 2, c);
   }
   Fz<ZBn254Fp> beta;
   zset_const(beta, Bn254Glv::BETA);
 #pragma unroll 1
   for (int j = 1; j <= 3; j++) {
     // E(0, j) = phi(E(j, 0)), negated when the signs of k1 and k2 differ; E(i, j) = E(i, 0) + E(0, j)
-    XyzzZ<C> b;
-    zxyzz_tab_load<C>(b, tt, j - 1);
+    JacCZ<C> b;
+    zjac_tab_load<C>(b, tt, j - 1);
     zmul(b.x, b.x, beta);
-    if (n1 != n2) zxyzz_neg(b, b);
-    zxyzz_tab_store<C>(tt, 4 * j - 1, b);
+    if (n1 != n2) zjac_neg(b);
+    zjac_tab_store<C>(tt, 4 * j - 1, b);
 #pragma unroll 1
     for (int i = 1; i <= 3; i++) {
-      XyzzZ<C> a, s;
-      zxyzz_tab_load<C>(a, tt, i - 1);
-      zxyzz_add(s, a, b);
-      zxyzz_tab_store<C>(tt, 4 * j + i - 1, s);
+      JacCZ<C> a;
+      JacZ<C> s;
+      zjac_tab_load<C>(a, tt, i - 1);
+      zjac_uncache(s, a);
+      zjac_add(s, s, b);
+      zjac_cache(a, s);
+      zjac_tab_store<C>(tt, 4 * j + i - 1, a);
     }
   }
   auto digit = [&](int s) -> uint32_t {  // entry + 1 of window s (0: both digits zero)
@@ -161,15 +172,21 @@ ZK_HD void zxyzz_scalar_mul_glv_w2(XyzzZ<C> &r, const XyzzZ<C> &p, const uint32_
     zxyzz_set_inf(r);
     return;
   }
-  zxyzz_tab_load<C>(r, tt, (int)digit(top) - 1);
+  JacZ<C> acc;
+  {
+    JacCZ<C> e;
+    zjac_tab_load<C>(e, tt, (int)digit(top) - 1);
+    zjac_uncache(acc, e);
+  }
   for (int s = top - 1; s >= 0; s--) {
     const uint32_t d = s >= wlo ? digit(s) : 0u;
-    XyzzZ<C> e, u;
-    if (d) zxyzz_tab_load<C>(e, tt, (int)d - 1);  // in flight during the doublings
-    zxyzz_dbl(u, r);
-    zxyzz_dbl(r, u);
-    if (d) zxyzz_add(r, r, e);
+    JacCZ<C> e;
+    if (d) zjac_tab_load<C>(e, tt, (int)d - 1);  // in flight during the doublings
+    zjac_dbl(acc, acc);
+    zjac_dbl(acc, acc);
+    if (d) zjac_add(acc, acc, e);
   }
+  zjac_to_xyzz(r, acc);
 }
 
 }  // namespace zk
