@@ -117,6 +117,29 @@ static void run_curve(int op, int n, const int32_t *a, const int32_t *b, int32_t
         zxyzz_scalar_mul_w2(lo, p, sc, tab, 0, 1, 0, 80);
       }
       zxyzz_add(r, hi, lo);
+    } else if (op == 12 || op == 13) {  // the three window parts of the G1 FFT's 3-way split, apart, added
+      uint32_t sc[8];
+      memcpy(sc, b + k * 8, sizeof sc);
+      XyzzZ<C> hi, mid, lo, t;
+      if (op == 12) {
+        if constexpr (C::id == 0) {
+          alignas(16) uint32_t tab[kGlvW2Entries * ZLayout<C>::JACC];
+          zxyzz_scalar_mul_glv_w2(hi, p, sc, tab, 52, 64);
+          zxyzz_scalar_mul_glv_w2(mid, p, sc, tab, 33, 52);
+          zxyzz_scalar_mul_glv_w2(lo, p, sc, tab, 0, 33);
+        } else {
+          zxyzz_set_inf(hi);
+          zxyzz_set_inf(mid);
+          zxyzz_set_inf(lo);
+        }
+      } else {
+        uint32_t tab[3 * 5 * ZLayout<C>::N];
+        zxyzz_scalar_mul_w2(hi, p, sc, tab, 0, 1, 104, 128);
+        zxyzz_scalar_mul_w2(mid, p, sc, tab, 0, 1, 65, 104);
+        zxyzz_scalar_mul_w2(lo, p, sc, tab, 0, 1, 0, 65);
+      }
+      zxyzz_add(t, hi, mid);
+      zxyzz_add(r, t, lo);
     } else if (op == 9 || op == 10 || op == 11) {  // Jacobian steps, XYZZ in and out: p + q (q cached), 2p, 4p + q
       JacZ<C> jp, jq;
       JacCZ<C> cq;

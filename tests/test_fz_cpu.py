@@ -347,11 +347,11 @@ def test_jacobian_formulas(h, C):
 
 
 @pytest.mark.parametrize("C", [BN, BLS], ids=["bn254", "bls12-381"])
-@pytest.mark.parametrize("op", [3, 6, 8], ids=["binary", "window2", "window2-halves"])
+@pytest.mark.parametrize("op", [3, 6, 8, 13], ids=["binary", "window2", "window2-halves", "window2-thirds"])
 def test_xyzz_scalar_mul(h, C, op):
     """zxyzz_scalar_mul (double-and-add over the scalar's bits) and zxyzz_scalar_mul_w2 (2-bit
     windows over a (P, 2P, 3P) table: the G1 FFT's BLS12-381 butterflies; whole, and as the sum
-    of its high and low window halves) equal k P for scalars
+    of its window parts for a 2- and a 3-way split) equal k P for scalars
     0, 1, 2, r - 1, 2^255 + 5 and random ones, on points given in lazy XYZZ form (ZZ, ZZZ != 1)
     and on infinity."""
     rng = splitmix64(950 + C.id)
@@ -393,7 +393,7 @@ def test_glv_scalar_mul_forms(h):
     P[-1] = None
     Sw = np.ascontiguousarray(arr([[(s >> (32 * j)) & 0xFFFFFFFF for j in range(8)] for s in S])
                               .astype(np.uint32).view(np.int32))
-    for op in (4, 5, 7):  # 7: the high and low window halves of the 2-bit form, added
+    for op in (4, 5, 7, 12):  # 7, 12: the 2-bit form's window parts of a 2- and a 3-way split, added
         out = np.zeros_like(A)
         h.fzh_curve(C.id, op, len(P), A.ctypes.data_as(i32p), Sw.ctypes.data_as(i32p), out.ctypes.data_as(i32p))
         for i, (pt, k) in enumerate(zip(P, S)):

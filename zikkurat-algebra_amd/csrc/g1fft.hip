@@ -159,24 +159,43 @@ template <class C>
 __host__ __device__ constexpr int g1_tab_words() {
   return C::id == 0 ? kGlvW2Entries * ZLayout<C>::JACC : 3 * 5 * ZLayout<C>::N;
 }
-// part 0: the whole product; parts 1 and 2: the windows above and below a split point, whose
-// results sum to it.  The high part still doubles down to window 0, so the point balances the
-// two chains: BN254 windows [40, 64) (128 doublings, ~24 additions) against [0, 40) (80, ~40);
-// BLS12-381 [80, 128) (256, ~48) against [0, 80) (160, ~80).  Measured at 2^12 against 38, 42
-// and 45 (76, 84 and 90): 1-5% faster.
+// 2-bit windows of a product: 64 joint GLV windows (BN254), 128 windows on k (BLS12-381)
 template <class C>
-__host__ __device__ constexpr int g1_split_window() {
-  return C::id == 0 ? 40 : 80;
+__host__ __device__ constexpr int g1_nwin() {
+  return C::id == 0 ? 64 : 128;
+}
+// Part j of a product split over nparts threads (1 to 3): the windows [wlo, whi) it adds, part 0
+// the highest.  Every part but the lowest still doubles down to window 0, so the split points
+// balance the chains, a window's addition costing about three doublings (measured): for two
+// parts BN254 windows [40, 64) (128 doublings, ~24 additions) and [0, 40) (80, ~40), BLS12-381
+// [80, 128) and [0, 80) (1-5% faster at 2^12 than 38, 42 or 45 / 76, 84 or 90); for three
+// parts [52, 64), [33, 52), [0, 33) and [104, 128), [65, 104), [0, 65) (longest chain about
+// 160 doubling-equivalents for BN254, against 200 with two parts and 320 whole; at 2^12 these
+// points measured 1-2% faster than 50 / 31, 54 / 35 and 52 / 30).
+template <class C>
+__host__ __device__ __forceinline__ void g1_part_windows(int nparts, int j, int &wlo, int &whi) {
+  constexpr int W = g1_nwin<C>();
+  constexpr int A2 = C::id == 0 ? 40 : 80;
+  constexpr int A3 = C::id == 0 ? 52 : 104, B3 = C::id == 0 ? 33 : 65;
+  if (nparts == 3) {
+    wlo = j == 0 ? A3 : (j == 1 ? B3 : 0);
+    whi = j == 0 ? W : (j == 1 ? A3 : B3);
+  } else if (nparts == 2) {
+    wlo = j == 0 ? A2 : 0;
+    whi = j == 0 ? W : A2;
+  } else {
+    wlo = 0;
+    whi = W;
+  }
 }
 template <class C>
 __device__ __forceinline__ void table_scalar_mul(XyzzZ<C> &r, const XyzzZ<C> &p, const uint32_t (&k)[8],
                                                  uint32_t *__restrict__ tab, uint64_t t, uint64_t stride,
-                                                 int part = 0) {
-  constexpr int S = g1_split_window<C>();
+                                                 int wlo = 0, int whi = g1_nwin<C>()) {
   if constexpr (C::id == 0)
-    zxyzz_scalar_mul_glv_w2(r, p, k, tab + t * (uint64_t)g1_tab_words<C>(), part == 1 ? S : 0, part == 2 ? S : 64);
+    zxyzz_scalar_mul_glv_w2(r, p, k, tab + t * (uint64_t)g1_tab_words<C>(), wlo, whi);
   else
-    zxyzz_scalar_mul_w2(r, p, k, tab, t, stride, part == 1 ? S : 0, part == 2 ? S : 128);
+    zxyzz_scalar_mul_w2(r, p, k, tab, t, stride, wlo, whi);
 }
 
 // one radix-2 stage over blocks of 2^s points (DIT: u + w v, u - w v; DIF: u + v, w (u - v)),
@@ -322,15 +341,52 @@ __device__ __forceinline__ void g1_sum8(XyzzZ<C> &acc, uint32_t nterm, uint32_t 
   }
 }
 
-// Split launches (small transforms, where one thread per slot leaves most SIMDs idle): threads
-// [0, T0) take the high windows of slot tid and threads [T0, 2 T0) the low windows of slot
-// tid - T0 (T0 = ngroups (K + 1)), in separate wavefronts, so the longest chain is about a third
-// shorter; the halves land in planes 0 and 1 of scr and the sum kernels add them.  The launch
-// keeps one wavefront per SIMD (a wavefront sharing its SIMD runs its chain at most half as
-// fast): BN254 through workgroups of four wavefronts and LDS reserved so that a CU holds one,
-// BLS12-381 through its register count.  Without the split, threads [0, T0) take whole products
-// into plane 0.  (With 128-thread workgroups and two per CU, BN254 2^12 measured 4.1 against
-// 2.6 ms: the launch's wavefronts were paired on SIMDs.)
+// Split launches (small transforms, where one thread per slot leaves most SIMDs idle): each
+// product is split over nparts threads by windows (g1_part_windows), in separate wavefronts,
+// so the longest chain is shorter; part j lands in plane j of scr and the sum kernels add the
+// planes.  Plane 0 holds the K + 1 slots of every group (T0 = ngroups (K + 1) threads, thread
+// = slot); planes 1.. hold only the slots that are products, k1 a group (K, or K + 1 when
+// slot K is a product too: the inverse's last group), T1 = ngroups k1 threads each; a
+// thread's index is also its record in scr.  The launch keeps one wavefront per SIMD (a
+// wavefront sharing its SIMD runs its chain at most half as fast): BN254 through workgroups
+// of four wavefronts and LDS reserved so that a CU holds one, BLS12-381 through its register
+// count.  (With 128-thread workgroups and two per CU, BN254 2^12 measured 4.1 against 2.6 ms:
+// the launch's wavefronts were paired on SIMDs.)  At 2^12 three parts take 256 x 86 + 2 x 256
+// x 85 = 65536 threads: one wavefront for each of the 1024 SIMDs.
+
+// thread -> (part, group, slot) of a split launch (nparts = 1: thread = slot of plane 0)
+struct G1Slot {
+  int part;
+  uint32_t G, k;
+};
+__device__ __forceinline__ G1Slot g1_thread_slot(uint64_t tid, uint32_t K, uint32_t k1, uint64_t T0, uint64_t T1) {
+  G1Slot r;
+  if (tid < T0) {
+    r.part = 0;
+    r.G = (uint32_t)(tid / (K + 1));
+    r.k = (uint32_t)(tid % (K + 1));
+  } else {
+    const uint64_t u = tid - T0;
+    r.part = 1 + (int)(u / T1);
+    const uint64_t v = u % T1;
+    r.G = (uint32_t)(v / k1);
+    r.k = (uint32_t)(v % k1);
+  }
+  return r;
+}
+// slot k of group G: the sum of its parts over the planes
+template <class C>
+__device__ __forceinline__ void g1_slot_load(XyzzZ<C> &t, const int32_t *__restrict__ scr, uint32_t G, uint32_t k,
+                                             uint32_t K, uint32_t k1, uint64_t T0, uint64_t T1, int nparts) {
+  constexpr int PW = ZLayout<C>::XYZZ;
+  zxyzz_load(t, scr + ((uint64_t)G * (K + 1) + k) * PW);
+  if (k < k1)
+    for (int j = 1; j < nparts; j++) {
+      XyzzZ<C> u;
+      zxyzz_load(u, scr + (T0 + (uint64_t)(j - 1) * T1 + (uint64_t)G * k1 + k) * PW);
+      zxyzz_add(t, t, u);
+    }
+}
 
 // DIT: slot k < K of group G = M(q, L) x_q; slot K = x_0
 template <class C>
@@ -338,22 +394,20 @@ __global__ __launch_bounds__(kG1SplitThreads) void g1_group_dit_mul_kernel(const
                                                                      int32_t *__restrict__ scr,
                                                                      const uint32_t *__restrict__ twm, int m, int s0,
                                                                      int r, uint32_t ngroups, uint32_t *__restrict__ tab,
-                                                                     uint64_t stride, int split) {
+                                                                     uint64_t stride, int nparts) {
   using FR = typename C::Fr;
   constexpr int PW = ZLayout<C>::XYZZ;
   const uint32_t K = g1_kcount(r);
-  const uint64_t T0 = (uint64_t)ngroups * (K + 1);
+  const uint64_t T0 = (uint64_t)ngroups * (K + 1), T1 = (uint64_t)ngroups * K;
   const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= (split ? 2 : 1) * T0) return;
-  const int part = split ? (tid < T0 ? 1 : 2) : 0;
-  const uint64_t slot = tid < T0 ? tid : tid - T0;
-  const uint32_t G = (uint32_t)(slot / (K + 1)), k = (uint32_t)(slot % (K + 1));
+  if (tid >= T0 + (uint64_t)(nparts - 1) * T1) return;
+  const G1Slot sl = g1_thread_slot(tid, K, K, T0, T1);
+  const uint32_t G = sl.G, k = sl.k;
   const uint32_t p0 = G & ((1u << s0) - 1), ghi = G >> s0;
   const uint64_t b = ((uint64_t)ghi << (s0 + r)) | p0;
   XyzzZ<C> x;
-  if (k == K) {  // x_0 in plane 0, infinity in plane 1
-    if (part == 2) zxyzz_set_inf(x);
-    else zxyzz_load(x, pts + b * PW);
+  if (k == K) {  // x_0 (plane 0 only)
+    zxyzz_load(x, pts + b * PW);
     zxyzz_store(scr + tid * PW, x);
     return;
   }
@@ -364,7 +418,9 @@ __global__ __launch_bounds__(kG1SplitThreads) void g1_group_dit_mul_kernel(const
   g1_path_coeff<C>(c, twm, m, s0, p0, q, L, h, false, c);
   zxyzz_load(x, pts + (b + ((uint64_t)q << s0)) * PW);
   XyzzZ<C> y;
-  table_scalar_mul<C>(y, x, c.w, tab, tid, stride, part);
+  int wlo, whi;
+  g1_part_windows<C>(nparts, sl.part, wlo, whi);
+  table_scalar_mul<C>(y, x, c.w, tab, tid, stride, wlo, whi);
   zxyzz_store(scr + tid * PW, y);
 }
 
@@ -372,25 +428,16 @@ __global__ __launch_bounds__(kG1SplitThreads) void g1_group_dit_mul_kernel(const
 template <class C>
 __global__ __launch_bounds__(kG1Threads) void g1_group_dit_sum_kernel(int32_t *__restrict__ pts,
                                                                      const int32_t *__restrict__ scr, int s0, int r,
-                                                                     uint32_t ngroups, int split) {
+                                                                     uint32_t ngroups, int nparts) {
   constexpr int PW = ZLayout<C>::XYZZ;
   const uint32_t K = g1_kcount(r), Q = 1u << r;
-  const uint64_t T0 = (uint64_t)ngroups * (K + 1);
+  const uint64_t T0 = (uint64_t)ngroups * (K + 1), T1 = (uint64_t)ngroups * K;
   const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t out = tid >> 3;
   const uint32_t lane = (uint32_t)(tid & 7);
   const bool live = out < (uint64_t)ngroups * Q;  // whole 8-lane groups are live or not
   const uint32_t G = live ? (uint32_t)(out / Q) : 0u, qq = live ? (uint32_t)(out % Q) : 0u;
-  const int32_t *g = scr + (uint64_t)G * (K + 1) * PW;
-  // slot k: plane 0, plus plane 1 when the products were split
-  auto slot = [&](uint32_t k, XyzzZ<C> &t) {
-    zxyzz_load(t, g + (uint64_t)k * PW);
-    if (split) {
-      XyzzZ<C> u;
-      zxyzz_load(u, g + (T0 + k) * PW);
-      zxyzz_add(t, t, u);
-    }
-  };
+  auto slot = [&](uint32_t k, XyzzZ<C> &t) { g1_slot_load<C>(t, scr, G, k, K, K, T0, T1, nparts); };
   // term 0: x_0; term q >= 1: slot (q, q' mod 2^h(q)) with its sign
   const uint32_t nterm = live ? Q : 0u;
   XyzzZ<C> acc;
@@ -418,16 +465,15 @@ __global__ __launch_bounds__(kG1SplitThreads) void g1_group_dif_mul_kernel(const
                                                                      const uint32_t *__restrict__ twm, int m, int s0,
                                                                      int r, uint32_t ngroups, int has_fin,
                                                                      Fe<typename C::Fr> fin, uint32_t *__restrict__ tab,
-                                                                     uint64_t stride, int split) {
+                                                                     uint64_t stride, int nparts) {
   using FR = typename C::Fr;
   constexpr int PW = ZLayout<C>::XYZZ;
-  const uint32_t K = g1_kcount(r), Q = 1u << r;
-  const uint64_t T0 = (uint64_t)ngroups * (K + 1);
+  const uint32_t K = g1_kcount(r), Q = 1u << r, k1 = has_fin ? K + 1 : K;
+  const uint64_t T0 = (uint64_t)ngroups * (K + 1), T1 = (uint64_t)ngroups * k1;
   const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= (split ? 2 : 1) * T0) return;
-  const int part = split ? (tid < T0 ? 1 : 2) : 0;
-  const uint64_t slot = tid < T0 ? tid : tid - T0;
-  const uint32_t G = (uint32_t)(slot / (K + 1)), k = (uint32_t)(slot % (K + 1));
+  if (tid >= T0 + (uint64_t)(nparts - 1) * T1) return;
+  const G1Slot sl = g1_thread_slot(tid, K, k1, T0, T1);
+  const uint32_t G = sl.G, k = sl.k;
   const uint32_t p0 = G & ((1u << s0) - 1), ghi = G >> s0;
   const uint64_t b = ((uint64_t)ghi << (s0 + r)) | p0;
   XyzzZ<C> s, t;
@@ -449,13 +495,14 @@ __global__ __launch_bounds__(kG1SplitThreads) void g1_group_dif_mul_kernel(const
   } else if (has_fin) {
     c = fin;
     fto_std(c, c);
-  } else {  // the sum in plane 0, infinity in plane 1
-    if (part == 2) zxyzz_set_inf(s);
+  } else {  // the sum (plane 0 only)
     zxyzz_store(scr + tid * PW, s);
     return;
   }
   XyzzZ<C> y;
-  table_scalar_mul<C>(y, s, c.w, tab, tid, stride, part);
+  int wlo, whi;
+  g1_part_windows<C>(nparts, sl.part, wlo, whi);
+  table_scalar_mul<C>(y, s, c.w, tab, tid, stride, wlo, whi);
   zxyzz_store(scr + tid * PW, y);
 }
 
@@ -463,27 +510,20 @@ __global__ __launch_bounds__(kG1SplitThreads) void g1_group_dif_mul_kernel(const
 template <class C>
 __global__ __launch_bounds__(kG1Threads) void g1_group_dif_sum_kernel(int32_t *__restrict__ pts,
                                                                      const int32_t *__restrict__ scr, int s0, int r,
-                                                                     uint32_t ngroups, int split) {
+                                                                     uint32_t ngroups, int nparts, int has_fin) {
   constexpr int PW = ZLayout<C>::XYZZ;
-  const uint32_t K = g1_kcount(r), Q = 1u << r;
-  const uint64_t T0 = (uint64_t)ngroups * (K + 1);
+  const uint32_t K = g1_kcount(r), Q = 1u << r, k1 = has_fin ? K + 1 : K;
+  const uint64_t T0 = (uint64_t)ngroups * (K + 1), T1 = (uint64_t)ngroups * k1;
   const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t out = tid >> 3;
   const uint32_t lane = (uint32_t)(tid & 7);
   const bool live = out < (uint64_t)ngroups * Q;
   const uint32_t G = live ? (uint32_t)(out / Q) : 0u, qq = live ? (uint32_t)(out % Q) : 0u;
-  const int32_t *g = scr + (uint64_t)G * (K + 1) * PW;
   const int h = qq ? 31 - __clz(qq) : 0;
   const uint32_t nterm = live ? (qq ? (1u << h) : 1u) : 0u;
   XyzzZ<C> acc;
   g1_sum8<C>(acc, nterm, lane, [&](uint32_t L, XyzzZ<C> &t) {
-    const uint32_t k = qq ? g1_slot(h, qq, L) : K;
-    zxyzz_load(t, g + (uint64_t)k * PW);
-    if (split) {  // plus plane 1
-      XyzzZ<C> u;
-      zxyzz_load(u, g + (T0 + k) * PW);
-      zxyzz_add(t, t, u);
-    }
+    g1_slot_load<C>(t, scr, G, qq ? g1_slot(h, qq, L) : K, K, k1, T0, T1, nparts);
   });
   if (live && lane == 0) {
     const uint32_t p0 = G & ((1u << s0) - 1), ghi = G >> s0;
@@ -531,8 +571,8 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
   std::vector<int> plan;
   const uint64_t round_threads = (uint64_t)ctx->num_cus * 4 * 3 * 64;
   g1_stage_plan(m, round_threads, plan);
-  // a group launch splits each product over two threads while the doubled launch still gets a
-  // SIMD per wavefront (the chains are latency-bound there)
+  // a group launch splits each product over as many threads (up to three) as still give every
+  // wavefront a SIMD (the chains are latency-bound there)
   const uint64_t split_cap = (uint64_t)ctx->num_cus * 4 * 64;
   if (g1_split_lds<C>() > 0) {
     static std::once_flag lds_once;  // the split launches reserve more than the default 64 KiB
@@ -546,13 +586,23 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
     });
     ZK_HIP(lds_err);
   }
-  auto group_split = [&](int r) { return 2 * (uint64_t)(N >> r) * (g1_kcount(r) + 1) <= split_cap ? 1 : 0; };
+  // threads of a group launch split in S parts (prod_k: slot K is a product, the inverse's last group)
+  auto group_threads = [&](int r, int S, bool prod_k) {
+    const uint64_t ng = N >> r, K = g1_kcount(r);
+    return ng * (K + 1) + (uint64_t)(S - 1) * ng * (K + (prod_k ? 1 : 0));
+  };
+  auto group_parts = [&](int r, bool prod_k) {
+    for (int S = 3; S > 1; S--)
+      if (group_threads(r, S, prod_k) <= split_cap) return S;
+    return 1;
+  };
   uint64_t scr_pts = 0;
   bool grouped = false;
   for (int r : plan)
     if (r > 1) {
       grouped = true;
-      scr_pts = std::max<uint64_t>(scr_pts, (uint64_t)(N >> r) * (g1_kcount(r) + 1) * (group_split(r) ? 2 : 1));
+      for (bool pk : {false, true})
+        scr_pts = std::max<uint64_t>(scr_pts, group_threads(r, group_parts(r, pk), pk));
     }
   // g1_tab_words per thread of every launch that multiplies (BN254: 15 cached entries, 2880
   // bytes, 127 MB at 2^12 with split products; the radix-2 stages and the scaling pass stride
@@ -591,15 +641,15 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
                            tw.as<uint32_t>(), m, s0 + 1, tab.as<uint32_t>(), stage_threads);
       } else {
         const uint32_t ng = N >> r;
-        const int sp = group_split(r);
-        hipLaunchKernelGGL(g1_group_dit_mul_kernel<C>,
-                           sp ? grid_n((uint64_t)ng * (g1_kcount(r) + 1) * 2, g1_split_threads<C>())
-                              : grid((uint64_t)ng * (g1_kcount(r) + 1)),
+        const int S = group_parts(r, false);
+        const bool sp = S > 1;
+        const uint64_t nt = group_threads(r, S, false);
+        hipLaunchKernelGGL(g1_group_dit_mul_kernel<C>, sp ? grid_n(nt, g1_split_threads<C>()) : grid(nt),
                            sp ? dim3(g1_split_threads<C>()) : blk, sp ? g1_split_lds<C>() : 0, st,
                            (const int32_t *)pts.as<int32_t>(), scr.as<int32_t>(), twm.as<uint32_t>(), m, s0, r, ng,
-                           tab.as<uint32_t>(), tab_threads, sp);
+                           tab.as<uint32_t>(), tab_threads, S);
         hipLaunchKernelGGL(g1_group_dit_sum_kernel<C>, grid(((uint64_t)ng << r) * 8), blk, 0, st, pts.as<int32_t>(),
-                           (const int32_t *)scr.as<int32_t>(), s0, r, ng, sp);
+                           (const int32_t *)scr.as<int32_t>(), s0, r, ng, S);
       }
       s0 += r;
     }
@@ -618,15 +668,15 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
         const uint32_t ng = N >> r;
         const int fin = s0 == 0 ? 1 : 0;
         folded = folded || fin;
-        const int sp = group_split(r);
-        hipLaunchKernelGGL(g1_group_dif_mul_kernel<C>,
-                           sp ? grid_n((uint64_t)ng * (g1_kcount(r) + 1) * 2, g1_split_threads<C>())
-                              : grid((uint64_t)ng * (g1_kcount(r) + 1)),
+        const int S = group_parts(r, fin != 0);
+        const bool sp = S > 1;
+        const uint64_t nt = group_threads(r, S, fin != 0);
+        hipLaunchKernelGGL(g1_group_dif_mul_kernel<C>, sp ? grid_n(nt, g1_split_threads<C>()) : grid(nt),
                            sp ? dim3(g1_split_threads<C>()) : blk, sp ? g1_split_lds<C>() : 0, st,
                            (const int32_t *)pts.as<int32_t>(), scr.as<int32_t>(), twm.as<uint32_t>(), m, s0, r, ng,
-                           fin, to_fr(ninv_m), tab.as<uint32_t>(), tab_threads, sp);
+                           fin, to_fr(ninv_m), tab.as<uint32_t>(), tab_threads, S);
         hipLaunchKernelGGL(g1_group_dif_sum_kernel<C>, grid(((uint64_t)ng << r) * 8), blk, 0, st, pts.as<int32_t>(),
-                           (const int32_t *)scr.as<int32_t>(), s0, r, ng, sp);
+                           (const int32_t *)scr.as<int32_t>(), s0, r, ng, S, fin);
       }
     }
     if (!folded) {
