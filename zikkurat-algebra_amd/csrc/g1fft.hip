@@ -7,7 +7,9 @@
 // normalised (X/Z, Y/Z, 1), infinity (0, 1, 0).  For any gen these networks are the same
 // group-linear maps as the iterative forms below (bit-reversed DIT with gen_s = gen^(N/2^s),
 // DIF with gen^-j, and the m halvings of every output path applied once as N^-1), and the
-// normalised point is unique, so the outputs match bit for bit.
+// normalised point is unique, so the outputs match bit for bit.  When gen is a primitive 2^m-th
+// root of unity both networks compute the DFT, and the inverse runs as N^-1 times the forward
+// network by gen^-1 (g1_fft_run).
 //
 // Device form: points in XYZZ reduced-radix coordinates (one record per point, in HBM),
 // twiddles as standard-form scalars; each butterfly multiplies one point by its twiddle (2-bit
@@ -346,7 +348,7 @@ __device__ __forceinline__ void g1_sum8(XyzzZ<C> &acc, uint32_t nterm, uint32_t 
 // so the longest chain is shorter; part j lands in plane j of scr and the sum kernels add the
 // planes.  Plane 0 holds the K + 1 slots of every group (T0 = ngroups (K + 1) threads, thread
 // = slot); planes 1.. hold only the slots that are products, k1 a group (K, or K + 1 when
-// slot K is a product too: the inverse's last group), T1 = ngroups k1 threads each; a
+// slot K is a product too: the group that takes N^-1), T1 = ngroups k1 threads each; a
 // thread's index is also its record in scr.  The launch keeps one wavefront per SIMD (a
 // wavefront sharing its SIMD runs its chain at most half as fast): BN254 through workgroups
 // of four wavefronts and LDS reserved so that a CU holds one, BLS12-381 through its register
@@ -388,35 +390,43 @@ __device__ __forceinline__ void g1_slot_load(XyzzZ<C> &t, const int32_t *__restr
     }
 }
 
-// DIT: slot k < K of group G = M(q, L) x_q; slot K = x_0
+// DIT: slot k < K of group G = M(q, L) x_q; slot K = x_0; with has_fin every slot is also
+// multiplied by fin (N^-1, on the first group of an inverse run as a DIT network), slot K then
+// being a product too
 template <class C>
 __global__ __launch_bounds__(kG1SplitThreads) void g1_group_dit_mul_kernel(const int32_t *__restrict__ pts,
                                                                      int32_t *__restrict__ scr,
                                                                      const uint32_t *__restrict__ twm, int m, int s0,
-                                                                     int r, uint32_t ngroups, uint32_t *__restrict__ tab,
+                                                                     int r, uint32_t ngroups, int has_fin,
+                                                                     Fe<typename C::Fr> fin, uint32_t *__restrict__ tab,
                                                                      uint64_t stride, int nparts) {
   using FR = typename C::Fr;
   constexpr int PW = ZLayout<C>::XYZZ;
-  const uint32_t K = g1_kcount(r);
-  const uint64_t T0 = (uint64_t)ngroups * (K + 1), T1 = (uint64_t)ngroups * K;
+  const uint32_t K = g1_kcount(r), k1 = has_fin ? K + 1 : K;
+  const uint64_t T0 = (uint64_t)ngroups * (K + 1), T1 = (uint64_t)ngroups * k1;
   const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (tid >= T0 + (uint64_t)(nparts - 1) * T1) return;
-  const G1Slot sl = g1_thread_slot(tid, K, K, T0, T1);
+  const G1Slot sl = g1_thread_slot(tid, K, k1, T0, T1);
   const uint32_t G = sl.G, k = sl.k;
   const uint32_t p0 = G & ((1u << s0) - 1), ghi = G >> s0;
   const uint64_t b = ((uint64_t)ghi << (s0 + r)) | p0;
   XyzzZ<C> x;
-  if (k == K) {  // x_0 (plane 0 only)
-    zxyzz_load(x, pts + b * PW);
-    zxyzz_store(scr + tid * PW, x);
-    return;
-  }
-  int h;
-  uint32_t q, L;
-  g1_slot_decode(k, h, q, L);
   Fe<FR> c;
-  g1_path_coeff<C>(c, twm, m, s0, p0, q, L, h, false, c);
-  zxyzz_load(x, pts + (b + ((uint64_t)q << s0)) * PW);
+  if (k == K) {
+    zxyzz_load(x, pts + b * PW);
+    if (!has_fin) {  // x_0 (plane 0 only)
+      zxyzz_store(scr + tid * PW, x);
+      return;
+    }
+    c = fin;
+    fto_std(c, c);
+  } else {
+    int h;
+    uint32_t q, L;
+    g1_slot_decode(k, h, q, L);
+    g1_path_coeff<C>(c, twm, m, s0, p0, q, L, h, has_fin != 0, fin);
+    zxyzz_load(x, pts + (b + ((uint64_t)q << s0)) * PW);
+  }
   XyzzZ<C> y;
   int wlo, whi;
   g1_part_windows<C>(nparts, sl.part, wlo, whi);
@@ -428,16 +438,16 @@ __global__ __launch_bounds__(kG1SplitThreads) void g1_group_dit_mul_kernel(const
 template <class C>
 __global__ __launch_bounds__(kG1Threads) void g1_group_dit_sum_kernel(int32_t *__restrict__ pts,
                                                                      const int32_t *__restrict__ scr, int s0, int r,
-                                                                     uint32_t ngroups, int nparts) {
+                                                                     uint32_t ngroups, int nparts, int has_fin) {
   constexpr int PW = ZLayout<C>::XYZZ;
-  const uint32_t K = g1_kcount(r), Q = 1u << r;
-  const uint64_t T0 = (uint64_t)ngroups * (K + 1), T1 = (uint64_t)ngroups * K;
+  const uint32_t K = g1_kcount(r), Q = 1u << r, k1 = has_fin ? K + 1 : K;
+  const uint64_t T0 = (uint64_t)ngroups * (K + 1), T1 = (uint64_t)ngroups * k1;
   const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t out = tid >> 3;
   const uint32_t lane = (uint32_t)(tid & 7);
   const bool live = out < (uint64_t)ngroups * Q;  // whole 8-lane groups are live or not
   const uint32_t G = live ? (uint32_t)(out / Q) : 0u, qq = live ? (uint32_t)(out % Q) : 0u;
-  auto slot = [&](uint32_t k, XyzzZ<C> &t) { g1_slot_load<C>(t, scr, G, k, K, K, T0, T1, nparts); };
+  auto slot = [&](uint32_t k, XyzzZ<C> &t) { g1_slot_load<C>(t, scr, G, k, K, k1, T0, T1, nparts); };
   // term 0: x_0; term q >= 1: slot (q, q' mod 2^h(q)) with its sign
   const uint32_t nterm = live ? Q : 0u;
   XyzzZ<C> acc;
@@ -562,6 +572,15 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
     }
     g = zkh::inv(fr, g);
   }
+  // For a primitive N-th root (g^(N/2) = -1) both networks compute the DFT, so the inverse is
+  // N^-1 times the forward network by g^-1: it runs the DIT kernels, whose products need no
+  // input sums (the DIF products sum up to 2^r inputs first), with N^-1 folded into the first
+  // group.  Any other generator keeps the reference's DIF network.
+  bool dit = !inverse;
+  if (inverse && m > 0) {
+    const zkh::Fe h = zkh::pow_u64(fr, g, halfN), mone = zkh::neg(fr, zkh::one(fr));
+    dit = memcmp(h.w, mone.w, 8 * fr.nl) == 0;
+  }
   DevBuf pts, tw, twm, scr, tab;
   StreamSyncGuard drain(st);  // the buffers outlive the queued kernels on every return
   ZK_TRY(pts.alloc((size_t)N * ZW * 4));
@@ -586,7 +605,7 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
     });
     ZK_HIP(lds_err);
   }
-  // threads of a group launch split in S parts (prod_k: slot K is a product, the inverse's last group)
+  // threads of a group launch split in S parts (prod_k: slot K is a product, in the group that takes N^-1)
   auto group_threads = [&](int r, int S, bool prod_k) {
     const uint64_t ng = N >> r, K = g1_kcount(r);
     return ng * (K + 1) + (uint64_t)(S - 1) * ng * (K + (prod_k ? 1 : 0));
@@ -626,36 +645,46 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
     hipLaunchKernelGGL(g1_twiddle_kernel<C>, gH, blk, 0, st, tw.as<uint32_t>(),
                        grouped ? twm.as<uint32_t>() : (uint32_t *)nullptr, to_fr(g), halfN);
   if (jac)
-    hipLaunchKernelGGL((g1_proj_in_kernel<C, true>), gN, blk, 0, st, (const uint32_t *)d_src, N, m, inverse ? 0 : 1,
+    hipLaunchKernelGGL((g1_proj_in_kernel<C, true>), gN, blk, 0, st, (const uint32_t *)d_src, N, m, dit ? 1 : 0,
                        pts.as<int32_t>());
   else
-    hipLaunchKernelGGL((g1_proj_in_kernel<C, false>), gN, blk, 0, st, (const uint32_t *)d_src, N, m, inverse ? 0 : 1,
+    hipLaunchKernelGGL((g1_proj_in_kernel<C, false>), gN, blk, 0, st, (const uint32_t *)d_src, N, m, dit ? 1 : 0,
                        pts.as<int32_t>());
   auto grid = [](uint64_t threads) { return dim3((unsigned)((threads + kG1Threads - 1) / kG1Threads)); };
   auto grid_n = [](uint64_t threads, int per) { return dim3((unsigned)((threads + per - 1) / per)); };
-  if (!inverse) {
+  const zkh::Fe ninv_m = zkh::inv(fr, zkh::from_u64(fr, N));
+  if (dit) {
+    // an inverse run folds N^-1 into the first group when it is a grouped stage, else a scaling
+    // pass follows
     int s0 = 0;
+    bool folded = false;
     for (int r : plan) {
       if (r == 1) {
         hipLaunchKernelGGL((g1_fft_stage_kernel<C, false>), grid(stage_threads), blk, 0, st, pts.as<int32_t>(),
                            tw.as<uint32_t>(), m, s0 + 1, tab.as<uint32_t>(), stage_threads);
       } else {
         const uint32_t ng = N >> r;
-        const int S = group_parts(r, false);
+        const int fin = inverse && s0 == 0 ? 1 : 0;
+        folded = folded || fin;
+        const int S = group_parts(r, fin != 0);
         const bool sp = S > 1;
-        const uint64_t nt = group_threads(r, S, false);
+        const uint64_t nt = group_threads(r, S, fin != 0);
         hipLaunchKernelGGL(g1_group_dit_mul_kernel<C>, sp ? grid_n(nt, g1_split_threads<C>()) : grid(nt),
                            sp ? dim3(g1_split_threads<C>()) : blk, sp ? g1_split_lds<C>() : 0, st,
                            (const int32_t *)pts.as<int32_t>(), scr.as<int32_t>(), twm.as<uint32_t>(), m, s0, r, ng,
-                           tab.as<uint32_t>(), tab_threads, S);
+                           fin, to_fr(ninv_m), tab.as<uint32_t>(), tab_threads, S);
         hipLaunchKernelGGL(g1_group_dit_sum_kernel<C>, grid(((uint64_t)ng << r) * 8), blk, 0, st, pts.as<int32_t>(),
-                           (const int32_t *)scr.as<int32_t>(), s0, r, ng, S);
+                           (const int32_t *)scr.as<int32_t>(), s0, r, ng, S, fin);
       }
       s0 += r;
     }
+    if (inverse && !folded) {
+      const zkh::Fe ninv = zkh::to_std(fr, ninv_m);
+      hipLaunchKernelGGL(g1_scale_kernel<C>, grid(scale_threads), blk, 0, st, pts.as<int32_t>(), N, to_fr(ninv),
+                         tab.as<uint32_t>(), scale_threads);
+    }
   } else if (m > 0) {
     // N^-1 folds into the last group when it is a grouped stage, else a scaling pass follows
-    const zkh::Fe ninv_m = zkh::inv(fr, zkh::from_u64(fr, N));
     int s0 = m;
     bool folded = false;
     for (size_t i = plan.size(); i-- > 0;) {
@@ -685,7 +714,7 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
                          tab.as<uint32_t>(), scale_threads);
     }
   }
-  hipLaunchKernelGGL(g1_proj_out_kernel<C>, gN, blk, 0, st, pts.as<int32_t>(), N, m, inverse ? 1 : 0,
+  hipLaunchKernelGGL(g1_proj_out_kernel<C>, gN, blk, 0, st, pts.as<int32_t>(), N, m, dit ? 0 : 1,
                      (uint32_t *)d_dst);
   ZK_HIP(hipGetLastError());
   ZK_HIP(hipStreamSynchronize(st));  // pts / tw / twm / scr / tab are released on return
