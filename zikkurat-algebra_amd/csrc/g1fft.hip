@@ -544,8 +544,11 @@ __global__ __launch_bounds__(kG1Threads) void g1_group_dif_sum_kernel(int32_t *_
 
 
 // Stage plan: group sizes r summing to m, each the largest r <= 4 (and <= what remains) whose
-// launch, ngroups (K_r + 1) threads, stays within about one round of resident threads; r = 1
-// keeps the radix-2 kernel
+// launch, ngroups (K_r + 1) threads, stays within one round of resident threads; r = 1 keeps
+// the radix-2 kernel.  A launch's time is about one chain per round of wavefronts (a chain is
+// latency-bound: two wavefronts on a SIMD run almost as fast as one), so a launch of two rounds
+// costs twice one of a single round: at BN254 2^16 eight r = 2 launches (98304 threads each)
+// beat five r = 3 launches of two rounds each (180224 threads) and a radix-2 stage.
 static void g1_stage_plan(int m, uint64_t round_threads, std::vector<int> &rs) {
   rs.clear();
   int left = m;
@@ -589,7 +592,8 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
   ZK_TRY(context(&ctx));
   std::vector<int> plan;
   const uint64_t round_threads = (uint64_t)ctx->num_cus * 4 * 3 * 64;
-  g1_stage_plan(m, round_threads, plan);
+  // the group mul kernels hold two wavefronts per SIMD (BN254) or one (BLS12-381, registers)
+  g1_stage_plan(m, (uint64_t)ctx->num_cus * 4 * 64 * (C::id == 0 ? 2 : 1), plan);
   // a group launch splits each product over as many threads (up to three) as still give every
   // wavefront a SIMD (the chains are latency-bound there)
   const uint64_t split_cap = (uint64_t)ctx->num_cus * 4 * 64;
