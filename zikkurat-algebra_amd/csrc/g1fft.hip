@@ -23,9 +23,8 @@
 // whose entries are +-(products of twiddles along the unique network path), so every scalar
 // multiplication of the group can run at once, followed (DIT) or preceded and followed (DIF)
 // by additions.  Entries equal up to sign are shared: (4^r - 1) / 3 scalar multiplications per
-// group instead of r 2^(r-1), in one chain instead of r.  The stage plan takes the largest
-// r <= 4 whose thread count still fits about one round of the GPU, so large transforms keep
-// the cheaper radix-2 stages.
+// group instead of r 2^(r-1), in one chain instead of r.  The stage plan picks the group sizes
+// by a launch-cost model (g1_stage_plan), so large transforms keep the cheaper radix-2 stages.
 #include <string.h>
 
 #include <algorithm>
@@ -43,9 +42,10 @@
 namespace zk {
 
 static constexpr int kG1Threads = 128;
-// split group launches of BN254 (whose mul kernels could hold two wavefronts per SIMD):
-// workgroups of four wavefronts with LDS reserved so that a CU holds one.  BLS12-381's hold one
-// wavefront per SIMD by their registers and launch as the other kernels.
+// group launches of at most one wavefront per SIMD (split or not) of BN254, whose mul kernels
+// could hold two wavefronts per SIMD: workgroups of four wavefronts with LDS reserved so that a
+// CU holds one (with two-wavefront workgroups the dispatcher pairs wavefronts on SIMDs).
+// BLS12-381's hold one wavefront per SIMD by their registers and launch as the other kernels.
 static constexpr int kG1SplitThreads = 256;
 static constexpr size_t kG1SplitLds = 96 * 1024;
 template <class C>
@@ -543,21 +543,32 @@ __global__ __launch_bounds__(kG1Threads) void g1_group_dif_sum_kernel(int32_t *_
 }
 
 
-// Stage plan: group sizes r summing to m, each the largest r <= 4 (and <= what remains) whose
-// launch, ngroups (K_r + 1) threads, stays within one round of resident threads; r = 1 keeps
-// the radix-2 kernel.  A launch's time is about one chain per round of wavefronts (a chain is
-// latency-bound: two wavefronts on a SIMD run almost as fast as one), so a launch of two rounds
-// costs twice one of a single round: at BN254 2^16 eight r = 2 launches (98304 threads each)
-// beat five r = 3 launches of two rounds each (180224 threads) and a radix-2 stage.
-static void g1_stage_plan(int m, uint64_t round_threads, std::vector<int> &rs) {
+// Stage plan: group sizes r <= 4 summing to m (r = 1: the radix-2 kernel), the composition
+// of least modelled time (dynamic programming over m; a launch's cost does not depend on where
+// it sits).  A launch costs its chain (whole, or split in S parts: 1, 0.73, 0.67, measured at
+// BN254 2^12) times the wavefronts on its busiest SIMD, two co-resident wavefronts (BN254's
+// mul kernels hold two a SIMD, BLS12-381's and the radix-2 kernel one) costing 1.75 chains
+// (the chains are latency-bound, so sharing a SIMD gains little), plus 0.08 of a chain for the
+// sum launch.  Measured at BN254: 2^14 6.24 predicted / 6.2-6.4 ms, 2^15 9.15 / 9.3, 2^16 14.6 /
+// 14.8 (in chains of ~1 ms).  The model holds where each wavefront that could have a SIMD of
+// its own gets one: BN254's group launches of at most one wavefront a SIMD take the split
+// launch shape, split or not.  group_cost(r) gives a group launch's cost, or the radix-2
+// stage's for r = 1.
+template <class Cost>
+static void g1_stage_plan(int m, Cost group_cost, std::vector<int> &rs) {
+  std::vector<double> best(m + 1, 1e300);
+  std::vector<int> choice(m + 1, 1);
+  best[0] = 0;
+  for (int k = 1; k <= m; k++)
+    for (int r = std::min(4, k); r >= 1; r--) {  // the larger r on (near) ties: fewer launches
+      const double c = best[k - r] + group_cost(r);
+      if (c < best[k] - 1e-9) {
+        best[k] = c;
+        choice[k] = r;
+      }
+    }
   rs.clear();
-  int left = m;
-  while (left > 0) {
-    int r = std::min(4, left);
-    while (r > 1 && ((1ull << m) >> r) * (g1_kcount(r) + 1) > round_threads) r--;
-    rs.push_back(r);
-    left -= r;
-  }
+  for (int k = m; k > 0; k -= choice[k]) rs.push_back(choice[k]);
 }
 
 template <class C>
@@ -592,8 +603,6 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
   ZK_TRY(context(&ctx));
   std::vector<int> plan;
   const uint64_t round_threads = (uint64_t)ctx->num_cus * 4 * 3 * 64;
-  // the group mul kernels hold two wavefronts per SIMD (BN254) or one (BLS12-381, registers)
-  g1_stage_plan(m, (uint64_t)ctx->num_cus * 4 * 64 * (C::id == 0 ? 2 : 1), plan);
   // a group launch splits each product over as many threads (up to three) as still give every
   // wavefront a SIMD (the chains are latency-bound there)
   const uint64_t split_cap = (uint64_t)ctx->num_cus * 4 * 64;
@@ -619,6 +628,20 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
       if (group_threads(r, S, prod_k) <= split_cap) return S;
     return 1;
   };
+  {
+    const uint64_t simds = (uint64_t)ctx->num_cus * 4;
+    // chains on the busiest SIMD for `threads` threads, co-resident wavefronts `occ` a SIMD
+    auto launch_cost = [&](uint64_t threads, double chain, int occ) {
+      const uint64_t w = ((threads + 63) / 64 + simds - 1) / simds;
+      return chain * (occ >= 2 ? 1.75 * (double)(w / 2) + (double)(w % 2) : (double)w);
+    };
+    auto group_cost = [&](int r) {
+      if (r == 1) return launch_cost(halfN, 1.0, 1);
+      const int S = group_parts(r, false);
+      return launch_cost(group_threads(r, S, false), S == 3 ? 0.67 : (S == 2 ? 0.73 : 1.0), C::id == 0 ? 2 : 1) + 0.08;
+    };
+    g1_stage_plan(m, group_cost, plan);
+  }
   uint64_t scr_pts = 0;
   bool grouped = false;
   for (int r : plan)
@@ -671,8 +694,8 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
         const int fin = inverse && s0 == 0 ? 1 : 0;
         folded = folded || fin;
         const int S = group_parts(r, fin != 0);
-        const bool sp = S > 1;
         const uint64_t nt = group_threads(r, S, fin != 0);
+        const bool sp = nt <= split_cap;  // one wavefront a SIMD: the split launch shape
         hipLaunchKernelGGL(g1_group_dit_mul_kernel<C>, sp ? grid_n(nt, g1_split_threads<C>()) : grid(nt),
                            sp ? dim3(g1_split_threads<C>()) : blk, sp ? g1_split_lds<C>() : 0, st,
                            (const int32_t *)pts.as<int32_t>(), scr.as<int32_t>(), twm.as<uint32_t>(), m, s0, r, ng,
@@ -702,8 +725,8 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
         const int fin = s0 == 0 ? 1 : 0;
         folded = folded || fin;
         const int S = group_parts(r, fin != 0);
-        const bool sp = S > 1;
         const uint64_t nt = group_threads(r, S, fin != 0);
+        const bool sp = nt <= split_cap;  // one wavefront a SIMD: the split launch shape
         hipLaunchKernelGGL(g1_group_dif_mul_kernel<C>, sp ? grid_n(nt, g1_split_threads<C>()) : grid(nt),
                            sp ? dim3(g1_split_threads<C>()) : blk, sp ? g1_split_lds<C>() : 0, st,
                            (const int32_t *)pts.as<int32_t>(), scr.as<int32_t>(), twm.as<uint32_t>(), m, s0, r, ng,
