@@ -42,9 +42,10 @@
 namespace zk {
 
 static constexpr int kG1Threads = 128;
-// group launches of at most one wavefront per SIMD (split or not) of BN254, whose mul kernels
-// could hold two wavefronts per SIMD: workgroups of four wavefronts with LDS reserved so that a
-// CU holds one (with two-wavefront workgroups the dispatcher pairs wavefronts on SIMDs).
+// group and radix-2 launches of at most one wavefront per SIMD (split or not) of BN254, whose
+// kernels could hold two wavefronts per SIMD: workgroups of four wavefronts with LDS reserved so
+// that a CU holds one (with two-wavefront workgroups the dispatcher pairs wavefronts on SIMDs:
+// BN254 2^17, 1024 wavefronts a radix-2 stage, 17.2 against 28.1 ms).
 // BLS12-381's hold one wavefront per SIMD by their registers and launch as the other kernels.
 static constexpr int kG1SplitThreads = 256;
 static constexpr size_t kG1SplitLds = 96 * 1024;
@@ -204,7 +205,7 @@ __device__ __forceinline__ void table_scalar_mul(XyzzZ<C> &r, const XyzzZ<C> &p,
 // nthr threads striding over the N/2 butterflies (thread t owns table block t of tab, stride
 // nthr)
 template <class C, bool DIF>
-__global__ __launch_bounds__(kG1Threads) void g1_fft_stage_kernel(int32_t *__restrict__ pts,
+__global__ __launch_bounds__(kG1SplitThreads) void g1_fft_stage_kernel(int32_t *__restrict__ pts,
                                                                  const uint32_t *__restrict__ tw, int m, int s,
                                                                  uint32_t *__restrict__ tab, uint32_t nthr) {
   constexpr int PW = ZLayout<C>::XYZZ;
@@ -213,31 +214,40 @@ __global__ __launch_bounds__(kG1Threads) void g1_fft_stage_kernel(int32_t *__res
   for (uint32_t t = tid; t < (1u << (m - 1)); t += nthr) {
     const uint32_t half = 1u << (s - 1), pos = t & (half - 1), grp = t >> (s - 1);
     const uint32_t i0 = (grp << s) | pos, i1 = i0 + half;
-    XyzzZ<C> u, v, w, a;
-    zxyzz_load(u, pts + (size_t)i0 * PW);
-    zxyzz_load(v, pts + (size_t)i1 * PW);
     uint32_t k[8];
     const uint32_t *kp = tw + ((size_t)pos << (m - s)) * 8;
 #pragma unroll
     for (int j = 0; j < 8; j++) k[j] = kp[j];
+    // u is read after (DIT) or stored before (DIF) the scalar multiplication, so that no point
+    // but the one multiplied is live across the chain (the register count sets the occupancy)
     if (!DIF) {
+      XyzzZ<C> v, w;
+      zxyzz_load(v, pts + (size_t)i1 * PW);
       if (pos == 0) w = v;  // twiddle 1
       else table_scalar_mul<C>(w, v, k, tab, tid, nthr);
+      XyzzZ<C> u, a;
+      zxyzz_load(u, pts + (size_t)i0 * PW);
       zxyzz_add(a, u, w);
       zxyzz_neg(w, w);
       zxyzz_add(v, u, w);
       zxyzz_store(pts + (size_t)i0 * PW, a);
       zxyzz_store(pts + (size_t)i1 * PW, v);
     } else {
-      zxyzz_add(a, u, v);
-      zxyzz_neg(v, v);
-      zxyzz_add(w, u, v);
+      XyzzZ<C> w;
+      {
+        XyzzZ<C> u, v, a;
+        zxyzz_load(u, pts + (size_t)i0 * PW);
+        zxyzz_load(v, pts + (size_t)i1 * PW);
+        zxyzz_add(a, u, v);
+        zxyzz_store(pts + (size_t)i0 * PW, a);
+        zxyzz_neg(v, v);
+        zxyzz_add(w, u, v);
+      }
       if (pos != 0) {
         XyzzZ<C> r;
         table_scalar_mul<C>(r, w, k, tab, tid, nthr);
         w = r;
       }
-      zxyzz_store(pts + (size_t)i0 * PW, a);
       zxyzz_store(pts + (size_t)i1 * PW, w);
     }
   }
@@ -547,7 +557,7 @@ __global__ __launch_bounds__(kG1Threads) void g1_group_dif_sum_kernel(int32_t *_
 // of least modelled time (dynamic programming over m; a launch's cost does not depend on where
 // it sits).  A launch costs its chain (whole, or split in S parts: 1, 0.73, 0.67, measured at
 // BN254 2^12) times the wavefronts on its busiest SIMD, two co-resident wavefronts (BN254's
-// mul kernels hold two a SIMD, BLS12-381's and the radix-2 kernel one) costing 1.75 chains
+// kernels hold two a SIMD, BLS12-381's one) costing 1.75 chains
 // (the chains are latency-bound, so sharing a SIMD gains little), plus 0.08 of a chain for the
 // sum launch.  Measured at BN254: 2^14 6.24 predicted / 6.2-6.4 ms, 2^15 9.15 / 9.3, 2^16 14.6 /
 // 14.8 (in chains of ~1 ms).  The model holds where each wavefront that could have a SIMD of
@@ -615,6 +625,12 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
       if (lds_err == hipSuccess)
         lds_err = hipFuncSetAttribute((const void *)g1_group_dif_mul_kernel<C>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)g1_split_lds<C>());
+      if (lds_err == hipSuccess)
+        lds_err = hipFuncSetAttribute((const void *)g1_fft_stage_kernel<C, false>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)g1_split_lds<C>());
+      if (lds_err == hipSuccess)
+        lds_err = hipFuncSetAttribute((const void *)g1_fft_stage_kernel<C, true>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)g1_split_lds<C>());
     });
     ZK_HIP(lds_err);
   }
@@ -636,7 +652,7 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
       return chain * (occ >= 2 ? 1.75 * (double)(w / 2) + (double)(w % 2) : (double)w);
     };
     auto group_cost = [&](int r) {
-      if (r == 1) return launch_cost(halfN, 1.0, 1);
+      if (r == 1) return launch_cost(halfN, 1.0, C::id == 0 ? 2 : 1);
       const int S = group_parts(r, false);
       return launch_cost(group_threads(r, S, false), S == 3 ? 0.67 : (S == 2 ? 0.73 : 1.0), C::id == 0 ? 2 : 1) + 0.08;
     };
@@ -656,6 +672,7 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
   // BLS12-381: 3 entries, 840 bytes)
   const uint32_t tab_threads = (uint32_t)std::max<uint64_t>(scr_pts, std::min<uint64_t>(N, round_threads));
   const uint32_t stage_threads = std::min<uint32_t>(halfN, tab_threads), scale_threads = std::min<uint32_t>(N, tab_threads);
+  const bool stage_sp = stage_threads <= split_cap;  // one wavefront a SIMD: the split launch shape
   ZK_TRY(tab.alloc((size_t)tab_threads * g1_tab_words<C>() * 4));
   if (grouped) {
     ZK_TRY(twm.alloc((size_t)(halfN ? halfN : 1) * 32));
@@ -687,8 +704,10 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
     bool folded = false;
     for (int r : plan) {
       if (r == 1) {
-        hipLaunchKernelGGL((g1_fft_stage_kernel<C, false>), grid(stage_threads), blk, 0, st, pts.as<int32_t>(),
-                           tw.as<uint32_t>(), m, s0 + 1, tab.as<uint32_t>(), stage_threads);
+        hipLaunchKernelGGL((g1_fft_stage_kernel<C, false>), stage_sp ? grid_n(stage_threads, g1_split_threads<C>())
+                                                                     : grid(stage_threads),
+                           stage_sp ? dim3(g1_split_threads<C>()) : blk, stage_sp ? g1_split_lds<C>() : 0, st,
+                           pts.as<int32_t>(), tw.as<uint32_t>(), m, s0 + 1, tab.as<uint32_t>(), stage_threads);
       } else {
         const uint32_t ng = N >> r;
         const int fin = inverse && s0 == 0 ? 1 : 0;
@@ -718,8 +737,10 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
       const int r = plan[i];
       s0 -= r;
       if (r == 1) {
-        hipLaunchKernelGGL((g1_fft_stage_kernel<C, true>), grid(stage_threads), blk, 0, st, pts.as<int32_t>(),
-                           tw.as<uint32_t>(), m, s0 + 1, tab.as<uint32_t>(), stage_threads);
+        hipLaunchKernelGGL((g1_fft_stage_kernel<C, true>), stage_sp ? grid_n(stage_threads, g1_split_threads<C>())
+                                                                    : grid(stage_threads),
+                           stage_sp ? dim3(g1_split_threads<C>()) : blk, stage_sp ? g1_split_lds<C>() : 0, st,
+                           pts.as<int32_t>(), tw.as<uint32_t>(), m, s0 + 1, tab.as<uint32_t>(), stage_threads);
       } else {
         const uint32_t ng = N >> r;
         const int fin = s0 == 0 ? 1 : 0;
