@@ -328,18 +328,15 @@ __device__ __forceinline__ void g1_path_coeff(Fe<typename C::Fr> &c, const uint3
 // One thread per slot; slot k of group G lives at scr[G (K + 1) + k].  (Splitting a BN254 slot
 // into its two GLV halves on two threads doubles the doublings and measured 1.75x slower.)
 
-// Sum of the nterm terms term(t) of one output over 8 lanes: lane j adds terms j, j + 8, ...,
-// then three xor-shuffle rounds combine the 8 partial sums (a chain of nterm / 8 + 3 additions
-// instead of nterm).  All 8 lanes of an output group must call it.
-template <class C, class Term>
-__device__ __forceinline__ void g1_sum8(XyzzZ<C> &acc, uint32_t nterm, uint32_t lane, Term term) {
+// Sum of the nterm terms of one output over 8 lanes: lane j adds terms j, j + 8, ... (add_term(i,
+// acc) adds term i into acc), then three xor-shuffle rounds combine the 8 partial sums (a chain
+// of nterm / 8 + 3 additions instead of nterm).  All 8 lanes of an output group must call it.
+template <class C, class AddTerm>
+__device__ __forceinline__ void g1_sum8(XyzzZ<C> &acc, uint32_t nterm, uint32_t lane, AddTerm add_term) {
   constexpr int N = ZP<C>::T::N;
-  XyzzZ<C> t, o;
+  XyzzZ<C> o;
   zxyzz_set_inf(acc);
-  for (uint32_t i = lane; i < nterm; i += 8) {
-    term(i, t);
-    zxyzz_add(acc, acc, t);
-  }
+  for (uint32_t i = lane; i < nterm; i += 8) add_term(i, acc);
 #pragma unroll
   for (int off = 4; off >= 1; off >>= 1) {
 #pragma unroll
@@ -386,18 +383,19 @@ __device__ __forceinline__ G1Slot g1_thread_slot(uint64_t tid, uint32_t K, uint3
   }
   return r;
 }
-// slot k of group G: the sum of its parts over the planes
+// acc += (-1)^neg slot k of group G, one plane at a time (one point live beside acc)
 template <class C>
-__device__ __forceinline__ void g1_slot_load(XyzzZ<C> &t, const int32_t *__restrict__ scr, uint32_t G, uint32_t k,
-                                             uint32_t K, uint32_t k1, uint64_t T0, uint64_t T1, int nparts) {
+__device__ __forceinline__ void g1_slot_add(XyzzZ<C> &acc, const int32_t *__restrict__ scr, uint32_t G, uint32_t k,
+                                            uint32_t K, uint32_t k1, uint64_t T0, uint64_t T1, int nparts,
+                                            bool neg) {
   constexpr int PW = ZLayout<C>::XYZZ;
-  zxyzz_load(t, scr + ((uint64_t)G * (K + 1) + k) * PW);
-  if (k < k1)
-    for (int j = 1; j < nparts; j++) {
-      XyzzZ<C> u;
-      zxyzz_load(u, scr + (T0 + (uint64_t)(j - 1) * T1 + (uint64_t)G * k1 + k) * PW);
-      zxyzz_add(t, t, u);
-    }
+  const int planes = k < k1 ? nparts : 1;
+  for (int j = 0; j < planes; j++) {
+    XyzzZ<C> t;
+    zxyzz_load(t, scr + (j == 0 ? (uint64_t)G * (K + 1) + k : T0 + (uint64_t)(j - 1) * T1 + (uint64_t)G * k1 + k) * PW);
+    if (neg) zxyzz_neg(t, t);
+    zxyzz_add(acc, acc, t);
+  }
 }
 
 // DIT: slot k < K of group G = M(q, L) x_q; slot K = x_0; with has_fin every slot is also
@@ -457,18 +455,16 @@ __global__ __launch_bounds__(kG1Threads) void g1_group_dit_sum_kernel(int32_t *_
   const uint32_t lane = (uint32_t)(tid & 7);
   const bool live = out < (uint64_t)ngroups * Q;  // whole 8-lane groups are live or not
   const uint32_t G = live ? (uint32_t)(out / Q) : 0u, qq = live ? (uint32_t)(out % Q) : 0u;
-  auto slot = [&](uint32_t k, XyzzZ<C> &t) { g1_slot_load<C>(t, scr, G, k, K, k1, T0, T1, nparts); };
   // term 0: x_0; term q >= 1: slot (q, q' mod 2^h(q)) with its sign
   const uint32_t nterm = live ? Q : 0u;
   XyzzZ<C> acc;
-  g1_sum8<C>(acc, nterm, lane, [&](uint32_t q, XyzzZ<C> &t) {
+  g1_sum8<C>(acc, nterm, lane, [&](uint32_t q, XyzzZ<C> &a) {
     if (q == 0) {
-      slot(K, t);
+      g1_slot_add<C>(a, scr, G, K, K, k1, T0, T1, nparts, false);
       return;
     }
     const int h = 31 - __clz(q);
-    slot(g1_slot(h, q, qq & ((1u << h) - 1)), t);
-    if (__popc(q & qq) & 1) zxyzz_neg(t, t);
+    g1_slot_add<C>(a, scr, G, g1_slot(h, q, qq & ((1u << h) - 1)), K, k1, T0, T1, nparts, (__popc(q & qq) & 1) != 0);
   });
   if (live && lane == 0) {
     const uint32_t p0 = G & ((1u << s0) - 1), ghi = G >> s0;
@@ -542,8 +538,8 @@ __global__ __launch_bounds__(kG1Threads) void g1_group_dif_sum_kernel(int32_t *_
   const int h = qq ? 31 - __clz(qq) : 0;
   const uint32_t nterm = live ? (qq ? (1u << h) : 1u) : 0u;
   XyzzZ<C> acc;
-  g1_sum8<C>(acc, nterm, lane, [&](uint32_t L, XyzzZ<C> &t) {
-    g1_slot_load<C>(t, scr, G, qq ? g1_slot(h, qq, L) : K, K, k1, T0, T1, nparts);
+  g1_sum8<C>(acc, nterm, lane, [&](uint32_t L, XyzzZ<C> &a) {
+    g1_slot_add<C>(a, scr, G, qq ? g1_slot(h, qq, L) : K, K, k1, T0, T1, nparts, false);
   });
   if (live && lane == 0) {
     const uint32_t p0 = G & ((1u << s0) - 1), ghi = G >> s0;
