@@ -325,8 +325,9 @@ __device__ __forceinline__ void g1_path_coeff(Fe<typename C::Fr> &c, const uint3
   fto_std(c, c);
 }
 
-// One thread per slot; slot k of group G lives at scr[G (K + 1) + k].  (Splitting a BN254 slot
-// into its two GLV halves on two threads doubles the doublings and measured 1.75x slower.)
+// Unsplit, one thread per slot; slot k of group G lives at scr[G (K + 1) + k] (plane 0).
+// (Splitting a BN254 slot into its two GLV halves k1 P and k2 phi(P) on two threads gives each
+// the whole doubling chain and measured 1.75x slower; the window splits below shorten it.)
 
 // Sum of the nterm terms of one output over 8 lanes: lane j adds terms j, j + 8, ... (add_term(i,
 // acc) adds term i into acc), then three xor-shuffle rounds combine the 8 partial sums (a chain
@@ -608,6 +609,7 @@ static int g1_fft_run(int m, const uint64_t *gen, bool inverse, bool jac, const 
   Context *ctx = nullptr;
   ZK_TRY(context(&ctx));
   std::vector<int> plan;
+  // thread count cap of the radix-2 stages and the scaling pass (they stride over the rest)
   const uint64_t round_threads = (uint64_t)ctx->num_cus * 4 * 3 * 64;
   // a group launch splits each product over as many threads (up to three) as still give every
   // wavefront a SIMD (the chains are latency-bound there)
